@@ -1,0 +1,228 @@
+"""ctypes bindings of the CPU FP64 oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may import this
+package, and only to check or time the GPU library against it: the product (``scatrs_amd``)
+never imports it. Build it with ``make -C oracle``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+
+class _Cx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib() -> C.CDLL:
+    """The oracle library, built on first use if absent."""
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            os.system(f"make -s -C {_HERE}")
+        L = C.CDLL(str(path))
+        L.orc_default_n_max.restype = C.c_int64
+        L.orc_default_n_max.argtypes = [C.c_double]
+        L.orc_start_index.restype = C.c_int64
+        L.orc_start_index.argtypes = [C.c_int64, C.c_double]
+        L.orc_mie_solve.argtypes = [C.c_double, _Cx, C.c_int64, C.c_int,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_efficiencies.argtypes = [C.c_double, C.c_int64, C.c_void_p, C.c_void_p,
+                                       C.c_double, C.c_void_p]
+        L.orc_efficiencies_jacobian.argtypes = [C.c_double, C.c_int64] + [C.c_void_p] * 6 + [
+            C.c_double, C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_mie_efficiencies_batch.argtypes = [_dp, _cp, C.c_int64, C.c_int64, _dp,
+                                                 C.c_void_p, C.c_int]
+        L.orc_amplitudes.argtypes = [C.c_int64, _cp, _cp, _dp, C.c_int64, _cp, _cp]
+        L.orc_gauss_legendre.argtypes = [C.c_int64, _dp, _dp]
+        L.orc_wigner_d.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int64, _dp]
+        L.orc_greek_project.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int64, _dp]
+        L.orc_layered_solve.argtypes = [C.c_int, _dp, _cp, C.c_int64, C.c_int, _cp, _cp,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_density.restype = C.c_double
+        L.orc_density.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p]
+        L.orc_radius_range.argtypes = [C.c_int, C.c_double, C.c_double,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_ensemble_fixed.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                         C.c_void_p, _cp, C.c_int64, C.c_int64, C.c_int64,
+                                         C.c_int, _dp, C.c_void_p, C.c_void_p]
+        L.orc_ensemble_batch.argtypes = [C.c_int64, _ip, _dp, _dp, _dp, _cp, C.c_int64,
+                                         C.c_int64, C.c_int64, C.c_int, _dp, C.c_void_p,
+                                         C.c_void_p, C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+def default_n_max(x: float) -> int:
+    return int(lib().orc_default_n_max(float(x)))
+
+
+def start_index(n_top: int, rho_abs: float) -> int:
+    return int(lib().orc_start_index(int(n_top), float(rho_abs)))
+
+
+def mie_solve(x: float, m: complex, n: int | None = None, derivatives: bool = False) -> dict:
+    """a, b (n,), abs_sum and, with derivatives, da_dx, da_dm, db_dx, db_dm, abs_jac (3,)."""
+    n = default_n_max(x) if n is None else int(n)
+    a = np.zeros(n, np.complex128)
+    b = np.zeros(n, np.complex128)
+    d = [np.zeros(n, np.complex128) for _ in range(4)] if derivatives else [None] * 4
+    abs_sum = np.zeros(1)
+    abs_jac = np.zeros(3)
+    mm = _Cx(m.real, m.imag)
+    lib().orc_mie_solve(float(x), mm, n, int(derivatives), _ptr(a), _ptr(b), *map(_ptr, d),
+                        _ptr(abs_sum), _ptr(abs_jac))
+    out = {"a": a, "b": b, "abs_sum": float(abs_sum[0]), "n": n}
+    if derivatives:
+        out.update(da_dx=d[0], da_dm=d[1], db_dx=d[2], db_dm=d[3], abs_jac=abs_jac)
+    return out
+
+
+def efficiencies(x: float, s: dict) -> np.ndarray:
+    q = np.zeros(5)
+    lib().orc_efficiencies(float(x), s["n"], _ptr(s["a"]), _ptr(s["b"]), s["abs_sum"], _ptr(q))
+    return q
+
+
+def efficiencies_jacobian(x: float, s: dict, outer: bool = True) -> np.ndarray:
+    j = np.zeros(15)
+    lib().orc_efficiencies_jacobian(float(x), s["n"], _ptr(s["a"]), _ptr(s["b"]),
+                                    _ptr(s["da_dx"]), _ptr(s["da_dm"]), _ptr(s["db_dx"]),
+                                    _ptr(s["db_dm"]), s["abs_sum"], _ptr(s["abs_jac"]),
+                                    int(outer), _ptr(j))
+    return j.reshape(3, 5)
+
+
+def mie_efficiencies_batch(x: np.ndarray, m: np.ndarray, derivatives: bool = False,
+                           n_max: int = 0, threads: int = 0):
+    x = np.ascontiguousarray(x, np.float64)
+    m = np.ascontiguousarray(m, np.complex128)
+    n = x.size
+    values = np.zeros((5, n))
+    jac = np.zeros((3, 5, n)) if derivatives else None
+    rc = lib().orc_mie_efficiencies_batch(x, m, n, int(n_max), values, _ptr(jac), int(threads))
+    if rc != 0:
+        raise ValueError(f"invalid input at element {-rc - 1}")
+    return values, jac
+
+
+def amplitudes(s: dict, mu: np.ndarray):
+    mu = np.ascontiguousarray(mu, np.float64)
+    s1 = np.zeros(mu.size, np.complex128)
+    s2 = np.zeros(mu.size, np.complex128)
+    lib().orc_amplitudes(s["n"], s["a"], s["b"], mu, mu.size, s1, s2)
+    return s1, s2
+
+
+def gauss_legendre(n: int):
+    nodes = np.zeros(n)
+    weights = np.zeros(n)
+    lib().orc_gauss_legendre(int(n), nodes, weights)
+    return nodes, weights
+
+
+def wigner_d(m: int, n: int, x: float, l_max: int) -> np.ndarray:
+    out = np.zeros(l_max + 1)
+    if lib().orc_wigner_d(int(m), int(n), float(x), int(l_max), out) != 0:
+        raise ValueError("pair not supported by the oracle")
+    return out
+
+
+def greek_project(nodes, weights, f11, f12, f33, f34, order: int) -> np.ndarray:
+    out = np.zeros(6 * (order + 1))
+    arrs = [np.ascontiguousarray(v, np.float64) for v in (nodes, weights, f11, f12, f33, f34)]
+    lib().orc_greek_project(arrs[0].size, *arrs, int(order), out)
+    return out.reshape(6, order + 1)
+
+
+def layered_solve(x, m, n: int | None = None, derivatives: bool = False) -> dict:
+    x = np.ascontiguousarray(x, np.float64)
+    m = np.ascontiguousarray(m, np.complex128)
+    L = x.size
+    n = default_n_max(x[-1]) if n is None else int(n)
+    a = np.zeros(n, np.complex128)
+    b = np.zeros(n, np.complex128)
+    dgrp = np.zeros(L * 4 * n, np.complex128) if derivatives else None
+    abs_sum = np.zeros(1)
+    abs_jac = np.zeros(3 * L) if derivatives else None
+    rc = lib().orc_layered_solve(L, x, m, n, int(derivatives), a, b, _ptr(dgrp),
+                                 _ptr(abs_sum), _ptr(abs_jac))
+    if rc != 0:
+        raise ValueError("invalid layered input")
+    out = {"a": a, "b": b, "abs_sum": float(abs_sum[0]), "n": n}
+    if derivatives:
+        out["d"] = dgrp.reshape(L, 4, n)
+        out["abs_jac"] = abs_jac.reshape(L, 3)
+    return out
+
+
+def density(kind: int, p0: float, p1: float, r: float):
+    g = np.zeros(2)
+    u = lib().orc_density(kind, p0, p1, r, g.ctypes.data)
+    return u, g
+
+
+def radius_range(kind: int, p0: float, p1: float):
+    lo, hi = C.c_double(), C.c_double()
+    if lib().orc_radius_range(kind, p0, p1, C.byref(lo), C.byref(hi)) != 0:
+        raise ValueError("invalid distribution")
+    return lo.value, hi.value
+
+
+LOGNORMAL, GAMMA = 0, 1
+
+
+def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fractions=None,
+                   n_radius: int = 128, n_angles: int = 128, order: int = 32,
+                   normalized: bool = True) -> dict:
+    """One ensemble on the fixed discrete path: scalars (11,), greek (6, L+1), phase (6, n_q)."""
+    m = np.atleast_1d(np.asarray(m, np.complex128)).copy()
+    layers = m.size
+    fr = None if fractions is None else np.ascontiguousarray(fractions, np.float64)
+    scalars = np.zeros(11)
+    greek = np.zeros(6 * (order + 1)) if order >= 0 else None
+    phase = np.zeros(6 * n_angles)
+    rc = lib().orc_ensemble_fixed(kind, p0, p1, wavelength, layers, _ptr(fr), m, n_radius,
+                                  n_angles, order, int(normalized), scalars, _ptr(greek),
+                                  _ptr(phase))
+    if rc != 0:
+        raise ValueError(f"invalid ensemble input ({rc})")
+    return {"scalars": scalars,
+            "greek": None if greek is None else greek.reshape(6, order + 1),
+            "phase": phase.reshape(6, n_angles)}
+
+
+def ensemble_batch(kind, p0, p1, wavelength, m, n_radius: int, n_angles: int, order: int,
+                   normalized: bool = True, threads: int = 0, want_tables: bool = True):
+    kind = np.ascontiguousarray(kind, np.int32)
+    cnt = kind.size
+    p0 = np.ascontiguousarray(p0, np.float64)
+    p1 = np.ascontiguousarray(p1, np.float64)
+    wl = np.ascontiguousarray(wavelength, np.float64)
+    m = np.ascontiguousarray(m, np.complex128)
+    scalars = np.zeros((cnt, 11))
+    greek = np.zeros((cnt, 6, order + 1)) if want_tables else None
+    phase = np.zeros((cnt, 6, n_angles)) if want_tables else None
+    rc = lib().orc_ensemble_batch(cnt, kind, p0, p1, wl, m, n_radius, n_angles, order,
+                                  int(normalized), scalars, _ptr(greek), _ptr(phase),
+                                  int(threads))
+    if rc != 0:
+        raise ValueError("invalid ensemble input")
+    return scalars, greek, phase
