@@ -1,0 +1,102 @@
+/* scatrs_hip: MI355X (gfx950) library for the size-distribution ensemble path of scatrs over
+ * homogeneous and layered spheres. C ABI: plain pointers and sizes, no C++ or torch types.
+ *
+ * Every pointer argument named d_* is DEVICE memory on the context's GPU; outputs are
+ * caller-allocated. Calls are asynchronous on the context's stream except where a result
+ * must come back to the host (validation status, sizing); each documents whether it
+ * synchronizes. A context is not thread-safe: use one per host thread (the reference's
+ * one-workspace-per-thread contract, scatrs/src/lib.rs:186-190).
+ *
+ * Status codes mirror the reference's error enum (scatrs/src/error.rs:14-32):
+ * InvalidInput (nothing computed), OutsideValidatedDomain (inputs valid but outside the
+ * validated domain, unless allow_unvalidated), NotConverged; plus a device-error code for
+ * HIP failures. scatrs_last_error() returns the message of the last failure on a context,
+ * with the element index or the wavelength and radius as the reference's messages carry them
+ * (scatrs-py/src/mie.rs:108-117; scatrs/src/ensemble/engine.rs:124-131). */
+#ifndef SCATRS_HIP_H
+#define SCATRS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCATRS_OK 0
+#define SCATRS_INVALID_INPUT 1
+#define SCATRS_OUTSIDE_VALIDATED_DOMAIN 2
+#define SCATRS_NOT_CONVERGED 3
+#define SCATRS_DEVICE_ERROR 4
+
+#define SCATRS_ABI_VERSION 1
+
+typedef struct scatrs_ctx scatrs_ctx;
+
+/* ABI version of the loaded library (SCATRS_ABI_VERSION). */
+int scatrs_abi_version(void);
+
+/* Creates a context on HIP device `device`, issuing work on `stream` (a hipStream_t; NULL for
+ * the device's null stream). No reference counterpart: the reference's workspaces are
+ * `Mie::new()` / `Layered::new()` (scatrs/src/mie/mod.rs:190-198), which this replaces. */
+int scatrs_create(int device, void *stream, scatrs_ctx **out);
+int scatrs_destroy(scatrs_ctx *ctx);
+int scatrs_set_stream(scatrs_ctx *ctx, void *stream);
+const char *scatrs_last_error(const scatrs_ctx *ctx);
+
+/* Instrumentation, no reference counterpart: with profiling on, the library brackets the
+ * dominant kernel of each call (the persistent solve of a sphere batch, the fused node kernel
+ * of an ensemble batch) with HIP events on the context's stream, up to 1024 launches;
+ * scatrs_kernel_times waits for them, writes up to `cap` durations in milliseconds to the host
+ * array ms_out, clears the record and returns how many it wrote. set_profiling clears it too. */
+int scatrs_set_profiling(scatrs_ctx *ctx, int on);
+int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap);
+
+/* The default truncation N = ceil(x + 6 x^(1/3) + 3) (host; scatrs/src/mie/mod.rs:122-124). */
+int64_t scatrs_default_n_max(double x);
+
+/* Efficiencies [Q_ext, Q_sca, Q_abs, Q_back, g] of `len` spheres, and with d_jacobian != NULL
+ * their derivatives with respect to x, Re m and Im m. Replaces the batch loop of
+ * _scatrs.mie_efficiencies (scatrs-py/src/mie.rs:303-364) over Mie::compute /
+ * compute_with_derivatives (scatrs/src/mie/mod.rs:226-251) and MieSolution::efficiencies /
+ * efficiencies_jacobian (scatrs/src/mie/efficiencies.rs:148-160).
+ *   d_x[len], d_m[2*len] (re, im interleaved: complex128), n_max <= 0 for the default
+ *   truncation; d_values (5, len) row-major; d_jacobian (3, 5, len) or NULL.
+ *   total_terms (host, may be NULL): sum over spheres of N, the (particle, order) terms.
+ * Synchronizes once (validation and sizing); the solve itself is queued on the stream. */
+int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+                            int64_t n_max, int allow_unvalidated, double *d_values,
+                            double *d_jacobian, int64_t *total_terms);
+
+/* Size distributions (scatrs/src/ensemble/distribution.rs:15-68). */
+#define SCATRS_LOGNORMAL 0 /* p0 = median radius r_g, p1 = width sigma_g */
+#define SCATRS_GAMMA 1     /* p0 = effective radius, p1 = effective variance */
+
+/* The fixed discrete path of a batch of ensembles (one distribution at one wavelength each;
+ * the reference's EnsembleResult granularity, scatrs/src/ensemble/mod.rs:508-529):
+ *   n_radius Gauss-Legendre nodes in ln r over the distribution's fixed range
+ *   (lognormal: ln r_g +- 5 ln sigma_g; gamma: ln r_eff - 8 s .. ln r_eff + 5 s,
+ *   s = sqrt(v_eff / (1 - 2 v_eff))), weights w (t_hi - t_lo)/2 u(r) r;
+ *   the phase matrix at the n_angles-point Gauss-Legendre cosines (2 <= n_angles <= 4096);
+ *   Greek to `order` (d_greek == NULL: none).
+ * Spheres: layers == 1 and d_m (count, 2) is each ensemble's index. Layered spheres:
+ *   layers = L > 1, d_fractions (count, L-1) the interface radii as fractions of the outer
+ *   radius (nondecreasing in (0, 1]), d_m (count, L, 2) each layer's index.
+ * Outputs (normalized as the reference's Normalization::Normalized when normalized != 0):
+ *   d_scalars (count, 11) in the reference's SCALARS order (python/scatrs/ensemble.py:51-63);
+ *   d_greek (count, 6, order+1) [alpha1..alpha4, beta1, beta2] or NULL;
+ *   d_phase (count, 6, n_angles) [F11, F12, F22, F33, F34, F44] or NULL.
+ * Replaces ensemble::compute_with_particle / compute_batch_with_particle
+ * (scatrs/src/ensemble/mod.rs:589-686) and _scatrs.ensemble_compute{,_batch}
+ * (scatrs-py/src/ensemble.rs:295-355,456-500) on this fixed discrete path (the reference's
+ * adaptive Gauss-Kronrod panels are not part of this path). Synchronizes once per call. */
+int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                          const double *d_p0, const double *d_p1, const double *d_wavelength,
+                          int layers, const double *d_fractions, const double *d_m,
+                          int64_t n_radius, int64_t n_angles, int64_t order, int normalized,
+                          int allow_unvalidated, double *d_scalars, double *d_greek,
+                          double *d_phase, int64_t *total_terms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

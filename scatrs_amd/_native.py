@@ -1,0 +1,144 @@
+"""Loads the HIP library (libscatrs_hip.so, built in-tree for gfx950) and binds its C ABI.
+
+There is no CPU fallback: if the library is missing or no GPU is visible, every entry point
+raises NativeLibraryError. Device memory and streams come from PyTorch-ROCm; the library
+itself takes plain device pointers.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+from .errors import (InvalidInputError, NativeLibraryError, NotConvergedError,
+                     OutsideValidatedDomainError)
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libscatrs_hip.so"
+
+OK, INVALID_INPUT, OUTSIDE_VALIDATED_DOMAIN, NOT_CONVERGED, DEVICE_ERROR = 0, 1, 2, 3, 4
+ABI_VERSION = 1
+
+_lib = None
+_lock = threading.Lock()
+_tls = threading.local()
+
+# Every symbol include/scatrs_hip.h declares.
+SYMBOLS = (
+    "scatrs_abi_version",
+    "scatrs_create",
+    "scatrs_destroy",
+    "scatrs_set_stream",
+    "scatrs_last_error",
+    "scatrs_set_profiling",
+    "scatrs_kernel_times",
+    "scatrs_default_n_max",
+    "scatrs_mie_efficiencies",
+    "scatrs_ensemble_fixed",
+)
+
+
+def load() -> C.CDLL:
+    """The library, loaded once; raises NativeLibraryError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise NativeLibraryError(
+                f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`"
+                " or `python -m scatrs_amd.build`")
+        try:
+            L = C.CDLL(str(LIB_PATH))
+        except OSError as e:
+            raise NativeLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+        vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
+        L.scatrs_abi_version.restype = i32
+        L.scatrs_create.argtypes = [i32, vp, C.POINTER(vp)]
+        L.scatrs_destroy.argtypes = [vp]
+        L.scatrs_set_stream.argtypes = [vp, vp]
+        L.scatrs_last_error.argtypes = [vp]
+        L.scatrs_last_error.restype = C.c_char_p
+        L.scatrs_set_profiling.argtypes = [vp, i32]
+        L.scatrs_kernel_times.argtypes = [vp, C.POINTER(dbl), i32]
+        L.scatrs_default_n_max.argtypes = [dbl]
+        L.scatrs_default_n_max.restype = i64
+        L.scatrs_mie_efficiencies.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
+                                              C.POINTER(i64)]
+        if hasattr(L, "scatrs_ensemble_fixed"):
+            L.scatrs_ensemble_fixed.argtypes = [vp, i64, vp, vp, vp, vp, i32, vp, vp, i64,
+                                                i64, i64, i32, i32, vp, vp, vp,
+                                                C.POINTER(i64)]
+        if hasattr(L, "scatrs_ensemble_fixed_vjp"):
+            L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, i32, vp, vp, i64,
+                                                    i64, i64, i32, vp, vp, vp]
+        if L.scatrs_abi_version() != ABI_VERSION:
+            raise NativeLibraryError("libscatrs_hip.so has a different ABI version")
+        _lib = L
+        return L
+
+
+class Context:
+    """A library context on one device, bound to torch's current stream at each call."""
+
+    def __init__(self, device: int) -> None:
+        self.lib = load()
+        self.device = device
+        h = C.c_void_p()
+        rc = self.lib.scatrs_create(device, None, C.byref(h))
+        if rc != OK:
+            raise NativeLibraryError(f"scatrs_create failed on device {device} (code {rc})")
+        self.handle = h
+
+    def set_profiling(self, on: bool) -> None:
+        self.lib.scatrs_set_profiling(self.handle, int(bool(on)))
+
+    def kernel_times(self) -> list[float]:
+        """Durations (ms) of the dominant kernel of each call since profiling was set."""
+        buf = (C.c_double * 1024)()
+        n = self.lib.scatrs_kernel_times(self.handle, buf, 1024)
+        return [buf[i] for i in range(n)]
+
+    def bind_stream(self, stream_ptr: int) -> None:
+        self.lib.scatrs_set_stream(self.handle, C.c_void_p(stream_ptr))
+
+    def check(self, rc: int) -> None:
+        if rc == OK:
+            return
+        msg = self.lib.scatrs_last_error(self.handle).decode()
+        if rc == INVALID_INPUT:
+            raise InvalidInputError(msg)
+        if rc == OUTSIDE_VALIDATED_DOMAIN:
+            raise OutsideValidatedDomainError(msg)
+        if rc == NOT_CONVERGED:
+            raise NotConvergedError(msg)
+        raise NativeLibraryError(msg or f"device error (code {rc})")
+
+    def __del__(self) -> None:
+        try:
+            if getattr(self, "handle", None):
+                self.lib.scatrs_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def context(device: int | None = None) -> Context:
+    """This thread's context on `device` (default: torch's current device), bound to torch's
+    current stream."""
+    import torch
+
+    if not torch.cuda.is_available():
+        raise NativeLibraryError("no GPU is visible: the HIP library needs an MI355X")
+    if device is None:
+        device = torch.cuda.current_device()
+    ctxs = getattr(_tls, "ctxs", None)
+    if ctxs is None:
+        ctxs = _tls.ctxs = {}
+    ctx = ctxs.get(device)
+    if ctx is None:
+        ctx = ctxs[device] = Context(device)
+    ctx.bind_stream(torch.cuda.current_stream(device).cuda_stream)
+    return ctx

@@ -1,0 +1,93 @@
+// Input checks of a sphere, as Mie::compute makes them (scatrs/src/mie/mod.rs:274-320) with the
+// limits of mie/mod.rs:95-111 and the validated domain of mie/domain.rs:123-140: a device form
+// that returns the code, and a host form that also writes the reference's message.
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+
+#include "context.h"
+#include "mie_device.h"
+
+namespace scatrs {
+
+constexpr double kMinSize = 1e-30, kMaxSize = 1e6;
+constexpr double kMinIndex = 1e-6, kMaxIndex = 1e3;
+constexpr long long kMaxOrder = 1ll << 20;
+constexpr double kDomXMin = 1e-6, kDomXMax = 2e4, kDomMReMax = 10.0, kDomMImMax = 10.0,
+                 kDomAbsMMin = 0.1;
+
+SCATRS_HD int sphere_code(double x, double mr, double mi, long long n_fixed, int allow,
+                          long long *n_out) {
+    *n_out = 0;
+    if (!(x >= kMinSize && x <= kMaxSize)) return kInvalid;
+    if (!(__builtin_isfinite(mr) && __builtin_isfinite(mi))) return kInvalid;
+    if (mr < 0.0 || mi < 0.0) return kInvalid;
+    const double abs_m = hypot(mr, mi);
+    if (!(abs_m >= kMinIndex && abs_m <= kMaxIndex)) return kInvalid;
+    const long long def = default_n_max(x);
+    const long long n = n_fixed > 0 ? n_fixed : def;
+    if (n == 0 || n > kMaxOrder) return kInvalid;
+    if (!allow) {
+        const bool in = x >= kDomXMin && x <= kDomXMax && mr > 0.0 && mr <= kDomMReMax &&
+                        mi >= 0.0 && mi <= kDomMImMax && abs_m >= kDomAbsMMin;
+        if (!in || n < def) return kOutside;
+    }
+    *n_out = n;
+    return kValid;
+}
+
+inline std::string fmt_e(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%e", v);
+    return b;
+}
+
+// The code and the reference's message for one sphere (host).
+inline int sphere_message(double x, double mr, double mi, long long n_fixed, int allow,
+                          std::string *msg) {
+    if (!(x >= kMinSize && x <= kMaxSize)) {
+        *msg = "the size parameter x must be in [1e-30, 1e6], got " + fmt_e(x);
+        return kInvalid;
+    }
+    if (!(std::isfinite(mr) && std::isfinite(mi))) {
+        *msg = "the refractive index must be finite, got " + fmt_e(mr) + " + " + fmt_e(mi) + "i";
+        return kInvalid;
+    }
+    if (mr < 0.0 || mi < 0.0) {
+        *msg = "the refractive index must have Re m >= 0 and Im m >= 0 (Im m < 0 is a gain "
+               "medium), got " + fmt_e(mr) + " + " + fmt_e(mi) + "i";
+        return kInvalid;
+    }
+    const double abs_m = std::hypot(mr, mi);
+    if (!(abs_m >= kMinIndex && abs_m <= kMaxIndex)) {
+        *msg = "|m| must be in [1e-6, 1e3], got " + fmt_e(abs_m);
+        return kInvalid;
+    }
+    const long long def = default_n_max(x);
+    const long long n = n_fixed > 0 ? n_fixed : def;
+    if (n == 0 || n > kMaxOrder) {
+        *msg = "the number of terms must be in [1, 1048576], got " + std::to_string(n);
+        return kInvalid;
+    }
+    if (!allow) {
+        const bool in = x >= kDomXMin && x <= kDomXMax && mr > 0.0 && mr <= kDomMReMax &&
+                        mi >= 0.0 && mi <= kDomMImMax && abs_m >= kDomAbsMMin;
+        if (!in) {
+            *msg = "x = " + fmt_e(x) + ", m = " + fmt_e(mr) + " + " + fmt_e(mi) +
+                   "i is outside the validated domain x in [1e-6, 2e4], 0 < Re m <= 10, "
+                   "0 <= Im m <= 10, |m| >= 0.1";
+            return kOutside;
+        }
+        if (n < def) {
+            *msg = "n_max = " + std::to_string(n) + " is below the default truncation " +
+                   std::to_string(def) + " for x = " + fmt_e(x);
+            return kOutside;
+        }
+    }
+    *msg = "";
+    return kValid;
+}
+
+}  // namespace scatrs

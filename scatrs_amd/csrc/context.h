@@ -1,0 +1,66 @@
+// Host-side context of the library: the device, the stream, grow-only device buffers and the
+// last error message.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+struct scatrs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    std::string error;
+    // Grow-only device buffers, indexed by purpose.
+    enum Buf {
+        kScratch = 0,
+        kPerm,
+        kKeys,
+        kCounters,
+        kHist,
+        kStaging,
+        kNodes,
+        kTables,
+        kAccum,
+        kMisc,
+        kMisc2,
+        kMisc3,
+        kNumBufs
+    };
+    void *buf[kNumBufs] = {};
+    size_t cap[kNumBufs] = {};
+    void *host_pinned = nullptr;  // small pinned status block
+    bool profiling = false;
+    static constexpr int kMaxEvents = 1024;
+    hipEvent_t ev_begin[kMaxEvents] = {}, ev_end[kMaxEvents] = {};
+    int ev_count = 0;
+};
+
+namespace scatrs {
+
+// A device buffer of at least `bytes`, grown (never shrunk) on demand.
+hipError_t ensure(scatrs_ctx *ctx, int which, size_t bytes);
+
+int fail(scatrs_ctx *ctx, int code, const std::string &msg);
+
+// Events around the dominant kernel of a call, when profiling is on.
+void mark_begin(scatrs_ctx *ctx);
+void mark_end(scatrs_ctx *ctx);
+int check_hip(scatrs_ctx *ctx, hipError_t e, const char *what);
+
+// Validation codes written by the prep kernels.
+enum : int { kValid = 0, kInvalid = 1, kOutside = 2 };
+
+// The status block the prep kernels fill and the host reads back: the first invalid element
+// (atomicMin of its index), its code, the largest N and the sum of N.
+struct Status {
+    unsigned long long first_bad;
+    int code_invalid;  // bitmask of codes seen at first_bad is resolved on the host
+    int n_max;
+    unsigned long long total_terms;
+    unsigned long long first_outside;
+    int pad[2];
+};
+
+}  // namespace scatrs

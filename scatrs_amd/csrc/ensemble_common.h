@@ -1,0 +1,63 @@
+// Shared pieces of the ensemble path: the distributions' densities and radius ranges, and the
+// per-ensemble input record the kernels read.
+#pragma once
+
+#include <cmath>
+
+#include "cx.h"
+
+namespace scatrs {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr int kMaxLayers = 16;
+
+// The radius where the unnormalized density is scaled near 1 (distribution.rs:293-309).
+SCATRS_HD double dist_reference(int kind, double p0, double p1) {
+    if (kind == 0) return p0;
+    const double alpha = (1.0 - 3.0 * p1) / p1;
+    return alpha > 0.0 ? alpha * p0 * p1 : p0;
+}
+
+// Whether the distribution's parameters are valid (distribution.rs:99-185).
+SCATRS_HD bool dist_valid(int kind, double p0, double p1) {
+    if (kind == 0) return __builtin_isfinite(p0) && p0 > 0.0 && __builtin_isfinite(p1) && p1 > 1.0;
+    if (kind == 1) return __builtin_isfinite(p0) && p0 > 0.0 && p1 > 0.0 && p1 < 0.5;
+    return false;
+}
+
+// The fixed ln r range: lognormal ln r_g -+ 5 ln sigma_g; gamma ln a - 8 s .. ln a + 5 s with
+// s = sqrt(b / (1 - 2 b)), the reference's log-width of the gamma peak (distribution.rs:319-323).
+SCATRS_HD void radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi) {
+    if (kind == 0) {
+        const double c = log(p0), h = 5.0 * log(p1);
+        *t_lo = c - h;
+        *t_hi = c + h;
+    } else {
+        const double s = sqrt(p1 / (1.0 - 2.0 * p1));
+        const double c = log(p0);
+        *t_lo = c - 8.0 * s;
+        *t_hi = c + 5.0 * s;
+    }
+}
+
+// ln u(r) and d(ln u)/d(p0, p1), with the reference radius held fixed
+// (distribution.rs:366-430).
+SCATRS_HD double log_density(int kind, double p0, double p1, double r, double *d0, double *d1) {
+    const double r0 = dist_reference(kind, p0, p1);
+    if (kind == 0) {
+        const double mu = log(p0), s = log(p1);
+        const double z = log(r) - mu;
+        if (d0) *d0 = z / (s * s * p0);
+        if (d1) *d1 = (z * z / (s * s * s) - 1.0 / s) / p1;
+        return -z * z / (2.0 * s * s) - log(s) - log(r / r0);
+    }
+    const double a = p0, b = p1;
+    const double alpha = (1.0 - 3.0 * b) / b;
+    const double scale = a * b;
+    const double l = log(r / r0);
+    if (d0) *d0 = (r - r0) * b / (scale * scale);
+    if (d1) *d1 = -l / (b * b) + (r - r0) * a / (scale * scale);
+    return alpha * l - (r - r0) / scale;
+}
+
+}  // namespace scatrs

@@ -1,0 +1,380 @@
+// Per-lane Lorenz-Mie coefficient solver for gfx950: one sphere per lane of a 64-wide
+// wavefront. The downward ratio chains u_k(mx) and u_k(x) are written to a lane-strided
+// scratch column (element k of lane l at base[k * stride + l], so a wavefront's 64 lanes touch
+// 64 consecutive doubles per order) and read back by the upward order loop, which keeps
+// psi, chi, T_n, 1/xi_n, R_n and the running sums in registers.
+//
+// The algorithm is the reference's (scatrs/src/mie/kernel.rs:99-559): the same start index,
+// the same ratio-region numerators free of cancellation, the same absorption identity and the
+// same closed-form derivatives; only the summation order of the efficiency series differs
+// (the g pair term is added when order k+1 arrives).
+#pragma once
+
+#include "cx.h"
+
+namespace scatrs {
+
+constexpr double kEps = 2.220446049250313080847e-16;  // 2^-52
+constexpr double kBits = 52.0;
+constexpr int kMargin = 30;  // ceil(bits / 2) + 4 (mie/kernel.rs:82-84)
+
+// N = ceil(x + 6 x^(1/3) + 3) (scatrs/src/mie/mod.rs:122-124).
+SCATRS_HD long long default_n_max(double x) {
+    const double n = ceil(x + 6.0 * cbrt(x) + 3.0);
+    return (n < 9.0e15) ? (long long)n : (long long)9.0e15;
+}
+
+// The first index of a downward ratio recurrence reaching n_top with the start value
+// forgotten to rounding level (scatrs/src/mie/kernel.rs:99-112).
+SCATRS_HD long long start_index(long long n_top, double rho_abs) {
+    long long k0 = n_top;
+    const long long c15 = (long long)ceil(1.5 * rho_abs);
+    if (c15 > k0) k0 = c15;
+    const double q = (double)(2 * k0 + 1) / rho_abs - 0.5;
+    const long long enclosed = k0 + (long long)ceil(kBits / (2.0 * log2(q))) + 2;
+    if ((double)n_top >= 1.5 * rho_abs) return enclosed;
+    const double s0 = (double)n_top - rho_abs;
+    const double s0_32 = s0 > 0.0 ? s0 * sqrt(s0) : 0.0;
+    const double s = pow(s0_32 + 1.4 * 0.37 * kBits * sqrt(rho_abs), 2.0 / 3.0);
+    const long long asymptotic = (long long)ceil(rho_abs + s) + kMargin;
+    return asymptotic < enclosed ? asymptotic : enclosed;
+}
+
+SCATRS_HD int upward_orders(double x, int n) {
+    if (x >= 1.0) {
+        const double f = floor(x);
+        return f < (double)n ? (int)f : n;
+    }
+    return 0;
+}
+
+__device__ __forceinline__ cx step_complex(int k, cx inv_z, cx u) {
+    const cx c = inv_z * (double)(2 * k + 1);
+    cx den = c - u;
+    if (is_zero(den)) den = real(kEps * (cabs(c) + cabs(u)));
+    return inv(den);
+}
+
+__device__ __forceinline__ double step_real(int k, double inv_rho, double u) {
+    const double c = inv_rho * (double)(2 * k + 1);
+    double den = c - u;
+    if (den == 0.0) den = kEps * (fabs(c) + fabs(u));
+    return 1.0 / den;
+}
+
+// u_k(m x) for k = 1..n_top+1 into col[k * stride] (scatrs/src/mie/kernel.rs:159-205).
+__device__ inline void ratio_chain(cx m, double x, int n_top, cx *col, int stride) {
+    const cx z = m * x;
+    if (m.im == 0.0) {
+        const double inv_rho = 1.0 / z.re;
+        const long long start = start_index(n_top + 1, fabs(z.re));
+        double u = 0.0;
+        for (long long k = start; k >= 1; --k) {
+            u = step_real((int)k, inv_rho, u);
+            if (k <= n_top + 1) col[k * stride] = real(u);
+        }
+    } else {
+        const cx inv_z = inv(z);
+        const long long start = start_index(n_top + 1, cabs(z));
+        cx u = real(0.0);
+        for (long long k = start; k >= 1; --k) {
+            u = step_complex((int)k, inv_z, u);
+            if (k <= n_top + 1) col[k * stride] = u;
+        }
+    }
+}
+
+// u_k(x) above the upward orders, k = max(n_s + 1, 1)..n+1 (scatrs/src/mie/kernel.rs:249-270).
+__device__ inline void x_chain(double x, int n, int n_s, double *col, int stride) {
+    if (n_s >= n) return;
+    const int top = n + 1;
+    const long long start = start_index(top, fabs(x));
+    const int stop = n_s + 1 > 1 ? n_s + 1 : 1;
+    const double inv_x = 1.0 / x;
+    double u = 0.0;
+    for (long long k = start; k >= stop; --k) {
+        u = step_real((int)k, inv_x, u);
+        if (k <= top) col[k * stride] = u;
+    }
+}
+
+// One coefficient's effective log-derivative at a layered sphere's outer surface: as the
+// correction delta to D_n(mx) or, near its poles, as H itself (scatrs/src/mie/kernel.rs:16-27).
+struct OuterSide {
+    cx va, vb;
+    int direct_a, direct_b;
+};
+
+// The outer layer's quantities for a layered sphere's derivatives (mie/kernel.rs:29-38).
+struct OuterDerivs {
+    cx hp[2], hplus[2], extra_m[2];
+};
+
+// No outer sides: a homogeneous sphere.
+struct NoOuter {
+    static constexpr bool kLayered = false;
+    __device__ OuterSide side(int) { return OuterSide{}; }
+    __device__ OuterDerivs derivs(int) { return OuterDerivs{}; }
+};
+
+__device__ __forceinline__ double absorption_term(cx f, cx df, cx l, double n2) {
+    return -((df.im - 2.0 * f.im * l.re) * n2);
+}
+
+// The upward order loop (scatrs/src/mie/kernel.rs:325-559) after ratio_chain and x_chain.
+// Calls sink.order(k, a, b) (or sink.order_d with the four derivatives) for k = 1..n and
+// sink.done(abs_sum, jac). With a layered Outer source the per-order sides replace D_n(mx);
+// the sink then also receives sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md).
+template <bool DERIVS, class Outer, class Sink>
+__device__ inline void mie_orders(double x, cx m, int n, const cx *uz, const double *ux,
+                                  int stride, Outer &outer, Sink &sink) {
+    const cx z = m * x;
+    const int n_s = upward_orders(x, n);
+    const double inv_x = 1.0 / x;
+    const cx inv_z = inv(z);
+    const cx inv_m = inv(m);
+    const cx m2 = m * m;
+    const cx inv_m2 = inv_m * inv_m;
+    const cx m2_minus_1 = (m - 1.0) * (m + 1.0);
+    const cx c_a = -(m2_minus_1 * inv_m2);
+    double sin_x, cos_x;
+    sincos(x, &sin_x, &cos_x);
+    double psi_prev = cos_x, psi = sin_x, chi_prev = -sin_x, chi = cos_x;
+    cx t = real(0.0), inv_xi = real(0.0), r = real(0.0);
+    if (n_s == 0) {
+        t = real(sin_x) * inv(mk(sin_x, -cos_x));
+        inv_xi = mk(sin_x, cos_x);
+        r = mk(0.0, 1.0);
+    }
+    const bool m_is_one = (m.re == 1.0 && m.im == 0.0);
+    double abs_sum = 0.0;
+    double jac0 = 0.0, jac1 = 0.0, jac2 = 0.0;
+    for (int k = 1; k <= n; ++k) {
+        const double kf = (double)k;
+        const double k_x = kf * inv_x;
+        const cx u_z1 = uz[(k + 1) * stride];
+        const cx d = inv_z * (double)(k + 1) - u_z1;
+        OuterSide side{};
+        cx hh_a = d, hh_b = d;
+        if constexpr (Outer::kLayered) {
+            side = outer.side(k);
+            hh_a = side.direct_a ? side.va : d + side.va;
+            hh_b = side.direct_b ? side.vb : d + side.vb;
+        }
+        const cx d_m = hh_a * inv_m;
+        const cx md = hh_b * m;
+        const cx big_a = d_m + k_x;
+        const cx big_b = md + k_x;
+        cx an, bn, inv_da, inv_db, g_a, g_b, h_a, h_b;
+        if (k <= n_s) {
+            const double c = (double)(2 * k - 1) * inv_x;
+            const double psi_next = c * psi - psi_prev;
+            psi_prev = psi;
+            psi = psi_next;
+            const double chi_next = c * chi - chi_prev;
+            chi_prev = chi;
+            chi = chi_next;
+            const cx xi = mk(psi, -chi);
+            const cx xi_prev = mk(psi_prev, -chi_prev);
+            inv_da = inv(big_a * xi - xi_prev);
+            inv_db = inv(big_b * xi - xi_prev);
+            if constexpr (Outer::kLayered) {
+                if (!side.direct_a) {
+                    const cx own = m_is_one ? real(0.0) : ((d * inv_m + k_x) * psi - psi_prev);
+                    an = (own + (side.va * inv_m) * psi) * inv_da;
+                } else {
+                    an = (big_a * psi - psi_prev) * inv_da;
+                }
+                if (!side.direct_b) {
+                    const cx own = m_is_one ? real(0.0) : ((d * m + k_x) * psi - psi_prev);
+                    bn = (own + (m * side.vb) * psi) * inv_db;
+                } else {
+                    bn = (big_b * psi - psi_prev) * inv_db;
+                }
+            } else {
+                (void)m_is_one;
+                an = (big_a * psi - psi_prev) * inv_da;
+                bn = (big_b * psi - psi_prev) * inv_db;
+            }
+            g_a = xi * inv_da;
+            g_b = xi * inv_db;
+            h_a = xi_prev * inv_da;
+            h_b = xi_prev * inv_db;
+            if (k == n_s) {
+                inv_xi = inv(xi);
+                t = real(psi) * inv_xi;
+                r = xi_prev * inv_xi;
+            }
+        } else {
+            r = inv((double)(2 * k - 1) * inv_x - r);
+            t = (t * r) * ux[k * stride];
+            inv_xi = inv_xi * r;
+            const double ux1 = ux[(k + 1) * stride];
+            cx num_a = c_a * ((double)(k + 1) * inv_x) + ux1 - u_z1 * inv_m;
+            cx num_b = ux1 - m * u_z1;
+            if constexpr (Outer::kLayered) {
+                const cx d_x = real((double)(k + 1) * inv_x - ux1);
+                num_a = side.direct_a ? d_m - d_x : num_a + side.va * inv_m;
+                num_b = side.direct_b ? md - d_x : num_b + m * side.vb;
+            }
+            const cx ia = inv(big_a - r);
+            const cx ib = inv(big_b - r);
+            an = (t * num_a) * ia;
+            bn = (t * num_b) * ib;
+            inv_da = inv_xi * ia;
+            inv_db = inv_xi * ib;
+            g_a = ia;
+            g_b = ib;
+            h_a = r * ia;
+            h_b = r * ib;
+        }
+        const double na = norm2(inv_da), nb = norm2(inv_db);
+        const double weight = (double)(2 * k + 1);
+        abs_sum += weight * (-(d_m.im * na) - md.im * nb);
+        if constexpr (!DERIVS) {
+            sink.order(k, an, bn);
+        } else {
+            const double kk1 = kf * (double)(k + 1);
+            const cx ida2 = inv_da * inv_da, idb2 = inv_db * inv_db;
+            const double k_x2 = k_x * inv_x;
+            cx da_dm, db_dm, da_dx, db_dx, fa_m, fb_m, dpa, dpb;
+            if constexpr (!Outer::kLayered) {
+                const cx one_u2 = 1.0 + u_z1 * u_z1;
+                const cx dp = ((u_z1 * 2.0) * inv_z - inv_z * inv_z) * (double)(k + 1) - one_u2;
+                const cx d_plus = u_z1 * weight - z * one_u2;
+                const cx d_minus = d * 2.0 - d_plus;
+                da_dm = (times_i(d_minus) * inv_m2) * ida2;
+                db_dm = -(times_i(d_plus) * idb2);
+                da_dx = ((times_i(m2_minus_1) * (d * d + kk1 * inv_x * inv_x)) * inv_m2) * ida2;
+                db_dx = times_i(m2_minus_1) * idb2;
+                fa_m = -(d_minus * inv_m2);
+                fb_m = d_plus;
+                dpa = dp;
+                dpb = dp;
+            } else {
+                const OuterDerivs od = outer.derivs(k);
+                dpa = od.hp[0];
+                dpb = od.hp[1];
+                const cx minus_a = hh_a * 2.0 - od.hplus[0];
+                fa_m = -(minus_a * inv_m2) + od.extra_m[0] * inv_m;
+                fb_m = od.hplus[1] + m * od.extra_m[1];
+                da_dm = -(times_i(fa_m) * ida2);
+                db_dm = -(times_i(fb_m) * idb2);
+                da_dx = ((times_i(m2_minus_1) * (hh_a * hh_a + kk1 * inv_x * inv_x)) * inv_m2) * ida2;
+                db_dx = times_i(m2_minus_1) * idb2;
+                sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md);
+            }
+            const cx la_x = g_a * (dpa - k_x2 - big_a * k_x + 1.0) + h_a * d_m;
+            const cx lb_x = g_b * (m2 * dpb - k_x2 - big_b * k_x + 1.0) + h_b * md;
+            const cx la_m = fa_m * g_a, lb_m = fb_m * g_b;
+            jac0 += weight * (absorption_term(d_m, dpa, la_x, na) +
+                              absorption_term(md, m2 * dpb, lb_x, nb));
+            jac1 += weight * (absorption_term(d_m, fa_m, la_m, na) +
+                              absorption_term(md, fb_m, lb_m, nb));
+            jac2 += weight * (absorption_term(d_m, times_i(fa_m), times_i(la_m), na) +
+                              absorption_term(md, times_i(fb_m), times_i(lb_m), nb));
+            sink.order_d(k, an, bn, da_dx, da_dm, db_dx, db_dm);
+        }
+    }
+    sink.done(abs_sum, jac0, jac1, jac2);
+}
+
+// The efficiency series (scatrs/src/mie/efficiencies.rs:64-187) accumulated order by order:
+// Q_sca = (2/x^2) sca, Q_back = |back|^2 / x^2, g = 2 G / sca, and with derivatives the sums
+// of efficiencies.rs:89-146 for d/dx, d/dRe m, d/dIm m.
+template <bool DERIVS>
+struct EfficiencySink {
+    double sca = 0.0, g = 0.0, abs_sum = 0.0;
+    cx back = {0.0, 0.0};
+    cx a_prev = {0.0, 0.0}, b_prev = {0.0, 0.0};
+    double jac_abs[3] = {0.0, 0.0, 0.0};
+    double d_sca[3] = {0.0, 0.0, 0.0}, d_g[3] = {0.0, 0.0, 0.0};
+    cx d_back[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    cx da_prev[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    cx db_prev[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+
+    __device__ void order(int k, cx a, cx b) {
+        const double w = (double)(2 * k + 1), kf = (double)k;
+        sca += w * (norm2(a) + norm2(b));
+        const cx t = (a - b) * w;
+        back = (k % 2 == 0) ? back + t : back - t;
+        if (k > 1) {
+            const double km = kf - 1.0;
+            g += km * (km + 2.0) / (km + 1.0) * (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
+        }
+        g += w / (kf * (kf + 1.0)) * re_mul_conj(a, b);
+        a_prev = a;
+        b_prev = b;
+    }
+
+    __device__ void order_d(int k, cx a, cx b, cx da_dx, cx da_dm, cx db_dx, cx db_dm) {
+        const cx dak[3] = {da_dx, da_dm, times_i(da_dm)};
+        const cx dbk[3] = {db_dx, db_dm, times_i(db_dm)};
+        const double w = (double)(2 * k + 1), kf = (double)k;
+        const double own = w / (kf * (kf + 1.0));
+        const double km = kf - 1.0;
+        const double pair = km * (km + 2.0) / (km + 1.0);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            d_sca[p] += w * 2.0 * (re_mul_conj(dak[p], a) + re_mul_conj(dbk[p], b));
+            const cx t = (dak[p] - dbk[p]) * w;
+            d_back[p] = (k % 2 == 0) ? d_back[p] + t : d_back[p] - t;
+            if (k > 1) {
+                d_g[p] += pair * (re_mul_conj(da_prev[p], a) + re_mul_conj(a_prev, dak[p]) +
+                                  re_mul_conj(db_prev[p], b) + re_mul_conj(b_prev, dbk[p]));
+            }
+            d_g[p] += own * (re_mul_conj(dak[p], b) + re_mul_conj(a, dbk[p]));
+            da_prev[p] = dak[p];
+            db_prev[p] = dbk[p];
+        }
+        order(k, a, b);
+    }
+
+    __device__ void stored(int, cx, cx, cx, cx, cx, cx) {}
+
+    // A lossless layered sphere absorbs nothing, and stays lossless as x or Re m changes
+    // (layered/kernel.rs:453-464).
+    __device__ void zero_absorption() {
+        abs_sum = 0.0;
+        jac_abs[0] = 0.0;
+        jac_abs[1] = 0.0;
+    }
+
+    __device__ void done(double abs_s, double j0, double j1, double j2) {
+        abs_sum = abs_s;
+        jac_abs[0] = j0;
+        jac_abs[1] = j1;
+        jac_abs[2] = j2;
+    }
+
+    // [Q_ext, Q_sca, Q_abs, Q_back, g] (efficiencies.rs:169-187).
+    __device__ void values(double x, double q[5]) const {
+        const double inv_x = 1.0 / x, inv_x2 = inv_x * inv_x, two_x2 = 2.0 * inv_x2;
+        const double q_sca = two_x2 * sca, q_abs = two_x2 * abs_sum;
+        q[0] = q_sca + q_abs;
+        q[1] = q_sca;
+        q[2] = q_abs;
+        q[3] = norm2(back) * inv_x2;
+        q[4] = sca == 0.0 ? 0.0 : 2.0 * g / sca;
+    }
+
+    // The Jacobian [d/dx, d/dRe m, d/dIm m][Q_ext..g] (efficiencies.rs:192-230).
+    __device__ void jacobian(double x, const double q[5], bool outer, double out[15]) const {
+        const double inv_x = 1.0 / x, inv_x2 = inv_x * inv_x, two_x2 = 2.0 * inv_x2;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double xt = (p == 0 && outer) ? 2.0 * inv_x : 0.0;
+            const double q_sca = two_x2 * d_sca[p] - xt * q[1];
+            const double q_abs = two_x2 * jac_abs[p] - xt * q[2];
+            const double q_back = 2.0 * re_mul_conj(d_back[p], back) * inv_x2 - xt * q[3];
+            const double gg = sca == 0.0 ? 0.0 : (2.0 * d_g[p] - q[4] * d_sca[p]) / sca;
+            out[p * 5 + 0] = q_sca + q_abs;
+            out[p * 5 + 1] = q_sca;
+            out[p * 5 + 2] = q_abs;
+            out[p * 5 + 3] = q_back;
+            out[p * 5 + 4] = gg;
+        }
+    }
+};
+
+}  // namespace scatrs

@@ -1,0 +1,306 @@
+// scatrs_hip: context management, input checks and the batched-sphere path
+// (scatrs_mie_efficiencies) of the MI355X library. The ensemble path is in ensemble.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/scatrs_hip.h"
+#include "checks.h"
+#include "context.h"
+#include "mie_device.h"
+#include "sort.h"
+#include "util.h"
+
+namespace scatrs {
+
+hipError_t ensure(scatrs_ctx *ctx, int which, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (ctx->cap[which] >= bytes) return hipSuccess;
+    if (ctx->buf[which]) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+        e = hipFree(ctx->buf[which]);
+        if (e != hipSuccess) return e;
+        ctx->buf[which] = nullptr;
+        ctx->cap[which] = 0;
+    }
+    size_t want = std::max(bytes, ctx->cap[which] + ctx->cap[which] / 4);
+    want = (want + 4095) & ~size_t(4095);
+    hipError_t e = hipMalloc(&ctx->buf[which], want);
+    if (e != hipSuccess) {
+        // retry with the exact size
+        e = hipMalloc(&ctx->buf[which], bytes);
+        if (e != hipSuccess) return e;
+        want = bytes;
+    }
+    ctx->cap[which] = want;
+    return hipSuccess;
+}
+
+int fail(scatrs_ctx *ctx, int code, const std::string &msg) {
+    ctx->error = msg;
+    return code;
+}
+
+void mark_begin(scatrs_ctx *ctx) {
+    if (!ctx->profiling || ctx->ev_count >= scatrs_ctx::kMaxEvents) return;
+    const int i = ctx->ev_count;
+    if (!ctx->ev_begin[i]) {
+        (void)hipEventCreate(&ctx->ev_begin[i]);
+        (void)hipEventCreate(&ctx->ev_end[i]);
+    }
+    (void)hipEventRecord(ctx->ev_begin[i], ctx->stream);
+}
+
+void mark_end(scatrs_ctx *ctx) {
+    if (!ctx->profiling || ctx->ev_count >= scatrs_ctx::kMaxEvents) return;
+    (void)hipEventRecord(ctx->ev_end[ctx->ev_count], ctx->stream);
+    ctx->ev_count++;
+}
+
+int check_hip(scatrs_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return SCATRS_OK;
+    return fail(ctx, SCATRS_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Wavefront sum and max of per-lane values, then one atomic per wavefront.
+__device__ __forceinline__ void wave_accumulate(Status *st, int n, bool ok) {
+    unsigned long long sum = ok ? (unsigned long long)n : 0ull;
+    int mx = ok ? n : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off, 64);
+        const int o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&st->total_terms, sum);
+        atomicMax(&st->n_max, mx);
+    }
+}
+
+// Per sphere: the checks of Mie::compute (scatrs/src/mie/mod.rs:274-320), the number of terms,
+// and its N bucket for the largest-first order.
+__global__ void mie_prep(const double *x, const double *m, int64_t len, int64_t n_fixed,
+                         int allow_unvalidated, unsigned int *hist, int *keys, Status *st) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int n = 0;
+    bool ok = false;
+    if (i < len) {
+        const double xi = x[i], mr = m[2 * i], mi = m[2 * i + 1];
+        long long nn = 0;
+        const int code = sphere_code(xi, mr, mi, n_fixed, allow_unvalidated, &nn);
+        if (code == kInvalid) atomicMin(&st->first_bad, (unsigned long long)i);
+        if (code == kOutside) atomicMin(&st->first_outside, (unsigned long long)i);
+        ok = code == kValid;
+        n = ok ? (int)nn : 0;
+        const int key = bucket_of(n);
+        keys[i] = key;
+        atomicAdd(&hist[key], 1u);
+    }
+    wave_accumulate(st, n, ok);
+}
+
+// The persistent solve: each wavefront takes 64 spheres at a time, largest N first, from a
+// shared counter; each lane solves one sphere with its ratio columns in the wavefront's
+// scratch slab.
+template <bool DERIVS>
+__global__ __launch_bounds__(256) void mie_batch(const double *__restrict__ x,
+                                                 const double *__restrict__ m, int64_t len,
+                                                 int64_t n_fixed, const int *__restrict__ perm,
+                                                 unsigned long long *chunk_counter, cx *uz_base,
+                                                 double *ux_base, int64_t rows,
+                                                 double *__restrict__ values,
+                                                 double *__restrict__ jac) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *uz = uz_base + wave * rows * 64 + lane;
+    double *ux = ux_base + wave * rows * 64 + lane;
+    while (true) {
+        unsigned long long chunk = 0;
+        if (lane == 0) chunk = atomicAdd(chunk_counter, 1ull);
+        chunk = __shfl(chunk, 0, 64);
+        const int64_t idx = (int64_t)chunk * 64 + lane;
+        if ((int64_t)chunk * 64 >= len) break;
+        if (idx < len) {
+            const int i = perm[idx];
+            const double xi = x[i];
+            const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
+            const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
+            ratio_chain(mi, xi, n, uz, 64);
+            x_chain(xi, n, upward_orders(xi, n), ux, 64);
+            EfficiencySink<DERIVS> s;
+            NoOuter none;
+            mie_orders<DERIVS>(xi, mi, n, uz, ux, 64, none, s);
+            double q[5];
+            s.values(xi, q);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) values[k * len + i] = q[k];
+            if constexpr (DERIVS) {
+                double j[15];
+                s.jacobian(xi, q, true, j);
+#pragma unroll
+                for (int k = 0; k < 15; ++k) jac[k * len + i] = j[k];
+            }
+        }
+    }
+}
+
+}  // namespace scatrs
+
+using namespace scatrs;
+
+extern "C" {
+
+int scatrs_abi_version(void) { return SCATRS_ABI_VERSION; }
+
+int scatrs_create(int device, void *stream, scatrs_ctx **out) {
+    if (!out) return SCATRS_INVALID_INPUT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return SCATRS_DEVICE_ERROR;
+    if (hipSetDevice(device) != hipSuccess) return SCATRS_DEVICE_ERROR;
+    scatrs_ctx *ctx = new scatrs_ctx();
+    ctx->device = device;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        ctx->cus = cus;
+    if (hipHostMalloc(&ctx->host_pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return SCATRS_DEVICE_ERROR;
+    }
+    *out = ctx;
+    return SCATRS_OK;
+}
+
+int scatrs_destroy(scatrs_ctx *ctx) {
+    if (!ctx) return SCATRS_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    else (void)hipDeviceSynchronize();
+    for (int i = 0; i < scatrs_ctx::kNumBufs; ++i)
+        if (ctx->buf[i]) (void)hipFree(ctx->buf[i]);
+    if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+    for (int i = 0; i < scatrs_ctx::kMaxEvents; ++i) {
+        if (ctx->ev_begin[i]) (void)hipEventDestroy(ctx->ev_begin[i]);
+        if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);
+    }
+    delete ctx;
+    return SCATRS_OK;
+}
+
+int scatrs_set_stream(scatrs_ctx *ctx, void *stream) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    return SCATRS_OK;
+}
+
+const char *scatrs_last_error(const scatrs_ctx *ctx) { return ctx ? ctx->error.c_str() : ""; }
+
+int scatrs_set_profiling(scatrs_ctx *ctx, int on) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->profiling = on != 0;
+    ctx->ev_count = 0;
+    return SCATRS_OK;
+}
+
+int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap) {
+    if (!ctx) return 0;
+    int n = 0;
+    for (int i = 0; i < ctx->ev_count && n < cap; ++i) {
+        if (hipEventSynchronize(ctx->ev_end[i]) != hipSuccess) break;
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, ctx->ev_begin[i], ctx->ev_end[i]) != hipSuccess) break;
+        ms_out[n++] = (double)ms;
+    }
+    ctx->ev_count = 0;
+    return n;
+}
+
+int64_t scatrs_default_n_max(double x) { return (int64_t)default_n_max(x); }
+
+int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+                            int64_t n_max, int allow_unvalidated, double *d_values,
+                            double *d_jacobian, int64_t *total_terms) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->error.clear();
+    if (total_terms) *total_terms = 0;
+    if (len < 0) return fail(ctx, SCATRS_INVALID_INPUT, "the number of spheres must be >= 0");
+    if (len == 0) return SCATRS_OK;
+    if (len > (int64_t)INT32_MAX)
+        return fail(ctx, SCATRS_INVALID_INPUT, "at most 2^31 - 1 spheres per call");
+    if (!d_x || !d_m || !d_values)
+        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    int rc;
+    if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, sizeof(Status) + 64), "alloc")))
+        return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHist, 2 * kBuckets * sizeof(unsigned)), "alloc")))
+        return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kKeys, (size_t)len * sizeof(int)), "alloc")))
+        return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kPerm, (size_t)len * sizeof(int)), "alloc")))
+        return rc;
+    Status *st = static_cast<Status *>(ctx->buf[scatrs_ctx::kCounters]);
+    unsigned long long *chunk_counter =
+        reinterpret_cast<unsigned long long *>(static_cast<char *>(ctx->buf[scatrs_ctx::kCounters]) + sizeof(Status));
+    unsigned *hist = static_cast<unsigned *>(ctx->buf[scatrs_ctx::kHist]);
+    int *keys = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
+    int *perm = static_cast<int *>(ctx->buf[scatrs_ctx::kPerm]);
+    hipStream_t s = ctx->stream;
+    zero_status(ctx, st);
+    fill_u64<<<1, 1, 0, s>>>(chunk_counter, 0ull, 1);
+    fill_u32<<<(2 * kBuckets + 255) / 256, 256, 0, s>>>(hist, 0u, 2 * kBuckets);
+    const int threads = 256;
+    const int64_t blocks = (len + threads - 1) / threads;
+    mie_prep<<<(unsigned)blocks, threads, 0, s>>>(d_x, d_m, len, n_max, allow_unvalidated, hist,
+                                                    keys, st);
+    Status *hst = static_cast<Status *>(ctx->host_pinned);
+    if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
+                        "status copy")))
+        return rc;
+    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "prep"))) return rc;
+    const unsigned long long bad = std::min(hst->first_bad, hst->first_outside);
+    if (bad != ~0ull) {
+        double h[3];
+        (void)hipMemcpy(&h[0], d_x + bad, sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&h[1], d_m + 2 * bad, 2 * sizeof(double), hipMemcpyDeviceToHost);
+        std::string msg;
+        const int code = sphere_message(h[0], h[1], h[2], n_max, allow_unvalidated, &msg);
+        return fail(ctx, code == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
+                    "element " + std::to_string(bad) + ": " + msg);
+    }
+    const int64_t nmax = hst->n_max;
+    if (total_terms) *total_terms = (int64_t)hst->total_terms;
+    sort_by_keys(keys, hist, perm, len, s);
+    // Persistent wavefronts: 8 per CU, fewer if the scratch columns would pass 8 GiB.
+    const int64_t rows = nmax + 2;
+    const size_t per_wave = (size_t)rows * 64 * (sizeof(cx) + sizeof(double));
+    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (len + 63) / 64);
+    waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((8ull << 30) / per_wave)));
+    waves = (waves + 3) / 4 * 4;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves), "scratch")))
+        return rc;
+    cx *uz = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
+    double *ux = reinterpret_cast<double *>(uz + (size_t)waves * rows * 64);
+    mark_begin(ctx);
+    if (d_jacobian)
+        mie_batch<true><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, perm,
+                                                             chunk_counter, uz, ux, rows,
+                                                             d_values, d_jacobian);
+    else
+        mie_batch<false><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, perm,
+                                                              chunk_counter, uz, ux, rows,
+                                                              d_values, nullptr);
+    mark_end(ctx);
+    return check_hip(ctx, hipGetLastError(), "mie_batch launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""Ensembles of spheres and layered spheres over a size distribution, on the GPU.
+
+Mirrors ``scatrs.Ensemble(particle, size).compute(wavelength, greek=..., normalized=...)``
+(python/scatrs/ensemble.py:484-591) and its result layout (SCALARS order,
+python/scatrs/ensemble.py:51-63; Greek and phase arrays of ensemble.py:110-140), on the fixed
+discrete path of the library: ``n_radius`` Gauss-Legendre nodes in ln r over the
+distribution's fixed range and the phase matrix at the ``n_angles`` Gauss-Legendre cosines.
+The reference's adaptive Gauss-Kronrod integration (its ``rtol``, ``max_panels``, error
+estimates and ``converged`` flags) is not part of this path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .errors import InvalidInputError
+from .particles import LayeredSphere, Sphere
+from .size import Gamma, LogNormal, check
+
+__all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
+           "ensemble_fixed_device"]
+
+SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
+           "effective_radius", "mean_area", "mean_volume")
+
+
+def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, n_radius: int = 128,
+                          n_angles: int = 128, order: int | None = 32, normalized: bool = True,
+                          allow_unvalidated: bool = False, want_phase: bool = True):
+    """Device-resident batch: every argument a CUDA tensor with one row per ensemble
+    (kind int32; p0, p1, wavelength float64; m complex128 (count,) for spheres or (count, L)
+    for layered spheres; fractions float64 (count, L-1) for layered spheres). Returns
+    (scalars (count, 11), greek (count, 6, order+1) or None, phase (count, 6, n_angles) or
+    None, total Mie terms)."""
+    import torch
+
+    dev = p0.device
+    count = p0.numel()
+    layers = 1 if m.dim() == 1 else m.shape[1]
+    if layers > 1 and (fractions is None or fractions.shape != (count, layers - 1)):
+        raise InvalidInputError("layered spheres need fractions of shape (count, L - 1)")
+    args = [kind.to(torch.int32).contiguous(), p0.contiguous(), p1.contiguous(),
+            wavelength.contiguous(), m.contiguous()]
+    fr = None if layers == 1 else fractions.contiguous()
+    scalars = torch.empty((count, 11), dtype=torch.float64, device=dev)
+    greek = None if order is None else torch.empty((count, 6, order + 1), dtype=torch.float64,
+                                                   device=dev)
+    phase = torch.empty((count, 6, n_angles), dtype=torch.float64, device=dev) if want_phase \
+        else None
+    ctx = _native.context(dev.index)
+    terms = C.c_int64(0)
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+    rc = ctx.lib.scatrs_ensemble_fixed(
+        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(args[3]), layers, p(fr),
+        p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
+        int(bool(normalized)), int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase),
+        C.byref(terms))
+    ctx.check(rc)
+    return scalars, greek, phase, terms.value
+
+
+@dataclass(frozen=True, eq=False)
+class EnsembleGreek:
+    """Greek coefficients, arrays of shape grid + wavelength.shape + (L + 1,)."""
+
+    alpha1: np.ndarray
+    alpha2: np.ndarray
+    alpha3: np.ndarray
+    alpha4: np.ndarray
+    beta1: np.ndarray
+    beta2: np.ndarray
+
+
+@dataclass(frozen=True, eq=False)
+class EnsemblePhase:
+    """The phase matrix at the cosines ``mu`` (the n_angles Gauss-Legendre nodes), arrays of
+    shape grid + wavelength.shape + (n_angles,); f22 = f11 and f44 = f33 for spheres."""
+
+    mu: np.ndarray
+    f11: np.ndarray
+    f12: np.ndarray
+    f22: np.ndarray
+    f33: np.ndarray
+    f34: np.ndarray
+    f44: np.ndarray
+
+
+@dataclass(frozen=True, eq=False)
+class EnsembleResult:
+    """Outputs of an ensemble, each of shape grid + wavelength.shape (SCALARS order)."""
+
+    c_ext: np.ndarray
+    c_sca: np.ndarray
+    c_abs: np.ndarray
+    c_back: np.ndarray
+    ssa: np.ndarray
+    g: np.ndarray
+    lidar_ratio: np.ndarray
+    depolarization: np.ndarray
+    effective_radius: np.ndarray
+    mean_area: np.ndarray
+    mean_volume: np.ndarray
+    greek: EnsembleGreek | None = None
+    phase: EnsemblePhase | None = None
+    wavelength: np.ndarray = field(default=None)
+    terms: int = 0
+
+
+def _frozen(a: np.ndarray) -> np.ndarray:
+    a.flags.writeable = False
+    return a
+
+
+@dataclass(frozen=True)
+class Ensemble:
+    """Particles of one kind over a size distribution (python/scatrs/ensemble.py:484-529).
+
+    ``size`` may have array-valued parameters: a grid of distributions, computed at once
+    (the results have the grid's axes first)."""
+
+    particle: Sphere | LayeredSphere
+    size: LogNormal | Gamma
+
+    def __post_init__(self) -> None:
+        if not isinstance(self.particle, (Sphere, LayeredSphere)):
+            raise TypeError("particle must be a Sphere or a LayeredSphere, not "
+                            f"{type(self.particle).__name__}")
+        check(self.size)
+
+    def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
+                n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
+                phase: bool = True) -> EnsembleResult:
+        """Averages over the ensemble at each wavelength on the fixed discrete path."""
+        import torch
+
+        wl = np.asarray(wavelength, np.float64)
+        p0, p1 = self.size.params()
+        grid = np.broadcast(p0, p1).shape
+        p0g, p1g = (np.broadcast_to(v, grid).reshape(-1) for v in (p0, p1))
+        nd, nw = p0g.size, wl.size
+        m = self.particle.indices(wl).reshape(self.particle.layers, -1)  # (L, nw)
+        L = self.particle.layers
+        dev = torch.device("cuda", torch.cuda.current_device())
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        kind = np.full(nd * nw, self.size.kind, np.int32)
+        mm = np.tile(m.T, (nd, 1))  # (nd * nw, L)
+        fr = None
+        if L > 1:
+            fr = t(np.tile(self.particle.fractions(), (nd * nw, 1)))
+        sc, gr, ph, terms = ensemble_fixed_device(
+            t(kind), t(np.repeat(p0g, nw)), t(np.repeat(p1g, nw)), t(np.tile(wl.reshape(-1), nd)),
+            t(mm[:, 0] if L == 1 else mm), fr, n_radius=n_radius, n_angles=n_angles,
+            order=greek, normalized=normalized, allow_unvalidated=allow_unvalidated,
+            want_phase=phase)
+        shape = grid + wl.shape
+        s = sc.cpu().numpy()
+        fields = [_frozen(s[:, k].reshape(shape)) for k in range(11)]
+        g = None
+        if gr is not None:
+            a = gr.cpu().numpy()
+            g = EnsembleGreek(*(_frozen(a[:, k].reshape(shape + (a.shape[-1],)))
+                                for k in range(6)))
+        p = None
+        if ph is not None:
+            from .quadrature import gauss_legendre
+
+            a = ph.cpu().numpy()
+            mu, _ = gauss_legendre(n_angles)
+            p = EnsemblePhase(_frozen(mu), *(_frozen(a[:, k].reshape(shape + (n_angles,)))
+                                             for k in range(6)))
+        return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()),
+                              terms=terms)

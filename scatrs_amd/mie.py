@@ -1,0 +1,105 @@
+"""Batched homogeneous spheres on the GPU: the efficiencies and their Jacobian.
+
+Mirrors ``scatrs.mie.efficiencies`` (python/scatrs/mie.py:390-416) and the wire layout of
+``_scatrs.mie_efficiencies`` (scatrs-py/src/mie.rs:303-364): values (5, len) in the order
+Q_ext, Q_sca, Q_abs, Q_back, g and the Jacobian (3, 5, len) with respect to x, Re m, Im m.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _native
+
+__all__ = ["MieEfficiencies", "default_n_max", "efficiencies", "efficiencies_device"]
+
+_PARAMS = ("x", "m_real", "m_imag")
+
+
+def default_n_max(x) -> np.ndarray:
+    """N = ceil(x + 6 x^(1/3) + 3) (scatrs/src/mie/mod.rs:122-124)."""
+    xs = np.asarray(x, dtype=np.float64)
+    n = np.ceil(xs + 6.0 * np.cbrt(xs) + 3.0)
+    return n.astype(np.int64)
+
+
+@dataclass(frozen=True, eq=False)
+class MieEfficiencies:
+    """Efficiencies and asymmetry parameter, each an array of the broadcast shape of x and m."""
+
+    q_ext: np.ndarray
+    q_sca: np.ndarray
+    q_abs: np.ndarray
+    q_back: np.ndarray
+    g: np.ndarray
+    params: tuple[str, ...] = ()
+    _jacobian: np.ndarray | None = field(default=None, repr=False)
+
+    def d(self, param: str) -> "MieEfficiencies":
+        """The derivatives with respect to ``param`` ("x", "m_real" or "m_imag")."""
+        if param not in self.params:
+            raise ValueError(f"no derivative with respect to {param!r}; computed: {self.params}")
+        rows = self._jacobian[_PARAMS.index(param)]
+        return MieEfficiencies(*rows)
+
+
+def _parameters(derivatives: bool | Sequence[str]) -> tuple[str, ...]:
+    if isinstance(derivatives, (bool, np.bool_)):
+        return _PARAMS if derivatives else ()
+    params = tuple(derivatives)
+    for p in params:
+        if p not in _PARAMS:
+            raise ValueError(f"unknown parameter {p!r}; expected one of {_PARAMS}")
+    return params
+
+
+def efficiencies_device(x, m, *, n_max: int | None = None, derivatives: bool = False,
+                        allow_unvalidated: bool = False):
+    """Device-resident form: ``x`` a float64 CUDA tensor (len,), ``m`` a complex128 CUDA tensor
+    (len,); returns (values (5, len), jacobian (3, 5, len) or None, total terms)."""
+    import torch
+
+    if x.dtype != torch.float64 or m.dtype != torch.complex128:
+        raise TypeError("x must be float64 and m complex128")
+    if x.device.type != "cuda" or m.device != x.device:
+        raise ValueError("x and m must be on the same GPU")
+    x = x.contiguous().reshape(-1)
+    m = m.contiguous().reshape(-1)
+    if m.numel() != x.numel():
+        raise ValueError("x and m must have the same number of elements")
+    n = x.numel()
+    ctx = _native.context(x.device.index)
+    values = torch.empty((5, n), dtype=torch.float64, device=x.device)
+    jac = torch.empty((3, 5, n), dtype=torch.float64, device=x.device) if derivatives else None
+    terms = C.c_int64(0)
+    rc = ctx.lib.scatrs_mie_efficiencies(
+        ctx.handle, C.c_void_p(x.data_ptr()), C.c_void_p(m.data_ptr()), n,
+        0 if n_max is None else int(n_max), int(bool(allow_unvalidated)),
+        C.c_void_p(values.data_ptr()), C.c_void_p(jac.data_ptr()) if jac is not None else None,
+        C.byref(terms))
+    ctx.check(rc)
+    return values, jac, terms.value
+
+
+def efficiencies(x, m, *, n_max: int | None = None, derivatives: bool | Sequence[str] = False,
+                 allow_unvalidated: bool = False) -> MieEfficiencies:
+    """Efficiencies and asymmetry parameter of spheres (host arrays in, host arrays out)."""
+    import torch
+
+    params = _parameters(derivatives)
+    xb, mb = np.broadcast_arrays(np.asarray(x, dtype=np.float64),
+                                 np.asarray(m, dtype=np.complex128))
+    shape = xb.shape
+    dev = torch.device("cuda", torch.cuda.current_device())
+    xt = torch.from_numpy(np.ascontiguousarray(xb).reshape(-1)).to(dev)
+    mt = torch.from_numpy(np.ascontiguousarray(mb).reshape(-1)).to(dev)
+    values, jac, _ = efficiencies_device(xt, mt, n_max=n_max, derivatives=bool(params),
+                                         allow_unvalidated=allow_unvalidated)
+    v = values.cpu().numpy()
+    fields = [v[k].reshape(shape) for k in range(5)]
+    j = None if jac is None else jac.cpu().numpy().reshape((3, 5, *shape))
+    return MieEfficiencies(*fields, params=params, _jacobian=j)
