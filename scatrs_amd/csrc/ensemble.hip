@@ -101,11 +101,13 @@ __device__ int layered_code(double x, const double *fr, const double *m, int L, 
     return kValid;
 }
 
-__global__ void ens_prep(EnsArgs a, int64_t count, int allow, int *node_n, Status *st) {
+__global__ void __launch_bounds__(256) ens_prep(EnsArgs a, int64_t count, int allow,
+                                                int *node_n, Partial *partial) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t total = count * a.n_radius;
     int n = 0;
     bool ok = false;
+    unsigned long long bad = ~0ull, outside = ~0ull;
     if (g < total) {
         const int64_t e = g / a.n_radius, i = g % a.n_radius;
         int code = kValid;
@@ -127,23 +129,12 @@ __global__ void ens_prep(EnsArgs a, int64_t count, int allow, int *node_n, Statu
             code = c;
             n = (int)nn;
         }
-        if ((code & 15) == kInvalid) atomicMin(&st->first_bad, (unsigned long long)g);
-        if ((code & 15) == kOutside) atomicMin(&st->first_outside, (unsigned long long)g);
+        if ((code & 15) == kInvalid) bad = (unsigned long long)g;
+        if ((code & 15) == kOutside) outside = (unsigned long long)g;
         ok = code == kValid;
         node_n[g] = ok ? n : 0;
     }
-    unsigned long long sum = ok ? (unsigned long long)n : 0ull;
-    int mx = ok ? n : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        sum += __shfl_xor(sum, off, 64);
-        const int o = __shfl_xor(mx, off, 64);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&st->total_terms, sum);
-        atomicMax(&st->n_max, mx);
-    }
+    block_partial(n, ok, bad, outside, partial);
 }
 
 // Exclusive scan of N over each ensemble's nodes: node_off (terms) and ens_terms.
@@ -216,8 +207,11 @@ __global__ __launch_bounds__(256) void ens_nodes(EnsArgs a, int64_t e0, int64_t 
                                                  unsigned long long *counter, cx *scratch,
                                                  int64_t rows, cx *staging, double *vals,
                                                  double *tabc) {
+    __shared__ cx ztile[L == 1 ? 4 : 1][L == 1 ? kZBlock * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *tile = L == 1 ? &ztile[threadIdx.x >> 6][lane] : nullptr;
+    (void)tile;
     constexpr int kCols = LayeredColumns<L>::kComplex;
     cx *slab = scratch + wave * rows * 64 * (kCols + 1) + lane;
     while (true) {
@@ -237,12 +231,13 @@ __global__ __launch_bounds__(256) void ens_nodes(EnsArgs a, int64_t e0, int64_t 
             sink.b_out = staging + 2 * off + n;
             if constexpr (L == 1) {
                 const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
-                cx *uz = slab;
-                double *ux = reinterpret_cast<double *>(slab + rows * 64);
-                ratio_chain(m, geo.x, n, uz, 64);
+                // Checkpoints in the first column, u_k(x) in the second, the block being
+                // used in the wavefront's LDS tile.
+                double *ux = real_column(slab, rows * 64);
+                const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
                 x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
                 NoOuter none;
-                mie_orders<false>(geo.x, m, n, uz, ux, 64, none, sink);
+                mie_orders<false>(geo.x, m, n, zs, ux, 64, none, sink);
             } else {
                 layered_solve<L, false>(geo.x, a.fr + e * (L - 1), a.m + 2 * e * L, n, slab,
                                         rows, sink);
@@ -602,8 +597,13 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
     int64_t *node_off = static_cast<int64_t *>(ctx->buf[scatrs_ctx::kPerm]);
     int64_t *ens_terms = node_off + total;
     int64_t *ens_base = ens_terms + count;
-    zero_status(ctx, st);
-    ens_prep<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(a, count, allow_unvalidated, node_n, st);
+    const int64_t pblocks = (total + 255) / 256;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHist, (size_t)pblocks * sizeof(Partial)),
+                        "alloc")))
+        return rc;
+    Partial *partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kHist]);
+    ens_prep<<<(unsigned)pblocks, 256, 0, s>>>(a, count, allow_unvalidated, node_n, partial);
+    reduce_status<<<1, 1024, 0, s>>>(partial, pblocks, st);
     ens_offsets<<<(unsigned)count, 256, 0, s>>>(node_n, n_radius, node_off, ens_terms);
     Status *hst = static_cast<Status *>(ctx->host_pinned);
     if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),

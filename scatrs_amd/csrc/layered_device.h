@@ -320,7 +320,7 @@ __device__ inline void layered_solve(double x, const double *fr, const double *m
     }
     o.slab = slab;
     o.cs = cs;
-    double *ux = reinterpret_cast<double *>(slab + (2 * L - 1) * cs);
+    double *ux = real_column(slab, (2 * L - 1) * cs);
     ratio_chain(o.m[L - 1], x, n, slab, 64);
     x_chain(x, n, upward_orders(x, n), ux, 64);
     ratio_chain(o.m[0], o.xl[0], n, slab + cs, 64);
@@ -330,7 +330,8 @@ __device__ inline void layered_solve(double x, const double *fr, const double *m
         if (l < L - 1) ratio_chain(o.m[l], o.xl[l], n, slab + (2 * l + 1) * cs, 64);
     }
     o.init();
-    mie_orders<DERIVS>(x, o.m[L - 1], n, slab, ux, 64, o, sink);
+    const ZColumn zs{slab, 64};
+    mie_orders<DERIVS>(x, o.m[L - 1], n, zs, ux, 64, o, sink);
     bool lossless = true;
 #pragma unroll
     for (int l = 0; l < L; ++l) lossless = lossless && o.m[l].im == 0.0;

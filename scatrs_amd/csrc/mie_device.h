@@ -63,7 +63,7 @@ __device__ __forceinline__ double step_real(int k, double inv_rho, double u) {
 }
 
 // u_k(m x) for k = 1..n_top+1 into col[k * stride] (scatrs/src/mie/kernel.rs:159-205).
-__device__ inline void ratio_chain(cx m, double x, int n_top, cx *col, int stride) {
+__device__ __forceinline__ void ratio_chain(cx m, double x, int n_top, cx *col, int stride) {
     const cx z = m * x;
     if (m.im == 0.0) {
         const double inv_rho = 1.0 / z.re;
@@ -85,7 +85,7 @@ __device__ inline void ratio_chain(cx m, double x, int n_top, cx *col, int strid
 }
 
 // u_k(x) above the upward orders, k = max(n_s + 1, 1)..n+1 (scatrs/src/mie/kernel.rs:249-270).
-__device__ inline void x_chain(double x, int n, int n_s, double *col, int stride) {
+__device__ __forceinline__ void x_chain(double x, int n, int n_s, double *col, int stride) {
     if (n_s >= n) return;
     const int top = n + 1;
     const long long start = start_index(top, fabs(x));
@@ -96,6 +96,96 @@ __device__ inline void x_chain(double x, int n, int n_s, double *col, int stride
         u = step_real((int)k, inv_x, u);
         if (k <= top) col[k * stride] = u;
     }
+}
+
+// The order loop reads u_{k+1}(mx) in ascending k, after a downward pass from the start
+// index. Two sources:
+//  * ZColumn: the downward pass stored every value in a lane-strided global column;
+//  * ZCheckpoint: the downward pass kept one value per block of kZBlock indices (the value
+//    just above each block) in a short global column; the order loop regenerates each block
+//    into the wavefront's LDS tile by the same downward steps before using it, so the full
+//    column never goes through HBM.
+constexpr int kZBlock = 16;
+
+struct ZColumn {
+    static constexpr bool kBlocked = false;
+    const cx *uz;
+    int stride;
+};
+
+struct ZCheckpoint {
+    static constexpr bool kBlocked = true;
+    const cx *ckpt;     // ckpt[j * stride]: u at index 2 + (j + 1) kZBlock
+    int stride;
+    cx *lds;            // the lane's tile: element i at lds[i * 64]
+    cx inv_z;           // 1/(mx), complex chain
+    double inv_rho;     // 1/Re(mx), real chain
+    bool real_chain;
+    long long start;
+
+    // u_i for i in [2 + j B, 2 + j B + B - 1] into lds[(i - 2 - j B) * 64], regenerated
+    // downward from the checkpoint above the block (or from 0 above the start index).
+    __device__ __forceinline__ void regen(int j) const {
+        const long long lo = 2 + (long long)j * kZBlock;
+        const long long top = lo + kZBlock - 1;
+        const long long hi = top < start ? top : start;
+        if (real_chain) {
+            double u = (top + 1 <= start) ? ckpt[(int64_t)j * stride].re : 0.0;
+            for (long long i = hi; i >= lo; --i) {
+                u = step_real((int)i, inv_rho, u);
+                lds[(i - lo) * 64] = real(u);
+            }
+        } else {
+            cx u = (top + 1 <= start) ? ckpt[(int64_t)j * stride] : real(0.0);
+            for (long long i = hi; i >= lo; --i) {
+                u = step_complex((int)i, inv_z, u);
+                lds[(i - lo) * 64] = u;
+            }
+        }
+    }
+};
+
+// The downward pass of u_k(m x) from the start index, keeping only the checkpoints of
+// ZCheckpoint (indices 2 + (j + 1) kZBlock for the blocks that hold orders 1..n).
+__device__ __forceinline__ ZCheckpoint ratio_chain_ckpt(cx m, double x, int n, cx *ckpt,
+                                                       int stride, cx *lds) {
+    ZCheckpoint zs;
+    const cx z = m * x;
+    zs.ckpt = ckpt;
+    zs.stride = stride;
+    zs.lds = lds;
+    zs.real_chain = m.im == 0.0;
+    const int jmax = (n - 1) / kZBlock;  // block of the last index used, n + 1
+    const long long cmax = 2 + (long long)(jmax + 1) * kZBlock;
+    const long long c0 = 2 + kZBlock;
+    if (zs.real_chain) {
+        zs.inv_rho = 1.0 / z.re;
+        zs.inv_z = real(0.0);
+        zs.start = start_index(n + 1, fabs(z.re));
+        double u = 0.0;
+        for (long long k = zs.start; k >= c0; --k) {
+            u = step_real((int)k, zs.inv_rho, u);
+            if (k <= cmax && (k - 2) % kZBlock == 0) ckpt[((k - 2) / kZBlock - 1) * stride] = real(u);
+        }
+    } else {
+        zs.inv_z = inv(z);
+        zs.inv_rho = 0.0;
+        zs.start = start_index(n + 1, cabs(z));
+        cx u = real(0.0);
+        for (long long k = zs.start; k >= c0; --k) {
+            u = step_complex((int)k, zs.inv_z, u);
+            if (k <= cmax && (k - 2) % kZBlock == 0) ckpt[((k - 2) / kZBlock - 1) * stride] = u;
+        }
+    }
+    return zs;
+}
+
+// The real column starting `col` complex elements past a lane's complex slab pointer
+// (slab_lane = slab + lane): a double column must be offset by the lane in doubles, not in
+// complex elements, or lanes 32 apart would share slots.
+__device__ __forceinline__ double *real_column(cx *slab_lane, int64_t col) {
+    const int lane = threadIdx.x & 63;
+    return reinterpret_cast<double *>(slab_lane - lane + col) + lane;
 }
 
 // One coefficient's effective log-derivative at a layered sphere's outer surface: as the
@@ -125,9 +215,10 @@ __device__ __forceinline__ double absorption_term(cx f, cx df, cx l, double n2) 
 // Calls sink.order(k, a, b) (or sink.order_d with the four derivatives) for k = 1..n and
 // sink.done(abs_sum, jac). With a layered Outer source the per-order sides replace D_n(mx);
 // the sink then also receives sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md).
-template <bool DERIVS, class Outer, class Sink>
-__device__ inline void mie_orders(double x, cx m, int n, const cx *uz, const double *ux,
-                                  int stride, Outer &outer, Sink &sink) {
+template <bool DERIVS, class ZSrc, class Outer, class Sink>
+__device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs,
+                                           const double *ux, int stride, Outer &outer,
+                                           Sink &sink) {
     const cx z = m * x;
     const int n_s = upward_orders(x, n);
     const double inv_x = 1.0 / x;
@@ -149,10 +240,11 @@ __device__ inline void mie_orders(double x, cx m, int n, const cx *uz, const dou
     const bool m_is_one = (m.re == 1.0 && m.im == 0.0);
     double abs_sum = 0.0;
     double jac0 = 0.0, jac1 = 0.0, jac2 = 0.0;
-    for (int k = 1; k <= n; ++k) {
+    // One order of the loop, with u_{k+1}(mx) in registers.
+    auto order_step = [&](int k, const cx u_z1) {
         const double kf = (double)k;
         const double k_x = kf * inv_x;
-        const cx u_z1 = uz[(k + 1) * stride];
+
         const cx d = inv_z * (double)(k + 1) - u_z1;
         OuterSide side{};
         cx hh_a = d, hh_b = d;
@@ -275,6 +367,29 @@ __device__ inline void mie_orders(double x, cx m, int n, const cx *uz, const dou
                               absorption_term(md, times_i(fb_m), times_i(lb_m), nb));
             sink.order_d(k, an, bn, da_dx, da_dm, db_dx, db_dm);
         }
+    };
+    if constexpr (ZSrc::kBlocked) {
+        for (int j = 0; 1 + j * kZBlock <= n; ++j) {
+            zs.regen(j);
+            const int k0 = 1 + j * kZBlock;
+            const int kend = n < k0 + kZBlock - 1 ? n : k0 + kZBlock - 1;
+            for (int k = k0; k <= kend; ++k) order_step(k, zs.lds[(k - k0) * 64]);
+        }
+    } else {
+        // Read two orders ahead with one prefetch buffer per order of a pair, so every load
+        // has a whole order of arithmetic to arrive and stays in its registers.
+        const cx *uz = zs.uz;
+        const int zst = zs.stride;
+        cx pa = uz[2 * zst];
+        cx pb = n >= 2 ? uz[3 * zst] : real(0.0);
+        int k = 1;
+        for (; k + 1 <= n; k += 2) {
+            order_step(k, pa);
+            if (k + 3 <= n + 1) pa = uz[(k + 3) * zst];
+            order_step(k + 1, pb);
+            if (k + 4 <= n + 1) pb = uz[(k + 4) * zst];
+        }
+        if (k == n) order_step(k, pa);
     }
     sink.done(abs_sum, jac0, jac1, jac2);
 }
@@ -300,9 +415,10 @@ struct EfficiencySink {
         back = (k % 2 == 0) ? back + t : back - t;
         if (k > 1) {
             const double km = kf - 1.0;
-            g += km * (km + 2.0) / (km + 1.0) * (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
+            g += km * (km + 2.0) * recip(km + 1.0) *
+                 (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
         }
-        g += w / (kf * (kf + 1.0)) * re_mul_conj(a, b);
+        g += w * recip(kf * (kf + 1.0)) * re_mul_conj(a, b);
         a_prev = a;
         b_prev = b;
     }
@@ -311,9 +427,9 @@ struct EfficiencySink {
         const cx dak[3] = {da_dx, da_dm, times_i(da_dm)};
         const cx dbk[3] = {db_dx, db_dm, times_i(db_dm)};
         const double w = (double)(2 * k + 1), kf = (double)k;
-        const double own = w / (kf * (kf + 1.0));
+        const double own = w * recip(kf * (kf + 1.0));
         const double km = kf - 1.0;
-        const double pair = km * (km + 2.0) / (km + 1.0);
+        const double pair = km * (km + 2.0) * recip(km + 1.0);
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             d_sca[p] += w * 2.0 * (re_mul_conj(dak[p], a) + re_mul_conj(dbk[p], b));
@@ -349,6 +465,8 @@ struct EfficiencySink {
 
     // [Q_ext, Q_sca, Q_abs, Q_back, g] (efficiencies.rs:169-187).
     __device__ void values(double x, double q[5]) const {
+        // Q_ext is Q_sca + Q_abs exactly, as the reference forms it: no contraction here.
+#pragma clang fp contract(off)
         const double inv_x = 1.0 / x, inv_x2 = inv_x * inv_x, two_x2 = 2.0 * inv_x2;
         const double q_sca = two_x2 * sca, q_abs = two_x2 * abs_sum;
         q[0] = q_sca + q_abs;
@@ -360,6 +478,7 @@ struct EfficiencySink {
 
     // The Jacobian [d/dx, d/dRe m, d/dIm m][Q_ext..g] (efficiencies.rs:192-230).
     __device__ void jacobian(double x, const double q[5], bool outer, double out[15]) const {
+#pragma clang fp contract(off)
         const double inv_x = 1.0 / x, inv_x2 = inv_x * inv_x, two_x2 = 2.0 * inv_x2;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {

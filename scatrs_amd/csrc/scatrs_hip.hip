@@ -67,59 +67,45 @@ int check_hip(scatrs_ctx *ctx, hipError_t e, const char *what) {
     return fail(ctx, SCATRS_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Wavefront sum and max of per-lane values, then one atomic per wavefront.
-__device__ __forceinline__ void wave_accumulate(Status *st, int n, bool ok) {
-    unsigned long long sum = ok ? (unsigned long long)n : 0ull;
-    int mx = ok ? n : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        sum += __shfl_xor(sum, off, 64);
-        const int o = __shfl_xor(mx, off, 64);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&st->total_terms, sum);
-        atomicMax(&st->n_max, mx);
-    }
-}
-
-// Per sphere: the checks of Mie::compute (scatrs/src/mie/mod.rs:274-320), the number of terms,
-// and its N bucket for the largest-first order.
-__global__ void mie_prep(const double *x, const double *m, int64_t len, int64_t n_fixed,
-                         int allow_unvalidated, unsigned int *hist, int *keys, Status *st) {
+// Per sphere: the checks of Mie::compute (scatrs/src/mie/mod.rs:274-320), the number of terms
+// and its N bucket for the largest-first order. The first failing element is found with one
+// atomic per wavefront (a wavefront minimum first).
+__global__ void __launch_bounds__(256) mie_prep(const double *x, const double *m, int64_t len,
+                                                int64_t n_fixed, int allow_unvalidated,
+                                                int *keys, Partial *partial) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int n = 0;
     bool ok = false;
+    unsigned long long bad = ~0ull, outside = ~0ull;
     if (i < len) {
         const double xi = x[i], mr = m[2 * i], mi = m[2 * i + 1];
         long long nn = 0;
         const int code = sphere_code(xi, mr, mi, n_fixed, allow_unvalidated, &nn);
-        if (code == kInvalid) atomicMin(&st->first_bad, (unsigned long long)i);
-        if (code == kOutside) atomicMin(&st->first_outside, (unsigned long long)i);
+        if (code == kInvalid) bad = (unsigned long long)i;
+        if (code == kOutside) outside = (unsigned long long)i;
         ok = code == kValid;
         n = ok ? (int)nn : 0;
-        const int key = bucket_of(n);
-        keys[i] = key;
-        atomicAdd(&hist[key], 1u);
+        keys[i] = bucket_of(n);
     }
-    wave_accumulate(st, n, ok);
+    block_partial(n, ok, bad, outside, partial);
 }
 
 // The persistent solve: each wavefront takes 64 spheres at a time, largest N first, from a
-// shared counter; each lane solves one sphere with its ratio columns in the wavefront's
-// scratch slab.
+// shared counter; each lane solves one sphere. The downward u_k(mx) pass keeps one checkpoint
+// per 16 indices in the wavefront's global slab; the order loop regenerates each block of 16
+// into the wavefront's LDS tile. u_k(x) above floor(x) (a few dozen orders) stays in the slab.
 template <bool DERIVS>
-__global__ __launch_bounds__(256) void mie_batch(const double *__restrict__ x,
-                                                 const double *__restrict__ m, int64_t len,
-                                                 int64_t n_fixed, const int *__restrict__ perm,
-                                                 unsigned long long *chunk_counter, cx *uz_base,
-                                                 double *ux_base, int64_t rows,
-                                                 double *__restrict__ values,
-                                                 double *__restrict__ jac) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mie_batch(
+    const double *__restrict__ x, const double *__restrict__ m, int64_t len, int64_t n_fixed,
+    const int *__restrict__ perm, unsigned long long *chunk_counter, cx *ck_base,
+    double *ux_base, int64_t rows_ck, int64_t rows, double *__restrict__ values,
+    double *__restrict__ jac) {
+    __shared__ cx ztile[4][kZBlock * 64];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    cx *uz = uz_base + wave * rows * 64 + lane;
+    cx *ck = ck_base + wave * rows_ck * 64 + lane;
     double *ux = ux_base + wave * rows * 64 + lane;
+    cx *tile = &ztile[threadIdx.x >> 6][lane];
     while (true) {
         unsigned long long chunk = 0;
         if (lane == 0) chunk = atomicAdd(chunk_counter, 1ull);
@@ -131,11 +117,11 @@ __global__ __launch_bounds__(256) void mie_batch(const double *__restrict__ x,
             const double xi = x[i];
             const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
             const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
-            ratio_chain(mi, xi, n, uz, 64);
+            const ZCheckpoint zs = ratio_chain_ckpt(mi, xi, n, ck, 64, tile);
             x_chain(xi, n, upward_orders(xi, n), ux, 64);
             EfficiencySink<DERIVS> s;
             NoOuter none;
-            mie_orders<DERIVS>(xi, mi, n, uz, ux, 64, none, s);
+            mie_orders<DERIVS>(xi, mi, n, zs, ux, 64, none, s);
             double q[5];
             s.values(xi, q);
 #pragma unroll
@@ -248,6 +234,12 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kPerm, (size_t)len * sizeof(int)), "alloc")))
         return rc;
+    const int threads = 256;
+    const int64_t blocks = (len + threads - 1) / threads;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kNodes, (size_t)blocks * sizeof(Partial)),
+                        "alloc")))
+        return rc;
+    Partial *partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kNodes]);
     Status *st = static_cast<Status *>(ctx->buf[scatrs_ctx::kCounters]);
     unsigned long long *chunk_counter =
         reinterpret_cast<unsigned long long *>(static_cast<char *>(ctx->buf[scatrs_ctx::kCounters]) + sizeof(Status));
@@ -255,13 +247,13 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
     int *keys = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
     int *perm = static_cast<int *>(ctx->buf[scatrs_ctx::kPerm]);
     hipStream_t s = ctx->stream;
-    zero_status(ctx, st);
     fill_u64<<<1, 1, 0, s>>>(chunk_counter, 0ull, 1);
     fill_u32<<<(2 * kBuckets + 255) / 256, 256, 0, s>>>(hist, 0u, 2 * kBuckets);
-    const int threads = 256;
-    const int64_t blocks = (len + threads - 1) / threads;
-    mie_prep<<<(unsigned)blocks, threads, 0, s>>>(d_x, d_m, len, n_max, allow_unvalidated, hist,
-                                                    keys, st);
+    mie_prep<<<(unsigned)blocks, threads, 0, s>>>(d_x, d_m, len, n_max, allow_unvalidated, keys,
+                                                    partial);
+    reduce_status<<<1, 1024, 0, s>>>(partial, blocks, st);
+    histogram_keys<<<(unsigned)((len + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(keys, len,
+                                                                                   hist);
     Status *hst = static_cast<Status *>(ctx->host_pinned);
     if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
                         "status copy")))
@@ -280,25 +272,27 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
     const int64_t nmax = hst->n_max;
     if (total_terms) *total_terms = (int64_t)hst->total_terms;
     sort_by_keys(keys, hist, perm, len, s);
-    // Persistent wavefronts: 8 per CU, fewer if the scratch columns would pass 8 GiB.
+    // Persistent wavefronts: 8 per CU (two 4-wave blocks share the CU's LDS tiles), fewer if
+    // the slabs would pass 8 GiB.
     const int64_t rows = nmax + 2;
-    const size_t per_wave = (size_t)rows * 64 * (sizeof(cx) + sizeof(double));
+    const int64_t rows_ck = nmax / kZBlock + 2;
+    const size_t per_wave = (size_t)64 * (rows_ck * sizeof(cx) + rows * sizeof(double));
     int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (len + 63) / 64);
     waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((8ull << 30) / per_wave)));
     waves = (waves + 3) / 4 * 4;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves), "scratch")))
         return rc;
-    cx *uz = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
-    double *ux = reinterpret_cast<double *>(uz + (size_t)waves * rows * 64);
+    cx *ck = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
+    double *ux = reinterpret_cast<double *>(ck + (size_t)waves * rows_ck * 64);
     mark_begin(ctx);
     if (d_jacobian)
         mie_batch<true><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, perm,
-                                                             chunk_counter, uz, ux, rows,
-                                                             d_values, d_jacobian);
+                                                             chunk_counter, ck, ux, rows_ck,
+                                                             rows, d_values, d_jacobian);
     else
         mie_batch<false><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, perm,
-                                                              chunk_counter, uz, ux, rows,
-                                                              d_values, nullptr);
+                                                              chunk_counter, ck, ux, rows_ck,
+                                                              rows, d_values, nullptr);
     mark_end(ctx);
     return check_hip(ctx, hipGetLastError(), "mie_batch launch");
 }

@@ -44,11 +44,60 @@ __global__ void __launch_bounds__(1024) scan_buckets(const unsigned *hist, unsig
     }
 }
 
-__global__ void scatter_by_key(const int *keys, unsigned *offsets, int *perm, int64_t len) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= len) return;
-    const unsigned pos = atomicAdd(&offsets[keys[i]], 1u);
-    perm[pos] = (int)i;
+// Elements per thread of the histogram and scatter kernels (256 threads per block).
+constexpr int kSortPerThread = 4;
+constexpr int kSortBlock = 256 * kSortPerThread;
+
+// Block-local histogram in LDS, then one global atomic per non-empty bucket of the block.
+__device__ inline void block_histogram(const int *keys_local, int nkeys, unsigned *lcount,
+                                       unsigned *hist) {
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) lcount[b] = 0u;
+    __syncthreads();
+    for (int j = 0; j < nkeys; ++j) atomicAdd(&lcount[keys_local[j]], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x)
+        if (lcount[b]) atomicAdd(&hist[b], lcount[b]);
+}
+
+// Histogram of keys[0, len) (keys already written).
+__global__ void __launch_bounds__(256) histogram_keys(const int *keys, int64_t len,
+                                                      unsigned *hist) {
+    __shared__ unsigned lcount[kBuckets];
+    int local[kSortPerThread];
+    int nk = 0;
+    for (int j = 0; j < kSortPerThread; ++j) {
+        const int64_t i = (int64_t)blockIdx.x * kSortBlock + j * 256 + threadIdx.x;
+        if (i < len) local[nk++] = keys[i];
+    }
+    block_histogram(local, nk, lcount, hist);
+}
+
+// Each block ranks its elements per bucket in LDS, reserves one range per non-empty bucket
+// from the global offsets, and writes the permutation.
+__global__ void __launch_bounds__(256) scatter_by_key(const int *keys, unsigned *offsets,
+                                                      int *perm, int64_t len) {
+    __shared__ unsigned lcount[kBuckets];
+    __shared__ unsigned lbase[kBuckets];
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) lcount[b] = 0u;
+    __syncthreads();
+    int key[kSortPerThread];
+    unsigned rank[kSortPerThread];
+    for (int j = 0; j < kSortPerThread; ++j) {
+        const int64_t i = (int64_t)blockIdx.x * kSortBlock + j * 256 + threadIdx.x;
+        key[j] = -1;
+        if (i < len) {
+            key[j] = keys[i];
+            rank[j] = atomicAdd(&lcount[key[j]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x)
+        if (lcount[b]) lbase[b] = atomicAdd(&offsets[b], lcount[b]);
+    __syncthreads();
+    for (int j = 0; j < kSortPerThread; ++j) {
+        const int64_t i = (int64_t)blockIdx.x * kSortBlock + j * 256 + threadIdx.x;
+        if (key[j] >= 0) perm[lbase[key[j]] + rank[j]] = (int)i;
+    }
 }
 
 // hist has 2 * kBuckets entries: the counts, then room for the offsets.
@@ -56,7 +105,8 @@ inline void sort_by_keys(const int *keys, unsigned *hist, int *perm, int64_t len
                          hipStream_t s) {
     unsigned *offsets = hist + kBuckets;
     scan_buckets<<<1, 1024, 0, s>>>(hist, offsets);
-    scatter_by_key<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(keys, offsets, perm, len);
+    scatter_by_key<<<(unsigned)((len + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(keys, offsets,
+                                                                                  perm, len);
 }
 
 }  // namespace scatrs
