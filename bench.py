@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 METRIC = "Mie (particle·order) terms/s + ensembles/s, fused vs plain FP64 GPU at 1e-6 rel"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of mie_batch on this workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
+# two rocprofv3 --pmc runs, scaled by what the counters read for a 1 GiB device copy
+# (profiles/mie_batch_spheres_2p20.md).
+MIE_BATCH_TRAFFIC_BYTES = 1.207e9
 
 
 def log(msg: str) -> None:
@@ -117,7 +121,7 @@ def main() -> int:
     ap.add_argument("--workload", default="spheres", choices=["spheres"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--traffic", type=float, default=None,
+    ap.add_argument("--traffic", type=float, default=MIE_BATCH_TRAFFIC_BYTES,
                     help="HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first copy this many MiB device-to-device (a PMC byte-count reference)")
