@@ -128,7 +128,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's path runs on a single MI355X and no "
-              "figure at several GPUs is produced (see DESIGN.md for how it would shard).",
+              "figure at several GPUs is produced.",
               file=sys.stderr)
         return 2
     import torch
