@@ -51,6 +51,11 @@ const char *scatrs_last_error(const scatrs_ctx *ctx);
 int scatrs_set_profiling(scatrs_ctx *ctx, int on);
 int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap);
 
+/* Selects the angular-product kernel of scatrs_ensemble_fixed: 0 (default) the FP64
+ * matrix-core GEMM kernel, 1 the plain FP64 vector kernel (same results to rounding; kept as
+ * the baseline the matrix-core kernel is measured against). No reference counterpart. */
+int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
+
 /* The default truncation N = ceil(x + 6 x^(1/3) + 3) (host; scatrs/src/mie/mod.rs:122-124). */
 int64_t scatrs_default_n_max(double x);
 

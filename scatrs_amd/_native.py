@@ -33,6 +33,7 @@ SYMBOLS = (
     "scatrs_last_error",
     "scatrs_set_profiling",
     "scatrs_kernel_times",
+    "scatrs_set_angular_kernel",
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
     "scatrs_ensemble_fixed",
@@ -64,6 +65,7 @@ def load() -> C.CDLL:
         L.scatrs_last_error.restype = C.c_char_p
         L.scatrs_set_profiling.argtypes = [vp, i32]
         L.scatrs_kernel_times.argtypes = [vp, C.POINTER(dbl), i32]
+        L.scatrs_set_angular_kernel.argtypes = [vp, i32]
         L.scatrs_default_n_max.argtypes = [dbl]
         L.scatrs_default_n_max.restype = i64
         L.scatrs_mie_efficiencies.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
@@ -101,6 +103,10 @@ class Context:
         buf = (C.c_double * 1024)()
         n = self.lib.scatrs_kernel_times(self.handle, buf, 1024)
         return [buf[i] for i in range(n)]
+
+    def set_angular_kernel(self, kind: int) -> None:
+        """0: FP64 matrix cores (default); 1: the plain FP64 vector kernel."""
+        self.check(self.lib.scatrs_set_angular_kernel(self.handle, int(kind)))
 
     def bind_stream(self, stream_ptr: int) -> None:
         self.lib.scatrs_set_stream(self.handle, C.c_void_p(stream_ptr))

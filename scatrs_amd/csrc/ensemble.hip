@@ -364,6 +364,146 @@ __global__ __launch_bounds__(512) void ens_angular(int64_t e0, int64_t n_radius,
     }
 }
 
+// The angular products on FP64 matrix cores (v_mfma_f64_16x16x4f64), as GEMMs per group of
+// 16 radius nodes: C1 = G1 X and C2 = G2 X with G1, G2 (angles x orders) the w pi_n, w tau_n
+// table by parity and X (orders x 64) the nodes' u, v (a_n, b_n by parity) split into re, im:
+// column 4q + c of a 16-column tile is node q's (u.re, u.im, v.re, v.im)[c]. A workgroup is
+// one ensemble and 128 non-negative cosines (8 wavefronts x 16); the coefficients of the
+// group are staged in LDS 32 orders at a time. After a group's K loop each lane holds, for 4
+// cosines and its (node, component), C1 and C2; the quad of lanes of one node exchanges the
+// four components, forms S1(+-mu) = A +- B, S2(+-mu) = C +- D (A, C = G1 u, G1 v; B, D =
+// G2 v, G2 u), and the lane of component c keeps F11, F12, F33 or F34 (c = 0..3), times the
+// node's factor, summed over the groups in registers; the four node lanes are added at the
+// end, always in the same order.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGroupNodes = 16;
+constexpr int kMfmaOrders = 32;
+constexpr int kSliceAngles = 128;
+
+__global__ __launch_bounds__(512) void ens_angular_mfma(
+    int64_t e0, int64_t n_radius, int n_angles, int h, int slices, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double *tabc,
+    const double2 *table, double *tab_out) {
+    __shared__ cx lds_a[kGroupNodes][kMfmaOrders];
+    __shared__ cx lds_b[kGroupNodes][kMfmaOrders];
+    const int64_t eb = blockIdx.x / slices;
+    const int slice = blockIdx.x % slices;
+    const int64_t e = e0 + eb;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int row = lane & 15, kq = lane >> 4;  // A: angle row, order; B: column, order
+    const int comp = lane & 3, qin = (lane & 15) >> 2;
+    const int jj = slice * kSliceAngles + wave * 16 + row;  // this lane's A-operand cosine
+    const int64_t base = ens_base[e];
+    const int *nn = node_n + e * n_radius;
+    const int64_t *noff = node_off + e * n_radius;
+    const int64_t gbase = eb * n_radius;
+    double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][v]: this lane's element, rows kq+4v
+    for (int64_t g0 = 0; g0 < n_radius; g0 += kGroupNodes) {
+        int nmax = 0;
+        for (int q = 0; q < kGroupNodes; ++q) {
+            const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
+            nmax = nq > nmax ? nq : nmax;
+        }
+        mfma_d4 acc1[4], acc2[4];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            acc1[tt] = mfma_d4{0, 0, 0, 0};
+            acc2[tt] = mfma_d4{0, 0, 0, 0};
+        }
+        for (int kc = 0; kc < nmax; kc += kMfmaOrders) {
+            __syncthreads();
+            for (int idx = t; idx < kGroupNodes * kMfmaOrders; idx += blockDim.x) {
+                const int q = idx / kMfmaOrders, k = idx % kMfmaOrders;
+                const int n = kc + k;
+                const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
+                cx va = real(0.0), vb = real(0.0);
+                if (n < nq) {
+                    const cx *c = staging + 2 * (base + noff[g0 + q]);
+                    va = c[n];
+                    vb = c[nq + n];
+                }
+                lds_a[q][k] = va;
+                lds_b[q][k] = vb;
+            }
+            __syncthreads();
+            const int kend = nmax - kc < kMfmaOrders ? nmax - kc : kMfmaOrders;
+            for (int k4 = 0; k4 < kend; k4 += 4) {
+                const int kk = k4 + kq;              // order index in the chunk
+                const int n = kc + kk + 1;           // the order
+                double2 pt = make_double2(0.0, 0.0);
+                if (jj < h && n <= nmax) pt = table[(size_t)(n - 1) * h + jj];
+                const bool odd = n & 1;
+                const double g1 = odd ? pt.x : pt.y;
+                const double g2 = odd ? pt.y : pt.x;
+#pragma unroll
+                for (int tt = 0; tt < 4; ++tt) {
+                    const int q = 4 * tt + qin;
+                    const cx a = lds_a[q][kk], b = lds_b[q][kk];
+                    const cx u = odd ? a : b, v = odd ? b : a;
+                    const double val = comp == 0 ? u.re : comp == 1 ? u.im : comp == 2 ? v.re : v.im;
+                    acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, val, acc1[tt], 0, 0, 0);
+                    acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g2, val, acc2[tt], 0, 0, 0);
+                }
+            }
+        }
+        // Epilogue of the group: the lane's column is node 4 tt + qin, component comp.
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            const int64_t node = g0 + 4 * tt + qin;
+            const double cfac = node < n_radius ? tabc[gbase + node] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double c1[4], c2[4];
+                const double x1 = acc1[tt][v], x2 = acc2[tt][v];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
+                    c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
+                }
+                const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
+                const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
+#pragma unroll
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    const cx s1 = sgn == 0 ? A + B : A - B;
+                    const cx s2 = sgn == 0 ? C + D : C - D;
+                    double f;
+                    if (comp == 0) f = 0.5 * (norm2(s1) + norm2(s2));
+                    else if (comp == 1) f = 0.5 * (norm2(s2) - norm2(s1));
+                    else {
+                        const cx pp = s2 * conj(s1);
+                        f = comp == 2 ? pp.re : pp.im;
+                    }
+                    tp[sgn][v] += cfac * f;
+                }
+            }
+        }
+    }
+    // Add the four node lanes (qin = 0..3) of each element, in a fixed order.
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double x = tp[sgn][v];
+            x += __shfl_xor(x, 4, 64);
+            x += __shfl_xor(x, 8, 64);
+            tp[sgn][v] = x;
+        }
+    if (qin == 0) {
+        double *out = tab_out + eb * 4 * (int64_t)n_angles;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16 (i % 4), element i / 4.
+            const int jr = slice * kSliceAngles + wave * 16 + kq + 4 * v;  // output cosine
+            if (jr >= h) continue;
+            const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
+            out[comp * n_angles + jp] = tp[0][v];
+            if (jm != jp) out[comp * n_angles + jm] = tp[1][v];
+        }
+    }
+}
+
 // Averages and outputs of one ensemble (ensemble/engine.rs:1782-1836,2396-2497): scalars in
 // SCALARS order, Greek coefficients projected with the Wigner d table
 // dtab[(p * n_angles + j) * (order + 1) + l] (p = d00, d22, d2,-2, d02), the phase matrix.
@@ -700,9 +840,17 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
             case 7: launch_nodes<7>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
             default: launch_nodes<8>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
         }
-        const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
-        ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, node_n, node_off,
-                                                ens_base, staging, tabc, table, tab_out);
+        if (ctx->angular_kernel == 1) {
+            const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
+            ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, node_n,
+                                                    node_off, ens_base, staging, tabc, table,
+                                                    tab_out);
+        } else {
+            const int slices = (h + kSliceAngles - 1) / kSliceAngles;
+            ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
+                e0, n_radius, (int)n_angles, h, slices, node_n, node_off, ens_base, staging,
+                tabc, table, tab_out);
+        }
         mark_end(ctx);
         const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
         ens_finish<<<(unsigned)ne, 256, shmem, s>>>(e0, n_radius, (int)n_angles, d_atab, vals,

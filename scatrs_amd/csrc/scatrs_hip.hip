@@ -197,6 +197,14 @@ int scatrs_set_profiling(scatrs_ctx *ctx, int on) {
     return SCATRS_OK;
 }
 
+int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    if (kind != 0 && kind != 1)
+        return fail(ctx, SCATRS_INVALID_INPUT, "the angular kernel must be 0 (MFMA) or 1 (vector)");
+    ctx->angular_kernel = kind;
+    return SCATRS_OK;
+}
+
 int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap) {
     if (!ctx) return 0;
     int n = 0;
