@@ -101,6 +101,28 @@ int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
                           int allow_unvalidated, double *d_scalars, double *d_greek,
                           double *d_phase, int64_t *total_terms);
 
+/* Reverse mode of scatrs_ensemble_fixed (spheres or layered spheres): given the cotangents of
+ * its outputs (d_bar_scalars (count, 11), d_bar_greek (count, 6, order+1), d_bar_phase
+ * (count, 6, n_angles); each may be NULL for zero), the vector-Jacobian product with respect to
+ * each ensemble's parameters, d_grad (count, P) with P = 2 + 2 L + (L - 1):
+ *   [p0, p1, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 .. fraction_{L-2}]
+ * (layer 0 the core; the fractions act at a fixed outer radius, the reference's
+ * layer{l}.radius_fraction direction, scatrs/src/ensemble/particle.rs:221-249).
+ * fixed_range = 0: the derivative of this discrete path, whose radius nodes and range move with
+ * (p0, p1); fixed_range = 1: the ln r range held at its value, so (p0, p1) act through the
+ * density alone, the reference's definition (docs/theory/ensembles.md:150,339-340,
+ * ensemble/engine.rs:1793-1804). a_n, b_n and their derivatives are recomputed, not stored.
+ * Replaces the reference's forward-mode derivatives (EnsembleOptions::derivatives,
+ * scatrs/src/ensemble/mod.rs:356-357, contracted by the caller) with one reverse pass. */
+int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                              const double *d_p0, const double *d_p1,
+                              const double *d_wavelength, int layers,
+                              const double *d_fractions, const double *d_m, int64_t n_radius,
+                              int64_t n_angles, int64_t order, int normalized, int fixed_range,
+                              int allow_unvalidated, const double *d_bar_scalars,
+                              const double *d_bar_greek, const double *d_bar_phase,
+                              double *d_grad, int64_t *total_terms);
+
 #ifdef __cplusplus
 }
 #endif

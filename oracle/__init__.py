@@ -59,6 +59,15 @@ def lib() -> C.CDLL:
         L.orc_ensemble_fixed.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                          C.c_void_p, _cp, C.c_int64, C.c_int64, C.c_int64,
                                          C.c_int, _dp, C.c_void_p, C.c_void_p]
+        L.orc_ensemble_fixed_ex.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_int, C.c_void_p, _cp, C.c_int64, C.c_int64,
+                                            C.c_int64, C.c_int, C.c_void_p, C.c_void_p, _dp,
+                                            C.c_void_p, C.c_void_p]
+        L.orc_ensemble_fixed_jacobian.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                                  C.c_int, C.c_void_p, _cp, C.c_int64,
+                                                  C.c_int64, C.c_int64, C.c_int, C.c_int, _dp,
+                                                  C.c_void_p, C.c_void_p, _dp, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p]
         L.orc_ensemble_batch.argtypes = [C.c_int64, _ip, _dp, _dp, _dp, _cp, C.c_int64,
                                          C.c_int64, C.c_int64, C.c_int, _dp, C.c_void_p,
                                          C.c_void_p, C.c_int]
@@ -191,22 +200,55 @@ LOGNORMAL, GAMMA = 0, 1
 
 def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fractions=None,
                    n_radius: int = 128, n_angles: int = 128, order: int = 32,
-                   normalized: bool = True) -> dict:
-    """One ensemble on the fixed discrete path: scalars (11,), greek (6, L+1), phase (6, n_q)."""
+                   normalized: bool = True, t_range=None, n_nodes=None) -> dict:
+    """One ensemble on the fixed discrete path: scalars (11,), greek (6, L+1), phase (6, n_q).
+    t_range = (t_lo, t_hi) replaces the distribution's ln r range and n_nodes the per-node N
+    (for finite differences at a fixed range or fixed truncations)."""
     m = np.atleast_1d(np.asarray(m, np.complex128)).copy()
     layers = m.size
     fr = None if fractions is None else np.ascontiguousarray(fractions, np.float64)
     scalars = np.zeros(11)
     greek = np.zeros(6 * (order + 1)) if order >= 0 else None
     phase = np.zeros(6 * n_angles)
-    rc = lib().orc_ensemble_fixed(kind, p0, p1, wavelength, layers, _ptr(fr), m, n_radius,
-                                  n_angles, order, int(normalized), scalars, _ptr(greek),
-                                  _ptr(phase))
+    rng = None if t_range is None else np.ascontiguousarray(t_range, np.float64)
+    nn = None if n_nodes is None else np.ascontiguousarray(n_nodes, np.int64)
+    rc = lib().orc_ensemble_fixed_ex(kind, p0, p1, wavelength, layers, _ptr(fr), m, n_radius,
+                                     n_angles, order, int(normalized), _ptr(rng), _ptr(nn),
+                                     scalars, _ptr(greek), _ptr(phase))
     if rc != 0:
         raise ValueError(f"invalid ensemble input ({rc})")
     return {"scalars": scalars,
             "greek": None if greek is None else greek.reshape(6, order + 1),
             "phase": phase.reshape(6, n_angles)}
+
+
+def ensemble_fixed_jacobian(kind: int, p0: float, p1: float, wavelength: float, m,
+                            fractions=None, n_radius: int = 128, n_angles: int = 128,
+                            order: int = 32, normalized: bool = True,
+                            fixed_range: bool = False) -> dict:
+    """Values and forward-mode derivatives of one ensemble along [p0, p1, Re m_0, Im m_0, ...,
+    fraction_0, ...]: d_scalars (P, 11), d_greek (P, 6, L+1), d_phase (P, 6, n_q), n_nodes."""
+    m = np.atleast_1d(np.asarray(m, np.complex128)).copy()
+    layers = m.size
+    P = 2 + 2 * layers + (layers - 1)
+    fr = None if fractions is None else np.ascontiguousarray(fractions, np.float64)
+    scalars = np.zeros(11)
+    greek = np.zeros(6 * (order + 1))
+    phase = np.zeros(6 * n_angles)
+    ds = np.zeros(P * 11)
+    dg = np.zeros(P * 6 * (order + 1))
+    dp = np.zeros(P * 6 * n_angles)
+    nn = np.zeros(n_radius, np.int64)
+    rc = lib().orc_ensemble_fixed_jacobian(kind, p0, p1, wavelength, layers, _ptr(fr), m,
+                                           n_radius, n_angles, order, int(normalized),
+                                           int(fixed_range), scalars, _ptr(greek), _ptr(phase),
+                                           ds, _ptr(dg), _ptr(dp), _ptr(nn))
+    if rc != 0:
+        raise ValueError(f"invalid ensemble input ({rc})")
+    return {"scalars": scalars, "greek": greek.reshape(6, order + 1),
+            "phase": phase.reshape(6, n_angles), "d_scalars": ds.reshape(P, 11),
+            "d_greek": dg.reshape(P, 6, order + 1), "d_phase": dp.reshape(P, 6, n_angles),
+            "n_nodes": nn}
 
 
 def ensemble_batch(kind, p0, p1, wavelength, m, n_radius: int, n_angles: int, order: int,

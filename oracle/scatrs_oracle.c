@@ -1123,60 +1123,12 @@ int orc_radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi)
 }
 
 /* ---- the fixed-node ensemble ---- */
-int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers,
-                       const double *fractions, const orc_cx *m, int64_t n_radius,
-                       int64_t n_angles, int64_t order, int normalized, double *scalars,
-                       double *greek, double *phase) {
-    double t_lo, t_hi;
-    if (orc_radius_range(kind, p0, p1, &t_lo, &t_hi) != 0) return -1;
-    if (!(lambda > 0.0 && isfinite(lambda)) || layers < 1 || n_radius < 1 || n_angles < 1) return -2;
-    double *rn = (double *)malloc(sizeof(double) * (size_t)n_radius);
-    double *rw = (double *)malloc(sizeof(double) * (size_t)n_radius);
-    orc_gauss_legendre(n_radius, rn, rw);
-    double *mu = (double *)malloc(sizeof(double) * (size_t)n_angles);
-    double *mw = (double *)malloc(sizeof(double) * (size_t)n_angles);
-    orc_gauss_legendre(n_angles, mu, mw);
-    double half = 0.5 * (t_hi - t_lo), mid = 0.5 * (t_hi + t_lo);
-    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double *T = (double *)calloc(4 * (size_t)n_angles, sizeof(double));
-    double *xl = (double *)malloc(sizeof(double) * (size_t)layers);
-    cx *s1 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
-    cx *s2 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
-    for (int64_t i = 0; i < n_radius; i++) {
-        double t = mid + half * rn[i];
-        double r = exp(t);
-        double u = orc_density(kind, p0, p1, r, NULL);
-        double W = rw[i] * half * u * r;
-        double x = 2.0 * PI * r / lambda;
-        int64_t n = orc_default_n_max(x);
-        cx *ab = (cx *)malloc(sizeof(cx) * 2 * (size_t)n);
-        double abs_sum;
-        if (layers == 1) {
-            double aj[3];
-            orc_mie_solve(x, m[0], n, 0, ab, ab + n, NULL, NULL, NULL, NULL, &abs_sum, aj);
-        } else {
-            for (int l = 0; l < layers - 1; l++) xl[l] = fractions[l] * x;
-            xl[layers - 1] = x;
-            orc_layered_solve(layers, xl, m, n, 0, ab, ab + n, NULL, &abs_sum, NULL);
-        }
-        double q[5];
-        orc_efficiencies(x, n, ab, ab + n, abs_sum, q);
-        double area = PI * r * r;
-        double g[8] = {1.0, area * q[0], area * q[1], area * q[1] * q[4], area * q[2],
-                       area * q[3], r * r, r * r * r};
-        for (int k = 0; k < 8; k++) S[k] += W * g[k];
-        orc_amplitudes(n, ab, ab + n, mu, n_angles, s1, s2);
-        double c = area * (4.0 / (x * x));
-        for (int64_t j = 0; j < n_angles; j++) {
-            double n1 = norm2(s1[j]), n2 = norm2(s2[j]);
-            cx pp = mul(s2[j], C(s1[j].re, -s1[j].im));
-            T[j] += W * (c * (0.5 * (n1 + n2)));
-            T[n_angles + j] += W * (c * (0.5 * (n2 - n1)));
-            T[2 * n_angles + j] += W * (c * pp.re);
-            T[3 * n_angles + j] += W * (c * pp.im);
-        }
-        free(ab);
-    }
+
+/* The outputs of an ensemble from its sums S (8) and summed tables T (4 x n_angles, divided by
+ * Z on return) (ensemble/engine.rs:1782-1836,2396-2497). */
+static void ensemble_outputs(const double S[8], double *T, int64_t n_angles, const double *mu,
+                             const double *mw, int64_t order, int normalized, double *scalars,
+                             double *greek, double *phase) {
     double z = S[0];
     double ext = S[1] / z, sca = S[2] / z, scag = S[3] / z, absb = S[4] / z, back = S[5] / z;
     double r2 = S[6] / z, r3 = S[7] / z;
@@ -1209,7 +1161,341 @@ int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers
                 phase[e * n_angles + j] = normalized ? v / sca : v;
             }
     }
+}
+
+int orc_ensemble_fixed_ex(int kind, double p0, double p1, double lambda, int layers,
+                          const double *fractions, const orc_cx *m, int64_t n_radius,
+                          int64_t n_angles, int64_t order, int normalized, const double *range,
+                          const int64_t *n_nodes, double *scalars, double *greek,
+                          double *phase) {
+    double t_lo, t_hi;
+    if (orc_radius_range(kind, p0, p1, &t_lo, &t_hi) != 0) return -1;
+    if (range) {
+        t_lo = range[0];
+        t_hi = range[1];
+    }
+    if (!(lambda > 0.0 && isfinite(lambda)) || layers < 1 || n_radius < 1 || n_angles < 1) return -2;
+    double *rn = (double *)malloc(sizeof(double) * (size_t)n_radius);
+    double *rw = (double *)malloc(sizeof(double) * (size_t)n_radius);
+    orc_gauss_legendre(n_radius, rn, rw);
+    double *mu = (double *)malloc(sizeof(double) * (size_t)n_angles);
+    double *mw = (double *)malloc(sizeof(double) * (size_t)n_angles);
+    orc_gauss_legendre(n_angles, mu, mw);
+    double half = 0.5 * (t_hi - t_lo), mid = 0.5 * (t_hi + t_lo);
+    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double *T = (double *)calloc(4 * (size_t)n_angles, sizeof(double));
+    double *xl = (double *)malloc(sizeof(double) * (size_t)layers);
+    cx *s1 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    cx *s2 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    for (int64_t i = 0; i < n_radius; i++) {
+        double t = mid + half * rn[i];
+        double r = exp(t);
+        double u = orc_density(kind, p0, p1, r, NULL);
+        double W = rw[i] * half * u * r;
+        double x = 2.0 * PI * r / lambda;
+        int64_t n = n_nodes ? n_nodes[i] : orc_default_n_max(x);
+        cx *ab = (cx *)malloc(sizeof(cx) * 2 * (size_t)n);
+        double abs_sum;
+        if (layers == 1) {
+            double aj[3];
+            orc_mie_solve(x, m[0], n, 0, ab, ab + n, NULL, NULL, NULL, NULL, &abs_sum, aj);
+        } else {
+            for (int l = 0; l < layers - 1; l++) xl[l] = fractions[l] * x;
+            xl[layers - 1] = x;
+            orc_layered_solve(layers, xl, m, n, 0, ab, ab + n, NULL, &abs_sum, NULL);
+        }
+        double q[5];
+        orc_efficiencies(x, n, ab, ab + n, abs_sum, q);
+        double area = PI * r * r;
+        double g[8] = {1.0, area * q[0], area * q[1], area * q[1] * q[4], area * q[2],
+                       area * q[3], r * r, r * r * r};
+        for (int k = 0; k < 8; k++) S[k] += W * g[k];
+        orc_amplitudes(n, ab, ab + n, mu, n_angles, s1, s2);
+        double c = area * (4.0 / (x * x));
+        for (int64_t j = 0; j < n_angles; j++) {
+            double n1 = norm2(s1[j]), n2 = norm2(s2[j]);
+            cx pp = mul(s2[j], C(s1[j].re, -s1[j].im));
+            T[j] += W * (c * (0.5 * (n1 + n2)));
+            T[n_angles + j] += W * (c * (0.5 * (n2 - n1)));
+            T[2 * n_angles + j] += W * (c * pp.re);
+            T[3 * n_angles + j] += W * (c * pp.im);
+        }
+        free(ab);
+    }
+    ensemble_outputs(S, T, n_angles, mu, mw, order, normalized, scalars, greek, phase);
     free(rn); free(rw); free(mu); free(mw); free(T); free(xl); free(s1); free(s2);
+    return 0;
+}
+
+int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers,
+                       const double *fractions, const orc_cx *m, int64_t n_radius,
+                       int64_t n_angles, int64_t order, int normalized, double *scalars,
+                       double *greek, double *phase) {
+    return orc_ensemble_fixed_ex(kind, p0, p1, lambda, layers, fractions, m, n_radius, n_angles,
+                                 order, normalized, NULL, NULL, scalars, greek, phase);
+}
+
+/* d(ln u)/d(ln r) at fixed parameters, from the densities of distribution.rs:366-430. */
+static double log_density_slope(int kind, double p0, double p1, double r) {
+    if (kind == ORC_LOGNORMAL) {
+        double s = log(p1);
+        return -(log(r) - log(p0)) / (s * s) - 1.0;
+    }
+    return (1.0 - 3.0 * p1) / p1 - r / (p0 * p1);
+}
+
+/* Derivatives of the range's midpoint and half-width in ln r with respect to (p0, p1). */
+static double range_derivatives(int kind, double p0, double p1, double dmid[2], double dhalf[2]) {
+    if (kind == ORC_LOGNORMAL) {
+        dmid[0] = 1.0 / p0;
+        dmid[1] = 0.0;
+        dhalf[0] = 0.0;
+        dhalf[1] = 5.0 / p1;
+        return 5.0 * log(p1);
+    }
+    double q = 1.0 - 2.0 * p1, s = sqrt(p1 / q);
+    double ds = 0.5 / (s * q * q);
+    dmid[0] = 1.0 / p0;
+    dmid[1] = -1.5 * ds;
+    dhalf[0] = 0.0;
+    dhalf[1] = 6.5 * ds;
+    return 6.5 * s;
+}
+
+/* The derivatives of the phase elements from those of S1, S2 (mie/phase.rs:60-73). */
+static void element_derivatives(cx s1, cx s2, cx d1, cx d2, double out[4]) {
+    double r1 = re_mul_conj(s1, d1), r2 = re_mul_conj(s2, d2);
+    cx p = add(mul(d2, C(s1.re, -s1.im)), mul(s2, C(d1.re, -d1.im)));
+    out[0] = r1 + r2;
+    out[1] = r2 - r1;
+    out[2] = p.re;
+    out[3] = p.im;
+}
+
+/* Forward-mode derivatives of the fixed path, direction by direction, as the reference forms
+ * its ensemble derivatives: per node the efficiency Jacobians of every layer
+ * (efficiencies.rs:192-230, layered/kernel.rs:549-587), the radius derivative at fixed
+ * fractions 2 pi r Q + pi r^2 (2 pi / lambda) dQ/dx (engine.rs:1111-1123, particle.rs:221-249),
+ * the phase-element derivatives from the amplitude derivatives (phase.rs:60-73); the weights'
+ * derivatives; then the quotient rule through the averages, the Greek normalization and the
+ * phase normalization (engine.rs:1782-1836,2396-2497; lin.rs:7-68). Directions
+ * P = 2 + 2L + (L - 1): [p0, p1, Re m_0, Im m_0, ..., fraction_0 ...]. With fixed_range the ln r
+ * range is held fixed (the parameters act through the density alone, the reference's
+ * definition); otherwise the nodes move with (p0, p1). n_nodes (may be NULL) receives N per
+ * node. Returns 0 or a negative code. */
+int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, int layers,
+                                const double *fractions, const orc_cx *m, int64_t n_radius,
+                                int64_t n_angles, int64_t order, int normalized,
+                                int fixed_range, double *scalars, double *greek, double *phase,
+                                double *d_scalars, double *d_greek, double *d_phase,
+                                int64_t *n_nodes) {
+    double t_lo, t_hi;
+    if (orc_radius_range(kind, p0, p1, &t_lo, &t_hi) != 0) return -1;
+    if (!(lambda > 0.0 && isfinite(lambda)) || layers < 1 || n_radius < 1 || n_angles < 1) return -2;
+    const int L = layers;
+    const int P = 2 + 2 * L + (L - 1);
+    double *rn = (double *)malloc(sizeof(double) * (size_t)n_radius);
+    double *rw = (double *)malloc(sizeof(double) * (size_t)n_radius);
+    orc_gauss_legendre(n_radius, rn, rw);
+    double *mu = (double *)malloc(sizeof(double) * (size_t)n_angles);
+    double *mw = (double *)malloc(sizeof(double) * (size_t)n_angles);
+    orc_gauss_legendre(n_angles, mu, mw);
+    double half = 0.5 * (t_hi - t_lo), mid = 0.5 * (t_hi + t_lo);
+    double dmid[2], dhalf[2];
+    range_derivatives(kind, p0, p1, dmid, dhalf);
+    double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double *dS = (double *)calloc(8 * (size_t)P, sizeof(double));
+    double *T = (double *)calloc(4 * (size_t)n_angles, sizeof(double));
+    double *dT = (double *)calloc(4 * (size_t)n_angles * (size_t)P, sizeof(double));
+    double *xl = (double *)malloc(sizeof(double) * (size_t)L);
+    cx *s1 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    cx *s2 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    cx *ds1 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    cx *ds2 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    double *jac = (double *)malloc(sizeof(double) * 15 * (size_t)L);
+    double *abs_jac = (double *)malloc(sizeof(double) * 3 * (size_t)L);
+    for (int64_t i = 0; i < n_radius; i++) {
+        double t = mid + half * rn[i];
+        double r = exp(t);
+        double ugrad[2];
+        double u = orc_density(kind, p0, p1, r, ugrad);
+        double W = rw[i] * half * u * r;
+        double x = 2.0 * PI * r / lambda;
+        int64_t n = orc_default_n_max(x);
+        if (n_nodes) n_nodes[i] = n;
+        cx *ab = (cx *)malloc(sizeof(cx) * 2 * (size_t)n);
+        cx *a = ab, *b = ab + n;
+        cx *dg = (cx *)malloc(sizeof(cx) * 4 * (size_t)n * (size_t)L); /* [l][da_dx, da_dm, db_dx, db_dm][k] */
+        cx *da = (cx *)malloc(sizeof(cx) * 2 * (size_t)n), *db = da + n;
+        double abs_sum;
+        if (L == 1) {
+            orc_mie_solve(x, m[0], n, 1, a, b, dg, dg + n, dg + 2 * n, dg + 3 * n, &abs_sum, abs_jac);
+        } else {
+            for (int l = 0; l < L - 1; l++) xl[l] = fractions[l] * x;
+            xl[L - 1] = x;
+            orc_layered_solve(L, xl, m, n, 1, a, b, dg, &abs_sum, abs_jac);
+        }
+        double q[5];
+        orc_efficiencies(x, n, a, b, abs_sum, q);
+        for (int l = 0; l < L; l++) {
+            const cx *g = dg + (size_t)l * 4 * (size_t)n;
+            orc_efficiencies_jacobian(x, n, a, b, g, g + n, g + 2 * n, g + 3 * n, abs_sum,
+                                      abs_jac + 3 * l, l == L - 1, jac + 15 * l);
+        }
+        double area = PI * r * r;
+        double h[8] = {1.0, area * q[0], area * q[1], area * q[1] * q[4], area * q[2],
+                       area * q[3], r * r, r * r * r};
+        for (int k = 0; k < 8; k++) S[k] += W * h[k];
+        orc_amplitudes(n, a, b, mu, n_angles, s1, s2);
+        double c = area * (4.0 / (x * x));
+        for (int64_t j = 0; j < n_angles; j++) {
+            double n1 = norm2(s1[j]), n2 = norm2(s2[j]);
+            cx pp = mul(s2[j], C(s1[j].re, -s1[j].im));
+            T[j] += W * (c * (0.5 * (n1 + n2)));
+            T[n_angles + j] += W * (c * (0.5 * (n2 - n1)));
+            T[2 * n_angles + j] += W * (c * pp.re);
+            T[3 * n_angles + j] += W * (c * pp.im);
+        }
+        /* Directions of the particle: d = 0 the size at fixed fractions (x d/dx), then
+         * Re m_l, Im m_l per layer, then the fractions (x d/dx_l). */
+        const int ND = 1 + 2 * L + (L - 1);
+        for (int d = 0; d < ND; d++) {
+            double dq[5] = {0, 0, 0, 0, 0};
+            for (int64_t k = 0; k < n; k++) {
+                da[k] = R(0.0);
+                db[k] = R(0.0);
+            }
+            if (d == 0) {
+                for (int l = 0; l < L; l++) {
+                    double f = (l == L - 1) ? x : fractions[l] * x;
+                    const cx *g = dg + (size_t)l * 4 * (size_t)n;
+                    for (int e = 0; e < 5; e++) dq[e] += f * jac[15 * l + e];
+                    for (int64_t k = 0; k < n; k++) {
+                        da[k] = add(da[k], scl(g[k], f));
+                        db[k] = add(db[k], scl(g[2 * n + k], f));
+                    }
+                }
+            } else if (d <= 2 * L) {
+                int l = (d - 1) / 2, im = (d - 1) % 2;
+                const cx *g = dg + (size_t)l * 4 * (size_t)n;
+                for (int e = 0; e < 5; e++) dq[e] = jac[15 * l + 5 * (1 + im) + e];
+                for (int64_t k = 0; k < n; k++) {
+                    da[k] = im ? times_i(g[n + k]) : g[n + k];
+                    db[k] = im ? times_i(g[3 * n + k]) : g[3 * n + k];
+                }
+            } else {
+                int l = d - 1 - 2 * L;
+                const cx *g = dg + (size_t)l * 4 * (size_t)n;
+                for (int e = 0; e < 5; e++) dq[e] = x * jac[15 * l + e];
+                for (int64_t k = 0; k < n; k++) {
+                    da[k] = scl(g[k], x);
+                    db[k] = scl(g[2 * n + k], x);
+                }
+            }
+            /* the node's per-particle quantities along this direction */
+            double dh[8] = {0.0, area * dq[0], area * dq[1], area * (dq[1] * q[4] + q[1] * dq[4]),
+                            area * dq[2], area * dq[3], 0.0, 0.0};
+            orc_amplitudes(n, da, db, mu, n_angles, ds1, ds2);
+            if (d == 0) {
+                /* d/d(ln r) at fixed fractions: the explicit r^2 of pi r^2 and r^2, r^3 */
+                for (int k = 1; k <= 5; k++) dh[k] += 2.0 * h[k];
+                dh[6] = 2.0 * h[6];
+                dh[7] = 3.0 * h[7];
+                for (int th = 0; th < 2; th++) {
+                    double dlnr = fixed_range ? 0.0 : dmid[th] + rn[i] * dhalf[th];
+                    double dlnw = (fixed_range ? 0.0 : dhalf[th] / half) + ugrad[th] / u +
+                                  (log_density_slope(kind, p0, p1, r) + 1.0) * dlnr;
+                    double *dSp = dS + 8 * th;
+                    for (int k = 0; k < 8; k++) dSp[k] += W * dlnw * h[k] + W * dh[k] * dlnr;
+                    double *dTp = dT + (size_t)th * 4 * (size_t)n_angles;
+                    for (int64_t j = 0; j < n_angles; j++) {
+                        double f[4], df[4];
+                        double n1 = norm2(s1[j]), n2 = norm2(s2[j]);
+                        cx pp = mul(s2[j], C(s1[j].re, -s1[j].im));
+                        f[0] = 0.5 * (n1 + n2);
+                        f[1] = 0.5 * (n2 - n1);
+                        f[2] = pp.re;
+                        f[3] = pp.im;
+                        element_derivatives(s1[j], s2[j], ds1[j], ds2[j], df);
+                        for (int e = 0; e < 4; e++)
+                            dTp[e * n_angles + j] += W * c * (dlnw * f[e] + df[e] * dlnr);
+                    }
+                }
+            } else {
+                int p = d + 1;
+                double *dSp = dS + 8 * p;
+                for (int k = 0; k < 8; k++) dSp[k] += W * dh[k];
+                double *dTp = dT + (size_t)p * 4 * (size_t)n_angles;
+                for (int64_t j = 0; j < n_angles; j++) {
+                    double df[4];
+                    element_derivatives(s1[j], s2[j], ds1[j], ds2[j], df);
+                    for (int e = 0; e < 4; e++) dTp[e * n_angles + j] += W * c * df[e];
+                }
+            }
+        }
+        free(ab);
+        free(dg);
+        free(da);
+    }
+    /* values */
+    double *Tv = (double *)malloc(sizeof(double) * 4 * (size_t)n_angles);
+    memcpy(Tv, T, sizeof(double) * 4 * (size_t)n_angles);
+    ensemble_outputs(S, Tv, n_angles, mu, mw, order, normalized, scalars, greek, phase);
+    /* the quotient rule per direction (Tv now holds F = T / Z) */
+    double z = S[0];
+    double qv[8];
+    for (int k = 0; k < 8; k++) qv[k] = S[k] / z;
+    const double ext = qv[1], sca = qv[2], scag = qv[3], back = qv[5], r2 = qv[6], r3 = qv[7];
+    const int64_t len = order + 1;
+    double *dF = (double *)malloc(sizeof(double) * 4 * (size_t)n_angles);
+    double *dgam = (greek && d_greek && order >= 0) ? (double *)malloc(sizeof(double) * 6 * (size_t)len) : NULL;
+    double *gam = dgam ? (double *)malloc(sizeof(double) * 6 * (size_t)len) : NULL;
+    if (gam) orc_greek_project(n_angles, mu, mw, Tv, Tv + n_angles, Tv + 2 * n_angles,
+                               Tv + 3 * n_angles, order, gam);
+    for (int p = 0; p < P; p++) {
+        const double *dSp = dS + 8 * p;
+        double dz = dSp[0];
+        double dq[8];
+        for (int k = 1; k < 8; k++) dq[k] = (dSp[k] - qv[k] * dz) / z;
+        double *ds = d_scalars + 11 * p;
+        ds[0] = dq[1];
+        ds[1] = dq[2];
+        ds[2] = dq[4];
+        ds[3] = dq[5];
+        ds[4] = (dq[2] - (sca / ext) * dq[1]) / ext;
+        ds[5] = (dq[3] - (scag / sca) * dq[2]) / sca;
+        ds[6] = 4.0 * PI * (dq[1] * back - ext * dq[5]) / (back * back);
+        ds[7] = 0.0;
+        ds[8] = (dq[7] - (r3 / r2) * dq[6]) / r2;
+        ds[9] = PI * dq[6];
+        ds[10] = 4.0 * PI / 3.0 * dq[7];
+        const double *dTp = dT + (size_t)p * 4 * (size_t)n_angles;
+        for (int64_t j = 0; j < 4 * n_angles; j++) dF[j] = (dTp[j] - Tv[j] * dz) / z;
+        if (dgam) {
+            orc_greek_project(n_angles, mu, mw, dF, dF + n_angles, dF + 2 * n_angles,
+                              dF + 3 * n_angles, order, dgam);
+            double *dg = d_greek + (size_t)p * 6 * (size_t)len;
+            if (normalized) {
+                double a0 = gam[0], da0 = dgam[0];
+                for (int64_t l = 0; l < 6 * len; l++) dg[l] = (dgam[l] - (gam[l] / a0) * da0) / a0;
+                dg[0] = 0.0;
+            } else {
+                for (int64_t l = 0; l < 6 * len; l++) dg[l] = dgam[l];
+            }
+        }
+        if (d_phase) {
+            int src[6] = {0, 1, 0, 2, 3, 2};
+            double *dp = d_phase + (size_t)p * 6 * (size_t)n_angles;
+            for (int e = 0; e < 6; e++)
+                for (int64_t j = 0; j < n_angles; j++) {
+                    double f = Tv[src[e] * n_angles + j], df = dF[src[e] * n_angles + j];
+                    dp[e * n_angles + j] = normalized ? (df - (f / sca) * dq[2]) / sca : df;
+                }
+        }
+    }
+    free(rn); free(rw); free(mu); free(mw); free(dS); free(T); free(dT); free(xl);
+    free(s1); free(s2); free(ds1); free(ds2); free(jac); free(abs_jac); free(Tv); free(dF);
+    free(dgam); free(gam);
     return 0;
 }
 

@@ -16,6 +16,9 @@
  *   orc_ensemble_fixed                     -> invariants (monodisperse limit, alpha1_0 = 1,
  *                                             alpha1_1 = 3g, 1/2 int F11 = 1, Rayleigh r^6 law,
  *                                             closed-form moments, finite differences)
+ *   orc_ensemble_fixed_jacobian            -> central differences of orc_ensemble_fixed_ex
+ *                                             (moving and fixed range, fixed N per node);
+ *                                             per-sphere Jacobians pinned by mie_derivatives.jsonl
  */
 #ifndef SCATRS_ORACLE_H
 #define SCATRS_ORACLE_H
@@ -107,6 +110,27 @@ int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers
                        const double *fractions, const orc_cx *m, int64_t n_radius,
                        int64_t n_angles, int64_t order, int normalized, double *scalars,
                        double *greek, double *phase);
+
+/* orc_ensemble_fixed with the ln r range given as range[2] = (t_lo, t_hi) instead of the
+ * distribution's, and N per node given as n_nodes[n_radius] (either may be NULL): for finite
+ * differences at a fixed range and fixed truncations. */
+int orc_ensemble_fixed_ex(int kind, double p0, double p1, double lambda, int layers,
+                          const double *fractions, const orc_cx *m, int64_t n_radius,
+                          int64_t n_angles, int64_t order, int normalized, const double *range,
+                          const int64_t *n_nodes, double *scalars, double *greek,
+                          double *phase);
+
+/* The outputs of orc_ensemble_fixed and their forward-mode derivatives along the P = 2 + 2L +
+ * (L - 1) directions [p0, p1, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 ..
+ * fraction_{L-2}]: d_scalars (P, 11), d_greek (P, 6, order+1) or NULL, d_phase (P, 6, n_angles)
+ * or NULL. fixed_range = 1 holds the ln r range fixed (the reference's derivative definition);
+ * 0 moves the nodes with (p0, p1). n_nodes (may be NULL) receives N per node. */
+int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, int layers,
+                                const double *fractions, const orc_cx *m, int64_t n_radius,
+                                int64_t n_angles, int64_t order, int normalized,
+                                int fixed_range, double *scalars, double *greek, double *phase,
+                                double *d_scalars, double *d_greek, double *d_phase,
+                                int64_t *n_nodes);
 
 /* Many ensembles (each its own distribution and wavelength, spheres of one index per
  * ensemble), in parallel over ensembles on `threads` OpenMP threads: the CPU baseline. */

@@ -37,6 +37,7 @@ SYMBOLS = (
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
     "scatrs_ensemble_fixed",
+    "scatrs_ensemble_fixed_vjp",
 )
 
 
@@ -76,7 +77,8 @@ def load() -> C.CDLL:
                                                 C.POINTER(i64)]
         if hasattr(L, "scatrs_ensemble_fixed_vjp"):
             L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, i32, vp, vp, i64,
-                                                    i64, i64, i32, vp, vp, vp]
+                                                    i64, i64, i32, i32, i32, vp, vp, vp, vp,
+                                                    C.POINTER(i64)]
         if L.scatrs_abi_version() != ABI_VERSION:
             raise NativeLibraryError("libscatrs_hip.so has a different ABI version")
         _lib = L

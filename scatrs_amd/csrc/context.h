@@ -26,6 +26,11 @@ struct scatrs_ctx {
         kMisc,
         kMisc2,
         kMisc3,
+        kAdjStaging,  // reverse mode: coefficient cotangents, laid out like kStaging
+        kAdjNodes,    // reverse mode: per-node table sums, backscatter sums, gradients
+        kAdjEns,      // reverse mode: per-ensemble cotangents of the sums and tables
+        kAdjSlabs,    // reverse mode: per-workgroup angle-by-column panels
+        kOrder,       // ensembles: per-node sort keys and the nodes ranked by N, histogram
         kNumBufs
     };
     void *buf[kNumBufs] = {};

@@ -1,678 +1,9 @@
 // The ensemble path on the fixed discrete grid (scatrs_ensemble_fixed): per ensemble (one
 // distribution at one wavelength), n_radius Gauss-Legendre nodes in ln r, a Mie solve per node,
 // the phase matrix at the n_angles Gauss-Legendre cosines by the parity-split angular
-// products, the quadrature-weighted reduction, and the Greek projection.
-//
-// Kernels, per chunk of ensembles sized to the coefficient staging buffer:
-//   ens_prep     - inputs checked per node (distribution, wavelength, index, validated domain),
-//                  N per node, status;
-//   ens_offsets  - per ensemble, the exclusive scan of N over its nodes (staging offsets);
-//   ens_nodes    - persistent, one node per lane: ratio chains in the wavefront's scratch slab,
-//                  a_n, b_n to the staging buffer, the node's weighted scalars;
-//   ens_angular  - one workgroup per ensemble: S1, S2 at +-mu from the staged coefficients and
-//                  the w*pi_n, w*tau_n table, F11, F12, F33, F34 weighted and summed over the
-//                  nodes in registers (fixed node order: the sums do not depend on scheduling);
-//   ens_finish   - number averages, the scalar outputs, the Greek projection with the Wigner d
-//                  table, the phase matrix.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <cstring>
-#include <string>
-#include <vector>
-
-#include "../../include/scatrs_hip.h"
-#include "checks.h"
-#include "context.h"
-#include "ensemble_common.h"
-#include "layered_device.h"
-#include "mie_device.h"
-#include "quadrature.h"
-#include "util.h"
-
-namespace scatrs {
-
-// Per-node output of the node kernel: the weighted per-particle quantities
-// [1, C_ext, C_sca, C_sca g, C_abs, C_back, r^2, r^3] (ensemble/engine.rs:1046-1056) times the
-// quadrature weight, and the node's table factor W pi r^2 4 / x^2.
-constexpr int kNodeVals = 8;
-
-struct EnsArgs {
-    const int32_t *kind;
-    const double *p0, *p1, *wl, *fr, *m;
-    int layers;
-    int64_t n_radius;
-    const double *rnodes, *rweights;  // GL rule on [-1, 1]
-};
-
-struct NodeGeom {
-    double r, x, W;
-};
-
-__device__ __forceinline__ NodeGeom node_geometry(const EnsArgs &a, int64_t e, int64_t i) {
-    const int kind = a.kind[e];
-    const double p0 = a.p0[e], p1 = a.p1[e];
-    double t_lo, t_hi;
-    radius_range(kind, p0, p1, &t_lo, &t_hi);
-    const double half = 0.5 * (t_hi - t_lo), mid = 0.5 * (t_hi + t_lo);
-    const double r = exp(mid + half * a.rnodes[i]);
-    const double u = exp(log_density(kind, p0, p1, r, nullptr, nullptr));
-    NodeGeom g;
-    g.r = r;
-    g.x = 2.0 * kPi * r / a.wl[e];
-    g.W = a.rweights[i] * half * u * r;
-    return g;
-}
-
-// Status codes of the ensemble prep, beyond kValid/kInvalid/kOutside: which input failed.
-enum : int { kBadDist = 1 << 4, kBadWl = 2 << 4, kBadM = 3 << 4, kBadFr = 4 << 4 };
-
-// Checks of one layered node (layered/mod.rs:229-319, layered/domain.rs:53-95), x_l = f_l x.
-__device__ int layered_code(double x, const double *fr, const double *m, int L, int allow,
-                            long long *n_out) {
-    *n_out = 0;
-    double xl_prev = 0.0;
-    for (int l = 0; l < L; ++l) {
-        const double xl = (l < L - 1) ? fr[l] * x : x;
-        if (!isfinite(xl)) return kInvalid;
-        if (l > 0 && xl < xl_prev) return kInvalid;
-        xl_prev = xl;
-        const double mr = m[2 * l], mi = m[2 * l + 1];
-        if (!(isfinite(mr) && isfinite(mi))) return kInvalid;
-        if (mr < 0.0 || mi < 0.0) return kInvalid;
-        const double am = hypot(mr, mi);
-        if (!(am >= kMinIndex && am <= kMaxIndex)) return kInvalid;
-    }
-    const double x0 = L > 1 ? fr[0] * x : x;
-    if (x0 < kMinSize || x > kMaxSize) return kInvalid;
-    const long long n = default_n_max(x);
-    if (n > kMaxOrder) return kInvalid;
-    if (!allow) {
-        bool in = x >= 1e-6 && x <= 1e4 && L <= 16;
-        for (int l = 0; l < L; ++l) {
-            const double mr = m[2 * l], mi = m[2 * l + 1], am = hypot(mr, mi);
-            in = in && am * x <= 3e4 && mr > 0.0 && mr <= 10.0 && mi >= 0.0 && mi <= 10.0 &&
-                 am >= 0.1;
-        }
-        if (!in) return kOutside;
-    }
-    *n_out = n;
-    return kValid;
-}
-
-__global__ void __launch_bounds__(256) ens_prep(EnsArgs a, int64_t count, int allow,
-                                                int *node_n, Partial *partial) {
-    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t total = count * a.n_radius;
-    int n = 0;
-    bool ok = false;
-    unsigned long long bad = ~0ull, outside = ~0ull;
-    if (g < total) {
-        const int64_t e = g / a.n_radius, i = g % a.n_radius;
-        int code = kValid;
-        if (!dist_valid(a.kind[e], a.p0[e], a.p1[e])) code = kInvalid | kBadDist;
-        else if (!(isfinite(a.wl[e]) && a.wl[e] > 0.0)) code = kInvalid | kBadWl;
-        else if (a.layers > 1) {
-            const double *fr = a.fr + e * (a.layers - 1);
-            for (int l = 0; l < a.layers - 1; ++l)
-                if (!(fr[l] > 0.0 && fr[l] <= 1.0) || (l > 0 && fr[l] < fr[l - 1]))
-                    code = kInvalid | kBadFr;
-        }
-        if (code == kValid) {
-            const NodeGeom geo = node_geometry(a, e, i);
-            long long nn = 0;
-            const int c = a.layers == 1
-                              ? sphere_code(geo.x, a.m[2 * e], a.m[2 * e + 1], 0, allow, &nn)
-                              : layered_code(geo.x, a.fr + e * (a.layers - 1),
-                                             a.m + 2 * e * a.layers, a.layers, allow, &nn);
-            code = c;
-            n = (int)nn;
-        }
-        if ((code & 15) == kInvalid) bad = (unsigned long long)g;
-        if ((code & 15) == kOutside) outside = (unsigned long long)g;
-        ok = code == kValid;
-        node_n[g] = ok ? n : 0;
-    }
-    block_partial(n, ok, bad, outside, partial);
-}
-
-// Exclusive scan of N over each ensemble's nodes: node_off (terms) and ens_terms.
-__global__ void __launch_bounds__(256) ens_offsets(const int *node_n, int64_t n_radius,
-                                                   int64_t *node_off, int64_t *ens_terms) {
-    __shared__ int64_t part[256];
-    const int64_t e = blockIdx.x;
-    const int t = threadIdx.x;
-    const int64_t per = (n_radius + 255) / 256;
-    const int64_t lo = t * per < n_radius ? t * per : n_radius;
-    const int64_t hi = lo + per < n_radius ? lo + per : n_radius;
-    const int *nn = node_n + e * n_radius;
-    int64_t local = 0;
-    for (int64_t i = lo; i < hi; ++i) local += nn[i];
-    part[t] = local;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int64_t add = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += add;
-        __syncthreads();
-    }
-    int64_t run = part[t] - local;
-    for (int64_t i = lo; i < hi; ++i) {
-        node_off[e * n_radius + i] = run;
-        run += nn[i];
-    }
-    if (t == 255) ens_terms[e] = part[255];
-}
-
-// w pi_n(mu_j), w tau_n(mu_j) at the non-negative cosines, n = 1..n_max (amplitudes.rs:52-65),
-// table[(n - 1) * h + jj].
-__global__ void pt_table(const double *mu_pos, int h, int n_max, double2 *table) {
-    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jj >= h) return;
-    const double mu = mu_pos[jj];
-    double pi_prev = 0.0, pi = 1.0;
-    for (int n = 1; n <= n_max; ++n) {
-        const double nf = n, n1 = nf + 1.0;
-        const double tau = nf * mu * pi - n1 * pi_prev;
-        const double w = (double)(2 * n + 1) / (nf * n1);
-        table[(size_t)(n - 1) * h + jj] = make_double2(w * pi, w * tau);
-        const double pi_next = ((double)(2 * n + 1) * mu * pi - n1 * pi_prev) / nf;
-        pi_prev = pi;
-        pi = pi_next;
-    }
-}
-
-// Writes a node's coefficients to the staging buffer and accumulates its efficiency sums.
-struct StagingSink {
-    cx *a_out, *b_out;
-    EfficiencySink<false> eff;
-    __device__ void order(int k, cx a, cx b) {
-        a_out[k - 1] = a;
-        b_out[k - 1] = b;
-        eff.order(k, a, b);
-    }
-    __device__ void order_d(int, cx, cx, cx, cx, cx, cx) {}
-    __device__ void stored(int, cx, cx, cx, cx, cx, cx) {}
-    __device__ void done(double s, double j0, double j1, double j2) { eff.done(s, j0, j1, j2); }
-    __device__ void zero_absorption() { eff.zero_absorption(); }
-};
-
-// Persistent node solves over the chunk's nodes [0, chunk_nodes): node g is ensemble
-// e0 + g / n_radius, radius node g % n_radius.
-template <int L>
-__global__ __launch_bounds__(256) void ens_nodes(EnsArgs a, int64_t e0, int64_t chunk_nodes,
-                                                 const int *node_n, const int64_t *node_off,
-                                                 const int64_t *ens_base,
-                                                 unsigned long long *counter, cx *scratch,
-                                                 int64_t rows, cx *staging, double *vals,
-                                                 double *tabc) {
-    __shared__ cx ztile[L == 1 ? 4 : 1][L == 1 ? kZBlock * 64 : 1];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    cx *tile = L == 1 ? &ztile[threadIdx.x >> 6][lane] : nullptr;
-    (void)tile;
-    constexpr int kCols = LayeredColumns<L>::kComplex;
-    cx *slab = scratch + wave * rows * 64 * (kCols + 1) + lane;
-    while (true) {
-        unsigned long long chunk = 0;
-        if (lane == 0) chunk = atomicAdd(counter, 1ull);
-        chunk = __shfl(chunk, 0, 64);
-        if ((int64_t)chunk * 64 >= chunk_nodes) break;
-        const int64_t g = (int64_t)chunk * 64 + lane;
-        if (g < chunk_nodes) {
-            const int64_t e = e0 + g / a.n_radius, i = g % a.n_radius;
-            const int64_t gi = e * a.n_radius + i;
-            const NodeGeom geo = node_geometry(a, e, i);
-            const int n = node_n[gi];
-            const int64_t off = ens_base[e] + node_off[gi];
-            StagingSink sink;
-            sink.a_out = staging + 2 * off;
-            sink.b_out = staging + 2 * off + n;
-            if constexpr (L == 1) {
-                const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
-                // Checkpoints in the first column, u_k(x) in the second, the block being
-                // used in the wavefront's LDS tile.
-                double *ux = real_column(slab, rows * 64);
-                const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
-                x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
-                NoOuter none;
-                mie_orders<false>(geo.x, m, n, zs, ux, 64, none, sink);
-            } else {
-                layered_solve<L, false>(geo.x, a.fr + e * (L - 1), a.m + 2 * e * L, n, slab,
-                                        rows, sink);
-            }
-            double q[5];
-            sink.eff.values(geo.x, q);
-            const double area = kPi * geo.r * geo.r;
-            const double W = geo.W;
-            double *v = vals + g * kNodeVals;
-            v[0] = W;
-            v[1] = W * (area * q[0]);
-            v[2] = W * (area * q[1]);
-            v[3] = W * (area * q[1] * q[4]);
-            v[4] = W * (area * q[2]);
-            v[5] = W * (area * q[3]);
-            v[6] = W * (geo.r * geo.r);
-            v[7] = W * (geo.r * geo.r * geo.r);
-            tabc[g] = W * (area * (4.0 / (geo.x * geo.x)));
-        }
-    }
-}
-
-constexpr int kTileNodes = 4;
-constexpr int kChunkOrders = 64;
-
-// One workgroup per ensemble; thread t owns the non-negative cosine jj = t (+ blockDim per
-// pass) and its mirror. For each tile of 4 nodes the coefficients are staged in LDS 64 orders
-// at a time; per order, the E/O parity split: A += G1 u, B += G2 v, C += G1 v, D += G2 u with
-// G1, G2 = w pi_n, w tau_n (odd n) or w tau_n, w pi_n (even n) and u, v = a_n, b_n (odd) or
-// b_n, a_n (even), so that S1(+-mu) = A +- B and S2(+-mu) = C +- D.
-__global__ __launch_bounds__(512) void ens_angular(int64_t e0, int64_t n_radius, int n_angles,
-                                                   int h, const int *node_n,
-                                                   const int64_t *node_off,
-                                                   const int64_t *ens_base, const cx *staging,
-                                                   const double *tabc, const double2 *table,
-                                                   double *tab_out) {
-    __shared__ cx lds_a[kTileNodes][kChunkOrders];
-    __shared__ cx lds_b[kTileNodes][kChunkOrders];
-    const int64_t e = e0 + blockIdx.x;
-    const int t = threadIdx.x;
-    const int64_t base = ens_base[e];
-    const int *nn = node_n + e * n_radius;
-    const int64_t *noff = node_off + e * n_radius;
-    const int64_t gbase = blockIdx.x * n_radius;  // node index within the chunk
-    double *out = tab_out + (int64_t)blockIdx.x * 4 * n_angles;
-    for (int pass = 0; pass * (int)blockDim.x < h; ++pass) {
-        const int jj = pass * blockDim.x + t;
-        const bool active = jj < h;
-        double T[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][F11, F12, F33, F34]
-        for (int64_t i0 = 0; i0 < n_radius; i0 += kTileNodes) {
-            int ns[kTileNodes];
-            int nmax = 0;
-#pragma unroll
-            for (int q = 0; q < kTileNodes; ++q) {
-                ns[q] = (i0 + q < n_radius) ? nn[i0 + q] : 0;
-                nmax = ns[q] > nmax ? ns[q] : nmax;
-            }
-            cx A[kTileNodes], B[kTileNodes], C[kTileNodes], D[kTileNodes];
-#pragma unroll
-            for (int q = 0; q < kTileNodes; ++q) A[q] = B[q] = C[q] = D[q] = real(0.0);
-            for (int kc = 0; kc < nmax; kc += kChunkOrders) {
-                __syncthreads();
-                for (int idx = t; idx < kTileNodes * kChunkOrders; idx += blockDim.x) {
-                    const int q = idx / kChunkOrders, k = idx % kChunkOrders;
-                    const int n = kc + k;
-                    cx va = real(0.0), vb = real(0.0);
-                    if (n < ns[q]) {
-                        const cx *c = staging + 2 * (base + noff[i0 + q]);
-                        va = c[n];
-                        vb = c[ns[q] + n];
-                    }
-                    lds_a[q][k] = va;
-                    lds_b[q][k] = vb;
-                }
-                __syncthreads();
-                if (active) {
-                    const int kend = nmax - kc < kChunkOrders ? nmax - kc : kChunkOrders;
-                    for (int k = 0; k < kend; ++k) {
-                        const int n = kc + k + 1;  // the order
-                        const double2 pt = table[(size_t)(n - 1) * h + jj];
-                        const double g1 = (n & 1) ? pt.x : pt.y;
-                        const double g2 = (n & 1) ? pt.y : pt.x;
-#pragma unroll
-                        for (int q = 0; q < kTileNodes; ++q) {
-                            const cx a = lds_a[q][k], b = lds_b[q][k];
-                            const cx u = (n & 1) ? a : b;
-                            const cx v = (n & 1) ? b : a;
-                            A[q] = A[q] + u * g1;
-                            B[q] = B[q] + v * g2;
-                            C[q] = C[q] + v * g1;
-                            D[q] = D[q] + u * g2;
-                        }
-                    }
-                }
-            }
-            if (active) {
-#pragma unroll
-                for (int q = 0; q < kTileNodes; ++q) {
-                    if (i0 + q >= n_radius) continue;
-                    const double c = tabc[gbase + i0 + q];
-#pragma unroll
-                    for (int sgn = 0; sgn < 2; ++sgn) {
-                        const cx s1 = sgn == 0 ? A[q] + B[q] : A[q] - B[q];
-                        const cx s2 = sgn == 0 ? C[q] + D[q] : C[q] - D[q];
-                        const double n1 = norm2(s1), n2 = norm2(s2);
-                        const cx p = s2 * conj(s1);
-                        T[sgn][0] += c * (0.5 * (n1 + n2));
-                        T[sgn][1] += c * (0.5 * (n2 - n1));
-                        T[sgn][2] += c * p.re;
-                        T[sgn][3] += c * p.im;
-                    }
-                }
-            }
-        }
-        if (active) {
-            const int jp = n_angles - h + jj;     // the cosine's index in the ascending rule
-            const int jm = n_angles - 1 - jp;     // its mirror
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                out[k * n_angles + jp] = T[0][k];
-                if (jm != jp) out[k * n_angles + jm] = T[1][k];
-            }
-        }
-    }
-}
-
-// The angular products on FP64 matrix cores (v_mfma_f64_16x16x4f64), as GEMMs per group of
-// 16 radius nodes: C1 = G1 X and C2 = G2 X with G1, G2 (angles x orders) the w pi_n, w tau_n
-// table by parity and X (orders x 64) the nodes' u, v (a_n, b_n by parity) split into re, im:
-// column 4q + c of a 16-column tile is node q's (u.re, u.im, v.re, v.im)[c]. A workgroup is
-// one ensemble and 128 non-negative cosines (8 wavefronts x 16); the coefficients of the
-// group are staged in LDS 32 orders at a time. After a group's K loop each lane holds, for 4
-// cosines and its (node, component), C1 and C2; the quad of lanes of one node exchanges the
-// four components, forms S1(+-mu) = A +- B, S2(+-mu) = C +- D (A, C = G1 u, G1 v; B, D =
-// G2 v, G2 u), and the lane of component c keeps F11, F12, F33 or F34 (c = 0..3), times the
-// node's factor, summed over the groups in registers; the four node lanes are added at the
-// end, always in the same order.
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
-
-constexpr int kGroupNodes = 16;
-constexpr int kMfmaOrders = 32;
-constexpr int kSliceAngles = 128;
-
-__global__ __launch_bounds__(512) void ens_angular_mfma(
-    int64_t e0, int64_t n_radius, int n_angles, int h, int slices, const int *node_n,
-    const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double *tabc,
-    const double2 *table, double *tab_out) {
-    __shared__ cx lds_a[kGroupNodes][kMfmaOrders];
-    __shared__ cx lds_b[kGroupNodes][kMfmaOrders];
-    const int64_t eb = blockIdx.x / slices;
-    const int slice = blockIdx.x % slices;
-    const int64_t e = e0 + eb;
-    const int t = threadIdx.x;
-    const int lane = t & 63, wave = t >> 6;
-    const int row = lane & 15, kq = lane >> 4;  // A: angle row, order; B: column, order
-    const int comp = lane & 3, qin = (lane & 15) >> 2;
-    const int jj = slice * kSliceAngles + wave * 16 + row;  // this lane's A-operand cosine
-    const int64_t base = ens_base[e];
-    const int *nn = node_n + e * n_radius;
-    const int64_t *noff = node_off + e * n_radius;
-    const int64_t gbase = eb * n_radius;
-    double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][v]: this lane's element, rows kq+4v
-    for (int64_t g0 = 0; g0 < n_radius; g0 += kGroupNodes) {
-        int nmax = 0;
-        for (int q = 0; q < kGroupNodes; ++q) {
-            const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-            nmax = nq > nmax ? nq : nmax;
-        }
-        mfma_d4 acc1[4], acc2[4];
-#pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
-            acc1[tt] = mfma_d4{0, 0, 0, 0};
-            acc2[tt] = mfma_d4{0, 0, 0, 0};
-        }
-        for (int kc = 0; kc < nmax; kc += kMfmaOrders) {
-            __syncthreads();
-            for (int idx = t; idx < kGroupNodes * kMfmaOrders; idx += blockDim.x) {
-                const int q = idx / kMfmaOrders, k = idx % kMfmaOrders;
-                const int n = kc + k;
-                const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-                cx va = real(0.0), vb = real(0.0);
-                if (n < nq) {
-                    const cx *c = staging + 2 * (base + noff[g0 + q]);
-                    va = c[n];
-                    vb = c[nq + n];
-                }
-                lds_a[q][k] = va;
-                lds_b[q][k] = vb;
-            }
-            __syncthreads();
-            const int kend = nmax - kc < kMfmaOrders ? nmax - kc : kMfmaOrders;
-            for (int k4 = 0; k4 < kend; k4 += 4) {
-                const int kk = k4 + kq;              // order index in the chunk
-                const int n = kc + kk + 1;           // the order
-                double2 pt = make_double2(0.0, 0.0);
-                if (jj < h && n <= nmax) pt = table[(size_t)(n - 1) * h + jj];
-                const bool odd = n & 1;
-                const double g1 = odd ? pt.x : pt.y;
-                const double g2 = odd ? pt.y : pt.x;
-#pragma unroll
-                for (int tt = 0; tt < 4; ++tt) {
-                    const int q = 4 * tt + qin;
-                    const cx a = lds_a[q][kk], b = lds_b[q][kk];
-                    const cx u = odd ? a : b, v = odd ? b : a;
-                    const double val = comp == 0 ? u.re : comp == 1 ? u.im : comp == 2 ? v.re : v.im;
-                    acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, val, acc1[tt], 0, 0, 0);
-                    acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g2, val, acc2[tt], 0, 0, 0);
-                }
-            }
-        }
-        // Epilogue of the group: the lane's column is node 4 tt + qin, component comp.
-#pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
-            const int64_t node = g0 + 4 * tt + qin;
-            const double cfac = node < n_radius ? tabc[gbase + node] : 0.0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double c1[4], c2[4];
-                const double x1 = acc1[tt][v], x2 = acc2[tt][v];
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
-                    c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
-                }
-                const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
-                const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
-#pragma unroll
-                for (int sgn = 0; sgn < 2; ++sgn) {
-                    const cx s1 = sgn == 0 ? A + B : A - B;
-                    const cx s2 = sgn == 0 ? C + D : C - D;
-                    double f;
-                    if (comp == 0) f = 0.5 * (norm2(s1) + norm2(s2));
-                    else if (comp == 1) f = 0.5 * (norm2(s2) - norm2(s1));
-                    else {
-                        const cx pp = s2 * conj(s1);
-                        f = comp == 2 ? pp.re : pp.im;
-                    }
-                    tp[sgn][v] += cfac * f;
-                }
-            }
-        }
-    }
-    // Add the four node lanes (qin = 0..3) of each element, in a fixed order.
-#pragma unroll
-    for (int sgn = 0; sgn < 2; ++sgn)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double x = tp[sgn][v];
-            x += __shfl_xor(x, 4, 64);
-            x += __shfl_xor(x, 8, 64);
-            tp[sgn][v] = x;
-        }
-    if (qin == 0) {
-        double *out = tab_out + eb * 4 * (int64_t)n_angles;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16 (i % 4), element i / 4.
-            const int jr = slice * kSliceAngles + wave * 16 + kq + 4 * v;  // output cosine
-            if (jr >= h) continue;
-            const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
-            out[comp * n_angles + jp] = tp[0][v];
-            if (jm != jp) out[comp * n_angles + jm] = tp[1][v];
-        }
-    }
-}
-
-// Averages and outputs of one ensemble (ensemble/engine.rs:1782-1836,2396-2497): scalars in
-// SCALARS order, Greek coefficients projected with the Wigner d table
-// dtab[(p * n_angles + j) * (order + 1) + l] (p = d00, d22, d2,-2, d02), the phase matrix.
-__global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_radius, int n_angles,
-                                                  const double *aw, const double *vals,
-                                                  const double *tab, const double *dtab,
-                                                  int64_t order, int normalized,
-                                                  double *scalars, double *greek,
-                                                  double *phase) {
-    extern __shared__ double sh[];
-    double *S = sh;           // 8 sums
-    double *F = sh + 8;       // 4 * n_angles, divided by Z
-    const int64_t e = e0 + blockIdx.x;
-    const int t = threadIdx.x;
-    const double *v = vals + blockIdx.x * n_radius * kNodeVals;
-    if (t < kNodeVals) {
-        double s = 0.0;
-        for (int64_t i = 0; i < n_radius; ++i) s += v[i * kNodeVals + t];
-        S[t] = s;
-    }
-    __syncthreads();
-    const double z = S[0];
-    const double *te = tab + (int64_t)blockIdx.x * 4 * n_angles;
-    for (int j = t; j < 4 * n_angles; j += blockDim.x) F[j] = te[j] / z;
-    __syncthreads();
-    const double ext = S[1] / z, sca = S[2] / z, scag = S[3] / z, absb = S[4] / z,
-                 back = S[5] / z, r2 = S[6] / z, r3 = S[7] / z;
-    if (t == 0) {
-        double *o = scalars + e * 11;
-        o[0] = ext;
-        o[1] = sca;
-        o[2] = absb;
-        o[3] = back;
-        o[4] = sca / ext;
-        o[5] = scag / sca;
-        o[6] = 4.0 * kPi * (ext / back);
-        o[7] = 0.0;
-        o[8] = r3 / r2;
-        o[9] = kPi * r2;
-        o[10] = 4.0 * kPi / 3.0 * r3;
-    }
-    const double *F11 = F, *F12 = F + n_angles, *F33 = F + 2 * n_angles, *F34 = F + 3 * n_angles;
-    if (phase) {
-        // [F11, F12, F22 = F11, F33, F34, F44 = F33], divided by <C_sca> when normalized
-        // (ensemble/engine.rs:2459-2490).
-        double *ph = phase + e * 6 * (int64_t)n_angles;
-        const double *src[6] = {F11, F12, F11, F33, F34, F33};
-        for (int k = 0; k < 6; ++k)
-            for (int j = t; j < n_angles; j += blockDim.x)
-                ph[k * n_angles + j] = normalized ? src[k][j] / sca : src[k][j];
-    }
-    if (!greek) return;
-    const int64_t len = order + 1;
-    double *gr = greek + e * 6 * len;
-    // alpha1_0 first, for the normalization (greek.rs:499-509; engine.rs:2447-2458).
-    __shared__ double a0;
-    if (t == 0) {
-        double s = 0.0;
-        for (int j = 0; j < n_angles; ++j) s += (aw[j] * F11[j]) * dtab[(0 * n_angles + j) * len];
-        a0 = 0.5 * s;
-    }
-    __syncthreads();
-    for (int64_t l = t; l < len; l += blockDim.x) {
-        double s11 = 0, s33 = 0, sp = 0, sm = 0, s12 = 0, s34 = 0;
-        for (int j = 0; j < n_angles; ++j) {
-            const double w = aw[j];
-            const double f11 = w * F11[j], f33 = w * F33[j];
-            const double d00 = dtab[(0 * n_angles + j) * len + l];
-            const double d22 = dtab[(1 * (int64_t)n_angles + j) * len + l];
-            const double d2m2 = dtab[(2 * (int64_t)n_angles + j) * len + l];
-            const double d02 = dtab[(3 * (int64_t)n_angles + j) * len + l];
-            s11 += f11 * d00;
-            s33 += f33 * d00;
-            sp += (f11 + f33) * d22;
-            sm += (f11 - f33) * d2m2;
-            s12 += (w * F12[j]) * d02;
-            s34 += (w * F34[j]) * d02;
-        }
-        const double c = (double)(2 * l + 1) * 0.5;
-        double g6[6] = {c * s11, c * 0.5 * (sp + sm), c * 0.5 * (sp - sm), c * s33, c * s12,
-                        c * s34};
-        if (normalized) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) g6[k] /= a0;
-            if (l == 0) g6[0] = 1.0;
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) gr[k * len + l] = g6[k];
-    }
-}
-
-// ---- host side ----
-
-struct Rule {
-    std::vector<double> nodes, weights;
-};
-
-// Uploads host data to a grow-only device buffer.
-template <class T>
-int upload(scatrs_ctx *ctx, int which, const std::vector<T> &v, T **out) {
-    int rc = check_hip(ctx, ensure(ctx, which, v.size() * sizeof(T)), "alloc");
-    if (rc) return rc;
-    *out = static_cast<T *>(ctx->buf[which]);
-    return check_hip(ctx, hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T),
-                                         hipMemcpyHostToDevice, ctx->stream),
-                     "upload");
-}
-
-std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &dev, int64_t n_radius,
-                             const std::vector<double> &rnodes, int allow, int *code_out) {
-    const int64_t e = g / n_radius, i = g % n_radius;
-    const int L = dev.layers;
-    int32_t kind = 0;
-    double p0 = 0, p1 = 0, wl = 0;
-    std::vector<double> fr(std::max(L - 1, 0)), m(2 * L);
-    (void)hipMemcpy(&kind, dev.kind + e, sizeof kind, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&p0, dev.p0 + e, sizeof p0, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&p1, dev.p1 + e, sizeof p1, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&wl, dev.wl + e, sizeof wl, hipMemcpyDeviceToHost);
-    if (L > 1)
-        (void)hipMemcpy(fr.data(), dev.fr + e * (L - 1), fr.size() * sizeof(double),
-                        hipMemcpyDeviceToHost);
-    (void)hipMemcpy(m.data(), dev.m + e * 2 * L, m.size() * sizeof(double), hipMemcpyDeviceToHost);
-    const std::string pre = "ensemble " + std::to_string(e) + ": ";
-    *code_out = kInvalid;
-    if (!dist_valid(kind, p0, p1)) {
-        if (kind == 0) {
-            if (!(std::isfinite(p0) && p0 > 0.0))
-                return pre + "median_radius must be finite and > 0, got " + fmt_e(p0);
-            return pre + "the lognormal width must be finite and > 1, got " + fmt_e(p1);
-        }
-        if (kind == 1) {
-            if (!(std::isfinite(p0) && p0 > 0.0))
-                return pre + "effective_radius must be finite and > 0, got " + fmt_e(p0);
-            return pre + "effective_variance must be in (0, 0.5), got " + fmt_e(p1);
-        }
-        return pre + "unknown distribution kind " + std::to_string(kind);
-    }
-    if (!(std::isfinite(wl) && wl > 0.0))
-        return pre + "the wavelength must be finite and > 0, got " + fmt_e(wl);
-    for (int l = 0; l + 1 < L; ++l)
-        if (!(fr[l] > 0.0 && fr[l] <= 1.0) || (l > 0 && fr[l] < fr[l - 1]))
-            return pre + "the radius fractions must be nondecreasing in (0, 1]";
-    double t_lo, t_hi;
-    radius_range(kind, p0, p1, &t_lo, &t_hi);
-    const double r = std::exp(0.5 * (t_hi + t_lo) + 0.5 * (t_hi - t_lo) * rnodes[i]);
-    const double x = 2.0 * kPi * r / wl;
-    const std::string at = pre + "at wavelength " + fmt_e(wl) + ", radius " + fmt_e(r) + ": ";
-    std::string msg;
-    if (L == 1) {
-        *code_out = sphere_message(x, m[0], m[1], 0, allow, &msg);
-        return at + msg;
-    }
-    *code_out = kOutside;
-    return at + "a layered sphere of " + std::to_string(L) + " layers with x_L = " + fmt_e(x) +
-           " is outside the validated domain (x_L in [1e-6, 1e4], |m| x_L <= 3e4, every "
-           "layer's 0 < Re m <= 10, 0 <= Im m <= 10, |m| >= 0.1), or its inputs are invalid";
-}
-
-template <int L>
-void launch_nodes(unsigned blocks, hipStream_t s, const EnsArgs &a, int64_t e0, int64_t nodes,
-                  const int *node_n, const int64_t *node_off, const int64_t *ens_base,
-                  unsigned long long *counter, cx *scratch, int64_t rows, cx *staging,
-                  double *vals, double *tabc) {
-    ens_nodes<L><<<blocks, 256, 0, s>>>(a, e0, nodes, node_n, node_off, ens_base, counter,
-                                        scratch, rows, staging, vals, tabc);
-}
-
-}  // namespace scatrs
+// products, the quadrature-weighted reduction, and the Greek projection. The kernels live in
+// ensemble_kernels.h.
+#include "ensemble_kernels.h"
 
 using namespace scatrs;
 
@@ -686,175 +17,31 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
     if (!ctx) return SCATRS_INVALID_INPUT;
     ctx->error.clear();
     if (total_terms) *total_terms = 0;
-    if (count < 0) return fail(ctx, SCATRS_INVALID_INPUT, "the number of ensembles must be >= 0");
-    if (layers < 1 || layers > kMaxLayersDevice)
-        return fail(ctx, SCATRS_INVALID_INPUT,
-                    "layers must be in [1, " + std::to_string(kMaxLayersDevice) + "], got " +
-                        std::to_string(layers));
-    if (n_radius < 1 || n_radius > 65536)
-        return fail(ctx, SCATRS_INVALID_INPUT, "n_radius must be in [1, 65536]");
-    if (n_angles < 2 || n_angles > 4096)
-        return fail(ctx, SCATRS_INVALID_INPUT, "n_angles must be in [2, 4096]");
-    if (d_greek && (order < 0 || order > 65536))
-        return fail(ctx, SCATRS_INVALID_INPUT, "the Greek order must be in [0, 65536]");
+    int rc = ens_check_args(ctx, count, layers, n_radius, n_angles, d_greek != nullptr, order);
+    if (rc) return rc;
     if (count == 0) return SCATRS_OK;
     if (!d_kind || !d_p0 || !d_p1 || !d_wavelength || !d_m || !d_scalars ||
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
-    int rc;
-    if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
+    EnsSetup S;
+    if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_wavelength, layers, d_fractions, d_m,
+                        n_radius, n_angles, d_greek != nullptr, order, allow_unvalidated, S)))
+        return rc;
+    if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
-    // Quadrature tables (host, then uploaded).
-    Rule rr, ar;
-    gauss_legendre(n_radius, rr.nodes, rr.weights);
-    gauss_legendre(n_angles, ar.nodes, ar.weights);
-    std::vector<double> rtab(2 * n_radius);
-    std::copy(rr.nodes.begin(), rr.nodes.end(), rtab.begin());
-    std::copy(rr.weights.begin(), rr.weights.end(), rtab.begin() + n_radius);
-    const int h = (int)((n_angles + 1) / 2);
-    std::vector<double> atab(n_angles + h);  // weights, then the non-negative cosines
-    std::copy(ar.weights.begin(), ar.weights.end(), atab.begin());
-    for (int jj = 0; jj < h; ++jj) atab[n_angles + jj] = ar.nodes[n_angles - h + jj];
-    double *d_rtab = nullptr, *d_atab = nullptr;
-    if ((rc = upload(ctx, scatrs_ctx::kMisc, rtab, &d_rtab))) return rc;
-    if ((rc = upload(ctx, scatrs_ctx::kMisc2, atab, &d_atab))) return rc;
-    EnsArgs a{d_kind, d_p0, d_p1, d_wavelength, d_fractions, d_m, layers, n_radius,
-              d_rtab, d_rtab + n_radius};
-    // Per-node N and status.
-    const int64_t total = count * n_radius;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, sizeof(Status) + 64), "alloc")))
-        return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kKeys, (size_t)total * sizeof(int)), "alloc")))
-        return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kPerm,
-                                    (size_t)total * sizeof(int64_t) + (size_t)count * 2 * sizeof(int64_t)),
-                        "alloc")))
-        return rc;
-    Status *st = static_cast<Status *>(ctx->buf[scatrs_ctx::kCounters]);
-    unsigned long long *counter = reinterpret_cast<unsigned long long *>(
-        static_cast<char *>(ctx->buf[scatrs_ctx::kCounters]) + sizeof(Status));
-    int *node_n = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
-    int64_t *node_off = static_cast<int64_t *>(ctx->buf[scatrs_ctx::kPerm]);
-    int64_t *ens_terms = node_off + total;
-    int64_t *ens_base = ens_terms + count;
-    const int64_t pblocks = (total + 255) / 256;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHist, (size_t)pblocks * sizeof(Partial)),
-                        "alloc")))
-        return rc;
-    Partial *partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kHist]);
-    ens_prep<<<(unsigned)pblocks, 256, 0, s>>>(a, count, allow_unvalidated, node_n, partial);
-    reduce_status<<<1, 1024, 0, s>>>(partial, pblocks, st);
-    ens_offsets<<<(unsigned)count, 256, 0, s>>>(node_n, n_radius, node_off, ens_terms);
-    Status *hst = static_cast<Status *>(ctx->host_pinned);
-    if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
-                        "status copy")))
-        return rc;
-    std::vector<int64_t> terms(count);
-    if ((rc = check_hip(ctx, hipMemcpyAsync(terms.data(), ens_terms, count * sizeof(int64_t),
-                                            hipMemcpyDeviceToHost, s),
-                        "terms copy")))
-        return rc;
-    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "ensemble prep"))) return rc;
-    const unsigned long long bad = std::min(hst->first_bad, hst->first_outside);
-    if (bad != ~0ull) {
-        int code = kInvalid;
-        const std::string msg = ensemble_message(ctx, (int64_t)bad, a, n_radius, rr.nodes,
-                                                 allow_unvalidated, &code);
-        return fail(ctx, code == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
-                    msg);
-    }
-    if (total_terms) *total_terms = (int64_t)hst->total_terms;
-    const int n_max = hst->n_max;
-    // The w pi_n, w tau_n table at the non-negative cosines up to the largest N.
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables, (size_t)n_max * h * sizeof(double2)),
-                        "table")))
-        return rc;
-    double2 *table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
-    pt_table<<<(h + 63) / 64, 64, 0, s>>>(d_atab + n_angles, h, n_max, table);
-    // The Wigner d table (host).
-    double *d_dtab = nullptr;
-    if (d_greek) {
-        const int64_t len = order + 1;
-        std::vector<double> dt(4 * (size_t)n_angles * len);
-        const int pm[4][2] = {{0, 0}, {2, 2}, {2, -2}, {0, 2}};
-        for (int p = 0; p < 4; ++p)
-            for (int64_t j = 0; j < n_angles; ++j)
-                wigner_d_small(pm[p][0], pm[p][1], ar.nodes[j], order,
-                               dt.data() + ((size_t)p * n_angles + j) * len);
-        if ((rc = upload(ctx, scatrs_ctx::kMisc3, dt, &d_dtab))) return rc;
-    }
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B).
-    const size_t staging_cap = (size_t)3 << 30;
-    const int64_t max_terms_chunk = (int64_t)(staging_cap / (2 * sizeof(cx)));
-    std::vector<int64_t> base(count);
-    // Node scratch slab per persistent wavefront: (complex columns + 1) x rows x 64.
-    const int64_t rows = n_max + 2;
-    const int ccols = layers == 1 ? 1 : 2 * layers - 1;
-    const size_t per_wave = (size_t)rows * 64 * (ccols + 1) * sizeof(cx);
+    const int64_t max_terms = (int64_t)(((size_t)3 << 30) / (2 * sizeof(cx)));
     int64_t e0 = 0;
     while (e0 < count) {
-        int64_t e1 = e0, acc = 0;
-        while (e1 < count && (e1 == e0 || acc + terms[e1] <= max_terms_chunk) &&
-               (e1 - e0) * n_radius < (int64_t)1 << 26) {
-            base[e1] = acc;
-            acc += terms[e1];
-            ++e1;
-        }
-        const int64_t ne = e1 - e0, nodes = ne * n_radius;
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kStaging, (size_t)std::max<int64_t>(acc, 1) * 2 * sizeof(cx)),
-                            "staging")))
+        int64_t chunk_terms = 0;
+        const int64_t e1 = next_chunk(S, e0, count, n_radius, max_terms, &chunk_terms);
+        const int64_t ne = e1 - e0;
+        ChunkBufs B;
+        if ((rc = forward_chunk(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, nullptr)))
             return rc;
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kNodes,
-                                        (size_t)nodes * (kNodeVals + 1) * sizeof(double)),
-                            "nodes")))
-            return rc;
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAccum,
-                                        (size_t)ne * 4 * n_angles * sizeof(double)),
-                            "tables")))
-            return rc;
-        int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (nodes + 63) / 64);
-        waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((6ull << 30) / per_wave)));
-        waves = (waves + 3) / 4 * 4;
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves), "scratch")))
-            return rc;
-        // ens_base holds base - (staging offset of e0) for this chunk's ensembles.
-        if ((rc = check_hip(ctx, hipMemcpyAsync(ens_base + e0, base.data() + e0, ne * sizeof(int64_t),
-                                                hipMemcpyHostToDevice, s),
-                            "base copy")))
-            return rc;
-        cx *staging = static_cast<cx *>(ctx->buf[scatrs_ctx::kStaging]);
-        double *vals = static_cast<double *>(ctx->buf[scatrs_ctx::kNodes]);
-        double *tabc = vals + nodes * kNodeVals;
-        double *tab_out = static_cast<double *>(ctx->buf[scatrs_ctx::kAccum]);
-        cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
-        fill_u64<<<1, 1, 0, s>>>(counter, 0ull, 1);
-        mark_begin(ctx);
-        const unsigned blocks = (unsigned)(waves / 4);
-        switch (layers) {
-            case 1: launch_nodes<1>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            case 2: launch_nodes<2>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            case 3: launch_nodes<3>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            case 4: launch_nodes<4>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            case 5: launch_nodes<5>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            case 6: launch_nodes<6>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            case 7: launch_nodes<7>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-            default: launch_nodes<8>(blocks, s, a, e0, nodes, node_n, node_off, ens_base, counter, scratch, rows, staging, vals, tabc); break;
-        }
-        if (ctx->angular_kernel == 1) {
-            const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
-            ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, node_n,
-                                                    node_off, ens_base, staging, tabc, table,
-                                                    tab_out);
-        } else {
-            const int slices = (h + kSliceAngles - 1) / kSliceAngles;
-            ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
-                e0, n_radius, (int)n_angles, h, slices, node_n, node_off, ens_base, staging,
-                tabc, table, tab_out);
-        }
-        mark_end(ctx);
         const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
-        ens_finish<<<(unsigned)ne, 256, shmem, s>>>(e0, n_radius, (int)n_angles, d_atab, vals,
-                                                    tab_out, d_dtab, order, normalized,
+        ens_finish<<<(unsigned)ne, 256, shmem, s>>>(e0, n_radius, (int)n_angles, S.d_atab, B.vals,
+                                                    B.tab_out, S.d_dtab, order, normalized,
                                                     d_scalars, d_greek, d_phase);
         if ((rc = check_hip(ctx, hipGetLastError(), "ensemble launch"))) return rc;
         e0 = e1;

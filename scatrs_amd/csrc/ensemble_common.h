@@ -40,6 +40,37 @@ SCATRS_HD void radius_range(int kind, double p0, double p1, double *t_lo, double
     }
 }
 
+// Derivatives of the range's midpoint and half-width in ln r with respect to (p0, p1), and the
+// half-width itself.
+SCATRS_HD double range_derivatives(int kind, double p0, double p1, double dmid[2],
+                                   double dhalf[2]) {
+    if (kind == 0) {
+        dmid[0] = 1.0 / p0;
+        dmid[1] = 0.0;
+        dhalf[0] = 0.0;
+        dhalf[1] = 5.0 / p1;
+        return 5.0 * log(p1);
+    }
+    const double q = 1.0 - 2.0 * p1;
+    const double s = sqrt(p1 / q);
+    const double ds = 0.5 / (s * q * q);  // d s / d b
+    dmid[0] = 1.0 / p0;
+    dmid[1] = -1.5 * ds;
+    dhalf[0] = 0.0;
+    dhalf[1] = 6.5 * ds;
+    return 6.5 * s;
+}
+
+// d(ln u)/d(ln r) at fixed parameters.
+SCATRS_HD double log_density_slope(int kind, double p0, double p1, double r) {
+    if (kind == 0) {
+        const double s = log(p1);
+        return -(log(r) - log(p0)) / (s * s) - 1.0;
+    }
+    const double alpha = (1.0 - 3.0 * p1) / p1;
+    return alpha - r / (p0 * p1);
+}
+
 // ln u(r) and d(ln u)/d(p0, p1), with the reference radius held fixed
 // (distribution.rs:366-430).
 SCATRS_HD double log_density(int kind, double p0, double p1, double r, double *d0, double *d1) {

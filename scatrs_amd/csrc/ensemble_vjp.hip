@@ -1,0 +1,704 @@
+// Reverse mode of the fixed-grid ensemble path (scatrs_ensemble_fixed_vjp): the vector-Jacobian
+// product of the scalar, Greek and phase outputs with respect to each ensemble's distribution
+// parameters, layer indices and radius fractions.
+//
+// The loss of one ensemble is L = sum_k Sbar_k S_k + sum_{k,j} Tbar_kj T_kj over the sums the
+// forward kernels form, S = sum_i W_i h_i (h = [1, C_ext, C_sca, C_sca g, C_abs, C_back, r^2,
+// r^3]) and T = sum_i W_i (lambda^2 / pi) F_i (the table factor pi r^2 4 / x^2 is
+// lambda^2 / pi). Every cross section is lambda^2 times a function of a_n, b_n alone (C_abs of
+// the absorption sum), so a node's share phi_i = W_i H_i depends on its radius only through
+// the coefficients, r^2, r^3 and the weight. Per chunk:
+//   forward_chunk    - the forward node solves and angular products (coefficients staged,
+//                      per-node backscatter sums kept);
+//   ens_finish_vjp   - per ensemble, the reverse of ens_finish: Sbar (8) and Tbar (4 x angles);
+//   ens_angular_vjp  - per (ensemble, 16 nodes): S1, S2 at every cosine again on the FP64
+//                      matrix cores, the cotangents of S1, S2 from Tbar, and the transposed
+//                      products back onto the orders: the table part of each node's a_n, b_n
+//                      cotangents, and the node's sum Tbar . F;
+//   ens_nodes_vjp    - per node, a_n, b_n and their derivatives recomputed (not stored): the
+//                      cotangents completed with the scalar outputs' part, contracted with the
+//                      coefficient derivatives (and, for C_abs, the absorption sum's closed-form
+//                      Jacobian);
+//   ens_node_grads   - per node, those contractions with the node's weight and radius
+//                      derivatives: its share of the gradient;
+//   ens_grad_reduce  - per ensemble, the nodes' gradients summed in node order.
+#include "adjoint_device.h"
+#include "ensemble_kernels.h"
+
+namespace scatrs {
+
+// ---- the reverse of ens_finish ----
+
+// Sum of one value per thread over a block of 256, in a fixed tree order.
+static __device__ double block_sum(double v, double *red) {
+    const int t = threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) red[t] += red[t + off];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+// From the output cotangents of one ensemble (bar_scalars (11), bar_greek (6 x (order + 1)),
+// bar_phase (6 x n_angles); each may be null) those of the sums, sbar[eb * 8 + k], and of the
+// summed tables, tbar[(eb * 4 + k) * n_angles + j]. Follows ens_finish step by step in reverse
+// (ensemble/engine.rs:1782-1836,2396-2497 for the forward steps).
+static __global__ __launch_bounds__(256) void ens_finish_vjp(
+    int64_t e0, int64_t n_radius, int n_angles, const double *aw, const double *vals,
+    const double *tab, const double *dtab, int64_t order, int normalized,
+    const double *bar_scalars, const double *bar_greek, const double *bar_phase, double *sbar,
+    double *tbar) {
+    extern __shared__ double sh[];
+    double *S = sh;      // 8 sums
+    double *F = sh + 8;  // 4 * n_angles, divided by Z
+    __shared__ double red[256];
+    __shared__ double a0_sh;
+    const int64_t e = e0 + blockIdx.x;
+    const int t = threadIdx.x;
+    const double *v = vals + blockIdx.x * n_radius * kNodeVals;
+    if (t < kNodeVals) {
+        double s = 0.0;
+        for (int64_t i = 0; i < n_radius; ++i) s += v[i * kNodeVals + t];
+        S[t] = s;
+    }
+    __syncthreads();
+    const double z = S[0];
+    const double *te = tab + (int64_t)blockIdx.x * 4 * n_angles;
+    for (int j = t; j < 4 * n_angles; j += blockDim.x) F[j] = te[j] / z;
+    __syncthreads();
+    const double ext = S[1] / z, sca = S[2] / z, scag = S[3] / z, back = S[5] / z, r2 = S[6] / z,
+                 r3 = S[7] / z;
+    const double *F11 = F, *F12 = F + n_angles, *F33 = F + 2 * n_angles, *F34 = F + 3 * n_angles;
+    const int64_t len = order + 1;
+    const bool greek = bar_greek && dtab;
+    const double *gb = greek ? bar_greek + e * 6 * len : nullptr;
+    // alpha1_0 and, normalized, the cotangent it receives through the division.
+    double a0 = 1.0, a0bar = 0.0;
+    if (greek) {
+        if (t == 0) {
+            double s = 0.0;
+            for (int j = 0; j < n_angles; ++j)
+                s += (aw[j] * F11[j]) * dtab[(0 * n_angles + j) * len];
+            a0_sh = 0.5 * s;
+        }
+        __syncthreads();
+        a0 = a0_sh;
+        if (normalized) {
+            // sum over (k, l) != (0, 0) of gbar_kl G_kl, G = gamma / a0 the normalized output
+            double part = 0.0;
+            for (int64_t l = t; l < len; l += blockDim.x) {
+                double s11 = 0, s33 = 0, sp = 0, sm = 0, s12 = 0, s34 = 0;
+                for (int j = 0; j < n_angles; ++j) {
+                    const double w = aw[j];
+                    const double f11 = w * F11[j], f33 = w * F33[j];
+                    const double d00 = dtab[(0 * n_angles + j) * len + l];
+                    const double d22 = dtab[(1 * (int64_t)n_angles + j) * len + l];
+                    const double d2m2 = dtab[(2 * (int64_t)n_angles + j) * len + l];
+                    const double d02 = dtab[(3 * (int64_t)n_angles + j) * len + l];
+                    s11 += f11 * d00;
+                    s33 += f33 * d00;
+                    sp += (f11 + f33) * d22;
+                    sm += (f11 - f33) * d2m2;
+                    s12 += (w * F12[j]) * d02;
+                    s34 += (w * F34[j]) * d02;
+                }
+                const double c = (double)(2 * l + 1) * 0.5;
+                const double g6[6] = {c * s11, c * 0.5 * (sp + sm), c * 0.5 * (sp - sm),
+                                      c * s33, c * s12, c * s34};
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k != 0 || l != 0) part += gb[k * len + l] * (g6[k] / a0);
+            }
+            a0bar = -block_sum(part, red) / a0;
+        }
+    }
+    // Phase: ph = src / sca_n with sca_n = <C_sca> when normalized.
+    const double sca_n = normalized ? sca : 1.0;
+    const double *pb = bar_phase ? bar_phase + e * 6 * (int64_t)n_angles : nullptr;
+    double fbf = 0.0, ph_sca = 0.0;
+    double *tb = tbar + (int64_t)blockIdx.x * 4 * n_angles;
+    for (int j = t; j < n_angles; j += blockDim.x) {
+        double b11 = 0.0, b12 = 0.0, b33 = 0.0, b34 = 0.0;
+        if (greek) {
+            const double w = aw[j];
+            const double *d00 = dtab + (0 * (int64_t)n_angles + j) * len;
+            const double *d22 = dtab + (1 * (int64_t)n_angles + j) * len;
+            const double *d2m2 = dtab + (2 * (int64_t)n_angles + j) * len;
+            const double *d02 = dtab + (3 * (int64_t)n_angles + j) * len;
+            double s11 = 0, s33 = 0, s12 = 0, s34 = 0;
+            for (int64_t l = 0; l < len; ++l) {
+                const double c = (double)(2 * l + 1) * 0.5;
+                double gbar[6];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) gbar[k] = normalized ? gb[k * len + l] / a0 : gb[k * len + l];
+                if (normalized && l == 0) gbar[0] = a0bar;
+                const double sp = 0.5 * (gbar[1] + gbar[2]), sm = 0.5 * (gbar[1] - gbar[2]);
+                s11 += c * (gbar[0] * d00[l] + sp * d22[l] + sm * d2m2[l]);
+                s33 += c * (gbar[3] * d00[l] + sp * d22[l] - sm * d2m2[l]);
+                s12 += c * (gbar[4] * d02[l]);
+                s34 += c * (gbar[5] * d02[l]);
+            }
+            b11 = w * s11;
+            b12 = w * s12;
+            b33 = w * s33;
+            b34 = w * s34;
+        }
+        if (pb) {
+            const double p11 = pb[0 * n_angles + j] + pb[2 * n_angles + j];
+            const double p12 = pb[1 * n_angles + j];
+            const double p33 = pb[3 * n_angles + j] + pb[5 * n_angles + j];
+            const double p34 = pb[4 * n_angles + j];
+            b11 += p11 / sca_n;
+            b12 += p12 / sca_n;
+            b33 += p33 / sca_n;
+            b34 += p34 / sca_n;
+            if (normalized)
+                ph_sca -= (p11 * F11[j] + p12 * F12[j] + p33 * F33[j] + p34 * F34[j]) / (sca * sca);
+        }
+        fbf += b11 * F11[j] + b12 * F12[j] + b33 * F33[j] + b34 * F34[j];
+        tb[0 * n_angles + j] = b11 / z;
+        tb[1 * n_angles + j] = b12 / z;
+        tb[2 * n_angles + j] = b33 / z;
+        tb[3 * n_angles + j] = b34 / z;
+    }
+    fbf = block_sum(fbf, red);
+    ph_sca = block_sum(ph_sca, red);
+    if (t == 0) {
+        double ob[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (bar_scalars)
+            for (int k = 0; k < 11; ++k) ob[k] = bar_scalars[e * 11 + k];
+        // o = [ext, sca, abs, back, sca/ext, scag/sca, 4 pi ext/back, 0, r3/r2, pi r2, 4pi/3 r3]
+        const double q_ext = ob[0] - ob[4] * sca / (ext * ext) + ob[6] * 4.0 * kPi / back;
+        const double q_sca = ob[1] + ob[4] / ext - ob[5] * scag / (sca * sca) + ph_sca;
+        const double q_scag = ob[5] / sca;
+        const double q_abs = ob[2];
+        const double q_back = ob[3] - ob[6] * 4.0 * kPi * ext / (back * back);
+        const double q_r2 = -ob[8] * r3 / (r2 * r2) + ob[9] * kPi;
+        const double q_r3 = ob[8] / r2 + ob[10] * 4.0 * kPi / 3.0;
+        const double q[7] = {q_ext, q_sca, q_scag, q_abs, q_back, q_r2, q_r3};
+        double zbar = -fbf;
+        double *sb = sbar + blockIdx.x * 8;
+        for (int k = 0; k < 7; ++k) {
+            zbar -= q[k] * (S[k + 1] / z);
+            sb[k + 1] = q[k] / z;
+        }
+        sb[0] = zbar / z;
+    }
+}
+
+// ---- the angular products in reverse ----
+
+constexpr int kVjpNodes = 16;   // nodes per group: 64 matrix-core columns
+constexpr int kVjpOrders = 32;  // coefficient chunk of the forward products
+constexpr int kVjpAngles = 32;  // panel chunk of the transposed products
+constexpr int kVjpOrderRows = 128;
+
+// Persistent: each workgroup takes (ensemble, group of 16 nodes) items from `counter`.
+//  1. S1, S2 at every non-negative cosine and its mirror (as ens_angular_mfma, but per node).
+//  2. With Tbar at +-mu, g1 = dL/dS1 = (b11 - b12) S1 + (b33 - i b34) S2 and
+//     g2 = dL/dS2 = (b11 + b12) S2 + (b33 + i b34) S1 (L = Tbar . F as a real function of the
+//     real and imaginary parts); the node's Tbar . F is summed on the way.
+//  3. S1(+-mu) = A +- B, S2(+-mu) = C +- D with A, C = G1 u, G1 v and B, D = G2 v, G2 u give
+//     ubar = G1^T (g1+ + g1-) + G2^T (g2+ - g2-), vbar = G2^T (g1+ - g1-) + G1^T (g2+ + g2-),
+//     the cotangents (dL = Re sum conj(ubar) du) of u, v = a_n, b_n (odd n) or b_n, a_n (even n).
+//     The panels P = [g1+ + g1-, g2+ + g2-] and M = [g2+ - g2-, g1+ - g1-] (re, im per node)
+//     are written to the workgroup's slab, and two transposed products per 128 orders form
+//     G1^T P + G2^T M: columns 4q + c hold node q's (ubar.re, ubar.im, vbar.re, vbar.im)[c].
+static __global__ __launch_bounds__(512) void ens_angular_vjp(
+    int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double2 *table,
+    const double *tbar, unsigned long long *counter, double *slabs, cx *abar, double *ht) {
+    __shared__ cx lds_a[kVjpNodes][kVjpOrders];
+    __shared__ cx lds_b[kVjpNodes][kVjpOrders];
+    __shared__ double lds_g[2][kVjpAngles][64];
+    __shared__ double lds_ht[8][kVjpNodes];
+    __shared__ unsigned long long item_sh;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    const int comp = lane & 3, qin = (lane & 15) >> 2;
+    double *slab = slabs + (size_t)blockIdx.x * 2 * 64 * h;
+    const int64_t groups = (n_radius + kVjpNodes - 1) / kVjpNodes;
+    const int64_t items = ne * groups;
+    while (true) {
+        __syncthreads();  // the previous item is done with the LDS and the slab
+        if (t == 0) item_sh = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t item = (int64_t)item_sh;
+        if (item >= items) break;
+        const int64_t eb = item / groups, g0 = (item % groups) * kVjpNodes;
+        const int64_t e = e0 + eb;
+        const int64_t base = ens_base[e];
+        const int *nn = node_n + e * n_radius;
+        const int64_t *noff = node_off + e * n_radius;
+        const double *tb = tbar + eb * 4 * (int64_t)n_angles;
+        int nmax = 0;
+        for (int q = 0; q < kVjpNodes; ++q) {
+            const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
+            nmax = nq > nmax ? nq : nmax;
+        }
+        double hts[4] = {0.0, 0.0, 0.0, 0.0};
+        // 1-2: forward products per slice of 128 cosines, cotangent panels to the slab.
+        for (int slice = 0; slice * kSliceAngles < h; ++slice) {
+            const int jj = slice * kSliceAngles + wave * 16 + row;
+            mfma_d4 acc1[4], acc2[4];
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) {
+                acc1[tt] = mfma_d4{0, 0, 0, 0};
+                acc2[tt] = mfma_d4{0, 0, 0, 0};
+            }
+            for (int kc = 0; kc < nmax; kc += kVjpOrders) {
+                __syncthreads();
+                for (int idx = t; idx < kVjpNodes * kVjpOrders; idx += blockDim.x) {
+                    const int q = idx / kVjpOrders, k = idx % kVjpOrders;
+                    const int n = kc + k;
+                    const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
+                    cx va = real(0.0), vb = real(0.0);
+                    if (n < nq) {
+                        const cx *c = staging + 2 * (base + noff[g0 + q]);
+                        va = c[n];
+                        vb = c[nq + n];
+                    }
+                    lds_a[q][k] = va;
+                    lds_b[q][k] = vb;
+                }
+                __syncthreads();
+                const int kend = nmax - kc < kVjpOrders ? nmax - kc : kVjpOrders;
+                for (int k4 = 0; k4 < kend; k4 += 4) {
+                    const int kk = k4 + kq;
+                    const int n = kc + kk + 1;
+                    double2 pt = make_double2(0.0, 0.0);
+                    if (jj < h && n <= nmax) pt = table[(size_t)(n - 1) * h + jj];
+                    const bool odd = n & 1;
+                    const double g1 = odd ? pt.x : pt.y;
+                    const double g2 = odd ? pt.y : pt.x;
+#pragma unroll
+                    for (int tt = 0; tt < 4; ++tt) {
+                        const int q = 4 * tt + qin;
+                        const cx a = lds_a[q][kk], b = lds_b[q][kk];
+                        const cx u = odd ? a : b, v = odd ? b : a;
+                        const double val = comp == 0 ? u.re : comp == 1 ? u.im : comp == 2 ? v.re : v.im;
+                        acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, val, acc1[tt], 0, 0, 0);
+                        acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g2, val, acc2[tt], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double c1[4], c2[4];
+                    const double x1 = acc1[tt][v], x2 = acc2[tt][v];
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
+                        c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
+                    }
+                    // D[i][j] sits in lane j + 16 (i % 4), element i / 4: this lane's cosine row.
+                    const int jr = slice * kSliceAngles + wave * 16 + kq + 4 * v;
+                    if (jr >= h) continue;
+                    const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
+                    const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
+                    const cx s1p = A + B, s1m = A - B, s2p = C + D, s2m = C - D;
+                    const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
+                    double bp[4], bm[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        bp[k] = tb[k * n_angles + jp];
+                        bm[k] = jm != jp ? tb[k * n_angles + jm] : 0.0;
+                    }
+                    const cx g1p = s1p * (bp[0] - bp[1]) + s2p * bp[2] - times_i(s2p) * bp[3];
+                    const cx g2p = s2p * (bp[0] + bp[1]) + s1p * bp[2] + times_i(s1p) * bp[3];
+                    const cx g1m = s1m * (bm[0] - bm[1]) + s2m * bm[2] - times_i(s2m) * bm[3];
+                    const cx g2m = s2m * (bm[0] + bm[1]) + s1m * bm[2] + times_i(s1m) * bm[3];
+                    const cx P1 = g1p + g1m, M1 = g1p - g1m, P2 = g2p + g2m, M2 = g2p - g2m;
+                    const double pv = comp == 0 ? P1.re : comp == 1 ? P1.im : comp == 2 ? P2.re : P2.im;
+                    const double mv = comp == 0 ? M2.re : comp == 1 ? M2.im : comp == 2 ? M1.re : M1.im;
+                    slab[(size_t)jr * 64 + 16 * tt + row] = pv;
+                    slab[((size_t)h + jr) * 64 + 16 * tt + row] = mv;
+                    // this lane's element of F (comp: F11, F12, F33, F34) at +mu and -mu
+                    double fp, fm;
+                    if (comp == 0) {
+                        fp = 0.5 * (norm2(s1p) + norm2(s2p));
+                        fm = 0.5 * (norm2(s1m) + norm2(s2m));
+                    } else if (comp == 1) {
+                        fp = 0.5 * (norm2(s2p) - norm2(s1p));
+                        fm = 0.5 * (norm2(s2m) - norm2(s1m));
+                    } else {
+                        const cx pp = s2p * conj(s1p), pm = s2m * conj(s1m);
+                        fp = comp == 2 ? pp.re : pp.im;
+                        fm = comp == 2 ? pm.re : pm.im;
+                    }
+                    hts[tt] += bp[comp] * fp + bm[comp] * fm;
+                }
+            }
+        }
+        // The node's Tbar . F: over the components and rows of the wave, then over the waves.
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            double x = hts[tt];
+            x += __shfl_xor(x, 1, 64);
+            x += __shfl_xor(x, 2, 64);
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+            hts[tt] = x;
+        }
+        if (comp == 0 && kq == 0)
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) lds_ht[wave][4 * tt + qin] = hts[tt];
+        __syncthreads();  // the slab's panels and lds_ht are complete
+        if (t < kVjpNodes && g0 + t < n_radius) {
+            double s = 0.0;
+            for (int w = 0; w < 8; ++w) s += lds_ht[w][t];
+            ht[eb * n_radius + g0 + t] = s;
+        }
+        // 3: the transposed products, 128 orders (16 per wave) at a time.
+        for (int nb = 0; nb < nmax; nb += kVjpOrderRows) {
+            const int n_row = nb + wave * 16 + row + 1;  // this lane's A-operand order
+            const bool odd_row = n_row & 1;
+            mfma_d4 d1[4], d2[4];
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) {
+                d1[tt] = mfma_d4{0, 0, 0, 0};
+                d2[tt] = mfma_d4{0, 0, 0, 0};
+            }
+            for (int jc = 0; jc < h; jc += kVjpAngles) {
+                __syncthreads();
+                for (int idx = t; idx < 2 * kVjpAngles * 64; idx += blockDim.x) {
+                    const int s = idx / (kVjpAngles * 64), k = (idx / 64) % kVjpAngles, col = idx % 64;
+                    const int j = jc + k;
+                    lds_g[s][k][col] = j < h ? slab[((size_t)s * h + j) * 64 + col] : 0.0;
+                }
+                __syncthreads();
+                const int kend = h - jc < kVjpAngles ? h - jc : kVjpAngles;
+                for (int k4 = 0; k4 < kend; k4 += 4) {
+                    const int jb = jc + k4 + kq;
+                    double2 pt = make_double2(0.0, 0.0);
+                    if (n_row <= nmax && jb < h) pt = table[(size_t)(n_row - 1) * h + jb];
+                    const double g1 = odd_row ? pt.x : pt.y;
+                    const double g2 = odd_row ? pt.y : pt.x;
+#pragma unroll
+                    for (int tt = 0; tt < 4; ++tt) {
+                        const double b1 = lds_g[0][k4 + kq][16 * tt + row];
+                        const double b2 = lds_g[1][k4 + kq][16 * tt + row];
+                        d1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, b1, d1[tt], 0, 0, 0);
+                        d2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g2, b2, d2[tt], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) {
+                const int64_t node = g0 + 4 * tt + qin;
+                if (node >= n_radius) continue;
+                const int nq = nn[node];
+                cx *cb = abar + 2 * (base + noff[node]);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int n = nb + wave * 16 + kq + 4 * v + 1;
+                    if (n > nq) continue;
+                    const double val = d1[tt][v] + d2[tt][v];
+                    const bool to_a = (comp < 2) == (bool)(n & 1);  // u is a_n at odd n
+                    double *dst = reinterpret_cast<double *>(cb + (to_a ? 0 : nq) + n - 1);
+                    dst[comp & 1] = val;
+                }
+            }
+        }
+    }
+}
+
+// ---- the node solves in reverse ----
+
+// One node's coefficients and their derivatives recomputed and contracted by AdjointSink.
+// raw[g * 6L + ...] receives, per layer l (core first), t_x[l] = dH/dx_l and
+// t_m[l] = sum conj(cot) da/dm_l (re, im) through the coefficients, and the absorption sum's
+// derivatives [d/dx_l, d/dRe m_l, d/dIm m_l]: [t_x (L), t_m.re (L), t_m.im (L), jabs (3 L)].
+// The solve holds most of the lane's registers, so the per-node gradient is assembled by
+// ens_node_grads afterwards. Without table cotangents (abar null) the staged coefficients
+// stand in for them with weight 0, so the order loop carries no branch on their presence.
+template <int L>
+__device__ __forceinline__ void vjp_node(const EnsArgs &a, int64_t e0, int64_t g,
+                                         const int *node_n, const int64_t *node_off,
+                                         const int64_t *ens_base, cx *slab, int64_t rows,
+                                         cx *tile, const cx *staging, const cx *abar,
+                                         const cx *backs, const double *sbar, double *raw) {
+    const int64_t eb = g / a.n_radius, i = g % a.n_radius;
+    const int64_t e = e0 + eb;
+    const int64_t gi = e * a.n_radius + i;
+    const NodeGeom geo = node_geometry(a, e, i);
+    const int n = node_n[gi];
+    const int64_t off = ens_base[e] + node_off[gi];
+    const double lam = a.wl[e], l2 = lam * lam;
+    const double *sb = sbar + eb * 8;
+    AdjointSink<L> sink;
+    sink.init();
+    sink.sa = staging + 2 * off;
+    sink.sb = staging + 2 * off + n;
+    const cx *tcot = abar ? abar : staging;
+    sink.ta = tcot + 2 * off;
+    sink.tb = tcot + 2 * off + n;
+    sink.n = n;
+    sink.tfac = abar ? l2 / kPi : 0.0;
+    sink.alpha = (sb[1] + sb[2]) * l2 / (2.0 * kPi);
+    sink.beta = sb[5] * l2 / (2.0 * kPi);
+    sink.gamma = sb[3] * l2 / kPi;
+    sink.back = backs[g];
+    if constexpr (L == 1) {
+        const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
+        double *ux = real_column(slab, rows * 64);
+        const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
+        x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
+        NoOuter none;
+        mie_orders<true>(geo.x, m, n, zs, ux, 64, none, sink);
+    } else {
+        (void)tile;
+        layered_solve<L, true>(geo.x, a.fr + e * (L - 1), a.m + 2 * e * L, n, slab, rows, sink);
+    }
+    double *o = raw + g * (6 * L);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        o[l] = sink.tx[l];
+        o[L + l] = sink.tm[l].re;
+        o[2 * L + l] = sink.tm[l].im;
+        o[3 * L + 3 * l] = sink.jabs[l][0];
+        o[3 * L + 3 * l + 1] = sink.jabs[l][1];
+        o[3 * L + 3 * l + 2] = sink.jabs[l][2];
+    }
+}
+
+// Spheres: persistent, as ens_nodes.
+__global__ __launch_bounds__(256) void ens_nodes_vjp(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *node_n, const int64_t *node_off,
+    const int64_t *ens_base, unsigned long long *counter, cx *scratch, int64_t rows,
+    const cx *staging, const cx *abar, const cx *backs, const double *sbar, double *raw) {
+    __shared__ cx ztile[4][kZBlock * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *tile = &ztile[threadIdx.x >> 6][lane];
+    cx *slab = scratch + wave * rows * 64 * 2 + lane;
+    while (true) {
+        unsigned long long chunk = 0;
+        if (lane == 0) chunk = atomicAdd(counter, 1ull);
+        chunk = __shfl(chunk, 0, 64);
+        if ((int64_t)chunk * 64 >= chunk_nodes) break;
+        const int64_t g = (int64_t)chunk * 64 + lane;
+        if (g < chunk_nodes)
+            vjp_node<1>(a, e0, g, node_n, node_off, ens_base, slab, rows, tile, staging, abar,
+                        backs, sbar, raw);
+    }
+}
+
+// Layered spheres: one group of 64 ranked nodes per wavefront, no work loop (as
+// ens_nodes_ranked, for the same reason).
+template <int L>
+__global__ __launch_bounds__(256) void ens_nodes_vjp_ranked(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, int64_t start,
+    const int *node_n, const int64_t *node_off, const int64_t *ens_base, cx *scratch,
+    int64_t rows, const cx *staging, const cx *abar, const cx *backs, const double *sbar,
+    double *raw) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    constexpr int kCols = LayeredColumns<L>::kComplex;
+    cx *slab = scratch + wave * rows * 64 * (kCols + 1) + lane;
+    const int64_t idx = start + wave * 64 + lane;
+    if (idx < chunk_nodes)
+        vjp_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, nullptr, staging,
+                    abar, backs, sbar, raw);
+}
+
+// One thread per node of the chunk: gnode[g * P + p], the node's share of the gradient,
+// p = [p0, p1, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, f_0 .. f_{L-2}], from the solve's
+// contractions and the node's weight and radius. dH/dx_l, dH/dRe m_l, dH/dIm m_l with
+// H = phi / W; C_abs = (lambda^2/2pi) abs_sum enters C_ext and C_abs. With fixed_range the
+// ln r range is held at its value (the distribution parameters act through the density alone,
+// as the reference's derivatives do); otherwise the nodes and the range's half-width move with
+// (p0, p1), the derivative of this discrete path.
+static __global__ __launch_bounds__(256) void ens_node_grads(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const double *raw, const double *vals,
+    const double *tabc, const double *ht, const double *sbar, int fixed_range, double *gnode) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= chunk_nodes) return;
+    const int L = a.layers;
+    const int P = 2 + 2 * L + (L - 1);
+    const int64_t eb = g / a.n_radius, i = g % a.n_radius;
+    const int64_t e = e0 + eb;
+    const NodeGeom geo = node_geometry(a, e, i);
+    const double lam = a.wl[e], l2 = lam * lam;
+    const double *sb = sbar + eb * 8;
+    const double *rw = raw + g * (6 * L);
+    const double *fr = a.fr + e * (L - 1);
+    const double cabs = (sb[1] + sb[4]) * l2 / (2.0 * kPi);
+    const double r = geo.r, W = geo.W;
+    double *out = gnode + g * P;
+    double dxt = 0.0;
+    for (int l = 0; l < L; ++l) {
+        const double dx = rw[l] + cabs * rw[3 * L + 3 * l];
+        const double dre = rw[L + l] + cabs * rw[3 * L + 3 * l + 1];
+        const double dim = -rw[2 * L + l] + cabs * rw[3 * L + 3 * l + 2];
+        out[2 + 2 * l] = W * dre;
+        out[3 + 2 * l] = W * dim;
+        if (l < L - 1) {
+            out[2 + 2 * L + l] = W * geo.x * dx;
+            dxt += fr[l] * dx;
+        } else {
+            dxt += dx;
+        }
+    }
+    const double dh_dlnr = geo.x * dxt + 2.0 * sb[6] * (r * r) + 3.0 * sb[7] * (r * r * r);
+    const double *v = vals + g * kNodeVals;
+    double phi = tabc[g] * ht[g];
+    for (int k = 0; k < kNodeVals; ++k) phi += sb[k] * v[k];
+    const int kind = a.kind[e];
+    const double p0 = a.p0[e], p1 = a.p1[e];
+    double dmid[2], dhalf[2], du[2];
+    const double half = range_derivatives(kind, p0, p1, dmid, dhalf);
+    log_density(kind, p0, p1, r, &du[0], &du[1]);
+    const double slope = log_density_slope(kind, p0, p1, r);
+    const double xi = a.rnodes[i];
+    for (int th = 0; th < 2; ++th) {
+        const double dlnr = fixed_range ? 0.0 : dmid[th] + xi * dhalf[th];
+        const double dlnw = (fixed_range ? 0.0 : dhalf[th] / half) + du[th] + (slope + 1.0) * dlnr;
+        out[th] = phi * dlnw + W * dh_dlnr * dlnr;
+    }
+}
+
+// grad[(e0 + eb) * P + p] = sum over the ensemble's nodes, in node order.
+static __global__ void ens_grad_reduce(int64_t n_radius, int P, const double *gnode,
+                                       double *grad) {
+    const int64_t eb = blockIdx.x;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        double s = 0.0;
+        const double *gp = gnode + eb * n_radius * P + p;
+        for (int64_t i = 0; i < n_radius; ++i) s += gp[i * P];
+        grad[eb * P + p] = s;
+    }
+}
+
+template <int L>
+static void launch_nodes_vjp_ranked(unsigned blocks, int64_t per_launch, hipStream_t s,
+                                    const EnsArgs &a, int64_t e0, int64_t nodes,
+                                    const EnsSetup &S, cx *scratch, int64_t rows,
+                                    const ChunkBufs &B, const cx *abar, const cx *backs,
+                                    const double *sbar, double *raw) {
+    for (int64_t start = 0; start < nodes; start += per_launch)
+        ens_nodes_vjp_ranked<L><<<blocks, 256, 0, s>>>(a, e0, nodes, B.order, start, S.node_n,
+                                                       S.node_off, S.ens_base, scratch, rows,
+                                                       B.staging, abar, backs, sbar, raw);
+}
+
+}  // namespace scatrs
+
+using namespace scatrs;
+
+extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                                         const double *d_p0, const double *d_p1,
+                                         const double *d_wavelength, int layers,
+                                         const double *d_fractions, const double *d_m,
+                                         int64_t n_radius, int64_t n_angles, int64_t order,
+                                         int normalized, int fixed_range,
+                                         int allow_unvalidated, const double *d_bar_scalars,
+                                         const double *d_bar_greek, const double *d_bar_phase,
+                                         double *d_grad, int64_t *total_terms) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->error.clear();
+    if (total_terms) *total_terms = 0;
+    int rc = ens_check_args(ctx, count, layers, n_radius, n_angles, d_bar_greek != nullptr, order);
+    if (rc) return rc;
+    if (count == 0) return SCATRS_OK;
+    if (!d_kind || !d_p0 || !d_p1 || !d_wavelength || !d_m || !d_grad ||
+        (layers > 1 && !d_fractions))
+        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    EnsSetup S;
+    if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_wavelength, layers, d_fractions, d_m,
+                        n_radius, n_angles, d_bar_greek != nullptr, order, allow_unvalidated, S)))
+        return rc;
+    if (total_terms) *total_terms = S.total_terms;
+    hipStream_t s = ctx->stream;
+    const int P = 2 + 2 * layers + (layers - 1);
+    const int h = S.h;
+    const bool tables = d_bar_greek || d_bar_phase;
+    // Staging and cotangents take 32 B per term each.
+    const int64_t max_terms = (int64_t)(((size_t)3 << 30) / (2 * sizeof(cx)));
+    const unsigned vjp_blocks = (unsigned)std::max(1, ctx->cus * 2);
+    int64_t e0 = 0;
+    while (e0 < count) {
+        int64_t chunk_terms = 0;
+        const int64_t e1 = next_chunk(S, e0, count, n_radius, max_terms, &chunk_terms);
+        const int64_t ne = e1 - e0, nodes = ne * n_radius;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjNodes,
+                                        (size_t)nodes * (3 + P + 6 * layers) * sizeof(double)),
+                            "alloc")))
+            return rc;
+        double *ht = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjNodes]);
+        cx *backs = reinterpret_cast<cx *>(ht + nodes);
+        double *gnode = ht + 3 * nodes;
+        double *raw = gnode + P * nodes;
+        ChunkBufs B;
+        if ((rc = forward_chunk(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, backs)))
+            return rc;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjEns,
+                                        (size_t)ne * (8 + 4 * n_angles) * sizeof(double)),
+                            "alloc")))
+            return rc;
+        double *sbar = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjEns]);
+        double *tbar = sbar + ne * 8;
+        const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
+        ens_finish_vjp<<<(unsigned)ne, 256, shmem, s>>>(
+            e0, n_radius, (int)n_angles, S.d_atab, B.vals, B.tab_out, S.d_dtab, order, normalized,
+            d_bar_scalars, d_bar_greek, d_bar_phase, sbar, tbar);
+        cx *abar = nullptr;
+        if (tables) {
+            if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjStaging,
+                                            (size_t)std::max<int64_t>(chunk_terms, 1) * 2 * sizeof(cx)),
+                                "alloc")))
+                return rc;
+            if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjSlabs,
+                                            (size_t)vjp_blocks * 2 * 64 * h * sizeof(double)),
+                                "alloc")))
+                return rc;
+            abar = static_cast<cx *>(ctx->buf[scatrs_ctx::kAdjStaging]);
+            double *slabs = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjSlabs]);
+            fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+            ens_angular_vjp<<<vjp_blocks, 512, 0, s>>>(e0, ne, n_radius, (int)n_angles, h,
+                                                       S.node_n, S.node_off, S.ens_base,
+                                                       B.staging, S.table, tbar, S.counter,
+                                                       slabs, abar, ht);
+        } else {
+            fill_f64<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(ht, 0.0, nodes);
+        }
+        const NodeShape ns = node_shape(ctx, S, nodes);
+        cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
+        const unsigned blocks = (unsigned)(ns.waves / 4);
+        const EnsArgs &a = S.a;
+        if (layers == 1) {
+            fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+            ens_nodes_vjp<<<blocks, 256, 0, s>>>(a, e0, nodes, S.node_n, S.node_off, S.ens_base,
+                                                 S.counter, scratch, ns.rows, B.staging, abar,
+                                                 backs, sbar, raw);
+        } else {
+            const int64_t per = ns.waves * 64;
+#define SCATRS_NODES_VJP(LL)                                                                    \
+    launch_nodes_vjp_ranked<LL>(blocks, per, s, a, e0, nodes, S, scratch, ns.rows, B, abar, backs, \
+                                sbar, raw)
+            switch (layers) {
+                case 2: SCATRS_NODES_VJP(2); break;
+                case 3: SCATRS_NODES_VJP(3); break;
+                case 4: SCATRS_NODES_VJP(4); break;
+                case 5: SCATRS_NODES_VJP(5); break;
+                case 6: SCATRS_NODES_VJP(6); break;
+                case 7: SCATRS_NODES_VJP(7); break;
+                default: SCATRS_NODES_VJP(8); break;
+            }
+        }
+#undef SCATRS_NODES_VJP
+        ens_node_grads<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(
+            a, e0, nodes, raw, B.vals, B.tabc, ht, sbar, fixed_range, gnode);
+        ens_grad_reduce<<<(unsigned)ne, 64, 0, s>>>(n_radius, P, gnode, d_grad + e0 * P);
+        if ((rc = check_hip(ctx, hipGetLastError(), "ensemble vjp launch"))) return rc;
+        e0 = e1;
+    }
+    return SCATRS_OK;
+}

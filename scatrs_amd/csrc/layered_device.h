@@ -28,18 +28,17 @@ struct ArgState {
 
 __device__ inline cx two_i(cx z) { return mk(-(z.im * 2.0), z.re * 2.0); }
 
+// Both forms are evaluated and one is selected per lane, so the lanes of a wavefront that
+// take different forms run one instruction stream (the unused form may overflow harmlessly).
 __device__ inline ArgState arg_new(cx z, bool chi) {
     ArgState a;
     a.z = z;
     a.inv_z = inv(z);
-    if (chi) {
-        const cx s = csin(z), c = ccos(z);
-        a.pi = s * c;
-        a.w = a.inv_z + s * inv(c);
-    } else {
-        a.pi = -(exp_m1(two_i(z)) * 0.5);
-        a.w = a.inv_z - mk(0.0, 1.0);
-    }
+    const cx s = csin(z), c = ccos(z);
+    const cx pi_chi = s * c, w_chi = a.inv_z + s * inv(c);
+    const cx pi_xi = -(exp_m1(two_i(z)) * 0.5), w_xi = a.inv_z - mk(0.0, 1.0);
+    a.pi = chi ? pi_chi : pi_xi;
+    a.w = chi ? w_chi : w_xi;
     a.w_n = real(0.0);
     return a;
 }
@@ -164,23 +163,19 @@ struct LayeredOuter {
             const cx z1 = m[l] * xl[l - 1], z2 = m[l] * xl[l];
             in[l - 1] = arg_new(z1, c);
             out[l - 1] = arg_new(z2, c);
-            if (c) {
-                const cx c2 = ccos(z2);
-                p[l - 1] = ((csin(z1) * inv(ccos(z1))) * c2) * c2;
-            } else {
-                const double dx = xl[l] - xl[l - 1];
-                const double ea = -(m[l].im * dx * 2.0), eb = m[l].re * dx * 2.0;
-                const cx e = mk(cos(eb), sin(eb)) * exp(ea);
-                p[l - 1] = -((e * exp_m1(two_i(z1))) * 0.5);
-            }
+            // chi and xi forms selected per lane, as in arg_new
+            const cx c1 = ccos(z1), c2 = ccos(z2);
+            const double dx = xl[l] - xl[l - 1];
+            const double ea = -(m[l].im * dx * 2.0), eb = m[l].re * dx * 2.0;
+            const cx e = mk(cos(eb), sin(eb)) * exp(ea);
+            const cx p_chi = ((csin(z1) * inv(c1)) * c2) * c2;
+            const cx p_xi = -((e * exp_m1(two_i(z1))) * 0.5);
+            p[l - 1] = c ? p_chi : p_xi;
             if constexpr (DERIVS) {
-                if (c) {
-                    sv[l - 1] = ccos(z2) * inv(ccos(z1));
-                } else {
-                    const double dx = xl[l] - xl[l - 1];
-                    const double b2 = m[l].re * dx;
-                    sv[l - 1] = mk(cos(b2), sin(b2)) * exp(-(m[l].im * dx));
-                }
+                const double b2 = m[l].re * dx;
+                const cx s_chi = c2 * inv(c1);
+                const cx s_xi = mk(cos(b2), sin(b2)) * exp(-(m[l].im * dx));
+                sv[l - 1] = c ? s_chi : s_xi;
             }
         }
         core_z = m[0] * xl[0];

@@ -214,7 +214,9 @@ __device__ __forceinline__ double absorption_term(cx f, cx df, cx l, double n2) 
 // The upward order loop (scatrs/src/mie/kernel.rs:325-559) after ratio_chain and x_chain.
 // Calls sink.order(k, a, b) (or sink.order_d with the four derivatives) for k = 1..n and
 // sink.done(abs_sum, jac). With a layered Outer source the per-order sides replace D_n(mx);
-// the sink then also receives sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md).
+// with derivatives the sink then also receives, after order_d of the same order,
+// sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md, outer), outer holding that order's
+// inner-layer sensitivities.
 template <bool DERIVS, class ZSrc, class Outer, class Sink>
 __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs,
                                            const double *ux, int stride, Outer &outer,
@@ -354,7 +356,6 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
                 db_dm = -(times_i(fb_m) * idb2);
                 da_dx = ((times_i(m2_minus_1) * (hh_a * hh_a + kk1 * inv_x * inv_x)) * inv_m2) * ida2;
                 db_dx = times_i(m2_minus_1) * idb2;
-                sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md);
             }
             const cx la_x = g_a * (dpa - k_x2 - big_a * k_x + 1.0) + h_a * d_m;
             const cx lb_x = g_b * (m2 * dpb - k_x2 - big_b * k_x + 1.0) + h_b * md;
@@ -366,6 +367,7 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             jac2 += weight * (absorption_term(d_m, times_i(fa_m), times_i(la_m), na) +
                               absorption_term(md, times_i(fb_m), times_i(lb_m), nb));
             sink.order_d(k, an, bn, da_dx, da_dm, db_dx, db_dm);
+            if constexpr (Outer::kLayered) sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md, outer);
         }
     };
     if constexpr (ZSrc::kBlocked) {
@@ -446,7 +448,8 @@ struct EfficiencySink {
         order(k, a, b);
     }
 
-    __device__ void stored(int, cx, cx, cx, cx, cx, cx) {}
+    template <class O>
+    __device__ void stored(int, cx, cx, cx, cx, cx, cx, const O &) {}
 
     // A lossless layered sphere absorbs nothing, and stays lossless as x or Re m changes
     // (layered/kernel.rs:453-464).

@@ -18,7 +18,7 @@ __host__ __device__ __forceinline__ int bucket_of(int n) {
 }
 
 // Exclusive scan of kBuckets counts in one block of 1024 threads (4 buckets each).
-__global__ void __launch_bounds__(1024) scan_buckets(const unsigned *hist, unsigned *offsets) {
+static __global__ void __launch_bounds__(1024) scan_buckets(const unsigned *hist, unsigned *offsets) {
     __shared__ unsigned part[1024];
     const int t = threadIdx.x;
     unsigned v[4];
@@ -49,7 +49,7 @@ constexpr int kSortPerThread = 4;
 constexpr int kSortBlock = 256 * kSortPerThread;
 
 // Block-local histogram in LDS, then one global atomic per non-empty bucket of the block.
-__device__ inline void block_histogram(const int *keys_local, int nkeys, unsigned *lcount,
+static __device__ inline void block_histogram(const int *keys_local, int nkeys, unsigned *lcount,
                                        unsigned *hist) {
     for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) lcount[b] = 0u;
     __syncthreads();
@@ -60,7 +60,7 @@ __device__ inline void block_histogram(const int *keys_local, int nkeys, unsigne
 }
 
 // Histogram of keys[0, len) (keys already written).
-__global__ void __launch_bounds__(256) histogram_keys(const int *keys, int64_t len,
+static __global__ void __launch_bounds__(256) histogram_keys(const int *keys, int64_t len,
                                                       unsigned *hist) {
     __shared__ unsigned lcount[kBuckets];
     int local[kSortPerThread];
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) histogram_keys(const int *keys, int64_t l
 
 // Each block ranks its elements per bucket in LDS, reserves one range per non-empty bucket
 // from the global offsets, and writes the permutation.
-__global__ void __launch_bounds__(256) scatter_by_key(const int *keys, unsigned *offsets,
+static __global__ void __launch_bounds__(256) scatter_by_key(const int *keys, unsigned *offsets,
                                                       int *perm, int64_t len) {
     __shared__ unsigned lcount[kBuckets];
     __shared__ unsigned lbase[kBuckets];
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) scatter_by_key(const int *keys, unsigned 
 }
 
 // hist has 2 * kBuckets entries: the counts, then room for the offsets.
-inline void sort_by_keys(const int *keys, unsigned *hist, int *perm, int64_t len,
+static inline void sort_by_keys(const int *keys, unsigned *hist, int *perm, int64_t len,
                          hipStream_t s) {
     unsigned *offsets = hist + kBuckets;
     scan_buckets<<<1, 1024, 0, s>>>(hist, offsets);

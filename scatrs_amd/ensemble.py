@@ -22,7 +22,8 @@ from .particles import LayeredSphere, Sphere
 from .size import Gamma, LogNormal, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
-           "ensemble_fixed_device"]
+           "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
+           "parameter_names"]
 
 SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
            "effective_radius", "mean_area", "mean_volume")
@@ -61,6 +62,126 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, n_radi
         C.byref(terms))
     ctx.check(rc)
     return scalars, greek, phase, terms.value
+
+
+def parameter_names(layers: int) -> list[str]:
+    """Names of the gradient columns of ensemble_fixed_vjp_device, in the reference's
+    parameter vocabulary (scatrs/src/ensemble/mod.rs:182-250): the distribution's two
+    parameters, then Re m and Im m per layer (core first), then the radius fractions."""
+    names = ["p0", "p1"]
+    if layers == 1:
+        return names + ["m_real", "m_imag"]
+    for l in range(layers):
+        names += [f"layer{l}.m_real", f"layer{l}.m_imag"]
+    return names + [f"layer{l}.radius_fraction" for l in range(layers - 1)]
+
+
+def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, bar_scalars=None,
+                              bar_greek=None, bar_phase=None, n_radius: int = 128,
+                              n_angles: int = 128, order: int | None = 32,
+                              normalized: bool = True, fixed_range: bool = False,
+                              allow_unvalidated: bool = False):
+    """Reverse mode of ensemble_fixed_device: with the output cotangents bar_scalars
+    (count, 11), bar_greek (count, 6, order+1) and bar_phase (count, 6, n_angles) (each may be
+    None), the gradient (count, P) with respect to parameter_names(L), and the total Mie
+    terms of one pass. fixed_range holds the ln r range at its value (the reference's
+    derivative definition); otherwise the radius nodes move with (p0, p1)."""
+    import torch
+
+    dev = p0.device
+    count = p0.numel()
+    layers = 1 if m.dim() == 1 else m.shape[1]
+    if layers > 1 and (fractions is None or fractions.shape != (count, layers - 1)):
+        raise InvalidInputError("layered spheres need fractions of shape (count, L - 1)")
+    if bar_greek is not None and (order is None or bar_greek.shape != (count, 6, order + 1)):
+        raise InvalidInputError("bar_greek must have shape (count, 6, order + 1)")
+    if bar_scalars is not None and bar_scalars.shape != (count, 11):
+        raise InvalidInputError("bar_scalars must have shape (count, 11)")
+    if bar_phase is not None and bar_phase.shape != (count, 6, n_angles):
+        raise InvalidInputError("bar_phase must have shape (count, 6, n_angles)")
+    c = lambda t: None if t is None else t.to(torch.float64).contiguous()  # noqa: E731
+    args = [kind.to(torch.int32).contiguous(), p0.contiguous(), p1.contiguous(),
+            wavelength.contiguous(), m.contiguous()]
+    fr = None if layers == 1 else fractions.contiguous()
+    bs, bg, bp = c(bar_scalars), c(bar_greek), c(bar_phase)
+    P = 2 + 2 * layers + (layers - 1)
+    grad = torch.empty((count, P), dtype=torch.float64, device=dev)
+    ctx = _native.context(dev.index)
+    terms = C.c_int64(0)
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+    rc = ctx.lib.scatrs_ensemble_fixed_vjp(
+        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(args[3]), layers, p(fr),
+        p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
+        int(bool(normalized)), int(bool(fixed_range)), int(bool(allow_unvalidated)), p(bs),
+        p(bg), p(bp), p(grad), C.byref(terms))
+    ctx.check(rc)
+    return grad, terms.value
+
+
+def _autograd_function():
+    import torch
+
+    class EnsembleFixedFunction(torch.autograd.Function):
+        """(p0, p1, m_real, m_imag, fractions) -> (scalars, greek, phase) on the fixed path;
+        backward is the library's reverse-mode kernel (one VJP pass per backward)."""
+
+        @staticmethod
+        def forward(ctx, p0, p1, m_real, m_imag, fractions, kind, wavelength, opts):
+            ctx.set_materialize_grads(False)  # unused outputs reach backward as None
+            m = torch.complex(m_real, m_imag)
+            if m.dim() == 2 and m.shape[1] == 1:
+                m = m[:, 0]
+            sc, gr, ph, _ = ensemble_fixed_device(
+                kind, p0, p1, wavelength, m, fractions, n_radius=opts["n_radius"],
+                n_angles=opts["n_angles"], order=opts["order"], normalized=opts["normalized"],
+                allow_unvalidated=opts["allow_unvalidated"], want_phase=True)
+            ctx.save_for_backward(p0, p1, m_real, m_imag,
+                                  fractions if fractions is not None else torch.empty(0),
+                                  kind, wavelength)
+            ctx.opts = opts
+            ctx.has_fr = fractions is not None
+            return sc, gr, ph
+
+        @staticmethod
+        def backward(ctx, g_sc, g_gr, g_ph):
+            p0, p1, m_real, m_imag, fr, kind, wl = ctx.saved_tensors
+            o = ctx.opts
+            m = torch.complex(m_real, m_imag)
+            layers = 1 if m.dim() == 1 else m.shape[1]
+            if m.dim() == 2 and layers == 1:
+                m = m[:, 0]
+            grad, _ = ensemble_fixed_vjp_device(
+                kind, p0, p1, wl, m, fr if ctx.has_fr else None, bar_scalars=g_sc,
+                bar_greek=g_gr, bar_phase=g_ph, n_radius=o["n_radius"], n_angles=o["n_angles"],
+                order=o["order"], normalized=o["normalized"], fixed_range=o["fixed_range"],
+                allow_unvalidated=o["allow_unvalidated"])
+            gm_re = grad[:, 2:2 + 2 * layers:2].reshape(m_real.shape)
+            gm_im = grad[:, 3:3 + 2 * layers:2].reshape(m_imag.shape)
+            gfr = grad[:, 2 + 2 * layers:] if ctx.has_fr else None
+            return grad[:, 0], grad[:, 1], gm_re, gm_im, gfr, None, None, None
+
+    return EnsembleFixedFunction
+
+
+_FUNCTION = None
+
+
+def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *,
+                   n_radius: int = 128, n_angles: int = 128, order: int = 32,
+                   normalized: bool = True, fixed_range: bool = False,
+                   allow_unvalidated: bool = False):
+    """Differentiable batch of ensembles on the fixed path (torch.autograd): p0, p1,
+    wavelength (count,); m_real, m_imag (count,) for spheres or (count, L) for layered spheres
+    (core first); fractions (count, L-1). Returns (scalars (count, 11), greek
+    (count, 6, order+1), phase (count, 6, n_angles)); gradients flow to p0, p1, m_real,
+    m_imag and fractions through the library's reverse-mode kernel."""
+    global _FUNCTION
+    if _FUNCTION is None:
+        _FUNCTION = _autograd_function()
+    opts = dict(n_radius=int(n_radius), n_angles=int(n_angles), order=int(order),
+                normalized=bool(normalized), fixed_range=bool(fixed_range),
+                allow_unvalidated=bool(allow_unvalidated))
+    return _FUNCTION.apply(p0, p1, m_real, m_imag, fractions, kind, wavelength, opts)
 
 
 @dataclass(frozen=True, eq=False)

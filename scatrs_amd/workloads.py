@@ -11,6 +11,9 @@ Workloads:
   * ``lognormals``: 4096 lognormals (r_g log-uniform on [0.01, 5] um, sigma_g uniform on
     [1.2, 2.5], m as for ``spheres``) at 64 wavelengths log-spaced over 0.3-2.5 um, 512 radius
     nodes, 1024 angles, Greek to 256;
+  * ``adjoint``: the first 1024 distributions of ``lognormals`` at every other of its
+    wavelengths (32), with a seeded loss sum w [(ssa - t)^2 + (g - t)^2 +
+    sum_{l<64} (alpha1_l - t_l)^2] for the reverse-mode path;
   * ``layered``: gamma distributions (r_eff log-uniform on [0.05, 3] um, v_eff uniform on
     [0.05, 0.4]) of a core and 1-4 shells (fractions from Dirichlet(2) cumulative sums,
     Re m uniform on [1.3, 2.0], Im m log-uniform on [1e-6, 1]) at 32 wavelengths.
@@ -52,6 +55,39 @@ def lognormals(n_dist: int = 4096, n_wl: int = 64, seed: int = SEED):
     kind = np.zeros(n_dist * n_wl, np.int32)
     return (kind, np.repeat(rg, n_wl), np.repeat(sg, n_wl), np.tile(wl, n_dist),
             np.repeat(m, n_wl))
+
+
+def adjoint(n_dist: int = 1024, n_wl: int = 32, seed: int = SEED):
+    """The first n_dist distributions of ``lognormals`` at n_wl of its 64 wavelengths (every
+    64 // n_wl-th), distribution-major: (kind, p0, p1, wavelength, m) and the loss's seeded
+    targets and weights {"ssa", "g", "alpha1" (count, 64), "w"}. The alpha1 targets are a
+    Henyey-Greenstein phase function's, (2l + 1) t_g^l."""
+    kind, p0, p1, wl, m = lognormals(4096, 64, seed)
+    step = 64 // n_wl
+    sel = (np.arange(n_dist)[:, None] * 64 + np.arange(0, 64, step)[None, :n_wl]).reshape(-1)
+    rng = np.random.default_rng(seed + 3)
+    cnt = sel.size
+    t_g = rng.uniform(0.3, 0.9, cnt)
+    targets = {"ssa": rng.uniform(0.5, 1.0, cnt), "g": t_g,
+               "alpha1": (2.0 * np.arange(64) + 1.0)[None, :] * t_g[:, None] ** np.arange(64),
+               "w": rng.uniform(0.5, 1.5, cnt)}
+    return (kind[sel], p0[sel], p1[sel], wl[sel], m[sel]), targets
+
+
+def adjoint_loss(scalars, greek, targets):
+    """The loss of ``adjoint`` and its cotangents (bar_scalars (count, 11), bar_greek
+    (count, 6, order + 1)) from the outputs; numpy or torch arrays alike."""
+    w = targets["w"]
+    d_ssa = scalars[:, 4] - targets["ssa"]
+    d_g = scalars[:, 5] - targets["g"]
+    d_a = greek[:, 0, :64] - targets["alpha1"]
+    loss = (w * (d_ssa ** 2 + d_g ** 2 + (d_a ** 2).sum(1))).sum()
+    bar_s = 0.0 * scalars
+    bar_g = 0.0 * greek
+    bar_s[:, 4] = 2.0 * w * d_ssa
+    bar_s[:, 5] = 2.0 * w * d_g
+    bar_g[:, 0, :64] = 2.0 * w[:, None] * d_a
+    return loss, bar_s, bar_g
 
 
 def layered(n_dist: int = 2048, n_wl: int = 32, layers: int | None = None, seed: int = SEED):
