@@ -4,11 +4,13 @@
 ``python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload spheres|lognormals]``
 
 A step is one pass of the hot path over one batch of synthetic input already resident in
-HBM. The default workload is 2^20 independent spheres (x log-uniform on [0.1, 1000],
-Re m uniform on [1.30, 1.70], Im m log-uniform on [1e-8, 1e-1]); the metric counts
+HBM. The headline workload is 2^20 independent spheres (x log-uniform on [0.1, 1000],
+Re m uniform on [1.30, 1.70], Im m log-uniform on [1e-8, 1e-1]); `value` counts
 (particle, order) Mie terms per second with N = ceil(x + 6 x^(1/3) + 3) terms per sphere.
-Prints one JSON line with the roofline of the dominant kernel (timed with HIP events on the
-library's stream) and a CPU baseline (the FP64 oracle on the host's cores).
+The same line carries the ensembles/s of the ensemble workloads (4096 lognormals x 64
+wavelengths with the matrix-core and the plain FP64 angular kernels; the adjoint workload; the
+layered workload), each with its CPU baseline (the FP64 oracle on the host's cores), and the
+rooflines of the dominant kernels (timed with HIP events on the library's stream).
 """
 
 from __future__ import annotations
@@ -113,6 +115,187 @@ def bench_spheres(args, torch):
     }
 
 
+def _dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _timed(torch, ctx, fn, steps):
+    """Mean wall seconds per call of fn over `steps` calls (after a synchronize) and the library's
+    marked kernel regions over them."""
+    torch.cuda.synchronize()
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times()
+    ctx.set_profiling(False)
+    return (t1 - t0) / steps, kt, out
+
+
+def bench_ensembles(args, torch):
+    """4096 lognormals x 64 wavelengths, 512 radius nodes, 1024 angles, Greek to 256 and the
+    phase matrix: ensembles/s with the FP64 matrix-core angular kernel (fused) and with the
+    plain FP64 vector kernel, the angular kernel's roofline, and the oracle on a sample."""
+    from scatrs_amd import _native, workloads
+    from scatrs_amd.ensemble import ensemble_fixed_device
+
+    kind, p0, p1, wl, m = workloads.lognormals()
+    nr, na, order = 512, 1024, 256
+    h = (na + 1) // 2
+    ctx = _native.context(0)
+    t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
+    fn = lambda: ensemble_fixed_device(*t, n_radius=nr, n_angles=na, order=order)  # noqa: E731
+    log(f"ensembles: {kind.size} (4096 lognormals x 64 wavelengths), warm-up")
+    fn()
+    sec, kt, out = _timed(torch, ctx, fn, args.ens_steps)
+    terms = out[3]
+    node_ms = float(np.sum(kt[0::2])) / args.ens_steps
+    ang_ms = float(np.sum(kt[1::2])) / args.ens_steps
+    flops = 16.0 * h * terms  # two real GEMMs x 2 tables x 4 columns per (node, order, cosine)
+    log(f"fused: {sec:.3f} s/step, nodes {node_ms:.0f} ms, angular {ang_ms:.0f} ms")
+    res = {"workload": "4096 lognormals (r_g log-uniform [0.01, 5] um, sigma_g U[1.2, 2.5], m as "
+                       "the sphere batch) x 64 wavelengths log-spaced 0.3-2.5 um; 512 radius nodes, "
+                       "1024 angles, Greek to 256, F11..F44",
+           "ensembles": int(kind.size), "terms": int(terms),
+           "ensembles_per_s": kind.size / sec, "s_per_step": sec, "steps": args.ens_steps,
+           "node_ms_per_step": node_ms, "angular_ms_per_step": ang_ms}
+    ach = flops / (ang_ms * 1e-3) / 1e12
+    res["roofline_angular"] = {"bound": "mfma", "pipe": "fp64 matrix cores (v_mfma_f64_16x16x4f64)",
+                               "kernel": "ens_angular_mfma", "achieved": ach,
+                               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": ach / FP64_PEAK_TFLOPS, "flops_per_step": flops}
+    if not args.no_plain:
+        ctx.set_angular_kernel(1)
+        try:
+            sec_p, kt_p, _ = _timed(torch, ctx, fn, 1)
+        finally:
+            ctx.set_angular_kernel(0)
+        res["plain_fp64"] = {"ensembles_per_s": kind.size / sec_p, "s_per_step": sec_p,
+                             "angular_ms_per_step": float(np.sum(kt_p[1::2]))}
+        res["speedup_vs_plain"] = sec_p / sec
+        log(f"plain FP64: {sec_p:.3f} s/step, fused speedup {sec_p / sec:.2f}x")
+    if not args.no_check:
+        import oracle
+
+        idx = np.random.default_rng(2).choice(kind.size, 4, replace=False)
+        sc, gr = out[0].cpu().numpy(), out[1].cpu().numpy()
+        err_s, err_g = 0.0, 0.0
+        for e in idx:
+            ref = oracle.ensemble_fixed(0, p0[e], p1[e], wl[e], m[e], n_radius=nr, n_angles=na,
+                                        order=order)
+            rs = ref["scalars"]
+            k = [0, 1, 4, 5]  # c_ext, c_sca, ssa, g
+            err_s = max(err_s, float(np.max(np.abs(sc[e][k] - rs[k]) / np.abs(rs[k]))))
+            err_g = max(err_g, float(np.max(np.abs(gr[e][0] - ref["greek"][0])) /
+                                     np.max(np.abs(ref["greek"][0]))))
+        res["max_rel_err_vs_oracle"] = {"c_ext_c_sca_ssa_g": err_s, "alpha1": err_g,
+                                        "sample": "4 ensembles"}
+        log(f"vs oracle on 4 ensembles: scalars {err_s:.2e}, alpha1 {err_g:.2e}")
+    if not args.no_cpu_baseline:
+        import oracle
+
+        th = cpu_threads()
+        sel = np.arange(0, kind.size, kind.size // args.ens_cpu_sample)[:args.ens_cpu_sample]
+        c0 = time.perf_counter()
+        oracle.ensemble_batch(kind[sel], p0[sel], p1[sel], wl[sel], m[sel], nr, na, order,
+                              threads=th)
+        c1 = time.perf_counter()
+        res["cpu_baseline"] = {"value": sel.size / (c1 - c0), "unit": "ensembles/s",
+                               "cores": th, "kind": "port",
+                               "sample": f"{sel.size} of the ensembles, evenly spaced ({c1 - c0:.1f} s)"}
+        log(f"cpu baseline: {sel.size / (c1 - c0):.2f} ensembles/s on {th} threads")
+    return res
+
+
+def bench_adjoint(args, torch):
+    """The adjoint workload: 1024 of the lognormals x 32 wavelengths, forward plus the reverse
+    pass of a loss on ssa, g and alpha1[0:64] (one step = forward + VJP)."""
+    from scatrs_amd import _native, workloads
+    from scatrs_amd.ensemble import ensemble_fixed_device, ensemble_fixed_vjp_device
+
+    (kind, p0, p1, wl, m), tg = workloads.adjoint()
+    nr, na, order = 512, 1024, 256
+    ctx = _native.context(0)
+    t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
+    tt = {k: _dev(torch, v) for k, v in tg.items()}
+
+    def step():
+        sc, gr, _, _ = ensemble_fixed_device(*t, n_radius=nr, n_angles=na, order=order,
+                                             want_phase=False)
+        _, bs, bg = workloads.adjoint_loss(sc, gr, tt)
+        g, _ = ensemble_fixed_vjp_device(*t, bar_scalars=bs, bar_greek=bg, n_radius=nr,
+                                         n_angles=na, order=order)
+        return g
+
+    log(f"adjoint: {kind.size} ensembles (1024 lognormals x 32 wavelengths), warm-up")
+    step()
+    sec, kt, g = _timed(torch, ctx, step, args.ens_steps)
+    res = {"workload": "first 1024 of the 4096 lognormals x every other wavelength (32); "
+                       "512 radius nodes, 1024 angles, Greek to 256; loss sum w[(ssa-t)^2 + "
+                       "(g-t)^2 + sum_{l<64} (alpha1_l-t_l)^2]; step = forward + VJP w.r.t. "
+                       "r_g, sigma_g, Re m, Im m",
+           "ensembles": int(kind.size), "ensembles_per_s": kind.size / sec, "s_per_step": sec,
+           "finite": bool(torch.isfinite(g).all().item())}
+    log(f"adjoint: {sec:.3f} s per forward + VJP step")
+    if not args.no_cpu_baseline:
+        import oracle
+
+        n = args.adj_cpu_sample
+        sel = np.arange(0, kind.size, kind.size // n)[:n]
+        c0 = time.perf_counter()
+        for e in sel:
+            oracle.ensemble_fixed_jacobian(0, p0[e], p1[e], wl[e], m[e], None, nr, na, order)
+        c1 = time.perf_counter()
+        res["cpu_baseline"] = {"value": n / (c1 - c0), "unit": "ensembles/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"forward-mode Jacobian (all 4 parameters) of {n} evenly "
+                                         f"spaced ensembles, one thread ({c1 - c0:.1f} s)"}
+    return res
+
+
+def bench_layered(args, torch):
+    """Gamma distributions of a core and 1-4 shells (2048 x 32 wavelengths), 256 radius nodes,
+    512 angles, Greek to 128: forward + VJP for every layer and radius fraction."""
+    from scatrs_amd import _native, workloads
+    from scatrs_amd.ensemble import ensemble_fixed_device, ensemble_fixed_vjp_device
+
+    nr, na, order = 256, 512, 128
+    ctx = _native.context(0)
+    total_s, total_e, total_terms = 0.0, 0, 0
+    per_l = {}
+    for L, (kind, p0, p1, wl, fr, m) in sorted(workloads.layered().items()):
+        t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
+        tfr = _dev(torch, fr)
+        rng = np.random.default_rng(L)
+        bs = _dev(torch, rng.normal(size=(kind.size, 11)))
+        bg = _dev(torch, rng.normal(size=(kind.size, 6, order + 1)))
+
+        def step():
+            _, _, _, terms = ensemble_fixed_device(*t, tfr, n_radius=nr, n_angles=na, order=order,
+                                                   want_phase=False)
+            g, _ = ensemble_fixed_vjp_device(*t, tfr, bar_scalars=bs, bar_greek=bg, n_radius=nr,
+                                             n_angles=na, order=order)
+            return terms
+
+        step()
+        sec, _, terms = _timed(torch, ctx, step, 1)
+        per_l[str(L)] = {"ensembles": int(kind.size), "s": sec, "terms": int(terms)}
+        total_s += sec
+        total_e += kind.size
+        total_terms += terms
+        log(f"layered L={L}: {kind.size} ensembles, {sec:.3f} s forward + VJP")
+    return {"workload": "2048 gamma distributions (r_eff log-uniform [0.05, 3] um, v_eff U[0.05, "
+                        "0.4]) of a core and 1-4 shells (fractions Dirichlet(2), Re m U[1.3, 2.0], "
+                        "Im m log-uniform [1e-6, 1]) x 32 wavelengths; 256 radius nodes, 512 "
+                        "angles, Greek to 128; step = forward + VJP (all layer indices and "
+                        "radius fractions)",
+            "ensembles": total_e, "terms": int(total_terms), "s_per_step": total_s,
+            "ensembles_per_s": total_e / total_s, "per_layer_count": per_l}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +306,13 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--traffic", type=float, default=MIE_BATCH_TRAFFIC_BYTES,
                     help="HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc")
+    ap.add_argument("--no-ensembles", action="store_true",
+                    help="only the sphere batch (skip the ensemble, adjoint and layered lines)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip timing the plain FP64 angular kernel")
+    ap.add_argument("--ens-steps", type=int, default=1)
+    ap.add_argument("--ens-cpu-sample", type=int, default=32)
+    ap.add_argument("--adj-cpu-sample", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="first copy this many MiB device-to-device (a PMC byte-count reference)")
     args = ap.parse_args()
@@ -146,6 +336,10 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy calibration: {args.copy_calibration} MiB x 3")
     res = bench_spheres(args, torch)
+    if not args.no_ensembles:
+        res["ensembles"] = bench_ensembles(args, torch)
+        res["adjoint"] = bench_adjoint(args, torch)
+        res["layered"] = bench_layered(args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

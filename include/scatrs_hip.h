@@ -44,8 +44,11 @@ int scatrs_set_stream(scatrs_ctx *ctx, void *stream);
 const char *scatrs_last_error(const scatrs_ctx *ctx);
 
 /* Instrumentation, no reference counterpart: with profiling on, the library brackets the
- * dominant kernel of each call (the persistent solve of a sphere batch, the fused node kernel
- * of an ensemble batch) with HIP events on the context's stream, up to 1024 launches;
+ * dominant kernels of each call with HIP events on the context's stream (up to 1024 regions):
+ * a sphere batch, its persistent solve; a forward ensemble batch, per chunk of ensembles, the
+ * node solves and then the angular products (two regions); a reverse ensemble batch, per chunk,
+ * those two, then the reverse angular products (when Greek or phase cotangents are given) and
+ * the reverse node solves.
  * scatrs_kernel_times waits for them, writes up to `cap` durations in milliseconds to the host
  * array ms_out, clears the record and returns how many it wrote. set_profiling clears it too. */
 int scatrs_set_profiling(scatrs_ctx *ctx, int on);
@@ -79,8 +82,9 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
 /* The fixed discrete path of a batch of ensembles (one distribution at one wavelength each;
  * the reference's EnsembleResult granularity, scatrs/src/ensemble/mod.rs:508-529):
  *   n_radius Gauss-Legendre nodes in ln r over the distribution's fixed range
- *   (lognormal: ln r_g +- 5 ln sigma_g; gamma: ln r_eff - 8 s .. ln r_eff + 5 s,
- *   s = sqrt(v_eff / (1 - 2 v_eff))), weights w (t_hi - t_lo)/2 u(r) r;
+ *   (lognormal: ln r_g +- 5 ln sigma_g; gamma: ln r_eff - 8 s .. ln r_eff +
+ *   ln(1 + 37 v_eff + sqrt(74 v_eff)), s = sqrt(v_eff / (1 - 2 v_eff))), weights
+ *   w (t_hi - t_lo)/2 u(r) r;
  *   the phase matrix at the n_angles-point Gauss-Legendre cosines (2 <= n_angles <= 4096);
  *   Greek to `order` (d_greek == NULL: none).
  * Spheres: layers == 1 and d_m (count, 2) is each ensemble's index. Layered spheres:

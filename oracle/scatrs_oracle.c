@@ -1100,9 +1100,12 @@ double orc_density(int kind, double p0, double p1, double r, double grad[2]) {
 }
 
 /* The fixed radius range in ln r. Lognormal: ln r_g -+ 5 ln sigma_g. Gamma (a = r_eff,
- * b = v_eff): ln a - 8 s .. ln a + 5 s with s = sqrt(b / (1 - 2b)), the reference's log-width
- * of the gamma peak (distribution.rs:319-323); the lower side is longer for the left skew of
- * ln r under a gamma law. */
+ * b = v_eff): from ln a - 8 s, s = sqrt(b / (1 - 2b)) the reference's log-width of the gamma
+ * peak (distribution.rs:319-323), the lower side long for the left skew of ln r under a gamma
+ * law; to ln a + ln(1 + G b + sqrt(2 G b)), G = 37: the area-weighted law is Gamma(k = 1/b,
+ * scale a b) in r, whose upper tail beyond k + c scale units is at most
+ * exp(-(c - k ln(1 + c/k))) (Chernoff), below e^-G for c = G + sqrt(2 k G). */
+static const double GAMMA_TAIL = 37.0;
 int orc_radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi) {
     if (kind == ORC_LOGNORMAL) {
         if (!(p0 > 0.0 && isfinite(p0) && p1 > 1.0 && isfinite(p1))) return -1;
@@ -1116,7 +1119,7 @@ int orc_radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi)
         double s = sqrt(p1 / (1.0 - 2.0 * p1));
         double c = log(p0);
         *t_lo = c - 8.0 * s;
-        *t_hi = c + 5.0 * s;
+        *t_hi = c + log(1.0 + GAMMA_TAIL * p1 + sqrt(2.0 * GAMMA_TAIL * p1));
         return 0;
     }
     return -1;
@@ -1255,11 +1258,14 @@ static double range_derivatives(int kind, double p0, double p1, double dmid[2], 
     }
     double q = 1.0 - 2.0 * p1, s = sqrt(p1 / q);
     double ds = 0.5 / (s * q * q);
+    double rt = sqrt(2.0 * GAMMA_TAIL * p1);
+    double up = log(1.0 + GAMMA_TAIL * p1 + rt);
+    double dup = (GAMMA_TAIL + GAMMA_TAIL / rt) / (1.0 + GAMMA_TAIL * p1 + rt);
     dmid[0] = 1.0 / p0;
-    dmid[1] = -1.5 * ds;
+    dmid[1] = 0.5 * (dup - 8.0 * ds);
     dhalf[0] = 0.0;
-    dhalf[1] = 6.5 * ds;
-    return 6.5 * s;
+    dhalf[1] = 0.5 * (dup + 8.0 * ds);
+    return 0.5 * (up + 8.0 * s);
 }
 
 /* The derivatives of the phase elements from those of S1, S2 (mie/phase.rs:60-73). */

@@ -25,8 +25,14 @@ SCATRS_HD bool dist_valid(int kind, double p0, double p1) {
     return false;
 }
 
-// The fixed ln r range: lognormal ln r_g -+ 5 ln sigma_g; gamma ln a - 8 s .. ln a + 5 s with
-// s = sqrt(b / (1 - 2 b)), the reference's log-width of the gamma peak (distribution.rs:319-323).
+// Upper end of the gamma range: r_hi = a (1 + G b + sqrt(2 G b)). The area-weighted gamma law is
+// Gamma(1/b, scale a b) in r, whose upper tail beyond k + c (k = 1/b, in units of the scale) is
+// at most exp(-(c - k ln(1 + c/k))) (Chernoff); c = G + sqrt(2 k G) keeps that below e^-G.
+constexpr double kGammaTail = 37.0;
+
+// The fixed ln r range: lognormal ln r_g -+ 5 ln sigma_g; gamma from ln a - 8 s, with
+// s = sqrt(b / (1 - 2 b)) the reference's log-width of the gamma peak (distribution.rs:319-323),
+// to ln a + ln(1 + G b + sqrt(2 G b)), G = kGammaTail.
 SCATRS_HD void radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi) {
     if (kind == 0) {
         const double c = log(p0), h = 5.0 * log(p1);
@@ -36,7 +42,7 @@ SCATRS_HD void radius_range(int kind, double p0, double p1, double *t_lo, double
         const double s = sqrt(p1 / (1.0 - 2.0 * p1));
         const double c = log(p0);
         *t_lo = c - 8.0 * s;
-        *t_hi = c + 5.0 * s;
+        *t_hi = c + log(1.0 + kGammaTail * p1 + sqrt(2.0 * kGammaTail * p1));
     }
 }
 
@@ -54,11 +60,14 @@ SCATRS_HD double range_derivatives(int kind, double p0, double p1, double dmid[2
     const double q = 1.0 - 2.0 * p1;
     const double s = sqrt(p1 / q);
     const double ds = 0.5 / (s * q * q);  // d s / d b
+    const double rt = sqrt(2.0 * kGammaTail * p1);
+    const double up = log(1.0 + kGammaTail * p1 + rt);  // t_hi - ln a
+    const double dup = (kGammaTail + kGammaTail / rt) / (1.0 + kGammaTail * p1 + rt);
     dmid[0] = 1.0 / p0;
-    dmid[1] = -1.5 * ds;
+    dmid[1] = 0.5 * (dup - 8.0 * ds);
     dhalf[0] = 0.0;
-    dhalf[1] = 6.5 * ds;
-    return 6.5 * s;
+    dhalf[1] = 0.5 * (dup + 8.0 * ds);
+    return 0.5 * (up + 8.0 * s);
 }
 
 // d(ln u)/d(ln r) at fixed parameters.

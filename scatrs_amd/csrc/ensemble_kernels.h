@@ -960,6 +960,8 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         }
 #undef SCATRS_RANKED
     }
+    mark_end(ctx);
+    mark_begin(ctx);
     if (ctx->angular_kernel == 1) {
         const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
         ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,

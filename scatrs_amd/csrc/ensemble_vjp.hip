@@ -662,10 +662,12 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
             abar = static_cast<cx *>(ctx->buf[scatrs_ctx::kAdjStaging]);
             double *slabs = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjSlabs]);
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+            mark_begin(ctx);
             ens_angular_vjp<<<vjp_blocks, 512, 0, s>>>(e0, ne, n_radius, (int)n_angles, h,
                                                        S.node_n, S.node_off, S.ens_base,
                                                        B.staging, S.table, tbar, S.counter,
                                                        slabs, abar, ht);
+            mark_end(ctx);
         } else {
             fill_f64<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(ht, 0.0, nodes);
         }
@@ -673,6 +675,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
         const unsigned blocks = (unsigned)(ns.waves / 4);
         const EnsArgs &a = S.a;
+        mark_begin(ctx);
         if (layers == 1) {
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             ens_nodes_vjp<<<blocks, 256, 0, s>>>(a, e0, nodes, S.node_n, S.node_off, S.ens_base,
@@ -694,6 +697,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
             }
         }
 #undef SCATRS_NODES_VJP
+        mark_end(ctx);
         ens_node_grads<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(
             a, e0, nodes, raw, B.vals, B.tabc, ht, sbar, fixed_range, gnode);
         ens_grad_reduce<<<(unsigned)ne, 64, 0, s>>>(n_radius, P, gnode, d_grad + e0 * P);
