@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mie (particle·order) terms/s + ensembles/s, fused vs plain FP64 GPU at 1e-6 rel"
 FP64_PEAK_TFLOPS = 78.6
+FP16_PEAK_TFLOPS = 2500.0  # dense
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of mie_batch on this workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # two rocprofv3 --pmc runs, scaled by what the counters read for a 1 GiB device copy
@@ -149,13 +150,19 @@ def bench_ensembles(args, torch):
     t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
     fn = lambda: ensemble_fixed_device(*t, n_radius=nr, n_angles=na, order=order)  # noqa: E731
     log(f"ensembles: {kind.size} (4096 lognormals x 64 wavelengths), warm-up")
-    fn()
-    sec, kt, out = _timed(torch, ctx, fn, args.ens_steps)
+    # The fused path: fp16-split matrix-core angular products (kernel 2).
+    ctx.set_angular_kernel(2)
+    try:
+        fn()
+        sec, kt, out = _timed(torch, ctx, fn, args.ens_steps)
+    finally:
+        ctx.set_angular_kernel(0)
     terms = out[3]
     node_ms = float(np.sum(kt[0::2])) / args.ens_steps
     ang_ms = float(np.sum(kt[1::2])) / args.ens_steps
     flops = 16.0 * h * terms  # two real GEMMs x 2 tables x 4 columns per (node, order, cosine)
-    log(f"fused: {sec:.3f} s/step, nodes {node_ms:.0f} ms, angular {ang_ms:.0f} ms")
+    log(f"fused (fp16-split angular): {sec:.3f} s/step, nodes {node_ms:.0f} ms, angular "
+        f"{ang_ms:.0f} ms")
     res = {"workload": "4096 lognormals (r_g log-uniform [0.01, 5] um, sigma_g U[1.2, 2.5], m as "
                        "the sphere batch) x 64 wavelengths log-spaced 0.3-2.5 um; 512 radius nodes, "
                        "1024 angles, Greek to 256, F11..F44",
@@ -163,10 +170,27 @@ def bench_ensembles(args, torch):
            "ensembles_per_s": kind.size / sec, "s_per_step": sec, "steps": args.ens_steps,
            "node_ms_per_step": node_ms, "angular_ms_per_step": ang_ms}
     ach = flops / (ang_ms * 1e-3) / 1e12
-    res["roofline_angular"] = {"bound": "mfma", "pipe": "fp64 matrix cores (v_mfma_f64_16x16x4f64)",
-                               "kernel": "ens_angular_mfma", "achieved": ach,
-                               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                               "frac": ach / FP64_PEAK_TFLOPS, "flops_per_step": flops}
+    # Three fp16 products per FP64-equivalent product: the fp16 dense peak over 3.
+    peak16 = FP16_PEAK_TFLOPS / 3.0
+    res["roofline_angular"] = {"bound": "mfma",
+                               "pipe": "fp16 matrix cores (v_mfma_f32_16x16x32_f16), 3 products "
+                                       "per split product; peak = 2500/3 TFLOP/s",
+                               "kernel": "ens_angular_f16", "achieved": ach, "peak": peak16,
+                               "unit": "TFLOP/s", "frac": ach / peak16, "flops_per_step": flops}
+    ctx.set_angular_kernel(0)
+    sec64, kt64, out64 = _timed(torch, ctx, fn, 1)
+    ang64 = float(np.sum(kt64[1::2]))
+    res["fp64_matrix_cores"] = {"ensembles_per_s": kind.size / sec64, "s_per_step": sec64,
+                                "angular_ms_per_step": ang64,
+                                "angular_tflops": flops / (ang64 * 1e-3) / 1e12,
+                                "angular_frac_of_fp64_peak": flops / (ang64 * 1e-3) / 1e12 /
+                                FP64_PEAK_TFLOPS}
+    log(f"FP64 matrix cores: {sec64:.3f} s/step, angular {ang64:.0f} ms")
+    # alpha1 of every ensemble, fp16-split against FP64 matrix cores (on the device)
+    d_all = float(((out[1][:, 0] - out64[1][:, 0]).abs().amax(dim=1) /
+                   out64[1][:, 0].abs().amax(dim=1)).max().item())
+    del out64
+    log(f"alpha1, fp16-split vs FP64 matrix cores over all {kind.size} ensembles: {d_all:.2e}")
     if not args.no_plain:
         ctx.set_angular_kernel(1)
         try:
@@ -181,18 +205,20 @@ def bench_ensembles(args, torch):
         import oracle
 
         idx = np.random.default_rng(2).choice(kind.size, 4, replace=False)
-        sc, gr = out[0].cpu().numpy(), out[1].cpu().numpy()
+        ti = torch.as_tensor(idx, device=out[0].device)
+        sc, gr = out[0][ti].cpu().numpy(), out[1][ti].cpu().numpy()
         err_s, err_g = 0.0, 0.0
-        for e in idx:
+        for r, e in enumerate(idx):
             ref = oracle.ensemble_fixed(0, p0[e], p1[e], wl[e], m[e], n_radius=nr, n_angles=na,
                                         order=order)
             rs = ref["scalars"]
             k = [0, 1, 4, 5]  # c_ext, c_sca, ssa, g
-            err_s = max(err_s, float(np.max(np.abs(sc[e][k] - rs[k]) / np.abs(rs[k]))))
-            err_g = max(err_g, float(np.max(np.abs(gr[e][0] - ref["greek"][0])) /
+            err_s = max(err_s, float(np.max(np.abs(sc[r][k] - rs[k]) / np.abs(rs[k]))))
+            err_g = max(err_g, float(np.max(np.abs(gr[r][0] - ref["greek"][0])) /
                                      np.max(np.abs(ref["greek"][0]))))
         res["max_rel_err_vs_oracle"] = {"c_ext_c_sca_ssa_g": err_s, "alpha1": err_g,
-                                        "sample": "4 ensembles"}
+                                        "sample": "4 ensembles",
+                                        "alpha1_fp16_split_vs_fp64_all_ensembles": d_all}
         log(f"vs oracle on 4 ensembles: scalars {err_s:.2e}, alpha1 {err_g:.2e}")
     if not args.no_cpu_baseline:
         import oracle

@@ -56,7 +56,10 @@ int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap);
 
 /* Selects the angular-product kernel of scatrs_ensemble_fixed: 0 (default) the FP64
  * matrix-core GEMM kernel, 1 the plain FP64 vector kernel (same results to rounding; kept as
- * the baseline the matrix-core kernel is measured against). No reference counterpart. */
+ * the baseline the matrix-core kernels are measured against), 2 the fp16 matrix cores with
+ * operands split in two fp16 slices and fp64 accumulation per 32 orders (the phase tables to
+ * about 1e-7 of their largest element: Greek coefficients within ~1e-7 of max|alpha1|; the
+ * scalars do not depend on it). The reverse pass keeps FP64. No reference counterpart. */
 int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
 
 /* The default truncation N = ceil(x + 6 x^(1/3) + 3) (host; scatrs/src/mie/mod.rs:122-124). */

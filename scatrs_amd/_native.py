@@ -107,7 +107,8 @@ class Context:
         return [buf[i] for i in range(n)]
 
     def set_angular_kernel(self, kind: int) -> None:
-        """0: FP64 matrix cores (default); 1: the plain FP64 vector kernel."""
+        """0: FP64 matrix cores (default); 1: the plain FP64 vector kernel; 2: fp16 matrix
+        cores with two-slice split operands (phase tables to ~1e-7 of their largest element)."""
         self.check(self.lib.scatrs_set_angular_kernel(self.handle, int(kind)))
 
     def bind_stream(self, stream_ptr: int) -> None:

@@ -31,13 +31,14 @@ struct scatrs_ctx {
         kAdjEns,      // reverse mode: per-ensemble cotangents of the sums and tables
         kAdjSlabs,    // reverse mode: per-workgroup angle-by-column panels
         kOrder,       // ensembles: per-node sort keys and the nodes ranked by N, histogram
+        kTables16,    // ensembles: the w pi_n, w tau_n table split in fp16 slices
         kNumBufs
     };
     void *buf[kNumBufs] = {};
     size_t cap[kNumBufs] = {};
     void *host_pinned = nullptr;  // small pinned status block
     bool profiling = false;
-    int angular_kernel = 0;  // 0: FP64 MFMA (default), 1: FP64 vector (the plain path)
+    int angular_kernel = 0;  // 0: FP64 MFMA (default), 1: FP64 vector (plain), 2: fp16-split MFMA
     static constexpr int kMaxEvents = 1024;
     hipEvent_t ev_begin[kMaxEvents] = {}, ev_end[kMaxEvents] = {};
     int ev_count = 0;

@@ -547,6 +547,184 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
     }
 }
 
+// The angular products on the fp16 matrix cores (v_mfma_f32_16x16x32_f16) with operands split
+// in two fp16 slices: g = g_hi + 2^-11 g_lo and x = x_hi + 2^-11 x_lo, so that
+// G X = G_hi X_hi + 2^-11 (G_hi X_lo + G_lo X_hi) up to the dropped 2^-22 G_lo X_lo term. No
+// scaling is needed: the table entries are bounded by (2n + 1)/2 and |a_n|, |b_n| <= 1 for
+// passive particles, inside fp16's range. Each 32-order chunk's three products accumulate in
+// fp32 and are added into fp64 accumulators, so the fp32 sums never span more than 32 orders.
+// The table hi/lo slices of G1 and G2 are prepared once per call (pt_table_f16); the
+// coefficients are split while they are staged in LDS. Otherwise as ens_angular_mfma: a
+// workgroup is one ensemble and 128 non-negative cosines, groups of 16 nodes, the same
+// epilogue; v_mfma_f32_16x16x32_f16 leaves D[i][j] in lane j + 16 (i / 4), element i % 4.
+typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kF16Orders = 32;   // one v_mfma_f32_16x16x32_f16 K step
+constexpr int kF16Pitch = 40;    // LDS row of 32 orders padded to 40 halves (bank spread)
+constexpr double kF16Lo = 1.0 / 2048.0;
+
+// tab16[((s * h) + jj) * npad + n - 1], s = G1 hi, G1 lo, G2 hi, G2 lo; zero beyond n_max.
+static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int npad,
+                                    _Float16 *tab16) {
+    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)h * npad) return;
+    const int jj = (int)(idx / npad), k = (int)(idx % npad);  // k = n - 1
+    double g1 = 0.0, g2 = 0.0;
+    if (k < n_max) {
+        const double2 pt = table[(size_t)k * h + jj];
+        const bool odd = (k + 1) & 1;
+        g1 = odd ? pt.x : pt.y;
+        g2 = odd ? pt.y : pt.x;
+    }
+    const _Float16 g1h = (_Float16)g1, g2h = (_Float16)g2;
+    const _Float16 g1l = (_Float16)((g1 - (double)g1h) * 2048.0);
+    const _Float16 g2l = (_Float16)((g2 - (double)g2h) * 2048.0);
+    const size_t plane = (size_t)h * npad;
+    const size_t o = (size_t)jj * npad + k;
+    tab16[o] = g1h;
+    tab16[plane + o] = g1l;
+    tab16[2 * plane + o] = g2h;
+    tab16[3 * plane + o] = g2l;
+}
+
+static __global__ __launch_bounds__(512) void ens_angular_f16(
+    int64_t e0, int64_t n_radius, int n_angles, int h, int slices, int npad, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double *tabc,
+    const _Float16 *tab16, double *tab_out) {
+    __shared__ _Float16 lds_x[2][64][kF16Pitch];  // [hi, lo][column 4 q + c][order]
+    const int64_t eb = blockIdx.x / slices;
+    const int slice = blockIdx.x % slices;
+    const int64_t e = e0 + eb;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    const int comp = lane & 3, qin = (lane & 15) >> 2;
+    const int jj = slice * kSliceAngles + wave * 16 + row;  // this lane's A-operand cosine
+    const bool row_ok = jj < h;
+    const size_t plane = (size_t)h * npad;
+    const _Float16 *a_row = tab16 + (size_t)(row_ok ? jj : 0) * npad + 8 * kq;
+    const int64_t base = ens_base[e];
+    const int *nn = node_n + e * n_radius;
+    const int64_t *noff = node_off + e * n_radius;
+    const int64_t gbase = eb * n_radius;
+    double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][v]: rows 4 kq + v
+    for (int64_t g0 = 0; g0 < n_radius; g0 += kGroupNodes) {
+        int nmax = 0;
+        for (int q = 0; q < kGroupNodes; ++q) {
+            const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
+            nmax = nq > nmax ? nq : nmax;
+        }
+        double acc1[4][4], acc2[4][4];
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc1[tt][v] = acc2[tt][v] = 0.0;
+        for (int kc = 0; kc < nmax; kc += kF16Orders) {
+            __syncthreads();
+            {  // one (node, order) per thread: u, v by parity, split into hi and lo
+                const int q = t / kF16Orders, k = t % kF16Orders;
+                const int n = kc + k;  // n - 1
+                const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
+                cx va = real(0.0), vb = real(0.0);
+                if (n < nq) {
+                    const cx *c = staging + 2 * (base + noff[g0 + q]);
+                    va = c[n];
+                    vb = c[nq + n];
+                }
+                const bool odd = (n + 1) & 1;
+                const cx u = odd ? va : vb, w = odd ? vb : va;
+                const double xs[4] = {u.re, u.im, w.re, w.im};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const _Float16 hi = (_Float16)xs[c];
+                    lds_x[0][4 * q + c][k] = hi;
+                    lds_x[1][4 * q + c][k] = (_Float16)((xs[c] - (double)hi) * 2048.0);
+                }
+            }
+            __syncthreads();
+            mfma_h8 g1h, g1l, g2h, g2l;
+            if (row_ok) {
+                g1h = *reinterpret_cast<const mfma_h8 *>(a_row + kc);
+                g1l = *reinterpret_cast<const mfma_h8 *>(a_row + plane + kc);
+                g2h = *reinterpret_cast<const mfma_h8 *>(a_row + 2 * plane + kc);
+                g2l = *reinterpret_cast<const mfma_h8 *>(a_row + 3 * plane + kc);
+            } else {
+                g1h = g1l = g2h = g2l = mfma_h8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) {
+                const int col = 16 * tt + row;
+                const mfma_h8 xh = *reinterpret_cast<const mfma_h8 *>(&lds_x[0][col][8 * kq]);
+                const mfma_h8 xl = *reinterpret_cast<const mfma_h8 *>(&lds_x[1][col][8 * kq]);
+                const mfma_f4 z = {0.f, 0.f, 0.f, 0.f};
+                const mfma_f4 h1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g1h, xh, z, 0, 0, 0);
+                mfma_f4 m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g1h, xl, z, 0, 0, 0);
+                m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g1l, xh, m1, 0, 0, 0);
+                const mfma_f4 h2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g2h, xh, z, 0, 0, 0);
+                mfma_f4 m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g2h, xl, z, 0, 0, 0);
+                m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g2l, xh, m2, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    acc1[tt][v] += (double)h1[v] + (double)m1[v] * kF16Lo;
+                    acc2[tt][v] += (double)h2[v] + (double)m2[v] * kF16Lo;
+                }
+            }
+        }
+        // Epilogue of the group: the lane's column is node 4 tt + qin, component comp.
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            const int64_t node = g0 + 4 * tt + qin;
+            const double cfac = node < n_radius ? tabc[gbase + node] : 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double c1[4], c2[4];
+                const double x1 = acc1[tt][v], x2 = acc2[tt][v];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
+                    c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
+                }
+                const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
+                const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
+#pragma unroll
+                for (int sgn = 0; sgn < 2; ++sgn) {
+                    const cx s1 = sgn == 0 ? A + B : A - B;
+                    const cx s2 = sgn == 0 ? C + D : C - D;
+                    double f;
+                    if (comp == 0) f = 0.5 * (norm2(s1) + norm2(s2));
+                    else if (comp == 1) f = 0.5 * (norm2(s2) - norm2(s1));
+                    else {
+                        const cx pp = s2 * conj(s1);
+                        f = comp == 2 ? pp.re : pp.im;
+                    }
+                    tp[sgn][v] += cfac * f;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            double x = tp[sgn][v];
+            x += __shfl_xor(x, 4, 64);
+            x += __shfl_xor(x, 8, 64);
+            tp[sgn][v] = x;
+        }
+    if (qin == 0) {
+        double *out = tab_out + eb * 4 * (int64_t)n_angles;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int jr = slice * kSliceAngles + wave * 16 + 4 * kq + v;  // output cosine
+            if (jr >= h) continue;
+            const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
+            out[comp * n_angles + jp] = tp[0][v];
+            if (jm != jp) out[comp * n_angles + jm] = tp[1][v];
+        }
+    }
+}
+
 // Averages and outputs of one ensemble (ensemble/engine.rs:1782-1836,2396-2497): scalars in
 // SCALARS order, Greek coefficients projected with the Wigner d table
 // dtab[(p * n_angles + j) * (order + 1) + l] (p = d00, d22, d2,-2, d02), the phase matrix.
@@ -738,6 +916,8 @@ struct EnsSetup {
     int n_max = 0;
     int64_t total_terms = 0;
     double2 *table = nullptr;
+    _Float16 *tab16 = nullptr;  // the fp16-split table (angular kernel 2)
+    int npad = 0;               // its rows' length: n_max rounded up to 32
 };
 
 static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, const double *d_p0,
@@ -814,6 +994,16 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
         return rc;
     S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
     pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
+    if (ctx->angular_kernel == 2) {
+        S.npad = (S.n_max + kF16Orders - 1) / kF16Orders * kF16Orders;
+        const size_t cells = (size_t)h * S.npad;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables16, 4 * cells * sizeof(_Float16)),
+                            "table")))
+            return rc;
+        S.tab16 = static_cast<_Float16 *>(ctx->buf[scatrs_ctx::kTables16]);
+        pt_table_f16<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(S.table, h, S.n_max, S.npad,
+                                                                     S.tab16);
+    }
     if (greek) {
         const int64_t len = order + 1;
         std::vector<double> dt(4 * (size_t)n_angles * len);
@@ -967,6 +1157,11 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,
                                                 S.node_off, S.ens_base, B.staging, B.tabc,
                                                 S.table, B.tab_out);
+    } else if (ctx->angular_kernel == 2) {
+        const int slices = (h + kSliceAngles - 1) / kSliceAngles;
+        ens_angular_f16<<<(unsigned)(ne * slices), 512, 0, s>>>(
+            e0, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off, S.ens_base,
+            B.staging, B.tabc, S.tab16, B.tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
