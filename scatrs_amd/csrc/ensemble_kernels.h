@@ -6,8 +6,9 @@
 //   ens_prep     - inputs checked per node (distribution, wavelength, index, validated domain),
 //                  N per node, status;
 //   ens_offsets  - per ensemble, the exclusive scan of N over its nodes (staging offsets);
-//   ens_nodes    - persistent, one node per lane: ratio chains in the wavefront's scratch slab,
-//                  a_n, b_n to the staging buffer, the node's weighted scalars;
+//   ens_nodes_ranked - one node per lane over the chunk's nodes ranked by N, in launches whose
+//                  wavefront slabs are sized by the launch's largest N: ratio chains in the
+//                  slab, a_n, b_n to the staging buffer, the node's weighted scalars;
 //   ens_angular  - one workgroup per ensemble: S1, S2 at +-mu from the staged coefficients and
 //                  the w*pi_n, w*tau_n table, F11, F12, F33, F34 weighted and summed over the
 //                  nodes in registers (fixed node order: the sums do not depend on scheduling);
@@ -184,6 +185,17 @@ static __global__ void pt_table(const double *mu_pos, int h, int n_max, double2 
     }
 }
 
+// Rows of a sphere slab's checkpoint column: one ratio per kZBlock orders.
+__host__ __device__ inline int64_t sphere_ckpt_rows(int64_t rows) { return (rows - 2) / kZBlock + 2; }
+
+// Complex elements of one wavefront's scratch slab for nodes of at most rows - 2 terms, 64
+// lanes interleaved: spheres keep the ratio checkpoints and a real column of u_k(x); layered
+// spheres 2 L - 1 complex columns and a real one.
+__host__ __device__ inline int64_t slab_elems(int layers, int64_t rows) {
+    if (layers == 1) return 64 * (sphere_ckpt_rows(rows) + (rows + 1) / 2);
+    return rows * 64 * 2 * layers;
+}
+
 // Writes a node's coefficients to the staging buffer and accumulates its efficiency sums.
 struct StagingSink {
     cx *a_out, *b_out;
@@ -203,7 +215,8 @@ struct StagingSink {
 // One node's forward solve: node g of the chunk is ensemble e0 + g / n_radius, radius node
 // g % n_radius. Writes its coefficients to the staging buffer, its weighted per-particle
 // quantities to vals, its table factor to tabc and, if back_out is not null, its backscatter sum
-// sum (2n+1) (-1)^n (a_n - b_n). slab is the lane's column in its wavefront's scratch slab.
+// sum (2n+1) (-1)^n (a_n - b_n). slab is the lane's column in its wavefront's scratch slab of
+// `rows` rows (slab_elems); tile the lane's column of the wavefront's LDS tile (spheres).
 template <int L>
 __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64_t g,
                                              const int *node_n, const int64_t *node_off,
@@ -222,7 +235,7 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
         // Checkpoints in the first column, u_k(x) in the second, the block being used in the
         // wavefront's LDS tile.
-        double *ux = real_column(slab, rows * 64);
+        double *ux = real_column(slab, sphere_ckpt_rows(rows) * 64);
         const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
         x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
         NoOuter none;
@@ -248,52 +261,28 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     if (back_out) back_out[g] = sink.eff.back;
 }
 
-// Spheres: persistent, each wavefront takes 64 consecutive nodes of [0, chunk_nodes) at a time
-// from the counter.
-static __global__ __launch_bounds__(256) void ens_nodes(EnsArgs a, int64_t e0, int64_t chunk_nodes,
-                                                 const int *node_n, const int64_t *node_off,
-                                                 const int64_t *ens_base,
-                                                 unsigned long long *counter, cx *scratch,
-                                                 int64_t rows, cx *staging, double *vals,
-                                                 double *tabc, cx *back_out) {
-    __shared__ cx ztile[4][kZBlock * 64];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    cx *tile = &ztile[threadIdx.x >> 6][lane];
-    cx *slab = scratch + wave * rows * 64 * 2 + lane;
-    while (true) {
-        unsigned long long chunk = 0;
-        if (lane == 0) chunk = atomicAdd(counter, 1ull);
-        chunk = __shfl(chunk, 0, 64);
-        if ((int64_t)chunk * 64 >= chunk_nodes) break;
-        const int64_t g = (int64_t)chunk * 64 + lane;
-        if (g < chunk_nodes)
-            forward_node<1>(a, e0, g, node_n, node_off, ens_base, slab, rows, tile, staging, vals,
-                            tabc, back_out);
-    }
-}
-
-// Layered spheres: one launch covers the ranked nodes [start, start + 64 * waves), one group of
-// 64 per wavefront and no work loop. The layered solves keep enough state per lane that their
-// registers spill; run inside the persistent work loop, that solve has given wrong results, so
-// these launches carry no loop. order[] ranks the chunk's nodes by N (largest first), so the
-// 64 lanes of a wavefront solve nodes of about the same N.
+// One launch covers the ranked nodes [start, end), one group of 64 per wavefront and no work
+// loop; order[] ranks the chunk's nodes by N (largest first), so the 64 lanes of a wavefront
+// solve nodes of about the same N, and `rows` (the slab's) is set by the launch's first node.
+// The layered solves keep enough state per lane that their registers spill; run inside a
+// persistent work loop, that solve has given wrong results, so no launch carries one.
 template <int L>
-__global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, int64_t chunk_nodes,
-                                                        const int *order, int64_t start,
+__global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, const int *order,
+                                                        int64_t start, int64_t end,
                                                         const int *node_n,
                                                         const int64_t *node_off,
                                                         const int64_t *ens_base, cx *scratch,
                                                         int64_t rows, cx *staging, double *vals,
                                                         double *tabc, cx *back_out) {
+    __shared__ cx ztile[4][L == 1 ? kZBlock * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    constexpr int kCols = LayeredColumns<L>::kComplex;
-    cx *slab = scratch + wave * rows * 64 * (kCols + 1) + lane;
+    cx *slab = scratch + wave * slab_elems(L, rows) + lane;
+    cx *tile = L == 1 ? &ztile[threadIdx.x >> 6][lane] : nullptr;
     const int64_t idx = start + wave * 64 + lane;
-    if (idx < chunk_nodes)
-        forward_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, nullptr,
-                        staging, vals, tabc, back_out);
+    if (idx < end)
+        forward_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile, staging,
+                        vals, tabc, back_out);
 }
 
 // Sort keys of the chunk's nodes: bucket_of(N), N descending.
@@ -547,181 +536,286 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
     }
 }
 
-// The angular products on the fp16 matrix cores (v_mfma_f32_16x16x32_f16) with operands split
-// in two fp16 slices: g = g_hi + 2^-11 g_lo and x = x_hi + 2^-11 x_lo, so that
-// G X = G_hi X_hi + 2^-11 (G_hi X_lo + G_lo X_hi) up to the dropped 2^-22 G_lo X_lo term. No
-// scaling is needed: the table entries are bounded by (2n + 1)/2 and |a_n|, |b_n| <= 1 for
-// passive particles, inside fp16's range. Each 32-order chunk's three products accumulate in
-// fp32 and are added into fp64 accumulators, so the fp32 sums never span more than 32 orders.
-// The table hi/lo slices of G1 and G2 are prepared once per call (pt_table_f16); the
-// coefficients are split while they are staged in LDS. Otherwise as ens_angular_mfma: a
-// workgroup is one ensemble and 128 non-negative cosines, groups of 16 nodes, the same
-// epilogue; v_mfma_f32_16x16x32_f16 leaves D[i][j] in lane j + 16 (i / 4), element i % 4.
+// The angular products on the fp16 matrix cores (v_mfma_f32_16x16x32_f16) with both operands
+// split in S fp16 slices, g = g_0 + 2^-11 g_1 (+ 2^-22 g_2) and likewise x, and the slice
+// products summed by level: level 0 (g_0 x_0) in fp32 per 32-order chunk, added into fp64
+// after each chunk; level 1 (g_0 x_1 + g_1 x_0) and, with three slices, level 2
+// (g_0 x_2 + g_1 x_1 + g_2 x_0) in fp32 over the whole group of nodes, their rounding entering
+// scaled by 2^-11 and 2^-22, added at its end. Two slices leave errors of order 2^-22 of the
+// terms (the slices' own rounding and the dropped g_1 x_1); three, of order 2^-24 (fp32
+// rounding of the level-0 chunk sums). No scaling is needed: the table entries are bounded by
+// (2n + 1)/2 and |a_n|, |b_n| <= 1 for passive particles, inside fp16's range.
+// The coefficients are the A operand and the table the B operand, C^T = X^T G^T: row 4 q + c of
+// a group's X^T is node q's (u.re, u.im, v.re, v.im)[c] (four MFMA tiles of 4 nodes), a
+// wavefront's 16 columns its cosines. v_mfma_f32_16x16x32_f16 leaves D[i][j] in lane
+// j + 16 (i / 4), element i % 4, so each lane receives the four components of node 4 tt + lane/16
+// at its cosine from both tables, A = G1 u, C = G1 v, D = G2 u, B = G2 v, and forms
+// S1(+-mu) = A +- B, S2(+-mu) = C +- D and F11, F12, F33, F34 itself. The table slices are
+// prepared once per call (pt_table_f16), the coefficients split while they are staged in LDS.
+// A workgroup is one ensemble and 128 non-negative cosines, groups of 16 nodes as
+// ens_angular_mfma; each lane sums its nodes in group order and the four node lanes of a cosine
+// are added at the end, always in the same order.
 typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kF16Orders = 32;   // one v_mfma_f32_16x16x32_f16 K step
 constexpr int kF16Pitch = 40;    // LDS row of 32 orders padded to 40 halves (bank spread)
-constexpr double kF16Lo = 1.0 / 2048.0;
+constexpr double kF16Step = 2048.0;  // 2^11 between consecutive slices
 
-// tab16[((s * h) + jj) * npad + n - 1], s = G1 hi, G1 lo, G2 hi, G2 lo; zero beyond n_max.
+// r to fp16 through fp32: v_cvt_f32_f64, a clamp to fp16's finite range and v_cvt_f16_f32 (a
+// correctly rounded fp64 -> fp16 conversion has no instruction and expands to some 35; the
+// clamp keeps the compiler from merging the two conversions into that one). The rare double
+// rounding moves the slice by at most one fp16 ulp; the split stays exact, since each residual
+// r - slice is formed in fp64 and carried to the next slice.
+__device__ __forceinline__ _Float16 to_f16(double r) {
+    return (_Float16)__builtin_amdgcn_fmed3f((float)r, -65504.0f, 65504.0f);
+}
+
+// tab16[((t * S + s) * h + jj) * npad + n - 1]: slice s of table t (G1, G2); zero beyond n_max.
+template <int S>
 static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int npad,
                                     _Float16 *tab16) {
     const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (idx >= (int64_t)h * npad) return;
     const int jj = (int)(idx / npad), k = (int)(idx % npad);  // k = n - 1
-    double g1 = 0.0, g2 = 0.0;
+    double g[2] = {0.0, 0.0};
     if (k < n_max) {
         const double2 pt = table[(size_t)k * h + jj];
         const bool odd = (k + 1) & 1;
-        g1 = odd ? pt.x : pt.y;
-        g2 = odd ? pt.y : pt.x;
+        g[0] = odd ? pt.x : pt.y;
+        g[1] = odd ? pt.y : pt.x;
     }
-    const _Float16 g1h = (_Float16)g1, g2h = (_Float16)g2;
-    const _Float16 g1l = (_Float16)((g1 - (double)g1h) * 2048.0);
-    const _Float16 g2l = (_Float16)((g2 - (double)g2h) * 2048.0);
     const size_t plane = (size_t)h * npad;
     const size_t o = (size_t)jj * npad + k;
-    tab16[o] = g1h;
-    tab16[plane + o] = g1l;
-    tab16[2 * plane + o] = g2h;
-    tab16[3 * plane + o] = g2l;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        double r = g[t];
+#pragma unroll
+        for (int sl = 0; sl < S; ++sl) {
+            const _Float16 v = (_Float16)r;
+            tab16[(size_t)(t * S + sl) * plane + o] = v;
+            r = (r - (double)v) * kF16Step;
+        }
+    }
 }
 
+// The logical block of physical block b of nb. The dispatcher deals physical blocks round-robin
+// over the 8 XCDs; this numbering gives consecutive logical blocks to one XCD, so that the
+// cosine slices of an ensemble read its coefficients through the same L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+    const int64_t per = nb / 8, rem = nb % 8;
+    const int64_t x = b % 8, j = b / 8;
+    return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
+}
+
+// The loop runs over the items (group of 16 nodes, 32-order chunk) of the ensemble, one ahead:
+// while an item's products run, the next item's coefficients and table slices are in flight,
+// issued in that order and unconditionally (clamped to valid addresses, masked after arrival),
+// so that the wait for this item's table slices leaves them outstanding. The ensemble's node
+// list (N, staging offsets, table factors) and each group's largest N are copied to LDS at the
+// start, so that moving to the next group issues no global loads. Dynamic LDS:
+// f16_lds_bytes(n_radius); n_radius is at most kF16MaxNodes.
+constexpr int64_t kF16MaxNodes = 2048;  // node list within 33 KB of LDS
+
+__host__ __device__ inline size_t f16_lds_bytes(int64_t n_radius) {
+    return (size_t)n_radius * (sizeof(double) + 2 * sizeof(int)) +
+           (size_t)((n_radius + kGroupNodes - 1) / kGroupNodes) * sizeof(int);
+}
+
+template <int S>
 static __global__ __launch_bounds__(512) void ens_angular_f16(
-    int64_t e0, int64_t n_radius, int n_angles, int h, int slices, int npad, const int *node_n,
-    const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double *tabc,
-    const _Float16 *tab16, double *tab_out) {
-    __shared__ _Float16 lds_x[2][64][kF16Pitch];  // [hi, lo][column 4 q + c][order]
-    const int64_t eb = blockIdx.x / slices;
-    const int slice = blockIdx.x % slices;
+    int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, int slices, int npad,
+    const int *node_n, const int64_t *node_off, const int64_t *ens_base, const cx *staging,
+    const double *tabc, const _Float16 *tab16, double *tab_out) {
+    __shared__ _Float16 lds_x[S][64][kF16Pitch];  // [slice][row 4 q + c][order]
+    extern __shared__ double w_cf[];              // [n_radius] table factors, then:
+    int *w_n = reinterpret_cast<int *>(w_cf + n_radius);  // [n_radius] N
+    int *w_off = w_n + n_radius;                           // [n_radius] staging offsets
+    int *gmax = w_off + n_radius;                          // [groups] largest N
+    const int64_t lb = xcd_block(blockIdx.x, ne * slices);
+    const int64_t eb = lb / slices;
+    const int slice = (int)(lb % slices);
     const int64_t e = e0 + eb;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int row = lane & 15, kq = lane >> 4;
-    const int comp = lane & 3, qin = (lane & 15) >> 2;
-    const int jj = slice * kSliceAngles + wave * 16 + row;  // this lane's A-operand cosine
-    const bool row_ok = jj < h;
+    const int col = lane & 15, kq = lane >> 4;  // B column (cosine), order set; D node in tile
+    const int sq = t / kF16Orders, sk = t % kF16Orders;  // staging: node in group, order
+    const int jj = slice * kSliceAngles + wave * 16 + col;  // this lane's cosine
+    const bool col_ok = jj < h;
     const size_t plane = (size_t)h * npad;
-    const _Float16 *a_row = tab16 + (size_t)(row_ok ? jj : 0) * npad + 8 * kq;
-    const int64_t base = ens_base[e];
-    const int *nn = node_n + e * n_radius;
-    const int64_t *noff = node_off + e * n_radius;
-    const int64_t gbase = eb * n_radius;
-    double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][v]: rows 4 kq + v
-    for (int64_t g0 = 0; g0 < n_radius; g0 += kGroupNodes) {
-        int nmax = 0;
-        for (int q = 0; q < kGroupNodes; ++q) {
-            const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-            nmax = nq > nmax ? nq : nmax;
+    // Columns beyond h read row 0 of the table: finite values whose products are not written.
+    const _Float16 *b_col = tab16 + (size_t)(col_ok ? jj : 0) * npad + 8 * kq;
+    const cx *stg = staging + 2 * ens_base[e];
+    const int nr = (int)n_radius;
+    const int ngroups = (nr + kGroupNodes - 1) / kGroupNodes;
+    {
+        const int *nn = node_n + e * n_radius;
+        const int64_t *noff = node_off + e * n_radius;
+        const double *tc = tabc + eb * n_radius;
+        for (int i = t; i < ngroups; i += blockDim.x) gmax[i] = 0;
+        for (int i = t; i < nr; i += blockDim.x) {
+            w_cf[i] = tc[i];
+            w_n[i] = nn[i];
+            w_off[i] = (int)noff[i];
         }
-        double acc1[4][4], acc2[4][4];
+        __syncthreads();
+        for (int i = t; i < nr; i += blockDim.x) atomicMax(&gmax[i / kGroupNodes], w_n[i]);
+        __syncthreads();
+    }
+    // This thread's (node, order) of item (g, kc): its two coefficients, from addresses
+    // clamped into the ensemble's staged terms; *in says whether they are real.
+    auto fetch = [&](int g, int kc, cx &va, cx &vb, bool &in) {
+        const int node = g + sq;
+        const bool ok = node < nr;
+        const int nq = ok ? w_n[node] : 0;
+        const cx *cp = stg + 2 * (int64_t)(ok ? w_off[node] : 0);
+        const int n = kc + sk;  // n - 1
+        in = n < nq;
+        va = cp[in ? n : 0];
+        vb = cp[in ? nq + n : 0];
+    };
+    auto table = [&](int kc, mfma_h8 (&g)[2][S]) {
 #pragma unroll
-        for (int tt = 0; tt < 4; ++tt)
+        for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) acc1[tt][v] = acc2[tt][v] = 0.0;
-        for (int kc = 0; kc < nmax; kc += kF16Orders) {
-            __syncthreads();
-            {  // one (node, order) per thread: u, v by parity, split into hi and lo
-                const int q = t / kF16Orders, k = t % kF16Orders;
-                const int n = kc + k;  // n - 1
-                const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-                cx va = real(0.0), vb = real(0.0);
-                if (n < nq) {
-                    const cx *c = staging + 2 * (base + noff[g0 + q]);
-                    va = c[n];
-                    vb = c[nq + n];
-                }
-                const bool odd = (n + 1) & 1;
-                const cx u = odd ? va : vb, w = odd ? vb : va;
-                const double xs[4] = {u.re, u.im, w.re, w.im};
+            for (int sl = 0; sl < S; ++sl)
+                g[tb][sl] = *reinterpret_cast<const mfma_h8 *>(b_col + (size_t)(tb * S + sl) * plane + kc);
+    };
+
+    const mfma_f4 zero = {0.f, 0.f, 0.f, 0.f};
+    double T[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][2 F11, 2 F12, F33, F34]
+    double acc[4][2][4];                             // [tile][table][component]: level 0
+    mfma_f4 lv1[4][2], lv2[4][2];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const _Float16 hi = (_Float16)xs[c];
-                    lds_x[0][4 * q + c][k] = hi;
-                    lds_x[1][4 * q + c][k] = (_Float16)((xs[c] - (double)hi) * 2048.0);
-                }
-            }
-            __syncthreads();
-            mfma_h8 g1h, g1l, g2h, g2l;
-            if (row_ok) {
-                g1h = *reinterpret_cast<const mfma_h8 *>(a_row + kc);
-                g1l = *reinterpret_cast<const mfma_h8 *>(a_row + plane + kc);
-                g2h = *reinterpret_cast<const mfma_h8 *>(a_row + 2 * plane + kc);
-                g2l = *reinterpret_cast<const mfma_h8 *>(a_row + 3 * plane + kc);
-            } else {
-                g1h = g1l = g2h = g2l = mfma_h8{0, 0, 0, 0, 0, 0, 0, 0};
-            }
+    for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) {
-                const int col = 16 * tt + row;
-                const mfma_h8 xh = *reinterpret_cast<const mfma_h8 *>(&lds_x[0][col][8 * kq]);
-                const mfma_h8 xl = *reinterpret_cast<const mfma_h8 *>(&lds_x[1][col][8 * kq]);
-                const mfma_f4 z = {0.f, 0.f, 0.f, 0.f};
-                const mfma_f4 h1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g1h, xh, z, 0, 0, 0);
-                mfma_f4 m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g1h, xl, z, 0, 0, 0);
-                m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g1l, xh, m1, 0, 0, 0);
-                const mfma_f4 h2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g2h, xh, z, 0, 0, 0);
-                mfma_f4 m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g2h, xl, z, 0, 0, 0);
-                m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(g2l, xh, m2, 0, 0, 0);
+        for (int tb = 0; tb < 2; ++tb) {
+            lv1[tt][tb] = zero;
+            lv2[tt][tb] = zero;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    acc1[tt][v] += (double)h1[v] + (double)m1[v] * kF16Lo;
-                    acc2[tt][v] += (double)h2[v] + (double)m2[v] * kF16Lo;
+            for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
+        }
+    int g0 = 0, kc = 0, nmax = gmax[0];
+    cx va, vb;
+    bool in;
+    // One item: its coefficients to LDS, the next item's loads, its products with the table
+    // slices gc (loaded one item earlier) and, at the end of a group, the epilogue. The loop
+    // alternates two sets of table registers, so that no copy waits for a load in flight.
+    auto step = [&](mfma_h8 (&gc)[2][S], mfma_h8 (&gn)[2][S]) __attribute__((always_inline)) {
+        __syncthreads();  // every wavefront is done with the previous item's LDS
+        {  // the item's (node, order): u, v by parity, split into S slices
+            const bool odd = (kc + sk + 1) & 1;
+            const cx a = in ? va : real(0.0), b = in ? vb : real(0.0);
+            const cx u = odd ? a : b, w = odd ? b : a;
+            const double xs[4] = {u.re, u.im, w.re, w.im};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double r = xs[c];
+#pragma unroll
+                for (int sl = 0; sl < S; ++sl) {
+                    const _Float16 v = to_f16(r);
+                    lds_x[sl][4 * sq + c][sk] = v;
+                    r = (r - (double)(float)v) * kF16Step;
                 }
             }
         }
-        // Epilogue of the group: the lane's column is node 4 tt + qin, component comp.
+        __syncthreads();
+        // The next item; its loads are issued in every step (the last one's are unused).
+        const bool same = kc + kF16Orders < nmax;
+        const int g1 = same ? g0 : g0 + kGroupNodes;
+        const int kc1 = same ? kc + kF16Orders : 0;
+        const bool more = g1 < nr;
+        fetch(more ? g1 : 0, kc1, va, vb, in);
+        table(kc1, gn);
+        __builtin_amdgcn_sched_barrier(0);  // keep the products behind those loads
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
-            const int64_t node = g0 + 4 * tt + qin;
-            const double cfac = node < n_radius ? tabc[gbase + node] : 0.0;
+            mfma_h8 x[S];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double c1[4], c2[4];
-                const double x1 = acc1[tt][v], x2 = acc2[tt][v];
+            for (int sl = 0; sl < S; ++sl)
+                x[sl] = *reinterpret_cast<const mfma_h8 *>(&lds_x[sl][16 * tt + col][8 * kq]);
 #pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
-                    c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
+            for (int tb = 0; tb < 2; ++tb) {
+                const mfma_f4 h0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], zero, 0, 0, 0);
+                mfma_f4 m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][1], lv1[tt][tb], 0, 0, 0);
+                lv1[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[1], gc[tb][0], m1, 0, 0, 0);
+                if constexpr (S == 3) {
+                    mfma_f4 m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][2], lv2[tt][tb], 0, 0, 0);
+                    m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[1], gc[tb][1], m2, 0, 0, 0);
+                    lv2[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[2], gc[tb][0], m2, 0, 0, 0);
                 }
-                const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
-                const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)h0[v];
+            }
+        }
+        if (!same) {
+            // Epilogue of the group: node 4 tt + kq at cosine jj.
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) {
+                const int node = g0 + 4 * tt + kq;
+                const double cfac = node < nr ? w_cf[node] : 0.0;
+                double c[2][4];
+#pragma unroll
+                for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        double y = acc[tt][tb][v] + (double)lv1[tt][tb][v] * (1.0 / kF16Step);
+                        if constexpr (S == 3)
+                            y += (double)lv2[tt][tb][v] * (1.0 / (kF16Step * kF16Step));
+                        c[tb][v] = y;
+                        acc[tt][tb][v] = 0.0;
+                    }
+                lv1[tt][0] = lv1[tt][1] = zero;
+                lv2[tt][0] = lv2[tt][1] = zero;
+                const cx A = mk(c[0][0], c[0][1]), C = mk(c[0][2], c[0][3]);
+                const cx D = mk(c[1][0], c[1][1]), B = mk(c[1][2], c[1][3]);
 #pragma unroll
                 for (int sgn = 0; sgn < 2; ++sgn) {
                     const cx s1 = sgn == 0 ? A + B : A - B;
                     const cx s2 = sgn == 0 ? C + D : C - D;
-                    double f;
-                    if (comp == 0) f = 0.5 * (norm2(s1) + norm2(s2));
-                    else if (comp == 1) f = 0.5 * (norm2(s2) - norm2(s1));
-                    else {
-                        const cx pp = s2 * conj(s1);
-                        f = comp == 2 ? pp.re : pp.im;
-                    }
-                    tp[sgn][v] += cfac * f;
+                    const double n1 = norm2(s1), n2 = norm2(s2);
+                    const cx pp = s2 * conj(s1);
+                    T[sgn][0] += cfac * (n1 + n2);
+                    T[sgn][1] += cfac * (n2 - n1);
+                    T[sgn][2] += cfac * pp.re;
+                    T[sgn][3] += cfac * pp.im;
                 }
             }
         }
+        if (!more) return false;
+        if (!same) nmax = gmax[g1 / kGroupNodes];
+        g0 = g1;
+        kc = kc1;
+        return true;
+    };
+    fetch(0, 0, va, vb, in);
+    mfma_h8 gA[2][S], gB[2][S];
+    table(0, gA);
+    while (step(gA, gB) && step(gB, gA)) {
     }
+    // Add the four node lanes (kq = 0..3) of the cosine, in a fixed order.
 #pragma unroll
     for (int sgn = 0; sgn < 2; ++sgn)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double x = tp[sgn][v];
-            x += __shfl_xor(x, 4, 64);
-            x += __shfl_xor(x, 8, 64);
-            tp[sgn][v] = x;
+        for (int k = 0; k < 4; ++k) {
+            double x = T[sgn][k];
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+            T[sgn][k] = x;
         }
-    if (qin == 0) {
+    if (col_ok) {
+        // Lane kq writes component kq at +mu and -mu.
         double *out = tab_out + eb * 4 * (int64_t)n_angles;
+        const int jp = n_angles - h + jj, jm = n_angles - 1 - jp;
+        double vp = T[0][0], vm = T[1][0];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int jr = slice * kSliceAngles + wave * 16 + 4 * kq + v;  // output cosine
-            if (jr >= h) continue;
-            const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
-            out[comp * n_angles + jp] = tp[0][v];
-            if (jm != jp) out[comp * n_angles + jm] = tp[1][v];
+        for (int k = 1; k < 4; ++k) {
+            vp = kq == k ? T[0][k] : vp;
+            vm = kq == k ? T[1][k] : vm;
         }
+        const double f = kq < 2 ? 0.5 : 1.0;
+        out[kq * n_angles + jp] = f * vp;
+        if (jm != jp) out[kq * n_angles + jm] = f * vm;
     }
 }
 
@@ -994,15 +1088,20 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
         return rc;
     S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
     pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
-    if (ctx->angular_kernel == 2) {
+    if (ctx->angular_kernel >= 2 && n_radius <= kF16MaxNodes) {
+        const int sl = ctx->angular_kernel;  // fp16 slices
         S.npad = (S.n_max + kF16Orders - 1) / kF16Orders * kF16Orders;
         const size_t cells = (size_t)h * S.npad;
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables16, 4 * cells * sizeof(_Float16)),
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables16,
+                                        2 * sl * cells * sizeof(_Float16)),
                             "table")))
             return rc;
         S.tab16 = static_cast<_Float16 *>(ctx->buf[scatrs_ctx::kTables16]);
-        pt_table_f16<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(S.table, h, S.n_max, S.npad,
-                                                                     S.tab16);
+        const unsigned tb = (unsigned)((cells + 255) / 256);
+        if (sl == 2)
+            pt_table_f16<2><<<tb, 256, 0, s>>>(S.table, h, S.n_max, S.npad, S.tab16);
+        else
+            pt_table_f16<3><<<tb, 256, 0, s>>>(S.table, h, S.n_max, S.npad, S.tab16);
     }
     if (greek) {
         const int64_t len = order + 1;
@@ -1032,26 +1131,39 @@ static int64_t next_chunk(EnsSetup &S, int64_t e0, int64_t count, int64_t n_radi
     return e1;
 }
 
+// A launch of the node kernels: the ranked nodes [start, end), slabs of `rows` rows.
+struct NodeLaunch {
+    int64_t start, end, rows;
+};
+
 // Buffers of one forward chunk.
 struct ChunkBufs {
     cx *staging = nullptr;
     double *vals = nullptr, *tabc = nullptr, *tab_out = nullptr;
-    int *order = nullptr;  // layered spheres: the chunk's nodes ranked by N
+    int *order = nullptr;            // the chunk's nodes ranked by N, largest first
+    std::vector<NodeLaunch> plan;    // the node-kernel launches over that ranking
 };
 
+// Scratch budget of one node launch's slabs and its largest number of wavefronts.
+constexpr size_t kSlabBudget = (size_t)4 << 30;
+constexpr int64_t kMaxNodeWaves = 16384;
+
 template <int L>
-static void launch_nodes_ranked(unsigned blocks, int64_t per_launch, hipStream_t s,
-                                const EnsArgs &a, int64_t e0, int64_t nodes, const EnsSetup &S,
-                                cx *scratch, int64_t rows, const ChunkBufs &B, cx *back_out) {
-    for (int64_t start = 0; start < nodes; start += per_launch)
-        ens_nodes_ranked<L><<<blocks, 256, 0, s>>>(a, e0, nodes, B.order, start, S.node_n,
-                                                   S.node_off, S.ens_base, scratch, rows,
-                                                   B.staging, B.vals, B.tabc, back_out);
+static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, const EnsSetup &S,
+                                cx *scratch, const ChunkBufs &B, cx *back_out) {
+    for (const NodeLaunch &q : B.plan)
+        ens_nodes_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
+            a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch, q.rows,
+            B.staging, B.vals, B.tabc, back_out);
 }
 
-// Ranks the chunk's nodes by N, largest first, into B.order (layered spheres).
-static int rank_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t nodes,
-                      ChunkBufs &B) {
+// Ranks the chunk's nodes by N, largest first, into B.order, and cuts the ranking into the
+// launches of B.plan: consecutive ranges, each with slabs sized by its first (largest) node's N
+// and as many wavefronts as kSlabBudget holds (at most kMaxNodeWaves), so that a few nodes of
+// large N do not shrink the launches of the many small ones. *scratch_bytes receives the slab
+// bytes of the largest launch. Synchronizes: the histogram of N comes back to the host.
+static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t nodes,
+                      ChunkBufs &B, size_t *scratch_bytes) {
     hipStream_t s = ctx->stream;
     int rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kOrder,
@@ -1067,26 +1179,30 @@ static int rank_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
     histogram_keys<<<(unsigned)((nodes + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(keys, nodes,
                                                                                     hist);
     sort_by_keys(keys, hist, B.order, nodes, s);
+    std::vector<unsigned> h(kBuckets);
+    if ((rc = check_hip(ctx, hipMemcpyAsync(h.data(), hist, kBuckets * sizeof(unsigned),
+                                            hipMemcpyDeviceToHost, s),
+                        "histogram copy")))
+        return rc;
+    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "node ranking"))) return rc;
+    B.plan.clear();
+    *scratch_bytes = 0;
+    int64_t pos = 0, below = 0;  // below: ranked nodes in the buckets before b
+    int b = 0;
+    while (pos < nodes) {
+        while (b < kBuckets - 1 && below + h[b] <= pos) below += h[b++];
+        // Bucket 0 holds every N above kBuckets - 2: its rows come from the largest N.
+        const int64_t n = b == 0 ? S.n_max : std::max(kBuckets - 2 - b, 0);
+        const int64_t rows = n + 2;
+        const size_t per_wave = (size_t)slab_elems(S.a.layers, rows) * sizeof(cx);
+        const int64_t waves = std::min<int64_t>(
+            kMaxNodeWaves, std::max<int64_t>(4, (int64_t)(kSlabBudget / per_wave)));
+        const int64_t end = std::min<int64_t>(nodes, pos + waves * 64);
+        B.plan.push_back(NodeLaunch{pos, end, rows});
+        *scratch_bytes = std::max(*scratch_bytes, per_wave * (size_t)((end - pos + 255) / 256 * 4));
+        pos = end;
+    }
     return SCATRS_OK;
-}
-
-// Launch shape of the persistent node kernels over `nodes` nodes: the scratch slab rows, the
-// bytes of one wavefront's slab and the number of wavefronts (a multiple of 4).
-struct NodeShape {
-    int64_t rows = 0, waves = 0;
-    size_t per_wave = 0;
-};
-
-static NodeShape node_shape(const scatrs_ctx *ctx, const EnsSetup &S, int64_t nodes) {
-    NodeShape ns;
-    const int layers = S.a.layers;
-    ns.rows = S.n_max + 2;
-    const int ccols = layers == 1 ? 1 : 2 * layers - 1;
-    ns.per_wave = (size_t)ns.rows * 64 * (ccols + 1) * sizeof(cx);
-    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (nodes + 63) / 64);
-    waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((6ull << 30) / ns.per_wave)));
-    ns.waves = (waves + 3) / 4 * 4;
-    return ns;
 }
 
 // Runs the node solves and the angular products of the chunk [e0, e0 + ne) (chunk_terms staged
@@ -1100,8 +1216,8 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     hipStream_t s = ctx->stream;
     const int layers = S.a.layers, h = S.h;
     const int64_t nodes = ne * n_radius;
-    const NodeShape ns = node_shape(ctx, S, nodes);
-    const int64_t rows = ns.rows;
+    size_t scratch_bytes = 0;
+    if ((rc = plan_nodes(ctx, S, e0, nodes, B, &scratch_bytes))) return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kStaging,
                                     (size_t)std::max<int64_t>(chunk_terms, 1) * 2 * sizeof(cx)),
                         "staging")))
@@ -1113,8 +1229,7 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAccum, (size_t)ne * 4 * n_angles * sizeof(double)),
                         "tables")))
         return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, ns.per_wave * (size_t)ns.waves),
-                        "scratch")))
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, scratch_bytes), "scratch")))
         return rc;
     // ens_base holds the staging offsets of this chunk's ensembles, relative to e0's.
     if ((rc = check_hip(ctx, hipMemcpyAsync(S.ens_base + e0, S.base.data() + e0, ne * sizeof(int64_t),
@@ -1126,30 +1241,20 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     B.tabc = B.vals + nodes * kNodeVals;
     B.tab_out = static_cast<double *>(ctx->buf[scatrs_ctx::kAccum]);
     cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
-    fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
     mark_begin(ctx);
-    const unsigned blocks = (unsigned)(ns.waves / 4);
     const EnsArgs &a = S.a;
-    if (layers == 1) {
-        ens_nodes<<<blocks, 256, 0, s>>>(a, e0, nodes, S.node_n, S.node_off, S.ens_base,
-                                         S.counter, scratch, rows, B.staging, B.vals, B.tabc,
-                                         back_out);
-    } else {
-        if ((rc = rank_nodes(ctx, S, e0, nodes, B))) return rc;
-        const int64_t per = ns.waves * 64;
-#define SCATRS_RANKED(LL) \
-    launch_nodes_ranked<LL>(blocks, per, s, a, e0, nodes, S, scratch, rows, B, back_out)
-        switch (layers) {
-            case 2: SCATRS_RANKED(2); break;
-            case 3: SCATRS_RANKED(3); break;
-            case 4: SCATRS_RANKED(4); break;
-            case 5: SCATRS_RANKED(5); break;
-            case 6: SCATRS_RANKED(6); break;
-            case 7: SCATRS_RANKED(7); break;
-            default: SCATRS_RANKED(8); break;
-        }
-#undef SCATRS_RANKED
+#define SCATRS_RANKED(LL) launch_nodes_ranked<LL>(s, a, e0, S, scratch, B, back_out)
+    switch (layers) {
+        case 1: SCATRS_RANKED(1); break;
+        case 2: SCATRS_RANKED(2); break;
+        case 3: SCATRS_RANKED(3); break;
+        case 4: SCATRS_RANKED(4); break;
+        case 5: SCATRS_RANKED(5); break;
+        case 6: SCATRS_RANKED(6); break;
+        case 7: SCATRS_RANKED(7); break;
+        default: SCATRS_RANKED(8); break;
     }
+#undef SCATRS_RANKED
     mark_end(ctx);
     mark_begin(ctx);
     if (ctx->angular_kernel == 1) {
@@ -1157,11 +1262,17 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,
                                                 S.node_off, S.ens_base, B.staging, B.tabc,
                                                 S.table, B.tab_out);
-    } else if (ctx->angular_kernel == 2) {
+    } else if (ctx->angular_kernel >= 2 && n_radius <= kF16MaxNodes) {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
-        ens_angular_f16<<<(unsigned)(ne * slices), 512, 0, s>>>(
-            e0, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off, S.ens_base,
-            B.staging, B.tabc, S.tab16, B.tab_out);
+        const size_t gm = f16_lds_bytes(n_radius);
+        if (ctx->angular_kernel == 2)
+            ens_angular_f16<2><<<(unsigned)(ne * slices), 512, gm, s>>>(
+                e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
+                S.ens_base, B.staging, B.tabc, S.tab16, B.tab_out);
+        else
+            ens_angular_f16<3><<<(unsigned)(ne * slices), 512, gm, s>>>(
+                e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
+                S.ens_base, B.staging, B.tabc, S.tab16, B.tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(

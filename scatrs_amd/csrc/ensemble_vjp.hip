@@ -15,7 +15,8 @@
 //                      matrix cores, the cotangents of S1, S2 from Tbar, and the transposed
 //                      products back onto the orders: the table part of each node's a_n, b_n
 //                      cotangents, and the node's sum Tbar . F;
-//   ens_nodes_vjp    - per node, a_n, b_n and their derivatives recomputed (not stored): the
+//   ens_nodes_vjp_ranked - per node (the forward chunk's ranking and launches), a_n, b_n and
+//                      their derivatives recomputed (not stored): the
 //                      cotangents completed with the scalar outputs' part, contracted with the
 //                      coefficient derivatives (and, for C_abs, the absorption sum's closed-form
 //                      Jacobian);
@@ -448,7 +449,7 @@ __device__ __forceinline__ void vjp_node(const EnsArgs &a, int64_t e0, int64_t g
     sink.back = backs[g];
     if constexpr (L == 1) {
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
-        double *ux = real_column(slab, rows * 64);
+        double *ux = real_column(slab, sphere_ckpt_rows(rows) * 64);
         const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
         x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
         NoOuter none;
@@ -469,43 +470,21 @@ __device__ __forceinline__ void vjp_node(const EnsArgs &a, int64_t e0, int64_t g
     }
 }
 
-// Spheres: persistent, as ens_nodes.
-__global__ __launch_bounds__(256) void ens_nodes_vjp(
-    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *node_n, const int64_t *node_off,
-    const int64_t *ens_base, unsigned long long *counter, cx *scratch, int64_t rows,
-    const cx *staging, const cx *abar, const cx *backs, const double *sbar, double *raw) {
-    __shared__ cx ztile[4][kZBlock * 64];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    cx *tile = &ztile[threadIdx.x >> 6][lane];
-    cx *slab = scratch + wave * rows * 64 * 2 + lane;
-    while (true) {
-        unsigned long long chunk = 0;
-        if (lane == 0) chunk = atomicAdd(counter, 1ull);
-        chunk = __shfl(chunk, 0, 64);
-        if ((int64_t)chunk * 64 >= chunk_nodes) break;
-        const int64_t g = (int64_t)chunk * 64 + lane;
-        if (g < chunk_nodes)
-            vjp_node<1>(a, e0, g, node_n, node_off, ens_base, slab, rows, tile, staging, abar,
-                        backs, sbar, raw);
-    }
-}
-
-// Layered spheres: one group of 64 ranked nodes per wavefront, no work loop (as
-// ens_nodes_ranked, for the same reason).
+// One group of 64 ranked nodes per wavefront over [start, end), no work loop (as
+// ens_nodes_ranked, for the same reason, with the same launches).
 template <int L>
 __global__ __launch_bounds__(256) void ens_nodes_vjp_ranked(
-    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, int64_t start,
-    const int *node_n, const int64_t *node_off, const int64_t *ens_base, cx *scratch,
-    int64_t rows, const cx *staging, const cx *abar, const cx *backs, const double *sbar,
-    double *raw) {
+    EnsArgs a, int64_t e0, const int *order, int64_t start, int64_t end, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, cx *scratch, int64_t rows,
+    const cx *staging, const cx *abar, const cx *backs, const double *sbar, double *raw) {
+    __shared__ cx ztile[4][L == 1 ? kZBlock * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    constexpr int kCols = LayeredColumns<L>::kComplex;
-    cx *slab = scratch + wave * rows * 64 * (kCols + 1) + lane;
+    cx *slab = scratch + wave * slab_elems(L, rows) + lane;
+    cx *tile = L == 1 ? &ztile[threadIdx.x >> 6][lane] : nullptr;
     const int64_t idx = start + wave * 64 + lane;
-    if (idx < chunk_nodes)
-        vjp_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, nullptr, staging,
+    if (idx < end)
+        vjp_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile, staging,
                     abar, backs, sbar, raw);
 }
 
@@ -578,15 +557,14 @@ static __global__ void ens_grad_reduce(int64_t n_radius, int P, const double *gn
 }
 
 template <int L>
-static void launch_nodes_vjp_ranked(unsigned blocks, int64_t per_launch, hipStream_t s,
-                                    const EnsArgs &a, int64_t e0, int64_t nodes,
-                                    const EnsSetup &S, cx *scratch, int64_t rows,
-                                    const ChunkBufs &B, const cx *abar, const cx *backs,
-                                    const double *sbar, double *raw) {
-    for (int64_t start = 0; start < nodes; start += per_launch)
-        ens_nodes_vjp_ranked<L><<<blocks, 256, 0, s>>>(a, e0, nodes, B.order, start, S.node_n,
-                                                       S.node_off, S.ens_base, scratch, rows,
-                                                       B.staging, abar, backs, sbar, raw);
+static void launch_nodes_vjp_ranked(hipStream_t s, const EnsArgs &a, int64_t e0,
+                                    const EnsSetup &S, cx *scratch, const ChunkBufs &B,
+                                    const cx *abar, const cx *backs, const double *sbar,
+                                    double *raw) {
+    for (const NodeLaunch &q : B.plan)
+        ens_nodes_vjp_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
+            a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch, q.rows,
+            B.staging, abar, backs, sbar, raw);
 }
 
 }  // namespace scatrs
@@ -671,30 +649,21 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         } else {
             fill_f64<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(ht, 0.0, nodes);
         }
-        const NodeShape ns = node_shape(ctx, S, nodes);
+        // The forward chunk's ranking and launch plan, and its scratch, serve the reverse solves.
         cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
-        const unsigned blocks = (unsigned)(ns.waves / 4);
         const EnsArgs &a = S.a;
         mark_begin(ctx);
-        if (layers == 1) {
-            fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
-            ens_nodes_vjp<<<blocks, 256, 0, s>>>(a, e0, nodes, S.node_n, S.node_off, S.ens_base,
-                                                 S.counter, scratch, ns.rows, B.staging, abar,
-                                                 backs, sbar, raw);
-        } else {
-            const int64_t per = ns.waves * 64;
-#define SCATRS_NODES_VJP(LL)                                                                    \
-    launch_nodes_vjp_ranked<LL>(blocks, per, s, a, e0, nodes, S, scratch, ns.rows, B, abar, backs, \
-                                sbar, raw)
-            switch (layers) {
-                case 2: SCATRS_NODES_VJP(2); break;
-                case 3: SCATRS_NODES_VJP(3); break;
-                case 4: SCATRS_NODES_VJP(4); break;
-                case 5: SCATRS_NODES_VJP(5); break;
-                case 6: SCATRS_NODES_VJP(6); break;
-                case 7: SCATRS_NODES_VJP(7); break;
-                default: SCATRS_NODES_VJP(8); break;
-            }
+#define SCATRS_NODES_VJP(LL) \
+    launch_nodes_vjp_ranked<LL>(s, a, e0, S, scratch, B, abar, backs, sbar, raw)
+        switch (layers) {
+            case 1: SCATRS_NODES_VJP(1); break;
+            case 2: SCATRS_NODES_VJP(2); break;
+            case 3: SCATRS_NODES_VJP(3); break;
+            case 4: SCATRS_NODES_VJP(4); break;
+            case 5: SCATRS_NODES_VJP(5); break;
+            case 6: SCATRS_NODES_VJP(6); break;
+            case 7: SCATRS_NODES_VJP(7); break;
+            default: SCATRS_NODES_VJP(8); break;
         }
 #undef SCATRS_NODES_VJP
         mark_end(ctx);
