@@ -32,6 +32,7 @@ struct scatrs_ctx {
         kAdjSlabs,    // reverse mode: per-workgroup angle-by-column panels
         kOrder,       // ensembles: per-node sort keys and the nodes ranked by N, histogram
         kTables16,    // ensembles: the w pi_n, w tau_n table split in fp16 slices
+        kGreekFac,    // ensembles: per-ensemble factor of the Greek coefficients
         kNumBufs
     };
     void *buf[kNumBufs] = {};

@@ -30,7 +30,7 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B).
-    const int64_t max_terms = (int64_t)(((size_t)3 << 30) / (2 * sizeof(cx)));
+    const int64_t max_terms = (int64_t)(kStagingBytes / (2 * sizeof(cx)));
     int64_t e0 = 0;
     while (e0 < count) {
         int64_t chunk_terms = 0;
@@ -40,9 +40,22 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         if ((rc = forward_chunk(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, nullptr)))
             return rc;
         const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
+        double *gfac = nullptr;
+        if (d_greek) {
+            if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kGreekFac, ne * sizeof(double)),
+                                "alloc")))
+                return rc;
+            gfac = static_cast<double *>(ctx->buf[scatrs_ctx::kGreekFac]);
+        }
         ens_finish<<<(unsigned)ne, 256, shmem, s>>>(e0, n_radius, (int)n_angles, S.d_atab, B.vals,
                                                     B.tab_out, S.d_dtab, order, normalized,
-                                                    d_scalars, d_greek, d_phase);
+                                                    d_scalars, gfac, d_phase);
+        if (d_greek) {
+            const int64_t len = order + 1;
+            const dim3 grid((unsigned)((ne + 15) / 16), (unsigned)((len + 63) / 64));
+            ens_greek_mfma<<<grid, 256, 0, s>>>(e0, ne, (int)n_angles, S.d_atab, B.tab_out,
+                                                S.d_dtab, len, gfac, normalized, d_greek);
+        }
         if ((rc = check_hip(ctx, hipGetLastError(), "ensemble launch"))) return rc;
         e0 = e1;
     }

@@ -6,14 +6,17 @@
 //   ens_prep     - inputs checked per node (distribution, wavelength, index, validated domain),
 //                  N per node, status;
 //   ens_offsets  - per ensemble, the exclusive scan of N over its nodes (staging offsets);
-//   ens_nodes_ranked - one node per lane over the chunk's nodes ranked by N, in launches whose
-//                  wavefront slabs are sized by the launch's largest N: ratio chains in the
-//                  slab, a_n, b_n to the staging buffer, the node's weighted scalars;
+//   ens_nodes_persist / ens_nodes_ranked - one node per lane over the chunk's nodes ranked by
+//                  N, largest first (spheres: one persistent launch; layered spheres: launches
+//                  whose wavefront slabs are sized by the launch's largest N): ratio chains in
+//                  the slab, a_n, b_n to the staging buffer, the node's weighted scalars;
 //   ens_angular  - one workgroup per ensemble: S1, S2 at +-mu from the staged coefficients and
 //                  the w*pi_n, w*tau_n table, F11, F12, F33, F34 weighted and summed over the
 //                  nodes in registers (fixed node order: the sums do not depend on scheduling);
-//   ens_finish   - number averages, the scalar outputs, the Greek projection with the Wigner d
-//                  table, the phase matrix.
+//   ens_finish   - number averages, the scalar outputs, the phase matrix and the factor of the
+//                  Greek coefficients;
+//   ens_greek_mfma - the Greek projection with the Wigner d table as FP64 matrix-core GEMMs over
+//                  the chunk's ensembles.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -261,11 +264,34 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     if (back_out) back_out[g] = sink.eff.back;
 }
 
-// One launch covers the ranked nodes [start, end), one group of 64 per wavefront and no work
-// loop; order[] ranks the chunk's nodes by N (largest first), so the 64 lanes of a wavefront
-// solve nodes of about the same N, and `rows` (the slab's) is set by the launch's first node.
-// The layered solves keep enough state per lane that their registers spill; run inside a
-// persistent work loop, that solve has given wrong results, so no launch carries one.
+// Spheres: persistent; each wavefront takes the next 64 ranked nodes from the counter, so the
+// nodes of largest N start first and the rest fill in behind them, and the 64 lanes of a
+// wavefront solve nodes of about the same N. Slabs of `rows` rows (the chunk's largest N).
+static __global__ __launch_bounds__(256) void ens_nodes_persist(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, cx *scratch,
+    int64_t rows, cx *staging, double *vals, double *tabc, cx *back_out) {
+    __shared__ cx ztile[4][kZBlock * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *tile = &ztile[threadIdx.x >> 6][lane];
+    cx *slab = scratch + wave * slab_elems(1, rows) + lane;
+    while (true) {
+        unsigned long long chunk = 0;
+        if (lane == 0) chunk = atomicAdd(counter, 1ull);
+        chunk = __shfl(chunk, 0, 64);
+        if ((int64_t)chunk * 64 >= chunk_nodes) break;
+        const int64_t idx = (int64_t)chunk * 64 + lane;
+        if (idx < chunk_nodes)
+            forward_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile,
+                            staging, vals, tabc, back_out);
+    }
+}
+
+// Layered spheres: one launch covers the ranked nodes [start, end), one group of 64 per
+// wavefront and no work loop; `rows` (the slab's) is set by the launch's first node. The layered
+// solves keep enough state per lane that their registers spill; run inside a persistent work
+// loop, that solve has given wrong results, so these launches carry none.
 template <int L>
 __global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, const int *order,
                                                         int64_t start, int64_t end,
@@ -599,15 +625,6 @@ static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int 
     }
 }
 
-// The logical block of physical block b of nb. The dispatcher deals physical blocks round-robin
-// over the 8 XCDs; this numbering gives consecutive logical blocks to one XCD, so that the
-// cosine slices of an ensemble read its coefficients through the same L2.
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
-    const int64_t per = nb / 8, rem = nb % 8;
-    const int64_t x = b % 8, j = b / 8;
-    return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
-}
-
 // The loop runs over the items (group of 16 nodes, 32-order chunk) of the ensemble, one ahead:
 // while an item's products run, the next item's coefficients and table slices are in flight,
 // issued in that order and unconditionally (clamped to valid addresses, masked after arrival),
@@ -622,25 +639,33 @@ __host__ __device__ inline size_t f16_lds_bytes(int64_t n_radius) {
            (size_t)((n_radius + kGroupNodes - 1) / kGroupNodes) * sizeof(int);
 }
 
-template <int S>
-static __global__ __launch_bounds__(512) void ens_angular_f16(
+// A workgroup is W wavefronts and 16 W cosines (a cosine slice); each thread stages P = 8 / W of
+// an item's 512 (node, order) pairs.
+constexpr int kF16Waves = 8;
+
+template <int S, int W>
+static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, int slices, int npad,
     const int *node_n, const int64_t *node_off, const int64_t *ens_base, const cx *staging,
     const double *tabc, const _Float16 *tab16, double *tab_out) {
-    __shared__ _Float16 lds_x[S][64][kF16Pitch];  // [slice][row 4 q + c][order]
+    constexpr int P = 8 / W;
+    __shared__ _Float16 lds_x[2][S][64][kF16Pitch];  // [buffer][slice][row 4 q + c][order]
     extern __shared__ double w_cf[];              // [n_radius] table factors, then:
     int *w_n = reinterpret_cast<int *>(w_cf + n_radius);  // [n_radius] N
     int *w_off = w_n + n_radius;                           // [n_radius] staging offsets
     int *gmax = w_off + n_radius;                          // [groups] largest N
-    const int64_t lb = xcd_block(blockIdx.x, ne * slices);
-    const int64_t eb = lb / slices;
-    const int slice = (int)(lb % slices);
+    // Block b is slice b % slices of ensemble b / slices. The dispatcher deals blocks round-robin
+    // over the 8 XCDs, so with a power-of-two number of slices each XCD serves fixed slices and
+    // its L2 holds only their part of the table (the larger operand stream of this kernel).
+    const int64_t eb = blockIdx.x / slices;
+    const int slice = (int)(blockIdx.x % slices);
     const int64_t e = e0 + eb;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int col = lane & 15, kq = lane >> 4;  // B column (cosine), order set; D node in tile
-    const int sq = t / kF16Orders, sk = t % kF16Orders;  // staging: node in group, order
-    const int jj = slice * kSliceAngles + wave * 16 + col;  // this lane's cosine
+    // Staging pair p of this thread: node sq + p * 2 W of the group, order sk of the chunk.
+    const int sq = t / kF16Orders, sk = t % kF16Orders;
+    const int jj = slice * 16 * W + wave * 16 + col;  // this lane's cosine
     const bool col_ok = jj < h;
     const size_t plane = (size_t)h * npad;
     // Columns beyond h read row 0 of the table: finite values whose products are not written.
@@ -662,17 +687,20 @@ static __global__ __launch_bounds__(512) void ens_angular_f16(
         for (int i = t; i < nr; i += blockDim.x) atomicMax(&gmax[i / kGroupNodes], w_n[i]);
         __syncthreads();
     }
-    // This thread's (node, order) of item (g, kc): its two coefficients, from addresses
-    // clamped into the ensemble's staged terms; *in says whether they are real.
-    auto fetch = [&](int g, int kc, cx &va, cx &vb, bool &in) {
-        const int node = g + sq;
-        const bool ok = node < nr;
-        const int nq = ok ? w_n[node] : 0;
-        const cx *cp = stg + 2 * (int64_t)(ok ? w_off[node] : 0);
-        const int n = kc + sk;  // n - 1
-        in = n < nq;
-        va = cp[in ? n : 0];
-        vb = cp[in ? nq + n : 0];
+    // This thread's (node, order) pairs of item (g, kc): their two coefficients, from addresses
+    // clamped into the ensemble's staged terms; in[p] says whether they are real.
+    auto fetch = [&](int g, int kc, cx (&va)[P], cx (&vb)[P], bool (&in)[P]) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int node = g + sq + p * 2 * W;
+            const bool ok = node < nr;
+            const int nq = ok ? w_n[node] : 0;
+            const cx *cp = stg + 2 * (int64_t)(ok ? w_off[node] : 0);
+            const int n = kc + sk;  // n - 1
+            in[p] = n < nq;
+            va[p] = cp[in[p] ? n : 0];
+            vb[p] = cp[in[p] ? nq + n : 0];
+        }
     };
     auto table = [&](int kc, mfma_h8 (&g)[2][S]) {
 #pragma unroll
@@ -696,25 +724,30 @@ static __global__ __launch_bounds__(512) void ens_angular_f16(
             for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
         }
     int g0 = 0, kc = 0, nmax = gmax[0];
-    cx va, vb;
-    bool in;
-    // One item: its coefficients to LDS, the next item's loads, its products with the table
-    // slices gc (loaded one item earlier) and, at the end of a group, the epilogue. The loop
-    // alternates two sets of table registers, so that no copy waits for a load in flight.
-    auto step = [&](mfma_h8 (&gc)[2][S], mfma_h8 (&gn)[2][S]) __attribute__((always_inline)) {
-        __syncthreads();  // every wavefront is done with the previous item's LDS
-        {  // the item's (node, order): u, v by parity, split into S slices
-            const bool odd = (kc + sk + 1) & 1;
-            const cx a = in ? va : real(0.0), b = in ? vb : real(0.0);
+    cx va[P], vb[P];
+    bool in[P];
+    // One item: its coefficients to LDS buffer `buf`, the next item's loads, its products with
+    // the table slices gc (loaded one item earlier) and, at the end of a group, the epilogue.
+    // The loop alternates two sets of table registers, so that no copy waits for a load in
+    // flight, and two LDS buffers, so that one barrier per item suffices: the buffer written
+    // here was last read two items ago, before every wavefront passed the previous barrier.
+    auto step = [&](mfma_h8 (&gc)[2][S], mfma_h8 (&gn)[2][S], int buf)
+                    __attribute__((always_inline)) {
+        // The item's (node, order) pairs: u, v by parity, split into S slices.
+        const bool odd = (kc + sk + 1) & 1;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const cx a = in[p] ? va[p] : real(0.0), b = in[p] ? vb[p] : real(0.0);
             const cx u = odd ? a : b, w = odd ? b : a;
             const double xs[4] = {u.re, u.im, w.re, w.im};
+            const int row = 4 * (sq + p * 2 * W);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 double r = xs[c];
 #pragma unroll
                 for (int sl = 0; sl < S; ++sl) {
                     const _Float16 v = to_f16(r);
-                    lds_x[sl][4 * sq + c][sk] = v;
+                    lds_x[buf][sl][row + c][sk] = v;
                     r = (r - (double)(float)v) * kF16Step;
                 }
             }
@@ -733,7 +766,7 @@ static __global__ __launch_bounds__(512) void ens_angular_f16(
             mfma_h8 x[S];
 #pragma unroll
             for (int sl = 0; sl < S; ++sl)
-                x[sl] = *reinterpret_cast<const mfma_h8 *>(&lds_x[sl][16 * tt + col][8 * kq]);
+                x[sl] = *reinterpret_cast<const mfma_h8 *>(&lds_x[buf][sl][16 * tt + col][8 * kq]);
 #pragma unroll
             for (int tb = 0; tb < 2; ++tb) {
                 const mfma_f4 h0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], zero, 0, 0, 0);
@@ -791,7 +824,7 @@ static __global__ __launch_bounds__(512) void ens_angular_f16(
     fetch(0, 0, va, vb, in);
     mfma_h8 gA[2][S], gB[2][S];
     table(0, gA);
-    while (step(gA, gB) && step(gB, gA)) {
+    while (step(gA, gB, 0) && step(gB, gA, 1)) {
     }
     // Add the four node lanes (kq = 0..3) of the cosine, in a fixed order.
 #pragma unroll
@@ -820,17 +853,21 @@ static __global__ __launch_bounds__(512) void ens_angular_f16(
 }
 
 // Averages and outputs of one ensemble (ensemble/engine.rs:1782-1836,2396-2497): scalars in
-// SCALARS order, Greek coefficients projected with the Wigner d table
-// dtab[(p * n_angles + j) * (order + 1) + l] (p = d00, d22, d2,-2, d02), the phase matrix.
+// SCALARS order, the phase matrix and, for the Greek projection (ens_greek_mfma), the factor
+// gfac[eb] its coefficients take: 1/Z, or with normalization 1/alpha1_0 in the units of the
+// unnormalized sums, alpha1_0 = (1/2) sum_j w_j F11(mu_j) d00_0(mu_j) (greek.rs:499-509;
+// engine.rs:2447-2458), summed per thread in angle order and then over the threads in a fixed
+// tree, so the result does not depend on scheduling.
 static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_radius, int n_angles,
                                                   const double *aw, const double *vals,
                                                   const double *tab, const double *dtab,
                                                   int64_t order, int normalized,
-                                                  double *scalars, double *greek,
+                                                  double *scalars, double *gfac,
                                                   double *phase) {
     extern __shared__ double sh[];
     double *S = sh;           // 8 sums
     double *F = sh + 8;       // 4 * n_angles, divided by Z
+    __shared__ double part[256];
     const int64_t e = e0 + blockIdx.x;
     const int t = threadIdx.x;
     const double *v = vals + blockIdx.x * n_radius * kNodeVals;
@@ -870,43 +907,79 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
             for (int j = t; j < n_angles; j += blockDim.x)
                 ph[k * n_angles + j] = normalized ? src[k][j] / sca : src[k][j];
     }
-    if (!greek) return;
+    if (!gfac) return;
     const int64_t len = order + 1;
-    double *gr = greek + e * 6 * len;
-    // alpha1_0 first, for the normalization (greek.rs:499-509; engine.rs:2447-2458).
-    __shared__ double a0;
-    if (t == 0) {
-        double s = 0.0;
-        for (int j = 0; j < n_angles; ++j) s += (aw[j] * F11[j]) * dtab[(0 * n_angles + j) * len];
-        a0 = 0.5 * s;
-    }
+    double a = 0.0;
+    for (int j = t; j < n_angles; j += blockDim.x) a += (aw[j] * te[j]) * dtab[(int64_t)j * len];
+    part[t] = a;
     __syncthreads();
-    for (int64_t l = t; l < len; l += blockDim.x) {
-        double s11 = 0, s33 = 0, sp = 0, sm = 0, s12 = 0, s34 = 0;
-        for (int j = 0; j < n_angles; ++j) {
-            const double w = aw[j];
-            const double f11 = w * F11[j], f33 = w * F33[j];
-            const double d00 = dtab[(0 * n_angles + j) * len + l];
-            const double d22 = dtab[(1 * (int64_t)n_angles + j) * len + l];
-            const double d2m2 = dtab[(2 * (int64_t)n_angles + j) * len + l];
-            const double d02 = dtab[(3 * (int64_t)n_angles + j) * len + l];
-            s11 += f11 * d00;
-            s33 += f33 * d00;
-            sp += (f11 + f33) * d22;
-            sm += (f11 - f33) * d2m2;
-            s12 += (w * F12[j]) * d02;
-            s34 += (w * F34[j]) * d02;
-        }
-        const double c = (double)(2 * l + 1) * 0.5;
-        double g6[6] = {c * s11, c * 0.5 * (sp + sm), c * 0.5 * (sp - sm), c * s33, c * s12,
-                        c * s34};
-        if (normalized) {
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) part[t] += part[t + w];
+        __syncthreads();
+    }
+    if (t == 0) gfac[blockIdx.x] = normalized ? 1.0 / (0.5 * part[0]) : 1.0 / z;
+}
+
+// The Greek coefficients of a chunk's ensembles on the FP64 matrix cores (greek.rs:499-560;
+// engine.rs:2396-2458). Per ensemble e and order l the six projections
+// s_k(l) = sum_j A_k[e][j] D_k[j][l] of the weighted summed tables, A = w_j times
+// (F11, F33, F11 + F33, F11 - F33, F12, F34), on the Wigner d functions
+// D = (d00, d00, d22, d2-2, d02, d02) at the Gauss-Legendre cosines, as GEMMs in tiles of 16
+// ensembles x 16 orders with v_mfma_f64_16x16x4f64 (A[i][k] from lane i + 16 k, B[k][j] from
+// lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4). A workgroup is 16 ensembles and
+// 64 orders, a wavefront 16 orders. Then alpha1 = c s11, alpha2, alpha3 = c (sp +- sm)/2,
+// alpha4 = c s33, beta1 = c s12, beta2 = c s34 with c = (2l + 1)/2 gfac[e]; alpha1_0 = 1 exactly
+// when normalized. dtab[(p * n_angles + j) * len + l], p = d00, d22, d2-2, d02.
+static __global__ __launch_bounds__(256) void ens_greek_mfma(int64_t e0, int64_t ne, int n_angles,
+                                                            const double *aw, const double *tab,
+                                                            const double *dtab, int64_t len,
+                                                            const double *gfac, int normalized,
+                                                            double *greek) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    const int64_t et = (int64_t)blockIdx.x * 16;               // the tile's first ensemble
+    const int64_t lt = (int64_t)blockIdx.y * 64 + wave * 16;   // the wavefront's first order
+    const int64_t ea = et + r;                                 // this lane's A row
+    const bool ea_ok = ea < ne;
+    const double *te = tab + (ea_ok ? ea : 0) * 4 * (int64_t)n_angles;
+    const int64_t lb = lt + r;                                 // this lane's B column
+    const double *dcol = dtab + (lb < len ? lb : 0);
+    const int64_t plane = (int64_t)n_angles * len;
+    mfma_d4 acc[6];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) g6[k] /= a0;
-            if (l == 0) g6[0] = 1.0;
-        }
+    for (int k = 0; k < 6; ++k) acc[k] = mfma_d4{0, 0, 0, 0};
+#pragma unroll 4
+    for (int j0 = 0; j0 < n_angles; j0 += 4) {
+        const int j = j0 + kq;
+        const bool j_ok = j < n_angles;
+        const int jc = j_ok ? j : 0;
+        // Angles beyond the rule and rows beyond the chunk weigh 0; columns beyond `order` read
+        // finite values that are not written.
+        const double w = (ea_ok && j_ok) ? aw[jc] : 0.0;
+        const double f11 = w * te[jc], f12 = w * te[n_angles + jc];
+        const double f33 = w * te[2 * n_angles + jc], f34 = w * te[3 * n_angles + jc];
+        const double *dj = dcol + (int64_t)jc * len;
+        const double d00 = dj[0], d22 = dj[plane], d2m2 = dj[2 * plane], d02 = dj[3 * plane];
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11, d00, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(f33, d00, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11 + f33, d22, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11 - f33, d2m2, acc[3], 0, 0, 0);
+        acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(f12, d02, acc[4], 0, 0, 0);
+        acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(f34, d02, acc[5], 0, 0, 0);
+    }
+    if (lb >= len) return;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) gr[k * len + l] = g6[k];
+    for (int v = 0; v < 4; ++v) {
+        const int64_t e = et + kq + 4 * v;
+        if (e >= ne) continue;
+        const double c = (double)(2 * lb + 1) * 0.5 * gfac[e];
+        const double sp = acc[2][v], sm = acc[3][v];
+        double g6[6] = {c * acc[0][v], c * 0.5 * (sp + sm), c * 0.5 * (sp - sm), c * acc[1][v],
+                        c * acc[4][v], c * acc[5][v]};
+        if (normalized && lb == 0) g6[0] = 1.0;
+        double *gr = greek + (e0 + e) * 6 * len + lb;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gr[k * len] = g6[k];
     }
 }
 
@@ -1131,9 +1204,10 @@ static int64_t next_chunk(EnsSetup &S, int64_t e0, int64_t count, int64_t n_radi
     return e1;
 }
 
-// A launch of the node kernels: the ranked nodes [start, end), slabs of `rows` rows.
+// A launch of the node kernels: the ranked nodes [start, end), slabs of `rows` rows; `waves`
+// is the persistent launch's number of wavefronts (spheres), 0 for a loop-free launch.
 struct NodeLaunch {
-    int64_t start, end, rows;
+    int64_t start, end, rows, waves;
 };
 
 // Buffers of one forward chunk.
@@ -1144,24 +1218,40 @@ struct ChunkBufs {
     std::vector<NodeLaunch> plan;    // the node-kernel launches over that ranking
 };
 
-// Scratch budget of one node launch's slabs and its largest number of wavefronts.
+// Scratch budgets of the node launches' slabs: a loop-free layered launch (and its largest
+// number of wavefronts), and the persistent sphere launch, whose slabs all take the chunk's
+// largest N.
 constexpr size_t kSlabBudget = (size_t)4 << 30;
 constexpr int64_t kMaxNodeWaves = 16384;
+constexpr size_t kSphereSlabBudget = (size_t)16 << 30;
+
+// Staging buffer of a chunk's coefficients (a_n, b_n: 32 B per term); it bounds the chunk size.
+constexpr size_t kStagingBytes = (size_t)12 << 30;
 
 template <int L>
 static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, const EnsSetup &S,
                                 cx *scratch, const ChunkBufs &B, cx *back_out) {
-    for (const NodeLaunch &q : B.plan)
-        ens_nodes_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
-            a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch, q.rows,
-            B.staging, B.vals, B.tabc, back_out);
+    for (const NodeLaunch &q : B.plan) {
+        if constexpr (L == 1) {
+            fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+            ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
+                a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
+                q.rows, B.staging, B.vals, B.tabc, back_out);
+        } else {
+            ens_nodes_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
+                a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch,
+                q.rows, B.staging, B.vals, B.tabc, back_out);
+        }
+    }
 }
 
-// Ranks the chunk's nodes by N, largest first, into B.order, and cuts the ranking into the
-// launches of B.plan: consecutive ranges, each with slabs sized by its first (largest) node's N
-// and as many wavefronts as kSlabBudget holds (at most kMaxNodeWaves), so that a few nodes of
-// large N do not shrink the launches of the many small ones. *scratch_bytes receives the slab
-// bytes of the largest launch. Synchronizes: the histogram of N comes back to the host.
+// Ranks the chunk's nodes by N, largest first, into B.order, and plans the node launches of
+// B.plan over that ranking. Spheres: one persistent launch, slabs sized by the chunk's largest
+// N, as many wavefronts as kSphereSlabBudget holds (at most 8 per CU). Layered spheres:
+// consecutive ranges, each with slabs sized by its first (largest) node's N and as many
+// wavefronts as kSlabBudget holds (at most kMaxNodeWaves), so that a few nodes of large N do not
+// shrink the launches of the many small ones. *scratch_bytes receives the slab bytes of the
+// largest launch. Synchronizes: the histogram of N comes back to the host.
 static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t nodes,
                       ChunkBufs &B, size_t *scratch_bytes) {
     hipStream_t s = ctx->stream;
@@ -1187,6 +1277,20 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
     if ((rc = check_hip(ctx, hipStreamSynchronize(s), "node ranking"))) return rc;
     B.plan.clear();
     *scratch_bytes = 0;
+    if (S.a.layers == 1) {
+        int b0 = 0;
+        while (b0 < kBuckets - 1 && h[b0] == 0) ++b0;
+        // Bucket 0 holds every N above kBuckets - 2: its rows come from the largest N.
+        const int64_t n = b0 == 0 ? S.n_max : std::max(kBuckets - 2 - b0, 0);
+        const int64_t rows = n + 2;
+        const size_t per_wave = (size_t)slab_elems(1, rows) * sizeof(cx);
+        int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (nodes + 63) / 64);
+        waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)(kSphereSlabBudget / per_wave)));
+        waves = (waves + 3) / 4 * 4;
+        B.plan.push_back(NodeLaunch{0, nodes, rows, waves});
+        *scratch_bytes = per_wave * (size_t)waves;
+        return SCATRS_OK;
+    }
     int64_t pos = 0, below = 0;  // below: ranked nodes in the buckets before b
     int b = 0;
     while (pos < nodes) {
@@ -1198,7 +1302,7 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         const int64_t waves = std::min<int64_t>(
             kMaxNodeWaves, std::max<int64_t>(4, (int64_t)(kSlabBudget / per_wave)));
         const int64_t end = std::min<int64_t>(nodes, pos + waves * 64);
-        B.plan.push_back(NodeLaunch{pos, end, rows});
+        B.plan.push_back(NodeLaunch{pos, end, rows, 0});
         *scratch_bytes = std::max(*scratch_bytes, per_wave * (size_t)((end - pos + 255) / 256 * 4));
         pos = end;
     }
@@ -1263,14 +1367,15 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                                                 S.node_off, S.ens_base, B.staging, B.tabc,
                                                 S.table, B.tab_out);
     } else if (ctx->angular_kernel >= 2 && n_radius <= kF16MaxNodes) {
-        const int slices = (h + kSliceAngles - 1) / kSliceAngles;
+        constexpr int W = kF16Waves;
+        const int slices = (h + 16 * W - 1) / (16 * W);
         const size_t gm = f16_lds_bytes(n_radius);
         if (ctx->angular_kernel == 2)
-            ens_angular_f16<2><<<(unsigned)(ne * slices), 512, gm, s>>>(
+            ens_angular_f16<2, W><<<(unsigned)(ne * slices), 64 * W, gm, s>>>(
                 e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
                 S.ens_base, B.staging, B.tabc, S.tab16, B.tab_out);
         else
-            ens_angular_f16<3><<<(unsigned)(ne * slices), 512, gm, s>>>(
+            ens_angular_f16<3, W><<<(unsigned)(ne * slices), 64 * W, gm, s>>>(
                 e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
                 S.ens_base, B.staging, B.tabc, S.tab16, B.tab_out);
     } else {

@@ -470,8 +470,31 @@ __device__ __forceinline__ void vjp_node(const EnsArgs &a, int64_t e0, int64_t g
     }
 }
 
-// One group of 64 ranked nodes per wavefront over [start, end), no work loop (as
-// ens_nodes_ranked, for the same reason, with the same launches).
+// Spheres: persistent over the ranked nodes, as ens_nodes_persist.
+static __global__ __launch_bounds__(256) void ens_nodes_vjp_persist(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, cx *scratch,
+    int64_t rows, const cx *staging, const cx *abar, const cx *backs, const double *sbar,
+    double *raw) {
+    __shared__ cx ztile[4][kZBlock * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *tile = &ztile[threadIdx.x >> 6][lane];
+    cx *slab = scratch + wave * slab_elems(1, rows) + lane;
+    while (true) {
+        unsigned long long chunk = 0;
+        if (lane == 0) chunk = atomicAdd(counter, 1ull);
+        chunk = __shfl(chunk, 0, 64);
+        if ((int64_t)chunk * 64 >= chunk_nodes) break;
+        const int64_t idx = (int64_t)chunk * 64 + lane;
+        if (idx < chunk_nodes)
+            vjp_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile, staging,
+                        abar, backs, sbar, raw);
+    }
+}
+
+// Layered spheres: one group of 64 ranked nodes per wavefront over [start, end), no work loop
+// (as ens_nodes_ranked, for the same reason, with the same launches).
 template <int L>
 __global__ __launch_bounds__(256) void ens_nodes_vjp_ranked(
     EnsArgs a, int64_t e0, const int *order, int64_t start, int64_t end, const int *node_n,
@@ -561,10 +584,18 @@ static void launch_nodes_vjp_ranked(hipStream_t s, const EnsArgs &a, int64_t e0,
                                     const EnsSetup &S, cx *scratch, const ChunkBufs &B,
                                     const cx *abar, const cx *backs, const double *sbar,
                                     double *raw) {
-    for (const NodeLaunch &q : B.plan)
-        ens_nodes_vjp_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
-            a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch, q.rows,
-            B.staging, abar, backs, sbar, raw);
+    for (const NodeLaunch &q : B.plan) {
+        if constexpr (L == 1) {
+            fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+            ens_nodes_vjp_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
+                a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
+                q.rows, B.staging, abar, backs, sbar, raw);
+        } else {
+            ens_nodes_vjp_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
+                a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch,
+                q.rows, B.staging, abar, backs, sbar, raw);
+        }
+    }
 }
 
 }  // namespace scatrs
@@ -599,7 +630,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     const int h = S.h;
     const bool tables = d_bar_greek || d_bar_phase;
     // Staging and cotangents take 32 B per term each.
-    const int64_t max_terms = (int64_t)(((size_t)3 << 30) / (2 * sizeof(cx)));
+    const int64_t max_terms = (int64_t)(kStagingBytes / (2 * sizeof(cx)));
     const unsigned vjp_blocks = (unsigned)std::max(1, ctx->cus * 2);
     int64_t e0 = 0;
     while (e0 < count) {
