@@ -39,7 +39,6 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         ChunkBufs B;
         if ((rc = forward_chunk(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, nullptr)))
             return rc;
-        const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
         double *gfac = nullptr;
         if (d_greek) {
             if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kGreekFac, ne * sizeof(double)),
@@ -47,7 +46,7 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
                 return rc;
             gfac = static_cast<double *>(ctx->buf[scatrs_ctx::kGreekFac]);
         }
-        ens_finish<<<(unsigned)ne, 256, shmem, s>>>(e0, n_radius, (int)n_angles, S.d_atab, B.vals,
+        ens_finish<<<(unsigned)ne, 256, 0, s>>>(e0, n_radius, (int)n_angles, S.d_atab, B.vals,
                                                     B.tab_out, S.d_dtab, order, normalized,
                                                     d_scalars, gfac, d_phase);
         if (d_greek) {

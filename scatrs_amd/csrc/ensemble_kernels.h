@@ -203,9 +203,11 @@ __host__ __device__ inline int64_t slab_elems(int layers, int64_t rows) {
 struct StagingSink {
     cx *a_out, *b_out;
     EfficiencySink<false> eff;
+    double amax = 0.0;  // the largest |Re|, |Im| of the node's coefficients
     __device__ void order(int k, cx a, cx b) {
         a_out[k - 1] = a;
         b_out[k - 1] = b;
+        amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
         eff.order(k, a, b);
     }
     __device__ void order_d(int, cx, cx, cx, cx, cx, cx) {}
@@ -215,17 +217,28 @@ struct StagingSink {
     __device__ void zero_absorption() { eff.zero_absorption(); }
 };
 
+// The exponent s that brings a node's largest coefficient component amax into [1/2, 1) by 2^s,
+// at least 0 (|a_n|, |b_n| <= 1), at most 60: the fp16-split angular products scale the node's
+// coefficients by 2^s, so that the small coefficients of small particles stay in fp16's normal
+// range, and its table factor by 2^-2s, both exactly.
+__device__ __forceinline__ int coefficient_exponent(double amax) {
+    if (!(amax > 0.0)) return 0;
+    const int s = -ilogb(amax) - 1;
+    return s < 0 ? 0 : (s > 60 ? 60 : s);
+}
+
 // One node's forward solve: node g of the chunk is ensemble e0 + g / n_radius, radius node
 // g % n_radius. Writes its coefficients to the staging buffer, its weighted per-particle
-// quantities to vals, its table factor to tabc and, if back_out is not null, its backscatter sum
-// sum (2n+1) (-1)^n (a_n - b_n). slab is the lane's column in its wavefront's scratch slab of
-// `rows` rows (slab_elems); tile the lane's column of the wavefront's LDS tile (spheres).
+// quantities to vals, its table factor to tabc, its coefficient exponent to xexp and, if
+// back_out is not null, its backscatter sum sum (2n+1) (-1)^n (a_n - b_n). slab is the lane's
+// column in its wavefront's scratch slab of `rows` rows (slab_elems); tile the lane's column of
+// the wavefront's LDS tile (spheres).
 template <int L>
 __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64_t g,
                                              const int *node_n, const int64_t *node_off,
                                              const int64_t *ens_base, cx *slab, int64_t rows,
                                              cx *tile, cx *staging, double *vals, double *tabc,
-                                             cx *back_out) {
+                                             int *xexp, cx *back_out) {
     const int64_t e = e0 + g / a.n_radius, i = g % a.n_radius;
     const int64_t gi = e * a.n_radius + i;
     const NodeGeom geo = node_geometry(a, e, i);
@@ -261,6 +274,7 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     v[6] = W * (geo.r * geo.r);
     v[7] = W * (geo.r * geo.r * geo.r);
     tabc[g] = W * (area * (4.0 / (geo.x * geo.x)));
+    xexp[g] = coefficient_exponent(sink.amax);
     if (back_out) back_out[g] = sink.eff.back;
 }
 
@@ -270,7 +284,7 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
 static __global__ __launch_bounds__(256) void ens_nodes_persist(
     EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, cx *scratch,
-    int64_t rows, cx *staging, double *vals, double *tabc, cx *back_out) {
+    int64_t rows, cx *staging, double *vals, double *tabc, int *xexp, cx *back_out) {
     __shared__ cx ztile[4][kZBlock * 64];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -284,7 +298,7 @@ static __global__ __launch_bounds__(256) void ens_nodes_persist(
         const int64_t idx = (int64_t)chunk * 64 + lane;
         if (idx < chunk_nodes)
             forward_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile,
-                            staging, vals, tabc, back_out);
+                            staging, vals, tabc, xexp, back_out);
     }
 }
 
@@ -299,7 +313,7 @@ __global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, c
                                                         const int64_t *node_off,
                                                         const int64_t *ens_base, cx *scratch,
                                                         int64_t rows, cx *staging, double *vals,
-                                                        double *tabc, cx *back_out) {
+                                                        double *tabc, int *xexp, cx *back_out) {
     __shared__ cx ztile[4][L == 1 ? kZBlock * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -308,7 +322,7 @@ __global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, c
     const int64_t idx = start + wave * 64 + lane;
     if (idx < end)
         forward_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile, staging,
-                        vals, tabc, back_out);
+                        vals, tabc, xexp, back_out);
 }
 
 // Sort keys of the chunk's nodes: bucket_of(N), N descending.
@@ -564,13 +578,15 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
 
 // The angular products on the fp16 matrix cores (v_mfma_f32_16x16x32_f16) with both operands
 // split in S fp16 slices, g = g_0 + 2^-11 g_1 (+ 2^-22 g_2) and likewise x, and the slice
-// products summed by level: level 0 (g_0 x_0) in fp32 per 32-order chunk, added into fp64
-// after each chunk; level 1 (g_0 x_1 + g_1 x_0) and, with three slices, level 2
+// products summed by level: level 0 (g_0 x_0) in fp32 per 32-order chunk (two slices: per two
+// chunks), added into fp64 after it; level 1 (g_0 x_1 + g_1 x_0) and, with three slices, level 2
 // (g_0 x_2 + g_1 x_1 + g_2 x_0) in fp32 over the whole group of nodes, their rounding entering
 // scaled by 2^-11 and 2^-22, added at its end. Two slices leave errors of order 2^-22 of the
 // terms (the slices' own rounding and the dropped g_1 x_1); three, of order 2^-24 (fp32
-// rounding of the level-0 chunk sums). No scaling is needed: the table entries are bounded by
-// (2n + 1)/2 and |a_n|, |b_n| <= 1 for passive particles, inside fp16's range.
+// rounding of the level-0 chunk sums). The table entries are bounded by (2n + 1)/2; each node's
+// coefficients (|a_n|, |b_n| <= 1) are scaled by 2^s, s its coefficient exponent, so that those
+// of small particles (|a_1| ~ x^3) keep fp16's full precision instead of falling below its
+// normal range; the node's table factor carries 2^-2s. Both scalings are exact.
 // The coefficients are the A operand and the table the B operand, C^T = X^T G^T: row 4 q + c of
 // a group's X^T is node q's (u.re, u.im, v.re, v.im)[c] (four MFMA tiles of 4 nodes), a
 // wavefront's 16 columns its cosines. v_mfma_f32_16x16x32_f16 leaves D[i][j] in lane
@@ -578,7 +594,7 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
 // at its cosine from both tables, A = G1 u, C = G1 v, D = G2 u, B = G2 v, and forms
 // S1(+-mu) = A +- B, S2(+-mu) = C +- D and F11, F12, F33, F34 itself. The table slices are
 // prepared once per call (pt_table_f16), the coefficients split while they are staged in LDS.
-// A workgroup is one ensemble and 128 non-negative cosines, groups of 16 nodes as
+// A workgroup is one ensemble and 16 W non-negative cosines, groups of 16 nodes as
 // ens_angular_mfma; each lane sums its nodes in group order and the four node lanes of a cosine
 // are added at the end, always in the same order.
 typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
@@ -639,6 +655,11 @@ __host__ __device__ inline size_t f16_lds_bytes(int64_t n_radius) {
            (size_t)((n_radius + kGroupNodes - 1) / kGroupNodes) * sizeof(int);
 }
 
+// An item of the fp16 angular kernel: group g (its first node) and chunk kc (its first order).
+struct Item {
+    int g, kc;
+};
+
 // A workgroup is W wavefronts and 16 W cosines (a cosine slice); each thread stages P = 8 / W of
 // an item's 512 (node, order) pairs.
 constexpr int kF16Waves = 8;
@@ -647,11 +668,11 @@ template <int S, int W>
 static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, int slices, int npad,
     const int *node_n, const int64_t *node_off, const int64_t *ens_base, const cx *staging,
-    const double *tabc, const _Float16 *tab16, double *tab_out) {
+    const double *tabc, const int *xexp, const _Float16 *tab16, double *tab_out) {
     constexpr int P = 8 / W;
     __shared__ _Float16 lds_x[2][S][64][kF16Pitch];  // [buffer][slice][row 4 q + c][order]
-    extern __shared__ double w_cf[];              // [n_radius] table factors, then:
-    int *w_n = reinterpret_cast<int *>(w_cf + n_radius);  // [n_radius] N
+    extern __shared__ double w_cf[];              // [n_radius] table factors times 2^-2s, then:
+    int *w_n = reinterpret_cast<int *>(w_cf + n_radius);  // [n_radius] N + (s << 24)
     int *w_off = w_n + n_radius;                           // [n_radius] staging offsets
     int *gmax = w_off + n_radius;                          // [groups] largest N
     // Block b is slice b % slices of ensemble b / slices. The dispatcher deals blocks round-robin
@@ -677,37 +698,79 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
         const int *nn = node_n + e * n_radius;
         const int64_t *noff = node_off + e * n_radius;
         const double *tc = tabc + eb * n_radius;
+        const int *xe = xexp + eb * n_radius;
         for (int i = t; i < ngroups; i += blockDim.x) gmax[i] = 0;
         for (int i = t; i < nr; i += blockDim.x) {
-            w_cf[i] = tc[i];
-            w_n[i] = nn[i];
+            const int sx = xe[i];
+            w_cf[i] = ldexp(tc[i], -2 * sx);
+            w_n[i] = nn[i] | (sx << 24);  // N < 2^24
             w_off[i] = (int)noff[i];
         }
         __syncthreads();
-        for (int i = t; i < nr; i += blockDim.x) atomicMax(&gmax[i / kGroupNodes], w_n[i]);
+        for (int i = t; i < nr; i += blockDim.x)
+            atomicMax(&gmax[i / kGroupNodes], w_n[i] & 0xffffff);
         __syncthreads();
     }
-    // This thread's (node, order) pairs of item (g, kc): their two coefficients, from addresses
-    // clamped into the ensemble's staged terms; in[p] says whether they are real.
-    auto fetch = [&](int g, int kc, cx (&va)[P], cx (&vb)[P], bool (&in)[P]) {
+    // A slot of staged coefficients: this thread's (node, order) pairs of one item, loaded from
+    // addresses clamped into the ensemble's staged terms, whether they are real, and the nodes'
+    // exponents.
+    struct Slot {
+        cx va[P], vb[P];
+        bool in[P];
+        int sx[P];
+    };
+    auto fetch = [&](Item it, Slot &sl) __attribute__((always_inline)) {
+        const int g = it.g < nr ? it.g : 0;  // items past the end load valid, unused data
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const int node = g + sq + p * 2 * W;
             const bool ok = node < nr;
-            const int nq = ok ? w_n[node] : 0;
+            const int packed = ok ? w_n[node] : 0;
+            const int nq = packed & 0xffffff;
+            sl.sx[p] = packed >> 24;
             const cx *cp = stg + 2 * (int64_t)(ok ? w_off[node] : 0);
-            const int n = kc + sk;  // n - 1
-            in[p] = n < nq;
-            va[p] = cp[in[p] ? n : 0];
-            vb[p] = cp[in[p] ? nq + n : 0];
+            const int n = it.kc + sk;  // n - 1
+            sl.in[p] = n < nq;
+            sl.va[p] = cp[sl.in[p] ? n : 0];
+            sl.vb[p] = cp[sl.in[p] ? nq + n : 0];
         }
     };
-    auto table = [&](int kc, mfma_h8 (&g)[2][S]) {
+    // The slot's pairs of item `it` to LDS buffer `buf`: u, v by parity, times 2^s, split into
+    // S slices.
+    auto convert = [&](Item it, const Slot &sl, int buf) __attribute__((always_inline)) {
+        const bool odd = (it.kc + sk + 1) & 1;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const cx a = sl.in[p] ? sl.va[p] : real(0.0), b = sl.in[p] ? sl.vb[p] : real(0.0);
+            const cx u = odd ? a : b, w = odd ? b : a;
+            const int e2 = sl.sx[p];
+            const double xs[4] = {ldexp(u.re, e2), ldexp(u.im, e2), ldexp(w.re, e2),
+                                  ldexp(w.im, e2)};
+            const int row = 4 * (sq + p * 2 * W);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                double r = xs[c];
+#pragma unroll
+                for (int q = 0; q < S; ++q) {
+                    const _Float16 v = to_f16(r);
+                    lds_x[buf][q][row + c][sk] = v;
+                    r = (r - (double)(float)v) * kF16Step;
+                }
+            }
+        }
+    };
+    auto table = [&](int kc, mfma_h8 (&g)[2][S]) __attribute__((always_inline)) {
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int sl = 0; sl < S; ++sl)
-                g[tb][sl] = *reinterpret_cast<const mfma_h8 *>(b_col + (size_t)(tb * S + sl) * plane + kc);
+            for (int q = 0; q < S; ++q)
+                g[tb][q] = *reinterpret_cast<const mfma_h8 *>(b_col + (size_t)(tb * S + q) * plane + kc);
+    };
+    // The item after `it` (groups past the end stay there).
+    auto next = [&](Item it) __attribute__((always_inline)) {
+        if (it.g >= nr) return it;
+        if (it.kc + kF16Orders < gmax[it.g / kGroupNodes]) return Item{it.g, it.kc + kF16Orders};
+        return Item{it.g + kGroupNodes, 0};
     };
 
     const mfma_f4 zero = {0.f, 0.f, 0.f, 0.f};
@@ -723,62 +786,70 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
         }
-    int g0 = 0, kc = 0, nmax = gmax[0];
-    cx va[P], vb[P];
-    bool in[P];
-    // One item: its coefficients to LDS buffer `buf`, the next item's loads, its products with
-    // the table slices gc (loaded one item earlier) and, at the end of a group, the epilogue.
-    // The loop alternates two sets of table registers, so that no copy waits for a load in
-    // flight, and two LDS buffers, so that one barrier per item suffices: the buffer written
-    // here was last read two items ago, before every wavefront passed the previous barrier.
-    auto step = [&](mfma_h8 (&gc)[2][S], mfma_h8 (&gn)[2][S], int buf)
+    // Item cursors: c0 is computed, c1 converted (its table slices loaded), c2 waits in the
+    // other slot, c3 is loaded into the slot that c1 frees. Coefficients thus have two steps to
+    // arrive and table slices one. The loads are issued in every step, table slices first: the
+    // memory counter retires in issue order, so the next step's wait for its table slices leaves
+    // these coefficients outstanding.
+    Item c0{0, 0};
+    Item c1 = next(c0), c2 = next(c1), c3 = next(c2);
+    Slot sa, sb;
+    fetch(c0, sb);
+    convert(c0, sb, 0);
+    fetch(c1, sa);
+    fetch(c2, sb);
+    mfma_h8 gA[2][S], gB[2][S];
+    table(c0.kc, gA);
+    // One step: c1 to LDS buffer bn from slot `sl`, c1's table slices into gn, c3's loads into
+    // `sl`, then c0's products from buffer bc with the table slices gc and, at the end of a
+    // group, the epilogue. The loop alternates the slots, the table registers and the LDS
+    // buffers, so that no copy waits for a load in flight and one barrier per step suffices:
+    // the buffer written here was last read one step ago, before every wavefront passed this
+    // step's barrier.
+    auto step = [&](Slot &sl, mfma_h8 (&gc)[2][S], mfma_h8 (&gn)[2][S], int bc, int bn)
                     __attribute__((always_inline)) {
-        // The item's (node, order) pairs: u, v by parity, split into S slices.
-        const bool odd = (kc + sk + 1) & 1;
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const cx a = in[p] ? va[p] : real(0.0), b = in[p] ? vb[p] : real(0.0);
-            const cx u = odd ? a : b, w = odd ? b : a;
-            const double xs[4] = {u.re, u.im, w.re, w.im};
-            const int row = 4 * (sq + p * 2 * W);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                double r = xs[c];
-#pragma unroll
-                for (int sl = 0; sl < S; ++sl) {
-                    const _Float16 v = to_f16(r);
-                    lds_x[buf][sl][row + c][sk] = v;
-                    r = (r - (double)(float)v) * kF16Step;
-                }
-            }
-        }
         __syncthreads();
-        // The next item; its loads are issued in every step (the last one's are unused).
-        const bool same = kc + kF16Orders < nmax;
-        const int g1 = same ? g0 : g0 + kGroupNodes;
-        const int kc1 = same ? kc + kF16Orders : 0;
-        const bool more = g1 < nr;
-        fetch(more ? g1 : 0, kc1, va, vb, in);
-        table(kc1, gn);
+        convert(c1, sl, bn);
+        table(c1.g < nr ? c1.kc : 0, gn);
+        fetch(c3, sl);
         __builtin_amdgcn_sched_barrier(0);  // keep the products behind those loads
+        const bool same = c1.g == c0.g;
+        const int buf = bc;
+        const int kc = c0.kc;
+        const int g0 = c0.g;
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
             mfma_h8 x[S];
 #pragma unroll
-            for (int sl = 0; sl < S; ++sl)
-                x[sl] = *reinterpret_cast<const mfma_h8 *>(&lds_x[buf][sl][16 * tt + col][8 * kq]);
+            for (int q = 0; q < S; ++q)
+                x[q] = *reinterpret_cast<const mfma_h8 *>(&lds_x[buf][q][16 * tt + col][8 * kq]);
 #pragma unroll
             for (int tb = 0; tb < 2; ++tb) {
-                const mfma_f4 h0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], zero, 0, 0, 0);
                 mfma_f4 m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][1], lv1[tt][tb], 0, 0, 0);
                 lv1[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[1], gc[tb][0], m1, 0, 0, 0);
                 if constexpr (S == 3) {
+                    const mfma_f4 h0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], zero, 0, 0, 0);
                     mfma_f4 m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][2], lv2[tt][tb], 0, 0, 0);
                     m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[1], gc[tb][1], m2, 0, 0, 0);
                     lv2[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[2], gc[tb][0], m2, 0, 0, 0);
-                }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)h0[v];
+                    for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)h0[v];
+                } else {
+                    // Two slices: level 0 runs in fp32 over two chunks (lv2 is free for it).
+                    lv2[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], lv2[tt][tb], 0, 0, 0);
+                }
+            }
+        }
+        if constexpr (S == 2) {
+            if (!same || ((kc / kF16Orders) & 1)) {  // every second chunk and the group's last
+#pragma unroll
+                for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                    for (int tb = 0; tb < 2; ++tb) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)lv2[tt][tb][v];
+                        lv2[tt][tb] = zero;
+                    }
             }
         }
         if (!same) {
@@ -815,16 +886,13 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
                 }
             }
         }
-        if (!more) return false;
-        if (!same) nmax = gmax[g1 / kGroupNodes];
-        g0 = g1;
-        kc = kc1;
-        return true;
+        c0 = c1;
+        c1 = c2;
+        c2 = c3;
+        c3 = next(c3);
+        return c0.g < nr;
     };
-    fetch(0, 0, va, vb, in);
-    mfma_h8 gA[2][S], gB[2][S];
-    table(0, gA);
-    while (step(gA, gB, 0) && step(gB, gA, 1)) {
+    while (step(sa, gA, gB, 0, 1) && step(sb, gB, gA, 1, 0)) {
     }
     // Add the four node lanes (kq = 0..3) of the cosine, in a fixed order.
 #pragma unroll
@@ -864,25 +932,30 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
                                                   int64_t order, int normalized,
                                                   double *scalars, double *gfac,
                                                   double *phase) {
-    extern __shared__ double sh[];
-    double *S = sh;           // 8 sums
-    double *F = sh + 8;       // 4 * n_angles, divided by Z
-    __shared__ double part[256];
+    __shared__ double part[kNodeVals][33];  // per value: 32 partial sums (padded row)
+    __shared__ double S[kNodeVals];
     const int64_t e = e0 + blockIdx.x;
     const int t = threadIdx.x;
     const double *v = vals + blockIdx.x * n_radius * kNodeVals;
-    if (t < kNodeVals) {
-        double s = 0.0;
-        for (int64_t i = 0; i < n_radius; ++i) s += v[i * kNodeVals + t];
-        S[t] = s;
+    {
+        // The 8 node sums: thread t adds value t % 8 over the nodes i = t / 8 (mod 32), in node
+        // order; the 32 partial sums of each value are then added in a fixed order.
+        const int k = t % kNodeVals, r = t / kNodeVals;
+        double a = 0.0;
+        for (int64_t i = r; i < n_radius; i += 32) a += v[i * kNodeVals + k];
+        part[k][r] = a;
+        __syncthreads();
+        if (t < kNodeVals) {
+            double s = 0.0;
+            for (int j = 0; j < 32; ++j) s += part[t][j];
+            S[t] = s;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    const double z = S[0];
+    const double z = S[0], iz = 1.0 / z;
     const double *te = tab + (int64_t)blockIdx.x * 4 * n_angles;
-    for (int j = t; j < 4 * n_angles; j += blockDim.x) F[j] = te[j] / z;
-    __syncthreads();
-    const double ext = S[1] / z, sca = S[2] / z, scag = S[3] / z, absb = S[4] / z,
-                 back = S[5] / z, r2 = S[6] / z, r3 = S[7] / z;
+    const double ext = S[1] * iz, sca = S[2] * iz, scag = S[3] * iz, absb = S[4] * iz,
+                 back = S[5] * iz, r2 = S[6] * iz, r3 = S[7] * iz;
     if (t == 0) {
         double *o = scalars + e * 11;
         o[0] = ext;
@@ -897,27 +970,28 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
         o[9] = kPi * r2;
         o[10] = 4.0 * kPi / 3.0 * r3;
     }
-    const double *F11 = F, *F12 = F + n_angles, *F33 = F + 2 * n_angles, *F34 = F + 3 * n_angles;
     if (phase) {
-        // [F11, F12, F22 = F11, F33, F34, F44 = F33], divided by <C_sca> when normalized
+        // [F11, F12, F22 = F11, F33, F34, F44 = F33] / Z, divided by <C_sca> when normalized
         // (ensemble/engine.rs:2459-2490).
+        const double f = normalized ? iz / sca : iz;
         double *ph = phase + e * 6 * (int64_t)n_angles;
-        const double *src[6] = {F11, F12, F11, F33, F34, F33};
+        const int src[6] = {0, 1, 0, 2, 3, 2};
         for (int k = 0; k < 6; ++k)
             for (int j = t; j < n_angles; j += blockDim.x)
-                ph[k * n_angles + j] = normalized ? src[k][j] / sca : src[k][j];
+                ph[k * n_angles + j] = te[src[k] * n_angles + j] * f;
     }
     if (!gfac) return;
     const int64_t len = order + 1;
+    __shared__ double red[256];
     double a = 0.0;
     for (int j = t; j < n_angles; j += blockDim.x) a += (aw[j] * te[j]) * dtab[(int64_t)j * len];
-    part[t] = a;
+    red[t] = a;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (t < w) part[t] += part[t + w];
+        if (t < w) red[t] += red[t + w];
         __syncthreads();
     }
-    if (t == 0) gfac[blockIdx.x] = normalized ? 1.0 / (0.5 * part[0]) : 1.0 / z;
+    if (t == 0) gfac[blockIdx.x] = normalized ? 1.0 / (0.5 * red[0]) : iz;
 }
 
 // The Greek coefficients of a chunk's ensembles on the FP64 matrix cores (greek.rs:499-560;
@@ -930,42 +1004,66 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
 // 64 orders, a wavefront 16 orders. Then alpha1 = c s11, alpha2, alpha3 = c (sp +- sm)/2,
 // alpha4 = c s33, beta1 = c s12, beta2 = c s34 with c = (2l + 1)/2 gfac[e]; alpha1_0 = 1 exactly
 // when normalized. dtab[(p * n_angles + j) * len + l], p = d00, d22, d2-2, d02.
+constexpr int kGreekAngles = 32;  // angles of the A tile staged per step
+
 static __global__ __launch_bounds__(256) void ens_greek_mfma(int64_t e0, int64_t ne, int n_angles,
                                                             const double *aw, const double *tab,
                                                             const double *dtab, int64_t len,
                                                             const double *gfac, int normalized,
                                                             double *greek) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // The A tile: [combination][ensemble][angle], the six weighted combinations of 16
+    // ensembles at kGreekAngles angles, staged with loads along each ensemble's rows.
+    __shared__ double lds_a[6][16][kGreekAngles + 1];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
     const int r = lane & 15, kq = lane >> 4;
     const int64_t et = (int64_t)blockIdx.x * 16;               // the tile's first ensemble
     const int64_t lt = (int64_t)blockIdx.y * 64 + wave * 16;   // the wavefront's first order
-    const int64_t ea = et + r;                                 // this lane's A row
-    const bool ea_ok = ea < ne;
-    const double *te = tab + (ea_ok ? ea : 0) * 4 * (int64_t)n_angles;
     const int64_t lb = lt + r;                                 // this lane's B column
     const double *dcol = dtab + (lb < len ? lb : 0);
     const int64_t plane = (int64_t)n_angles * len;
     mfma_d4 acc[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) acc[k] = mfma_d4{0, 0, 0, 0};
-#pragma unroll 4
-    for (int j0 = 0; j0 < n_angles; j0 += 4) {
-        const int j = j0 + kq;
-        const bool j_ok = j < n_angles;
-        const int jc = j_ok ? j : 0;
-        // Angles beyond the rule and rows beyond the chunk weigh 0; columns beyond `order` read
-        // finite values that are not written.
-        const double w = (ea_ok && j_ok) ? aw[jc] : 0.0;
-        const double f11 = w * te[jc], f12 = w * te[n_angles + jc];
-        const double f33 = w * te[2 * n_angles + jc], f34 = w * te[3 * n_angles + jc];
-        const double *dj = dcol + (int64_t)jc * len;
-        const double d00 = dj[0], d22 = dj[plane], d2m2 = dj[2 * plane], d02 = dj[3 * plane];
-        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11, d00, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(f33, d00, acc[1], 0, 0, 0);
-        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11 + f33, d22, acc[2], 0, 0, 0);
-        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(f11 - f33, d2m2, acc[3], 0, 0, 0);
-        acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(f12, d02, acc[4], 0, 0, 0);
-        acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(f34, d02, acc[5], 0, 0, 0);
+    for (int j0 = 0; j0 < n_angles; j0 += kGreekAngles) {
+        __syncthreads();
+        // Thread t stages (ensemble t / 16, angles t % 16 and t % 16 + 16) of the tile.
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int er = t / 16, ja = t % 16 + 16 * h;
+            const int64_t e = et + er;
+            const int j = j0 + ja;
+            double f11 = 0.0, f12 = 0.0, f33 = 0.0, f34 = 0.0;
+            if (e < ne && j < n_angles) {  // rows beyond the chunk, angles beyond the rule: 0
+                const double *te = tab + e * 4 * (int64_t)n_angles + j;
+                const double w = aw[j];
+                f11 = w * te[0];
+                f12 = w * te[n_angles];
+                f33 = w * te[2 * n_angles];
+                f34 = w * te[3 * n_angles];
+            }
+            lds_a[0][er][ja] = f11;
+            lds_a[1][er][ja] = f33;
+            lds_a[2][er][ja] = f11 + f33;
+            lds_a[3][er][ja] = f11 - f33;
+            lds_a[4][er][ja] = f12;
+            lds_a[5][er][ja] = f34;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int jj = 0; jj < kGreekAngles; jj += 4) {
+            const int ja = jj + kq;
+            const int j = j0 + ja;
+            // Angles beyond the rule meet zero A values; read a valid row of the table for them.
+            const double *dj = dcol + (int64_t)(j < n_angles ? j : 0) * len;
+            const double d00 = dj[0], d22 = dj[plane], d2m2 = dj[2 * plane], d02 = dj[3 * plane];
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_a[0][r][ja], d00, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_a[1][r][ja], d00, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_a[2][r][ja], d22, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_a[3][r][ja], d2m2, acc[3], 0, 0, 0);
+            acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_a[4][r][ja], d02, acc[4], 0, 0, 0);
+            acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_a[5][r][ja], d02, acc[5], 0, 0, 0);
+        }
     }
     if (lb >= len) return;
 #pragma unroll
@@ -1214,6 +1312,7 @@ struct NodeLaunch {
 struct ChunkBufs {
     cx *staging = nullptr;
     double *vals = nullptr, *tabc = nullptr, *tab_out = nullptr;
+    int *xexp = nullptr;             // per node: its coefficient exponent
     int *order = nullptr;            // the chunk's nodes ranked by N, largest first
     std::vector<NodeLaunch> plan;    // the node-kernel launches over that ranking
 };
@@ -1236,11 +1335,11 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
                 a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
-                q.rows, B.staging, B.vals, B.tabc, back_out);
+                q.rows, B.staging, B.vals, B.tabc, B.xexp, back_out);
         } else {
             ens_nodes_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
                 a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch,
-                q.rows, B.staging, B.vals, B.tabc, back_out);
+                q.rows, B.staging, B.vals, B.tabc, B.xexp, back_out);
         }
     }
 }
@@ -1327,7 +1426,7 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                         "staging")))
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kNodes,
-                                    (size_t)nodes * (kNodeVals + 1) * sizeof(double)),
+                                    (size_t)nodes * ((kNodeVals + 1) * sizeof(double) + sizeof(int))),
                         "nodes")))
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAccum, (size_t)ne * 4 * n_angles * sizeof(double)),
@@ -1343,6 +1442,7 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     B.staging = static_cast<cx *>(ctx->buf[scatrs_ctx::kStaging]);
     B.vals = static_cast<double *>(ctx->buf[scatrs_ctx::kNodes]);
     B.tabc = B.vals + nodes * kNodeVals;
+    B.xexp = reinterpret_cast<int *>(B.tabc + nodes);
     B.tab_out = static_cast<double *>(ctx->buf[scatrs_ctx::kAccum]);
     cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
     mark_begin(ctx);
@@ -1373,11 +1473,11 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         if (ctx->angular_kernel == 2)
             ens_angular_f16<2, W><<<(unsigned)(ne * slices), 64 * W, gm, s>>>(
                 e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
-                S.ens_base, B.staging, B.tabc, S.tab16, B.tab_out);
+                S.ens_base, B.staging, B.tabc, B.xexp, S.tab16, B.tab_out);
         else
             ens_angular_f16<3, W><<<(unsigned)(ne * slices), 64 * W, gm, s>>>(
                 e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
-                S.ens_base, B.staging, B.tabc, S.tab16, B.tab_out);
+                S.ens_base, B.staging, B.tabc, B.xexp, S.tab16, B.tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
