@@ -52,9 +52,12 @@ hipError_t ensure(scatrs_ctx *ctx, int which, size_t bytes);
 
 int fail(scatrs_ctx *ctx, int code, const std::string &msg);
 
-// Events around the dominant kernel of a call, when profiling is on.
+// Events around the dominant kernel of a call, when profiling is on, on the context's stream or
+// on `stream`.
 void mark_begin(scatrs_ctx *ctx);
 void mark_end(scatrs_ctx *ctx);
+void mark_begin(scatrs_ctx *ctx, hipStream_t stream);
+void mark_end(scatrs_ctx *ctx, hipStream_t stream);
 int check_hip(scatrs_ctx *ctx, hipError_t e, const char *what);
 
 // Validation codes written by the prep kernels.

@@ -30,7 +30,7 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B).
-    const int64_t max_terms = (int64_t)(kStagingBytes / (2 * sizeof(cx)));
+    const int64_t max_terms = chunk_terms_limit(4);
     int64_t e0 = 0;
     while (e0 < count) {
         int64_t chunk_terms = 0;
@@ -47,8 +47,8 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
             gfac = static_cast<double *>(ctx->buf[scatrs_ctx::kGreekFac]);
         }
         ens_finish<<<(unsigned)ne, 256, 0, s>>>(e0, n_radius, (int)n_angles, S.d_atab, B.vals,
-                                                    B.tab_out, S.d_dtab, order, normalized,
-                                                    d_scalars, gfac, d_phase);
+                                                B.tab_out, S.d_dtab, order, normalized,
+                                                d_scalars, gfac, d_phase);
         if (d_greek) {
             const int64_t len = order + 1;
             const dim3 grid((unsigned)((ne + 15) / 16), (unsigned)((len + 63) / 64));

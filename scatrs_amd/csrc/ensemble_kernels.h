@@ -1325,7 +1325,16 @@ constexpr int64_t kMaxNodeWaves = 16384;
 constexpr size_t kSphereSlabBudget = (size_t)16 << 30;
 
 // Staging buffer of a chunk's coefficients (a_n, b_n: 32 B per term); it bounds the chunk size.
-constexpr size_t kStagingBytes = (size_t)12 << 30;
+// Every chunk waits for its slowest node (the serial recurrence of the largest N), so chunks
+// are made as large as memory allows: 1/`share` of the device's free memory, at most 48 GiB and
+// at least 1 GiB. Returns the staged terms a chunk may hold.
+static int64_t chunk_terms_limit(int share) {
+    size_t free_b = 0, total_b = 0;
+    size_t bytes = (size_t)12 << 30;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+        bytes = std::min<size_t>((size_t)48 << 30, std::max<size_t>((size_t)1 << 30, free_b / share));
+    return (int64_t)(bytes / (2 * sizeof(cx)));
+}
 
 template <int L>
 static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, const EnsSetup &S,
@@ -1352,14 +1361,14 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
 // shrink the launches of the many small ones. *scratch_bytes receives the slab bytes of the
 // largest launch. Synchronizes: the histogram of N comes back to the host.
 static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t nodes,
-                      ChunkBufs &B, size_t *scratch_bytes) {
-    hipStream_t s = ctx->stream;
+                      ChunkBufs &B, size_t *scratch_bytes, hipStream_t s) {
     int rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kOrder,
+    const int kord = scatrs_ctx::kOrder;
+    if ((rc = check_hip(ctx, ensure(ctx, kord,
                                     (size_t)nodes * 2 * sizeof(int) + 2 * kBuckets * sizeof(unsigned)),
                         "order")))
         return rc;
-    int *keys = static_cast<int *>(ctx->buf[scatrs_ctx::kOrder]);
+    int *keys = static_cast<int *>(ctx->buf[kord]);
     B.order = keys + nodes;
     unsigned *hist = reinterpret_cast<unsigned *>(B.order + nodes);
     fill_u32<<<(2 * kBuckets + 255) / 256, 256, 0, s>>>(hist, 0u, 2 * kBuckets);
@@ -1408,28 +1417,28 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
     return SCATRS_OK;
 }
 
-// Runs the node solves and the angular products of the chunk [e0, e0 + ne) (chunk_terms staged
-// terms): afterwards B.vals holds the nodes' weighted scalars, B.tabc their table factors,
-// B.tab_out the ensembles' summed tables [F11, F12, F33, F34] x n_angles and B.staging the
-// coefficients. back_out (may be null) receives the nodes' backscatter sums.
-static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
+// The node solves of the chunk [e0, e0 + ne) (chunk_terms staged terms) on stream s:
+// afterwards B.vals holds the nodes' weighted scalars, B.tabc their table
+// factors, B.xexp their coefficient exponents and B.staging the coefficients. back_out (may be
+// null) receives the nodes' backscatter sums. Synchronizes s (the node ranking).
+static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                          int64_t chunk_terms, int64_t n_radius, int64_t n_angles, ChunkBufs &B,
-                         cx *back_out) {
+                         cx *back_out, hipStream_t s) {
     int rc;
-    hipStream_t s = ctx->stream;
-    const int layers = S.a.layers, h = S.h;
+    const int layers = S.a.layers;
     const int64_t nodes = ne * n_radius;
+    const int kstg = scatrs_ctx::kStaging, knod = scatrs_ctx::kNodes, kacc = scatrs_ctx::kAccum;
     size_t scratch_bytes = 0;
-    if ((rc = plan_nodes(ctx, S, e0, nodes, B, &scratch_bytes))) return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kStaging,
+    if ((rc = plan_nodes(ctx, S, e0, nodes, B, &scratch_bytes, s))) return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, kstg,
                                     (size_t)std::max<int64_t>(chunk_terms, 1) * 2 * sizeof(cx)),
                         "staging")))
         return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kNodes,
+    if ((rc = check_hip(ctx, ensure(ctx, knod,
                                     (size_t)nodes * ((kNodeVals + 1) * sizeof(double) + sizeof(int))),
                         "nodes")))
         return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAccum, (size_t)ne * 4 * n_angles * sizeof(double)),
+    if ((rc = check_hip(ctx, ensure(ctx, kacc, (size_t)ne * 4 * n_angles * sizeof(double)),
                         "tables")))
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, scratch_bytes), "scratch")))
@@ -1439,13 +1448,13 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                                             hipMemcpyHostToDevice, s),
                         "base copy")))
         return rc;
-    B.staging = static_cast<cx *>(ctx->buf[scatrs_ctx::kStaging]);
-    B.vals = static_cast<double *>(ctx->buf[scatrs_ctx::kNodes]);
+    B.staging = static_cast<cx *>(ctx->buf[kstg]);
+    B.vals = static_cast<double *>(ctx->buf[knod]);
     B.tabc = B.vals + nodes * kNodeVals;
     B.xexp = reinterpret_cast<int *>(B.tabc + nodes);
-    B.tab_out = static_cast<double *>(ctx->buf[scatrs_ctx::kAccum]);
+    B.tab_out = static_cast<double *>(ctx->buf[kacc]);
     cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
-    mark_begin(ctx);
+    mark_begin(ctx, s);
     const EnsArgs &a = S.a;
 #define SCATRS_RANKED(LL) launch_nodes_ranked<LL>(s, a, e0, S, scratch, B, back_out)
     switch (layers) {
@@ -1459,8 +1468,17 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         default: SCATRS_RANKED(8); break;
     }
 #undef SCATRS_RANKED
-    mark_end(ctx);
-    mark_begin(ctx);
+    mark_end(ctx, s);
+    return check_hip(ctx, hipGetLastError(), "ensemble launch");
+}
+
+// The angular products of the chunk [e0, e0 + ne) on stream s from the buffers forward_nodes
+// filled: afterwards B.tab_out holds the ensembles' summed tables [F11, F12, F33, F34] x
+// n_angles.
+static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
+                           int64_t n_radius, int64_t n_angles, ChunkBufs &B, hipStream_t s) {
+    const int h = S.h;
+    mark_begin(ctx, s);
     if (ctx->angular_kernel == 1) {
         const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
         ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,
@@ -1484,8 +1502,19 @@ static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
             e0, n_radius, (int)n_angles, h, slices, S.node_n, S.node_off, S.ens_base, B.staging,
             B.tabc, S.table, B.tab_out);
     }
-    mark_end(ctx);
+    mark_end(ctx, s);
     return check_hip(ctx, hipGetLastError(), "ensemble launch");
+}
+
+// The node solves and the angular products of a chunk on the context's stream.
+static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
+                         int64_t chunk_terms, int64_t n_radius, int64_t n_angles, ChunkBufs &B,
+                         cx *back_out) {
+    int rc;
+    if ((rc = forward_nodes(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, back_out,
+                            ctx->stream)))
+        return rc;
+    return forward_angular(ctx, S, e0, ne, n_radius, n_angles, B, ctx->stream);
 }
 
 }  // namespace scatrs

@@ -46,21 +46,24 @@ int fail(scatrs_ctx *ctx, int code, const std::string &msg) {
     return code;
 }
 
-void mark_begin(scatrs_ctx *ctx) {
+void mark_begin(scatrs_ctx *ctx, hipStream_t stream) {
     if (!ctx->profiling || ctx->ev_count >= scatrs_ctx::kMaxEvents) return;
     const int i = ctx->ev_count;
     if (!ctx->ev_begin[i]) {
         (void)hipEventCreate(&ctx->ev_begin[i]);
         (void)hipEventCreate(&ctx->ev_end[i]);
     }
-    (void)hipEventRecord(ctx->ev_begin[i], ctx->stream);
+    (void)hipEventRecord(ctx->ev_begin[i], stream);
 }
 
-void mark_end(scatrs_ctx *ctx) {
+void mark_end(scatrs_ctx *ctx, hipStream_t stream) {
     if (!ctx->profiling || ctx->ev_count >= scatrs_ctx::kMaxEvents) return;
-    (void)hipEventRecord(ctx->ev_end[ctx->ev_count], ctx->stream);
+    (void)hipEventRecord(ctx->ev_end[ctx->ev_count], stream);
     ctx->ev_count++;
 }
+
+void mark_begin(scatrs_ctx *ctx) { mark_begin(ctx, ctx->stream); }
+void mark_end(scatrs_ctx *ctx) { mark_end(ctx, ctx->stream); }
 
 int check_hip(scatrs_ctx *ctx, hipError_t e, const char *what) {
     if (e == hipSuccess) return SCATRS_OK;
