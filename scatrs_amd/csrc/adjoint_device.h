@@ -21,7 +21,7 @@ namespace scatrs {
 // are read from the forward pass's staging buffer.
 template <int L>
 struct AdjointSink {
-    const cx *sa, *sb;  // staged a_n, b_n (index n - 1)
+    const cx *sa, *sb;  // staged a_n, b_n (index 2 (n - 1): interleaved)
     const cx *ta, *tb;  // table cotangents of a_n, b_n (tfac = 0 and any finite n values: none)
     int n;
     double tfac, alpha, beta, gamma;
@@ -47,7 +47,7 @@ struct AdjointSink {
 
     __device__ void order_d(int k, cx a, cx b, cx dax, cx dam, cx dbx, cx dbm) {
         const double kf = (double)k, w = (double)(2 * k + 1);
-        const cx a1 = k < n ? sa[k] : real(0.0), b1 = k < n ? sb[k] : real(0.0);
+        const cx a1 = k < n ? sa[2 * k] : real(0.0), b1 = k < n ? sb[2 * k] : real(0.0);
         const double pp = k >= 2 ? (kf - 1.0) * (kf + 1.0) / kf : 0.0;
         const double pn = k < n ? kf * (kf + 2.0) / (kf + 1.0) : 0.0;
         const double own = w / (kf * (kf + 1.0));

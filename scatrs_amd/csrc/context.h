@@ -39,7 +39,7 @@ struct scatrs_ctx {
     size_t cap[kNumBufs] = {};
     void *host_pinned = nullptr;  // small pinned status block
     bool profiling = false;
-    int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2, 3: fp16 MFMA, 2 or 3 slices
+    int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2: fp16-split MFMA
     static constexpr int kMaxEvents = 1024;
     hipEvent_t ev_begin[kMaxEvents] = {}, ev_end[kMaxEvents] = {};
     int ev_count = 0;

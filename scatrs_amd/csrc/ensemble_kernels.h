@@ -199,14 +199,16 @@ __host__ __device__ inline int64_t slab_elems(int layers, int64_t rows) {
     return rows * 64 * 2 * layers;
 }
 
-// Writes a node's coefficients to the staging buffer and accumulates its efficiency sums.
+// Writes a node's coefficients to the staging buffer, interleaved (a_1, b_1, a_2, b_2, ...: one
+// 32-byte stream per lane, whose cache lines fill every four orders), and accumulates its
+// efficiency sums.
 struct StagingSink {
-    cx *a_out, *b_out;
+    cx *ab_out;
     EfficiencySink<false> eff;
     double amax = 0.0;  // the largest |Re|, |Im| of the node's coefficients
     __device__ void order(int k, cx a, cx b) {
-        a_out[k - 1] = a;
-        b_out[k - 1] = b;
+        ab_out[2 * (k - 1)] = a;
+        ab_out[2 * (k - 1) + 1] = b;
         amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
         eff.order(k, a, b);
     }
@@ -217,14 +219,17 @@ struct StagingSink {
     __device__ void zero_absorption() { eff.zero_absorption(); }
 };
 
-// The exponent s that brings a node's largest coefficient component amax into [1/2, 1) by 2^s,
-// at least 0 (|a_n|, |b_n| <= 1), at most 60: the fp16-split angular products scale the node's
-// coefficients by 2^s, so that the small coefficients of small particles stay in fp16's normal
-// range, and its table factor by 2^-2s, both exactly.
+// The exponent s that brings a node's largest coefficient component amax into [2^9, 2^10) by
+// 2^s (|a_n|, |b_n| <= 1), within [0, 100]: the fp16-split angular products scale the node's
+// coefficients by 2^s, so that their slices keep fp16's full precision (those of small
+// particles, |a_1| ~ x^3, would otherwise fall below its normal range), and its table factor by
+// 2^-2s, both exactly.
+constexpr int kCoefficientBits = 10;
+
 __device__ __forceinline__ int coefficient_exponent(double amax) {
-    if (!(amax > 0.0)) return 0;
-    const int s = -ilogb(amax) - 1;
-    return s < 0 ? 0 : (s > 60 ? 60 : s);
+    if (!(amax > 0.0)) return kCoefficientBits - 1;
+    const int s = -ilogb(amax) - 1 + kCoefficientBits;
+    return s < 0 ? 0 : (s > 100 ? 100 : s);
 }
 
 // One node's forward solve: node g of the chunk is ensemble e0 + g / n_radius, radius node
@@ -245,8 +250,7 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     const int n = node_n[gi];
     const int64_t off = ens_base[e] + node_off[gi];
     StagingSink sink;
-    sink.a_out = staging + 2 * off;
-    sink.b_out = staging + 2 * off + n;
+    sink.ab_out = staging + 2 * off;
     if constexpr (L == 1) {
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
         // Checkpoints in the first column, u_k(x) in the second, the block being used in the
@@ -378,8 +382,8 @@ static __global__ __launch_bounds__(512) void ens_angular(int64_t e0, int64_t n_
                     cx va = real(0.0), vb = real(0.0);
                     if (n < ns[q]) {
                         const cx *c = staging + 2 * (base + noff[i0 + q]);
-                        va = c[n];
-                        vb = c[ns[q] + n];
+                        va = c[2 * n];
+                        vb = c[2 * n + 1];
                     }
                     lds_a[q][k] = va;
                     lds_b[q][k] = vb;
@@ -493,8 +497,8 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
                 cx va = real(0.0), vb = real(0.0);
                 if (n < nq) {
                     const cx *c = staging + 2 * (base + noff[g0 + q]);
-                    va = c[n];
-                    vb = c[nq + n];
+                    va = c[2 * n];
+                    vb = c[2 * n + 1];
                 }
                 lds_a[q][k] = va;
                 lds_b[q][k] = vb;
@@ -577,16 +581,15 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
 }
 
 // The angular products on the fp16 matrix cores (v_mfma_f32_16x16x32_f16) with both operands
-// split in S fp16 slices, g = g_0 + 2^-11 g_1 (+ 2^-22 g_2) and likewise x, and the slice
-// products summed by level: level 0 (g_0 x_0) in fp32 per 32-order chunk (two slices: per two
-// chunks), added into fp64 after it; level 1 (g_0 x_1 + g_1 x_0) and, with three slices, level 2
-// (g_0 x_2 + g_1 x_1 + g_2 x_0) in fp32 over the whole group of nodes, their rounding entering
-// scaled by 2^-11 and 2^-22, added at its end. Two slices leave errors of order 2^-22 of the
-// terms (the slices' own rounding and the dropped g_1 x_1); three, of order 2^-24 (fp32
-// rounding of the level-0 chunk sums). The table entries are bounded by (2n + 1)/2; each node's
-// coefficients (|a_n|, |b_n| <= 1) are scaled by 2^s, s its coefficient exponent, so that those
-// of small particles (|a_1| ~ x^3) keep fp16's full precision instead of falling below its
-// normal range; the node's table factor carries 2^-2s. Both scalings are exact.
+// split in two fp16 slices, g = g_0 + g_1 and x = x_0 + x_1 (g_0, x_0 the nearest fp16 values,
+// g_1, x_1 the fp16 values nearest the remainders), and G X = G_0 X_0 + G_0 X_1 + G_1 X_0 up to
+// the dropped G_1 X_1, of order 2^-22 of the terms. The three products of a table run into one
+// fp32 accumulator over two 32-order chunks, which is then added into fp64: its rounding, 2^-24
+// of the two chunks' sum, is the error floor (about 1e-7 of the largest phase-function value
+// over the 4096 x 64 lognormal workload). So that the remainders stay within fp16's normal
+// range, each node's coefficients are scaled by 2^s (s its coefficient exponent: the largest
+// near 2^10) and the table by 2^t (the largest entry, (2n + 1)/2 at most, near 2^15); the node's
+// table factor carries 2^-2(s + t). All scalings are exact.
 // The coefficients are the A operand and the table the B operand, C^T = X^T G^T: row 4 q + c of
 // a group's X^T is node q's (u.re, u.im, v.re, v.im)[c] (four MFMA tiles of 4 nodes), a
 // wavefront's 16 columns its cosines. v_mfma_f32_16x16x32_f16 leaves D[i][j] in lane
@@ -602,20 +605,25 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kF16Orders = 32;   // one v_mfma_f32_16x16x32_f16 K step
 constexpr int kF16Pitch = 40;    // LDS row of 32 orders padded to 40 halves (bank spread)
-constexpr double kF16Step = 2048.0;  // 2^11 between consecutive slices
+
+// The exponent t of the table's scale 2^t: its entries are bounded by (2 n_max + 1)/2.
+static int table_exponent(int n_max) {
+    const double gmax = (2.0 * n_max + 1.0) / 2.0;
+    return (int)std::floor(std::log2(32768.0 / gmax));
+}
 
 // r to fp16 through fp32: v_cvt_f32_f64, a clamp to fp16's finite range and v_cvt_f16_f32 (a
 // correctly rounded fp64 -> fp16 conversion has no instruction and expands to some 35; the
 // clamp keeps the compiler from merging the two conversions into that one). The rare double
-// rounding moves the slice by at most one fp16 ulp; the split stays exact, since each residual
+// rounding moves a slice by at most one fp16 ulp; the split stays exact, since the remainder
 // r - slice is formed in fp64 and carried to the next slice.
 __device__ __forceinline__ _Float16 to_f16(double r) {
     return (_Float16)__builtin_amdgcn_fmed3f((float)r, -65504.0f, 65504.0f);
 }
 
-// tab16[((t * S + s) * h + jj) * npad + n - 1]: slice s of table t (G1, G2); zero beyond n_max.
-template <int S>
-static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int npad,
+// tab16[(2 t + q) * h * npad + jj * npad + n - 1]: slice q of table t (G1, G2) times 2^texp;
+// zero beyond n_max.
+static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int npad, int texp,
                                     _Float16 *tab16) {
     const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (idx >= (int64_t)h * npad) return;
@@ -624,30 +632,27 @@ static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int 
     if (k < n_max) {
         const double2 pt = table[(size_t)k * h + jj];
         const bool odd = (k + 1) & 1;
-        g[0] = odd ? pt.x : pt.y;
-        g[1] = odd ? pt.y : pt.x;
+        g[0] = ldexp(odd ? pt.x : pt.y, texp);
+        g[1] = ldexp(odd ? pt.y : pt.x, texp);
     }
     const size_t plane = (size_t)h * npad;
     const size_t o = (size_t)jj * npad + k;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-        double r = g[t];
-#pragma unroll
-        for (int sl = 0; sl < S; ++sl) {
-            const _Float16 v = (_Float16)r;
-            tab16[(size_t)(t * S + sl) * plane + o] = v;
-            r = (r - (double)v) * kF16Step;
-        }
+        const _Float16 g0 = (_Float16)g[t];
+        tab16[(size_t)(2 * t) * plane + o] = g0;
+        tab16[(size_t)(2 * t + 1) * plane + o] = (_Float16)(g[t] - (double)g0);
     }
 }
 
-// The loop runs over the items (group of 16 nodes, 32-order chunk) of the ensemble, one ahead:
-// while an item's products run, the next item's coefficients and table slices are in flight,
-// issued in that order and unconditionally (clamped to valid addresses, masked after arrival),
-// so that the wait for this item's table slices leaves them outstanding. The ensemble's node
-// list (N, staging offsets, table factors) and each group's largest N are copied to LDS at the
-// start, so that moving to the next group issues no global loads. Dynamic LDS:
-// f16_lds_bytes(n_radius); n_radius is at most kF16MaxNodes.
+// The loop runs over the items (group of 16 nodes, 32-order chunk) of the ensemble, ahead of
+// the products: coefficients are loaded two items ahead and table slices one, issued in every
+// step, table slices first (the memory counter retires in issue order, so the next step's wait
+// for its table slices leaves these coefficients outstanding), from addresses clamped into
+// valid data and masked after arrival. The ensemble's node list (N and coefficient exponent,
+// staging offsets, table factors) and each group's largest N are copied to LDS at the start, so
+// that moving to the next group issues no global loads. Dynamic LDS: f16_lds_bytes(n_radius);
+// n_radius is at most kF16MaxNodes.
 constexpr int64_t kF16MaxNodes = 2048;  // node list within 33 KB of LDS
 
 __host__ __device__ inline size_t f16_lds_bytes(int64_t n_radius) {
@@ -664,14 +669,15 @@ struct Item {
 // an item's 512 (node, order) pairs.
 constexpr int kF16Waves = 8;
 
-template <int S, int W>
+template <int W>
 static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, int slices, int npad,
-    const int *node_n, const int64_t *node_off, const int64_t *ens_base, const cx *staging,
-    const double *tabc, const int *xexp, const _Float16 *tab16, double *tab_out) {
+    int texp, const int *node_n, const int64_t *node_off, const int64_t *ens_base,
+    const cx *staging, const double *tabc, const int *xexp, const _Float16 *tab16,
+    double *tab_out) {
     constexpr int P = 8 / W;
-    __shared__ _Float16 lds_x[2][S][64][kF16Pitch];  // [buffer][slice][row 4 q + c][order]
-    extern __shared__ double w_cf[];              // [n_radius] table factors times 2^-2s, then:
+    __shared__ _Float16 lds_x[2][2][64][kF16Pitch];  // [buffer][slice][row 4 q + c][order]
+    extern __shared__ double w_cf[];  // [n_radius] table factors times 2^-2(s + t), then:
     int *w_n = reinterpret_cast<int *>(w_cf + n_radius);  // [n_radius] N + (s << 24)
     int *w_off = w_n + n_radius;                           // [n_radius] staging offsets
     int *gmax = w_off + n_radius;                          // [groups] largest N
@@ -702,8 +708,8 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
         for (int i = t; i < ngroups; i += blockDim.x) gmax[i] = 0;
         for (int i = t; i < nr; i += blockDim.x) {
             const int sx = xe[i];
-            w_cf[i] = ldexp(tc[i], -2 * sx);
-            w_n[i] = nn[i] | (sx << 24);  // N < 2^24
+            w_cf[i] = ldexp(tc[i], -2 * (sx + texp));
+            w_n[i] = nn[i] | (sx << 24);  // N < 2^24, s < 128
             w_off[i] = (int)noff[i];
         }
         __syncthreads();
@@ -711,9 +717,9 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
             atomicMax(&gmax[i / kGroupNodes], w_n[i] & 0xffffff);
         __syncthreads();
     }
-    // A slot of staged coefficients: this thread's (node, order) pairs of one item, loaded from
-    // addresses clamped into the ensemble's staged terms, whether they are real, and the nodes'
-    // exponents.
+    // A slot of staged coefficients: this thread's (node, order) pairs of one item (a_n, b_n
+    // interleaved), loaded from addresses clamped into the ensemble's staged terms, whether they
+    // are real, and the nodes' exponents.
     struct Slot {
         cx va[P], vb[P];
         bool in[P];
@@ -731,12 +737,11 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
             const cx *cp = stg + 2 * (int64_t)(ok ? w_off[node] : 0);
             const int n = it.kc + sk;  // n - 1
             sl.in[p] = n < nq;
-            sl.va[p] = cp[sl.in[p] ? n : 0];
-            sl.vb[p] = cp[sl.in[p] ? nq + n : 0];
+            sl.va[p] = cp[sl.in[p] ? 2 * n : 0];
+            sl.vb[p] = cp[sl.in[p] ? 2 * n + 1 : 0];
         }
     };
-    // The slot's pairs of item `it` to LDS buffer `buf`: u, v by parity, times 2^s, split into
-    // S slices.
+    // The slot's pairs of item `it` to LDS buffer `buf`: u, v by parity, times 2^s, in two slices.
     auto convert = [&](Item it, const Slot &sl, int buf) __attribute__((always_inline)) {
         const bool odd = (it.kc + sk + 1) & 1;
 #pragma unroll
@@ -749,22 +754,18 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
             const int row = 4 * (sq + p * 2 * W);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                double r = xs[c];
-#pragma unroll
-                for (int q = 0; q < S; ++q) {
-                    const _Float16 v = to_f16(r);
-                    lds_x[buf][q][row + c][sk] = v;
-                    r = (r - (double)(float)v) * kF16Step;
-                }
+                const _Float16 x0 = to_f16(xs[c]);
+                lds_x[buf][0][row + c][sk] = x0;
+                lds_x[buf][1][row + c][sk] = to_f16(xs[c] - (double)(float)x0);
             }
         }
     };
-    auto table = [&](int kc, mfma_h8 (&g)[2][S]) __attribute__((always_inline)) {
+    auto table = [&](int kc, mfma_h8 (&g)[2][2]) __attribute__((always_inline)) {
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-            for (int q = 0; q < S; ++q)
-                g[tb][q] = *reinterpret_cast<const mfma_h8 *>(b_col + (size_t)(tb * S + q) * plane + kc);
+            for (int q = 0; q < 2; ++q)
+                g[tb][q] = *reinterpret_cast<const mfma_h8 *>(b_col + (size_t)(2 * tb + q) * plane + kc);
     };
     // The item after `it` (groups past the end stay there).
     auto next = [&](Item it) __attribute__((always_inline)) {
@@ -775,22 +776,18 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
 
     const mfma_f4 zero = {0.f, 0.f, 0.f, 0.f};
     double T[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][2 F11, 2 F12, F33, F34]
-    double acc[4][2][4];                             // [tile][table][component]: level 0
-    mfma_f4 lv1[4][2], lv2[4][2];
+    double acc[4][2][4];                             // [tile][table][component]
+    mfma_f4 run[4][2];                               // fp32 sums of up to two chunks
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
-            lv1[tt][tb] = zero;
-            lv2[tt][tb] = zero;
+            run[tt][tb] = zero;
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
         }
     // Item cursors: c0 is computed, c1 converted (its table slices loaded), c2 waits in the
-    // other slot, c3 is loaded into the slot that c1 frees. Coefficients thus have two steps to
-    // arrive and table slices one. The loads are issued in every step, table slices first: the
-    // memory counter retires in issue order, so the next step's wait for its table slices leaves
-    // these coefficients outstanding.
+    // other slot, c3 is loaded into the slot that c1 frees.
     Item c0{0, 0};
     Item c1 = next(c0), c2 = next(c1), c3 = next(c2);
     Slot sa, sb;
@@ -798,7 +795,7 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     convert(c0, sb, 0);
     fetch(c1, sa);
     fetch(c2, sb);
-    mfma_h8 gA[2][S], gB[2][S];
+    mfma_h8 gA[2][2], gB[2][2];
     table(c0.kc, gA);
     // One step: c1 to LDS buffer bn from slot `sl`, c1's table slices into gn, c3's loads into
     // `sl`, then c0's products from buffer bc with the table slices gc and, at the end of a
@@ -806,7 +803,7 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     // buffers, so that no copy waits for a load in flight and one barrier per step suffices:
     // the buffer written here was last read one step ago, before every wavefront passed this
     // step's barrier.
-    auto step = [&](Slot &sl, mfma_h8 (&gc)[2][S], mfma_h8 (&gn)[2][S], int bc, int bn)
+    auto step = [&](Slot &sl, mfma_h8 (&gc)[2][2], mfma_h8 (&gn)[2][2], int bc, int bn)
                     __attribute__((always_inline)) {
         __syncthreads();
         convert(c1, sl, bn);
@@ -814,65 +811,39 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
         fetch(c3, sl);
         __builtin_amdgcn_sched_barrier(0);  // keep the products behind those loads
         const bool same = c1.g == c0.g;
-        const int buf = bc;
-        const int kc = c0.kc;
-        const int g0 = c0.g;
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
-            mfma_h8 x[S];
-#pragma unroll
-            for (int q = 0; q < S; ++q)
-                x[q] = *reinterpret_cast<const mfma_h8 *>(&lds_x[buf][q][16 * tt + col][8 * kq]);
+            const mfma_h8 x0 = *reinterpret_cast<const mfma_h8 *>(&lds_x[bc][0][16 * tt + col][8 * kq]);
+            const mfma_h8 x1 = *reinterpret_cast<const mfma_h8 *>(&lds_x[bc][1][16 * tt + col][8 * kq]);
 #pragma unroll
             for (int tb = 0; tb < 2; ++tb) {
-                mfma_f4 m1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][1], lv1[tt][tb], 0, 0, 0);
-                lv1[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[1], gc[tb][0], m1, 0, 0, 0);
-                if constexpr (S == 3) {
-                    const mfma_f4 h0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], zero, 0, 0, 0);
-                    mfma_f4 m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][2], lv2[tt][tb], 0, 0, 0);
-                    m2 = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[1], gc[tb][1], m2, 0, 0, 0);
-                    lv2[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[2], gc[tb][0], m2, 0, 0, 0);
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)h0[v];
-                } else {
-                    // Two slices: level 0 runs in fp32 over two chunks (lv2 is free for it).
-                    lv2[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x[0], gc[tb][0], lv2[tt][tb], 0, 0, 0);
-                }
+                mfma_f4 r = __builtin_amdgcn_mfma_f32_16x16x32_f16(x0, gc[tb][0], run[tt][tb], 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_f16(x0, gc[tb][1], r, 0, 0, 0);
+                run[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x1, gc[tb][0], r, 0, 0, 0);
             }
         }
-        if constexpr (S == 2) {
-            if (!same || ((kc / kF16Orders) & 1)) {  // every second chunk and the group's last
+        if (!same || ((c0.kc / kF16Orders) & 1)) {  // every second chunk and the group's last
 #pragma unroll
-                for (int tt = 0; tt < 4; ++tt)
+            for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-                    for (int tb = 0; tb < 2; ++tb) {
+                for (int tb = 0; tb < 2; ++tb) {
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)lv2[tt][tb][v];
-                        lv2[tt][tb] = zero;
-                    }
-            }
+                    for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)run[tt][tb][v];
+                    run[tt][tb] = zero;
+                }
         }
         if (!same) {
             // Epilogue of the group: node 4 tt + kq at cosine jj.
 #pragma unroll
             for (int tt = 0; tt < 4; ++tt) {
-                const int node = g0 + 4 * tt + kq;
+                const int node = c0.g + 4 * tt + kq;
                 const double cfac = node < nr ? w_cf[node] : 0.0;
-                double c[2][4];
+                const cx A = mk(acc[tt][0][0], acc[tt][0][1]), C = mk(acc[tt][0][2], acc[tt][0][3]);
+                const cx D = mk(acc[tt][1][0], acc[tt][1][1]), B = mk(acc[tt][1][2], acc[tt][1][3]);
 #pragma unroll
                 for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        double y = acc[tt][tb][v] + (double)lv1[tt][tb][v] * (1.0 / kF16Step);
-                        if constexpr (S == 3)
-                            y += (double)lv2[tt][tb][v] * (1.0 / (kF16Step * kF16Step));
-                        c[tb][v] = y;
-                        acc[tt][tb][v] = 0.0;
-                    }
-                lv1[tt][0] = lv1[tt][1] = zero;
-                lv2[tt][0] = lv2[tt][1] = zero;
-                const cx A = mk(c[0][0], c[0][1]), C = mk(c[0][2], c[0][3]);
-                const cx D = mk(c[1][0], c[1][1]), B = mk(c[1][2], c[1][3]);
+                    for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
 #pragma unroll
                 for (int sgn = 0; sgn < 2; ++sgn) {
                     const cx s1 = sgn == 0 ? A + B : A - B;
@@ -1183,6 +1154,7 @@ struct EnsSetup {
     double2 *table = nullptr;
     _Float16 *tab16 = nullptr;  // the fp16-split table (angular kernel 2)
     int npad = 0;               // its rows' length: n_max rounded up to 32
+    int texp = 0;               // its scale 2^texp
 };
 
 static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, const double *d_p0,
@@ -1259,20 +1231,16 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
         return rc;
     S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
     pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
-    if (ctx->angular_kernel >= 2 && n_radius <= kF16MaxNodes) {
-        const int sl = ctx->angular_kernel;  // fp16 slices
+    if (ctx->angular_kernel == 2 && n_radius <= kF16MaxNodes) {
         S.npad = (S.n_max + kF16Orders - 1) / kF16Orders * kF16Orders;
+        S.texp = table_exponent(S.n_max);
         const size_t cells = (size_t)h * S.npad;
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables16,
-                                        2 * sl * cells * sizeof(_Float16)),
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables16, 4 * cells * sizeof(_Float16)),
                             "table")))
             return rc;
         S.tab16 = static_cast<_Float16 *>(ctx->buf[scatrs_ctx::kTables16]);
-        const unsigned tb = (unsigned)((cells + 255) / 256);
-        if (sl == 2)
-            pt_table_f16<2><<<tb, 256, 0, s>>>(S.table, h, S.n_max, S.npad, S.tab16);
-        else
-            pt_table_f16<3><<<tb, 256, 0, s>>>(S.table, h, S.n_max, S.npad, S.tab16);
+        pt_table_f16<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(S.table, h, S.n_max, S.npad,
+                                                                     S.texp, S.tab16);
     }
     if (greek) {
         const int64_t len = order + 1;
@@ -1484,18 +1452,12 @@ static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,
                                                 S.node_off, S.ens_base, B.staging, B.tabc,
                                                 S.table, B.tab_out);
-    } else if (ctx->angular_kernel >= 2 && n_radius <= kF16MaxNodes) {
+    } else if (ctx->angular_kernel == 2 && n_radius <= kF16MaxNodes) {
         constexpr int W = kF16Waves;
         const int slices = (h + 16 * W - 1) / (16 * W);
-        const size_t gm = f16_lds_bytes(n_radius);
-        if (ctx->angular_kernel == 2)
-            ens_angular_f16<2, W><<<(unsigned)(ne * slices), 64 * W, gm, s>>>(
-                e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
-                S.ens_base, B.staging, B.tabc, B.xexp, S.tab16, B.tab_out);
-        else
-            ens_angular_f16<3, W><<<(unsigned)(ne * slices), 64 * W, gm, s>>>(
-                e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.node_n, S.node_off,
-                S.ens_base, B.staging, B.tabc, B.xexp, S.tab16, B.tab_out);
+        ens_angular_f16<W><<<(unsigned)(ne * slices), 64 * W, f16_lds_bytes(n_radius), s>>>(
+            e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.texp, S.node_n, S.node_off,
+            S.ens_base, B.staging, B.tabc, B.xexp, S.tab16, B.tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(

@@ -261,8 +261,8 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
                     cx va = real(0.0), vb = real(0.0);
                     if (n < nq) {
                         const cx *c = staging + 2 * (base + noff[g0 + q]);
-                        va = c[n];
-                        vb = c[nq + n];
+                        va = c[2 * n];
+                        vb = c[2 * n + 1];
                     }
                     lds_a[q][k] = va;
                     lds_b[q][k] = vb;
@@ -436,8 +436,8 @@ __device__ __forceinline__ void vjp_node(const EnsArgs &a, int64_t e0, int64_t g
     const double *sb = sbar + eb * 8;
     AdjointSink<L> sink;
     sink.init();
-    sink.sa = staging + 2 * off;
-    sink.sb = staging + 2 * off + n;
+    sink.sa = staging + 2 * off;  // a_n, b_n interleaved
+    sink.sb = staging + 2 * off + 1;
     const cx *tcot = abar ? abar : staging;
     sink.ta = tcot + 2 * off;
     sink.tb = tcot + 2 * off + n;

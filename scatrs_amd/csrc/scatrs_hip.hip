@@ -202,10 +202,10 @@ int scatrs_set_profiling(scatrs_ctx *ctx, int on) {
 
 int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind) {
     if (!ctx) return SCATRS_INVALID_INPUT;
-    if (kind < 0 || kind > 3)
+    if (kind < 0 || kind > 2)
         return fail(ctx, SCATRS_INVALID_INPUT,
-                    "the angular kernel must be 0 (FP64 matrix cores), 1 (FP64 vector), 2 or 3 "
-                    "(fp16 matrix cores, operands split in 2 or 3 slices)");
+                    "the angular kernel must be 0 (FP64 matrix cores), 1 (FP64 vector) or 2 "
+                    "(fp16 matrix cores, operands split in two slices)");
     ctx->angular_kernel = kind;
     return SCATRS_OK;
 }

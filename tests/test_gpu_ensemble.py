@@ -129,11 +129,9 @@ def test_errors_map_like_the_reference():
     assert np.all(np.isfinite(sc))
 
 
-@pytest.mark.parametrize("kernel", [2, 3])
-def test_fp16_split_angular_kernel_within_1e6_of_the_oracle(kernel):
-    """The fp16-split matrix-core angular kernels (scatrs_set_angular_kernel 2 and 3: two and
-    three slices): alpha1 within 1e-6 of max|alpha1| and the phase matrix within 1e-6 of max
-    F11 against the FP64 oracle, on
+def test_fp16_split_angular_kernel_within_1e6_of_the_oracle():
+    """The fp16-split matrix-core angular kernel (scatrs_set_angular_kernel 2): alpha1 within
+    1e-6 of max|alpha1| and the phase matrix within 1e-6 of max F11 against the FP64 oracle, on
     random lognormals and on the largest-N ensembles of the 4096 x 64 workload at its full
     resolution (512 radius nodes, 1024 angles, Greek to 256); the scalars, which do not go
     through the angular products, stay at FP64 parity."""
@@ -146,6 +144,7 @@ def test_fp16_split_angular_kernel_within_1e6_of_the_oracle(kernel):
     rng = np.random.default_rng(31)
     small = list(rng.choice(kind.size, 4, replace=False))
     idx = np.array(big + small)
+    kernel = 2
     ctx.set_angular_kernel(kernel)
     try:
         sc, gr, ph, _ = _gpu(kind[idx], p0[idx], p1[idx], wl[idx], m[idx], n_radius=512,
