@@ -25,7 +25,8 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
     EnsSetup S;
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_wavelength, layers, d_fractions, d_m,
-                        n_radius, n_angles, d_greek != nullptr, order, allow_unvalidated, S)))
+                        n_radius, n_angles, d_greek != nullptr, order, allow_unvalidated,
+                        ctx->angular_kernel, S)))
         return rc;
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;

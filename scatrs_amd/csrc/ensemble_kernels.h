@@ -199,16 +199,22 @@ __host__ __device__ inline int64_t slab_elems(int layers, int64_t rows) {
     return rows * 64 * 2 * layers;
 }
 
-// Writes a node's coefficients to the staging buffer, interleaved (a_1, b_1, a_2, b_2, ...: one
-// 32-byte stream per lane, whose cache lines fill every four orders), and accumulates its
-// efficiency sums.
+// Writes a node's coefficients to the staging buffer, interleaved: in FP64 (a_1, b_1, a_2, b_2,
+// ...: one 32-byte stream per lane, whose cache lines fill every four orders) or, for the
+// fp16-split angular products, in fp32 (one float4 (a.re, a.im, b.re, b.im) per order); and
+// accumulates its efficiency sums.
 struct StagingSink {
     cx *ab_out;
+    float4 *ab32;       // not null: the fp32 form
     EfficiencySink<false> eff;
     double amax = 0.0;  // the largest |Re|, |Im| of the node's coefficients
     __device__ void order(int k, cx a, cx b) {
-        ab_out[2 * (k - 1)] = a;
-        ab_out[2 * (k - 1) + 1] = b;
+        if (ab32) {
+            ab32[k - 1] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+        } else {
+            ab_out[2 * (k - 1)] = a;
+            ab_out[2 * (k - 1) + 1] = b;
+        }
         amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
         eff.order(k, a, b);
     }
@@ -237,13 +243,13 @@ __device__ __forceinline__ int coefficient_exponent(double amax) {
 // quantities to vals, its table factor to tabc, its coefficient exponent to xexp and, if
 // back_out is not null, its backscatter sum sum (2n+1) (-1)^n (a_n - b_n). slab is the lane's
 // column in its wavefront's scratch slab of `rows` rows (slab_elems); tile the lane's column of
-// the wavefront's LDS tile (spheres).
+// the wavefront's LDS tile (spheres). With stage32 the staging buffer holds the fp32 form.
 template <int L>
 __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64_t g,
                                              const int *node_n, const int64_t *node_off,
                                              const int64_t *ens_base, cx *slab, int64_t rows,
-                                             cx *tile, cx *staging, double *vals, double *tabc,
-                                             int *xexp, cx *back_out) {
+                                             cx *tile, cx *staging, int stage32, double *vals,
+                                             double *tabc, int *xexp, cx *back_out) {
     const int64_t e = e0 + g / a.n_radius, i = g % a.n_radius;
     const int64_t gi = e * a.n_radius + i;
     const NodeGeom geo = node_geometry(a, e, i);
@@ -251,6 +257,7 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     const int64_t off = ens_base[e] + node_off[gi];
     StagingSink sink;
     sink.ab_out = staging + 2 * off;
+    sink.ab32 = stage32 ? reinterpret_cast<float4 *>(staging) + off : nullptr;
     if constexpr (L == 1) {
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
         // Checkpoints in the first column, u_k(x) in the second, the block being used in the
@@ -288,7 +295,8 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
 static __global__ __launch_bounds__(256) void ens_nodes_persist(
     EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, cx *scratch,
-    int64_t rows, cx *staging, double *vals, double *tabc, int *xexp, cx *back_out) {
+    int64_t rows, cx *staging, int stage32, double *vals, double *tabc, int *xexp,
+    cx *back_out) {
     __shared__ cx ztile[4][kZBlock * 64];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -302,7 +310,7 @@ static __global__ __launch_bounds__(256) void ens_nodes_persist(
         const int64_t idx = (int64_t)chunk * 64 + lane;
         if (idx < chunk_nodes)
             forward_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile,
-                            staging, vals, tabc, xexp, back_out);
+                            staging, stage32, vals, tabc, xexp, back_out);
     }
 }
 
@@ -316,8 +324,9 @@ __global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, c
                                                         const int *node_n,
                                                         const int64_t *node_off,
                                                         const int64_t *ens_base, cx *scratch,
-                                                        int64_t rows, cx *staging, double *vals,
-                                                        double *tabc, int *xexp, cx *back_out) {
+                                                        int64_t rows, cx *staging, int stage32,
+                                                        double *vals, double *tabc, int *xexp,
+                                                        cx *back_out) {
     __shared__ cx ztile[4][L == 1 ? kZBlock * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -326,7 +335,7 @@ __global__ __launch_bounds__(256) void ens_nodes_ranked(EnsArgs a, int64_t e0, c
     const int64_t idx = start + wave * 64 + lane;
     if (idx < end)
         forward_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile, staging,
-                        vals, tabc, xexp, back_out);
+                        stage32, vals, tabc, xexp, back_out);
 }
 
 // Sort keys of the chunk's nodes: bucket_of(N), N descending.
@@ -584,9 +593,9 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
 // split in two fp16 slices, g = g_0 + g_1 and x = x_0 + x_1 (g_0, x_0 the nearest fp16 values,
 // g_1, x_1 the fp16 values nearest the remainders), and G X = G_0 X_0 + G_0 X_1 + G_1 X_0 up to
 // the dropped G_1 X_1, of order 2^-22 of the terms. The three products of a table run into one
-// fp32 accumulator over two 32-order chunks, which is then added into fp64: its rounding, 2^-24
-// of the two chunks' sum, is the error floor (about 1e-7 of the largest phase-function value
-// over the 4096 x 64 lognormal workload). So that the remainders stay within fp16's normal
+// fp32 accumulator over kF16Run 32-order chunks, which is then added into fp64: its rounding,
+// 2^-24 of the partial sums, sets the error floor (a few 1e-7 of the largest phase-function
+// value over the 4096 x 64 lognormal workload). So that the remainders stay within fp16's normal
 // range, each node's coefficients are scaled by 2^s (s its coefficient exponent: the largest
 // near 2^10) and the table by 2^t (the largest entry, (2n + 1)/2 at most, near 2^15); the node's
 // table factor carries 2^-2(s + t). All scalings are exact.
@@ -604,21 +613,13 @@ typedef _Float16 mfma_h8 __attribute__((ext_vector_type(8)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kF16Orders = 32;   // one v_mfma_f32_16x16x32_f16 K step
+constexpr int kF16Run = 4;       // chunks summed in fp32 before each fp64 flush
 constexpr int kF16Pitch = 40;    // LDS row of 32 orders padded to 40 halves (bank spread)
 
 // The exponent t of the table's scale 2^t: its entries are bounded by (2 n_max + 1)/2.
 static int table_exponent(int n_max) {
     const double gmax = (2.0 * n_max + 1.0) / 2.0;
     return (int)std::floor(std::log2(32768.0 / gmax));
-}
-
-// r to fp16 through fp32: v_cvt_f32_f64, a clamp to fp16's finite range and v_cvt_f16_f32 (a
-// correctly rounded fp64 -> fp16 conversion has no instruction and expands to some 35; the
-// clamp keeps the compiler from merging the two conversions into that one). The rare double
-// rounding moves a slice by at most one fp16 ulp; the split stays exact, since the remainder
-// r - slice is formed in fp64 and carried to the next slice.
-__device__ __forceinline__ _Float16 to_f16(double r) {
-    return (_Float16)__builtin_amdgcn_fmed3f((float)r, -65504.0f, 65504.0f);
 }
 
 // tab16[(2 t + q) * h * npad + jj * npad + n - 1]: slice q of table t (G1, G2) times 2^texp;
@@ -673,7 +674,7 @@ template <int W>
 static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, int slices, int npad,
     int texp, const int *node_n, const int64_t *node_off, const int64_t *ens_base,
-    const cx *staging, const double *tabc, const int *xexp, const _Float16 *tab16,
+    const float4 *staging, const double *tabc, const int *xexp, const _Float16 *tab16,
     double *tab_out) {
     constexpr int P = 8 / W;
     __shared__ _Float16 lds_x[2][2][64][kF16Pitch];  // [buffer][slice][row 4 q + c][order]
@@ -697,7 +698,7 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     const size_t plane = (size_t)h * npad;
     // Columns beyond h read row 0 of the table: finite values whose products are not written.
     const _Float16 *b_col = tab16 + (size_t)(col_ok ? jj : 0) * npad + 8 * kq;
-    const cx *stg = staging + 2 * ens_base[e];
+    const float4 *stg = staging + ens_base[e];
     const int nr = (int)n_radius;
     const int ngroups = (nr + kGroupNodes - 1) / kGroupNodes;
     {
@@ -717,11 +718,11 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
             atomicMax(&gmax[i / kGroupNodes], w_n[i] & 0xffffff);
         __syncthreads();
     }
-    // A slot of staged coefficients: this thread's (node, order) pairs of one item (a_n, b_n
-    // interleaved), loaded from addresses clamped into the ensemble's staged terms, whether they
-    // are real, and the nodes' exponents.
+    // A slot of staged coefficients: this thread's (node, order) pairs of one item (the fp32
+    // (a.re, a.im, b.re, b.im)), loaded from addresses clamped into the ensemble's staged terms,
+    // whether they are real, and the nodes' exponents.
     struct Slot {
-        cx va[P], vb[P];
+        float4 v[P];
         bool in[P];
         int sx[P];
     };
@@ -734,29 +735,28 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
             const int packed = ok ? w_n[node] : 0;
             const int nq = packed & 0xffffff;
             sl.sx[p] = packed >> 24;
-            const cx *cp = stg + 2 * (int64_t)(ok ? w_off[node] : 0);
+            const float4 *cp = stg + (ok ? w_off[node] : 0);
             const int n = it.kc + sk;  // n - 1
             sl.in[p] = n < nq;
-            sl.va[p] = cp[sl.in[p] ? 2 * n : 0];
-            sl.vb[p] = cp[sl.in[p] ? 2 * n + 1 : 0];
+            sl.v[p] = cp[sl.in[p] ? n : 0];
         }
     };
-    // The slot's pairs of item `it` to LDS buffer `buf`: u, v by parity, times 2^s, in two slices.
+    // The slot's pairs of item `it` to LDS buffer `buf`: u, v by parity, times 2^s, in two slices
+    // (in fp32: the scaling and the remainder x - x_0 are exact).
     auto convert = [&](Item it, const Slot &sl, int buf) __attribute__((always_inline)) {
         const bool odd = (it.kc + sk + 1) & 1;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const cx a = sl.in[p] ? sl.va[p] : real(0.0), b = sl.in[p] ? sl.vb[p] : real(0.0);
-            const cx u = odd ? a : b, w = odd ? b : a;
+            const float4 q = sl.in[p] ? sl.v[p] : make_float4(0.f, 0.f, 0.f, 0.f);
             const int e2 = sl.sx[p];
-            const double xs[4] = {ldexp(u.re, e2), ldexp(u.im, e2), ldexp(w.re, e2),
-                                  ldexp(w.im, e2)};
+            const float xs[4] = {ldexpf(odd ? q.x : q.z, e2), ldexpf(odd ? q.y : q.w, e2),
+                                 ldexpf(odd ? q.z : q.x, e2), ldexpf(odd ? q.w : q.y, e2)};
             const int row = 4 * (sq + p * 2 * W);
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const _Float16 x0 = to_f16(xs[c]);
+                const _Float16 x0 = (_Float16)xs[c];
                 lds_x[buf][0][row + c][sk] = x0;
-                lds_x[buf][1][row + c][sk] = to_f16(xs[c] - (double)(float)x0);
+                lds_x[buf][1][row + c][sk] = (_Float16)(xs[c] - (float)x0);
             }
         }
     };
@@ -776,16 +776,11 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
 
     const mfma_f4 zero = {0.f, 0.f, 0.f, 0.f};
     double T[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][2 F11, 2 F12, F33, F34]
-    double acc[4][2][4];                             // [tile][table][component]
-    mfma_f4 run[4][2];                               // fp32 sums of up to two chunks
+    double acc[4][2][4];  // [tile][table][component]: the group's sums so far (set by its first flush)
+    mfma_f4 run[4][2];    // fp32 sums of up to kF16Run chunks
+    bool fresh = true;    // no flush yet in this group
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-        for (int tb = 0; tb < 2; ++tb) {
-            run[tt][tb] = zero;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
-        }
+    for (int tt = 0; tt < 4; ++tt) run[tt][0] = run[tt][1] = zero;
     // Item cursors: c0 is computed, c1 converted (its table slices loaded), c2 waits in the
     // other slot, c3 is loaded into the slot that c1 frees.
     Item c0{0, 0};
@@ -822,15 +817,26 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
                 run[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x1, gc[tb][0], r, 0, 0, 0);
             }
         }
-        if (!same || ((c0.kc / kF16Orders) & 1)) {  // every second chunk and the group's last
+        // Into fp64 every kF16Run chunks and at the group's end; the group's first flush sets.
+        if (!same || (c0.kc / kF16Orders) % kF16Run == kF16Run - 1) {
+            if (fresh) {
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt)
+                for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-                for (int tb = 0; tb < 2; ++tb) {
+                    for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)run[tt][tb][v];
-                    run[tt][tb] = zero;
-                }
+                        for (int v = 0; v < 4; ++v) acc[tt][tb][v] = (double)run[tt][tb][v];
+            } else {
+#pragma unroll
+                for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[tt][tb][v] += (double)run[tt][tb][v];
+            }
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt) run[tt][0] = run[tt][1] = zero;
+            fresh = false;
         }
         if (!same) {
             // Epilogue of the group: node 4 tt + kq at cosine jj.
@@ -840,10 +846,6 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
                 const double cfac = node < nr ? w_cf[node] : 0.0;
                 const cx A = mk(acc[tt][0][0], acc[tt][0][1]), C = mk(acc[tt][0][2], acc[tt][0][3]);
                 const cx D = mk(acc[tt][1][0], acc[tt][1][1]), B = mk(acc[tt][1][2], acc[tt][1][3]);
-#pragma unroll
-                for (int tb = 0; tb < 2; ++tb)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) acc[tt][tb][v] = 0.0;
 #pragma unroll
                 for (int sgn = 0; sgn < 2; ++sgn) {
                     const cx s1 = sgn == 0 ? A + B : A - B;
@@ -856,6 +858,7 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
                     T[sgn][3] += cfac * pp.im;
                 }
             }
+            fresh = true;
         }
         c0 = c1;
         c1 = c2;
@@ -1155,13 +1158,14 @@ struct EnsSetup {
     _Float16 *tab16 = nullptr;  // the fp16-split table (angular kernel 2)
     int npad = 0;               // its rows' length: n_max rounded up to 32
     int texp = 0;               // its scale 2^texp
+    int kernel = 0;             // the angular kernel of the call (2: fp32 staging)
 };
 
 static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, const double *d_p0,
                      const double *d_p1, const double *d_wavelength, int layers,
                      const double *d_fractions, const double *d_m, int64_t n_radius,
                      int64_t n_angles, bool greek, int64_t order, int allow_unvalidated,
-                     EnsSetup &S) {
+                     int kernel, EnsSetup &S) {
     int rc;
     if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
     hipStream_t s = ctx->stream;
@@ -1231,7 +1235,8 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
         return rc;
     S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
     pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
-    if (ctx->angular_kernel == 2 && n_radius <= kF16MaxNodes) {
+    S.kernel = kernel == 2 && n_radius > kF16MaxNodes ? 0 : kernel;
+    if (S.kernel == 2) {
         S.npad = (S.n_max + kF16Orders - 1) / kF16Orders * kF16Orders;
         S.texp = table_exponent(S.n_max);
         const size_t cells = (size_t)h * S.npad;
@@ -1312,11 +1317,11 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
                 a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
-                q.rows, B.staging, B.vals, B.tabc, B.xexp, back_out);
+                q.rows, B.staging, S.kernel == 2, B.vals, B.tabc, B.xexp, back_out);
         } else {
             ens_nodes_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
                 a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch,
-                q.rows, B.staging, B.vals, B.tabc, B.xexp, back_out);
+                q.rows, B.staging, S.kernel == 2, B.vals, B.tabc, B.xexp, back_out);
         }
     }
 }
@@ -1396,10 +1401,10 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     const int layers = S.a.layers;
     const int64_t nodes = ne * n_radius;
     const int kstg = scatrs_ctx::kStaging, knod = scatrs_ctx::kNodes, kacc = scatrs_ctx::kAccum;
+    const size_t term_bytes = S.kernel == 2 ? sizeof(float4) : 2 * sizeof(cx);
     size_t scratch_bytes = 0;
     if ((rc = plan_nodes(ctx, S, e0, nodes, B, &scratch_bytes, s))) return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, kstg,
-                                    (size_t)std::max<int64_t>(chunk_terms, 1) * 2 * sizeof(cx)),
+    if ((rc = check_hip(ctx, ensure(ctx, kstg, (size_t)std::max<int64_t>(chunk_terms, 1) * term_bytes),
                         "staging")))
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, knod,
@@ -1447,17 +1452,18 @@ static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                            int64_t n_radius, int64_t n_angles, ChunkBufs &B, hipStream_t s) {
     const int h = S.h;
     mark_begin(ctx, s);
-    if (ctx->angular_kernel == 1) {
+    if (S.kernel == 1) {
         const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
         ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,
                                                 S.node_off, S.ens_base, B.staging, B.tabc,
                                                 S.table, B.tab_out);
-    } else if (ctx->angular_kernel == 2 && n_radius <= kF16MaxNodes) {
+    } else if (S.kernel == 2) {
         constexpr int W = kF16Waves;
         const int slices = (h + 16 * W - 1) / (16 * W);
         ens_angular_f16<W><<<(unsigned)(ne * slices), 64 * W, f16_lds_bytes(n_radius), s>>>(
             e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.texp, S.node_n, S.node_off,
-            S.ens_base, B.staging, B.tabc, B.xexp, S.tab16, B.tab_out);
+            S.ens_base, reinterpret_cast<const float4 *>(B.staging), B.tabc, B.xexp, S.tab16,
+            B.tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
