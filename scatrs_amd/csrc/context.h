@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 struct scatrs_ctx {
     int device = 0;
@@ -38,6 +39,10 @@ struct scatrs_ctx {
     void *buf[kNumBufs] = {};
     size_t cap[kNumBufs] = {};
     void *host_pinned = nullptr;  // small pinned status block
+    // Ensemble setup kept between calls of the same sizes: the Gauss-Legendre rules in radius
+    // and angle (device copies in kMisc, kMisc2) and the Wigner d table (kMisc3).
+    int64_t rule_nr = -1, rule_na = -1, dtab_na = -1, dtab_order = -1;
+    std::vector<double> rr_nodes, rr_weights, ar_nodes, ar_weights;
     bool profiling = false;
     int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2: fp16-split MFMA
     static constexpr int kMaxEvents = 1024;

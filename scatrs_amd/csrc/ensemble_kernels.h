@@ -206,10 +206,17 @@ __host__ __device__ inline int64_t slab_elems(int layers, int64_t rows) {
 struct StagingSink {
     cx *ab_out;
     float4 *ab32;       // not null: the fp32 form
+    // Spheres' fp32 form: each order's float4 goes to the lane's slot of the order loop's LDS
+    // tile (its u_{k+1}(mx) is consumed by then) and the block's orders go out together at its
+    // end, 16 B x 16 contiguous per lane, so the staging buffer receives whole cache lines.
+    float4 *tile32 = nullptr;
     EfficiencySink<false> eff;
     double amax = 0.0;  // the largest |Re|, |Im| of the node's coefficients
     __device__ void order(int k, cx a, cx b) {
-        if (ab32) {
+        if (tile32) {
+            tile32[((k - 1) % kZBlock) * 64] =
+                make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+        } else if (ab32) {
             ab32[k - 1] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
         } else {
             ab_out[2 * (k - 1)] = a;
@@ -217,6 +224,10 @@ struct StagingSink {
         }
         amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
         eff.order(k, a, b);
+    }
+    __device__ void block_end(int k0, int kend) {
+        if (!tile32) return;
+        for (int k = k0; k <= kend; ++k) ab32[k - 1] = tile32[(k - k0) * 64];
     }
     __device__ void order_d(int, cx, cx, cx, cx, cx, cx) {}
     template <class O>
@@ -258,6 +269,9 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     StagingSink sink;
     sink.ab_out = staging + 2 * off;
     sink.ab32 = stage32 ? reinterpret_cast<float4 *>(staging) + off : nullptr;
+    if constexpr (L == 1) {
+        if (stage32) sink.tile32 = reinterpret_cast<float4 *>(tile);
+    }
     if constexpr (L == 1) {
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
         // Checkpoints in the first column, u_k(x) in the second, the block being used in the
@@ -1169,18 +1183,33 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     int rc;
     if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
     hipStream_t s = ctx->stream;
-    gauss_legendre(n_radius, S.rr.nodes, S.rr.weights);
-    gauss_legendre(n_angles, S.ar.nodes, S.ar.weights);
-    std::vector<double> rtab(2 * n_radius);
-    std::copy(S.rr.nodes.begin(), S.rr.nodes.end(), rtab.begin());
-    std::copy(S.rr.weights.begin(), S.rr.weights.end(), rtab.begin() + n_radius);
     const int h = (int)((n_angles + 1) / 2);
     S.h = h;
-    std::vector<double> atab(n_angles + h);  // weights, then the non-negative cosines
-    std::copy(S.ar.weights.begin(), S.ar.weights.end(), atab.begin());
-    for (int jj = 0; jj < h; ++jj) atab[n_angles + jj] = S.ar.nodes[n_angles - h + jj];
-    if ((rc = upload(ctx, scatrs_ctx::kMisc, rtab, &S.d_rtab))) return rc;
-    if ((rc = upload(ctx, scatrs_ctx::kMisc2, atab, &S.d_atab))) return rc;
+    // The Gauss-Legendre rules (device copies in kMisc, kMisc2) and the Wigner d table (kMisc3)
+    // are kept in the context between calls of the same sizes.
+    if (ctx->rule_nr != n_radius || ctx->rule_na != n_angles) {
+        ctx->rule_nr = ctx->rule_na = -1;
+        gauss_legendre(n_radius, ctx->rr_nodes, ctx->rr_weights);
+        gauss_legendre(n_angles, ctx->ar_nodes, ctx->ar_weights);
+        std::vector<double> rtab(2 * n_radius);
+        std::copy(ctx->rr_nodes.begin(), ctx->rr_nodes.end(), rtab.begin());
+        std::copy(ctx->rr_weights.begin(), ctx->rr_weights.end(), rtab.begin() + n_radius);
+        std::vector<double> atab(n_angles + h);  // weights, then the non-negative cosines
+        std::copy(ctx->ar_weights.begin(), ctx->ar_weights.end(), atab.begin());
+        for (int jj = 0; jj < h; ++jj) atab[n_angles + jj] = ctx->ar_nodes[n_angles - h + jj];
+        double *dr = nullptr, *da = nullptr;
+        if ((rc = upload(ctx, scatrs_ctx::kMisc, rtab, &dr))) return rc;
+        if ((rc = upload(ctx, scatrs_ctx::kMisc2, atab, &da))) return rc;
+        ctx->rule_nr = n_radius;
+        ctx->rule_na = n_angles;
+        ctx->dtab_na = ctx->dtab_order = -1;
+    }
+    S.rr.nodes = ctx->rr_nodes;
+    S.rr.weights = ctx->rr_weights;
+    S.ar.nodes = ctx->ar_nodes;
+    S.ar.weights = ctx->ar_weights;
+    S.d_rtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc]);
+    S.d_atab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc2]);
     S.a = EnsArgs{d_kind, d_p0, d_p1, d_wavelength, d_fractions, d_m, layers, n_radius,
                   S.d_rtab, S.d_rtab + n_radius};
     // Per-node N and status.
@@ -1248,14 +1277,21 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                                                                      S.texp, S.tab16);
     }
     if (greek) {
-        const int64_t len = order + 1;
-        std::vector<double> dt(4 * (size_t)n_angles * len);
-        const int pm[4][2] = {{0, 0}, {2, 2}, {2, -2}, {0, 2}};
-        for (int p = 0; p < 4; ++p)
-            for (int64_t j = 0; j < n_angles; ++j)
-                wigner_d_small(pm[p][0], pm[p][1], S.ar.nodes[j], order,
-                               dt.data() + ((size_t)p * n_angles + j) * len);
-        if ((rc = upload(ctx, scatrs_ctx::kMisc3, dt, &S.d_dtab))) return rc;
+        if (ctx->dtab_na != n_angles || ctx->dtab_order != order) {
+            ctx->dtab_na = ctx->dtab_order = -1;
+            const int64_t len = order + 1;
+            std::vector<double> dt(4 * (size_t)n_angles * len);
+            const int pm[4][2] = {{0, 0}, {2, 2}, {2, -2}, {0, 2}};
+            for (int p = 0; p < 4; ++p)
+                for (int64_t j = 0; j < n_angles; ++j)
+                    wigner_d_small(pm[p][0], pm[p][1], S.ar.nodes[j], order,
+                                   dt.data() + ((size_t)p * n_angles + j) * len);
+            double *dd = nullptr;
+            if ((rc = upload(ctx, scatrs_ctx::kMisc3, dt, &dd))) return rc;
+            ctx->dtab_na = n_angles;
+            ctx->dtab_order = order;
+        }
+        S.d_dtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc3]);
     }
     return SCATRS_OK;
 }
@@ -1297,16 +1333,16 @@ constexpr size_t kSlabBudget = (size_t)4 << 30;
 constexpr int64_t kMaxNodeWaves = 16384;
 constexpr size_t kSphereSlabBudget = (size_t)16 << 30;
 
-// Staging buffer of a chunk's coefficients (a_n, b_n: 32 B per term); it bounds the chunk size.
-// Every chunk waits for its slowest node (the serial recurrence of the largest N), so chunks
-// are made as large as memory allows: 1/`share` of the device's free memory, at most 48 GiB and
-// at least 1 GiB. Returns the staged terms a chunk may hold.
-static int64_t chunk_terms_limit(int share) {
+// Staging buffer of a chunk's coefficients (term_bytes per term: 32 in FP64, 16 in fp32); it
+// bounds the chunk size. Every chunk waits for its slowest node (the serial recurrence of the
+// largest N), so chunks are made as large as memory allows: 1/`share` of the device's free
+// memory, at most 64 GiB and at least 1 GiB. Returns the staged terms a chunk may hold.
+static int64_t chunk_terms_limit(int share, size_t term_bytes) {
     size_t free_b = 0, total_b = 0;
     size_t bytes = (size_t)12 << 30;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
-        bytes = std::min<size_t>((size_t)48 << 30, std::max<size_t>((size_t)1 << 30, free_b / share));
-    return (int64_t)(bytes / (2 * sizeof(cx)));
+        bytes = std::min<size_t>((size_t)64 << 30, std::max<size_t>((size_t)1 << 30, free_b / share));
+    return (int64_t)(bytes / term_bytes);
 }
 
 template <int L>

@@ -631,7 +631,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     const int h = S.h;
     const bool tables = d_bar_greek || d_bar_phase;
     // Staging and cotangents take 32 B per term each.
-    const int64_t max_terms = chunk_terms_limit(6);  // staging and its cotangents
+    const int64_t max_terms = chunk_terms_limit(6, 2 * sizeof(cx));  // staging and its cotangents
     const unsigned vjp_blocks = (unsigned)std::max(1, ctx->cus * 2);
     int64_t e0 = 0;
     while (e0 < count) {

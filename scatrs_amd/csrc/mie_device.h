@@ -10,6 +10,8 @@
 // (the g pair term is added when order k+1 arrives).
 #pragma once
 
+#include <type_traits>
+
 #include "cx.h"
 
 namespace scatrs {
@@ -211,6 +213,12 @@ __device__ __forceinline__ double absorption_term(cx f, cx df, cx l, double n2) 
     return -((df.im - 2.0 * f.im * l.re) * n2);
 }
 
+// Whether a sink wants a call at the end of each block of kZBlock orders (block_end(k0, kend)).
+template <class S, class = void>
+struct has_block_end : std::false_type {};
+template <class S>
+struct has_block_end<S, std::void_t<decltype(&S::block_end)>> : std::true_type {};
+
 // The upward order loop (scatrs/src/mie/kernel.rs:325-559) after ratio_chain and x_chain.
 // Calls sink.order(k, a, b) (or sink.order_d with the four derivatives) for k = 1..n and
 // sink.done(abs_sum, jac). With a layered Outer source the per-order sides replace D_n(mx);
@@ -376,6 +384,8 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             const int k0 = 1 + j * kZBlock;
             const int kend = n < k0 + kZBlock - 1 ? n : k0 + kZBlock - 1;
             for (int k = k0; k <= kend; ++k) order_step(k, zs.lds[(k - k0) * 64]);
+            // A sink may use the block's tile slots, free once their order is done.
+            if constexpr (has_block_end<Sink>::value) sink.block_end(k0, kend);
         }
     } else {
         // Read two orders ahead with one prefetch buffer per order of a pair, so every load
