@@ -734,10 +734,10 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     }
     // A slot of staged coefficients: this thread's (node, order) pairs of one item (the fp32
     // (a.re, a.im, b.re, b.im)), loaded from addresses clamped into the ensemble's staged terms,
-    // whether they are real, and the nodes' exponents.
+    // and the nodes' exponents; a pair beyond its node's N (or the ensemble's nodes) gets the
+    // exponent -200, which takes its finite values to exactly zero in fp32.
     struct Slot {
         float4 v[P];
-        bool in[P];
         int sx[P];
     };
     auto fetch = [&](Item it, Slot &sl) __attribute__((always_inline)) {
@@ -748,11 +748,11 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
             const bool ok = node < nr;
             const int packed = ok ? w_n[node] : 0;
             const int nq = packed & 0xffffff;
-            sl.sx[p] = packed >> 24;
             const float4 *cp = stg + (ok ? w_off[node] : 0);
             const int n = it.kc + sk;  // n - 1
-            sl.in[p] = n < nq;
-            sl.v[p] = cp[sl.in[p] ? n : 0];
+            const bool in = n < nq;
+            sl.sx[p] = in ? packed >> 24 : -200;
+            sl.v[p] = cp[in ? n : 0];
         }
     };
     // The slot's pairs of item `it` to LDS buffer `buf`: u, v by parity, times 2^s, in two slices
@@ -761,7 +761,7 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
         const bool odd = (it.kc + sk + 1) & 1;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            const float4 q = sl.in[p] ? sl.v[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 q = sl.v[p];
             const int e2 = sl.sx[p];
             const float xs[4] = {ldexpf(odd ? q.x : q.z, e2), ldexpf(odd ? q.y : q.w, e2),
                                  ldexpf(odd ? q.z : q.x, e2), ldexpf(odd ? q.w : q.y, e2)};
@@ -806,20 +806,19 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
     fetch(c2, sb);
     mfma_h8 gA[2][2], gB[2][2];
     table(c0.kc, gA);
-    // One step: c1 to LDS buffer bn from slot `sl`, c1's table slices into gn, c3's loads into
-    // `sl`, then c0's products from buffer bc with the table slices gc and, at the end of a
-    // group, the epilogue. The loop alternates the slots, the table registers and the LDS
-    // buffers, so that no copy waits for a load in flight and one barrier per step suffices:
-    // the buffer written here was last read one step ago, before every wavefront passed this
-    // step's barrier.
+    // One step: c1's table slices into gn, c0's products from LDS buffer bc with the table
+    // slices gc alongside c1's split from slot `sl` into buffer bn, c3's loads into `sl` and, at
+    // the end of a group, the epilogue. The loop alternates the slots, the table registers and
+    // the LDS buffers, so that no copy waits for a load in flight and one barrier per step
+    // suffices: buffer bn was last read one step ago, before every wavefront passed this step's
+    // barrier, and buffer bc was written then.
     auto step = [&](Slot &sl, mfma_h8 (&gc)[2][2], mfma_h8 (&gn)[2][2], int bc, int bn)
                     __attribute__((always_inline)) {
         __syncthreads();
-        convert(c1, sl, bn);
         table(c1.g < nr ? c1.kc : 0, gn);
-        fetch(c3, sl);
-        __builtin_amdgcn_sched_barrier(0);  // keep the products behind those loads
+        __builtin_amdgcn_sched_barrier(0);  // the table slices go out before anything else
         const bool same = c1.g == c0.g;
+        // c0's products and c1's split into the other LDS buffer.
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
             const mfma_h8 x0 = *reinterpret_cast<const mfma_h8 *>(&lds_x[bc][0][16 * tt + col][8 * kq]);
@@ -831,6 +830,9 @@ static __global__ __launch_bounds__(64 * W) void ens_angular_f16(
                 run[tt][tb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(x1, gc[tb][0], r, 0, 0, 0);
             }
         }
+        convert(c1, sl, bn);
+        __builtin_amdgcn_sched_barrier(0);  // the slot is read before it is refilled
+        fetch(c3, sl);
         // Into fp64 every kF16Run chunks and at the group's end; the group's first flush sets.
         if (!same || (c0.kc / kF16Orders) % kF16Run == kF16Run - 1) {
             if (fresh) {
@@ -992,7 +994,7 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
 // 64 orders, a wavefront 16 orders. Then alpha1 = c s11, alpha2, alpha3 = c (sp +- sm)/2,
 // alpha4 = c s33, beta1 = c s12, beta2 = c s34 with c = (2l + 1)/2 gfac[e]; alpha1_0 = 1 exactly
 // when normalized. dtab[(p * n_angles + j) * len + l], p = d00, d22, d2-2, d02.
-constexpr int kGreekAngles = 32;  // angles of the A tile staged per step
+constexpr int kGreekAngles = 64;  // angles of the A tile staged per step
 
 static __global__ __launch_bounds__(256) void ens_greek_mfma(int64_t e0, int64_t ne, int n_angles,
                                                             const double *aw, const double *tab,
@@ -1015,9 +1017,9 @@ static __global__ __launch_bounds__(256) void ens_greek_mfma(int64_t e0, int64_t
     for (int k = 0; k < 6; ++k) acc[k] = mfma_d4{0, 0, 0, 0};
     for (int j0 = 0; j0 < n_angles; j0 += kGreekAngles) {
         __syncthreads();
-        // Thread t stages (ensemble t / 16, angles t % 16 and t % 16 + 16) of the tile.
+        // Thread t stages (ensemble t / 16, angles t % 16 + 16 h) of the tile.
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < kGreekAngles / 16; ++h) {
             const int er = t / 16, ja = t % 16 + 16 * h;
             const int64_t e = et + er;
             const int j = j0 + ja;
@@ -1038,7 +1040,7 @@ static __global__ __launch_bounds__(256) void ens_greek_mfma(int64_t e0, int64_t
             lds_a[5][er][ja] = f34;
         }
         __syncthreads();
-#pragma unroll 2
+#pragma unroll 4
         for (int jj = 0; jj < kGreekAngles; jj += 4) {
             const int ja = jj + kq;
             const int j = j0 + ja;
