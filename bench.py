@@ -256,16 +256,33 @@ def bench_adjoint(args, torch):
                                          n_angles=na, order=order)
         return g
 
+    def run(kernel):
+        ctx.set_angular_kernel(kernel)
+        try:
+            step()
+            return _timed(torch, ctx, step, args.ens_steps)
+        finally:
+            ctx.set_angular_kernel(0)
+
     log(f"adjoint: {kind.size} ensembles (1024 lognormals x 32 wavelengths), warm-up")
-    step()
-    sec, kt, g = _timed(torch, ctx, step, args.ens_steps)
+    # Fused: the forward tables (forward call and reverse pass) from the fp16-split kernel; the
+    # reverse products in FP64 on the matrix cores.
+    sec, _, g = run(2)
+    log(f"adjoint (fp16-split forward tables): {sec:.3f} s per forward + VJP step")
+    sec64, _, g64 = run(0)
+    log(f"adjoint (FP64 matrix cores throughout): {sec64:.3f} s per forward + VJP step")
+    gs = g64.abs().amax(dim=0).clamp_min(1e-300)
+    gdiff = float(((g - g64).abs().amax(dim=0) / gs).max().item())
     res = {"workload": "first 1024 of the 4096 lognormals x every other wavelength (32); "
                        "512 radius nodes, 1024 angles, Greek to 256; loss sum w[(ssa-t)^2 + "
                        "(g-t)^2 + sum_{l<64} (alpha1_l-t_l)^2]; step = forward + VJP w.r.t. "
                        "r_g, sigma_g, Re m, Im m",
            "ensembles": int(kind.size), "ensembles_per_s": kind.size / sec, "s_per_step": sec,
+           "angular": "fp16-split forward tables, FP64 matrix-core reverse products",
+           "fp64_matrix_cores": {"ensembles_per_s": kind.size / sec64, "s_per_step": sec64},
+           "speedup_vs_fp64_matrix_cores": sec64 / sec,
+           "grad_max_diff_vs_fp64": gdiff,
            "finite": bool(torch.isfinite(g).all().item())}
-    log(f"adjoint: {sec:.3f} s per forward + VJP step")
     if not args.no_cpu_baseline:
         import oracle
 
@@ -290,7 +307,7 @@ def bench_layered(args, torch):
 
     nr, na, order = 256, 512, 128
     ctx = _native.context(0)
-    total_s, total_e, total_terms = 0.0, 0, 0
+    total_s, total64, total_e, total_terms = 0.0, 0.0, 0, 0
     per_l = {}
     for L, (kind, p0, p1, wl, fr, m) in sorted(workloads.layered().items()):
         t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
@@ -306,20 +323,33 @@ def bench_layered(args, torch):
                                              n_angles=na, order=order)
             return terms
 
-        step()
-        sec, _, terms = _timed(torch, ctx, step, 1)
-        per_l[str(L)] = {"ensembles": int(kind.size), "s": sec, "terms": int(terms)}
+        secs = {}
+        for kernel in (2, 0):  # fp16-split forward tables, then FP64 throughout
+            ctx.set_angular_kernel(kernel)
+            try:
+                step()
+                secs[kernel], _, terms = _timed(torch, ctx, step, 1)
+            finally:
+                ctx.set_angular_kernel(0)
+        sec = secs[2]
+        per_l[str(L)] = {"ensembles": int(kind.size), "s": sec, "s_fp64_matrix_cores": secs[0],
+                         "terms": int(terms)}
         total_s += sec
+        total64 += secs[0]
         total_e += kind.size
         total_terms += terms
-        log(f"layered L={L}: {kind.size} ensembles, {sec:.3f} s forward + VJP")
+        log(f"layered L={L}: {kind.size} ensembles, {sec:.3f} s forward + VJP "
+            f"({secs[0]:.3f} s in FP64 throughout)")
     return {"workload": "2048 gamma distributions (r_eff log-uniform [0.05, 3] um, v_eff U[0.05, "
                         "0.4]) of a core and 1-4 shells (fractions Dirichlet(2), Re m U[1.3, 2.0], "
                         "Im m log-uniform [1e-6, 1]) x 32 wavelengths; 256 radius nodes, 512 "
                         "angles, Greek to 128; step = forward + VJP (all layer indices and "
                         "radius fractions)",
             "ensembles": total_e, "terms": int(total_terms), "s_per_step": total_s,
-            "ensembles_per_s": total_e / total_s, "per_layer_count": per_l}
+            "ensembles_per_s": total_e / total_s,
+            "angular": "fp16-split forward tables, FP64 matrix-core reverse products",
+            "fp64_matrix_cores": {"s_per_step": total64, "ensembles_per_s": total_e / total64},
+            "per_layer_count": per_l}
 
 
 def main() -> int:

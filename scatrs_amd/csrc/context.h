@@ -34,6 +34,7 @@ struct scatrs_ctx {
         kOrder,       // ensembles: per-node sort keys and the nodes ranked by N, histogram
         kTables16,    // ensembles: the w pi_n, w tau_n table split in fp16 slices
         kGreekFac,    // ensembles: per-ensemble factor of the Greek coefficients
+        kStaging32,   // reverse mode: the fp32 form of the staged coefficients (kernel 2)
         kNumBufs
     };
     void *buf[kNumBufs] = {};

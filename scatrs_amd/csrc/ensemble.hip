@@ -30,8 +30,8 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         return rc;
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
-    // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B).
-    const int64_t max_terms = chunk_terms_limit(4, S.kernel == 2 ? sizeof(float4) : 2 * sizeof(cx));
+    // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B, 16 B in fp32).
+    const int64_t max_terms = chunk_terms_limit(4, S.stage32 ? sizeof(float4) : 2 * sizeof(cx));
     int64_t e0 = 0;
     while (e0 < count) {
         int64_t chunk_terms = 0;

@@ -1163,6 +1163,7 @@ struct EnsSetup {
     Rule rr, ar;
     int h = 0;
     double *d_rtab = nullptr, *d_atab = nullptr, *d_dtab = nullptr;
+    double *d_dtab_t = nullptr;  // the Wigner d table transposed: [(p * len + l) * n_angles + j]
     EnsArgs a{};
     unsigned long long *counter = nullptr;
     int *node_n = nullptr;
@@ -1174,7 +1175,8 @@ struct EnsSetup {
     _Float16 *tab16 = nullptr;  // the fp16-split table (angular kernel 2)
     int npad = 0;               // its rows' length: n_max rounded up to 32
     int texp = 0;               // its scale 2^texp
-    int kernel = 0;             // the angular kernel of the call (2: fp32 staging)
+    int kernel = 0;             // the angular kernel of the call
+    bool stage32 = false;       // the node solves stage the fp32 form (kernel 2, forward calls)
 };
 
 static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, const double *d_p0,
@@ -1267,6 +1269,7 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
     pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
     S.kernel = kernel == 2 && n_radius > kF16MaxNodes ? 0 : kernel;
+    S.stage32 = S.kernel == 2;
     if (S.kernel == 2) {
         S.npad = (S.n_max + kF16Orders - 1) / kF16Orders * kF16Orders;
         S.texp = table_exponent(S.n_max);
@@ -1282,18 +1285,25 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
         if (ctx->dtab_na != n_angles || ctx->dtab_order != order) {
             ctx->dtab_na = ctx->dtab_order = -1;
             const int64_t len = order + 1;
-            std::vector<double> dt(4 * (size_t)n_angles * len);
+            // Both layouts: angle-major for the Greek projection, order-major (after it) for
+            // its transpose in reverse mode.
+            const size_t cells = 4 * (size_t)n_angles * len;
+            std::vector<double> dt(2 * cells);
             const int pm[4][2] = {{0, 0}, {2, 2}, {2, -2}, {0, 2}};
             for (int p = 0; p < 4; ++p)
-                for (int64_t j = 0; j < n_angles; ++j)
-                    wigner_d_small(pm[p][0], pm[p][1], S.ar.nodes[j], order,
-                                   dt.data() + ((size_t)p * n_angles + j) * len);
+                for (int64_t j = 0; j < n_angles; ++j) {
+                    double *row = dt.data() + ((size_t)p * n_angles + j) * len;
+                    wigner_d_small(pm[p][0], pm[p][1], S.ar.nodes[j], order, row);
+                    for (int64_t l = 0; l < len; ++l)
+                        dt[cells + ((size_t)p * len + l) * n_angles + j] = row[l];
+                }
             double *dd = nullptr;
             if ((rc = upload(ctx, scatrs_ctx::kMisc3, dt, &dd))) return rc;
             ctx->dtab_na = n_angles;
             ctx->dtab_order = order;
         }
         S.d_dtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc3]);
+        S.d_dtab_t = S.d_dtab + 4 * n_angles * (order + 1);
     }
     return SCATRS_OK;
 }
@@ -1322,6 +1332,7 @@ struct NodeLaunch {
 // Buffers of one forward chunk.
 struct ChunkBufs {
     cx *staging = nullptr;
+    const float4 *staging32 = nullptr;  // the fp32 form the fp16-split kernel reads
     double *vals = nullptr, *tabc = nullptr, *tab_out = nullptr;
     int *xexp = nullptr;             // per node: its coefficient exponent
     int *order = nullptr;            // the chunk's nodes ranked by N, largest first
@@ -1355,11 +1366,11 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
                 a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
-                q.rows, B.staging, S.kernel == 2, B.vals, B.tabc, B.xexp, back_out);
+                q.rows, B.staging, S.stage32, B.vals, B.tabc, B.xexp, back_out);
         } else {
             ens_nodes_ranked<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
                 a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch,
-                q.rows, B.staging, S.kernel == 2, B.vals, B.tabc, B.xexp, back_out);
+                q.rows, B.staging, S.stage32, B.vals, B.tabc, B.xexp, back_out);
         }
     }
 }
@@ -1439,7 +1450,7 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     const int layers = S.a.layers;
     const int64_t nodes = ne * n_radius;
     const int kstg = scatrs_ctx::kStaging, knod = scatrs_ctx::kNodes, kacc = scatrs_ctx::kAccum;
-    const size_t term_bytes = S.kernel == 2 ? sizeof(float4) : 2 * sizeof(cx);
+    const size_t term_bytes = S.stage32 ? sizeof(float4) : 2 * sizeof(cx);
     size_t scratch_bytes = 0;
     if ((rc = plan_nodes(ctx, S, e0, nodes, B, &scratch_bytes, s))) return rc;
     if ((rc = check_hip(ctx, ensure(ctx, kstg, (size_t)std::max<int64_t>(chunk_terms, 1) * term_bytes),
@@ -1460,6 +1471,7 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                         "base copy")))
         return rc;
     B.staging = static_cast<cx *>(ctx->buf[kstg]);
+    B.staging32 = S.stage32 ? reinterpret_cast<const float4 *>(B.staging) : nullptr;
     B.vals = static_cast<double *>(ctx->buf[knod]);
     B.tabc = B.vals + nodes * kNodeVals;
     B.xexp = reinterpret_cast<int *>(B.tabc + nodes);
@@ -1483,8 +1495,18 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     return check_hip(ctx, hipGetLastError(), "ensemble launch");
 }
 
+// The fp32 form (a.re, a.im, b.re, b.im per term) of FP64-staged coefficients, for the
+// fp16-split angular kernel where the FP64 coefficients are kept as well (reverse mode).
+static __global__ void stage_fp32(const cx *staging, int64_t terms, float4 *out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < terms;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const cx a = staging[2 * i], b = staging[2 * i + 1];
+        out[i] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+    }
+}
+
 // The angular products of the chunk [e0, e0 + ne) on stream s from the buffers forward_nodes
-// filled: afterwards B.tab_out holds the ensembles' summed tables [F11, F12, F33, F34] x
+// filled (B.staging32 with kernel 2): afterwards B.tab_out holds the ensembles' summed tables [F11, F12, F33, F34] x
 // n_angles.
 static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                            int64_t n_radius, int64_t n_angles, ChunkBufs &B, hipStream_t s) {
@@ -1500,8 +1522,7 @@ static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         const int slices = (h + 16 * W - 1) / (16 * W);
         ens_angular_f16<W><<<(unsigned)(ne * slices), 64 * W, f16_lds_bytes(n_radius), s>>>(
             e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.texp, S.node_n, S.node_off,
-            S.ens_base, reinterpret_cast<const float4 *>(B.staging), B.tabc, B.xexp, S.tab16,
-            B.tab_out);
+            S.ens_base, B.staging32, B.tabc, B.xexp, S.tab16, B.tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(

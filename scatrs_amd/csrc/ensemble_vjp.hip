@@ -8,8 +8,10 @@
 // lambda^2 / pi). Every cross section is lambda^2 times a function of a_n, b_n alone (C_abs of
 // the absorption sum), so a node's share phi_i = W_i H_i depends on its radius only through
 // the coefficients, r^2, r^3 and the weight. Per chunk:
-//   forward_chunk    - the forward node solves and angular products (coefficients staged,
-//                      per-node backscatter sums kept);
+//   forward_nodes, forward_angular - the forward node solves and angular products
+//                      (coefficients staged in FP64, per-node backscatter sums kept; with the
+//                      fp16-split kernel the products read an fp32 copy, stage_fp32);
+//   ens_greek_vjp_mfma - the transposed Greek projection of the Greek cotangents;
 //   ens_finish_vjp   - per ensemble, the reverse of ens_finish: Sbar (8) and Tbar (4 x angles);
 //   ens_angular_vjp  - per (ensemble, 16 nodes): S1, S2 at every cosine again on the FP64
 //                      matrix cores, the cotangents of S1, S2 from Tbar, and the transposed
@@ -57,7 +59,6 @@ static __global__ __launch_bounds__(256) void ens_finish_vjp(
     double *S = sh;      // 8 sums
     double *F = sh + 8;  // 4 * n_angles, divided by Z
     __shared__ double red[256];
-    __shared__ double a0_sh;
     const int64_t e = e0 + blockIdx.x;
     const int t = threadIdx.x;
     const double *v = vals + blockIdx.x * n_radius * kNodeVals;
@@ -76,77 +77,37 @@ static __global__ __launch_bounds__(256) void ens_finish_vjp(
     const double *F11 = F, *F12 = F + n_angles, *F33 = F + 2 * n_angles, *F34 = F + 3 * n_angles;
     const int64_t len = order + 1;
     const bool greek = bar_greek && dtab;
-    const double *gb = greek ? bar_greek + e * 6 * len : nullptr;
-    // alpha1_0 and, normalized, the cotangent it receives through the division.
+    double *tb = tbar + (int64_t)blockIdx.x * 4 * n_angles;
+    // With Greek cotangents, tb holds their transposed projection B (ens_greek_vjp_mfma), the
+    // cotangent of the weighted tables w F through gamma_kl = c_l sum_j w_j (F . d)_kl. When
+    // normalized, G = gamma / alpha1_0 with alpha1_0 = gamma_00: the outputs' cotangents act
+    // through gamma / alpha1_0 (B / alpha1_0) and through alpha1_0, whose cotangent is
+    // -sum_{kl != 00} gbar_kl gamma_kl / alpha1_0^2 = -(B . F) / alpha1_0^2 (B leaves gbar_00 out).
     double a0 = 1.0, a0bar = 0.0;
-    if (greek) {
-        if (t == 0) {
-            double s = 0.0;
-            for (int j = 0; j < n_angles; ++j)
-                s += (aw[j] * F11[j]) * dtab[(0 * n_angles + j) * len];
-            a0_sh = 0.5 * s;
+    if (greek && normalized) {
+        double a = 0.0, part = 0.0;
+        for (int j = t; j < n_angles; j += blockDim.x) {
+            a += (aw[j] * F11[j]) * dtab[(int64_t)j * len];
+            part += tb[j] * F11[j] + tb[n_angles + j] * F12[j] + tb[2 * n_angles + j] * F33[j] +
+                    tb[3 * n_angles + j] * F34[j];
         }
-        __syncthreads();
-        a0 = a0_sh;
-        if (normalized) {
-            // sum over (k, l) != (0, 0) of gbar_kl G_kl, G = gamma / a0 the normalized output
-            double part = 0.0;
-            for (int64_t l = t; l < len; l += blockDim.x) {
-                double s11 = 0, s33 = 0, sp = 0, sm = 0, s12 = 0, s34 = 0;
-                for (int j = 0; j < n_angles; ++j) {
-                    const double w = aw[j];
-                    const double f11 = w * F11[j], f33 = w * F33[j];
-                    const double d00 = dtab[(0 * n_angles + j) * len + l];
-                    const double d22 = dtab[(1 * (int64_t)n_angles + j) * len + l];
-                    const double d2m2 = dtab[(2 * (int64_t)n_angles + j) * len + l];
-                    const double d02 = dtab[(3 * (int64_t)n_angles + j) * len + l];
-                    s11 += f11 * d00;
-                    s33 += f33 * d00;
-                    sp += (f11 + f33) * d22;
-                    sm += (f11 - f33) * d2m2;
-                    s12 += (w * F12[j]) * d02;
-                    s34 += (w * F34[j]) * d02;
-                }
-                const double c = (double)(2 * l + 1) * 0.5;
-                const double g6[6] = {c * s11, c * 0.5 * (sp + sm), c * 0.5 * (sp - sm),
-                                      c * s33, c * s12, c * s34};
-#pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    if (k != 0 || l != 0) part += gb[k * len + l] * (g6[k] / a0);
-            }
-            a0bar = -block_sum(part, red) / a0;
-        }
+        a0 = 0.5 * block_sum(a, red);
+        a0bar = -block_sum(part, red) / (a0 * a0);
     }
     // Phase: ph = src / sca_n with sca_n = <C_sca> when normalized.
     const double sca_n = normalized ? sca : 1.0;
     const double *pb = bar_phase ? bar_phase + e * 6 * (int64_t)n_angles : nullptr;
     double fbf = 0.0, ph_sca = 0.0;
-    double *tb = tbar + (int64_t)blockIdx.x * 4 * n_angles;
     for (int j = t; j < n_angles; j += blockDim.x) {
         double b11 = 0.0, b12 = 0.0, b33 = 0.0, b34 = 0.0;
         if (greek) {
-            const double w = aw[j];
-            const double *d00 = dtab + (0 * (int64_t)n_angles + j) * len;
-            const double *d22 = dtab + (1 * (int64_t)n_angles + j) * len;
-            const double *d2m2 = dtab + (2 * (int64_t)n_angles + j) * len;
-            const double *d02 = dtab + (3 * (int64_t)n_angles + j) * len;
-            double s11 = 0, s33 = 0, s12 = 0, s34 = 0;
-            for (int64_t l = 0; l < len; ++l) {
-                const double c = (double)(2 * l + 1) * 0.5;
-                double gbar[6];
-#pragma unroll
-                for (int k = 0; k < 6; ++k) gbar[k] = normalized ? gb[k * len + l] / a0 : gb[k * len + l];
-                if (normalized && l == 0) gbar[0] = a0bar;
-                const double sp = 0.5 * (gbar[1] + gbar[2]), sm = 0.5 * (gbar[1] - gbar[2]);
-                s11 += c * (gbar[0] * d00[l] + sp * d22[l] + sm * d2m2[l]);
-                s33 += c * (gbar[3] * d00[l] + sp * d22[l] - sm * d2m2[l]);
-                s12 += c * (gbar[4] * d02[l]);
-                s34 += c * (gbar[5] * d02[l]);
-            }
-            b11 = w * s11;
-            b12 = w * s12;
-            b33 = w * s33;
-            b34 = w * s34;
+            const double ia0 = normalized ? 1.0 / a0 : 1.0;
+            b11 = tb[j] * ia0;
+            b12 = tb[n_angles + j] * ia0;
+            b33 = tb[2 * n_angles + j] * ia0;
+            b34 = tb[3 * n_angles + j] * ia0;
+            // alpha1_0's cotangent through gamma_00 = (1/2) sum_j w_j F11_j d00_0(mu_j).
+            if (normalized) b11 += aw[j] * 0.5 * a0bar * dtab[(int64_t)j * len];
         }
         if (pb) {
             const double p11 = pb[0 * n_angles + j] + pb[2 * n_angles + j];
@@ -188,6 +149,84 @@ static __global__ __launch_bounds__(256) void ens_finish_vjp(
             sb[k + 1] = q[k] / z;
         }
         sb[0] = zbar / z;
+    }
+}
+
+// The transpose of ens_greek_mfma, for the Greek cotangents gbar (count, 6, len) of a chunk:
+// per ensemble and angle, out = [w (Y0 + Y2 + Y3), w Y4, w (Y1 + Y2 - Y3), w Y5] in the table
+// components' order (F11, F12, F33, F34), Y_k[e][j] = sum_l U_k[e][l] D_k[l][j] with
+// U = c_l (gbar_0, gbar_3, (gbar_1 + gbar_2)/2, (gbar_1 - gbar_2)/2, gbar_4, gbar_5),
+// c_l = (2l + 1)/2 and D = (d00, d00, d22, d2-2, d02, d02), as GEMMs in tiles of 16 ensembles x
+// 16 angles with v_mfma_f64_16x16x4f64 (operand and result lanes as in ens_greek_mfma). A
+// workgroup is 16 ensembles and 64 angles, a wavefront 16 angles; the U tile of 64 orders is
+// staged in LDS, the table read order-major (dtab_t[(p * len + l) * n_angles + j]), so that a
+// wavefront's loads run along the angles. With normalization gbar_0 at l = 0 is left out
+// (alpha1_0 is 1 exactly); ens_finish_vjp adds the cotangent of alpha1_0 in its place.
+constexpr int kGreekOrders = 64;  // orders of the U tile staged per step
+
+static __global__ __launch_bounds__(256) void ens_greek_vjp_mfma(
+    int64_t e0, int64_t ne, int n_angles, const double *aw, const double *bar_greek,
+    const double *dtab_t, int64_t len, int normalized, double *out) {
+    __shared__ double lds_u[6][16][kGreekOrders + 1];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    const int64_t et = (int64_t)blockIdx.x * 16;                // the tile's first ensemble
+    const int jb = (int)blockIdx.y * 64 + wave * 16 + r;         // this lane's B column (angle)
+    const double *dcol = dtab_t + (jb < n_angles ? jb : 0);
+    const int64_t plane = len * n_angles;
+    mfma_d4 acc[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) acc[k] = mfma_d4{0, 0, 0, 0};
+    for (int64_t l0 = 0; l0 < len; l0 += kGreekOrders) {
+        __syncthreads();
+        // Thread t stages (ensemble t / 16, orders t % 16 + 16 q) of the tile.
+#pragma unroll
+        for (int q = 0; q < kGreekOrders / 16; ++q) {
+            const int er = t / 16, la = t % 16 + 16 * q;
+            const int64_t e = et + er, l = l0 + la;
+            double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (e < ne && l < len) {  // rows beyond the chunk, orders beyond len: 0
+                const double *gb = bar_greek + (e0 + e) * 6 * len + l;
+                const double c = (double)(2 * l + 1) * 0.5;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) g[k] = c * gb[k * len];
+                if (normalized && l == 0) g[0] = 0.0;
+            }
+            lds_u[0][er][la] = g[0];
+            lds_u[1][er][la] = g[3];
+            lds_u[2][er][la] = 0.5 * (g[1] + g[2]);
+            lds_u[3][er][la] = 0.5 * (g[1] - g[2]);
+            lds_u[4][er][la] = g[4];
+            lds_u[5][er][la] = g[5];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ll = 0; ll < kGreekOrders; ll += 4) {
+            const int la = ll + kq;
+            const int64_t l = l0 + la;
+            // Orders beyond len meet zero U values; read a valid row of the table for them.
+            const double *dl = dcol + (l < len ? l : 0) * n_angles;
+            const double d00 = dl[0], d22 = dl[plane], d2m2 = dl[2 * plane], d02 = dl[3 * plane];
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_u[0][r][la], d00, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_u[1][r][la], d00, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_u[2][r][la], d22, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_u[3][r][la], d2m2, acc[3], 0, 0, 0);
+            acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_u[4][r][la], d02, acc[4], 0, 0, 0);
+            acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(lds_u[5][r][la], d02, acc[5], 0, 0, 0);
+        }
+    }
+    if (jb >= n_angles) return;
+    const double w = aw[jb];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int64_t e = et + kq + 4 * v;
+        if (e >= ne) continue;
+        double *o = out + e * 4 * (int64_t)n_angles + jb;
+        o[0] = w * (acc[0][v] + acc[2][v] + acc[3][v]);
+        o[n_angles] = w * acc[4][v];
+        o[2 * n_angles] = w * (acc[1][v] + acc[2][v] - acc[3][v]);
+        o[3 * n_angles] = w * acc[5][v];
     }
 }
 
@@ -623,15 +662,18 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     EnsSetup S;
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_bar_greek != nullptr, order, allow_unvalidated,
-                        ctx->angular_kernel == 1 ? 1 : 0, S)))
+                        ctx->angular_kernel, S)))
         return rc;
+    // The reverse products read the FP64 coefficients: the node solves stage those, and with the
+    // fp16-split kernel the forward products read an fp32 copy.
+    S.stage32 = false;
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
     const int P = 2 + 2 * layers + (layers - 1);
     const int h = S.h;
     const bool tables = d_bar_greek || d_bar_phase;
-    // Staging and cotangents take 32 B per term each.
-    const int64_t max_terms = chunk_terms_limit(6, 2 * sizeof(cx));  // staging and its cotangents
+    // Staging and cotangents take 32 B per term each (and the fp32 copy 16 B with kernel 2).
+    const int64_t max_terms = chunk_terms_limit(6, 2 * sizeof(cx));
     const unsigned vjp_blocks = (unsigned)std::max(1, ctx->cus * 2);
     int64_t e0 = 0;
     while (e0 < count) {
@@ -647,8 +689,19 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         double *gnode = ht + 3 * nodes;
         double *raw = gnode + P * nodes;
         ChunkBufs B;
-        if ((rc = forward_chunk(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, backs)))
+        if ((rc = forward_nodes(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, backs, s)))
             return rc;
+        if (S.kernel == 2) {
+            if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kStaging32,
+                                            (size_t)std::max<int64_t>(chunk_terms, 1) * sizeof(float4)),
+                                "staging")))
+                return rc;
+            float4 *st32 = static_cast<float4 *>(ctx->buf[scatrs_ctx::kStaging32]);
+            stage_fp32<<<(unsigned)std::max(1, ctx->cus * 16), 256, 0, s>>>(B.staging, chunk_terms,
+                                                                            st32);
+            B.staging32 = st32;
+        }
+        if ((rc = forward_angular(ctx, S, e0, ne, n_radius, n_angles, B, s))) return rc;
         if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjEns,
                                         (size_t)ne * (8 + 4 * n_angles) * sizeof(double)),
                             "alloc")))
@@ -656,6 +709,11 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         double *sbar = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjEns]);
         double *tbar = sbar + ne * 8;
         const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
+        if (d_bar_greek) {
+            const dim3 grid((unsigned)((ne + 15) / 16), (unsigned)((n_angles + 63) / 64));
+            ens_greek_vjp_mfma<<<grid, 256, 0, s>>>(e0, ne, (int)n_angles, S.d_atab, d_bar_greek,
+                                                    S.d_dtab_t, order + 1, normalized, tbar);
+        }
         ens_finish_vjp<<<(unsigned)ne, 256, shmem, s>>>(
             e0, n_radius, (int)n_angles, S.d_atab, B.vals, B.tab_out, S.d_dtab, order, normalized,
             d_bar_scalars, d_bar_greek, d_bar_phase, sbar, tbar);
