@@ -232,12 +232,15 @@ static __global__ __launch_bounds__(256) void ens_greek_vjp_mfma(
 
 // ---- the angular products in reverse ----
 
-constexpr int kVjpNodes = 16;   // nodes per group: 64 matrix-core columns
+constexpr int kVjpTiles = 2;                // MFMA tiles of 4 nodes per group
+constexpr int kVjpNodes = 4 * kVjpTiles;    // nodes per group
+constexpr int kVjpCols = 16 * kVjpTiles;    // matrix-core columns of a group: 4 per node
 constexpr int kVjpOrders = 32;  // coefficient chunk of the forward products
 constexpr int kVjpAngles = 32;  // panel chunk of the transposed products
 constexpr int kVjpOrderRows = 128;
 
-// Persistent: each workgroup takes (ensemble, group of 16 nodes) items from `counter`.
+// Persistent: each workgroup takes (ensemble, group of kVjpNodes nodes) items from `counter`
+// (groups of 8 nodes keep a wavefront's accumulators within 128 registers: 4 wavefronts per SIMD).
 //  1. S1, S2 at every non-negative cosine and its mirror (as ens_angular_mfma, but per node).
 //  2. With Tbar at +-mu, g1 = dL/dS1 = (b11 - b12) S1 + (b33 - i b34) S2 and
 //     g2 = dL/dS2 = (b11 + b12) S2 + (b33 + i b34) S1 (L = Tbar . F as a real function of the
@@ -248,20 +251,21 @@ constexpr int kVjpOrderRows = 128;
 //     The panels P = [g1+ + g1-, g2+ + g2-] and M = [g2+ - g2-, g1+ - g1-] (re, im per node)
 //     are written to the workgroup's slab, and two transposed products per 128 orders form
 //     G1^T P + G2^T M: columns 4q + c hold node q's (ubar.re, ubar.im, vbar.re, vbar.im)[c].
-static __global__ __launch_bounds__(512) void ens_angular_vjp(
+//     A wavefront whose 16 orders lie beyond the group's largest N skips its products.
+static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double2 *table,
     const double *tbar, unsigned long long *counter, double *slabs, cx *abar, double *ht) {
     __shared__ cx lds_a[kVjpNodes][kVjpOrders];
     __shared__ cx lds_b[kVjpNodes][kVjpOrders];
-    __shared__ double lds_g[2][kVjpAngles][64];
+    __shared__ double lds_g[2][kVjpAngles][kVjpCols];
     __shared__ double lds_ht[8][kVjpNodes];
     __shared__ unsigned long long item_sh;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int row = lane & 15, kq = lane >> 4;
     const int comp = lane & 3, qin = (lane & 15) >> 2;
-    double *slab = slabs + (size_t)blockIdx.x * 2 * 64 * h;
+    double *slab = slabs + (size_t)blockIdx.x * 2 * kVjpCols * h;
     const int64_t groups = (n_radius + kVjpNodes - 1) / kVjpNodes;
     const int64_t items = ne * groups;
     while (true) {
@@ -281,13 +285,13 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
             const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
             nmax = nq > nmax ? nq : nmax;
         }
-        double hts[4] = {0.0, 0.0, 0.0, 0.0};
+        double hts[kVjpTiles] = {};
         // 1-2: forward products per slice of 128 cosines, cotangent panels to the slab.
         for (int slice = 0; slice * kSliceAngles < h; ++slice) {
             const int jj = slice * kSliceAngles + wave * 16 + row;
-            mfma_d4 acc1[4], acc2[4];
+            mfma_d4 acc1[kVjpTiles], acc2[kVjpTiles];
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) {
+            for (int tt = 0; tt < kVjpTiles; ++tt) {
                 acc1[tt] = mfma_d4{0, 0, 0, 0};
                 acc2[tt] = mfma_d4{0, 0, 0, 0};
             }
@@ -317,7 +321,7 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
                     const double g1 = odd ? pt.x : pt.y;
                     const double g2 = odd ? pt.y : pt.x;
 #pragma unroll
-                    for (int tt = 0; tt < 4; ++tt) {
+                    for (int tt = 0; tt < kVjpTiles; ++tt) {
                         const int q = 4 * tt + qin;
                         const cx a = lds_a[q][kk], b = lds_b[q][kk];
                         const cx u = odd ? a : b, v = odd ? b : a;
@@ -328,7 +332,7 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
                 }
             }
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) {
+            for (int tt = 0; tt < kVjpTiles; ++tt) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     double c1[4], c2[4];
@@ -358,8 +362,8 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
                     const cx P1 = g1p + g1m, M1 = g1p - g1m, P2 = g2p + g2m, M2 = g2p - g2m;
                     const double pv = comp == 0 ? P1.re : comp == 1 ? P1.im : comp == 2 ? P2.re : P2.im;
                     const double mv = comp == 0 ? M2.re : comp == 1 ? M2.im : comp == 2 ? M1.re : M1.im;
-                    slab[(size_t)jr * 64 + 16 * tt + row] = pv;
-                    slab[((size_t)h + jr) * 64 + 16 * tt + row] = mv;
+                    slab[(size_t)jr * kVjpCols + 16 * tt + row] = pv;
+                    slab[((size_t)h + jr) * kVjpCols + 16 * tt + row] = mv;
                     // this lane's element of F (comp: F11, F12, F33, F34) at +mu and -mu
                     double fp, fm;
                     if (comp == 0) {
@@ -379,7 +383,7 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
         }
         // The node's Tbar . F: over the components and rows of the wave, then over the waves.
 #pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
+        for (int tt = 0; tt < kVjpTiles; ++tt) {
             double x = hts[tt];
             x += __shfl_xor(x, 1, 64);
             x += __shfl_xor(x, 2, 64);
@@ -389,7 +393,7 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
         }
         if (comp == 0 && kq == 0)
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) lds_ht[wave][4 * tt + qin] = hts[tt];
+            for (int tt = 0; tt < kVjpTiles; ++tt) lds_ht[wave][4 * tt + qin] = hts[tt];
         __syncthreads();  // the slab's panels and lds_ht are complete
         if (t < kVjpNodes && g0 + t < n_radius) {
             double s = 0.0;
@@ -400,21 +404,22 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
         for (int nb = 0; nb < nmax; nb += kVjpOrderRows) {
             const int n_row = nb + wave * 16 + row + 1;  // this lane's A-operand order
             const bool odd_row = n_row & 1;
-            mfma_d4 d1[4], d2[4];
+            const bool active = nb + wave * 16 < nmax;   // the wavefront has orders to form
+            mfma_d4 d1[kVjpTiles], d2[kVjpTiles];
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) {
+            for (int tt = 0; tt < kVjpTiles; ++tt) {
                 d1[tt] = mfma_d4{0, 0, 0, 0};
                 d2[tt] = mfma_d4{0, 0, 0, 0};
             }
             for (int jc = 0; jc < h; jc += kVjpAngles) {
                 __syncthreads();
-                for (int idx = t; idx < 2 * kVjpAngles * 64; idx += blockDim.x) {
-                    const int s = idx / (kVjpAngles * 64), k = (idx / 64) % kVjpAngles, col = idx % 64;
+                for (int idx = t; idx < 2 * kVjpAngles * kVjpCols; idx += blockDim.x) {
+                    const int s = idx / (kVjpAngles * kVjpCols), k = (idx / kVjpCols) % kVjpAngles, col = idx % kVjpCols;
                     const int j = jc + k;
-                    lds_g[s][k][col] = j < h ? slab[((size_t)s * h + j) * 64 + col] : 0.0;
+                    lds_g[s][k][col] = j < h ? slab[((size_t)s * h + j) * kVjpCols + col] : 0.0;
                 }
                 __syncthreads();
-                const int kend = h - jc < kVjpAngles ? h - jc : kVjpAngles;
+                const int kend = !active ? 0 : h - jc < kVjpAngles ? h - jc : kVjpAngles;
                 for (int k4 = 0; k4 < kend; k4 += 4) {
                     const int jb = jc + k4 + kq;
                     double2 pt = make_double2(0.0, 0.0);
@@ -422,7 +427,7 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
                     const double g1 = odd_row ? pt.x : pt.y;
                     const double g2 = odd_row ? pt.y : pt.x;
 #pragma unroll
-                    for (int tt = 0; tt < 4; ++tt) {
+                    for (int tt = 0; tt < kVjpTiles; ++tt) {
                         const double b1 = lds_g[0][k4 + kq][16 * tt + row];
                         const double b2 = lds_g[1][k4 + kq][16 * tt + row];
                         d1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, b1, d1[tt], 0, 0, 0);
@@ -431,7 +436,7 @@ static __global__ __launch_bounds__(512) void ens_angular_vjp(
                 }
             }
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt) {
+            for (int tt = 0; tt < kVjpTiles; ++tt) {
                 const int64_t node = g0 + 4 * tt + qin;
                 if (node >= n_radius) continue;
                 const int nq = nn[node];
@@ -724,7 +729,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
                                 "alloc")))
                 return rc;
             if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjSlabs,
-                                            (size_t)vjp_blocks * 2 * 64 * h * sizeof(double)),
+                                            (size_t)vjp_blocks * 2 * kVjpCols * h * sizeof(double)),
                                 "alloc")))
                 return rc;
             abar = static_cast<cx *>(ctx->buf[scatrs_ctx::kAdjStaging]);
