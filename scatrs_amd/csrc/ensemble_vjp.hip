@@ -264,7 +264,8 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int row = lane & 15, kq = lane >> 4;
-    const int comp = lane & 3, qin = (lane & 15) >> 2;
+    const int comp = lane & 3, qin = (lane & 15) >> 2;  // transposed products' D column
+    const int ca = (lane & 15) >> 2, qa = lane & 3;        // forward products' A row 4 ca + qa
     double *slab = slabs + (size_t)blockIdx.x * 2 * kVjpCols * h;
     const int64_t groups = (n_radius + kVjpNodes - 1) / kVjpNodes;
     const int64_t items = ne * groups;
@@ -285,7 +286,7 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
             const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
             nmax = nq > nmax ? nq : nmax;
         }
-        double hts[kVjpTiles] = {};
+        double hts[kVjpTiles] = {};  // Tbar . F of node 4 tt + kq over this lane's cosines
         // 1-2: forward products per slice of 128 cosines, cotangent panels to the slab.
         for (int slice = 0; slice * kSliceAngles < h; ++slice) {
             const int jj = slice * kSliceAngles + wave * 16 + row;
@@ -322,78 +323,69 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                     const double g2 = odd ? pt.y : pt.x;
 #pragma unroll
                     for (int tt = 0; tt < kVjpTiles; ++tt) {
-                        const int q = 4 * tt + qin;
+                        // A row 4 c + q: component c (u.re, u.im, v.re, v.im) of node 4 tt + q.
+                        const int q = 4 * tt + qa;
                         const cx a = lds_a[q][kk], b = lds_b[q][kk];
                         const cx u = odd ? a : b, v = odd ? b : a;
-                        const double val = comp == 0 ? u.re : comp == 1 ? u.im : comp == 2 ? v.re : v.im;
-                        acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, val, acc1[tt], 0, 0, 0);
-                        acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g2, val, acc2[tt], 0, 0, 0);
+                        const double val = ca == 0 ? u.re : ca == 1 ? u.im : ca == 2 ? v.re : v.im;
+                        acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(val, g1, acc1[tt], 0, 0, 0);
+                        acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(val, g2, acc2[tt], 0, 0, 0);
                     }
                 }
             }
+            // D[4 c + q][j] sits in lane j + 16 q, element c: this lane holds the four components
+            // of node 4 tt + kq at its cosine (row).
+            if (jj < h) {
+                const int jp = n_angles - h + jj, jm = n_angles - 1 - jp;
+                double bp[4], bm[4];
 #pragma unroll
-            for (int tt = 0; tt < kVjpTiles; ++tt) {
+                for (int k = 0; k < 4; ++k) {
+                    bp[k] = tb[k * n_angles + jp];
+                    bm[k] = jm != jp ? tb[k * n_angles + jm] : 0.0;
+                }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    double c1[4], c2[4];
-                    const double x1 = acc1[tt][v], x2 = acc2[tt][v];
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) {
-                        c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
-                        c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
-                    }
-                    // D[i][j] sits in lane j + 16 (i % 4), element i / 4: this lane's cosine row.
-                    const int jr = slice * kSliceAngles + wave * 16 + kq + 4 * v;
-                    if (jr >= h) continue;
-                    const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
-                    const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
+                for (int tt = 0; tt < kVjpTiles; ++tt) {
+                    const cx A = mk(acc1[tt][0], acc1[tt][1]), C = mk(acc1[tt][2], acc1[tt][3]);
+                    const cx D = mk(acc2[tt][0], acc2[tt][1]), B = mk(acc2[tt][2], acc2[tt][3]);
                     const cx s1p = A + B, s1m = A - B, s2p = C + D, s2m = C - D;
-                    const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
-                    double bp[4], bm[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        bp[k] = tb[k * n_angles + jp];
-                        bm[k] = jm != jp ? tb[k * n_angles + jm] : 0.0;
-                    }
                     const cx g1p = s1p * (bp[0] - bp[1]) + s2p * bp[2] - times_i(s2p) * bp[3];
                     const cx g2p = s2p * (bp[0] + bp[1]) + s1p * bp[2] + times_i(s1p) * bp[3];
                     const cx g1m = s1m * (bm[0] - bm[1]) + s2m * bm[2] - times_i(s2m) * bm[3];
                     const cx g2m = s2m * (bm[0] + bm[1]) + s1m * bm[2] + times_i(s1m) * bm[3];
                     const cx P1 = g1p + g1m, M1 = g1p - g1m, P2 = g2p + g2m, M2 = g2p - g2m;
-                    const double pv = comp == 0 ? P1.re : comp == 1 ? P1.im : comp == 2 ? P2.re : P2.im;
-                    const double mv = comp == 0 ? M2.re : comp == 1 ? M2.im : comp == 2 ? M1.re : M1.im;
-                    slab[(size_t)jr * kVjpCols + 16 * tt + row] = pv;
-                    slab[((size_t)h + jr) * kVjpCols + 16 * tt + row] = mv;
-                    // this lane's element of F (comp: F11, F12, F33, F34) at +mu and -mu
-                    double fp, fm;
-                    if (comp == 0) {
-                        fp = 0.5 * (norm2(s1p) + norm2(s2p));
-                        fm = 0.5 * (norm2(s1m) + norm2(s2m));
-                    } else if (comp == 1) {
-                        fp = 0.5 * (norm2(s2p) - norm2(s1p));
-                        fm = 0.5 * (norm2(s2m) - norm2(s1m));
-                    } else {
-                        const cx pp = s2p * conj(s1p), pm = s2m * conj(s1m);
-                        fp = comp == 2 ? pp.re : pp.im;
-                        fm = comp == 2 ? pm.re : pm.im;
-                    }
-                    hts[tt] += bp[comp] * fp + bm[comp] * fm;
+                    double *sp = slab + (size_t)jj * kVjpCols + 16 * tt + 4 * kq;
+                    double *sm = slab + ((size_t)h + jj) * kVjpCols + 16 * tt + 4 * kq;
+                    sp[0] = P1.re;
+                    sp[1] = P1.im;
+                    sp[2] = P2.re;
+                    sp[3] = P2.im;
+                    sm[0] = M2.re;
+                    sm[1] = M2.im;
+                    sm[2] = M1.re;
+                    sm[3] = M1.im;
+                    // F11, F12, F33, F34 at +mu and -mu against their cotangents
+                    const double n1p = norm2(s1p), n2p = norm2(s2p), n1m = norm2(s1m), n2m = norm2(s2m);
+                    const cx pp = s2p * conj(s1p), pm = s2m * conj(s1m);
+                    hts[tt] += bp[0] * (0.5 * (n1p + n2p)) + bp[1] * (0.5 * (n2p - n1p)) +
+                               bp[2] * pp.re + bp[3] * pp.im;
+                    hts[tt] += bm[0] * (0.5 * (n1m + n2m)) + bm[1] * (0.5 * (n2m - n1m)) +
+                               bm[2] * pm.re + bm[3] * pm.im;
                 }
             }
         }
-        // The node's Tbar . F: over the components and rows of the wave, then over the waves.
+        // The node's Tbar . F: over the 16 cosine lanes of the node, then over the waves.
 #pragma unroll
         for (int tt = 0; tt < kVjpTiles; ++tt) {
             double x = hts[tt];
             x += __shfl_xor(x, 1, 64);
             x += __shfl_xor(x, 2, 64);
-            x += __shfl_xor(x, 16, 64);
-            x += __shfl_xor(x, 32, 64);
+            x += __shfl_xor(x, 4, 64);
+            x += __shfl_xor(x, 8, 64);
             hts[tt] = x;
         }
-        if (comp == 0 && kq == 0)
+        if (row == 0)
 #pragma unroll
-            for (int tt = 0; tt < kVjpTiles; ++tt) lds_ht[wave][4 * tt + qin] = hts[tt];
+            for (int tt = 0; tt < kVjpTiles; ++tt) lds_ht[wave][4 * tt + kq] = hts[tt];
         __syncthreads();  // the slab's panels and lds_ht are complete
         if (t < kVjpNodes && g0 + t < n_radius) {
             double s = 0.0;
