@@ -236,7 +236,7 @@ constexpr int kVjpTiles = 2;                // MFMA tiles of 4 nodes per group
 constexpr int kVjpNodes = 4 * kVjpTiles;    // nodes per group
 constexpr int kVjpCols = 16 * kVjpTiles;    // matrix-core columns of a group: 4 per node
 constexpr int kVjpOrders = 32;  // coefficient chunk of the forward products
-constexpr int kVjpAngles = 32;  // panel chunk of the transposed products
+constexpr int kVjpAngles = 64;  // panel chunk of the transposed products
 constexpr int kVjpOrderRows = 128;
 
 // Persistent: each workgroup takes (ensemble, group of kVjpNodes nodes) items from `counter`
@@ -256,8 +256,8 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double2 *table,
     const double *tbar, unsigned long long *counter, double *slabs, cx *abar, double *ht) {
-    __shared__ cx lds_a[kVjpNodes][kVjpOrders];
-    __shared__ cx lds_b[kVjpNodes][kVjpOrders];
+    // The chunk's coefficients as the A operand: [tile][row 4 c + q][order], u, v by parity.
+    __shared__ double lds_v[kVjpTiles][16][kVjpOrders + 1];
     __shared__ double lds_g[2][kVjpAngles][kVjpCols];
     __shared__ double lds_ht[8][kVjpNodes];
     __shared__ unsigned long long item_sh;
@@ -265,7 +265,6 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
     const int lane = t & 63, wave = t >> 6;
     const int row = lane & 15, kq = lane >> 4;
     const int comp = lane & 3, qin = (lane & 15) >> 2;  // transposed products' D column
-    const int ca = (lane & 15) >> 2, qa = lane & 3;        // forward products' A row 4 ca + qa
     double *slab = slabs + (size_t)blockIdx.x * 2 * kVjpCols * h;
     const int64_t groups = (n_radius + kVjpNodes - 1) / kVjpNodes;
     const int64_t items = ne * groups;
@@ -308,8 +307,13 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                         va = c[2 * n];
                         vb = c[2 * n + 1];
                     }
-                    lds_a[q][k] = va;
-                    lds_b[q][k] = vb;
+                    const bool odd = (n + 1) & 1;
+                    const cx u = odd ? va : vb, v = odd ? vb : va;
+                    double *col = &lds_v[q / 4][q % 4][k];
+                    col[0] = u.re;
+                    col[4 * (kVjpOrders + 1)] = u.im;
+                    col[8 * (kVjpOrders + 1)] = v.re;
+                    col[12 * (kVjpOrders + 1)] = v.im;
                 }
                 __syncthreads();
                 const int kend = nmax - kc < kVjpOrders ? nmax - kc : kVjpOrders;
@@ -324,10 +328,7 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
 #pragma unroll
                     for (int tt = 0; tt < kVjpTiles; ++tt) {
                         // A row 4 c + q: component c (u.re, u.im, v.re, v.im) of node 4 tt + q.
-                        const int q = 4 * tt + qa;
-                        const cx a = lds_a[q][kk], b = lds_b[q][kk];
-                        const cx u = odd ? a : b, v = odd ? b : a;
-                        const double val = ca == 0 ? u.re : ca == 1 ? u.im : ca == 2 ? v.re : v.im;
+                        const double val = lds_v[tt][row][kk];
                         acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(val, g1, acc1[tt], 0, 0, 0);
                         acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(val, g2, acc2[tt], 0, 0, 0);
                     }
