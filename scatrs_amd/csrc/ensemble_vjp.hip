@@ -235,7 +235,7 @@ static __global__ __launch_bounds__(256) void ens_greek_vjp_mfma(
 constexpr int kVjpTiles = 2;                // MFMA tiles of 4 nodes per group
 constexpr int kVjpNodes = 4 * kVjpTiles;    // nodes per group
 constexpr int kVjpCols = 16 * kVjpTiles;    // matrix-core columns of a group: 4 per node
-constexpr int kVjpOrders = 32;  // coefficient chunk of the forward products
+constexpr int kVjpOrders = 64;  // coefficient chunk of the forward products
 constexpr int kVjpAngles = 64;  // panel chunk of the transposed products
 constexpr int kVjpOrderRows = 128;
 
