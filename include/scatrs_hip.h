@@ -54,14 +54,16 @@ const char *scatrs_last_error(const scatrs_ctx *ctx);
 int scatrs_set_profiling(scatrs_ctx *ctx, int on);
 int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap);
 
-/* Selects the angular-product kernel of scatrs_ensemble_fixed: 0 (default) the FP64
- * matrix-core GEMM kernel, 1 the plain FP64 vector kernel (same results to rounding; kept as
- * the baseline the matrix-core kernels are measured against), 2 the fp16 matrix cores with
- * both operands split in two fp16 slices (exactly scaled per node and per table), three
- * products summed in fp32 over 64 orders and then in fp64: the phase tables to about 1e-7 of
- * their largest element, Greek coefficients to about 1e-7 of max|alpha1| (the scalars do not
- * depend on it); with n_radius above 2048 kernel 2 falls back to kernel 0. The reverse pass
- * keeps FP64. No reference counterpart. */
+/* Selects the angular-product kernel of scatrs_ensemble_fixed and of the forward tables that
+ * scatrs_ensemble_fixed_vjp recomputes: 0 (default) the FP64 matrix-core GEMM kernel, 1 the
+ * plain FP64 vector kernel (same results to rounding; kept as the baseline the matrix-core
+ * kernels are measured against), 2 the fp16 matrix cores with both operands split in two fp16
+ * slices (exactly scaled per node and per table), three products summed in fp32 over 128
+ * orders and then in fp64: the phase tables to about 1e-7 of their largest element, Greek
+ * coefficients to about 1e-7 of max|alpha1| (the scalars do not depend on it); with n_radius
+ * above 2048 kernel 2 falls back to kernel 0. The reverse products themselves always run in
+ * FP64 (with kernel 2 the gradients stay within about 1e-7 of sum |cotangent x derivative|).
+ * No reference counterpart. */
 int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
 
 /* The default truncation N = ceil(x + 6 x^(1/3) + 3) (host; scatrs/src/mie/mod.rs:122-124). */
