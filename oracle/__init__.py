@@ -268,3 +268,39 @@ def ensemble_batch(kind, p0, p1, wavelength, m, n_radius: int, n_angles: int, or
     if rc != 0:
         raise ValueError("invalid ensemble input")
     return scalars, greek, phase
+
+
+def ensemble_mixture(parts, wavelength: float, n_radius: int = 128, n_angles: int = 128,
+                     order: int = 32, normalized: bool = True) -> dict:
+    """A mixture of parts on the fixed discrete path, as the reference's engine mixes them
+    (scatrs/src/ensemble/engine.rs:2120-2150, parts from engine.rs:143-160): each part
+    (f, kind, p0, p1, m, fractions) integrated on its own, its number averages (<C_ext>,
+    <C_sca>, <C_abs>, <C_back>, <C_sca g>, <r^2>, <r^3> and the averaged tables, the outputs of
+    ensemble_fixed with normalized=False) weighted by its number fraction f (the component's
+    weight over the total times the mode's weight over its component's total), then the
+    outputs assembled from the mixed averages as for one part (engine.rs:2396-2497): ratios
+    ssa = <C_sca>/<C_ext>, g = <C_sca g>/<C_sca>, lidar ratio 4 pi <C_ext>/<C_back>, effective
+    radius <r^3>/<r^2>; the phase matrix divided by <C_sca> and the Greek coefficients by their
+    alpha1_0 when normalized (engine.rs:2447-2490)."""
+    acc = np.zeros(7)
+    greek = np.zeros((6, order + 1)) if order >= 0 else None
+    phase = np.zeros((6, n_angles))
+    for (f, kind, p0, p1, m, fr) in parts:
+        r = ensemble_fixed(kind, p0, p1, wavelength, m, fr, n_radius=n_radius,
+                           n_angles=n_angles, order=order, normalized=False)
+        s = r["scalars"]
+        avg = np.array([s[0], s[1], s[2], s[3], s[5] * s[1], s[9] / np.pi,
+                        s[10] * 3.0 / (4.0 * np.pi)])
+        acc += f * avg
+        phase += f * r["phase"]
+        if greek is not None:
+            greek += f * r["greek"]
+    ext, sca, absb, back, scag, r2, r3 = acc
+    scalars = np.array([ext, sca, absb, back, sca / ext, scag / sca, 4.0 * np.pi * ext / back,
+                        0.0, r3 / r2, np.pi * r2, 4.0 * np.pi / 3.0 * r3])
+    if normalized:
+        phase = phase / sca
+        if greek is not None:
+            greek = greek / greek[0, 0]
+            greek[0, 0] = 1.0
+    return {"scalars": scalars, "greek": greek, "phase": phase}

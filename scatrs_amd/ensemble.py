@@ -12,6 +12,7 @@ estimates and ``converged`` flags) is not part of this path.
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -19,11 +20,11 @@ import numpy as np
 from . import _native
 from .errors import InvalidInputError
 from .particles import LayeredSphere, Sphere
-from .size import Gamma, LogNormal, check
+from .size import Gamma, LogNormal, MultiModal, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
-           "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
-           "parameter_names"]
+           "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
+           "mix_parts", "parameter_names"]
 
 SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
            "effective_radius", "mean_area", "mean_volume")
@@ -184,6 +185,83 @@ def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *,
     return _FUNCTION.apply(p0, p1, m_real, m_imag, fractions, kind, wavelength, opts)
 
 
+def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
+    """A mixture of parts, as the reference's engine mixes them
+    (scatrs/src/ensemble/engine.rs:2120-2150): each part's number averages, taken from its
+    outputs on the fixed path computed with ``normalized=False`` (scalars (parts, ..., 11),
+    greek (parts, ..., 6, L+1) or None, phase (parts, ..., 6, n) or None), weighted by its number
+    fraction (``fractions`` (parts,): the component's weight over the total times the mode's
+    weight over its component's total), and the outputs assembled from the mixed averages as
+    for one part (engine.rs:2396-2497): <C_ext>, <C_sca>, <C_abs>, <C_back>, <r^2>, <r^3>
+    averaged, ssa, g, the lidar ratio and the effective radius as their ratios, the phase
+    matrix divided by <C_sca> and the Greek coefficients by their alpha1_0 when normalized.
+    torch tensors in and out, differentiable in every input, the fractions included (with
+    ``ensemble_fixed(..., normalized=False)`` for the parts, a mixture's gradient)."""
+    import torch
+
+    nb = scalars.dim() - 2  # batch axes between the parts and the values
+    f = fractions.to(scalars.dtype).reshape((-1,) + (1,) * nb)
+    sc = scalars
+    avg = torch.stack([sc[..., 0], sc[..., 1], sc[..., 2], sc[..., 3], sc[..., 5] * sc[..., 1],
+                       sc[..., 9] / math.pi, sc[..., 10] * (3.0 / (4.0 * math.pi))], dim=-1)
+    ext, sca, absb, back, scag, r2, r3 = (f.unsqueeze(-1) * avg).sum(0).unbind(-1)
+    out = torch.stack([ext, sca, absb, back, sca / ext, scag / sca, 4.0 * math.pi * ext / back,
+                       torch.zeros_like(ext), r3 / r2, math.pi * r2,
+                       4.0 * math.pi / 3.0 * r3], dim=-1)
+    ph = None
+    if phase is not None:
+        ph = (f[..., None, None] * phase).sum(0)
+        if normalized:
+            ph = ph / sca[..., None, None]
+    gr = None
+    if greek is not None:
+        gr = (f[..., None, None] * greek).sum(0)
+        if normalized:
+            gr = gr / gr[..., 0:1, 0:1]
+    return out, gr, ph
+
+
+def _parts_device(parts, wl: np.ndarray, *, order, n_radius, n_angles, allow_unvalidated,
+                  want_phase):
+    """Each part (kind, p0, p1, particle) at every wavelength on the fixed path with
+    normalized=False, as tensors (parts, wavelengths, ...); one library call per layer count."""
+    import torch
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    nw, P = wl.size, len(parts)
+    sc = torch.empty((P, nw, 11), dtype=torch.float64, device=dev)
+    gr = None if order is None else torch.empty((P, nw, 6, order + 1), dtype=torch.float64,
+                                                device=dev)
+    ph = torch.empty((P, nw, 6, n_angles), dtype=torch.float64, device=dev) if want_phase \
+        else None
+    groups: dict[int, list[int]] = {}
+    for i, part in enumerate(parts):
+        groups.setdefault(part[3].layers, []).append(i)
+    terms = 0
+    for L, idx in groups.items():
+        kind = np.repeat(np.array([parts[i][0] for i in idx], np.int32), nw)
+        p0 = np.repeat(np.array([parts[i][1] for i in idx], np.float64), nw)
+        p1 = np.repeat(np.array([parts[i][2] for i in idx], np.float64), nw)
+        m = np.concatenate([parts[i][3].indices(wl).reshape(L, -1).T for i in idx])  # (n, L)
+        fr = None
+        if L > 1:
+            fr = t(np.concatenate([np.tile(parts[i][3].fractions(), (nw, 1)) for i in idx]))
+        s, g, p, n = ensemble_fixed_device(
+            t(kind), t(p0), t(p1), t(np.tile(wl.reshape(-1), len(idx))),
+            t(m[:, 0] if L == 1 else m), fr, n_radius=n_radius, n_angles=n_angles,
+            order=order, normalized=False, allow_unvalidated=allow_unvalidated,
+            want_phase=want_phase)
+        sel = torch.tensor(idx, device=dev)
+        sc[sel] = s.reshape(len(idx), nw, 11)
+        if gr is not None:
+            gr[sel] = g.reshape(len(idx), nw, 6, order + 1)
+        if ph is not None:
+            ph[sel] = p.reshape(len(idx), nw, 6, n_angles)
+        terms += n
+    return sc, gr, ph, terms
+
+
 @dataclass(frozen=True, eq=False)
 class EnsembleGreek:
     """Greek coefficients, arrays of shape grid + wavelength.shape + (L + 1,)."""
@@ -236,21 +314,55 @@ def _frozen(a: np.ndarray) -> np.ndarray:
     return a
 
 
+def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int):
+    """EnsembleResult of outputs with rows in the C order of ``shape``."""
+    s = sc.reshape(-1, 11).cpu().numpy()
+    fields = [_frozen(s[:, k].reshape(shape)) for k in range(11)]
+    g = None
+    if gr is not None:
+        a = gr.reshape(-1, 6, gr.shape[-1]).cpu().numpy()
+        g = EnsembleGreek(*(_frozen(a[:, k].reshape(shape + (a.shape[-1],))) for k in range(6)))
+    p = None
+    if ph is not None:
+        from .quadrature import gauss_legendre
+
+        a = ph.reshape(-1, 6, n_angles).cpu().numpy()
+        mu, _ = gauss_legendre(n_angles)
+        p = EnsemblePhase(_frozen(mu), *(_frozen(a[:, k].reshape(shape + (n_angles,)))
+                                         for k in range(6)))
+    return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()), terms=terms)
+
+
+def _mode_parts(size: MultiModal, particle, weight: float = 1.0):
+    """The parts of a multimodal distribution: (kind, p0, p1, particle) and number fractions."""
+    parts = [(d.kind, float(d.params()[0]), float(d.params()[1]), particle)
+             for _, d in size.modes]
+    return parts, list(weight * size.fractions())
+
+
 @dataclass(frozen=True)
 class Ensemble:
     """Particles of one kind over a size distribution (python/scatrs/ensemble.py:484-529).
 
     ``size`` may have array-valued parameters: a grid of distributions, computed at once
-    (the results have the grid's axes first)."""
+    (the results have the grid's axes first). A MultiModal size is integrated mode by mode and
+    mixed (mix_parts)."""
 
     particle: Sphere | LayeredSphere
-    size: LogNormal | Gamma
+    size: LogNormal | Gamma | MultiModal
 
     def __post_init__(self) -> None:
         if not isinstance(self.particle, (Sphere, LayeredSphere)):
             raise TypeError("particle must be a Sphere or a LayeredSphere, not "
                             f"{type(self.particle).__name__}")
         check(self.size)
+
+    @property
+    def grid_shape(self) -> tuple[int, ...]:
+        """The shape of the grid of distributions (``()`` without array-valued parameters)."""
+        if isinstance(self.size, MultiModal):
+            return ()
+        return np.broadcast(*self.size.params()).shape
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
                 n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
@@ -259,6 +371,15 @@ class Ensemble:
         import torch
 
         wl = np.asarray(wavelength, np.float64)
+        if isinstance(self.size, MultiModal):
+            parts, fr = _mode_parts(self.size, self.particle)
+            sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
+                                              n_angles=n_angles,
+                                              allow_unvalidated=allow_unvalidated,
+                                              want_phase=phase)
+            f = torch.tensor(fr, dtype=torch.float64, device=sc.device)
+            sc, gr, ph = mix_parts(sc, gr, ph, f, normalized)
+            return _result(sc, gr, ph, wl.shape, wl, n_angles, terms)
         p0, p1 = self.size.params()
         grid = np.broadcast(p0, p1).shape
         p0g, p1g = (np.broadcast_to(v, grid).reshape(-1) for v in (p0, p1))
@@ -277,21 +398,71 @@ class Ensemble:
             t(mm[:, 0] if L == 1 else mm), fr, n_radius=n_radius, n_angles=n_angles,
             order=greek, normalized=normalized, allow_unvalidated=allow_unvalidated,
             want_phase=phase)
-        shape = grid + wl.shape
-        s = sc.cpu().numpy()
-        fields = [_frozen(s[:, k].reshape(shape)) for k in range(11)]
-        g = None
-        if gr is not None:
-            a = gr.cpu().numpy()
-            g = EnsembleGreek(*(_frozen(a[:, k].reshape(shape + (a.shape[-1],)))
-                                for k in range(6)))
-        p = None
-        if ph is not None:
-            from .quadrature import gauss_legendre
+        return _result(sc, gr, ph, grid + wl.shape, wl, n_angles, terms)
 
-            a = ph.cpu().numpy()
-            mu, _ = gauss_legendre(n_angles)
-            p = EnsemblePhase(_frozen(mu), *(_frozen(a[:, k].reshape(shape + (n_angles,)))
-                                             for k in range(6)))
-        return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()),
-                              terms=terms)
+
+@dataclass(frozen=True)
+class Mixture:
+    """An external mixture: ensembles with relative number concentrations
+    (python/scatrs/ensemble.py:614-657). Cross sections and moments are averaged with the
+    number fractions w_i / sum w, phase matrices with w_i times the <C_sca> of ensemble i
+    (scatrs/src/ensemble/engine.rs:2120-2150); a multimodal component contributes its modes with
+    its weight times their fractions.
+
+    ``components``: (weight w_i >= 0, finite, not all 0; Ensemble of one distribution, not a
+    grid) pairs."""
+
+    components: tuple
+
+    def __post_init__(self) -> None:
+        out = []
+        for i, c in enumerate(tuple(self.components)):
+            try:
+                w, e = c
+            except (TypeError, ValueError):
+                raise TypeError(f"component {i} must be a (weight, Ensemble) pair, not "
+                                f"{type(c).__name__}") from None
+            if not isinstance(e, Ensemble):
+                raise TypeError(f"component {i} must be a (weight, Ensemble) pair, not "
+                                f"({type(w).__name__}, {type(e).__name__})")
+            if e.grid_shape != ():
+                raise InvalidInputError(
+                    f"the size of component {i} is a grid of shape {e.grid_shape}; a mixture "
+                    "needs one distribution with scalar parameters per component")
+            w = float(w)
+            if not (np.isfinite(w) and w >= 0.0):
+                raise InvalidInputError(
+                    f"the weight of component {i} must be finite and >= 0, got {w}")
+            out.append((w, e))
+        if not out:
+            raise InvalidInputError("a mixture needs a component")
+        if sum(w for w, _ in out) <= 0.0:
+            raise InvalidInputError("the weights of a mixture must not all be 0")
+        object.__setattr__(self, "components", tuple(out))
+
+    def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
+                n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
+                phase: bool = True) -> EnsembleResult:
+        """Averages over the mixture at each wavelength on the fixed discrete path."""
+        import torch
+
+        wl = np.asarray(wavelength, np.float64)
+        total = sum(w for w, _ in self.components)
+        parts, fr = [], []
+        for w, e in self.components:
+            if w == 0.0:
+                continue
+            if isinstance(e.size, MultiModal):
+                p, f = _mode_parts(e.size, e.particle, w / total)
+                parts += p
+                fr += f
+            else:
+                p0, p1 = e.size.params()
+                parts.append((e.size.kind, float(p0), float(p1), e.particle))
+                fr.append(w / total)
+        sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
+                                          n_angles=n_angles, allow_unvalidated=allow_unvalidated,
+                                          want_phase=phase)
+        f = torch.tensor(fr, dtype=torch.float64, device=sc.device)
+        sc, gr, ph = mix_parts(sc, gr, ph, f, normalized)
+        return _result(sc, gr, ph, wl.shape, wl, n_angles, terms)

@@ -43,12 +43,53 @@ class Gamma:
                 np.asarray(self.effective_variance, np.float64))
 
 
-Distribution = LogNormal | Gamma
+@dataclass(frozen=True)
+class MultiModal:
+    """sum_j w_j n_j(r) with each mode n_j normalized: ``modes`` of (weight w_j > 0, LogNormal
+    or Gamma with scalar parameters) (scatrs/src/ensemble/distribution.rs:62-67,172-180). Each
+    mode is integrated on its own and the modes' averages are mixed with their number
+    fractions w_j / sum w (scatrs/src/ensemble/engine.rs:143-160,2120-2150)."""
+
+    modes: tuple
+
+    def __post_init__(self) -> None:
+        try:
+            modes = tuple(self.modes)
+        except TypeError:
+            raise TypeError("modes must be a sequence of (weight, distribution) pairs") from None
+        if not modes:
+            raise InvalidInputError("a multimodal distribution needs a mode")
+        out = []
+        for j, mode in enumerate(modes):
+            try:
+                w, d = mode
+            except (TypeError, ValueError):
+                raise TypeError(f"mode {j} must be a (weight, distribution) pair") from None
+            w = float(w)
+            if not (np.isfinite(w) and w > 0.0):
+                raise InvalidInputError(f"the weight of mode {j} must be finite and > 0, got {w}")
+            if not isinstance(d, (LogNormal, Gamma)):
+                raise TypeError(f"mode {j} must be a LogNormal or a Gamma distribution, not "
+                                f"{type(d).__name__}")
+            if any(np.ndim(p) for p in d.params()):
+                raise InvalidInputError(f"mode {j}: a mode takes scalar parameters")
+            out.append((w, d))
+        object.__setattr__(self, "modes", tuple(out))
+
+    def fractions(self) -> np.ndarray:
+        """The modes' number fractions w_j / sum w."""
+        w = np.array([w for w, _ in self.modes])
+        return w / w.sum()
 
 
-def check(d: object, what: str = "size") -> LogNormal | Gamma:
+Distribution = LogNormal | Gamma | MultiModal
+
+
+def check(d: object, what: str = "size") -> LogNormal | Gamma | MultiModal:
+    if isinstance(d, MultiModal):
+        return d
     if not isinstance(d, (LogNormal, Gamma)):
-        raise TypeError(f"{what} must be a LogNormal or a Gamma distribution, "
+        raise TypeError(f"{what} must be a LogNormal, Gamma or MultiModal distribution, "
                         f"not {type(d).__name__}")
     p0, p1 = d.params()
     try:

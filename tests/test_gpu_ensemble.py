@@ -163,3 +163,78 @@ def test_fp16_split_angular_kernel_within_1e6_of_the_oracle():
         pe = np.max(np.abs(ph[r] - ref["phase"])) / np.max(np.abs(ref["phase"][0]))
         print(f"kernel {kernel} ensemble {e}: alpha1 {ga:.2e}, greek {gall:.2e}, phase {pe:.2e}")
         assert ga <= 1e-6 and gall <= 1e-6 and pe <= 1e-6, (e, ga, gall, pe)
+
+
+def test_mixture_and_multimodal_match_the_oracle():
+    """Mixture (a lognormal of spheres, a gamma law of coated spheres, a bimodal lognormal of
+    spheres) and a MultiModal Ensemble through the GPU path (one library call per layer count,
+    mixed by mix_parts) against the oracle's mixture, normalized and not."""
+    from scatrs_amd.ensemble import SCALARS, Ensemble, Mixture
+    from scatrs_amd.particles import LayeredSphere, Sphere
+    from scatrs_amd.size import Gamma, LogNormal, MultiModal
+
+    wl = np.array([0.45, 0.8, 1.3])
+    sph = Sphere(1.45 + 0.01j)
+    coated = LayeredSphere([0.7], [1.6 + 0.1j, 1.35 + 1e-5j])
+    bimodal = MultiModal(((3.0, LogNormal(0.05, 1.4)), (1.0, LogNormal(0.6, 1.8))))
+    mix = Mixture(((2.0, Ensemble(sph, LogNormal(0.3, 1.5))),
+                   (1.0, Ensemble(coated, Gamma(0.4, 0.15))),
+                   (1.0, Ensemble(sph, bimodal))))
+    kw = dict(n_radius=64, n_angles=48)
+    parts = [(0.5, 0, 0.3, 1.5, 1.45 + 0.01j, None),
+             (0.25, 1, 0.4, 0.15, np.array([1.6 + 0.1j, 1.35 + 1e-5j]), np.array([0.7])),
+             (0.25 * 0.75, 0, 0.05, 1.4, 1.45 + 0.01j, None),
+             (0.25 * 0.25, 0, 0.6, 1.8, 1.45 + 0.01j, None)]
+    for normalized in (True, False):
+        r = mix.compute(wl, greek=16, normalized=normalized, **kw)
+        b = Ensemble(sph, bimodal).compute(wl, greek=16, normalized=normalized, **kw)
+        for j, w in enumerate(wl):
+            ref = oracle.ensemble_mixture(parts, w, order=16, normalized=normalized, **kw)
+            got = np.array([getattr(r, k)[j] for k in SCALARS])
+            rel = np.abs(got - ref["scalars"]) / np.maximum(np.abs(ref["scalars"]), 1e-300)
+            assert rel.max() <= 1e-10, (normalized, j, rel)
+            a1 = r.greek.alpha1[j]
+            assert np.max(np.abs(a1 - ref["greek"][0])) <= 1e-10 * np.max(np.abs(ref["greek"][0]))
+            assert np.max(np.abs(r.phase.f11[j] - ref["phase"][0])) <= \
+                1e-10 * np.max(np.abs(ref["phase"][0]))
+            bref = oracle.ensemble_mixture([(0.75, *parts[2][1:]), (0.25, *parts[3][1:])], w,
+                                           order=16, normalized=normalized, **kw)
+            assert abs(b.c_sca[j] / bref["scalars"][1] - 1) <= 1e-10
+            assert abs(b.effective_radius[j] / bref["scalars"][8] - 1) <= 1e-10
+            assert np.max(np.abs(b.greek.beta1[j] - bref["greek"][4])) <= \
+                1e-10 * np.max(np.abs(bref["greek"][0]))
+
+
+def test_mixture_gradient_through_autograd_matches_central_differences():
+    """ensemble_fixed(..., normalized=False) for the parts and mix_parts: the gradient of a
+    loss on the mixture's ssa, g and alpha1 with respect to the parts' parameters, indices and
+    number fractions, against central differences of the same GPU forward (h = 1e-6
+    relative)."""
+    from scatrs_amd.ensemble import ensemble_fixed, mix_parts
+
+    kw = dict(n_radius=48, n_angles=64, order=12, normalized=False)
+    p0 = torch.tensor([0.25, 0.7], dtype=torch.float64, device="cuda")
+    p1 = torch.tensor([1.5, 0.2], dtype=torch.float64, device="cuda")
+    kind = torch.tensor([0, 1], dtype=torch.int32, device="cuda")
+    wl = torch.tensor([0.6, 0.6], dtype=torch.float64, device="cuda")
+    mr = torch.tensor([1.45, 1.55], dtype=torch.float64, device="cuda")
+    mi = torch.tensor([0.01, 1e-3], dtype=torch.float64, device="cuda")
+    f = torch.tensor([0.3, 0.7], dtype=torch.float64, device="cuda")
+
+    def loss(p0, p1, mr, mi, f):
+        sc, gr, ph = ensemble_fixed(kind, p0, p1, wl, mr, mi, **kw)
+        s, g, _ = mix_parts(sc[:, None], gr[:, None], ph[:, None], f, True)
+        return s[0, 4] ** 2 + 3.0 * s[0, 5] + (g[0, 0, :6] * torch.arange(6, device="cuda")).sum()
+
+    xs = [t.clone().requires_grad_(True) for t in (p0, p1, mr, mi, f)]
+    loss(*xs).backward()
+    for i, x in enumerate(xs):
+        for k in range(2):
+            h = 1e-6 * abs(float(x[k].detach()))
+            up = [t.detach().clone() for t in xs]
+            dn = [t.detach().clone() for t in xs]
+            up[i][k] += h
+            dn[i][k] -= h
+            fd = (float(loss(*up)) - float(loss(*dn))) / (2 * h)
+            got = float(x.grad[k])
+            assert abs(got - fd) <= 1e-6 * max(abs(fd), 1e-3), (i, k, got, fd)
