@@ -28,7 +28,7 @@ extern "C" {
 #define SCATRS_NOT_CONVERGED 3
 #define SCATRS_DEVICE_ERROR 4
 
-#define SCATRS_ABI_VERSION 1
+#define SCATRS_ABI_VERSION 2
 
 typedef struct scatrs_ctx scatrs_ctx;
 
@@ -82,16 +82,24 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
                             int64_t n_max, int allow_unvalidated, double *d_values,
                             double *d_jacobian, int64_t *total_terms);
 
-/* Size distributions (scatrs/src/ensemble/distribution.rs:15-68). */
-#define SCATRS_LOGNORMAL 0 /* p0 = median radius r_g, p1 = width sigma_g */
-#define SCATRS_GAMMA 1     /* p0 = effective radius, p1 = effective variance */
+/* Size distributions (scatrs/src/ensemble/distribution.rs:15-68) and their parameters. */
+#define SCATRS_LOGNORMAL 0      /* p0 = median radius r_g, p1 = width sigma_g */
+#define SCATRS_GAMMA 1          /* p0 = effective radius, p1 = effective variance */
+#define SCATRS_MODIFIED_GAMMA 2 /* n ~ r^alpha exp(-b r^gamma): p0 = alpha, p1 = b, p2 = gamma */
+#define SCATRS_POWER_LAW 3      /* n ~ r^-alpha on [p1, p2]: p0 = alpha, p1 = r_min, p2 = r_max */
+#define SCATRS_MONODISPERSE 4   /* p0 = radius */
 
 /* The fixed discrete path of a batch of ensembles (one distribution at one wavelength each;
  * the reference's EnsembleResult granularity, scatrs/src/ensemble/mod.rs:508-529):
  *   n_radius Gauss-Legendre nodes in ln r over the distribution's fixed range
  *   (lognormal: ln r_g +- 5 ln sigma_g; gamma: ln r_eff - 8 s .. ln r_eff +
- *   ln(1 + 37 v_eff + sqrt(74 v_eff)), s = sqrt(v_eff / (1 - 2 v_eff))), weights
+ *   ln(1 + 37 v_eff + sqrt(74 v_eff)), s = sqrt(v_eff / (1 - 2 v_eff)); modified gamma, in
+ *   y = b r^gamma: ln y from (ln Gamma(kA + 1) - 37)/kA to ln(kA + 37 + sqrt(74 kA)),
+ *   kA = (alpha + 3)/gamma, t = (ln y - ln b)/gamma; power law: its
+ *   support; monodisperse: every node at the radius, weights w r), weights
  *   w (t_hi - t_lo)/2 u(r) r;
+ *   d_p0, d_p1 (count) and d_p2 (count; may be NULL when no ensemble needs it: 0) the
+ *   distribution parameters as the SCATRS_* kinds list them;
  *   the phase matrix at the n_angles-point Gauss-Legendre cosines (2 <= n_angles <= 4096);
  *   Greek to `order` (d_greek == NULL: none).
  * Spheres: layers == 1 and d_m (count, 2) is each ensemble's index. Layered spheres:
@@ -106,7 +114,8 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
  * (scatrs-py/src/ensemble.rs:295-355,456-500) on this fixed discrete path (the reference's
  * adaptive Gauss-Kronrod panels are not part of this path). Synchronizes once per call. */
 int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
-                          const double *d_p0, const double *d_p1, const double *d_wavelength,
+                          const double *d_p0, const double *d_p1, const double *d_p2,
+                          const double *d_wavelength,
                           int layers, const double *d_fractions, const double *d_m,
                           int64_t n_radius, int64_t n_angles, int64_t order, int normalized,
                           int allow_unvalidated, double *d_scalars, double *d_greek,
@@ -115,18 +124,21 @@ int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
 /* Reverse mode of scatrs_ensemble_fixed (spheres or layered spheres): given the cotangents of
  * its outputs (d_bar_scalars (count, 11), d_bar_greek (count, 6, order+1), d_bar_phase
  * (count, 6, n_angles); each may be NULL for zero), the vector-Jacobian product with respect to
- * each ensemble's parameters, d_grad (count, P) with P = 2 + 2 L + (L - 1):
- *   [p0, p1, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 .. fraction_{L-2}]
+ * each ensemble's parameters, d_grad (count, P) with P = 3 + 2 L + (L - 1):
+ *   [p0, p1, p2, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 .. fraction_{L-2}]
+ * (0 in the columns of unused parameters and of a power law's fixed limits, the reference's
+ * parameter_names, distribution.rs:189-198)
  * (layer 0 the core; the fractions act at a fixed outer radius, the reference's
  * layer{l}.radius_fraction direction, scatrs/src/ensemble/particle.rs:221-249).
  * fixed_range = 0: the derivative of this discrete path, whose radius nodes and range move with
- * (p0, p1); fixed_range = 1: the ln r range held at its value, so (p0, p1) act through the
- * density alone, the reference's definition (docs/theory/ensembles.md:150,339-340,
+ * the parameters; fixed_range = 1: the ln r range held at its value, so the parameters act
+ * through the density alone, the reference's definition (a monodisperse law's range is its
+ * radius, which moves its nodes either way) (docs/theory/ensembles.md:150,339-340,
  * ensemble/engine.rs:1793-1804). a_n, b_n and their derivatives are recomputed, not stored.
  * Replaces the reference's forward-mode derivatives (EnsembleOptions::derivatives,
  * scatrs/src/ensemble/mod.rs:356-357, contracted by the caller) with one reverse pass. */
 int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
-                              const double *d_p0, const double *d_p1,
+                              const double *d_p0, const double *d_p1, const double *d_p2,
                               const double *d_wavelength, int layers,
                               const double *d_fractions, const double *d_m, int64_t n_radius,
                               int64_t n_angles, int64_t order, int normalized, int fixed_range,

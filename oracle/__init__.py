@@ -53,24 +53,22 @@ def lib() -> C.CDLL:
         L.orc_layered_solve.argtypes = [C.c_int, _dp, _cp, C.c_int64, C.c_int, _cp, _cp,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_density.restype = C.c_double
-        L.orc_density.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p]
-        L.orc_radius_range.argtypes = [C.c_int, C.c_double, C.c_double,
-                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        L.orc_ensemble_fixed.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
-                                         C.c_void_p, _cp, C.c_int64, C.c_int64, C.c_int64,
-                                         C.c_int, _dp, C.c_void_p, C.c_void_p]
-        L.orc_ensemble_fixed_ex.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
-                                            C.c_int, C.c_void_p, _cp, C.c_int64, C.c_int64,
-                                            C.c_int64, C.c_int, C.c_void_p, C.c_void_p, _dp,
-                                            C.c_void_p, C.c_void_p]
-        L.orc_ensemble_fixed_jacobian.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
-                                                  C.c_int, C.c_void_p, _cp, C.c_int64,
-                                                  C.c_int64, C.c_int64, C.c_int, C.c_int, _dp,
-                                                  C.c_void_p, C.c_void_p, _dp, C.c_void_p,
-                                                  C.c_void_p, C.c_void_p]
-        L.orc_ensemble_batch.argtypes = [C.c_int64, _ip, _dp, _dp, _dp, _cp, C.c_int64,
-                                         C.c_int64, C.c_int64, C.c_int, _dp, C.c_void_p,
-                                         C.c_void_p, C.c_int]
+        L.orc_density.argtypes = [C.c_int, _dp, C.c_double, C.c_void_p]
+        L.orc_radius_range.argtypes = [C.c_int, _dp, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        L.orc_ensemble_fixed.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p, _cp,
+                                         C.c_int64, C.c_int64, C.c_int64, C.c_int, _dp,
+                                         C.c_void_p, C.c_void_p]
+        L.orc_ensemble_fixed_ex.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p, _cp,
+                                            C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_void_p,
+                                            C.c_void_p, _dp, C.c_void_p, C.c_void_p]
+        L.orc_ensemble_fixed_jacobian.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p,
+                                                  _cp, C.c_int64, C.c_int64, C.c_int64, C.c_int,
+                                                  C.c_int, _dp, C.c_void_p, C.c_void_p, _dp,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_ensemble_batch.argtypes = [C.c_int64, _ip, _dp, _dp, C.c_void_p, _dp, _cp,
+                                         C.c_int64, C.c_int64, C.c_int64, C.c_int, _dp,
+                                         C.c_void_p, C.c_void_p, C.c_int]
         _LIB = L
     return _LIB
 
@@ -182,28 +180,33 @@ def layered_solve(x, m, n: int | None = None, derivatives: bool = False) -> dict
     return out
 
 
-def density(kind: int, p0: float, p1: float, r: float):
-    g = np.zeros(2)
-    u = lib().orc_density(kind, p0, p1, r, g.ctypes.data)
+def _params(p0, p1, p2) -> np.ndarray:
+    return np.array([p0, p1, p2], np.float64)
+
+
+def density(kind: int, p0: float, p1: float, r: float, p2: float = 0.0):
+    g = np.zeros(3)
+    u = lib().orc_density(kind, _params(p0, p1, p2), r, g.ctypes.data)
     return u, g
 
 
-def radius_range(kind: int, p0: float, p1: float):
+def radius_range(kind: int, p0: float, p1: float, p2: float = 0.0):
     lo, hi = C.c_double(), C.c_double()
-    if lib().orc_radius_range(kind, p0, p1, C.byref(lo), C.byref(hi)) != 0:
+    if lib().orc_radius_range(kind, _params(p0, p1, p2), C.byref(lo), C.byref(hi)) != 0:
         raise ValueError("invalid distribution")
     return lo.value, hi.value
 
 
-LOGNORMAL, GAMMA = 0, 1
+LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE = 0, 1, 2, 3, 4
 
 
 def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fractions=None,
                    n_radius: int = 128, n_angles: int = 128, order: int = 32,
-                   normalized: bool = True, t_range=None, n_nodes=None) -> dict:
+                   normalized: bool = True, t_range=None, n_nodes=None, p2: float = 0.0) -> dict:
     """One ensemble on the fixed discrete path: scalars (11,), greek (6, L+1), phase (6, n_q).
-    t_range = (t_lo, t_hi) replaces the distribution's ln r range and n_nodes the per-node N
-    (for finite differences at a fixed range or fixed truncations)."""
+    p2 is the third parameter of a modified gamma or power law. t_range = (t_lo, t_hi) replaces
+    the distribution's ln r range and n_nodes the per-node N (for finite differences at a fixed
+    range or fixed truncations)."""
     m = np.atleast_1d(np.asarray(m, np.complex128)).copy()
     layers = m.size
     fr = None if fractions is None else np.ascontiguousarray(fractions, np.float64)
@@ -212,7 +215,8 @@ def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fracti
     phase = np.zeros(6 * n_angles)
     rng = None if t_range is None else np.ascontiguousarray(t_range, np.float64)
     nn = None if n_nodes is None else np.ascontiguousarray(n_nodes, np.int64)
-    rc = lib().orc_ensemble_fixed_ex(kind, p0, p1, wavelength, layers, _ptr(fr), m, n_radius,
+    rc = lib().orc_ensemble_fixed_ex(kind, _params(p0, p1, p2), wavelength, layers, _ptr(fr), m,
+                                     n_radius,
                                      n_angles, order, int(normalized), _ptr(rng), _ptr(nn),
                                      scalars, _ptr(greek), _ptr(phase))
     if rc != 0:
@@ -225,12 +229,13 @@ def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fracti
 def ensemble_fixed_jacobian(kind: int, p0: float, p1: float, wavelength: float, m,
                             fractions=None, n_radius: int = 128, n_angles: int = 128,
                             order: int = 32, normalized: bool = True,
-                            fixed_range: bool = False) -> dict:
-    """Values and forward-mode derivatives of one ensemble along [p0, p1, Re m_0, Im m_0, ...,
-    fraction_0, ...]: d_scalars (P, 11), d_greek (P, 6, L+1), d_phase (P, 6, n_q), n_nodes."""
+                            fixed_range: bool = False, p2: float = 0.0) -> dict:
+    """Values and forward-mode derivatives of one ensemble along [p0, p1, p2, Re m_0, Im m_0,
+    ..., fraction_0, ...]: d_scalars (P, 11), d_greek (P, 6, L+1), d_phase (P, 6, n_q),
+    n_nodes."""
     m = np.atleast_1d(np.asarray(m, np.complex128)).copy()
     layers = m.size
-    P = 2 + 2 * layers + (layers - 1)
+    P = 3 + 2 * layers + (layers - 1)
     fr = None if fractions is None else np.ascontiguousarray(fractions, np.float64)
     scalars = np.zeros(11)
     greek = np.zeros(6 * (order + 1))
@@ -239,7 +244,8 @@ def ensemble_fixed_jacobian(kind: int, p0: float, p1: float, wavelength: float, 
     dg = np.zeros(P * 6 * (order + 1))
     dp = np.zeros(P * 6 * n_angles)
     nn = np.zeros(n_radius, np.int64)
-    rc = lib().orc_ensemble_fixed_jacobian(kind, p0, p1, wavelength, layers, _ptr(fr), m,
+    rc = lib().orc_ensemble_fixed_jacobian(kind, _params(p0, p1, p2), wavelength, layers,
+                                           _ptr(fr), m,
                                            n_radius, n_angles, order, int(normalized),
                                            int(fixed_range), scalars, _ptr(greek), _ptr(phase),
                                            ds, _ptr(dg), _ptr(dp), _ptr(nn))
@@ -252,7 +258,8 @@ def ensemble_fixed_jacobian(kind: int, p0: float, p1: float, wavelength: float, 
 
 
 def ensemble_batch(kind, p0, p1, wavelength, m, n_radius: int, n_angles: int, order: int,
-                   normalized: bool = True, threads: int = 0, want_tables: bool = True):
+                   normalized: bool = True, threads: int = 0, want_tables: bool = True,
+                   p2=None):
     kind = np.ascontiguousarray(kind, np.int32)
     cnt = kind.size
     p0 = np.ascontiguousarray(p0, np.float64)
@@ -262,7 +269,8 @@ def ensemble_batch(kind, p0, p1, wavelength, m, n_radius: int, n_angles: int, or
     scalars = np.zeros((cnt, 11))
     greek = np.zeros((cnt, 6, order + 1)) if want_tables else None
     phase = np.zeros((cnt, 6, n_angles)) if want_tables else None
-    rc = lib().orc_ensemble_batch(cnt, kind, p0, p1, wl, m, n_radius, n_angles, order,
+    p2 = None if p2 is None else np.ascontiguousarray(p2, np.float64)
+    rc = lib().orc_ensemble_batch(cnt, kind, p0, p1, _ptr(p2), wl, m, n_radius, n_angles, order,
                                   int(normalized), scalars, _ptr(greek), _ptr(phase),
                                   int(threads))
     if rc != 0:
@@ -285,9 +293,10 @@ def ensemble_mixture(parts, wavelength: float, n_radius: int = 128, n_angles: in
     acc = np.zeros(7)
     greek = np.zeros((6, order + 1)) if order >= 0 else None
     phase = np.zeros((6, n_angles))
-    for (f, kind, p0, p1, m, fr) in parts:
+    for (f, kind, p0, p1, m, fr, *rest) in parts:
         r = ensemble_fixed(kind, p0, p1, wavelength, m, fr, n_radius=n_radius,
-                           n_angles=n_angles, order=order, normalized=False)
+                           n_angles=n_angles, order=order, normalized=False,
+                           p2=rest[0] if rest else 0.0)
         s = r["scalars"]
         avg = np.array([s[0], s[1], s[2], s[3], s[5] * s[1], s[9] / np.pi,
                         s[10] * 3.0 / (4.0 * np.pi)])

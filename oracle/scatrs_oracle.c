@@ -1066,37 +1066,114 @@ int orc_layered_solve(int layers, const double *x, const orc_cx *m, int64_t n, i
     return 0;
 }
 
-/* ---- size distributions (ensemble/distribution.rs:293-430) ---- */
-static double dist_reference(int kind, double p0, double p1) {
-    if (kind == ORC_LOGNORMAL) return p0;
-    double alpha = (1.0 - 3.0 * p1) / p1;
-    return alpha > 0.0 ? alpha * p0 * p1 : p0;
+#define GAMMA_TAIL_MG 37.0
+/* ---- size distributions (ensemble/distribution.rs:15-68,293-430) ----
+ * Parameters p[3] per kind: lognormal (r_g, sigma_g, -), gamma (a = r_eff, b = v_eff, -),
+ * modified gamma (alpha, b, gamma), power law (alpha; r_min, r_max fixed), monodisperse
+ * (radius, -, -). */
+static double dist_reference(int kind, const double *p) {
+    if (kind == ORC_LOGNORMAL) return p[0];
+    if (kind == ORC_GAMMA) {
+        double alpha = (1.0 - 3.0 * p[1]) / p[1];
+        return alpha > 0.0 ? alpha * p[0] * p[1] : p[0];
+    }
+    if (kind == ORC_MODIFIED_GAMMA) return pow(fmax(p[0], 1.0) / (p[1] * p[2]), 1.0 / p[2]);
+    if (kind == ORC_POWER_LAW) return p[0] >= 0.0 ? p[1] : p[2];
+    return 1.0;
 }
 
-double orc_density(int kind, double p0, double p1, double r, double grad[2]) {
-    double r0 = dist_reference(kind, p0, p1);
-    double ln_u, d0, d1;
+double orc_density(int kind, const double *p, double r, double grad[3]) {
+    double r0 = dist_reference(kind, p);
+    double ln_u = 0.0, d[3] = {0.0, 0.0, 0.0};
     if (kind == ORC_LOGNORMAL) {
-        double mu = log(p0), s = log(p1);
+        double mu = log(p[0]), s = log(p[1]);
         double z = log(r) - mu;
         ln_u = -z * z / (2.0 * s * s) - log(s) - log(r / r0);
-        d0 = z / (s * s * p0);
-        d1 = (z * z / (s * s * s) - 1.0 / s) / p1;
-    } else {
-        double a = p0, b = p1;
+        d[0] = z / (s * s * p[0]);
+        d[1] = (z * z / (s * s * s) - 1.0 / s) / p[1];
+    } else if (kind == ORC_GAMMA) {
+        double a = p[0], b = p[1];
         double alpha = (1.0 - 3.0 * b) / b;
         double scale = a * b;
         double l = log(r / r0);
         ln_u = alpha * l - (r - r0) / scale;
-        d0 = (r - r0) * b / (scale * scale);
-        d1 = -l / (b * b) + (r - r0) * a / (scale * scale);
+        d[0] = (r - r0) * b / (scale * scale);
+        d[1] = -l / (b * b) + (r - r0) * a / (scale * scale);
+    } else if (kind == ORC_MODIFIED_GAMMA) {
+        double alpha = p[0], b = p[1], g = p[2];
+        double rg = pow(r, g), r0g = pow(r0, g);
+        double l = log(r / r0);
+        ln_u = alpha * l - b * (rg - r0g);
+        d[0] = l;
+        d[1] = -(rg - r0g);
+        d[2] = -b * (rg * log(r) - r0g * log(r0));
+    } else if (kind == ORC_POWER_LAW) {
+        double l = log(r / r0);
+        ln_u = -p[0] * l;
+        d[0] = -l;
     }
     double u = exp(ln_u);
-    if (grad) {
-        grad[0] = u * d0;
-        grad[1] = u * d1;
-    }
+    if (grad)
+        for (int k = 0; k < 3; k++) grad[k] = u * d[k];
     return u;
+}
+
+/* Whether the parameters are valid (distribution.rs:99-185). */
+static int dist_valid(int kind, const double *p) {
+    if (kind == ORC_LOGNORMAL) return p[0] > 0.0 && isfinite(p[0]) && p[1] > 1.0 && isfinite(p[1]);
+    if (kind == ORC_GAMMA) return p[0] > 0.0 && isfinite(p[0]) && p[1] > 0.0 && p[1] < 0.5;
+    if (kind == ORC_MODIFIED_GAMMA)
+        return isfinite(p[0]) && p[0] > -1.0 && p[1] > 0.0 && isfinite(p[1]) && p[2] > 0.0 &&
+               isfinite(p[2]);
+    if (kind == ORC_POWER_LAW)
+        return isfinite(p[0]) && p[1] > 0.0 && isfinite(p[1]) && p[2] > p[1] && isfinite(p[2]);
+    if (kind == ORC_MONODISPERSE) return p[0] > 0.0 && isfinite(p[0]);
+    return 0;
+}
+
+/* psi(x), x > 0: the recurrence psi(x) = psi(x + 1) - 1/x up to x >= 6, then the asymptotic
+ * series ln x - 1/(2x) - sum B_2k / (2k x^2k) (Abramowitz & Stegun 6.3.18). */
+static double digamma(double x) {
+    double acc = 0.0;
+    while (x < 6.0) {
+        acc -= 1.0 / x;
+        x += 1.0;
+    }
+    double q = 1.0 / (x * x);
+    double series = q * (1.0 / 12.0 - q * (1.0 / 120.0 - q * (1.0 / 252.0 - q * (1.0 / 240.0 -
+                                                                                 q * (1.0 / 132.0)))));
+    return acc + log(x) - 0.5 / x - series;
+}
+
+/* The modified gamma law in y = b r^gamma: the number density is Gamma((alpha + 1)/gamma) in
+ * y, the area-weighted one Gamma(kA = (alpha + 3)/gamma). The range bounds both tails of the
+ * area-weighted law (the cross sections'): below, P(Y < y) <= y^kA / Gamma(kA + 1) = e^-G,
+ * ln y_lo = (ln Gamma(kA + 1) - G)/kA; above, as for the gamma law (Chernoff),
+ * y_hi = kA + G + sqrt(2 G kA); t = (ln y - ln b)/gamma. Number averages are those of the law
+ * truncated to the range. With the derivatives of (t_lo, t_hi) with respect to the parameters
+ * when dlo, dhi are not NULL. */
+static void mgamma_range(const double *p, double *t_lo, double *t_hi, double dlo[3],
+                         double dhi[3]) {
+    const double G = GAMMA_TAIL_MG;
+    double alpha = p[0], b = p[1], g = p[2];
+    double ka = (alpha + 3.0) / g;
+    double lylo = (lgamma(ka + 1.0) - G) / ka;
+    double rt = sqrt(2.0 * G * ka);
+    double yhi = ka + G + rt;
+    double lyhi = log(yhi);
+    *t_lo = (lylo - log(b)) / g;
+    *t_hi = (lyhi - log(b)) / g;
+    if (dlo) {
+        double dlylo_dka = (digamma(ka + 1.0) - lylo) / ka;
+        double dlyhi_dka = (1.0 + G / rt) / yhi;
+        /* dkA/dalpha = 1/gamma, dkA/dgamma = -kA/gamma */
+        dlo[0] = dlylo_dka / (g * g);
+        dlo[1] = -1.0 / (b * g);
+        dlo[2] = dlylo_dka * (-ka / g) / g - *t_lo / g;
+        dhi[0] = dlyhi_dka / (g * g);
+        dhi[1] = -1.0 / (b * g);
+        dhi[2] = dlyhi_dka * (-ka / g) / g - *t_hi / g;
+    }
 }
 
 /* The fixed radius range in ln r. Lognormal: ln r_g -+ 5 ln sigma_g. Gamma (a = r_eff,
@@ -1104,25 +1181,36 @@ double orc_density(int kind, double p0, double p1, double r, double grad[2]) {
  * peak (distribution.rs:319-323), the lower side long for the left skew of ln r under a gamma
  * law; to ln a + ln(1 + G b + sqrt(2 G b)), G = 37: the area-weighted law is Gamma(k = 1/b,
  * scale a b) in r, whose upper tail beyond k + c scale units is at most
- * exp(-(c - k ln(1 + c/k))) (Chernoff), below e^-G for c = G + sqrt(2 k G). */
+ * exp(-(c - k ln(1 + c/k))) (Chernoff), below e^-G for c = G + sqrt(2 k G). Modified gamma:
+ * mgamma_range. Power law: its support [ln r_min, ln r_max]. Monodisperse: the radius (a range
+ * of width 0). */
 static const double GAMMA_TAIL = 37.0;
-int orc_radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi) {
+int orc_radius_range(int kind, const double *p, double *t_lo, double *t_hi) {
+    if (!dist_valid(kind, p)) return -1;
     if (kind == ORC_LOGNORMAL) {
-        if (!(p0 > 0.0 && isfinite(p0) && p1 > 1.0 && isfinite(p1))) return -1;
-        double c = log(p0), h = 5.0 * log(p1);
+        double c = log(p[0]), h = 5.0 * log(p[1]);
         *t_lo = c - h;
         *t_hi = c + h;
-        return 0;
-    }
-    if (kind == ORC_GAMMA) {
-        if (!(p0 > 0.0 && isfinite(p0) && p1 > 0.0 && p1 < 0.5)) return -1;
-        double s = sqrt(p1 / (1.0 - 2.0 * p1));
-        double c = log(p0);
+    } else if (kind == ORC_GAMMA) {
+        double s = sqrt(p[1] / (1.0 - 2.0 * p[1]));
+        double c = log(p[0]);
         *t_lo = c - 8.0 * s;
-        *t_hi = c + log(1.0 + GAMMA_TAIL * p1 + sqrt(2.0 * GAMMA_TAIL * p1));
-        return 0;
+        *t_hi = c + log(1.0 + GAMMA_TAIL * p[1] + sqrt(2.0 * GAMMA_TAIL * p[1]));
+    } else if (kind == ORC_MODIFIED_GAMMA) {
+        mgamma_range(p, t_lo, t_hi, NULL, NULL);
+    } else if (kind == ORC_POWER_LAW) {
+        *t_lo = log(p[1]);
+        *t_hi = log(p[2]);
+    } else {
+        *t_lo = *t_hi = log(p[0]);
     }
-    return -1;
+    return 0;
+}
+
+/* The quadrature weight of radius node i: w_i (t_hi - t_lo)/2 u(r) r, and w_i r for a
+ * monodisperse law (all nodes at its radius). */
+static double node_weight(int kind, double w, double half, double u, double r) {
+    return w * (kind == ORC_MONODISPERSE ? 1.0 : half) * u * r;
 }
 
 /* ---- the fixed-node ensemble ---- */
@@ -1166,13 +1254,13 @@ static void ensemble_outputs(const double S[8], double *T, int64_t n_angles, con
     }
 }
 
-int orc_ensemble_fixed_ex(int kind, double p0, double p1, double lambda, int layers,
+int orc_ensemble_fixed_ex(int kind, const double *p, double lambda, int layers,
                           const double *fractions, const orc_cx *m, int64_t n_radius,
                           int64_t n_angles, int64_t order, int normalized, const double *range,
                           const int64_t *n_nodes, double *scalars, double *greek,
                           double *phase) {
     double t_lo, t_hi;
-    if (orc_radius_range(kind, p0, p1, &t_lo, &t_hi) != 0) return -1;
+    if (orc_radius_range(kind, p, &t_lo, &t_hi) != 0) return -1;
     if (range) {
         t_lo = range[0];
         t_hi = range[1];
@@ -1193,8 +1281,8 @@ int orc_ensemble_fixed_ex(int kind, double p0, double p1, double lambda, int lay
     for (int64_t i = 0; i < n_radius; i++) {
         double t = mid + half * rn[i];
         double r = exp(t);
-        double u = orc_density(kind, p0, p1, r, NULL);
-        double W = rw[i] * half * u * r;
+        double u = orc_density(kind, p, r, NULL);
+        double W = node_weight(kind, rw[i], half, u, r);
         double x = 2.0 * PI * r / lambda;
         int64_t n = n_nodes ? n_nodes[i] : orc_default_n_max(x);
         cx *ab = (cx *)malloc(sizeof(cx) * 2 * (size_t)n);
@@ -1230,42 +1318,58 @@ int orc_ensemble_fixed_ex(int kind, double p0, double p1, double lambda, int lay
     return 0;
 }
 
-int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers,
+int orc_ensemble_fixed(int kind, const double *p, double lambda, int layers,
                        const double *fractions, const orc_cx *m, int64_t n_radius,
                        int64_t n_angles, int64_t order, int normalized, double *scalars,
                        double *greek, double *phase) {
-    return orc_ensemble_fixed_ex(kind, p0, p1, lambda, layers, fractions, m, n_radius, n_angles,
+    return orc_ensemble_fixed_ex(kind, p, lambda, layers, fractions, m, n_radius, n_angles,
                                  order, normalized, NULL, NULL, scalars, greek, phase);
 }
 
 /* d(ln u)/d(ln r) at fixed parameters, from the densities of distribution.rs:366-430. */
-static double log_density_slope(int kind, double p0, double p1, double r) {
+static double log_density_slope(int kind, const double *p, double r) {
     if (kind == ORC_LOGNORMAL) {
-        double s = log(p1);
-        return -(log(r) - log(p0)) / (s * s) - 1.0;
+        double s = log(p[1]);
+        return -(log(r) - log(p[0])) / (s * s) - 1.0;
     }
-    return (1.0 - 3.0 * p1) / p1 - r / (p0 * p1);
+    if (kind == ORC_GAMMA) return (1.0 - 3.0 * p[1]) / p[1] - r / (p[0] * p[1]);
+    if (kind == ORC_MODIFIED_GAMMA) return p[0] - p[1] * p[2] * pow(r, p[2]);
+    if (kind == ORC_POWER_LAW) return -p[0];
+    return 0.0;
 }
 
-/* Derivatives of the range's midpoint and half-width in ln r with respect to (p0, p1). */
-static double range_derivatives(int kind, double p0, double p1, double dmid[2], double dhalf[2]) {
+/* Derivatives of the range's midpoint and half-width in ln r with respect to the three
+ * parameters; returns the half-width. */
+static double range_derivatives(int kind, const double *p, double dmid[3], double dhalf[3]) {
+    for (int k = 0; k < 3; k++) dmid[k] = dhalf[k] = 0.0;
     if (kind == ORC_LOGNORMAL) {
-        dmid[0] = 1.0 / p0;
-        dmid[1] = 0.0;
-        dhalf[0] = 0.0;
-        dhalf[1] = 5.0 / p1;
-        return 5.0 * log(p1);
+        dmid[0] = 1.0 / p[0];
+        dhalf[1] = 5.0 / p[1];
+        return 5.0 * log(p[1]);
     }
-    double q = 1.0 - 2.0 * p1, s = sqrt(p1 / q);
-    double ds = 0.5 / (s * q * q);
-    double rt = sqrt(2.0 * GAMMA_TAIL * p1);
-    double up = log(1.0 + GAMMA_TAIL * p1 + rt);
-    double dup = (GAMMA_TAIL + GAMMA_TAIL / rt) / (1.0 + GAMMA_TAIL * p1 + rt);
-    dmid[0] = 1.0 / p0;
-    dmid[1] = 0.5 * (dup - 8.0 * ds);
-    dhalf[0] = 0.0;
-    dhalf[1] = 0.5 * (dup + 8.0 * ds);
-    return 0.5 * (up + 8.0 * s);
+    if (kind == ORC_GAMMA) {
+        double q = 1.0 - 2.0 * p[1], s = sqrt(p[1] / q);
+        double ds = 0.5 / (s * q * q);
+        double rt = sqrt(2.0 * GAMMA_TAIL * p[1]);
+        double up = log(1.0 + GAMMA_TAIL * p[1] + rt);
+        double dup = (GAMMA_TAIL + GAMMA_TAIL / rt) / (1.0 + GAMMA_TAIL * p[1] + rt);
+        dmid[0] = 1.0 / p[0];
+        dmid[1] = 0.5 * (dup - 8.0 * ds);
+        dhalf[1] = 0.5 * (dup + 8.0 * ds);
+        return 0.5 * (up + 8.0 * s);
+    }
+    if (kind == ORC_MODIFIED_GAMMA) {
+        double t_lo, t_hi, dlo[3], dhi[3];
+        mgamma_range(p, &t_lo, &t_hi, dlo, dhi);
+        for (int k = 0; k < 3; k++) {
+            dmid[k] = 0.5 * (dhi[k] + dlo[k]);
+            dhalf[k] = 0.5 * (dhi[k] - dlo[k]);
+        }
+        return 0.5 * (t_hi - t_lo);
+    }
+    if (kind == ORC_POWER_LAW) return 0.5 * (log(p[2]) - log(p[1]));
+    dmid[0] = 1.0 / p[0]; /* monodisperse: every node at the radius */
+    return 0.0;
 }
 
 /* The derivatives of the phase elements from those of S1, S2 (mie/phase.rs:60-73). */
@@ -1285,21 +1389,22 @@ static void element_derivatives(cx s1, cx s2, cx d1, cx d2, double out[4]) {
  * the phase-element derivatives from the amplitude derivatives (phase.rs:60-73); the weights'
  * derivatives; then the quotient rule through the averages, the Greek normalization and the
  * phase normalization (engine.rs:1782-1836,2396-2497; lin.rs:7-68). Directions
- * P = 2 + 2L + (L - 1): [p0, p1, Re m_0, Im m_0, ..., fraction_0 ...]. With fixed_range the ln r
- * range is held fixed (the parameters act through the density alone, the reference's
- * definition); otherwise the nodes move with (p0, p1). n_nodes (may be NULL) receives N per
- * node. Returns 0 or a negative code. */
-int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, int layers,
+ * P = 3 + 2L + (L - 1): [p0, p1, p2, Re m_0, Im m_0, ..., fraction_0 ...] (the columns of a
+ * distribution's unused or fixed parameters are 0). With fixed_range the ln r range is held
+ * fixed (the parameters act through the density alone, the reference's definition); otherwise
+ * the nodes move with the parameters. n_nodes (may be NULL) receives N per node. Returns 0 or a
+ * negative code. */
+int orc_ensemble_fixed_jacobian(int kind, const double *p, double lambda, int layers,
                                 const double *fractions, const orc_cx *m, int64_t n_radius,
                                 int64_t n_angles, int64_t order, int normalized,
                                 int fixed_range, double *scalars, double *greek, double *phase,
                                 double *d_scalars, double *d_greek, double *d_phase,
                                 int64_t *n_nodes) {
     double t_lo, t_hi;
-    if (orc_radius_range(kind, p0, p1, &t_lo, &t_hi) != 0) return -1;
+    if (orc_radius_range(kind, p, &t_lo, &t_hi) != 0) return -1;
     if (!(lambda > 0.0 && isfinite(lambda)) || layers < 1 || n_radius < 1 || n_angles < 1) return -2;
     const int L = layers;
-    const int P = 2 + 2 * L + (L - 1);
+    const int P = 3 + 2 * L + (L - 1);
     double *rn = (double *)malloc(sizeof(double) * (size_t)n_radius);
     double *rw = (double *)malloc(sizeof(double) * (size_t)n_radius);
     orc_gauss_legendre(n_radius, rn, rw);
@@ -1307,8 +1412,8 @@ int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, i
     double *mw = (double *)malloc(sizeof(double) * (size_t)n_angles);
     orc_gauss_legendre(n_angles, mu, mw);
     double half = 0.5 * (t_hi - t_lo), mid = 0.5 * (t_hi + t_lo);
-    double dmid[2], dhalf[2];
-    range_derivatives(kind, p0, p1, dmid, dhalf);
+    double dmid[3], dhalf[3];
+    range_derivatives(kind, p, dmid, dhalf);
     double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double *dS = (double *)calloc(8 * (size_t)P, sizeof(double));
     double *T = (double *)calloc(4 * (size_t)n_angles, sizeof(double));
@@ -1323,9 +1428,9 @@ int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, i
     for (int64_t i = 0; i < n_radius; i++) {
         double t = mid + half * rn[i];
         double r = exp(t);
-        double ugrad[2];
-        double u = orc_density(kind, p0, p1, r, ugrad);
-        double W = rw[i] * half * u * r;
+        double ugrad[3];
+        double u = orc_density(kind, p, r, ugrad);
+        double W = node_weight(kind, rw[i], half, u, r);
         double x = 2.0 * PI * r / lambda;
         int64_t n = orc_default_n_max(x);
         if (n_nodes) n_nodes[i] = n;
@@ -1407,10 +1512,13 @@ int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, i
                 for (int k = 1; k <= 5; k++) dh[k] += 2.0 * h[k];
                 dh[6] = 2.0 * h[6];
                 dh[7] = 3.0 * h[7];
-                for (int th = 0; th < 2; th++) {
-                    double dlnr = fixed_range ? 0.0 : dmid[th] + rn[i] * dhalf[th];
-                    double dlnw = (fixed_range ? 0.0 : dhalf[th] / half) + ugrad[th] / u +
-                                  (log_density_slope(kind, p0, p1, r) + 1.0) * dlnr;
+                /* A monodisperse law's range is its radius: the radius moves its nodes
+                 * under either definition. */
+                const int hold = fixed_range && kind != ORC_MONODISPERSE;
+                for (int th = 0; th < 3; th++) {
+                    double dlnr = hold ? 0.0 : dmid[th] + rn[i] * dhalf[th];
+                    double dlnw = (hold || half == 0.0 ? 0.0 : dhalf[th] / half) +
+                                  ugrad[th] / u + (log_density_slope(kind, p, r) + 1.0) * dlnr;
                     double *dSp = dS + 8 * th;
                     for (int k = 0; k < 8; k++) dSp[k] += W * dlnw * h[k] + W * dh[k] * dlnr;
                     double *dTp = dT + (size_t)th * 4 * (size_t)n_angles;
@@ -1428,10 +1536,10 @@ int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, i
                     }
                 }
             } else {
-                int p = d + 1;
-                double *dSp = dS + 8 * p;
+                int pc = d + 2;
+                double *dSp = dS + 8 * pc;
                 for (int k = 0; k < 8; k++) dSp[k] += W * dh[k];
-                double *dTp = dT + (size_t)p * 4 * (size_t)n_angles;
+                double *dTp = dT + (size_t)pc * 4 * (size_t)n_angles;
                 for (int64_t j = 0; j < n_angles; j++) {
                     double df[4];
                     element_derivatives(s1[j], s2[j], ds1[j], ds2[j], df);
@@ -1506,7 +1614,7 @@ int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, i
 }
 
 int orc_ensemble_batch(int64_t count, const int *kind, const double *p0, const double *p1,
-                       const double *lambda, const double *m, int64_t n_radius,
+                       const double *p2, const double *lambda, const double *m, int64_t n_radius,
                        int64_t n_angles, int64_t order, int normalized, double *scalars,
                        double *greek, double *phase, int threads) {
     int bad = 0;
@@ -1516,7 +1624,8 @@ int orc_ensemble_batch(int64_t count, const int *kind, const double *p0, const d
 #endif
     for (int64_t e = 0; e < count; e++) {
         cx mm = C(m[2 * e], m[2 * e + 1]);
-        int rc = orc_ensemble_fixed(kind[e], p0[e], p1[e], lambda[e], 1, NULL, &mm, n_radius,
+        const double pe[3] = {p0[e], p1[e], p2 ? p2[e] : 0.0};
+        int rc = orc_ensemble_fixed(kind[e], pe, lambda[e], 1, NULL, &mm, n_radius,
                                     n_angles, order, normalized, scalars + 11 * e,
                                     greek ? greek + 6 * (order + 1) * e : NULL,
                                     phase ? phase + 6 * n_angles * e : NULL);

@@ -90,23 +90,31 @@ int orc_layered_solve(int layers, const double *x, const orc_cx *m, int64_t n, i
                       orc_cx *a, orc_cx *b, orc_cx *dgrp, double *abs_sum, double *abs_jac);
 
 /* Size distributions (ensemble/distribution.rs:15-68,366-430). */
-enum { ORC_LOGNORMAL = 0, ORC_GAMMA = 1 };
+enum {
+    ORC_LOGNORMAL = 0,      /* p = (r_g, sigma_g, -) */
+    ORC_GAMMA = 1,          /* p = (r_eff, v_eff, -) */
+    ORC_MODIFIED_GAMMA = 2, /* p = (alpha, b, gamma) */
+    ORC_POWER_LAW = 3,      /* p = (alpha, r_min, r_max): r_min, r_max fixed */
+    ORC_MONODISPERSE = 4    /* p = (radius, -, -) */
+};
 
-/* The unnormalized density u(r) with the reference radius fixed, and grad[2] = du/dtheta. */
-double orc_density(int kind, double p0, double p1, double r, double grad[2]);
+/* The unnormalized density u(r) of parameters p[3] with the reference radius fixed, and
+ * grad[3] = du/dp (0 for fixed or unused parameters). */
+double orc_density(int kind, const double *p, double r, double grad[3]);
 
 /* The fixed discrete path of one ensemble (one distribution at one wavelength):
  *   n_radius Gauss-Legendre nodes in t = ln r over [t_lo, t_hi] (orc_radius_range);
  *   per node the Mie (or layered) solution, its per-particle quantities
  *   [1, C_ext, C_sca, C_sca g, C_abs, C_back, r^2, r^3] (ensemble/engine.rs:1046-1056) and its
  *   phase tables pi r^2 (4/x^2) [F11, F12, F33, F34] at the n_angles GL cosines;
- *   quadrature weights W_i = w_i (t_hi - t_lo)/2 u(r_i) r_i; number averages S/Z;
+ *   quadrature weights W_i = w_i (t_hi - t_lo)/2 u(r_i) r_i (w_i r for a monodisperse law, every
+ *   node at its radius); number averages S/Z;
  *   outputs assembled as engine.rs:2396-2497 does: scalars[11] in SCALARS order, the Greek
  *   coefficients 6 x (order+1) (normalized by the projected alpha1_0 if normalized), and the
  *   phase matrix 6 x n_angles [F11, F12, F22, F33, F34, F44] (divided by <C_sca> if normalized).
  * layers == 1 is a sphere (fractions unused). Returns 0, or a negative code for invalid input. */
-int orc_radius_range(int kind, double p0, double p1, double *t_lo, double *t_hi);
-int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers,
+int orc_radius_range(int kind, const double *p, double *t_lo, double *t_hi);
+int orc_ensemble_fixed(int kind, const double *p, double lambda, int layers,
                        const double *fractions, const orc_cx *m, int64_t n_radius,
                        int64_t n_angles, int64_t order, int normalized, double *scalars,
                        double *greek, double *phase);
@@ -114,18 +122,18 @@ int orc_ensemble_fixed(int kind, double p0, double p1, double lambda, int layers
 /* orc_ensemble_fixed with the ln r range given as range[2] = (t_lo, t_hi) instead of the
  * distribution's, and N per node given as n_nodes[n_radius] (either may be NULL): for finite
  * differences at a fixed range and fixed truncations. */
-int orc_ensemble_fixed_ex(int kind, double p0, double p1, double lambda, int layers,
+int orc_ensemble_fixed_ex(int kind, const double *p, double lambda, int layers,
                           const double *fractions, const orc_cx *m, int64_t n_radius,
                           int64_t n_angles, int64_t order, int normalized, const double *range,
                           const int64_t *n_nodes, double *scalars, double *greek,
                           double *phase);
 
-/* The outputs of orc_ensemble_fixed and their forward-mode derivatives along the P = 2 + 2L +
- * (L - 1) directions [p0, p1, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 ..
+/* The outputs of orc_ensemble_fixed and their forward-mode derivatives along the P = 3 + 2L +
+ * (L - 1) directions [p0, p1, p2, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 ..
  * fraction_{L-2}]: d_scalars (P, 11), d_greek (P, 6, order+1) or NULL, d_phase (P, 6, n_angles)
  * or NULL. fixed_range = 1 holds the ln r range fixed (the reference's derivative definition);
- * 0 moves the nodes with (p0, p1). n_nodes (may be NULL) receives N per node. */
-int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, int layers,
+ * 0 moves the nodes with the parameters. n_nodes (may be NULL) receives N per node. */
+int orc_ensemble_fixed_jacobian(int kind, const double *p, double lambda, int layers,
                                 const double *fractions, const orc_cx *m, int64_t n_radius,
                                 int64_t n_angles, int64_t order, int normalized,
                                 int fixed_range, double *scalars, double *greek, double *phase,
@@ -135,6 +143,7 @@ int orc_ensemble_fixed_jacobian(int kind, double p0, double p1, double lambda, i
 /* Many ensembles (each its own distribution and wavelength, spheres of one index per
  * ensemble), in parallel over ensembles on `threads` OpenMP threads: the CPU baseline. */
 int orc_ensemble_batch(int64_t count, const int *kind, const double *p0, const double *p1,
+                       const double *p2,
                        const double *lambda, const double *m, int64_t n_radius,
                        int64_t n_angles, int64_t order, int normalized, double *scalars,
                        double *greek, double *phase, int threads);

@@ -18,7 +18,7 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libscatrs_hip.so"
 
 OK, INVALID_INPUT, OUTSIDE_VALIDATED_DOMAIN, NOT_CONVERGED, DEVICE_ERROR = 0, 1, 2, 3, 4
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _lib = None
 _lock = threading.Lock()
@@ -72,12 +72,12 @@ def load() -> C.CDLL:
         L.scatrs_mie_efficiencies.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
                                               C.POINTER(i64)]
         if hasattr(L, "scatrs_ensemble_fixed"):
-            L.scatrs_ensemble_fixed.argtypes = [vp, i64, vp, vp, vp, vp, i32, vp, vp, i64,
+            L.scatrs_ensemble_fixed.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp, i64,
                                                 i64, i64, i32, i32, vp, vp, vp,
                                                 C.POINTER(i64)]
         if hasattr(L, "scatrs_ensemble_fixed_vjp"):
-            L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, i32, vp, vp, i64,
-                                                    i64, i64, i32, i32, i32, vp, vp, vp, vp,
+            L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp,
+                                                    i64, i64, i64, i32, i32, i32, vp, vp, vp, vp,
                                                     C.POINTER(i64)]
         if L.scatrs_abi_version() != ABI_VERSION:
             raise NativeLibraryError("libscatrs_hip.so has a different ABI version")

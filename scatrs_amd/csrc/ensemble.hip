@@ -8,7 +8,7 @@
 using namespace scatrs;
 
 extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
-                                     const double *d_p0, const double *d_p1,
+                                     const double *d_p0, const double *d_p1, const double *d_p2,
                                      const double *d_wavelength, int layers,
                                      const double *d_fractions, const double *d_m,
                                      int64_t n_radius, int64_t n_angles, int64_t order,
@@ -24,7 +24,7 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
     EnsSetup S;
-    if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_wavelength, layers, d_fractions, d_m,
+    if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_greek != nullptr, order, allow_unvalidated,
                         ctx->angular_kernel, S)))
         return rc;

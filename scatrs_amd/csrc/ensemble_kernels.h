@@ -49,7 +49,15 @@ struct EnsArgs {
     int layers;
     int64_t n_radius;
     const double *rnodes, *rweights;  // GL rule on [-1, 1]
+    const double *p2;                 // third parameter (may be null: 0)
 };
+
+// Ensemble e's distribution parameters.
+__device__ __forceinline__ void ens_params(const EnsArgs &a, int64_t e, double p[3]) {
+    p[0] = a.p0[e];
+    p[1] = a.p1[e];
+    p[2] = a.p2 ? a.p2[e] : 0.0;
+}
 
 struct NodeGeom {
     double r, x, W;
@@ -57,16 +65,17 @@ struct NodeGeom {
 
 __device__ __forceinline__ NodeGeom node_geometry(const EnsArgs &a, int64_t e, int64_t i) {
     const int kind = a.kind[e];
-    const double p0 = a.p0[e], p1 = a.p1[e];
+    double p[3];
+    ens_params(a, e, p);
     double t_lo, t_hi;
-    radius_range(kind, p0, p1, &t_lo, &t_hi);
+    radius_range(kind, p, &t_lo, &t_hi);
     const double half = 0.5 * (t_hi - t_lo), mid = 0.5 * (t_hi + t_lo);
     const double r = exp(mid + half * a.rnodes[i]);
-    const double u = exp(log_density(kind, p0, p1, r, nullptr, nullptr));
+    const double u = exp(log_density(kind, p, r, nullptr));
     NodeGeom g;
     g.r = r;
     g.x = 2.0 * kPi * r / a.wl[e];
-    g.W = a.rweights[i] * half * u * r;
+    g.W = node_weight(kind, a.rweights[i], half, u, r);
     return g;
 }
 
@@ -116,7 +125,9 @@ static __global__ void __launch_bounds__(256) ens_prep(EnsArgs a, int64_t count,
     if (g < total) {
         const int64_t e = g / a.n_radius, i = g % a.n_radius;
         int code = kValid;
-        if (!dist_valid(a.kind[e], a.p0[e], a.p1[e])) code = kInvalid | kBadDist;
+        double pe[3];
+        ens_params(a, e, pe);
+        if (!dist_valid(a.kind[e], pe)) code = kInvalid | kBadDist;
         else if (!(isfinite(a.wl[e]) && a.wl[e] > 0.0)) code = kInvalid | kBadWl;
         else if (a.layers > 1) {
             const double *fr = a.fr + e * (a.layers - 1);
@@ -1093,11 +1104,13 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
     const int64_t e = g / n_radius, i = g % n_radius;
     const int L = dev.layers;
     int32_t kind = 0;
-    double p0 = 0, p1 = 0, wl = 0;
+    double p[3] = {0.0, 0.0, 0.0}, wl = 0;
     std::vector<double> fr(std::max(L - 1, 0)), m(2 * L);
     (void)hipMemcpy(&kind, dev.kind + e, sizeof kind, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&p0, dev.p0 + e, sizeof p0, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&p1, dev.p1 + e, sizeof p1, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(&p[0], dev.p0 + e, sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipMemcpy(&p[1], dev.p1 + e, sizeof(double), hipMemcpyDeviceToHost);
+    if (dev.p2) (void)hipMemcpy(&p[2], dev.p2 + e, sizeof(double), hipMemcpyDeviceToHost);
+    const double p0 = p[0], p1 = p[1], p2 = p[2];
     (void)hipMemcpy(&wl, dev.wl + e, sizeof wl, hipMemcpyDeviceToHost);
     if (L > 1)
         (void)hipMemcpy(fr.data(), dev.fr + e * (L - 1), fr.size() * sizeof(double),
@@ -1105,17 +1118,33 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
     (void)hipMemcpy(m.data(), dev.m + e * 2 * L, m.size() * sizeof(double), hipMemcpyDeviceToHost);
     const std::string pre = "ensemble " + std::to_string(e) + ": ";
     *code_out = kInvalid;
-    if (!dist_valid(kind, p0, p1)) {
-        if (kind == 0) {
-            if (!(std::isfinite(p0) && p0 > 0.0))
-                return pre + "median_radius must be finite and > 0, got " + fmt_e(p0);
+    if (!dist_valid(kind, p)) {
+        // The reference's messages (distribution.rs:99-185).
+        const auto positive = [&](const char *name, double v) {
+            return pre + name + " must be finite and > 0, got " + fmt_e(v);
+        };
+        const bool ok0 = std::isfinite(p0) && p0 > 0.0;
+        if (kind == kLogNormal) {
+            if (!ok0) return positive("median_radius", p0);
             return pre + "the lognormal width must be finite and > 1, got " + fmt_e(p1);
         }
-        if (kind == 1) {
-            if (!(std::isfinite(p0) && p0 > 0.0))
-                return pre + "effective_radius must be finite and > 0, got " + fmt_e(p0);
+        if (kind == kGamma) {
+            if (!ok0) return positive("effective_radius", p0);
             return pre + "effective_variance must be in (0, 0.5), got " + fmt_e(p1);
         }
+        if (kind == kModifiedGamma) {
+            if (!(std::isfinite(p0) && p0 > -1.0))
+                return pre + "alpha must be finite and > -1, got " + fmt_e(p0);
+            if (!(std::isfinite(p1) && p1 > 0.0)) return positive("b", p1);
+            return positive("gamma", p2);
+        }
+        if (kind == kPowerLaw) {
+            if (!std::isfinite(p0)) return pre + "exponent must be finite, got " + fmt_e(p0);
+            if (!(std::isfinite(p1) && p1 > 0.0)) return positive("min_radius", p1);
+            if (!(std::isfinite(p2) && p2 > 0.0)) return positive("max_radius", p2);
+            return pre + "min_radius must be < max_radius, got " + fmt_e(p1) + " and " + fmt_e(p2);
+        }
+        if (kind == kMonodisperse) return positive("radius", p0);
         return pre + "unknown distribution kind " + std::to_string(kind);
     }
     if (!(std::isfinite(wl) && wl > 0.0))
@@ -1124,7 +1153,7 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
         if (!(fr[l] > 0.0 && fr[l] <= 1.0) || (l > 0 && fr[l] < fr[l - 1]))
             return pre + "the radius fractions must be nondecreasing in (0, 1]";
     double t_lo, t_hi;
-    radius_range(kind, p0, p1, &t_lo, &t_hi);
+    radius_range(kind, p, &t_lo, &t_hi);
     const double r = std::exp(0.5 * (t_hi + t_lo) + 0.5 * (t_hi - t_lo) * rnodes[i]);
     const double x = 2.0 * kPi * r / wl;
     const std::string at = pre + "at wavelength " + fmt_e(wl) + ", radius " + fmt_e(r) + ": ";
@@ -1180,7 +1209,8 @@ struct EnsSetup {
 };
 
 static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, const double *d_p0,
-                     const double *d_p1, const double *d_wavelength, int layers,
+                     const double *d_p1, const double *d_p2, const double *d_wavelength,
+                     int layers,
                      const double *d_fractions, const double *d_m, int64_t n_radius,
                      int64_t n_angles, bool greek, int64_t order, int allow_unvalidated,
                      int kernel, EnsSetup &S) {
@@ -1215,7 +1245,7 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     S.d_rtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc]);
     S.d_atab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc2]);
     S.a = EnsArgs{d_kind, d_p0, d_p1, d_wavelength, d_fractions, d_m, layers, n_radius,
-                  S.d_rtab, S.d_rtab + n_radius};
+                  S.d_rtab, S.d_rtab + n_radius, d_p2};
     // Per-node N and status.
     const int64_t total = count * n_radius;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, sizeof(Status) + 64), "alloc")))

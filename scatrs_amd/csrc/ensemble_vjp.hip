@@ -549,19 +549,20 @@ __global__ __launch_bounds__(256) void ens_nodes_vjp_ranked(
 }
 
 // One thread per node of the chunk: gnode[g * P + p], the node's share of the gradient,
-// p = [p0, p1, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, f_0 .. f_{L-2}], from the solve's
-// contractions and the node's weight and radius. dH/dx_l, dH/dRe m_l, dH/dIm m_l with
+// p = [p0, p1, p2, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, f_0 .. f_{L-2}], from the
+// solve's contractions and the node's weight and radius. dH/dx_l, dH/dRe m_l, dH/dIm m_l with
 // H = phi / W; C_abs = (lambda^2/2pi) abs_sum enters C_ext and C_abs. With fixed_range the
 // ln r range is held at its value (the distribution parameters act through the density alone,
 // as the reference's derivatives do); otherwise the nodes and the range's half-width move with
-// (p0, p1), the derivative of this discrete path.
+// the parameters, the derivative of this discrete path. A monodisperse law's range is its
+// radius, which moves its nodes either way.
 static __global__ __launch_bounds__(256) void ens_node_grads(
     EnsArgs a, int64_t e0, int64_t chunk_nodes, const double *raw, const double *vals,
     const double *tabc, const double *ht, const double *sbar, int fixed_range, double *gnode) {
     const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (g >= chunk_nodes) return;
     const int L = a.layers;
-    const int P = 2 + 2 * L + (L - 1);
+    const int P = 3 + 2 * L + (L - 1);
     const int64_t eb = g / a.n_radius, i = g % a.n_radius;
     const int64_t e = e0 + eb;
     const NodeGeom geo = node_geometry(a, e, i);
@@ -577,10 +578,10 @@ static __global__ __launch_bounds__(256) void ens_node_grads(
         const double dx = rw[l] + cabs * rw[3 * L + 3 * l];
         const double dre = rw[L + l] + cabs * rw[3 * L + 3 * l + 1];
         const double dim = -rw[2 * L + l] + cabs * rw[3 * L + 3 * l + 2];
-        out[2 + 2 * l] = W * dre;
-        out[3 + 2 * l] = W * dim;
+        out[3 + 2 * l] = W * dre;
+        out[4 + 2 * l] = W * dim;
         if (l < L - 1) {
-            out[2 + 2 * L + l] = W * geo.x * dx;
+            out[3 + 2 * L + l] = W * geo.x * dx;
             dxt += fr[l] * dx;
         } else {
             dxt += dx;
@@ -591,15 +592,18 @@ static __global__ __launch_bounds__(256) void ens_node_grads(
     double phi = tabc[g] * ht[g];
     for (int k = 0; k < kNodeVals; ++k) phi += sb[k] * v[k];
     const int kind = a.kind[e];
-    const double p0 = a.p0[e], p1 = a.p1[e];
-    double dmid[2], dhalf[2], du[2];
-    const double half = range_derivatives(kind, p0, p1, dmid, dhalf);
-    log_density(kind, p0, p1, r, &du[0], &du[1]);
-    const double slope = log_density_slope(kind, p0, p1, r);
+    double p[3];
+    ens_params(a, e, p);
+    double dmid[3], dhalf[3], du[3];
+    const double half = range_derivatives(kind, p, dmid, dhalf);
+    log_density(kind, p, r, du);
+    const double slope = log_density_slope(kind, p, r);
     const double xi = a.rnodes[i];
-    for (int th = 0; th < 2; ++th) {
-        const double dlnr = fixed_range ? 0.0 : dmid[th] + xi * dhalf[th];
-        const double dlnw = (fixed_range ? 0.0 : dhalf[th] / half) + du[th] + (slope + 1.0) * dlnr;
+    const bool hold = fixed_range && kind != kMonodisperse;
+    for (int th = 0; th < 3; ++th) {
+        const double dlnr = hold ? 0.0 : dmid[th] + xi * dhalf[th];
+        const double dlnw = (hold || half == 0.0 ? 0.0 : dhalf[th] / half) + du[th] +
+                            (slope + 1.0) * dlnr;
         out[th] = phi * dlnw + W * dh_dlnr * dlnr;
     }
 }
@@ -641,7 +645,8 @@ using namespace scatrs;
 
 extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
                                          const double *d_p0, const double *d_p1,
-                                         const double *d_wavelength, int layers,
+                                         const double *d_p2, const double *d_wavelength,
+                                         int layers,
                                          const double *d_fractions, const double *d_m,
                                          int64_t n_radius, int64_t n_angles, int64_t order,
                                          int normalized, int fixed_range,
@@ -658,7 +663,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
     EnsSetup S;
-    if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_wavelength, layers, d_fractions, d_m,
+    if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_bar_greek != nullptr, order, allow_unvalidated,
                         ctx->angular_kernel, S)))
         return rc;
@@ -667,7 +672,7 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     S.stage32 = false;
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
-    const int P = 2 + 2 * layers + (layers - 1);
+    const int P = 3 + 2 * layers + (layers - 1);
     const int h = S.h;
     const bool tables = d_bar_greek || d_bar_phase;
     // Staging and cotangents take 32 B per term each (and the fp32 copy 16 B with kernel 2).

@@ -20,7 +20,7 @@ import numpy as np
 from . import _native
 from .errors import InvalidInputError
 from .particles import LayeredSphere, Sphere
-from .size import Gamma, LogNormal, MultiModal, check
+from .size import Distribution, MultiModal, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
@@ -30,14 +30,16 @@ SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depo
            "effective_radius", "mean_area", "mean_volume")
 
 
-def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, n_radius: int = 128,
-                          n_angles: int = 128, order: int | None = 32, normalized: bool = True,
-                          allow_unvalidated: bool = False, want_phase: bool = True):
+def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, p2=None,
+                          n_radius: int = 128, n_angles: int = 128, order: int | None = 32,
+                          normalized: bool = True, allow_unvalidated: bool = False,
+                          want_phase: bool = True):
     """Device-resident batch: every argument a CUDA tensor with one row per ensemble
-    (kind int32; p0, p1, wavelength float64; m complex128 (count,) for spheres or (count, L)
-    for layered spheres; fractions float64 (count, L-1) for layered spheres). Returns
-    (scalars (count, 11), greek (count, 6, order+1) or None, phase (count, 6, n_angles) or
-    None, total Mie terms)."""
+    (kind int32; p0, p1, wavelength float64; p2 float64 or None, the third parameter of a
+    modified gamma or power law; m complex128 (count,) for spheres or (count, L) for layered
+    spheres; fractions float64 (count, L-1) for layered spheres). Returns (scalars (count, 11),
+    greek (count, 6, order+1) or None, phase (count, 6, n_angles) or None, total Mie
+    terms)."""
     import torch
 
     dev = p0.device
@@ -48,6 +50,7 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, n_radi
     args = [kind.to(torch.int32).contiguous(), p0.contiguous(), p1.contiguous(),
             wavelength.contiguous(), m.contiguous()]
     fr = None if layers == 1 else fractions.contiguous()
+    q2 = None if p2 is None else p2.to(torch.float64).contiguous()
     scalars = torch.empty((count, 11), dtype=torch.float64, device=dev)
     greek = None if order is None else torch.empty((count, 6, order + 1), dtype=torch.float64,
                                                    device=dev)
@@ -57,7 +60,7 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, n_radi
     terms = C.c_int64(0)
     p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
     rc = ctx.lib.scatrs_ensemble_fixed(
-        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(args[3]), layers, p(fr),
+        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(q2), p(args[3]), layers, p(fr),
         p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
         int(bool(normalized)), int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase),
         C.byref(terms))
@@ -67,9 +70,10 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, n_radi
 
 def parameter_names(layers: int) -> list[str]:
     """Names of the gradient columns of ensemble_fixed_vjp_device, in the reference's
-    parameter vocabulary (scatrs/src/ensemble/mod.rs:182-250): the distribution's two
-    parameters, then Re m and Im m per layer (core first), then the radius fractions."""
-    names = ["p0", "p1"]
+    parameter vocabulary (scatrs/src/ensemble/mod.rs:182-250): the distribution's three
+    parameter slots (0 where a kind has no such parameter), then Re m and Im m per layer (core
+    first), then the radius fractions."""
+    names = ["p0", "p1", "p2"]
     if layers == 1:
         return names + ["m_real", "m_imag"]
     for l in range(layers):
@@ -77,8 +81,9 @@ def parameter_names(layers: int) -> list[str]:
     return names + [f"layer{l}.radius_fraction" for l in range(layers - 1)]
 
 
-def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, bar_scalars=None,
-                              bar_greek=None, bar_phase=None, n_radius: int = 128,
+def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, p2=None,
+                              bar_scalars=None, bar_greek=None, bar_phase=None,
+                              n_radius: int = 128,
                               n_angles: int = 128, order: int | None = 32,
                               normalized: bool = True, fixed_range: bool = False,
                               allow_unvalidated: bool = False):
@@ -86,7 +91,7 @@ def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, ba
     (count, 11), bar_greek (count, 6, order+1) and bar_phase (count, 6, n_angles) (each may be
     None), the gradient (count, P) with respect to parameter_names(L), and the total Mie
     terms of one pass. fixed_range holds the ln r range at its value (the reference's
-    derivative definition); otherwise the radius nodes move with (p0, p1)."""
+    derivative definition); otherwise the radius nodes move with the parameters."""
     import torch
 
     dev = p0.device
@@ -105,13 +110,14 @@ def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, ba
             wavelength.contiguous(), m.contiguous()]
     fr = None if layers == 1 else fractions.contiguous()
     bs, bg, bp = c(bar_scalars), c(bar_greek), c(bar_phase)
-    P = 2 + 2 * layers + (layers - 1)
+    q2 = c(p2)
+    P = 3 + 2 * layers + (layers - 1)
     grad = torch.empty((count, P), dtype=torch.float64, device=dev)
     ctx = _native.context(dev.index)
     terms = C.c_int64(0)
     p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
     rc = ctx.lib.scatrs_ensemble_fixed_vjp(
-        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(args[3]), layers, p(fr),
+        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(q2), p(args[3]), layers, p(fr),
         p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
         int(bool(normalized)), int(bool(fixed_range)), int(bool(allow_unvalidated)), p(bs),
         p(bg), p(bp), p(grad), C.byref(terms))
@@ -123,43 +129,45 @@ def _autograd_function():
     import torch
 
     class EnsembleFixedFunction(torch.autograd.Function):
-        """(p0, p1, m_real, m_imag, fractions) -> (scalars, greek, phase) on the fixed path;
-        backward is the library's reverse-mode kernel (one VJP pass per backward)."""
+        """(p0, p1, p2, m_real, m_imag, fractions) -> (scalars, greek, phase) on the fixed
+        path; backward is the library's reverse-mode kernel (one VJP pass per backward)."""
 
         @staticmethod
-        def forward(ctx, p0, p1, m_real, m_imag, fractions, kind, wavelength, opts):
+        def forward(ctx, p0, p1, p2, m_real, m_imag, fractions, kind, wavelength, opts):
             ctx.set_materialize_grads(False)  # unused outputs reach backward as None
             m = torch.complex(m_real, m_imag)
             if m.dim() == 2 and m.shape[1] == 1:
                 m = m[:, 0]
             sc, gr, ph, _ = ensemble_fixed_device(
-                kind, p0, p1, wavelength, m, fractions, n_radius=opts["n_radius"],
+                kind, p0, p1, wavelength, m, fractions, p2=p2, n_radius=opts["n_radius"],
                 n_angles=opts["n_angles"], order=opts["order"], normalized=opts["normalized"],
                 allow_unvalidated=opts["allow_unvalidated"], want_phase=True)
-            ctx.save_for_backward(p0, p1, m_real, m_imag,
-                                  fractions if fractions is not None else torch.empty(0),
-                                  kind, wavelength)
+            empty = torch.empty(0)
+            ctx.save_for_backward(p0, p1, p2 if p2 is not None else empty, m_real, m_imag,
+                                  fractions if fractions is not None else empty, kind, wavelength)
             ctx.opts = opts
             ctx.has_fr = fractions is not None
+            ctx.has_p2 = p2 is not None
             return sc, gr, ph
 
         @staticmethod
         def backward(ctx, g_sc, g_gr, g_ph):
-            p0, p1, m_real, m_imag, fr, kind, wl = ctx.saved_tensors
+            p0, p1, p2, m_real, m_imag, fr, kind, wl = ctx.saved_tensors
             o = ctx.opts
             m = torch.complex(m_real, m_imag)
             layers = 1 if m.dim() == 1 else m.shape[1]
             if m.dim() == 2 and layers == 1:
                 m = m[:, 0]
             grad, _ = ensemble_fixed_vjp_device(
-                kind, p0, p1, wl, m, fr if ctx.has_fr else None, bar_scalars=g_sc,
-                bar_greek=g_gr, bar_phase=g_ph, n_radius=o["n_radius"], n_angles=o["n_angles"],
-                order=o["order"], normalized=o["normalized"], fixed_range=o["fixed_range"],
-                allow_unvalidated=o["allow_unvalidated"])
-            gm_re = grad[:, 2:2 + 2 * layers:2].reshape(m_real.shape)
-            gm_im = grad[:, 3:3 + 2 * layers:2].reshape(m_imag.shape)
-            gfr = grad[:, 2 + 2 * layers:] if ctx.has_fr else None
-            return grad[:, 0], grad[:, 1], gm_re, gm_im, gfr, None, None, None
+                kind, p0, p1, wl, m, fr if ctx.has_fr else None, p2=p2 if ctx.has_p2 else None,
+                bar_scalars=g_sc, bar_greek=g_gr, bar_phase=g_ph, n_radius=o["n_radius"],
+                n_angles=o["n_angles"], order=o["order"], normalized=o["normalized"],
+                fixed_range=o["fixed_range"], allow_unvalidated=o["allow_unvalidated"])
+            gm_re = grad[:, 3:3 + 2 * layers:2].reshape(m_real.shape)
+            gm_im = grad[:, 4:4 + 2 * layers:2].reshape(m_imag.shape)
+            gfr = grad[:, 3 + 2 * layers:] if ctx.has_fr else None
+            gp2 = grad[:, 2] if ctx.has_p2 else None
+            return grad[:, 0], grad[:, 1], gp2, gm_re, gm_im, gfr, None, None, None
 
     return EnsembleFixedFunction
 
@@ -167,22 +175,22 @@ def _autograd_function():
 _FUNCTION = None
 
 
-def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *,
+def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *, p2=None,
                    n_radius: int = 128, n_angles: int = 128, order: int = 32,
                    normalized: bool = True, fixed_range: bool = False,
                    allow_unvalidated: bool = False):
     """Differentiable batch of ensembles on the fixed path (torch.autograd): p0, p1,
-    wavelength (count,); m_real, m_imag (count,) for spheres or (count, L) for layered spheres
-    (core first); fractions (count, L-1). Returns (scalars (count, 11), greek
-    (count, 6, order+1), phase (count, 6, n_angles)); gradients flow to p0, p1, m_real,
-    m_imag and fractions through the library's reverse-mode kernel."""
+    wavelength (count,) and p2 (count,) or None; m_real, m_imag (count,) for spheres or
+    (count, L) for layered spheres (core first); fractions (count, L-1). Returns (scalars
+    (count, 11), greek (count, 6, order+1), phase (count, 6, n_angles)); gradients flow to p0,
+    p1, p2, m_real, m_imag and fractions through the library's reverse-mode kernel."""
     global _FUNCTION
     if _FUNCTION is None:
         _FUNCTION = _autograd_function()
     opts = dict(n_radius=int(n_radius), n_angles=int(n_angles), order=int(order),
                 normalized=bool(normalized), fixed_range=bool(fixed_range),
                 allow_unvalidated=bool(allow_unvalidated))
-    return _FUNCTION.apply(p0, p1, m_real, m_imag, fractions, kind, wavelength, opts)
+    return _FUNCTION.apply(p0, p1, p2, m_real, m_imag, fractions, kind, wavelength, opts)
 
 
 def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
@@ -223,7 +231,7 @@ def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
 
 def _parts_device(parts, wl: np.ndarray, *, order, n_radius, n_angles, allow_unvalidated,
                   want_phase):
-    """Each part (kind, p0, p1, particle) at every wavelength on the fixed path with
+    """Each part (kind, p0, p1, p2, particle) at every wavelength on the fixed path with
     normalized=False, as tensors (parts, wavelengths, ...); one library call per layer count."""
     import torch
 
@@ -237,19 +245,20 @@ def _parts_device(parts, wl: np.ndarray, *, order, n_radius, n_angles, allow_unv
         else None
     groups: dict[int, list[int]] = {}
     for i, part in enumerate(parts):
-        groups.setdefault(part[3].layers, []).append(i)
+        groups.setdefault(part[4].layers, []).append(i)
     terms = 0
     for L, idx in groups.items():
         kind = np.repeat(np.array([parts[i][0] for i in idx], np.int32), nw)
         p0 = np.repeat(np.array([parts[i][1] for i in idx], np.float64), nw)
         p1 = np.repeat(np.array([parts[i][2] for i in idx], np.float64), nw)
-        m = np.concatenate([parts[i][3].indices(wl).reshape(L, -1).T for i in idx])  # (n, L)
+        p2 = np.repeat(np.array([parts[i][3] for i in idx], np.float64), nw)
+        m = np.concatenate([parts[i][4].indices(wl).reshape(L, -1).T for i in idx])  # (n, L)
         fr = None
         if L > 1:
-            fr = t(np.concatenate([np.tile(parts[i][3].fractions(), (nw, 1)) for i in idx]))
+            fr = t(np.concatenate([np.tile(parts[i][4].fractions(), (nw, 1)) for i in idx]))
         s, g, p, n = ensemble_fixed_device(
             t(kind), t(p0), t(p1), t(np.tile(wl.reshape(-1), len(idx))),
-            t(m[:, 0] if L == 1 else m), fr, n_radius=n_radius, n_angles=n_angles,
+            t(m[:, 0] if L == 1 else m), fr, p2=t(p2), n_radius=n_radius, n_angles=n_angles,
             order=order, normalized=False, allow_unvalidated=allow_unvalidated,
             want_phase=want_phase)
         sel = torch.tensor(idx, device=dev)
@@ -333,11 +342,14 @@ def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int)
     return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()), terms=terms)
 
 
+def _part(size, particle):
+    """(kind, p0, p1, p2, particle) of a distribution with scalar parameters."""
+    return (size.kind, *(float(v) for v in size.params()), particle)
+
+
 def _mode_parts(size: MultiModal, particle, weight: float = 1.0):
-    """The parts of a multimodal distribution: (kind, p0, p1, particle) and number fractions."""
-    parts = [(d.kind, float(d.params()[0]), float(d.params()[1]), particle)
-             for _, d in size.modes]
-    return parts, list(weight * size.fractions())
+    """The parts of a multimodal distribution and their number fractions."""
+    return [_part(d, particle) for _, d in size.modes], list(weight * size.fractions())
 
 
 @dataclass(frozen=True)
@@ -349,7 +361,7 @@ class Ensemble:
     mixed (mix_parts)."""
 
     particle: Sphere | LayeredSphere
-    size: LogNormal | Gamma | MultiModal
+    size: Distribution
 
     def __post_init__(self) -> None:
         if not isinstance(self.particle, (Sphere, LayeredSphere)):
@@ -380,9 +392,9 @@ class Ensemble:
             f = torch.tensor(fr, dtype=torch.float64, device=sc.device)
             sc, gr, ph = mix_parts(sc, gr, ph, f, normalized)
             return _result(sc, gr, ph, wl.shape, wl, n_angles, terms)
-        p0, p1 = self.size.params()
-        grid = np.broadcast(p0, p1).shape
-        p0g, p1g = (np.broadcast_to(v, grid).reshape(-1) for v in (p0, p1))
+        p0, p1, p2 = self.size.params()
+        grid = np.broadcast(p0, p1, p2).shape
+        p0g, p1g, p2g = (np.broadcast_to(v, grid).reshape(-1) for v in (p0, p1, p2))
         nd, nw = p0g.size, wl.size
         m = self.particle.indices(wl).reshape(self.particle.layers, -1)  # (L, nw)
         L = self.particle.layers
@@ -395,9 +407,9 @@ class Ensemble:
             fr = t(np.tile(self.particle.fractions(), (nd * nw, 1)))
         sc, gr, ph, terms = ensemble_fixed_device(
             t(kind), t(np.repeat(p0g, nw)), t(np.repeat(p1g, nw)), t(np.tile(wl.reshape(-1), nd)),
-            t(mm[:, 0] if L == 1 else mm), fr, n_radius=n_radius, n_angles=n_angles,
-            order=greek, normalized=normalized, allow_unvalidated=allow_unvalidated,
-            want_phase=phase)
+            t(mm[:, 0] if L == 1 else mm), fr, p2=t(np.repeat(p2g, nw)), n_radius=n_radius,
+            n_angles=n_angles, order=greek, normalized=normalized,
+            allow_unvalidated=allow_unvalidated, want_phase=phase)
         return _result(sc, gr, ph, grid + wl.shape, wl, n_angles, terms)
 
 
@@ -457,8 +469,7 @@ class Mixture:
                 parts += p
                 fr += f
             else:
-                p0, p1 = e.size.params()
-                parts.append((e.size.kind, float(p0), float(p1), e.particle))
+                parts.append(_part(e.size, e.particle))
                 fr.append(w / total)
         sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                           n_angles=n_angles, allow_unvalidated=allow_unvalidated,

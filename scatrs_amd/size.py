@@ -9,7 +9,8 @@ import numpy as np
 
 from .errors import InvalidInputError
 
-LOGNORMAL, GAMMA = 0, 1
+LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE = 0, 1, 2, 3, 4
+_ZERO = np.float64(0.0)
 
 
 @dataclass(frozen=True)
@@ -24,7 +25,9 @@ class LogNormal:
     names = ("median_radius", "width")
 
     def params(self):
-        return np.asarray(self.median_radius, np.float64), np.asarray(self.width, np.float64)
+        """(p0, p1, p2) as the library takes them."""
+        return (np.asarray(self.median_radius, np.float64), np.asarray(self.width, np.float64),
+                _ZERO)
 
 
 @dataclass(frozen=True)
@@ -39,14 +42,68 @@ class Gamma:
     names = ("effective_radius", "effective_variance")
 
     def params(self):
+        """(p0, p1, p2) as the library takes them."""
         return (np.asarray(self.effective_radius, np.float64),
-                np.asarray(self.effective_variance, np.float64))
+                np.asarray(self.effective_variance, np.float64), _ZERO)
+
+
+@dataclass(frozen=True)
+class ModifiedGamma:
+    """Deirmendjian's modified gamma distribution n(r) ∝ r^alpha exp(-b r^gamma): alpha > -1,
+    b > 0, gamma > 0 (distribution.rs:32-40)."""
+
+    alpha: float | np.ndarray
+    b: float | np.ndarray
+    gamma: float | np.ndarray
+
+    kind = MODIFIED_GAMMA
+    names = ("alpha", "b", "gamma")
+
+    def params(self):
+        """(p0, p1, p2) as the library takes them."""
+        return tuple(np.asarray(v, np.float64) for v in (self.alpha, self.b, self.gamma))
+
+
+@dataclass(frozen=True)
+class PowerLaw:
+    """n(r) ∝ r^(-exponent) on [min_radius, max_radius], the limits fixed
+    (distribution.rs:41-49)."""
+
+    exponent: float | np.ndarray
+    min_radius: float | np.ndarray
+    max_radius: float | np.ndarray
+
+    kind = POWER_LAW
+    names = ("exponent",)
+
+    def params(self):
+        """(p0, p1, p2) as the library takes them."""
+        return tuple(np.asarray(v, np.float64)
+                     for v in (self.exponent, self.min_radius, self.max_radius))
+
+
+@dataclass(frozen=True)
+class Monodisperse:
+    """All spheres of one radius > 0 (distribution.rs:50-54)."""
+
+    radius: float | np.ndarray
+
+    kind = MONODISPERSE
+    names = ("radius",)
+
+    def params(self):
+        """(p0, p1, p2) as the library takes them."""
+        return np.asarray(self.radius, np.float64), _ZERO, _ZERO
+
+
+_SINGLE = (LogNormal, Gamma, ModifiedGamma, PowerLaw, Monodisperse)
 
 
 @dataclass(frozen=True)
 class MultiModal:
-    """sum_j w_j n_j(r) with each mode n_j normalized: ``modes`` of (weight w_j > 0, LogNormal
-    or Gamma with scalar parameters) (scatrs/src/ensemble/distribution.rs:62-67,172-180). Each
+    """sum_j w_j n_j(r) with each mode n_j normalized: ``modes`` of (weight w_j > 0, a
+    distribution other than MultiModal with scalar parameters)
+    (scatrs/src/ensemble/distribution.rs:62-67,172-180). Each
     mode is integrated on its own and the modes' averages are mixed with their number
     fractions w_j / sum w (scatrs/src/ensemble/engine.rs:143-160,2120-2150)."""
 
@@ -68,9 +125,9 @@ class MultiModal:
             w = float(w)
             if not (np.isfinite(w) and w > 0.0):
                 raise InvalidInputError(f"the weight of mode {j} must be finite and > 0, got {w}")
-            if not isinstance(d, (LogNormal, Gamma)):
-                raise TypeError(f"mode {j} must be a LogNormal or a Gamma distribution, not "
-                                f"{type(d).__name__}")
+            if not isinstance(d, _SINGLE):
+                raise TypeError(f"mode {j} must be a LogNormal, Gamma, ModifiedGamma, PowerLaw "
+                                f"or Monodisperse distribution, not {type(d).__name__}")
             if any(np.ndim(p) for p in d.params()):
                 raise InvalidInputError(f"mode {j}: a mode takes scalar parameters")
             out.append((w, d))
@@ -82,18 +139,17 @@ class MultiModal:
         return w / w.sum()
 
 
-Distribution = LogNormal | Gamma | MultiModal
+Distribution = LogNormal | Gamma | ModifiedGamma | PowerLaw | Monodisperse | MultiModal
 
 
-def check(d: object, what: str = "size") -> LogNormal | Gamma | MultiModal:
+def check(d: object, what: str = "size") -> Distribution:
     if isinstance(d, MultiModal):
         return d
-    if not isinstance(d, (LogNormal, Gamma)):
-        raise TypeError(f"{what} must be a LogNormal, Gamma or MultiModal distribution, "
-                        f"not {type(d).__name__}")
-    p0, p1 = d.params()
+    if not isinstance(d, _SINGLE):
+        raise TypeError(f"{what} must be a LogNormal, Gamma, ModifiedGamma, PowerLaw, "
+                        f"Monodisperse or MultiModal distribution, not {type(d).__name__}")
     try:
-        np.broadcast(p0, p1)
+        np.broadcast(*d.params())
     except ValueError as e:
         raise InvalidInputError(f"{what}: parameters do not broadcast: {e}") from None
     return d

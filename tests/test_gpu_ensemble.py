@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
+from scatrs_amd.errors import InvalidInputError
 
 pytestmark = pytest.mark.gpu
 
@@ -238,3 +239,45 @@ def test_mixture_gradient_through_autograd_matches_central_differences():
             fd = (float(loss(*up)) - float(loss(*dn))) / (2 * h)
             got = float(x.grad[k])
             assert abs(got - fd) <= 1e-6 * max(abs(fd), 1e-3), (i, k, got, fd)
+
+
+def test_new_distributions_match_the_oracle():
+    """Modified gamma, power law and monodisperse ensembles of spheres and coated spheres through
+    the C ABI (the third parameter in d_p2) against the oracle; the Python classes give the same
+    numbers; a monodisperse ensemble is its sphere."""
+    from scatrs_amd.ensemble import Ensemble
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import ModifiedGamma, Monodisperse, PowerLaw
+
+    kind = np.array([2, 2, 3, 3, 4, 4], np.int32)
+    p0 = np.array([2.0, 0.5, 3.0, 0.0, 0.4, 2.5])
+    p1 = np.array([3.0, 8.0, 0.05, 0.2, 0.0, 0.0])
+    p2 = np.array([1.2, 0.7, 2.0, 0.6, 0.0, 0.0])
+    wl = np.array([0.55, 0.8, 0.45, 1.1, 0.55, 0.6])
+    m = np.array([1.45 + 0.01j, 1.33 + 1e-6j, 1.5 + 0.001j, 1.6 + 0.2j, 1.5 + 0.01j, 1.4 + 0j])
+    kw = dict(n_radius=96, n_angles=64, order=24)
+    from scatrs_amd.ensemble import ensemble_fixed_device
+
+    sc, gr, ph, _ = ensemble_fixed_device(_dev(kind), _dev(p0), _dev(p1), _dev(wl), _dev(m),
+                                          p2=_dev(p2), **kw)
+    sc, gr, ph = sc.cpu().numpy(), gr.cpu().numpy(), ph.cpu().numpy()
+    for e in range(kind.size):
+        ref = oracle.ensemble_fixed(int(kind[e]), p0[e], p1[e], wl[e], m[e], p2=p2[e], **kw)
+        _check(sc[e], gr[e], ph[e], ref, 1e-10)
+    # the same through the classes
+    ck = dict(greek=24, n_radius=96, n_angles=64)
+    r = Ensemble(Sphere(m[0]), ModifiedGamma(2.0, 3.0, 1.2)).compute(0.55, **ck)
+    assert abs(r.c_ext / sc[0, 0] - 1) < 1e-14 and abs(r.g / sc[0, 5] - 1) < 1e-14
+    r = Ensemble(Sphere(m[2]), PowerLaw(3.0, 0.05, 2.0)).compute(0.45, **ck)
+    assert abs(r.c_sca / sc[2, 1] - 1) < 1e-14
+    r = Ensemble(Sphere(m[4]), Monodisperse(0.4)).compute(0.55, **ck)
+    x = 2 * np.pi * 0.4 / 0.55
+    q = oracle.efficiencies(x, oracle.mie_solve(x, m[4]))
+    assert abs(r.c_ext / (np.pi * 0.16 * q[0]) - 1) < 1e-12
+    assert abs(r.effective_radius / 0.4 - 1) < 1e-14
+    with pytest.raises(InvalidInputError, match="gamma must be finite and > 0"):
+        _gpu([2], [1.0], [1.0], [0.5], [1.5 + 0j], None, **kw)
+    with pytest.raises(InvalidInputError, match="min_radius must be < max_radius"):
+        ensemble_fixed_device(_dev(np.array([3], np.int32)), _dev(np.array([1.0])),
+                              _dev(np.array([0.5])), _dev(np.array([0.5])),
+                              _dev(np.array([1.5 + 0j])), p2=_dev(np.array([0.2])), **kw)

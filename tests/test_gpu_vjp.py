@@ -32,6 +32,8 @@ def _cotangents(rng, count, order, n_angles, which=("scalars", "greek", "phase")
 def _gpu_vjp(kind, p0, p1, wl, m, fr, bs, bg, bp, **kw):
     from scatrs_amd.ensemble import ensemble_fixed_vjp_device
 
+    if kw.get("p2") is not None:
+        kw["p2"] = _dev(np.asarray(kw["p2"], np.float64))
     g, terms = ensemble_fixed_vjp_device(
         _dev(np.asarray(kind, np.int32)), _dev(p0), _dev(p1), _dev(wl), _dev(m),
         None if fr is None else _dev(fr), bar_scalars=None if bs is None else _dev(bs),
@@ -41,10 +43,10 @@ def _gpu_vjp(kind, p0, p1, wl, m, fr, bs, bg, bp, **kw):
 
 
 def _oracle_vjp(kind, p0, p1, wl, m, fr, bs, bg, bp, n_radius, n_angles, order, normalized,
-                fixed_range):
+                fixed_range, p2=0.0):
     """sum over outputs of cotangent x derivative, and the matching scale sum |cot x deriv|."""
     J = oracle.ensemble_fixed_jacobian(kind, p0, p1, wl, m, fr, n_radius, n_angles, order,
-                                       normalized, fixed_range)
+                                       normalized, fixed_range, p2=p2)
     terms = []
     if bs is not None:
         terms.append(J["d_scalars"] * bs[None, :])
@@ -57,7 +59,8 @@ def _oracle_vjp(kind, p0, p1, wl, m, fr, bs, bg, bp, n_radius, n_angles, order, 
 
 
 def _assert_close(got, ref, scale, tol, what):
-    err = np.abs(got - ref) / scale
+    # a column with no dependence (an unused distribution slot) must be exactly 0
+    err = np.abs(got - ref) / np.where(scale > 0, scale, 1.0)
     assert err.max() <= tol, (what, err, got, ref)
 
 
@@ -99,6 +102,28 @@ def test_sphere_vjp_each_output_alone_and_unnormalized():
                 _assert_close(g[e], ref, scale, 1e-10, (which, normalized, e))
 
 
+@pytest.mark.parametrize("fixed_range", [False, True])
+def test_new_distributions_vjp_match_oracle_jacobian(fixed_range):
+    """Modified gamma, power law and monodisperse ensembles (spheres and a coated sphere):
+    the VJP against the oracle's contracted Jacobian, with moving nodes and with the range
+    held."""
+    rng = np.random.default_rng(15)
+    kind = np.array([2, 2, 3, 3, 4])
+    p0 = np.array([2.0, 0.6, 3.0, -1.0, 0.45])
+    p1 = np.array([6.0, 15.0, 0.05, 0.1, 0.0])
+    p2 = np.array([1.5, 0.8, 1.0, 0.6, 0.0])
+    wl = np.array([0.6, 0.9, 0.5, 1.2, 0.55])
+    m = np.array([1.5 + 0.01j, 1.38 + 1e-4j, 1.45 + 1e-3j, 1.6 + 0.1j, 1.52 + 0.02j])
+    nr, na, order = 40, 64, 20
+    bs, bg, bp = _cotangents(rng, kind.size, order, na)
+    g, _ = _gpu_vjp(kind, p0, p1, wl, m, None, bs, bg, bp, p2=p2, n_radius=nr, n_angles=na,
+                    order=order, fixed_range=fixed_range)
+    for e in range(kind.size):
+        ref, scale = _oracle_vjp(kind[e], p0[e], p1[e], wl[e], m[e], None, bs[e], bg[e], bp[e],
+                                 nr, na, order, True, fixed_range, p2=p2[e])
+        _assert_close(g[e], ref, scale, 1e-10, f"ensemble {e}")
+
+
 @pytest.mark.parametrize("layers", [2, 3, 5])
 def test_layered_vjp_matches_oracle_jacobian(layers):
     rng = np.random.default_rng(10 + layers)
@@ -113,7 +138,7 @@ def test_layered_vjp_matches_oracle_jacobian(layers):
     nr, na, order = 24, 48, 16
     bs, bg, bp = _cotangents(rng, n, order, na)
     g, _ = _gpu_vjp(kind, p0, p1, wl, m, fr, bs, bg, bp, n_radius=nr, n_angles=na, order=order)
-    assert g.shape == (n, 2 + 2 * layers + layers - 1)
+    assert g.shape == (n, 3 + 2 * layers + layers - 1)
     for e in range(n):
         ref, scale = _oracle_vjp(1, p0[e], p1[e], wl[e], m[e], fr[e], bs[e], bg[e], bp[e], nr,
                                  na, order, True, False)
@@ -122,56 +147,73 @@ def test_layered_vjp_matches_oracle_jacobian(layers):
 
 def test_vjp_matches_central_differences_of_the_gpu_forward():
     """Central differences at h = 1e-6 relative of sum(bar . outputs) through the GPU
-    forward, for every parameter of a lognormal sphere ensemble and a 3-layer gamma one."""
+    forward, for every parameter of a lognormal sphere ensemble, a 3-layer gamma one, a modified
+    gamma law of spheres, a power law (its fixed limits get 0) and monodisperse coated spheres."""
     from scatrs_amd.ensemble import ensemble_fixed_device
 
     rng = np.random.default_rng(21)
     nr, na, order = 48, 64, 24
-    cases = [
-        (0, 0.25, 1.7, 0.6, np.array([1.45 + 0.003j]), None),
-        (1, 0.35, 0.2, 0.8, np.array([1.7 + 0.2j, 1.5 + 0.01j, 1.35 + 1e-5j]),
-         np.array([0.3, 0.65])),
+    cases = [  # kind, (p0, p1, p2), wavelength, m, fractions, free distribution parameters
+        (0, (0.25, 1.7, 0.0), 0.6, np.array([1.45 + 0.003j]), None, 2),
+        (1, (0.35, 0.2, 0.0), 0.8, np.array([1.7 + 0.2j, 1.5 + 0.01j, 1.35 + 1e-5j]),
+         np.array([0.3, 0.65]), 2),
+        (2, (2.0, 6.0, 1.5), 0.6, np.array([1.5 + 0.01j]), None, 3),
+        (3, (3.0, 0.05, 1.0), 0.7, np.array([1.45 + 1e-3j]), None, 1),
+        (4, (0.7, 0.0, 0.0), 0.55, np.array([1.6 + 0.1j, 1.4 + 1e-3j]), np.array([0.5]), 1),
     ]
-    for kind, q0, q1, wl, m, fr in cases:
+    for kind, q, wl, m, fr, free in cases:
         L = m.size
         bs, bg, bp = _cotangents(rng, 1, order, na)
 
-        def loss(q0, q1, m, fr):
+        def loss(q, m, fr):
             sc, gr, ph, _ = ensemble_fixed_device(
-                _dev(np.array([kind], np.int32)), _dev(np.array([q0])), _dev(np.array([q1])),
+                _dev(np.array([kind], np.int32)), _dev(np.array([q[0]])), _dev(np.array([q[1]])),
                 _dev(np.array([wl])), _dev(m[None] if L > 1 else m),
-                None if fr is None else _dev(fr[None]), n_radius=nr, n_angles=na, order=order)
+                None if fr is None else _dev(fr[None]), p2=_dev(np.array([q[2]])), n_radius=nr,
+                n_angles=na, order=order)
             return float((sc.cpu().numpy() * bs).sum() + (gr.cpu().numpy() * bg).sum() +
                          (ph.cpu().numpy() * bp).sum())
 
-        g, _ = _gpu_vjp(np.array([kind]), np.array([q0]), np.array([q1]), np.array([wl]),
-                        m[None] if L > 1 else m, None if fr is None else fr[None], bs, bg, bp,
-                        n_radius=nr, n_angles=na, order=order)
-        g = g[0]
+        from scatrs_amd.ensemble import ensemble_fixed_vjp_device
+
+        g, _ = ensemble_fixed_vjp_device(
+            _dev(np.array([kind], np.int32)), _dev(np.array([q[0]])), _dev(np.array([q[1]])),
+            _dev(np.array([wl])), _dev(m[None] if L > 1 else m),
+            None if fr is None else _dev(fr[None]), p2=_dev(np.array([q[2]])),
+            bar_scalars=_dev(bs), bar_greek=_dev(bg), bar_phase=_dev(bp), n_radius=nr,
+            n_angles=na, order=order)
+        g = g.cpu().numpy()[0]
+        assert g.size == 3 + 2 * L + L - 1
         fd = []
         h = 1e-6
         for p in range(g.size):
-            args_p = [q0, q1, m.copy(), None if fr is None else fr.copy()]
-            args_m = [q0, q1, m.copy(), None if fr is None else fr.copy()]
-            if p < 2:
-                d = h * (q0, q1)[p]
-                args_p[p] += d
-                args_m[p] -= d
-            elif p < 2 + 2 * L:
-                l, im = (p - 2) // 2, (p - 2) % 2
+            qp, qm = list(q), list(q)
+            args_p = [qp, m.copy(), None if fr is None else fr.copy()]
+            args_m = [qm, m.copy(), None if fr is None else fr.copy()]
+            if p < 3:
+                if p >= free:  # an unused slot or a power law's fixed limit
+                    assert g[p] == 0.0, (kind, p, g)
+                    fd.append(0.0)
+                    continue
+                d = h * abs(q[p])
+                qp[p] += d
+                qm[p] -= d
+            elif p < 3 + 2 * L:
+                l, im = (p - 3) // 2, (p - 3) % 2
                 d = h * abs(m[l])
                 step = 1j * d if im else d
-                args_p[2][l] += step
-                args_m[2][l] -= step
+                args_p[1][l] += step
+                args_m[1][l] -= step
             else:
-                l = p - 2 - 2 * L
+                l = p - 3 - 2 * L
                 d = h * fr[l]
-                args_p[3][l] += d
-                args_m[3][l] -= d
+                args_p[2][l] += d
+                args_m[2][l] -= d
             fd.append((loss(*args_p) - loss(*args_m)) / (2 * d))
         fd = np.array(fd)
         scale = np.abs(fd).max()
         err = np.abs(g - fd) / scale
+        print(f"kind {kind}: max |vjp - fd| / max|fd| {err.max():.2e}")
         assert err.max() <= 2e-6, (kind, err, g, fd)
 
 
@@ -211,7 +253,8 @@ def test_autograd_function_matches_vjp():
                     ws.cpu().numpy(), wg.cpu().numpy(), None, n_radius=48, n_angles=64,
                     order=24)
     got = np.stack([t.grad.cpu().numpy() for t in (p0, p1, mr, mi)], axis=1)
-    assert np.array_equal(got, g)
+    assert not g[:, 2].any()  # a lognormal has no third parameter
+    assert np.array_equal(got, g[:, [0, 1, 3, 4]])
 
 
 def test_adjoint_workload_full_size():
@@ -267,5 +310,6 @@ def test_adjoint_workload_with_fp16_split_forward_tables():
     for e in (0, 1, 40):
         ref, scale = _oracle_vjp(0, p0[e], p1[e], wl[e], m[e], None, bs[e], bg[e], None, nr, na,
                                  order, True, False)
-        print(f"row {e}: max |g - ref| / scale {np.max(np.abs(g[e] - ref) / scale):.2e}")
+        err = np.abs(g[e] - ref) / np.where(scale > 0, scale, 1.0)
+        print(f"row {e}: max |g - ref| / scale {err.max():.2e}")
         _assert_close(g[e], ref, scale, 2e-6, f"row {e}")

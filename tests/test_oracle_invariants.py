@@ -4,6 +4,7 @@ vector exists (scatrs/tests/ensemble.rs:61-202 are the reference's own such chec
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 import oracle
 
@@ -64,3 +65,53 @@ def test_ensemble_derivative_of_the_discrete_path_by_finite_differences():
     d1 = (f(rg * (1 + 1e-6)) - f(rg * (1 - 1e-6))) / (2e-6 * rg)
     d2 = (f(rg * (1 + 1e-5)) - f(rg * (1 - 1e-5))) / (2e-5 * rg)
     assert abs(d1 / d2 - 1) < 1e-6
+
+
+def test_new_distributions_match_closed_forms():
+    """The moments of the modified gamma law truncated to its range, in y = b r^gamma:
+    <r^k> = b^(-k/gamma) Gamma(s_k) [P(s_k, y_hi) - P(s_k, y_lo)] / (Gamma(s_0) [P(s_0, y_hi) -
+    P(s_0, y_lo)]), s_k = (alpha + 1 + k)/gamma (P the regularized incomplete gamma function;
+    untruncated, the closed form of distribution.rs:234-238); the power law's on [r_min, r_max],
+    int r^(k - alpha) / int r^(-alpha) (distribution.rs:239-252). A monodisperse law is the
+    sphere (distribution.rs:49-53)."""
+    from scipy.special import gammainc, gammaln
+
+    for alpha, b, g in ((2.0, 3.0, 1.2), (-0.5, 8.0, 0.7), (6.0, 20.0, 2.0)):
+        r = oracle.ensemble_fixed(oracle.MODIFIED_GAMMA, alpha, b, 0.5, 1.5 + 0j, n_radius=256,
+                                  n_angles=8, order=-1, p2=g)
+        t_lo, t_hi = oracle.radius_range(oracle.MODIFIED_GAMMA, alpha, b, g)
+        y_lo, y_hi = b * np.exp(g * t_lo), b * np.exp(g * t_hi)
+
+        def part(k):
+            sk = (alpha + 1 + k) / g
+            return np.exp(gammaln(sk)) * (gammainc(sk, y_hi) - gammainc(sk, y_lo))
+
+        mom = lambda k: b ** (-k / g) * part(k) / part(0)  # noqa: E731
+        assert abs(r["scalars"][9] / (np.pi * mom(2)) - 1) < 1e-11, (alpha, b, g)
+        assert abs(r["scalars"][10] / (4 * np.pi / 3 * mom(3)) - 1) < 1e-11, (alpha, b, g)
+        # the range holds all but e^-37 of the area-weighted law
+        sa = (alpha + 3) / g
+        assert gammainc(sa, y_lo) < 1e-16 and 1 - gammainc(sa, y_hi) < 1e-15
+    for alpha, lo, hi in ((3.0, 0.05, 2.0), (-1.0, 0.1, 0.3), (2.0, 0.01, 1.0)):
+        r = oracle.ensemble_fixed(oracle.POWER_LAW, alpha, lo, 0.5, 1.5 + 0j, n_radius=64,
+                                  n_angles=8, order=-1, p2=hi)
+
+        def integral(p):
+            return np.log(hi / lo) if abs(p) < 1e-12 else (hi ** p - lo ** p) / p
+
+        mom = lambda k: integral(k + 1 - alpha) / integral(1 - alpha)  # noqa: E731
+        assert abs(r["scalars"][9] / (np.pi * mom(2)) - 1) < 1e-12, (alpha, lo, hi)
+        assert abs(r["scalars"][8] / (mom(3) / mom(2)) - 1) < 1e-12, (alpha, lo, hi)
+    rad, wl, m = 0.4, 0.55, 1.5 + 0.01j
+    r = oracle.ensemble_fixed(oracle.MONODISPERSE, rad, 0.0, wl, m, n_radius=8, n_angles=64,
+                              order=16)
+    x = 2 * np.pi * rad / wl
+    q = oracle.efficiencies(x, oracle.mie_solve(x, m))
+    area = np.pi * rad * rad
+    assert abs(r["scalars"][0] / (area * q[0]) - 1) < 1e-14
+    assert abs(r["scalars"][5] / q[4] - 1) < 1e-14
+    assert abs(r["scalars"][8] / rad - 1) < 1e-15
+    with pytest.raises(ValueError):
+        oracle.ensemble_fixed(oracle.MODIFIED_GAMMA, -1.0, 1.0, 0.5, 1.5 + 0j, p2=1.0)
+    with pytest.raises(ValueError):
+        oracle.ensemble_fixed(oracle.POWER_LAW, 2.0, 0.5, 0.5, 1.5 + 0j, p2=0.5)
