@@ -352,6 +352,75 @@ def bench_layered(args, torch):
             "per_layer_count": per_l}
 
 
+def bench_distributions(args, torch):
+    """The other distribution kinds and mixtures: modified gamma, power-law and monodisperse
+    spheres (1536 x 32 wavelengths; 256 radius nodes, 512 angles, Greek to 128), forward +
+    VJP; and 1024 bimodal lognormals x 32 wavelengths, forward: both modes on the device with
+    normalized outputs off, mixed by number fraction (mix_parts)."""
+    from scatrs_amd import _native, workloads
+    from scatrs_amd.ensemble import ensemble_fixed_device, ensemble_fixed_vjp_device, mix_parts
+
+    nr, na, order = 256, 512, 128
+    ctx = _native.context(0)
+    kind, p0, p1, p2, wl, m = workloads.distributions()
+    t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
+    tp2 = _dev(torch, p2)
+    rng = np.random.default_rng(5)
+    bs = rng.normal(size=(kind.size, 11))
+    bs[:, 7] = 0.0
+    bs, bg = _dev(torch, bs), _dev(torch, rng.normal(size=(kind.size, 6, order + 1)))
+
+    def step():
+        _, _, _, terms = ensemble_fixed_device(*t, p2=tp2, n_radius=nr, n_angles=na,
+                                               order=order, want_phase=False)
+        ensemble_fixed_vjp_device(*t, p2=tp2, bar_scalars=bs, bar_greek=bg, n_radius=nr,
+                                  n_angles=na, order=order)
+        return terms
+
+    secs = {}
+    for kernel in (2, 0):
+        ctx.set_angular_kernel(kernel)
+        try:
+            step()
+            secs[kernel], _, terms = _timed(torch, ctx, step, 1)
+        finally:
+            ctx.set_angular_kernel(0)
+    log(f"distributions: {kind.size} ensembles, {secs[2]:.3f} s forward + VJP "
+        f"({secs[0]:.3f} s in FP64 throughout)")
+    (kk, q0, q1, w2, mm), frac = workloads.bimodal()
+    tb = [_dev(torch, v) for v in (kk, q0, q1, w2, mm)]
+    f = _dev(torch, frac)
+    cnt = kk.size // 2
+
+    def mstep():
+        sc, gr, ph, _ = ensemble_fixed_device(*tb, n_radius=nr, n_angles=na, order=order,
+                                              normalized=False)
+        return mix_parts(sc.reshape(2, cnt, 11), gr.reshape(2, cnt, 6, order + 1),
+                         ph.reshape(2, cnt, 6, na), f, True)
+
+    ctx.set_angular_kernel(2)
+    try:
+        mstep()
+        msec, _, _ = _timed(torch, ctx, mstep, 1)
+    finally:
+        ctx.set_angular_kernel(0)
+    log(f"bimodal mixtures: {cnt} mixtures ({kk.size} parts), {msec:.3f} s forward")
+    return {"workload": "modified gamma (alpha U[1, 8], gamma U[0.5, 1.5], mode radius "
+                        "log-uniform [0.05, 2] um), power law (exponent U[2, 4.5], r_min "
+                        "log-uniform [0.01, 0.1] um, r_max [1, 5] um) and monodisperse (radius "
+                        "log-uniform [0.05, 5] um) spheres, a third each, x 32 wavelengths; 256 "
+                        "radius nodes, 512 angles, Greek to 128; step = forward + VJP "
+                        "(fp16-split forward tables)",
+            "ensembles": int(kind.size), "terms": int(terms), "s_per_step": secs[2],
+            "ensembles_per_s": kind.size / secs[2],
+            "fp64_matrix_cores": {"s_per_step": secs[0], "ensembles_per_s": kind.size / secs[0]},
+            "mixtures": {"workload": "1024 bimodal lognormals (fine r_g log-uniform [0.02, 0.2] "
+                                     "um, coarse [0.5, 3] um, coarse number fraction "
+                                     "log-uniform [1e-4, 0.1]) x 32 wavelengths, forward, "
+                                     "fp16-split tables",
+                         "mixtures": cnt, "s_per_step": msec, "mixtures_per_s": cnt / msec}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +465,7 @@ def main() -> int:
         res["ensembles"] = bench_ensembles(args, torch)
         res["adjoint"] = bench_adjoint(args, torch)
         res["layered"] = bench_layered(args, torch)
+        res["distributions"] = bench_distributions(args, torch)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -199,7 +199,8 @@ def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
     outputs on the fixed path computed with ``normalized=False`` (scalars (parts, ..., 11),
     greek (parts, ..., 6, L+1) or None, phase (parts, ..., 6, n) or None), weighted by its number
     fraction (``fractions`` (parts,): the component's weight over the total times the mode's
-    weight over its component's total), and the outputs assembled from the mixed averages as
+    weight over its component's total; or (parts, ...) with the batch axes of the outputs,
+    for fractions that differ between rows), and the outputs assembled from the mixed averages as
     for one part (engine.rs:2396-2497): <C_ext>, <C_sca>, <C_abs>, <C_back>, <r^2>, <r^3>
     averaged, ssa, g, the lidar ratio and the effective radius as their ratios, the phase
     matrix divided by <C_sca> and the Greek coefficients by their alpha1_0 when normalized.
@@ -207,8 +208,8 @@ def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
     ``ensemble_fixed(..., normalized=False)`` for the parts, a mixture's gradient)."""
     import torch
 
-    nb = scalars.dim() - 2  # batch axes between the parts and the values
-    f = fractions.to(scalars.dtype).reshape((-1,) + (1,) * nb)
+    f = fractions.to(scalars.dtype)
+    f = f.reshape(f.shape + (1,) * (scalars.dim() - 1 - f.dim()))  # (parts, batch axes)
     sc = scalars
     avg = torch.stack([sc[..., 0], sc[..., 1], sc[..., 2], sc[..., 3], sc[..., 5] * sc[..., 1],
                        sc[..., 9] / math.pi, sc[..., 10] * (3.0 / (4.0 * math.pi))], dim=-1)

@@ -16,7 +16,9 @@ Workloads:
     sum_{l<64} (alpha1_l - t_l)^2] for the reverse-mode path;
   * ``layered``: gamma distributions (r_eff log-uniform on [0.05, 3] um, v_eff uniform on
     [0.05, 0.4]) of a core and 1-4 shells (fractions from Dirichlet(2) cumulative sums,
-    Re m uniform on [1.3, 2.0], Im m log-uniform on [1e-6, 1]) at 32 wavelengths.
+    Re m uniform on [1.3, 2.0], Im m log-uniform on [1e-6, 1]) at 32 wavelengths;
+  * ``distributions``: modified gamma, power-law and monodisperse spheres at 32 wavelengths;
+  * ``bimodal``: bimodal lognormals of spheres (a fine and a coarse mode) at 32 wavelengths.
 """
 
 from __future__ import annotations
@@ -113,6 +115,59 @@ def layered(n_dist: int = 2048, n_wl: int = 32, layers: int | None = None, seed:
                   np.repeat(veff[idx], n_wl), np.tile(wl, idx.size),
                   np.repeat(fr, n_wl, axis=0), np.repeat(mre + 1j * mim, n_wl, axis=0))
     return out
+
+
+def distributions(n_dist: int = 1536, n_wl: int = 32, seed: int = SEED):
+    """Spheres over the other distribution kinds, a third each: modified gamma (alpha uniform
+    on [1, 8], gamma on [0.5, 1.5], b set by a mode radius log-uniform on [0.05, 2] um), power
+    law (exponent uniform on [2, 4.5], r_min log-uniform on [0.01, 0.1] um, r_max on
+    [1, 5] um), monodisperse (radius log-uniform on [0.05, 5] um); m as for ``spheres``; 32
+    wavelengths as for ``layered``. (kind, p0, p1, p2, wavelength, m) arrays of
+    n_dist * n_wl."""
+    rng = np.random.default_rng(seed + 3)
+    kind = np.repeat(np.array([2, 3, 4], np.int32), n_dist // 3)
+    n = kind.size
+    p0, p1, p2 = np.zeros(n), np.zeros(n), np.zeros(n)
+    mg, pl, mono = kind == 2, kind == 3, kind == 4
+    alpha = rng.uniform(1.0, 8.0, n)
+    gam = rng.uniform(0.5, 1.5, n)
+    rmode = np.exp(rng.uniform(np.log(0.05), np.log(2.0), n))
+    p0[mg], p2[mg] = alpha[mg], gam[mg]
+    p1[mg] = alpha[mg] / (gam[mg] * rmode[mg] ** gam[mg])  # the mode (alpha / (b gamma))^(1/gamma)
+    p0[pl] = rng.uniform(2.0, 4.5, pl.sum())
+    p1[pl] = np.exp(rng.uniform(np.log(0.01), np.log(0.1), pl.sum()))
+    p2[pl] = np.exp(rng.uniform(np.log(1.0), np.log(5.0), pl.sum()))
+    p0[mono] = np.exp(rng.uniform(np.log(0.05), np.log(5.0), mono.sum()))
+    m = rng.uniform(1.30, 1.70, n) + 1j * np.exp(rng.uniform(np.log(1e-8), np.log(1e-1), n))
+    wl = np.exp(np.linspace(np.log(0.3), np.log(2.5), n_wl))
+    r = lambda a: np.repeat(a, n_wl)  # noqa: E731
+    return r(kind), r(p0), r(p1), r(p2), np.tile(wl, n), r(m)
+
+
+def bimodal(n_dist: int = 1024, n_wl: int = 32, seed: int = SEED):
+    """Bimodal lognormals of spheres: a fine mode (r_g log-uniform on [0.02, 0.2] um,
+    sigma_g uniform on [1.3, 1.8]) and a coarse one (r_g on [0.5, 3] um, sigma_g on
+    [1.5, 2.2]) with the coarse number fraction log-uniform on [1e-4, 1e-1]; m as for
+    ``spheres``; 32 wavelengths. Returns the parts (kind, p0, p1, wavelength, m) of
+    2 * n_dist * n_wl rows, parts first, and the number fractions (2, n_dist * n_wl)."""
+    rng = np.random.default_rng(seed + 4)
+    rf = np.exp(rng.uniform(np.log(0.02), np.log(0.2), n_dist))
+    sf = rng.uniform(1.3, 1.8, n_dist)
+    rc = np.exp(rng.uniform(np.log(0.5), np.log(3.0), n_dist))
+    sc = rng.uniform(1.5, 2.2, n_dist)
+    fc = np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), n_dist))
+    m = rng.uniform(1.30, 1.70, n_dist) + 1j * np.exp(rng.uniform(np.log(1e-8), np.log(1e-1),
+                                                                   n_dist))
+    wl = np.exp(np.linspace(np.log(0.3), np.log(2.5), n_wl))
+    r = lambda a: np.repeat(a, n_wl)  # noqa: E731
+    cnt = n_dist * n_wl
+    kind = np.zeros(2 * cnt, np.int32)
+    p0 = np.concatenate([r(rf), r(rc)])
+    p1 = np.concatenate([r(sf), r(sc)])
+    wls = np.tile(np.tile(wl, n_dist), 2)
+    ms = np.tile(r(m), 2)
+    frac = np.stack([1.0 - r(fc), r(fc)])
+    return (kind, p0, p1, wls, ms), frac
 
 
 # ---- algorithmic cost of the Mie recurrences (FP64 operations; a division counts 1) ----

@@ -103,6 +103,10 @@ def test_library_mixing_matches_the_oracle(normalized):
     t = lambda key: torch.tensor(np.array([[r[key] for r in row] for row in raw]))  # noqa: E731
     f = torch.tensor([x for x, _ in fp], dtype=torch.float64)
     sc, gr, ph = mix_parts(t("scalars"), t("greek"), t("phase"), f, normalized)
+    # fractions given per row (parts, wavelengths) give the same
+    sc2, gr2, ph2 = mix_parts(t("scalars"), t("greek"), t("phase"),
+                              f[:, None].expand(-1, len(wls)), normalized)
+    assert torch.equal(sc, sc2) and torch.equal(gr, gr2) and torch.equal(ph, ph2)
     for j, w in enumerate(wls):
         ref = oracle.ensemble_mixture(_parts(*fp), w, normalized=normalized, **KW)
         _close(sc[j].numpy(), ref["scalars"], 1e-14)
