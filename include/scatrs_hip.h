@@ -66,6 +66,15 @@ int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap);
  * No reference counterpart. */
 int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
 
+/* The cosines at which scatrs_ensemble_fixed writes the phase matrix: mu (host, n_mu values in
+ * [-1, 1], at most 65536; n_mu = 0 restores the default, the n_angles Gauss-Legendre cosines);
+ * d_phase is then (count, 6, n_mu). The angular products run once more at these cosines (with
+ * the selected kernel); the Greek coefficients still come from the n_angles-point rule. The
+ * reference's mu argument (python/scatrs/ensemble.py:550-562, scatrs-py/src/ensemble.rs:
+ * 295-355). scatrs_ensemble_fixed_vjp keeps phase cotangents at the Gauss-Legendre cosines and
+ * rejects d_bar_phase while cosines are set. Stays in effect until changed. */
+int scatrs_set_phase_cosines(scatrs_ctx *ctx, const double *mu, int64_t n_mu);
+
 /* The default truncation N = ceil(x + 6 x^(1/3) + 3) (host; scatrs/src/mie/mod.rs:122-124). */
 int64_t scatrs_default_n_max(double x);
 
@@ -108,7 +117,8 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
  * Outputs (normalized as the reference's Normalization::Normalized when normalized != 0):
  *   d_scalars (count, 11) in the reference's SCALARS order (python/scatrs/ensemble.py:51-63);
  *   d_greek (count, 6, order+1) [alpha1..alpha4, beta1, beta2] or NULL;
- *   d_phase (count, 6, n_angles) [F11, F12, F22, F33, F34, F44] or NULL.
+ *   d_phase (count, 6, n_angles) [F11, F12, F22, F33, F34, F44] or NULL ((count, 6, n_mu) at
+ *   the cosines of scatrs_set_phase_cosines when set).
  * Replaces ensemble::compute_with_particle / compute_batch_with_particle
  * (scatrs/src/ensemble/mod.rs:589-686) and _scatrs.ensemble_compute{,_batch}
  * (scatrs-py/src/ensemble.rs:295-355,456-500) on this fixed discrete path (the reference's

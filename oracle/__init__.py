@@ -62,6 +62,10 @@ def lib() -> C.CDLL:
         L.orc_ensemble_fixed_ex.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p, _cp,
                                             C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_void_p,
                                             C.c_void_p, _dp, C.c_void_p, C.c_void_p]
+        L.orc_ensemble_fixed_mu.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p, _cp,
+                                            C.c_int64, C.c_int64, C.c_int64, C.c_int, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int64, _dp, C.c_void_p,
+                                            C.c_void_p]
         L.orc_ensemble_fixed_jacobian.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_void_p,
                                                   _cp, C.c_int64, C.c_int64, C.c_int64, C.c_int,
                                                   C.c_int, _dp, C.c_void_p, C.c_void_p, _dp,
@@ -202,28 +206,31 @@ LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE = 0, 1, 2, 3, 4
 
 def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fractions=None,
                    n_radius: int = 128, n_angles: int = 128, order: int = 32,
-                   normalized: bool = True, t_range=None, n_nodes=None, p2: float = 0.0) -> dict:
-    """One ensemble on the fixed discrete path: scalars (11,), greek (6, L+1), phase (6, n_q).
-    p2 is the third parameter of a modified gamma or power law. t_range = (t_lo, t_hi) replaces
-    the distribution's ln r range and n_nodes the per-node N (for finite differences at a fixed
-    range or fixed truncations)."""
+                   normalized: bool = True, t_range=None, n_nodes=None, p2: float = 0.0,
+                   mu=None) -> dict:
+    """One ensemble on the fixed discrete path: scalars (11,), greek (6, L+1), phase (6, n_q),
+    or (6, len(mu)) at the cosines mu when given. p2 is the third parameter of a modified gamma
+    or power law. t_range = (t_lo, t_hi) replaces the distribution's ln r range and n_nodes the
+    per-node N (for finite differences at a fixed range or fixed truncations)."""
     m = np.atleast_1d(np.asarray(m, np.complex128)).copy()
     layers = m.size
     fr = None if fractions is None else np.ascontiguousarray(fractions, np.float64)
     scalars = np.zeros(11)
     greek = np.zeros(6 * (order + 1)) if order >= 0 else None
-    phase = np.zeros(6 * n_angles)
+    mu = None if mu is None else np.ascontiguousarray(mu, np.float64).reshape(-1)
+    nq = n_angles if mu is None else mu.size
+    phase = np.zeros(6 * nq)
     rng = None if t_range is None else np.ascontiguousarray(t_range, np.float64)
     nn = None if n_nodes is None else np.ascontiguousarray(n_nodes, np.int64)
-    rc = lib().orc_ensemble_fixed_ex(kind, _params(p0, p1, p2), wavelength, layers, _ptr(fr), m,
-                                     n_radius,
-                                     n_angles, order, int(normalized), _ptr(rng), _ptr(nn),
-                                     scalars, _ptr(greek), _ptr(phase))
+    rc = lib().orc_ensemble_fixed_mu(kind, _params(p0, p1, p2), wavelength, layers, _ptr(fr), m,
+                                     n_radius, n_angles, order, int(normalized), _ptr(rng),
+                                     _ptr(nn), _ptr(mu), 0 if mu is None else mu.size, scalars,
+                                     _ptr(greek), _ptr(phase))
     if rc != 0:
         raise ValueError(f"invalid ensemble input ({rc})")
     return {"scalars": scalars,
             "greek": None if greek is None else greek.reshape(6, order + 1),
-            "phase": phase.reshape(6, n_angles)}
+            "phase": phase.reshape(6, nq)}
 
 
 def ensemble_fixed_jacobian(kind: int, p0: float, p1: float, wavelength: float, m,

@@ -1259,6 +1259,15 @@ int orc_ensemble_fixed_ex(int kind, const double *p, double lambda, int layers,
                           int64_t n_angles, int64_t order, int normalized, const double *range,
                           const int64_t *n_nodes, double *scalars, double *greek,
                           double *phase) {
+    return orc_ensemble_fixed_mu(kind, p, lambda, layers, fractions, m, n_radius, n_angles, order,
+                                 normalized, range, n_nodes, NULL, 0, scalars, greek, phase);
+}
+
+int orc_ensemble_fixed_mu(int kind, const double *p, double lambda, int layers,
+                          const double *fractions, const orc_cx *m, int64_t n_radius,
+                          int64_t n_angles, int64_t order, int normalized, const double *range,
+                          const int64_t *n_nodes, const double *mu_user, int64_t n_mu,
+                          double *scalars, double *greek, double *phase) {
     double t_lo, t_hi;
     if (orc_radius_range(kind, p, &t_lo, &t_hi) != 0) return -1;
     if (range) {
@@ -1278,6 +1287,11 @@ int orc_ensemble_fixed_ex(int kind, const double *p, double lambda, int layers,
     double *xl = (double *)malloc(sizeof(double) * (size_t)layers);
     cx *s1 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
     cx *s2 = (cx *)malloc(sizeof(cx) * (size_t)n_angles);
+    /* the phase tables at the caller's cosines, when given */
+    const int64_t nu = mu_user ? n_mu : 0;
+    double *Tu = (double *)calloc(4 * (size_t)(nu > 0 ? nu : 1), sizeof(double));
+    cx *u1 = (cx *)malloc(sizeof(cx) * (size_t)(nu > 0 ? nu : 1));
+    cx *u2 = (cx *)malloc(sizeof(cx) * (size_t)(nu > 0 ? nu : 1));
     for (int64_t i = 0; i < n_radius; i++) {
         double t = mid + half * rn[i];
         double r = exp(t);
@@ -1311,10 +1325,34 @@ int orc_ensemble_fixed_ex(int kind, const double *p, double lambda, int layers,
             T[2 * n_angles + j] += W * (c * pp.re);
             T[3 * n_angles + j] += W * (c * pp.im);
         }
+        if (nu > 0) {
+            orc_amplitudes(n, ab, ab + n, mu_user, nu, u1, u2);
+            for (int64_t j = 0; j < nu; j++) {
+                double n1 = norm2(u1[j]), n2 = norm2(u2[j]);
+                cx pp = mul(u2[j], C(u1[j].re, -u1[j].im));
+                Tu[j] += W * (c * (0.5 * (n1 + n2)));
+                Tu[nu + j] += W * (c * (0.5 * (n2 - n1)));
+                Tu[2 * nu + j] += W * (c * pp.re);
+                Tu[3 * nu + j] += W * (c * pp.im);
+            }
+        }
         free(ab);
     }
-    ensemble_outputs(S, T, n_angles, mu, mw, order, normalized, scalars, greek, phase);
+    ensemble_outputs(S, T, n_angles, mu, mw, order, normalized, scalars, greek,
+                     nu > 0 ? NULL : phase);
+    if (nu > 0 && phase) {
+        /* [F11, F12, F22 = F11, F33, F34, F44 = F33] at the caller's cosines, divided by Z and,
+         * normalized, by <C_sca> (engine.rs:2459-2490) */
+        double z = S[0], sca = S[2] / z;
+        int src[6] = {0, 1, 0, 2, 3, 2};
+        for (int e = 0; e < 6; e++)
+            for (int64_t j = 0; j < nu; j++) {
+                double v = Tu[src[e] * nu + j] / z;
+                phase[e * nu + j] = normalized ? v / sca : v;
+            }
+    }
     free(rn); free(rw); free(mu); free(mw); free(T); free(xl); free(s1); free(s2);
+    free(Tu); free(u1); free(u2);
     return 0;
 }
 

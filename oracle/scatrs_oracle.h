@@ -128,6 +128,15 @@ int orc_ensemble_fixed_ex(int kind, const double *p, double lambda, int layers,
                           const int64_t *n_nodes, double *scalars, double *greek,
                           double *phase);
 
+/* orc_ensemble_fixed_ex with the phase matrix at the caller's cosines mu_user[n_mu] (each in
+ * [-1, 1]; phase is then 6 x n_mu), the Greek coefficients still from the n_angles-point rule
+ * (the reference's mu argument, python/scatrs/ensemble.py:550-562). mu_user NULL: as _ex. */
+int orc_ensemble_fixed_mu(int kind, const double *p, double lambda, int layers,
+                          const double *fractions, const orc_cx *m, int64_t n_radius,
+                          int64_t n_angles, int64_t order, int normalized, const double *range,
+                          const int64_t *n_nodes, const double *mu_user, int64_t n_mu,
+                          double *scalars, double *greek, double *phase);
+
 /* The outputs of orc_ensemble_fixed and their forward-mode derivatives along the P = 3 + 2L +
  * (L - 1) directions [p0, p1, p2, Re m_0, Im m_0, ..., Re m_{L-1}, Im m_{L-1}, fraction_0 ..
  * fraction_{L-2}]: d_scalars (P, 11), d_greek (P, 6, order+1) or NULL, d_phase (P, 6, n_angles)

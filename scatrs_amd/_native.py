@@ -34,6 +34,7 @@ SYMBOLS = (
     "scatrs_set_profiling",
     "scatrs_kernel_times",
     "scatrs_set_angular_kernel",
+    "scatrs_set_phase_cosines",
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
     "scatrs_ensemble_fixed",
@@ -67,6 +68,7 @@ def load() -> C.CDLL:
         L.scatrs_set_profiling.argtypes = [vp, i32]
         L.scatrs_kernel_times.argtypes = [vp, C.POINTER(dbl), i32]
         L.scatrs_set_angular_kernel.argtypes = [vp, i32]
+        L.scatrs_set_phase_cosines.argtypes = [vp, vp, i64]
         L.scatrs_default_n_max.argtypes = [dbl]
         L.scatrs_default_n_max.restype = i64
         L.scatrs_mie_efficiencies.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
@@ -105,6 +107,17 @@ class Context:
         buf = (C.c_double * 1024)()
         n = self.lib.scatrs_kernel_times(self.handle, buf, 1024)
         return [buf[i] for i in range(n)]
+
+    def set_phase_cosines(self, mu) -> None:
+        """The cosines of the ensemble phase matrix (None: the Gauss-Legendre rule's)."""
+        import numpy as np
+
+        if mu is None:
+            self.check(self.lib.scatrs_set_phase_cosines(self.handle, None, 0))
+            return
+        a = np.ascontiguousarray(mu, np.float64).reshape(-1)
+        self.check(self.lib.scatrs_set_phase_cosines(
+            self.handle, a.ctypes.data_as(C.c_void_p), a.size))
 
     def set_angular_kernel(self, kind: int) -> None:
         """0: FP64 matrix cores (default); 1: the plain FP64 vector kernel; 2: fp16 matrix

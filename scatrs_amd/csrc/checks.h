@@ -15,6 +15,7 @@ namespace scatrs {
 constexpr double kMinSize = 1e-30, kMaxSize = 1e6;
 constexpr double kMinIndex = 1e-6, kMaxIndex = 1e3;
 constexpr long long kMaxOrder = 1ll << 20;
+constexpr int64_t kMaxPhaseCosines = 1 << 16;  // scatrs_set_phase_cosines
 constexpr double kDomXMin = 1e-6, kDomXMax = 2e4, kDomMReMax = 10.0, kDomMImMax = 10.0,
                  kDomAbsMMin = 0.1;
 

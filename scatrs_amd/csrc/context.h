@@ -35,6 +35,9 @@ struct scatrs_ctx {
         kTables16,    // ensembles: the w pi_n, w tau_n table split in fp16 slices
         kGreekFac,    // ensembles: per-ensemble factor of the Greek coefficients
         kStaging32,   // reverse mode: the fp32 form of the staged coefficients (kernel 2)
+        kPhaseMu,     // phase at the caller's cosines: |mu| ascending, then each cosine's index
+        kTablesMu,    // ... the w pi_n, w tau_n table at |mu| (and its fp16 slices)
+        kAccumMu,     // ... the summed tables of a chunk, and each ensemble's phase factor
         kNumBufs
     };
     void *buf[kNumBufs] = {};
@@ -46,6 +49,9 @@ struct scatrs_ctx {
     std::vector<double> rr_nodes, rr_weights, ar_nodes, ar_weights;
     bool profiling = false;
     int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2: fp16-split MFMA
+    std::vector<double> phase_mu;  // the caller's phase cosines (empty: the angle rule's)
+    std::vector<double> phase_amu;  // their distinct |mu|, ascending
+    std::vector<int> phase_idx;     // each cosine's place among -|mu|, +|mu| (2 h of them)
     static constexpr int kMaxEvents = 1024;
     hipEvent_t ev_begin[kMaxEvents] = {}, ev_end[kMaxEvents] = {};
     int ev_count = 0;

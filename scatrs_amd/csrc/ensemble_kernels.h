@@ -932,7 +932,7 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
                                                   const double *tab, const double *dtab,
                                                   int64_t order, int normalized,
                                                   double *scalars, double *gfac,
-                                                  double *phase) {
+                                                  double *phase, double *phase_fac) {
     __shared__ double part[kNodeVals][33];  // per value: 32 partial sums (padded row)
     __shared__ double S[kNodeVals];
     const int64_t e = e0 + blockIdx.x;
@@ -971,6 +971,9 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
         o[9] = kPi * r2;
         o[10] = 4.0 * kPi / 3.0 * r3;
     }
+    // The phase matrix's factor: 1/Z, divided by <C_sca> when normalized; to phase_fac for the
+    // phase at the caller's cosines (ens_phase_mu).
+    if (phase_fac && t == 0) phase_fac[blockIdx.x] = normalized ? iz / sca : iz;
     if (phase) {
         // [F11, F12, F22 = F11, F33, F34, F44 = F33] / Z, divided by <C_sca> when normalized
         // (ensemble/engine.rs:2459-2490).
@@ -993,6 +996,22 @@ static __global__ __launch_bounds__(256) void ens_finish(int64_t e0, int64_t n_r
         __syncthreads();
     }
     if (t == 0) gfac[blockIdx.x] = normalized ? 1.0 / (0.5 * red[0]) : iz;
+}
+
+// The phase matrix at the caller's cosines (the reference's mu argument,
+// python/scatrs/ensemble.py:550-562): tab holds each ensemble's summed tables at the
+// symmetric set of n_out = 2 h cosines -|mu| and +|mu| (the distinct |mu| ascending), idx[j] the
+// position of cosine j in it; phase[e][k][j] = tab[src_k][idx[j]] times the ensemble's factor
+// from ens_finish, in [F11, F12, F22 = F11, F33, F34, F44 = F33] order.
+static __global__ void ens_phase_mu(int64_t e0, int64_t ne, int n_out, const int *idx, int n_mu,
+                                    const double *tab, const double *fac, double *phase) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= ne * 6 * (int64_t)n_mu) return;
+    const int64_t eb = g / (6 * (int64_t)n_mu);
+    const int k = (int)((g / n_mu) % 6), j = (int)(g % n_mu);
+    const int src = k == 0 || k == 2 ? 0 : k == 1 ? 1 : k == 4 ? 3 : 2;
+    phase[(e0 + eb) * 6 * (int64_t)n_mu + (int64_t)k * n_mu + j] =
+        tab[eb * 4 * (int64_t)n_out + (int64_t)src * n_out + idx[j]] * fac[eb];
 }
 
 // The Greek coefficients of a chunk's ensembles on the FP64 matrix cores (greek.rs:499-560;
@@ -1538,29 +1557,37 @@ static __global__ void stage_fp32(const cx *staging, int64_t terms, float4 *out)
 // The angular products of the chunk [e0, e0 + ne) on stream s from the buffers forward_nodes
 // filled (B.staging32 with kernel 2): afterwards B.tab_out holds the ensembles' summed tables [F11, F12, F33, F34] x
 // n_angles.
-static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
-                           int64_t n_radius, int64_t n_angles, ChunkBufs &B, hipStream_t s) {
-    const int h = S.h;
+// The angular products with the call's kernel at h non-negative cosines (table, and its fp16
+// slices for kernel 2), into tab_out laid out for n_out cosines (n_out - h + jj the cosine jj,
+// n_out - 1 - that its mirror).
+static int angular_products(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
+                            int64_t n_radius, int h, int n_out, const double2 *table,
+                            const _Float16 *tab16, ChunkBufs &B, double *tab_out, hipStream_t s) {
     mark_begin(ctx, s);
     if (S.kernel == 1) {
         const int bt = (int)std::min<int64_t>(512, ((h + 63) / 64) * 64);
-        ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, (int)n_angles, h, S.node_n,
-                                                S.node_off, S.ens_base, B.staging, B.tabc,
-                                                S.table, B.tab_out);
+        ens_angular<<<(unsigned)ne, bt, 0, s>>>(e0, n_radius, n_out, h, S.node_n, S.node_off,
+                                                S.ens_base, B.staging, B.tabc, table, tab_out);
     } else if (S.kernel == 2) {
         constexpr int W = kF16Waves;
         const int slices = (h + 16 * W - 1) / (16 * W);
         ens_angular_f16<W><<<(unsigned)(ne * slices), 64 * W, f16_lds_bytes(n_radius), s>>>(
-            e0, ne, n_radius, (int)n_angles, h, slices, S.npad, S.texp, S.node_n, S.node_off,
-            S.ens_base, B.staging32, B.tabc, B.xexp, S.tab16, B.tab_out);
+            e0, ne, n_radius, n_out, h, slices, S.npad, S.texp, S.node_n, S.node_off, S.ens_base,
+            B.staging32, B.tabc, B.xexp, tab16, tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
-            e0, n_radius, (int)n_angles, h, slices, S.node_n, S.node_off, S.ens_base, B.staging,
-            B.tabc, S.table, B.tab_out);
+            e0, n_radius, n_out, h, slices, S.node_n, S.node_off, S.ens_base, B.staging, B.tabc,
+            table, tab_out);
     }
     mark_end(ctx, s);
     return check_hip(ctx, hipGetLastError(), "ensemble launch");
+}
+
+static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
+                           int64_t n_radius, int64_t n_angles, ChunkBufs &B, hipStream_t s) {
+    return angular_products(ctx, S, e0, ne, n_radius, S.h, (int)n_angles, S.table, S.tab16, B,
+                            B.tab_out, s);
 }
 
 // The node solves and the angular products of a chunk on the context's stream.

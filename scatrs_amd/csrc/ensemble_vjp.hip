@@ -662,6 +662,10 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     if (!d_kind || !d_p0 || !d_p1 || !d_wavelength || !d_m || !d_grad ||
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    if (d_bar_phase && !ctx->phase_mu.empty())
+        return fail(ctx, SCATRS_INVALID_INPUT,
+                    "phase cotangents are taken at the Gauss-Legendre cosines: clear the phase "
+                    "cosines (scatrs_set_phase_cosines with n_mu = 0) for the reverse pass");
     EnsSetup S;
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_bar_greek != nullptr, order, allow_unvalidated,

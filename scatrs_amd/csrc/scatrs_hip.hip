@@ -210,6 +210,34 @@ int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind) {
     return SCATRS_OK;
 }
 
+int scatrs_set_phase_cosines(scatrs_ctx *ctx, const double *mu, int64_t n_mu) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->error.clear();
+    if (n_mu < 0 || n_mu > kMaxPhaseCosines || (n_mu > 0 && !mu))
+        return fail(ctx, SCATRS_INVALID_INPUT,
+                    "the phase cosines need 0 to " + std::to_string(kMaxPhaseCosines) +
+                        " values, got " + std::to_string(n_mu));
+    for (int64_t j = 0; j < n_mu; ++j)
+        if (!(mu[j] >= -1.0 && mu[j] <= 1.0))
+            return fail(ctx, SCATRS_INVALID_INPUT,
+                        "mu must be in [-1, 1], got " + fmt_e(mu[j]) + " at " + std::to_string(j));
+    ctx->phase_mu.assign(mu, mu + n_mu);
+    // The distinct |mu|, ascending, and each cosine's place among the 2 h cosines -|mu| (at
+    // h - 1 - i for the i-th |mu|) and +|mu| (at h + i).
+    std::vector<double> a(n_mu);
+    for (int64_t j = 0; j < n_mu; ++j) a[j] = std::fabs(mu[j]);
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+    const int h = (int)a.size();
+    ctx->phase_amu = a;
+    ctx->phase_idx.resize(n_mu);
+    for (int64_t j = 0; j < n_mu; ++j) {
+        const int i = (int)(std::lower_bound(a.begin(), a.end(), std::fabs(mu[j])) - a.begin());
+        ctx->phase_idx[j] = mu[j] >= 0.0 ? h + i : h - 1 - i;
+    }
+    return SCATRS_OK;
+}
+
 int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap) {
     if (!ctx) return 0;
     int n = 0;

@@ -33,13 +33,14 @@ SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depo
 def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, p2=None,
                           n_radius: int = 128, n_angles: int = 128, order: int | None = 32,
                           normalized: bool = True, allow_unvalidated: bool = False,
-                          want_phase: bool = True):
+                          want_phase: bool = True, mu=None):
     """Device-resident batch: every argument a CUDA tensor with one row per ensemble
     (kind int32; p0, p1, wavelength float64; p2 float64 or None, the third parameter of a
     modified gamma or power law; m complex128 (count,) for spheres or (count, L) for layered
     spheres; fractions float64 (count, L-1) for layered spheres). Returns (scalars (count, 11),
     greek (count, 6, order+1) or None, phase (count, 6, n_angles) or None, total Mie
-    terms)."""
+    terms); with ``mu`` (host cosines in [-1, 1]) the phase matrix is (count, 6, len(mu)) at
+    those cosines (the reference's mu argument)."""
     import torch
 
     dev = p0.device
@@ -54,16 +55,23 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, p2=Non
     scalars = torch.empty((count, 11), dtype=torch.float64, device=dev)
     greek = None if order is None else torch.empty((count, 6, order + 1), dtype=torch.float64,
                                                    device=dev)
-    phase = torch.empty((count, 6, n_angles), dtype=torch.float64, device=dev) if want_phase \
+    n_phase = n_angles if mu is None else int(np.size(mu))
+    phase = torch.empty((count, 6, n_phase), dtype=torch.float64, device=dev) if want_phase \
         else None
     ctx = _native.context(dev.index)
     terms = C.c_int64(0)
     p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
-    rc = ctx.lib.scatrs_ensemble_fixed(
-        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(q2), p(args[3]), layers, p(fr),
-        p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
-        int(bool(normalized)), int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase),
-        C.byref(terms))
+    if mu is not None:
+        ctx.set_phase_cosines(mu)
+    try:
+        rc = ctx.lib.scatrs_ensemble_fixed(
+            ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(q2), p(args[3]), layers,
+            p(fr), p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
+            int(bool(normalized)), int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase),
+            C.byref(terms))
+    finally:
+        if mu is not None:
+            ctx.set_phase_cosines(None)
     ctx.check(rc)
     return scalars, greek, phase, terms.value
 
@@ -231,7 +239,7 @@ def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
 
 
 def _parts_device(parts, wl: np.ndarray, *, order, n_radius, n_angles, allow_unvalidated,
-                  want_phase):
+                  want_phase, mu=None):
     """Each part (kind, p0, p1, p2, particle) at every wavelength on the fixed path with
     normalized=False, as tensors (parts, wavelengths, ...); one library call per layer count."""
     import torch
@@ -242,7 +250,8 @@ def _parts_device(parts, wl: np.ndarray, *, order, n_radius, n_angles, allow_unv
     sc = torch.empty((P, nw, 11), dtype=torch.float64, device=dev)
     gr = None if order is None else torch.empty((P, nw, 6, order + 1), dtype=torch.float64,
                                                 device=dev)
-    ph = torch.empty((P, nw, 6, n_angles), dtype=torch.float64, device=dev) if want_phase \
+    n_phase = n_angles if mu is None else int(np.size(mu))
+    ph = torch.empty((P, nw, 6, n_phase), dtype=torch.float64, device=dev) if want_phase \
         else None
     groups: dict[int, list[int]] = {}
     for i, part in enumerate(parts):
@@ -261,13 +270,13 @@ def _parts_device(parts, wl: np.ndarray, *, order, n_radius, n_angles, allow_unv
             t(kind), t(p0), t(p1), t(np.tile(wl.reshape(-1), len(idx))),
             t(m[:, 0] if L == 1 else m), fr, p2=t(p2), n_radius=n_radius, n_angles=n_angles,
             order=order, normalized=False, allow_unvalidated=allow_unvalidated,
-            want_phase=want_phase)
+            want_phase=want_phase, mu=mu)
         sel = torch.tensor(idx, device=dev)
         sc[sel] = s.reshape(len(idx), nw, 11)
         if gr is not None:
             gr[sel] = g.reshape(len(idx), nw, 6, order + 1)
         if ph is not None:
-            ph[sel] = p.reshape(len(idx), nw, 6, n_angles)
+            ph[sel] = p.reshape(len(idx), nw, 6, n_phase)
         terms += n
     return sc, gr, ph, terms
 
@@ -286,8 +295,9 @@ class EnsembleGreek:
 
 @dataclass(frozen=True, eq=False)
 class EnsemblePhase:
-    """The phase matrix at the cosines ``mu`` (the n_angles Gauss-Legendre nodes), arrays of
-    shape grid + wavelength.shape + (n_angles,); f22 = f11 and f44 = f33 for spheres."""
+    """The phase matrix at the cosines ``mu`` (the caller's, or the n_angles Gauss-Legendre
+    nodes), arrays of shape grid + wavelength.shape + (len(mu),); f22 = f11 and f44 = f33 for
+    spheres."""
 
     mu: np.ndarray
     f11: np.ndarray
@@ -324,8 +334,9 @@ def _frozen(a: np.ndarray) -> np.ndarray:
     return a
 
 
-def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int):
-    """EnsembleResult of outputs with rows in the C order of ``shape``."""
+def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int, mu=None):
+    """EnsembleResult of outputs with rows in the C order of ``shape``; the phase matrix at the
+    cosines mu, or at the n_angles Gauss-Legendre cosines."""
     s = sc.reshape(-1, 11).cpu().numpy()
     fields = [_frozen(s[:, k].reshape(shape)) for k in range(11)]
     g = None
@@ -336,9 +347,11 @@ def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int)
     if ph is not None:
         from .quadrature import gauss_legendre
 
-        a = ph.reshape(-1, 6, n_angles).cpu().numpy()
-        mu, _ = gauss_legendre(n_angles)
-        p = EnsemblePhase(_frozen(mu), *(_frozen(a[:, k].reshape(shape + (n_angles,)))
+        if mu is None:
+            mu, _ = gauss_legendre(n_angles)
+        mu = np.array(mu, np.float64).reshape(-1)
+        a = ph.reshape(-1, 6, mu.size).cpu().numpy()
+        p = EnsemblePhase(_frozen(mu), *(_frozen(a[:, k].reshape(shape + (mu.size,)))
                                          for k in range(6)))
     return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()), terms=terms)
 
@@ -379,8 +392,10 @@ class Ensemble:
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
                 n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
-                phase: bool = True) -> EnsembleResult:
-        """Averages over the ensemble at each wavelength on the fixed discrete path."""
+                phase: bool = True, mu=None) -> EnsembleResult:
+        """Averages over the ensemble at each wavelength on the fixed discrete path; the phase
+        matrix at the cosines ``mu`` (the reference's mu argument) or at the n_angles
+        Gauss-Legendre cosines."""
         import torch
 
         wl = np.asarray(wavelength, np.float64)
@@ -389,10 +404,10 @@ class Ensemble:
             sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                               n_angles=n_angles,
                                               allow_unvalidated=allow_unvalidated,
-                                              want_phase=phase)
+                                              want_phase=phase, mu=mu)
             f = torch.tensor(fr, dtype=torch.float64, device=sc.device)
             sc, gr, ph = mix_parts(sc, gr, ph, f, normalized)
-            return _result(sc, gr, ph, wl.shape, wl, n_angles, terms)
+            return _result(sc, gr, ph, wl.shape, wl, n_angles, terms, mu)
         p0, p1, p2 = self.size.params()
         grid = np.broadcast(p0, p1, p2).shape
         p0g, p1g, p2g = (np.broadcast_to(v, grid).reshape(-1) for v in (p0, p1, p2))
@@ -410,8 +425,8 @@ class Ensemble:
             t(kind), t(np.repeat(p0g, nw)), t(np.repeat(p1g, nw)), t(np.tile(wl.reshape(-1), nd)),
             t(mm[:, 0] if L == 1 else mm), fr, p2=t(np.repeat(p2g, nw)), n_radius=n_radius,
             n_angles=n_angles, order=greek, normalized=normalized,
-            allow_unvalidated=allow_unvalidated, want_phase=phase)
-        return _result(sc, gr, ph, grid + wl.shape, wl, n_angles, terms)
+            allow_unvalidated=allow_unvalidated, want_phase=phase, mu=mu)
+        return _result(sc, gr, ph, grid + wl.shape, wl, n_angles, terms, mu)
 
 
 @dataclass(frozen=True)
@@ -455,8 +470,9 @@ class Mixture:
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
                 n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
-                phase: bool = True) -> EnsembleResult:
-        """Averages over the mixture at each wavelength on the fixed discrete path."""
+                phase: bool = True, mu=None) -> EnsembleResult:
+        """Averages over the mixture at each wavelength on the fixed discrete path (the phase
+        matrix at the cosines ``mu`` when given)."""
         import torch
 
         wl = np.asarray(wavelength, np.float64)
@@ -474,7 +490,7 @@ class Mixture:
                 fr.append(w / total)
         sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                           n_angles=n_angles, allow_unvalidated=allow_unvalidated,
-                                          want_phase=phase)
+                                          want_phase=phase, mu=mu)
         f = torch.tensor(fr, dtype=torch.float64, device=sc.device)
         sc, gr, ph = mix_parts(sc, gr, ph, f, normalized)
-        return _result(sc, gr, ph, wl.shape, wl, n_angles, terms)
+        return _result(sc, gr, ph, wl.shape, wl, n_angles, terms, mu)

@@ -281,3 +281,62 @@ def test_new_distributions_match_the_oracle():
         ensemble_fixed_device(_dev(np.array([3], np.int32)), _dev(np.array([1.0])),
                               _dev(np.array([0.5])), _dev(np.array([0.5])),
                               _dev(np.array([1.5 + 0j])), p2=_dev(np.array([0.2])), **kw)
+
+
+@pytest.mark.parametrize("kernel", [0, 1, 2])
+def test_phase_at_the_callers_cosines(kernel):
+    """scatrs_set_phase_cosines: the phase matrix at arbitrary cosines (unsorted, repeated, 0 and
+    +-1) with each angular kernel against the oracle (1e-10, the fp16-split kernel 1e-6 of max
+    F11); at the rule's own cosines the rule's phase matrix; the scalars and Greek
+    coefficients unchanged; through Ensemble and Mixture; the reverse pass refuses phase
+    cotangents while cosines are set."""
+    from scatrs_amd import _native
+    from scatrs_amd.ensemble import Ensemble, Mixture, ensemble_fixed_device
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.quadrature import gauss_legendre
+    from scatrs_amd.size import LogNormal, PowerLaw
+
+    ctx = _native.context(0)
+    kind = np.array([0, 1, 3], np.int32)
+    p0, p1, p2 = np.array([0.3, 0.6, 3.0]), np.array([1.6, 0.15, 0.05]), np.array([0, 0, 1.5])
+    wl = np.array([0.55, 0.9, 0.45])
+    m = np.array([1.45 + 0.01j, 1.38 + 1e-5j, 1.5 + 0.002j])
+    mu = np.array([0.3, -1.0, 1.0, 0.0, -0.3, 0.95, 0.3, -0.7071])
+    kw = dict(n_radius=64, n_angles=96, order=24)
+    args = [_dev(kind), _dev(p0), _dev(p1), _dev(wl), _dev(m)]
+    ctx.set_angular_kernel(kernel)
+    try:
+        s0, g0, f0, _ = ensemble_fixed_device(*args, p2=_dev(p2), **kw)
+        s1, g1, f1, _ = ensemble_fixed_device(*args, p2=_dev(p2), mu=mu, **kw)
+        rule, _ = gauss_legendre(96)
+        _, _, f2, _ = ensemble_fixed_device(*args, p2=_dev(p2), mu=rule, **kw)
+    finally:
+        ctx.set_angular_kernel(0)
+    assert torch.equal(s0, s1) and torch.equal(g0, g1)
+    # numpy's rule labels the outputs; the library's differs from it in the last bits only
+    assert float((f0 - f2).abs().max() / f0[:, 0].abs().max()) <= 1e-12
+    f1 = f1.cpu().numpy()
+    tol = 1e-6 if kernel == 2 else 1e-10
+    for e in range(3):
+        ref = oracle.ensemble_fixed(int(kind[e]), p0[e], p1[e], wl[e], m[e], p2=p2[e], mu=mu,
+                                    **kw)["phase"]
+        err = np.max(np.abs(f1[e] - ref)) / np.max(np.abs(ref[0]))
+        assert err <= tol, (kernel, e, err)
+    r = Ensemble(Sphere(m[0]), LogNormal(0.3, 1.6)).compute(wl[:1], greek=24, mu=mu, n_radius=64,
+                                                             n_angles=96)
+    assert r.phase.f11.shape == (1, mu.size) and np.array_equal(r.phase.mu, mu)
+    mix = Mixture(((1.0, Ensemble(Sphere(m[0]), LogNormal(0.3, 1.6))),
+                   (2.0, Ensemble(Sphere(m[2]), PowerLaw(3.0, 0.05, 1.5)))))
+    rm = mix.compute(wl[:1], greek=24, mu=mu, n_radius=64, n_angles=96)
+    assert rm.phase.f33.shape == (1, mu.size) and np.all(np.isfinite(rm.phase.f11))
+    ctx.set_phase_cosines(mu)
+    try:
+        from scatrs_amd.ensemble import ensemble_fixed_vjp_device
+
+        with pytest.raises(InvalidInputError, match="phase cotangents"):
+            ensemble_fixed_vjp_device(*args, p2=_dev(p2), bar_phase=torch.ones_like(f0),
+                                      **kw)
+    finally:
+        ctx.set_phase_cosines(None)
+    with pytest.raises(InvalidInputError, match=r"mu must be in \[-1, 1\]"):
+        ctx.set_phase_cosines([0.5, 1.5])

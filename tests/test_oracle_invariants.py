@@ -115,3 +115,24 @@ def test_new_distributions_match_closed_forms():
         oracle.ensemble_fixed(oracle.MODIFIED_GAMMA, -1.0, 1.0, 0.5, 1.5 + 0j, p2=1.0)
     with pytest.raises(ValueError):
         oracle.ensemble_fixed(oracle.POWER_LAW, 2.0, 0.5, 0.5, 1.5 + 0j, p2=0.5)
+
+
+def test_phase_at_the_callers_cosines():
+    """The phase matrix at given cosines (the reference's mu argument): at the rule's own
+    cosines it is the rule's phase matrix exactly; at mu = +-1 the sphere identities
+    F12 = F34 = 0, F33 = F11 forward and -F11 backward; normalized, (1/2) sum w F11 = 1 over a
+    second Gauss-Legendre rule; the scalars and Greek coefficients do not change."""
+    kw = dict(n_radius=48, n_angles=64, order=16)
+    args = (0, 0.2, 1.5, 0.55, 1.45 + 0.01j)
+    a = oracle.ensemble_fixed(*args, **kw)
+    mu64, _ = oracle.gauss_legendre(64)
+    b = oracle.ensemble_fixed(*args, mu=mu64, **kw)
+    assert np.array_equal(a["phase"], b["phase"]) and np.array_equal(a["greek"], b["greek"])
+    assert np.array_equal(a["scalars"], b["scalars"])
+    c = oracle.ensemble_fixed(*args, mu=[-1.0, 1.0], **kw)["phase"]
+    f11 = c[0]
+    assert np.all(np.abs(c[1]) <= 1e-15 * f11) and np.all(np.abs(c[4]) <= 1e-15 * f11)
+    assert abs(c[3, 1] / f11[1] - 1) < 1e-15 and abs(c[3, 0] / f11[0] + 1) < 1e-15
+    mu40, w40 = oracle.gauss_legendre(40)
+    d = oracle.ensemble_fixed(*args, mu=mu40, **kw)["phase"]
+    assert abs(0.5 * np.sum(w40 * d[0]) - 1) < 1e-12
