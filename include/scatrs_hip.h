@@ -97,6 +97,10 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
 #define SCATRS_MODIFIED_GAMMA 2 /* n ~ r^alpha exp(-b r^gamma): p0 = alpha, p1 = b, p2 = gamma */
 #define SCATRS_POWER_LAW 3      /* n ~ r^-alpha on [p1, p2]: p0 = alpha, p1 = r_min, p2 = r_max */
 #define SCATRS_MONODISPERSE 4   /* p0 = radius */
+/* One segment of a tabulated law: n linear on [p0, p1], n ~ (1 - p2)(1 - s) + p2 s,
+ * s = (r - p0)/(p1 - p0), 0 <= p2 <= 1; no parameters to differentiate. A tabulated law
+ * (distribution.rs:55-61) is the mixture of its segments weighted by their number integrals. */
+#define SCATRS_SEGMENT 5
 
 /* The fixed discrete path of a batch of ensembles (one distribution at one wavelength each;
  * the reference's EnsembleResult granularity, scatrs/src/ensemble/mod.rs:508-529):

@@ -201,7 +201,7 @@ def radius_range(kind: int, p0: float, p1: float, p2: float = 0.0):
     return lo.value, hi.value
 
 
-LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE = 0, 1, 2, 3, 4
+LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE, SEGMENT = 0, 1, 2, 3, 4, 5
 
 
 def ensemble_fixed(kind: int, p0: float, p1: float, wavelength: float, m, fractions=None,

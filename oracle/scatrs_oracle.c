@@ -1111,6 +1111,9 @@ double orc_density(int kind, const double *p, double r, double grad[3]) {
         double l = log(r / r0);
         ln_u = -p[0] * l;
         d[0] = -l;
+    } else if (kind == ORC_SEGMENT) {
+        double s = (r - p[0]) / (p[1] - p[0]);
+        ln_u = log((1.0 - p[2]) * (1.0 - s) + p[2] * s);
     }
     double u = exp(ln_u);
     if (grad)
@@ -1128,6 +1131,9 @@ static int dist_valid(int kind, const double *p) {
     if (kind == ORC_POWER_LAW)
         return isfinite(p[0]) && p[1] > 0.0 && isfinite(p[1]) && p[2] > p[1] && isfinite(p[2]);
     if (kind == ORC_MONODISPERSE) return p[0] > 0.0 && isfinite(p[0]);
+    if (kind == ORC_SEGMENT)
+        return p[0] > 0.0 && isfinite(p[0]) && p[1] > p[0] && isfinite(p[1]) && p[2] >= 0.0 &&
+               p[2] <= 1.0;
     return 0;
 }
 
@@ -1201,6 +1207,9 @@ int orc_radius_range(int kind, const double *p, double *t_lo, double *t_hi) {
     } else if (kind == ORC_POWER_LAW) {
         *t_lo = log(p[1]);
         *t_hi = log(p[2]);
+    } else if (kind == ORC_SEGMENT) {
+        *t_lo = log(p[0]);
+        *t_hi = log(p[1]);
     } else {
         *t_lo = *t_hi = log(p[0]);
     }
@@ -1373,6 +1382,10 @@ static double log_density_slope(int kind, const double *p, double r) {
     if (kind == ORC_GAMMA) return (1.0 - 3.0 * p[1]) / p[1] - r / (p[0] * p[1]);
     if (kind == ORC_MODIFIED_GAMMA) return p[0] - p[1] * p[2] * pow(r, p[2]);
     if (kind == ORC_POWER_LAW) return -p[0];
+    if (kind == ORC_SEGMENT) {
+        double w = p[1] - p[0], s = (r - p[0]) / w;
+        return r * (2.0 * p[2] - 1.0) / (w * ((1.0 - p[2]) * (1.0 - s) + p[2] * s));
+    }
     return 0.0;
 }
 
@@ -1406,6 +1419,7 @@ static double range_derivatives(int kind, const double *p, double dmid[3], doubl
         return 0.5 * (t_hi - t_lo);
     }
     if (kind == ORC_POWER_LAW) return 0.5 * (log(p[2]) - log(p[1]));
+    if (kind == ORC_SEGMENT) return 0.5 * (log(p[1]) - log(p[0]));
     dmid[0] = 1.0 / p[0]; /* monodisperse: every node at the radius */
     return 0.0;
 }

@@ -95,7 +95,8 @@ enum {
     ORC_GAMMA = 1,          /* p = (r_eff, v_eff, -) */
     ORC_MODIFIED_GAMMA = 2, /* p = (alpha, b, gamma) */
     ORC_POWER_LAW = 3,      /* p = (alpha, r_min, r_max): r_min, r_max fixed */
-    ORC_MONODISPERSE = 4    /* p = (radius, -, -) */
+    ORC_MONODISPERSE = 4,   /* p = (radius, -, -) */
+    ORC_SEGMENT = 5         /* p = (r_0, r_1, tau): n ~ (1 - tau)(1 - s) + tau s on [r_0, r_1] */
 };
 
 /* The unnormalized density u(r) of parameters p[3] with the reference radius fixed, and

@@ -4,7 +4,10 @@
 // Distributions (scatrs/src/ensemble/distribution.rs:15-68) and their parameters p[3]:
 //   0 lognormal (r_g, sigma_g, -), 1 gamma (a = r_eff, b = v_eff, -),
 //   2 modified gamma (alpha, b, gamma), 3 power law (alpha; r_min, r_max, which are fixed),
-//   4 monodisperse (radius, -, -).
+//   4 monodisperse (radius, -, -), 5 one segment of a tabulated law (r_0, r_1, tau): n linear
+//   on [r_0, r_1], n ~ (1 - tau)(1 - s) + tau s with s = (r - r_0)/(r_1 - r_0), no parameters
+//   to differentiate (a tabulated law is the mixture of its segments, each weighted by its
+//   number integral; distribution.rs:55-61,432-441).
 #pragma once
 
 #include <cmath>
@@ -16,7 +19,14 @@ namespace scatrs {
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr int kMaxLayers = 16;
 
-enum : int { kLogNormal = 0, kGamma = 1, kModifiedGamma = 2, kPowerLaw = 3, kMonodisperse = 4 };
+enum : int {
+    kLogNormal = 0,
+    kGamma = 1,
+    kModifiedGamma = 2,
+    kPowerLaw = 3,
+    kMonodisperse = 4,
+    kSegment = 5
+};
 
 // The radius where the unnormalized density is scaled near 1 (distribution.rs:293-318).
 SCATRS_HD double dist_reference(int kind, const double *p) {
@@ -39,6 +49,7 @@ SCATRS_HD bool dist_valid(int kind, const double *p) {
     if (kind == kModifiedGamma) return f0 && p[0] > -1.0 && f1 && p[1] > 0.0 && f2 && p[2] > 0.0;
     if (kind == kPowerLaw) return f0 && f1 && p[1] > 0.0 && f2 && p[2] > p[1];
     if (kind == kMonodisperse) return f0 && p[0] > 0.0;
+    if (kind == kSegment) return f0 && p[0] > 0.0 && f1 && p[1] > p[0] && p[2] >= 0.0 && p[2] <= 1.0;
     return false;
 }
 
@@ -109,6 +120,9 @@ SCATRS_HD void radius_range(int kind, const double *p, double *t_lo, double *t_h
     } else if (kind == kPowerLaw) {
         *t_lo = log(p[1]);
         *t_hi = log(p[2]);
+    } else if (kind == kSegment) {
+        *t_lo = log(p[0]);
+        *t_hi = log(p[1]);
     } else {
         *t_lo = *t_hi = log(p[0]);
     }
@@ -151,6 +165,7 @@ SCATRS_HD double range_derivatives(int kind, const double *p, double dmid[3], do
         return 0.5 * (t_hi - t_lo);
     }
     if (kind == kPowerLaw) return 0.5 * (log(p[2]) - log(p[1]));
+    if (kind == kSegment) return 0.5 * (log(p[1]) - log(p[0]));
     dmid[0] = 1.0 / p[0];  // monodisperse: every node at the radius
     return 0.0;
 }
@@ -164,6 +179,10 @@ SCATRS_HD double log_density_slope(int kind, const double *p, double r) {
     if (kind == kGamma) return (1.0 - 3.0 * p[1]) / p[1] - r / (p[0] * p[1]);
     if (kind == kModifiedGamma) return p[0] - p[1] * p[2] * pow(r, p[2]);
     if (kind == kPowerLaw) return -p[0];
+    if (kind == kSegment) {
+        const double w = p[1] - p[0], s = (r - p[0]) / w;
+        return r * (2.0 * p[2] - 1.0) / (w * ((1.0 - p[2]) * (1.0 - s) + p[2] * s));
+    }
     return 0.0;
 }
 
@@ -198,6 +217,9 @@ SCATRS_HD double log_density(int kind, const double *p, double r, double *d) {
         const double l = log(r / r0);
         d0 = -l;
         ln_u = -p[0] * l;
+    } else if (kind == kSegment) {
+        const double s = (r - p[0]) / (p[1] - p[0]);
+        ln_u = log((1.0 - p[2]) * (1.0 - s) + p[2] * s);
     }
     if (d) {
         d[0] = d0;

@@ -1164,6 +1164,11 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
             return pre + "min_radius must be < max_radius, got " + fmt_e(p1) + " and " + fmt_e(p2);
         }
         if (kind == kMonodisperse) return positive("radius", p0);
+        if (kind == kSegment) {
+            if (!(std::isfinite(p0) && p0 > 0.0)) return positive("radius", p0);
+            if (!(std::isfinite(p1) && p1 > p0)) return pre + "tabulated radii must increase";
+            return pre + "a tabulated segment's tilt must be in [0, 1], got " + fmt_e(p2);
+        }
         return pre + "unknown distribution kind " + std::to_string(kind);
     }
     if (!(std::isfinite(wl) && wl > 0.0))

@@ -20,7 +20,7 @@ import numpy as np
 from . import _native
 from .errors import InvalidInputError
 from .particles import LayeredSphere, Sphere
-from .size import Distribution, MultiModal, check
+from .size import SEGMENT, Distribution, MultiModal, Tabulated, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
@@ -361,9 +361,22 @@ def _part(size, particle):
     return (size.kind, *(float(v) for v in size.params()), particle)
 
 
-def _mode_parts(size: MultiModal, particle, weight: float = 1.0):
-    """The parts of a multimodal distribution and their number fractions."""
-    return [_part(d, particle) for _, d in size.modes], list(weight * size.fractions())
+def _parts_of(size, particle, weight: float = 1.0):
+    """The parts (kind, p0, p1, p2, particle) a distribution is integrated as and their number
+    fractions times ``weight``: the distribution itself, a multimodal law's modes
+    (engine.rs:143-160), or a tabulated law's segments."""
+    if isinstance(size, MultiModal):
+        parts, fr = [], []
+        for (_, d), f in zip(size.modes, size.fractions()):
+            p, q = _parts_of(d, particle, weight * f)
+            parts += p
+            fr += q
+        return parts, fr
+    if isinstance(size, Tabulated):
+        segs, w = size.segments()
+        return ([(SEGMENT, r0, r1, tilt, particle) for r0, r1, tilt in segs],
+                list(weight * w / w.sum()))
+    return [_part(size, particle)], [weight]
 
 
 @dataclass(frozen=True)
@@ -371,8 +384,8 @@ class Ensemble:
     """Particles of one kind over a size distribution (python/scatrs/ensemble.py:484-529).
 
     ``size`` may have array-valued parameters: a grid of distributions, computed at once
-    (the results have the grid's axes first). A MultiModal size is integrated mode by mode and
-    mixed (mix_parts)."""
+    (the results have the grid's axes first). A MultiModal size is integrated mode by mode, a
+    Tabulated one segment by segment, and the parts mixed (mix_parts)."""
 
     particle: Sphere | LayeredSphere
     size: Distribution
@@ -386,7 +399,7 @@ class Ensemble:
     @property
     def grid_shape(self) -> tuple[int, ...]:
         """The shape of the grid of distributions (``()`` without array-valued parameters)."""
-        if isinstance(self.size, MultiModal):
+        if isinstance(self.size, (MultiModal, Tabulated)):
             return ()
         return np.broadcast(*self.size.params()).shape
 
@@ -399,8 +412,8 @@ class Ensemble:
         import torch
 
         wl = np.asarray(wavelength, np.float64)
-        if isinstance(self.size, MultiModal):
-            parts, fr = _mode_parts(self.size, self.particle)
+        if isinstance(self.size, (MultiModal, Tabulated)):
+            parts, fr = _parts_of(self.size, self.particle)
             sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                               n_angles=n_angles,
                                               allow_unvalidated=allow_unvalidated,
@@ -481,13 +494,9 @@ class Mixture:
         for w, e in self.components:
             if w == 0.0:
                 continue
-            if isinstance(e.size, MultiModal):
-                p, f = _mode_parts(e.size, e.particle, w / total)
-                parts += p
-                fr += f
-            else:
-                parts.append(_part(e.size, e.particle))
-                fr.append(w / total)
+            p, f = _parts_of(e.size, e.particle, w / total)
+            parts += p
+            fr += f
         sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                           n_angles=n_angles, allow_unvalidated=allow_unvalidated,
                                           want_phase=phase, mu=mu)

@@ -9,7 +9,7 @@ import numpy as np
 
 from .errors import InvalidInputError
 
-LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE = 0, 1, 2, 3, 4
+LOGNORMAL, GAMMA, MODIFIED_GAMMA, POWER_LAW, MONODISPERSE, SEGMENT = 0, 1, 2, 3, 4, 5
 _ZERO = np.float64(0.0)
 
 
@@ -96,6 +96,49 @@ class Monodisperse:
         return np.asarray(self.radius, np.float64), _ZERO, _ZERO
 
 
+@dataclass(frozen=True)
+class Tabulated:
+    """n(r) linear between the points (radius[i], density[i]) and 0 outside them: radii
+    increasing and positive, densities >= 0 and not all 0; data, not parameters
+    (distribution.rs:55-61,149-170). Integrated segment by segment, each a linear density on
+    its interval (the library's segment kind), and mixed by the segments' number integrals
+    (d_i + d_i+1)/2 (r_i+1 - r_i), so that the kinks at the points need no quadrature."""
+
+    radius: tuple
+    density: tuple
+
+    names = ()
+
+    def __post_init__(self) -> None:
+        r = np.asarray(self.radius, np.float64).reshape(-1)
+        d = np.asarray(self.density, np.float64).reshape(-1)
+        if r.size < 2 or r.size != d.size:
+            raise InvalidInputError(
+                "a tabulated distribution needs at least 2 points and as many densities as "
+                f"radii, got {r.size} radii and {d.size} densities")
+        for v in r:
+            if not (np.isfinite(v) and v > 0.0):
+                raise InvalidInputError(f"radius must be finite and > 0, got {v}")
+        if np.any(np.diff(r) <= 0.0):
+            raise InvalidInputError("tabulated radii must increase")
+        if not np.all(np.isfinite(d) & (d >= 0.0)):
+            raise InvalidInputError("tabulated densities must be finite and >= 0")
+        if not np.any(d > 0.0):
+            raise InvalidInputError("tabulated densities must not all be 0")
+        object.__setattr__(self, "radius", tuple(float(v) for v in r))
+        object.__setattr__(self, "density", tuple(float(v) for v in d))
+
+    def segments(self):
+        """The segments with any density: (r_0, r_1, tilt d_1 / (d_0 + d_1)) and their number
+        integrals."""
+        out, w = [], []
+        for r0, r1, d0, d1 in zip(self.radius, self.radius[1:], self.density, self.density[1:]):
+            if d0 + d1 > 0.0:
+                out.append((r0, r1, d1 / (d0 + d1)))
+                w.append(0.5 * (d0 + d1) * (r1 - r0))
+        return out, np.array(w)
+
+
 _SINGLE = (LogNormal, Gamma, ModifiedGamma, PowerLaw, Monodisperse)
 
 
@@ -125,10 +168,11 @@ class MultiModal:
             w = float(w)
             if not (np.isfinite(w) and w > 0.0):
                 raise InvalidInputError(f"the weight of mode {j} must be finite and > 0, got {w}")
-            if not isinstance(d, _SINGLE):
-                raise TypeError(f"mode {j} must be a LogNormal, Gamma, ModifiedGamma, PowerLaw "
-                                f"or Monodisperse distribution, not {type(d).__name__}")
-            if any(np.ndim(p) for p in d.params()):
+            if not isinstance(d, _SINGLE + (Tabulated,)):
+                raise TypeError(f"mode {j} must be a LogNormal, Gamma, ModifiedGamma, PowerLaw, "
+                                f"Monodisperse or Tabulated distribution, not "
+                                f"{type(d).__name__}")
+            if not isinstance(d, Tabulated) and any(np.ndim(p) for p in d.params()):
                 raise InvalidInputError(f"mode {j}: a mode takes scalar parameters")
             out.append((w, d))
         object.__setattr__(self, "modes", tuple(out))
@@ -139,15 +183,17 @@ class MultiModal:
         return w / w.sum()
 
 
-Distribution = LogNormal | Gamma | ModifiedGamma | PowerLaw | Monodisperse | MultiModal
+Distribution = (LogNormal | Gamma | ModifiedGamma | PowerLaw | Monodisperse | Tabulated |
+                MultiModal)
 
 
 def check(d: object, what: str = "size") -> Distribution:
-    if isinstance(d, MultiModal):
+    if isinstance(d, (MultiModal, Tabulated)):
         return d
     if not isinstance(d, _SINGLE):
         raise TypeError(f"{what} must be a LogNormal, Gamma, ModifiedGamma, PowerLaw, "
-                        f"Monodisperse or MultiModal distribution, not {type(d).__name__}")
+                        f"Monodisperse, Tabulated or MultiModal distribution, not "
+                        f"{type(d).__name__}")
     try:
         np.broadcast(*d.params())
     except ValueError as e:

@@ -340,3 +340,38 @@ def test_phase_at_the_callers_cosines(kernel):
         ctx.set_phase_cosines(None)
     with pytest.raises(InvalidInputError, match=r"mu must be in \[-1, 1\]"):
         ctx.set_phase_cosines([0.5, 1.5])
+
+
+def test_tabulated_ensembles_match_the_oracle():
+    """A tabulated law (its segments on the device, mixed by their number integrals) alone, as
+    a mode of a multimodal law and in a mixture, against the oracle's mixture of the same
+    segments; the segment kind's errors."""
+    from scatrs_amd.ensemble import SCALARS, Ensemble, Mixture
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import LogNormal, MultiModal, Tabulated
+
+    radius, density = [0.05, 0.2, 0.6, 1.5], [0.0, 3.0, 1.0, 0.2]
+    tab = Tabulated(radius, density)
+    m = 1.47 + 0.004j
+    wl = np.array([0.45, 0.9])
+    kw = dict(n_radius=64, n_angles=64)
+    segs, w = tab.segments()
+    f = w / w.sum()
+    parts = [(fi, 5, r0, r1, m, None, tilt) for fi, (r0, r1, tilt) in zip(f, segs)]
+    r = Ensemble(Sphere(m), tab).compute(wl, greek=16, **kw)
+    mm = Ensemble(Sphere(m), MultiModal(((1.0, tab), (1.0, LogNormal(0.3, 1.5))))).compute(
+        wl, greek=16, **kw)
+    mix = Mixture(((3.0, Ensemble(Sphere(m), tab)),)).compute(wl, greek=16, **kw)
+    for j, lam in enumerate(wl):
+        ref = oracle.ensemble_mixture(parts, lam, order=16, **kw)
+        got = np.array([getattr(r, k)[j] for k in SCALARS])
+        rel = np.abs(got - ref["scalars"]) / np.maximum(np.abs(ref["scalars"]), 1e-300)
+        assert rel.max() <= 1e-10, (j, rel)
+        assert np.max(np.abs(r.greek.alpha1[j] - ref["greek"][0])) <= 1e-10 * abs(ref["greek"][0]).max()
+        assert abs(mix.c_ext[j] / r.c_ext[j] - 1) < 1e-14
+        both = [(0.5 * p[0], *p[1:]) for p in parts] + [(0.5, 0, 0.3, 1.5, m, None)]
+        rb = oracle.ensemble_mixture(both, lam, order=16, **kw)
+        assert abs(mm.c_sca[j] / rb["scalars"][1] - 1) <= 1e-10
+    with pytest.raises(InvalidInputError, match="tilt must be in"):
+        _gpu([5], [0.1], [0.2], [0.5], [1.5 + 0j], None, n_radius=16, n_angles=16, order=4,
+             p2=_dev(np.array([1.5])))

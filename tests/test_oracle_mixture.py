@@ -139,3 +139,49 @@ def test_multimodal_and_mixture_validate_like_the_reference():
     mm = MultiModal(((3.0, LogNormal(0.1, 1.5)), (1.0, Gamma(0.2, 0.1))))
     assert np.allclose(mm.fractions(), [0.75, 0.25])
     assert Ensemble(sph, mm).grid_shape == ()
+
+
+def test_tabulated_law_is_its_segments_mixed():
+    """A tabulated law (n linear between its points, distribution.rs:55-61): its segments, each
+    a linear density on its interval, mixed by their number integrals, give the moments of the
+    piecewise-linear density in closed form (sum_i int r^k n dr / sum_i int n dr)."""
+    from scatrs_amd.size import Tabulated
+
+    radius, density = [0.05, 0.2, 0.6, 1.5], [0.0, 3.0, 1.0, 0.2]
+    segs, w = Tabulated(radius, density).segments()
+    f = w / w.sum()
+    parts = [(fi, oracle.SEGMENT, r0, r1, 1.5 + 0j, None, tilt)
+             for fi, (r0, r1, tilt) in zip(f, segs)]
+    r = oracle.ensemble_mixture(parts, 0.5, n_radius=64, n_angles=8, order=-1)
+
+    def integral(k):
+        out = 0.0
+        for r0, r1, d0, d1 in zip(radius, radius[1:], density, density[1:]):
+            b = (d1 - d0) / (r1 - r0)
+            a = d0 - b * r0
+            out += a * (r1 ** (k + 1) - r0 ** (k + 1)) / (k + 1) + \
+                b * (r1 ** (k + 2) - r0 ** (k + 2)) / (k + 2)
+        return out
+
+    mom = lambda k: integral(k) / integral(0)  # noqa: E731
+    assert abs(r["scalars"][9] / (np.pi * mom(2)) - 1) < 1e-13
+    assert abs(r["scalars"][8] / (mom(3) / mom(2)) - 1) < 1e-13
+    # one flat segment is the uniform law on its interval
+    u = oracle.ensemble_fixed(oracle.SEGMENT, 0.3, 0.9, 0.5, 1.5 + 0j, n_radius=32, n_angles=8,
+                              order=-1, p2=0.5)
+    assert abs(u["scalars"][9] / (np.pi * (0.9 ** 3 - 0.3 ** 3) / (3 * 0.6)) - 1) < 1e-14
+
+
+def test_tabulated_validates_like_the_reference():
+    """distribution.rs:148-170."""
+    from scatrs_amd.errors import InvalidInputError
+    from scatrs_amd.size import Tabulated
+
+    with pytest.raises(InvalidInputError, match="at least 2 points"):
+        Tabulated([0.1], [1.0])
+    with pytest.raises(InvalidInputError, match="must increase"):
+        Tabulated([0.2, 0.1], [1.0, 1.0])
+    with pytest.raises(InvalidInputError, match="finite and >= 0"):
+        Tabulated([0.1, 0.2], [1.0, -1.0])
+    with pytest.raises(InvalidInputError, match="not all be 0"):
+        Tabulated([0.1, 0.2], [0.0, 0.0])
