@@ -71,8 +71,8 @@ int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
  * d_phase is then (count, 6, n_mu). The angular products run once more at these cosines (with
  * the selected kernel); the Greek coefficients still come from the n_angles-point rule. The
  * reference's mu argument (python/scatrs/ensemble.py:550-562, scatrs-py/src/ensemble.rs:
- * 295-355). scatrs_ensemble_fixed_vjp keeps phase cotangents at the Gauss-Legendre cosines and
- * rejects d_bar_phase while cosines are set. Stays in effect until changed. */
+ * 295-355). scatrs_ensemble_fixed_vjp then takes d_bar_phase (count, 6, n_mu) at these cosines
+ * (its forward tables and reverse products run there too). Stays in effect until changed. */
 int scatrs_set_phase_cosines(scatrs_ctx *ctx, const double *mu, int64_t n_mu);
 
 /* The default truncation N = ceil(x + 6 x^(1/3) + 3) (host; scatrs/src/mie/mod.rs:122-124). */

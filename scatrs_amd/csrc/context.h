@@ -52,6 +52,7 @@ struct scatrs_ctx {
     std::vector<double> phase_mu;  // the caller's phase cosines (empty: the angle rule's)
     std::vector<double> phase_amu;  // their distinct |mu|, ascending
     std::vector<int> phase_idx;     // each cosine's place among -|mu|, +|mu| (2 h of them)
+    std::vector<int> phase_qptr, phase_qlist;  // the cosines at each place, gathered
     static constexpr int kMaxEvents = 1024;
     hipEvent_t ev_begin[kMaxEvents] = {}, ev_end[kMaxEvents] = {};
     int ev_count = 0;

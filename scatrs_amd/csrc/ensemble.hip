@@ -31,39 +31,11 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
     // The phase at the caller's cosines (scatrs_set_phase_cosines): the angular products once
-    // more, at the distinct |mu| as the non-negative cosines of a symmetric set of 2 h_u; the
+    // more, at the distinct |mu| as the non-negative cosines of a symmetric set of 2 h; the
     // Greek coefficients keep the angle rule's products (computed only when asked for).
     const bool user_mu = d_phase && !ctx->phase_mu.empty();
-    const int h_u = user_mu ? (int)ctx->phase_amu.size() : 0;
-    const int n_mu = user_mu ? (int)ctx->phase_mu.size() : 0;
-    double2 *table_u = nullptr;
-    _Float16 *tab16_u = nullptr;
-    const int *d_idx = nullptr;
-    if (user_mu) {
-        double *d_amu = nullptr;
-        if ((rc = upload(ctx, scatrs_ctx::kPhaseMu, ctx->phase_amu, &d_amu))) return rc;
-        const size_t tab_bytes = (size_t)S.n_max * h_u * sizeof(double2);
-        const size_t cells = (size_t)h_u * S.npad;
-        const size_t idx_off = tab_bytes + (S.kernel == 2 ? 4 * cells * sizeof(_Float16) : 0);
-        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTablesMu,
-                                        (idx_off + n_mu * sizeof(int) + 255) / 256 * 256),
-                            "table")))
-            return rc;
-        char *base = static_cast<char *>(ctx->buf[scatrs_ctx::kTablesMu]);
-        table_u = reinterpret_cast<double2 *>(base);
-        int *idx = reinterpret_cast<int *>(base + idx_off);
-        if ((rc = check_hip(ctx, hipMemcpyAsync(idx, ctx->phase_idx.data(), n_mu * sizeof(int),
-                                                hipMemcpyHostToDevice, s),
-                            "upload")))
-            return rc;
-        d_idx = idx;
-        pt_table<<<(h_u + 63) / 64, 64, 0, s>>>(d_amu, h_u, S.n_max, table_u);
-        if (S.kernel == 2) {
-            tab16_u = reinterpret_cast<_Float16 *>(base + tab_bytes);
-            pt_table_f16<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(table_u, h_u, S.n_max,
-                                                                         S.npad, S.texp, tab16_u);
-        }
-    }
+    PhaseMu U;
+    if (user_mu && (rc = phase_mu_setup(ctx, S, U))) return rc;
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B, 16 B in fp32).
     const int64_t max_terms = chunk_terms_limit(4, S.stage32 ? sizeof(float4) : 2 * sizeof(cx));
     int64_t e0 = 0;
@@ -82,12 +54,12 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         double *tab_u = nullptr, *fac_u = nullptr;
         if (user_mu) {
             if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAccumMu,
-                                            (size_t)ne * (4 * 2 * (size_t)h_u + 1) * sizeof(double)),
+                                            (size_t)ne * (4 * 2 * (size_t)U.h + 1) * sizeof(double)),
                                 "tables")))
                 return rc;
             tab_u = static_cast<double *>(ctx->buf[scatrs_ctx::kAccumMu]);
-            fac_u = tab_u + (size_t)ne * 4 * 2 * h_u;
-            if ((rc = angular_products(ctx, S, e0, ne, n_radius, h_u, 2 * h_u, table_u, tab16_u,
+            fac_u = tab_u + (size_t)ne * 4 * 2 * U.h;
+            if ((rc = angular_products(ctx, S, e0, ne, n_radius, U.h, 2 * U.h, U.table, U.tab16,
                                        B, tab_u, s)))
                 return rc;
         }
@@ -103,9 +75,9 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
                                                 d_scalars, gfac, user_mu ? nullptr : d_phase,
                                                 fac_u);
         if (user_mu) {
-            const int64_t cells = ne * 6 * (int64_t)n_mu;
+            const int64_t cells = ne * 6 * (int64_t)U.n_mu;
             ens_phase_mu<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(
-                e0, ne, 2 * h_u, d_idx, n_mu, tab_u, fac_u, d_phase);
+                e0, ne, 2 * U.h, U.idx, U.n_mu, tab_u, fac_u, d_phase);
         }
         if (d_greek) {
             const int64_t len = order + 1;

@@ -1595,6 +1595,53 @@ static int forward_angular(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                             B.tab_out, s);
 }
 
+// The caller's phase cosines of a call (scatrs_set_phase_cosines), on the device: the w pi_n,
+// w tau_n table at the h distinct |mu| (and its fp16 slices with kernel 2), each cosine's
+// place among the 2 h cosines -|mu|, +|mu|, and the cosines gathered by place.
+struct PhaseMu {
+    int h = 0, n_mu = 0;
+    double2 *table = nullptr;
+    _Float16 *tab16 = nullptr;
+    int *idx = nullptr, *q_ptr = nullptr, *q_list = nullptr;
+};
+
+static int phase_mu_setup(scatrs_ctx *ctx, const EnsSetup &S, PhaseMu &U) {
+    int rc;
+    hipStream_t s = ctx->stream;
+    U.h = (int)ctx->phase_amu.size();
+    U.n_mu = (int)ctx->phase_mu.size();
+    double *d_amu = nullptr;
+    if ((rc = upload(ctx, scatrs_ctx::kPhaseMu, ctx->phase_amu, &d_amu))) return rc;
+    const size_t tab_bytes = (size_t)S.n_max * U.h * sizeof(double2);
+    const size_t cells = (size_t)U.h * S.npad;
+    const size_t int_off = tab_bytes + (S.kernel == 2 ? 4 * cells * sizeof(_Float16) : 0);
+    const size_t n_int = (size_t)U.n_mu + (2 * (size_t)U.h + 1) + (size_t)U.n_mu;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTablesMu, int_off + n_int * sizeof(int)),
+                        "table")))
+        return rc;
+    char *base = static_cast<char *>(ctx->buf[scatrs_ctx::kTablesMu]);
+    U.table = reinterpret_cast<double2 *>(base);
+    U.idx = reinterpret_cast<int *>(base + int_off);
+    U.q_ptr = U.idx + U.n_mu;
+    U.q_list = U.q_ptr + 2 * U.h + 1;
+    const struct {
+        int *dst;
+        const std::vector<int> &src;
+    } copies[3] = {{U.idx, ctx->phase_idx}, {U.q_ptr, ctx->phase_qptr}, {U.q_list, ctx->phase_qlist}};
+    for (const auto &c : copies)
+        if ((rc = check_hip(ctx, hipMemcpyAsync(c.dst, c.src.data(), c.src.size() * sizeof(int),
+                                                hipMemcpyHostToDevice, s),
+                            "upload")))
+            return rc;
+    pt_table<<<(U.h + 63) / 64, 64, 0, s>>>(d_amu, U.h, S.n_max, U.table);
+    if (S.kernel == 2) {
+        U.tab16 = reinterpret_cast<_Float16 *>(base + tab_bytes);
+        pt_table_f16<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(U.table, U.h, S.n_max,
+                                                                     S.npad, S.texp, U.tab16);
+    }
+    return check_hip(ctx, hipGetLastError(), "table");
+}
+
 // The node solves and the angular products of a chunk on the context's stream.
 static int forward_chunk(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
                          int64_t chunk_terms, int64_t n_radius, int64_t n_angles, ChunkBufs &B,

@@ -49,12 +49,16 @@ static __device__ double block_sum(double v, double *red) {
 // From the output cotangents of one ensemble (bar_scalars (11), bar_greek (6 x (order + 1)),
 // bar_phase (6 x n_angles); each may be null) those of the sums, sbar[eb * 8 + k], and of the
 // summed tables, tbar[(eb * 4 + k) * n_angles + j]. Follows ens_finish step by step in reverse
-// (ensemble/engine.rs:1782-1836,2396-2497 for the forward steps).
+// (ensemble/engine.rs:1782-1836,2396-2497 for the forward steps). With n_out > 0 the phase
+// cotangents are at the caller's n_mu cosines: tab_u holds the tables at the n_out cosines
+// -|mu|, +|mu| (ens_phase_mu's layout) and position q gathers the cosines
+// q_list[q_ptr[q] .. q_ptr[q + 1]); their cotangents go to tbar_u in that layout.
 static __global__ __launch_bounds__(256) void ens_finish_vjp(
     int64_t e0, int64_t n_radius, int n_angles, const double *aw, const double *vals,
     const double *tab, const double *dtab, int64_t order, int normalized,
     const double *bar_scalars, const double *bar_greek, const double *bar_phase, double *sbar,
-    double *tbar) {
+    double *tbar, int n_out, int n_mu, const double *tab_u, const int *q_ptr, const int *q_list,
+    double *tbar_u) {
     extern __shared__ double sh[];
     double *S = sh;      // 8 sums
     double *F = sh + 8;  // 4 * n_angles, divided by Z
@@ -96,8 +100,37 @@ static __global__ __launch_bounds__(256) void ens_finish_vjp(
     }
     // Phase: ph = src / sca_n with sca_n = <C_sca> when normalized.
     const double sca_n = normalized ? sca : 1.0;
-    const double *pb = bar_phase ? bar_phase + e * 6 * (int64_t)n_angles : nullptr;
+    const double *pb = bar_phase && n_out == 0 ? bar_phase + e * 6 * (int64_t)n_angles : nullptr;
     double fbf = 0.0, ph_sca = 0.0;
+    if (n_out > 0 && bar_phase) {
+        // The caller's cosines: position q of the symmetric layout, its tables F_u = T_u / Z.
+        const double *pu = bar_phase + e * 6 * (int64_t)n_mu;
+        const double *tu = tab_u + (int64_t)blockIdx.x * 4 * n_out;
+        double *tbu = tbar_u + (int64_t)blockIdx.x * 4 * n_out;
+        for (int q = t; q < n_out; q += blockDim.x) {
+            const double f11 = tu[q] / z, f12 = tu[n_out + q] / z, f33 = tu[2 * n_out + q] / z,
+                         f34 = tu[3 * n_out + q] / z;
+            double b11 = 0.0, b12 = 0.0, b33 = 0.0, b34 = 0.0;
+            for (int k = q_ptr[q]; k < q_ptr[q + 1]; ++k) {
+                const int j = q_list[k];
+                const double p11 = pu[0 * n_mu + j] + pu[2 * n_mu + j];
+                const double p12 = pu[1 * n_mu + j];
+                const double p33 = pu[3 * n_mu + j] + pu[5 * n_mu + j];
+                const double p34 = pu[4 * n_mu + j];
+                b11 += p11 / sca_n;
+                b12 += p12 / sca_n;
+                b33 += p33 / sca_n;
+                b34 += p34 / sca_n;
+                if (normalized)
+                    ph_sca -= (p11 * f11 + p12 * f12 + p33 * f33 + p34 * f34) / (sca * sca);
+            }
+            fbf += b11 * f11 + b12 * f12 + b33 * f33 + b34 * f34;
+            tbu[q] = b11 / z;
+            tbu[n_out + q] = b12 / z;
+            tbu[2 * n_out + q] = b33 / z;
+            tbu[3 * n_out + q] = b34 / z;
+        }
+    }
     for (int j = t; j < n_angles; j += blockDim.x) {
         double b11 = 0.0, b12 = 0.0, b33 = 0.0, b34 = 0.0;
         if (greek) {
@@ -252,10 +285,13 @@ constexpr int kVjpOrderRows = 128;
 //     are written to the workgroup's slab, and two transposed products per 128 orders form
 //     G1^T P + G2^T M: columns 4q + c hold node q's (ubar.re, ubar.im, vbar.re, vbar.im)[c].
 //     A wavefront whose 16 orders lie beyond the group's largest N skips its products.
+// With accumulate set the cotangents and Tbar . F add to what abar and ht hold (a second pass
+// at the caller's phase cosines after the rule's).
 static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
     int64_t e0, int64_t ne, int64_t n_radius, int n_angles, int h, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double2 *table,
-    const double *tbar, unsigned long long *counter, double *slabs, cx *abar, double *ht) {
+    const double *tbar, unsigned long long *counter, double *slabs, cx *abar, double *ht,
+    int accumulate) {
     // The chunk's coefficients as the A operand: [tile][row 4 c + q][order], u, v by parity.
     __shared__ double lds_v[kVjpTiles][16][kVjpOrders + 1];
     __shared__ double lds_g[2][kVjpAngles][kVjpCols];
@@ -391,7 +427,8 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
         if (t < kVjpNodes && g0 + t < n_radius) {
             double s = 0.0;
             for (int w = 0; w < 8; ++w) s += lds_ht[w][t];
-            ht[eb * n_radius + g0 + t] = s;
+            double *o = ht + eb * n_radius + g0 + t;
+            *o = accumulate ? *o + s : s;
         }
         // 3: the transposed products, 128 orders (16 per wave) at a time.
         for (int nb = 0; nb < nmax; nb += kVjpOrderRows) {
@@ -441,7 +478,7 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                     const double val = d1[tt][v] + d2[tt][v];
                     const bool to_a = (comp < 2) == (bool)(n & 1);  // u is a_n at odd n
                     double *dst = reinterpret_cast<double *>(cb + (to_a ? 0 : nq) + n - 1);
-                    dst[comp & 1] = val;
+                    dst[comp & 1] = accumulate ? dst[comp & 1] + val : val;
                 }
             }
         }
@@ -662,10 +699,6 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     if (!d_kind || !d_p0 || !d_p1 || !d_wavelength || !d_m || !d_grad ||
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
-    if (d_bar_phase && !ctx->phase_mu.empty())
-        return fail(ctx, SCATRS_INVALID_INPUT,
-                    "phase cotangents are taken at the Gauss-Legendre cosines: clear the phase "
-                    "cosines (scatrs_set_phase_cosines with n_mu = 0) for the reverse pass");
     EnsSetup S;
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_bar_greek != nullptr, order, allow_unvalidated,
@@ -679,9 +712,16 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
     const int P = 3 + 2 * layers + (layers - 1);
     const int h = S.h;
     const bool tables = d_bar_greek || d_bar_phase;
+    // Phase cotangents at the caller's cosines (scatrs_set_phase_cosines): the forward tables
+    // and the reverse products there too, the latter adding to the rule's (Greek cotangents).
+    const bool user_mu = d_bar_phase && !ctx->phase_mu.empty();
+    const bool rule_pass = d_bar_greek || (d_bar_phase && !user_mu);
     // Staging and cotangents take 32 B per term each (and the fp32 copy 16 B with kernel 2).
     const int64_t max_terms = chunk_terms_limit(6, 2 * sizeof(cx));
     const unsigned vjp_blocks = (unsigned)std::max(1, ctx->cus * 2);
+    PhaseMu U;
+    if (user_mu && (rc = phase_mu_setup(ctx, S, U))) return rc;
+    const int n_out = user_mu ? 2 * U.h : 0;
     int64_t e0 = 0;
     while (e0 < count) {
         int64_t chunk_terms = 0;
@@ -710,11 +750,17 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         }
         if ((rc = forward_angular(ctx, S, e0, ne, n_radius, n_angles, B, s))) return rc;
         if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjEns,
-                                        (size_t)ne * (8 + 4 * n_angles) * sizeof(double)),
+                                        (size_t)ne * (8 + 4 * n_angles + 8 * (size_t)n_out) *
+                                            sizeof(double)),
                             "alloc")))
             return rc;
         double *sbar = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjEns]);
         double *tbar = sbar + ne * 8;
+        double *tab_u = tbar + ne * 4 * n_angles, *tbar_u = tab_u + ne * 4 * (size_t)n_out;
+        if (user_mu &&
+            (rc = angular_products(ctx, S, e0, ne, n_radius, U.h, n_out, U.table, U.tab16, B,
+                                   tab_u, s)))
+            return rc;
         const size_t shmem = (8 + 4 * (size_t)n_angles) * sizeof(double);
         if (d_bar_greek) {
             const dim3 grid((unsigned)((ne + 15) / 16), (unsigned)((n_angles + 63) / 64));
@@ -723,25 +769,36 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         }
         ens_finish_vjp<<<(unsigned)ne, 256, shmem, s>>>(
             e0, n_radius, (int)n_angles, S.d_atab, B.vals, B.tab_out, S.d_dtab, order, normalized,
-            d_bar_scalars, d_bar_greek, d_bar_phase, sbar, tbar);
+            d_bar_scalars, d_bar_greek, d_bar_phase, sbar, tbar, n_out, U.n_mu, tab_u, U.q_ptr,
+            U.q_list, tbar_u);
         cx *abar = nullptr;
         if (tables) {
             if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjStaging,
                                             (size_t)std::max<int64_t>(chunk_terms, 1) * 2 * sizeof(cx)),
                                 "alloc")))
                 return rc;
+            const int hs = std::max(h, U.h);
             if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kAdjSlabs,
-                                            (size_t)vjp_blocks * 2 * kVjpCols * h * sizeof(double)),
+                                            (size_t)vjp_blocks * 2 * kVjpCols * hs * sizeof(double)),
                                 "alloc")))
                 return rc;
             abar = static_cast<cx *>(ctx->buf[scatrs_ctx::kAdjStaging]);
             double *slabs = static_cast<double *>(ctx->buf[scatrs_ctx::kAdjSlabs]);
-            fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             mark_begin(ctx);
-            ens_angular_vjp<<<vjp_blocks, 512, 0, s>>>(e0, ne, n_radius, (int)n_angles, h,
-                                                       S.node_n, S.node_off, S.ens_base,
-                                                       B.staging, S.table, tbar, S.counter,
-                                                       slabs, abar, ht);
+            if (rule_pass) {
+                fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+                ens_angular_vjp<<<vjp_blocks, 512, 0, s>>>(e0, ne, n_radius, (int)n_angles, h,
+                                                           S.node_n, S.node_off, S.ens_base,
+                                                           B.staging, S.table, tbar, S.counter,
+                                                           slabs, abar, ht, 0);
+            }
+            if (user_mu) {
+                fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+                ens_angular_vjp<<<vjp_blocks, 512, 0, s>>>(e0, ne, n_radius, n_out, U.h,
+                                                           S.node_n, S.node_off, S.ens_base,
+                                                           B.staging, U.table, tbar_u, S.counter,
+                                                           slabs, abar, ht, rule_pass ? 1 : 0);
+            }
             mark_end(ctx);
         } else {
             fill_f64<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(ht, 0.0, nodes);

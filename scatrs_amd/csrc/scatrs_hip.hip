@@ -235,6 +235,13 @@ int scatrs_set_phase_cosines(scatrs_ctx *ctx, const double *mu, int64_t n_mu) {
         const int i = (int)(std::lower_bound(a.begin(), a.end(), std::fabs(mu[j])) - a.begin());
         ctx->phase_idx[j] = mu[j] >= 0.0 ? h + i : h - 1 - i;
     }
+    // The cosines at each place, in order (the reverse pass sums their cotangents there).
+    ctx->phase_qptr.assign(2 * h + 1, 0);
+    for (int64_t j = 0; j < n_mu; ++j) ++ctx->phase_qptr[ctx->phase_idx[j] + 1];
+    for (int q = 0; q < 2 * h; ++q) ctx->phase_qptr[q + 1] += ctx->phase_qptr[q];
+    ctx->phase_qlist.assign(n_mu, 0);
+    std::vector<int> fill(ctx->phase_qptr.begin(), ctx->phase_qptr.end() - 1);
+    for (int64_t j = 0; j < n_mu; ++j) ctx->phase_qlist[fill[ctx->phase_idx[j]]++] = (int)j;
     return SCATRS_OK;
 }
 

@@ -94,12 +94,13 @@ def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, p2
                               n_radius: int = 128,
                               n_angles: int = 128, order: int | None = 32,
                               normalized: bool = True, fixed_range: bool = False,
-                              allow_unvalidated: bool = False):
+                              allow_unvalidated: bool = False, mu=None):
     """Reverse mode of ensemble_fixed_device: with the output cotangents bar_scalars
     (count, 11), bar_greek (count, 6, order+1) and bar_phase (count, 6, n_angles) (each may be
     None), the gradient (count, P) with respect to parameter_names(L), and the total Mie
     terms of one pass. fixed_range holds the ln r range at its value (the reference's
-    derivative definition); otherwise the radius nodes move with the parameters."""
+    derivative definition); otherwise the radius nodes move with the parameters. With ``mu``
+    (host cosines) bar_phase is (count, 6, len(mu)) at those cosines."""
     import torch
 
     dev = p0.device
@@ -111,8 +112,10 @@ def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, p2
         raise InvalidInputError("bar_greek must have shape (count, 6, order + 1)")
     if bar_scalars is not None and bar_scalars.shape != (count, 11):
         raise InvalidInputError("bar_scalars must have shape (count, 11)")
-    if bar_phase is not None and bar_phase.shape != (count, 6, n_angles):
-        raise InvalidInputError("bar_phase must have shape (count, 6, n_angles)")
+    n_phase = n_angles if mu is None else int(np.size(mu))
+    if bar_phase is not None and bar_phase.shape != (count, 6, n_phase):
+        raise InvalidInputError("bar_phase must have shape (count, 6, n_angles), or "
+                                "(count, 6, len(mu)) with mu")
     c = lambda t: None if t is None else t.to(torch.float64).contiguous()  # noqa: E731
     args = [kind.to(torch.int32).contiguous(), p0.contiguous(), p1.contiguous(),
             wavelength.contiguous(), m.contiguous()]
@@ -124,11 +127,17 @@ def ensemble_fixed_vjp_device(kind, p0, p1, wavelength, m, fractions=None, *, p2
     ctx = _native.context(dev.index)
     terms = C.c_int64(0)
     p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
-    rc = ctx.lib.scatrs_ensemble_fixed_vjp(
-        ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(q2), p(args[3]), layers, p(fr),
-        p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
-        int(bool(normalized)), int(bool(fixed_range)), int(bool(allow_unvalidated)), p(bs),
-        p(bg), p(bp), p(grad), C.byref(terms))
+    if mu is not None:
+        ctx.set_phase_cosines(mu)
+    try:
+        rc = ctx.lib.scatrs_ensemble_fixed_vjp(
+            ctx.handle, count, p(args[0]), p(args[1]), p(args[2]), p(q2), p(args[3]), layers,
+            p(fr), p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
+            int(bool(normalized)), int(bool(fixed_range)), int(bool(allow_unvalidated)), p(bs),
+            p(bg), p(bp), p(grad), C.byref(terms))
+    finally:
+        if mu is not None:
+            ctx.set_phase_cosines(None)
     ctx.check(rc)
     return grad, terms.value
 
@@ -149,7 +158,7 @@ def _autograd_function():
             sc, gr, ph, _ = ensemble_fixed_device(
                 kind, p0, p1, wavelength, m, fractions, p2=p2, n_radius=opts["n_radius"],
                 n_angles=opts["n_angles"], order=opts["order"], normalized=opts["normalized"],
-                allow_unvalidated=opts["allow_unvalidated"], want_phase=True)
+                allow_unvalidated=opts["allow_unvalidated"], want_phase=True, mu=opts["mu"])
             empty = torch.empty(0)
             ctx.save_for_backward(p0, p1, p2 if p2 is not None else empty, m_real, m_imag,
                                   fractions if fractions is not None else empty, kind, wavelength)
@@ -170,7 +179,8 @@ def _autograd_function():
                 kind, p0, p1, wl, m, fr if ctx.has_fr else None, p2=p2 if ctx.has_p2 else None,
                 bar_scalars=g_sc, bar_greek=g_gr, bar_phase=g_ph, n_radius=o["n_radius"],
                 n_angles=o["n_angles"], order=o["order"], normalized=o["normalized"],
-                fixed_range=o["fixed_range"], allow_unvalidated=o["allow_unvalidated"])
+                fixed_range=o["fixed_range"], allow_unvalidated=o["allow_unvalidated"],
+                mu=o["mu"])
             gm_re = grad[:, 3:3 + 2 * layers:2].reshape(m_real.shape)
             gm_im = grad[:, 4:4 + 2 * layers:2].reshape(m_imag.shape)
             gfr = grad[:, 3 + 2 * layers:] if ctx.has_fr else None
@@ -186,18 +196,20 @@ _FUNCTION = None
 def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *, p2=None,
                    n_radius: int = 128, n_angles: int = 128, order: int = 32,
                    normalized: bool = True, fixed_range: bool = False,
-                   allow_unvalidated: bool = False):
+                   allow_unvalidated: bool = False, mu=None):
     """Differentiable batch of ensembles on the fixed path (torch.autograd): p0, p1,
     wavelength (count,) and p2 (count,) or None; m_real, m_imag (count,) for spheres or
     (count, L) for layered spheres (core first); fractions (count, L-1). Returns (scalars
-    (count, 11), greek (count, 6, order+1), phase (count, 6, n_angles)); gradients flow to p0,
-    p1, p2, m_real, m_imag and fractions through the library's reverse-mode kernel."""
+    (count, 11), greek (count, 6, order+1), phase (count, 6, n_angles), or at the host cosines
+    ``mu`` (count, 6, len(mu))); gradients flow to p0, p1, p2, m_real, m_imag and fractions
+    through the library's reverse-mode kernel."""
     global _FUNCTION
     if _FUNCTION is None:
         _FUNCTION = _autograd_function()
     opts = dict(n_radius=int(n_radius), n_angles=int(n_angles), order=int(order),
                 normalized=bool(normalized), fixed_range=bool(fixed_range),
-                allow_unvalidated=bool(allow_unvalidated))
+                allow_unvalidated=bool(allow_unvalidated),
+                mu=None if mu is None else np.array(mu, np.float64).reshape(-1))
     return _FUNCTION.apply(p0, p1, p2, m_real, m_imag, fractions, kind, wavelength, opts)
 
 

@@ -288,8 +288,7 @@ def test_phase_at_the_callers_cosines(kernel):
     """scatrs_set_phase_cosines: the phase matrix at arbitrary cosines (unsorted, repeated, 0 and
     +-1) with each angular kernel against the oracle (1e-10, the fp16-split kernel 1e-6 of max
     F11); at the rule's own cosines the rule's phase matrix; the scalars and Greek
-    coefficients unchanged; through Ensemble and Mixture; the reverse pass refuses phase
-    cotangents while cosines are set."""
+    coefficients unchanged; through Ensemble and Mixture."""
     from scatrs_amd import _native
     from scatrs_amd.ensemble import Ensemble, Mixture, ensemble_fixed_device
     from scatrs_amd.particles import Sphere
@@ -329,15 +328,6 @@ def test_phase_at_the_callers_cosines(kernel):
                    (2.0, Ensemble(Sphere(m[2]), PowerLaw(3.0, 0.05, 1.5)))))
     rm = mix.compute(wl[:1], greek=24, mu=mu, n_radius=64, n_angles=96)
     assert rm.phase.f33.shape == (1, mu.size) and np.all(np.isfinite(rm.phase.f11))
-    ctx.set_phase_cosines(mu)
-    try:
-        from scatrs_amd.ensemble import ensemble_fixed_vjp_device
-
-        with pytest.raises(InvalidInputError, match="phase cotangents"):
-            ensemble_fixed_vjp_device(*args, p2=_dev(p2), bar_phase=torch.ones_like(f0),
-                                      **kw)
-    finally:
-        ctx.set_phase_cosines(None)
     with pytest.raises(InvalidInputError, match=r"mu must be in \[-1, 1\]"):
         ctx.set_phase_cosines([0.5, 1.5])
 
