@@ -38,6 +38,7 @@ struct scatrs_ctx {
         kPhaseMu,     // phase at the caller's cosines: |mu| ascending, then each cosine's index
         kTablesMu,    // ... the w pi_n, w tau_n table at |mu| (and its fp16 slices)
         kAccumMu,     // ... the summed tables of a chunk, and each ensemble's phase factor
+        kGeom,        // ensembles: every node's radius and quadrature weight (ens_prep)
         kNumBufs
     };
     void *buf[kNumBufs] = {};

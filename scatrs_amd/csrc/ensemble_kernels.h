@@ -50,6 +50,7 @@ struct EnsArgs {
     int64_t n_radius;
     const double *rnodes, *rweights;  // GL rule on [-1, 1]
     const double *p2;                 // third parameter (may be null: 0)
+    double2 *geom;                    // per node (r, W), written by ens_prep
 };
 
 // Ensemble e's distribution parameters.
@@ -76,6 +77,17 @@ __device__ __forceinline__ NodeGeom node_geometry(const EnsArgs &a, int64_t e, i
     g.r = r;
     g.x = 2.0 * kPi * r / a.wl[e];
     g.W = node_weight(kind, a.rweights[i], half, u, r);
+    return g;
+}
+
+// Node i of ensemble e as ens_prep stored it: the solve kernels read it rather than evaluate
+// the distributions, whose code would otherwise add to their register pressure.
+__device__ __forceinline__ NodeGeom node_stored(const EnsArgs &a, int64_t e, int64_t i) {
+    const double2 rw = a.geom[e * a.n_radius + i];
+    NodeGeom g;
+    g.r = rw.x;
+    g.x = 2.0 * kPi * rw.x / a.wl[e];
+    g.W = rw.y;
     return g;
 }
 
@@ -137,6 +149,7 @@ static __global__ void __launch_bounds__(256) ens_prep(EnsArgs a, int64_t count,
         }
         if (code == kValid) {
             const NodeGeom geo = node_geometry(a, e, i);
+            a.geom[g] = make_double2(geo.r, geo.W);
             long long nn = 0;
             const int c = a.layers == 1
                               ? sphere_code(geo.x, a.m[2 * e], a.m[2 * e + 1], 0, allow, &nn)
@@ -274,7 +287,7 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
                                              double *tabc, int *xexp, cx *back_out) {
     const int64_t e = e0 + g / a.n_radius, i = g % a.n_radius;
     const int64_t gi = e * a.n_radius + i;
-    const NodeGeom geo = node_geometry(a, e, i);
+    const NodeGeom geo = node_stored(a, e, i);
     const int n = node_n[gi];
     const int64_t off = ens_base[e] + node_off[gi];
     StagingSink sink;
@@ -1292,6 +1305,10 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                         "alloc")))
         return rc;
     Partial *partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kHist]);
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kGeom, (size_t)total * sizeof(double2)),
+                        "alloc")))
+        return rc;
+    S.a.geom = static_cast<double2 *>(ctx->buf[scatrs_ctx::kGeom]);
     ens_prep<<<(unsigned)pblocks, 256, 0, s>>>(S.a, count, allow_unvalidated, S.node_n, partial);
     reduce_status<<<1, 1024, 0, s>>>(partial, pblocks, st);
     ens_offsets<<<(unsigned)count, 256, 0, s>>>(S.node_n, n_radius, S.node_off, S.ens_terms);

@@ -503,7 +503,7 @@ __device__ __forceinline__ void vjp_node(const EnsArgs &a, int64_t e0, int64_t g
     const int64_t eb = g / a.n_radius, i = g % a.n_radius;
     const int64_t e = e0 + eb;
     const int64_t gi = e * a.n_radius + i;
-    const NodeGeom geo = node_geometry(a, e, i);
+    const NodeGeom geo = node_stored(a, e, i);
     const int n = node_n[gi];
     const int64_t off = ens_base[e] + node_off[gi];
     const double lam = a.wl[e], l2 = lam * lam;
@@ -602,7 +602,7 @@ static __global__ __launch_bounds__(256) void ens_node_grads(
     const int P = 3 + 2 * L + (L - 1);
     const int64_t eb = g / a.n_radius, i = g % a.n_radius;
     const int64_t e = e0 + eb;
-    const NodeGeom geo = node_geometry(a, e, i);
+    const NodeGeom geo = node_stored(a, e, i);
     const double lam = a.wl[e], l2 = lam * lam;
     const double *sb = sbar + eb * 8;
     const double *rw = raw + g * (6 * L);
