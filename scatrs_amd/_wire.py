@@ -1,0 +1,528 @@
+"""The reference's ``_scatrs`` ensemble entry points over the fixed discrete path.
+
+``ensemble_parameters``, ``ensemble_compute`` and ``ensemble_compute_batch`` take what the
+reference's bindings take (scatrs-py/src/ensemble.rs:172-504, python/scatrs/_scatrs/
+__init__.pyi:141-168): distributions and particles as tagged tuples (ensemble.rs:31-135),
+components as (distribution spec, particle spec, weight), the options in the same order. They
+return the same dict of arrays, one row per wavelength (``ensemble_compute_batch``: per
+(distribution, wavelength), distribution-major):
+
+* ``values``, ``errors`` (11, n) in SCALARS order; ``d``, ``d_errors`` (P, 11, n);
+* ``greek``, ``greek_errors`` (6, n, L+1) and ``greek_d``, ``greek_d_errors`` (P, 6, n, L+1)
+  with a Greek order;
+* ``phase``, ``phase_errors`` (6, n, n_mu) and ``phase_d``, ``phase_d_errors`` (P, 6, n, n_mu)
+  with cosines mu (F11, F12, F22, F33, F34, F44);
+* ``converged`` bool (n) and ``panels`` uint64 (n).
+
+On the fixed path each part (a distribution, a multimodal law's mode, a tabulated law's
+segment) is one ``n_radius``-point Gauss-Legendre panel in ln r. What stands in for the
+reference's adaptive integration (engine.rs:307-943):
+
+* the error estimate of every output is its difference from the same computation on
+  ``n_radius // 2`` nodes (the estimate of the coarser rule, as the reference's Kronrod minus
+  Gauss difference is);
+* ``converged`` applies the reference's refinement targets (engine.rs:620-690) to those
+  differences part by part: C_ext, C_sca and C_abs (against C_ext's rtol), C_sca g against
+  C_sca's, and with ``refine_backscatter`` C_back against the larger of C_back and C_sca;
+* ``panels`` is the number of parts (one panel each); ``max_panels`` is checked, not used.
+
+Derivatives are the reference's forward-mode columns (the parameters named by
+``ensemble_parameters``, each result differentiated in its own wavelength's parameters), here
+from one reverse pass: every output component of a wavelength gets a replica of the whole
+computation with a one-hot cotangent, so one call of the library's reverse-mode kernel returns
+every (output, parameter) pair. The size range is held fixed, as the reference differentiates
+(``fixed_range``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .errors import InvalidInputError
+from .size import (SEGMENT, Gamma, LogNormal, ModifiedGamma, Monodisperse, MultiModal,
+                   PowerLaw, Tabulated)
+
+__all__ = ["MAX_GREEK_ORDER", "MAX_MIE_VALUES", "ensemble_compute", "ensemble_compute_batch",
+           "ensemble_parameters"]
+
+MAX_MIE_VALUES = 1 << 27  # scatrs-py/src/mie.rs:30
+MAX_GREEK_ORDER = 2 * (1 << 20)  # scatrs/src/mie/greek.rs:344, mie/mod.rs:101
+_N = 11  # scalar outputs
+
+
+def _distribution(spec):
+    """A distribution from its tagged tuple (scatrs-py/src/ensemble.rs:31-77)."""
+    if not isinstance(spec, tuple) or not spec:
+        raise TypeError("a distribution spec must be a tuple")
+    kind = spec[0]
+    f = lambda i: float(spec[i])  # noqa: E731
+    if kind == "lognormal":
+        return LogNormal(f(1), f(2))
+    if kind == "gamma":
+        return Gamma(f(1), f(2))
+    if kind == "modified_gamma":
+        return ModifiedGamma(f(1), f(2), f(3))
+    if kind == "power_law":
+        return PowerLaw(f(1), f(2), f(3))
+    if kind == "monodisperse":
+        return Monodisperse(f(1))
+    if kind == "tabulated":
+        return Tabulated(tuple(np.asarray(spec[1], np.float64).reshape(-1)),
+                         tuple(np.asarray(spec[2], np.float64).reshape(-1)))
+    if kind == "multimodal":
+        return MultiModal(tuple((float(w), _distribution(s)) for w, s in spec[1]))
+    raise TypeError(f"unknown distribution kind {kind!r}")
+
+
+def _particle(spec):
+    """(m (L, n) complex, radius fractions (L-1,)) from a particle's tagged tuple
+    (scatrs-py/src/ensemble.rs:79-135)."""
+    if not isinstance(spec, tuple) or not spec:
+        raise TypeError("a particle spec must be a tuple")
+    kind = spec[0]
+    if kind == "sphere":
+        if not isinstance(spec[1], np.ndarray):
+            raise TypeError("a sphere's refractive indices must be an array")
+        return np.asarray(spec[1], np.complex128).reshape(1, -1), np.zeros(0)
+    if kind == "layered":
+        if not isinstance(spec[1], np.ndarray):
+            raise TypeError("a layered sphere's radius fractions must be an array")
+        if not isinstance(spec[2], np.ndarray):
+            raise TypeError("a layered sphere's refractive indices must be an array")
+        m = np.asarray(spec[2], np.complex128)
+        if m.ndim != 2:
+            raise TypeError("a layered sphere's refractive indices must have shape (layers, "
+                            f"wavelengths), got {m.ndim} axes")
+        fr = np.asarray(spec[1], np.float64).reshape(-1)
+        if fr.size != m.shape[0] - 1:
+            raise InvalidInputError(f"a layered sphere of {m.shape[0]} layers needs "
+                                    f"{m.shape[0] - 1} radius fractions, got {fr.size}")
+        return m, fr
+    if kind == "spheroid":
+        raise InvalidInputError("spheroids are not on the ensemble path of this library "
+                                "(spheres and layered spheres)")
+    raise TypeError(f"unknown particle kind {kind!r}")
+
+
+class _Component:
+    """One component: its distribution, m (L, n), radius fractions and weight."""
+
+    def __init__(self, dist, m, fr, weight):
+        self.dist, self.m, self.fr, self.weight = dist, m, fr, float(weight)
+
+    @property
+    def layers(self) -> int:
+        return self.m.shape[0]
+
+
+def _components(items):
+    return [_Component(_distribution(s), *_particle(p), w) for s, p, w in items]
+
+
+def _validate(comps, n: int) -> None:
+    """The reference's checks of the components (scatrs/src/ensemble/mod.rs:745-780): one
+    refractive index per wavelength, finite weights >= 0 with a positive sum."""
+    total = 0.0
+    for i, c in enumerate(comps):
+        if c.m.shape[1] != n:
+            raise InvalidInputError(
+                f"component {i} has {c.m.shape[1]} refractive indices for {n} wavelengths")
+        if not (np.isfinite(c.weight) and c.weight >= 0.0):
+            raise InvalidInputError(
+                f"the weight of component {i} must be finite and >= 0, got {c.weight}")
+        total += c.weight
+    if not (np.isfinite(total) and total > 0.0):
+        raise InvalidInputError(
+            f"the mixture weights must have a finite, positive sum, got {total}")
+
+
+def _component_parameters(dist, layers: int, weight: bool):
+    """(name, (mode, what, index)) of a component's parameters, in the reference's order
+    (scatrs/src/ensemble/mod.rs:252-312): the distribution's (each mode's, then the mode's
+    weight), the particle's (Re m, Im m of each layer, then the radius fractions), then the
+    component's weight in a mixture."""
+    out = []
+    if isinstance(dist, MultiModal):
+        for j, (_, d) in enumerate(dist.modes):
+            out += [(f"mode{j}.{n}", (j, "dist", s)) for s, n in enumerate(d.names)]
+            out.append((f"mode{j}.weight", (j, "weight", 0)))
+    else:
+        out += [(n, (None, "dist", s)) for s, n in enumerate(dist.names)]
+    if layers == 1:
+        out += [("m_real", (None, "m_real", 0)), ("m_imag", (None, "m_imag", 0))]
+    else:
+        for l in range(layers):
+            out += [(f"layer{l}.m_real", (None, "m_real", l)),
+                    (f"layer{l}.m_imag", (None, "m_imag", l))]
+        out += [(f"layer{l}.radius_fraction", (None, "fraction", l)) for l in range(layers - 1)]
+    if weight:
+        out.append(("weight", (None, "weight", 0)))
+    return out
+
+
+def _parameters(comps, mixture: bool):
+    """The names and keys (component, mode, what, index) of every parameter
+    (scatrs-py/src/ensemble.rs:154-170; names of scatrs/src/ensemble/mod.rs:225-249)."""
+    if not mixture and len(comps) != 1:
+        raise InvalidInputError(
+            f"a single distribution needs exactly one component, got {len(comps)}")
+    names, keys = [], []
+    for i, c in enumerate(comps):
+        for n, k in _component_parameters(c.dist, c.layers, mixture):
+            names.append(f"c{i}.{n}" if mixture else n)
+            keys.append((i,) + k)
+    return names, keys
+
+
+def ensemble_parameters(items, mixture: bool) -> list[str]:
+    """The names of the parameters of the components (a single distribution unless
+    ``mixture``), in the order ``ensemble_compute`` indexes them
+    (scatrs-py/src/ensemble.rs:172-187)."""
+    comps = _components(items)
+    if not comps:
+        return []
+    return _parameters(comps, mixture)[0]
+
+
+def _order(greek, all_orders: bool):
+    """The Greek order, or None (scatrs-py/src/ensemble.rs:371-381)."""
+    if all_orders:
+        raise InvalidInputError(
+            "all_orders needs the adaptive engine's band layout; the fixed path takes a Greek "
+            "order (greek=L)")
+    if greek is None:
+        return None
+    if greek < 0 or greek > MAX_GREEK_ORDER:
+        raise InvalidInputError(f"the Greek order must be at most {MAX_GREEK_ORDER}, got {greek}")
+    return int(greek)
+
+
+def _check_options(rtol, max_panels, n, n_mu, order, p):
+    """The reference's option checks (scatrs/src/ensemble/mod.rs:689-703) and its bound on
+    the outputs (scatrs-py/src/ensemble.rs:382-393)."""
+    if not (1e-13 <= rtol <= 0.1):
+        raise InvalidInputError(f"rtol must be in [1e-13, 0.1], got {rtol}")
+    if max_panels < 1:
+        raise InvalidInputError("max_panels must be at least 1")
+    per = _N + 4 * n_mu + (0 if order is None else 6 * (order + 1))
+    if n * per * 2 * (1 + p) > 2 * MAX_MIE_VALUES:
+        raise InvalidInputError("the ensemble outputs would exceed 2 * MAX_MIE_VALUES = "
+                                f"{2 * MAX_MIE_VALUES} real values")
+
+
+class _Leaves:
+    """The parameters of the units' components as tensors with one row per (replica,
+    wavelength): dist[(u, c, mode)] (B, 3), m_real/m_imag[(u, c)] (B, L), fraction[(u, c)]
+    (B, L-1), weight[(u, c)] (B,), mode_weight[(u, c)] (B, modes)."""
+
+    def __init__(self, units, n: int, R: int, dev, grad: bool):
+        import torch
+
+        B = R * n
+        col = np.tile(np.arange(n), R)
+        t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64,  # noqa: E731
+                                   device=dev).requires_grad_(grad)
+        self.dist, self.m_real, self.m_imag, self.fraction = {}, {}, {}, {}
+        self.weight, self.mode_weight = {}, {}
+        for u, comps in enumerate(units):
+            for c, comp in enumerate(comps):
+                mb = comp.m[:, col].T  # (B, L)
+                self.m_real[u, c] = t(mb.real)
+                self.m_imag[u, c] = t(mb.imag)
+                self.fraction[u, c] = t(np.tile(comp.fr, (B, 1)))
+                self.weight[u, c] = t(np.full(B, comp.weight))
+                d = comp.dist
+                if isinstance(d, MultiModal):
+                    self.mode_weight[u, c] = t(np.tile([w for w, _ in d.modes], (B, 1)))
+                    for j, (_, dj) in enumerate(d.modes):
+                        if not isinstance(dj, Tabulated):
+                            self.dist[u, c, j] = t(np.tile(np.array(dj.params(), np.float64),
+                                                           (B, 1)))
+                elif not isinstance(d, Tabulated):
+                    self.dist[u, c, None] = t(np.tile(np.array(d.params(), np.float64), (B, 1)))
+
+    def tensors(self):
+        return [*self.dist.values(), *self.m_real.values(), *self.m_imag.values(),
+                *self.fraction.values(), *self.weight.values(), *self.mode_weight.values()]
+
+    def column(self, u: int, key, grads: dict):
+        """The gradient column (B,) of the parameter ``key`` = (component, mode, what, index) of
+        unit u, from ``grads`` (id(tensor) -> its gradient or None)."""
+        import torch
+
+        c, j, what, i = key
+        leaf = {"dist": lambda: self.dist[u, c, j],
+                "m_real": lambda: self.m_real[u, c], "m_imag": lambda: self.m_imag[u, c],
+                "fraction": lambda: self.fraction[u, c],
+                "weight": lambda: (self.weight[u, c] if j is None
+                                   else self.mode_weight[u, c])}[what]()
+        g = grads.get(id(leaf))
+        if g is None:
+            return torch.zeros(leaf.shape[0], dtype=torch.float64, device=leaf.device)
+        if what == "weight" and j is None:
+            return g
+        return g[:, j if what == "weight" else i]
+
+
+def _parts(units, leaves: _Leaves):
+    """Every part (unit, kind, p (B, 3), m_real, m_imag, fractions, number fraction (B,),
+    layers) of the units: a component's distribution, a multimodal law's modes
+    (engine.rs:143-160) or a tabulated law's segments, with the number fractions of the
+    reference's mixing (engine.rs:2131-2133): the component's weight over the unit's total
+    times the mode's weight over its component's total, times a segment's share of its
+    tabulated law's number."""
+    import torch
+
+    out = []
+    for u, comps in enumerate(units):
+        total = sum(leaves.weight[u, c] for c in range(len(comps)))
+        for c, comp in enumerate(comps):
+            fc = leaves.weight[u, c] / total
+            m_re, m_im, fr = leaves.m_real[u, c], leaves.m_imag[u, c], leaves.fraction[u, c]
+            modes = [(None, comp.dist, fc)]
+            if isinstance(comp.dist, MultiModal):
+                v = leaves.mode_weight[u, c]
+                share = v / v.sum(1, keepdim=True)
+                modes = [(j, d, fc * share[:, j]) for j, (_, d) in enumerate(comp.dist.modes)]
+            for j, d, f in modes:
+                if isinstance(d, Tabulated):
+                    segs, w = d.segments()
+                    w = w / w.sum()
+                    for (r0, r1, tilt), ws in zip(segs, w):
+                        p = torch.tensor([r0, r1, tilt], dtype=torch.float64,
+                                         device=f.device).expand(f.shape[0], 3)
+                        out.append((u, SEGMENT, p, m_re, m_im, fr, f * ws, comp.layers))
+                else:
+                    out.append((u, d.kind, leaves.dist[u, c, j], m_re, m_im, fr, f,
+                                comp.layers))
+    return out
+
+
+def _evaluate(units, wl: np.ndarray, *, R: int, grad: bool, order, mu, normalized: bool,
+              n_radius: int, n_angles: int, allow_unvalidated: bool):
+    """The outputs of every unit at R replicas of the wavelengths on the fixed path: (leaves,
+    scalars (U, B, 11), greek (U, B, 6, L+1) or None, phase (U, B, 6, n_mu) or None, the parts'
+    scalars (parts, B, 11), each part's unit)."""
+    import torch
+
+    from .ensemble import assemble, ensemble_fixed, number_averages
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n = wl.size
+    B = R * n
+    leaves = _Leaves(units, n, R, dev, grad)
+    parts = _parts(units, leaves)
+    wl_rows = torch.tensor(np.tile(wl, R), dtype=torch.float64, device=dev)
+    phase = mu is not None
+    by_layers: dict[int, list[int]] = {}
+    for i, part in enumerate(parts):
+        by_layers.setdefault(part[7], []).append(i)
+    sc_of, gr_of, ph_of = [None] * len(parts), [None] * len(parts), [None] * len(parts)
+    for L, idx in by_layers.items():
+        k = len(idx)
+        kind = torch.tensor(np.repeat([parts[i][1] for i in idx], B), dtype=torch.int32,
+                            device=dev)
+        p = torch.cat([parts[i][2] for i in idx])
+        m_re = torch.cat([parts[i][3] for i in idx])
+        m_im = torch.cat([parts[i][4] for i in idx])
+        fr = torch.cat([parts[i][5] for i in idx]) if L > 1 else None
+        if L == 1:
+            m_re, m_im = m_re[:, 0], m_im[:, 0]
+        sc, gr, ph = ensemble_fixed(
+            kind, p[:, 0], p[:, 1], wl_rows.repeat(k), m_re, m_im, fr, p2=p[:, 2],
+            n_radius=n_radius, n_angles=n_angles, order=order, normalized=False,
+            fixed_range=True, allow_unvalidated=allow_unvalidated, mu=mu, phase=phase)
+        for q, i in enumerate(idx):
+            sc_of[i] = sc[q * B:(q + 1) * B]
+            gr_of[i] = None if gr is None else gr[q * B:(q + 1) * B]
+            ph_of[i] = None if ph is None else ph[q * B:(q + 1) * B]
+    U = len(units)
+    unit = torch.tensor([part[0] for part in parts], device=dev)
+    f = torch.stack([part[6] for part in parts])  # (parts, B)
+    sc_parts = torch.stack(sc_of)
+    avg = torch.zeros((U, B, 7), dtype=torch.float64, device=dev).index_add(
+        0, unit, f[..., None] * number_averages(sc_parts))
+    gr = None
+    if order is not None:
+        g = torch.stack(gr_of)
+        gr = torch.zeros((U,) + g.shape[1:], dtype=torch.float64, device=dev).index_add(
+            0, unit, f[..., None, None] * g)
+    ph = None
+    if phase:
+        q = torch.stack(ph_of)
+        ph = torch.zeros((U,) + q.shape[1:], dtype=torch.float64, device=dev).index_add(
+            0, unit, f[..., None, None] * q)
+    sc, gr, ph = assemble(avg, gr, ph, normalized)
+    return leaves, sc, gr, ph, sc_parts.detach(), [part[0] for part in parts]
+
+
+def _flat(sc, gr, ph):
+    """(rows, K): the scalars, then the Greek coefficients, then the phase matrix."""
+    import torch
+
+    cols = [sc]
+    if gr is not None:
+        cols.append(gr.reshape(gr.shape[0], -1))
+    if ph is not None:
+        cols.append(ph.reshape(ph.shape[0], -1))
+    return torch.cat(cols, 1)
+
+
+def _converged(sc_parts, sc_half, part_unit, U: int, rtol: float, refine_backscatter: bool):
+    """(U, rows) bool: every part of a unit meets the reference's refinement targets
+    (engine.rs:620-690) with the difference of the two rules as its error."""
+    import torch
+
+    ext, sca, absb, back = (sc_parts[..., k] for k in range(4))
+    scag = sc_parts[..., 5] * sca
+    d = (sc_parts - sc_half).abs()
+    d_scag = (scag - sc_half[..., 5] * sc_half[..., 1]).abs()
+    ok = (d[..., 0] <= rtol * ext.abs()) & (d[..., 1] <= rtol * sca.abs())
+    ok &= (d[..., 2] <= rtol * ext.abs()) & (d_scag <= rtol * sca.abs())
+    if refine_backscatter:
+        ok &= d[..., 3] <= rtol * torch.maximum(back.abs(), sca.abs())
+    out = torch.ones((U, sc_parts.shape[1]), dtype=torch.bool, device=sc_parts.device)
+    for p, u in enumerate(part_unit):
+        out[u] &= ok[p]
+    return out
+
+
+def _run(units, wl, *, order, mu, normalized, keys, rtol, refine_backscatter, n_radius, n_angles,
+         allow_unvalidated):
+    """Values, derivatives (when ``keys``, for unit 0) and their two-rule errors; converged and
+    panels."""
+    import torch
+
+    n = wl.size
+    W = 0 if order is None else order + 1
+    n_mu = 0 if mu is None else mu.size
+    K = _N + 6 * W + 6 * n_mu
+    R = K if keys else 1
+
+    def one(nr):
+        with torch.enable_grad() if keys else torch.no_grad():
+            leaves, sc, gr, ph, sc_parts, part_unit = _evaluate(
+                units, wl, R=R, grad=bool(keys), order=order, mu=mu, normalized=normalized,
+                n_radius=nr, n_angles=n_angles, allow_unvalidated=allow_unvalidated)
+            Y = [_flat(sc[u], None if gr is None else gr[u], None if ph is None else ph[u])
+                 for u in range(len(units))]  # (B, K) per unit
+            D = None
+            if keys:
+                ts = leaves.tensors()
+                cot = torch.zeros((R, n, K), dtype=torch.float64, device=Y[0].device)
+                cot[torch.arange(K), :, torch.arange(K)] = 1.0
+                gs = torch.autograd.grad(Y[0], ts, cot.reshape(R * n, K), allow_unused=True)
+                grads = {id(t): g for t, g in zip(ts, gs)}
+                D = torch.stack([leaves.column(0, k, grads).reshape(R, n) for k in keys])
+        V = torch.stack([y.detach()[:n] for y in Y])  # (U, n, K), replica 0
+        return V, D, sc_parts[:, :n], part_unit
+
+    V, D, sc_parts, part_unit = one(n_radius)
+    Vh, Dh, sc_half, _ = one(max(1, n_radius // 2))
+    conv = _converged(sc_parts, sc_half, part_unit, len(units), rtol, refine_backscatter)
+    if n_radius < 2:
+        conv[:] = False
+    counts = np.bincount(np.asarray(part_unit), minlength=len(units))
+    return V, (V - Vh).abs(), D, None if D is None else (D - Dh).abs(), conv, counts
+
+
+def _dict(V, E, D, DE, conv, counts, n_per_unit: int, W: int, n_mu: int):
+    """The reference's output dict (scatrs-py/src/ensemble.rs:456-478) from values and
+    errors (U, n, K), derivatives and errors (P, K, n) or None."""
+    U = V.shape[0]
+    n = U * n_per_unit
+    P = 0 if D is None else D.shape[0]
+    v, e = (a.reshape(n, -1).cpu().numpy() for a in (V, E))
+    if D is None:
+        d = de = np.zeros((0, v.shape[1], n))
+    else:
+        d, de = D.cpu().numpy(), DE.cpu().numpy()
+    out = {"values": np.ascontiguousarray(v[:, :_N].T), "errors": np.ascontiguousarray(e[:, :_N].T),
+           "d": np.ascontiguousarray(d[:, :_N]), "d_errors": np.ascontiguousarray(de[:, :_N])}
+    if W:
+        s = slice(_N, _N + 6 * W)
+        out["greek"], out["greek_errors"] = (
+            np.ascontiguousarray(a[:, s].reshape(n, 6, W).transpose(1, 0, 2)) for a in (v, e))
+        out["greek_d"], out["greek_d_errors"] = (
+            np.ascontiguousarray(a[:, s].reshape(P, 6, W, n).transpose(0, 1, 3, 2))
+            for a in (d, de))
+    if n_mu:
+        s = slice(_N + 6 * W, _N + 6 * W + 6 * n_mu)
+        out["phase"], out["phase_errors"] = (
+            np.ascontiguousarray(a[:, s].reshape(n, 6, n_mu).transpose(1, 0, 2)) for a in (v, e))
+        out["phase_d"], out["phase_d_errors"] = (
+            np.ascontiguousarray(a[:, s].reshape(P, 6, n_mu, n).transpose(0, 1, 3, 2))
+            for a in (d, de))
+    out["converged"] = conv.reshape(-1).cpu().numpy().astype(bool)
+    out["panels"] = np.repeat(counts.astype(np.uint64), n_per_unit)
+    return out
+
+
+def _mu(mu):
+    mu = np.ascontiguousarray(np.asarray(mu, np.float64).reshape(-1))
+    return mu if mu.size else None
+
+
+def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool, mu,
+                     normalized: bool, derivatives, rtol: float, max_panels: int,
+                     allow_unvalidated: bool, refine_backscatter: bool, *,
+                     n_radius: int = 128, n_angles: int = 128) -> dict:
+    """The ensemble outputs of the components at each wavelength as the reference's
+    ``_scatrs.ensemble_compute`` returns them (scatrs-py/src/ensemble.rs:290-355), on the
+    fixed path (``n_radius`` radius nodes per part, the Greek coefficients projected on
+    ``n_angles`` Gauss-Legendre cosines); ``derivatives`` are indices into
+    ``ensemble_parameters(items, mixture)``."""
+    comps = _components(items)
+    wl = np.ascontiguousarray(np.asarray(wavelengths, np.float64).reshape(-1))
+    mu = _mu(mu)
+    if not comps:
+        raise InvalidInputError("a mixture needs a component")
+    names, keys = _parameters(comps, mixture)
+    chosen = []
+    for i in derivatives:
+        if not (0 <= i < len(keys)):
+            raise InvalidInputError(f"no parameter {i}")
+        if keys[i] in chosen:
+            raise InvalidInputError(f"parameter {names[i]} requested twice")
+        chosen.append(keys[i])
+    _validate(comps, wl.size)
+    order = _order(greek, all_orders)
+    n_mu = 0 if mu is None else mu.size
+    _check_options(rtol, max_panels, wl.size, n_mu, order, len(chosen))
+    V, E, D, DE, conv, counts = _run(
+        [comps], wl, order=order, mu=mu, normalized=normalized, keys=chosen, rtol=rtol,
+        refine_backscatter=refine_backscatter, n_radius=n_radius, n_angles=n_angles,
+        allow_unvalidated=allow_unvalidated)
+    W = 0 if order is None else order + 1
+    return _dict(V, E, D, DE, conv, counts, wl.size, W, n_mu)
+
+
+def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders: bool, mu,
+                           normalized: bool, rtol: float, max_panels: int,
+                           allow_unvalidated: bool, refine_backscatter: bool, *,
+                           n_radius: int = 128, n_angles: int = 128) -> dict:
+    """The outputs of each distribution (tagged tuples ``specs``) of one particle at each
+    wavelength, without derivatives, one row per (distribution, wavelength),
+    distribution-major, as the reference's ``_scatrs.ensemble_compute_batch``
+    (scatrs-py/src/ensemble.rs:480-504); every distribution in the one library call."""
+    m, fr = _particle(particle_spec)
+    wl = np.ascontiguousarray(np.asarray(wavelengths, np.float64).reshape(-1))
+    mu = _mu(mu)
+    units = [[_Component(_distribution(s), m, fr, 1.0)] for s in specs]
+    if units:
+        _validate(units[0], wl.size)
+    order = _order(greek, all_orders)
+    n_mu = 0 if mu is None else mu.size
+    _check_options(rtol, max_panels, len(units) * wl.size, n_mu, order, 0)
+    W = 0 if order is None else order + 1
+    if not units:
+        import torch
+
+        z = torch.zeros((0, wl.size, _N + 6 * W + 6 * n_mu), dtype=torch.float64)
+        return _dict(z, z, None, None, torch.zeros((0, wl.size), dtype=torch.bool),
+                     np.zeros(0, np.int64), wl.size, W, n_mu)
+    V, E, _, _, conv, counts = _run(
+        units, wl, order=order, mu=mu, normalized=normalized, keys=[], rtol=rtol,
+        refine_backscatter=refine_backscatter, n_radius=n_radius, n_angles=n_angles,
+        allow_unvalidated=allow_unvalidated)
+    return _dict(V, E, None, None, conv, counts, wl.size, W, n_mu)

@@ -24,7 +24,7 @@ from .size import SEGMENT, Distribution, MultiModal, Tabulated, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
-           "mix_parts", "parameter_names"]
+           "assemble", "mix_parts", "number_averages", "parameter_names"]
 
 SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
            "effective_radius", "mean_area", "mean_volume")
@@ -158,7 +158,8 @@ def _autograd_function():
             sc, gr, ph, _ = ensemble_fixed_device(
                 kind, p0, p1, wavelength, m, fractions, p2=p2, n_radius=opts["n_radius"],
                 n_angles=opts["n_angles"], order=opts["order"], normalized=opts["normalized"],
-                allow_unvalidated=opts["allow_unvalidated"], want_phase=True, mu=opts["mu"])
+                allow_unvalidated=opts["allow_unvalidated"], want_phase=opts["phase"],
+                mu=opts["mu"])
             empty = torch.empty(0)
             ctx.save_for_backward(p0, p1, p2 if p2 is not None else empty, m_real, m_imag,
                                   fractions if fractions is not None else empty, kind, wavelength)
@@ -194,21 +195,23 @@ _FUNCTION = None
 
 
 def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *, p2=None,
-                   n_radius: int = 128, n_angles: int = 128, order: int = 32,
+                   n_radius: int = 128, n_angles: int = 128, order: int | None = 32,
                    normalized: bool = True, fixed_range: bool = False,
-                   allow_unvalidated: bool = False, mu=None):
+                   allow_unvalidated: bool = False, mu=None, phase: bool = True):
     """Differentiable batch of ensembles on the fixed path (torch.autograd): p0, p1,
     wavelength (count,) and p2 (count,) or None; m_real, m_imag (count,) for spheres or
     (count, L) for layered spheres (core first); fractions (count, L-1). Returns (scalars
-    (count, 11), greek (count, 6, order+1), phase (count, 6, n_angles), or at the host cosines
-    ``mu`` (count, 6, len(mu))); gradients flow to p0, p1, p2, m_real, m_imag and fractions
-    through the library's reverse-mode kernel."""
+    (count, 11), greek (count, 6, order+1) or None without an order, phase (count, 6,
+    n_angles), or at the host cosines ``mu`` (count, 6, len(mu)), or None with phase=False);
+    gradients flow to p0, p1, p2, m_real, m_imag and fractions through the library's
+    reverse-mode kernel."""
     global _FUNCTION
     if _FUNCTION is None:
         _FUNCTION = _autograd_function()
-    opts = dict(n_radius=int(n_radius), n_angles=int(n_angles), order=int(order),
+    opts = dict(n_radius=int(n_radius), n_angles=int(n_angles),
+                order=None if order is None else int(order),
                 normalized=bool(normalized), fixed_range=bool(fixed_range),
-                allow_unvalidated=bool(allow_unvalidated),
+                allow_unvalidated=bool(allow_unvalidated), phase=bool(phase),
                 mu=None if mu is None else np.array(mu, np.float64).reshape(-1))
     return _FUNCTION.apply(p0, p1, p2, m_real, m_imag, fractions, kind, wavelength, opts)
 
@@ -226,27 +229,43 @@ def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
     matrix divided by <C_sca> and the Greek coefficients by their alpha1_0 when normalized.
     torch tensors in and out, differentiable in every input, the fractions included (with
     ``ensemble_fixed(..., normalized=False)`` for the parts, a mixture's gradient)."""
-    import torch
-
     f = fractions.to(scalars.dtype)
     f = f.reshape(f.shape + (1,) * (scalars.dim() - 1 - f.dim()))  # (parts, batch axes)
+    avg = (f.unsqueeze(-1) * number_averages(scalars)).sum(0)
+    ph = None if phase is None else (f[..., None, None] * phase).sum(0)
+    gr = None if greek is None else (f[..., None, None] * greek).sum(0)
+    return assemble(avg, gr, ph, normalized)
+
+
+def number_averages(scalars):
+    """The linear number averages behind the outputs of the fixed path with
+    ``normalized=False`` (scalars (..., 11)): (..., 7) of <C_ext>, <C_sca>, <C_abs>, <C_back>,
+    <C_sca g>, <r^2>, <r^3>, the quantities a mixture averages (engine.rs:2120-2150)."""
+    import torch
+
     sc = scalars
-    avg = torch.stack([sc[..., 0], sc[..., 1], sc[..., 2], sc[..., 3], sc[..., 5] * sc[..., 1],
-                       sc[..., 9] / math.pi, sc[..., 10] * (3.0 / (4.0 * math.pi))], dim=-1)
-    ext, sca, absb, back, scag, r2, r3 = (f.unsqueeze(-1) * avg).sum(0).unbind(-1)
+    return torch.stack([sc[..., 0], sc[..., 1], sc[..., 2], sc[..., 3], sc[..., 5] * sc[..., 1],
+                        sc[..., 9] / math.pi, sc[..., 10] * (3.0 / (4.0 * math.pi))], dim=-1)
+
+
+def assemble(avg, greek, phase, normalized: bool = True):
+    """The outputs from mixed number averages (avg (..., 7) as number_averages gives them;
+    greek (..., 6, L+1) and phase (..., 6, n) mixed alike, or None), as the reference assembles
+    them (engine.rs:2396-2497): ssa, g, the lidar ratio and the effective radius as ratios of
+    averages, the phase matrix divided by <C_sca> and the Greek coefficients by alpha1_0 when
+    normalized; depolarization 0 (spheres)."""
+    import torch
+
+    ext, sca, absb, back, scag, r2, r3 = avg.unbind(-1)
     out = torch.stack([ext, sca, absb, back, sca / ext, scag / sca, 4.0 * math.pi * ext / back,
                        torch.zeros_like(ext), r3 / r2, math.pi * r2,
                        4.0 * math.pi / 3.0 * r3], dim=-1)
-    ph = None
-    if phase is not None:
-        ph = (f[..., None, None] * phase).sum(0)
-        if normalized:
-            ph = ph / sca[..., None, None]
-    gr = None
-    if greek is not None:
-        gr = (f[..., None, None] * greek).sum(0)
-        if normalized:
-            gr = gr / gr[..., 0:1, 0:1]
+    ph = phase
+    if ph is not None and normalized:
+        ph = ph / sca[..., None, None]
+    gr = greek
+    if gr is not None and normalized:
+        gr = gr / gr[..., 0:1, 0:1]
     return out, gr, ph
 
 
