@@ -28,7 +28,7 @@ extern "C" {
 #define SCATRS_NOT_CONVERGED 3
 #define SCATRS_DEVICE_ERROR 4
 
-#define SCATRS_ABI_VERSION 2
+#define SCATRS_ABI_VERSION 3
 
 typedef struct scatrs_ctx scatrs_ctx;
 
@@ -159,6 +159,27 @@ int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_k
                               int allow_unvalidated, const double *d_bar_scalars,
                               const double *d_bar_greek, const double *d_bar_phase,
                               double *d_grad, int64_t *total_terms);
+
+/* The ensembles of scatrs_ensemble_fixed on the caller's radius nodes: d_nodes (count, n_nodes,
+ * 2) the (r, W) pairs of each ensemble's quadrature (r > 0, W >= 0; W the quadrature weight
+ * times the number density per ln r, so that an average is sum W q / sum W), every other
+ * argument and output as scatrs_ensemble_fixed's (n_nodes in place of n_radius). The node
+ * evaluator of the reference's adaptive engine: pass 1 (engine.rs:307-943) evaluates the
+ * Kronrod and Gauss nodes of its panels (kronrod.rs:5-52), pass 2 (engine.rs:1423-1766) all
+ * the panels' nodes at once; the host places the panels (scatrs_amd/adaptive.py). */
+int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes, const double *d_nodes,
+                          const double *d_wavelength, int layers, const double *d_fractions,
+                          const double *d_m, int64_t n_angles, int64_t order, int normalized,
+                          int allow_unvalidated, double *d_scalars, double *d_greek,
+                          double *d_phase, int64_t *total_terms);
+
+/* Host helpers of the adaptive engine, with the kernels' definitions: the fixed ln r range of
+ * a distribution (p[3] as the SCATRS_* kinds list them), and ln u(r) of its number density per
+ * unit radius, up to a constant, at n radii (distribution.rs:366-430; the density per ln r is
+ * u r). SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. */
+int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi);
+int scatrs_size_log_density(int32_t kind, const double *p, int64_t n, const double *r,
+                            double *ln_u);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libscatrs_hip.so"
 
 OK, INVALID_INPUT, OUTSIDE_VALIDATED_DOMAIN, NOT_CONVERGED, DEVICE_ERROR = 0, 1, 2, 3, 4
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _lib = None
 _lock = threading.Lock()
@@ -39,6 +39,9 @@ SYMBOLS = (
     "scatrs_mie_efficiencies",
     "scatrs_ensemble_fixed",
     "scatrs_ensemble_fixed_vjp",
+    "scatrs_ensemble_nodes",
+    "scatrs_size_range",
+    "scatrs_size_log_density",
 )
 
 
@@ -81,6 +84,13 @@ def load() -> C.CDLL:
             L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp,
                                                     i64, i64, i64, i32, i32, i32, vp, vp, vp, vp,
                                                     C.POINTER(i64)]
+        if hasattr(L, "scatrs_ensemble_nodes"):
+            L.scatrs_ensemble_nodes.argtypes = [vp, i64, i64, vp, vp, i32, vp, vp, i64, i64, i32,
+                                                i32, vp, vp, vp, C.POINTER(i64)]
+        if hasattr(L, "scatrs_size_range"):
+            L.scatrs_size_range.argtypes = [i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+            L.scatrs_size_log_density.argtypes = [i32, C.POINTER(dbl), i64, C.POINTER(dbl),
+                                                  C.POINTER(dbl)]
         if L.scatrs_abi_version() != ABI_VERSION:
             raise NativeLibraryError("libscatrs_hip.so has a different ABI version")
         _lib = L

@@ -458,6 +458,46 @@ def _dict(V, E, D, DE, conv, counts, n_per_unit: int, W: int, n_mu: int):
     return out
 
 
+def _particle_object(comp):
+    """The component's particle as the package's Sphere or LayeredSphere, with one index per
+    wavelength."""
+    from .particles import LayeredSphere, Sphere
+
+    if comp.layers == 1:
+        return Sphere(comp.m[0])
+    return LayeredSphere(tuple(float(f) for f in comp.fr), [comp.m[l] for l in range(comp.layers)])
+
+
+def _run_adaptive(units, wl, *, order, mu, normalized, rtol, max_panels, refine_backscatter,
+                  n_angles, allow_unvalidated):
+    """Values and errors (U, n, K), converged (U, n), panels per unit (U,) by the adaptive
+    engine (scatrs_amd.adaptive)."""
+    import torch
+
+    from .ensemble import _adaptive, _parts_of
+
+    parts, unit, frac = [], [], []
+    for u, comps in enumerate(units):
+        total = sum(c.weight for c in comps)
+        for c in comps:
+            if c.weight == 0.0:
+                continue
+            p, f = _parts_of(c.dist, _particle_object(c), c.weight / total)
+            parts += p
+            frac += f
+            unit += [u] * len(p)
+    k, g, _, conv, panels, _ = _adaptive(
+        parts, unit, frac, len(units), wl, greek=order, normalized=normalized,
+        n_angles=n_angles, rtol=rtol, max_panels=max_panels,
+        allow_unvalidated=allow_unvalidated, phase=mu is not None, mu=mu,
+        refine_backscatter=refine_backscatter)
+    V = torch.stack([_flat(k[0][u], None if k[1] is None else k[1][u],
+                           None if k[2] is None else k[2][u]) for u in range(len(units))])
+    Vg = torch.stack([_flat(g[0][u], None if g[1] is None else g[1][u],
+                            None if g[2] is None else g[2][u]) for u in range(len(units))])
+    return V, (V - Vg).abs(), torch.from_numpy(conv), panels
+
+
 def _mu(mu):
     mu = np.ascontiguousarray(np.asarray(mu, np.float64).reshape(-1))
     return mu if mu.size else None
@@ -466,12 +506,15 @@ def _mu(mu):
 def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool, mu,
                      normalized: bool, derivatives, rtol: float, max_panels: int,
                      allow_unvalidated: bool, refine_backscatter: bool, *,
-                     n_radius: int = 128, n_angles: int = 128) -> dict:
+                     n_radius: int = 128, n_angles: int | None = None,
+                     adaptive: bool = False) -> dict:
     """The ensemble outputs of the components at each wavelength as the reference's
     ``_scatrs.ensemble_compute`` returns them (scatrs-py/src/ensemble.rs:290-355), on the
     fixed path (``n_radius`` radius nodes per part, the Greek coefficients projected on
-    ``n_angles`` Gauss-Legendre cosines); ``derivatives`` are indices into
-    ``ensemble_parameters(items, mixture)``."""
+    ``n_angles`` Gauss-Legendre cosines, 128 by default) or, with ``adaptive``, by the adaptive
+    engine (scatrs_amd.adaptive: rtol, max_panels and refine_backscatter as the reference
+    uses them; n_angles=None sizes the Greek projection's rule to the largest N; no
+    derivatives); ``derivatives`` are indices into ``ensemble_parameters(items, mixture)``."""
     comps = _components(items)
     wl = np.ascontiguousarray(np.asarray(wavelengths, np.float64).reshape(-1))
     mu = _mu(mu)
@@ -489,18 +532,30 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
     order = _order(greek, all_orders)
     n_mu = 0 if mu is None else mu.size
     _check_options(rtol, max_panels, wl.size, n_mu, order, len(chosen))
+    W = 0 if order is None else order + 1
+    if adaptive:
+        if chosen:
+            raise InvalidInputError("the adaptive engine here computes no derivatives; the fixed "
+                                    "path (adaptive=False) does")
+        V, E, conv, panels = _run_adaptive(
+            [comps], wl, order=order, mu=mu, normalized=normalized, rtol=rtol,
+            max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
+            allow_unvalidated=allow_unvalidated)
+        out = _dict(V, E, None, None, conv, np.zeros(1, np.int64), wl.size, W, n_mu)
+        out["panels"] = panels.reshape(-1).astype(np.uint64)
+        return out
     V, E, D, DE, conv, counts = _run(
         [comps], wl, order=order, mu=mu, normalized=normalized, keys=chosen, rtol=rtol,
-        refine_backscatter=refine_backscatter, n_radius=n_radius, n_angles=n_angles,
-        allow_unvalidated=allow_unvalidated)
-    W = 0 if order is None else order + 1
+        refine_backscatter=refine_backscatter, n_radius=n_radius,
+        n_angles=128 if n_angles is None else n_angles, allow_unvalidated=allow_unvalidated)
     return _dict(V, E, D, DE, conv, counts, wl.size, W, n_mu)
 
 
 def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders: bool, mu,
                            normalized: bool, rtol: float, max_panels: int,
                            allow_unvalidated: bool, refine_backscatter: bool, *,
-                           n_radius: int = 128, n_angles: int = 128) -> dict:
+                           n_radius: int = 128, n_angles: int | None = None,
+                           adaptive: bool = False) -> dict:
     """The outputs of each distribution (tagged tuples ``specs``) of one particle at each
     wavelength, without derivatives, one row per (distribution, wavelength),
     distribution-major, as the reference's ``_scatrs.ensemble_compute_batch``
@@ -521,8 +576,16 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
         z = torch.zeros((0, wl.size, _N + 6 * W + 6 * n_mu), dtype=torch.float64)
         return _dict(z, z, None, None, torch.zeros((0, wl.size), dtype=torch.bool),
                      np.zeros(0, np.int64), wl.size, W, n_mu)
+    if adaptive:
+        V, E, conv, panels = _run_adaptive(
+            units, wl, order=order, mu=mu, normalized=normalized, rtol=rtol,
+            max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
+            allow_unvalidated=allow_unvalidated)
+        out = _dict(V, E, None, None, conv, np.zeros(len(units), np.int64), wl.size, W, n_mu)
+        out["panels"] = panels.reshape(-1).astype(np.uint64)
+        return out
     V, E, _, _, conv, counts = _run(
         units, wl, order=order, mu=mu, normalized=normalized, keys=[], rtol=rtol,
-        refine_backscatter=refine_backscatter, n_radius=n_radius, n_angles=n_angles,
-        allow_unvalidated=allow_unvalidated)
+        refine_backscatter=refine_backscatter, n_radius=n_radius,
+        n_angles=128 if n_angles is None else n_angles, allow_unvalidated=allow_unvalidated)
     return _dict(V, E, None, None, conv, counts, wl.size, W, n_mu)

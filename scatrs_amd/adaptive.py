@@ -1,0 +1,367 @@
+"""Adaptive size integration: the reference's Gauss-Kronrod engine (scatrs/src/ensemble/
+engine.rs:307-943,1423-1766, kronrod.rs:5-52, range.rs:47-93) on the GPU node evaluator
+``scatrs_ensemble_nodes``.
+
+Each task (one part of a distribution at one wavelength) is integrated over panels in t = ln r
+with the 15-point Kronrod rule and its embedded 7-point Gauss rule. The host places the
+panels; every evaluation is one batched library call over all open panels of all tasks.
+
+* Range: the ln r interval outside which the area-weighted law r^2 n(r) holds at most rtol/10
+  of its total, split evenly between the tails. The tails are found from the library's own
+  log-density on a fine grid over three times the fixed path's range. A power law, a tabulated
+  segment and a monodisperse law keep their supports.
+* Pass 1: the refined quantities are the reference's (engine.rs:620-690): <C_ext>, <C_sca>,
+  <C_sca g> and <C_abs>, plus <C_back> with ``refine_backscatter``. Each quantity's error is the
+  sum over panels of |Kronrod - Gauss|. Its target is rtol times |C_ext| (C_ext and C_abs),
+  |C_sca| (C_sca and C_sca g), or max(|C_back|, |C_sca|) (C_back). While a task misses a target:
+  * the range grows by one panel of the outer panel's width on a side whose outer panel holds
+    more than rtol/10 of a quantity (the reference's extension);
+  * every panel whose error exceeds the target over the number of panels is bisected, and
+    always the worst one.
+  A task stops unconverged when the next bisection would pass ``max_panels``.
+* Pass 2: every output (the scalars, the Greek coefficients, the phase matrix) is evaluated on
+  the final panels' Kronrod nodes, and again on their Gauss nodes; the error estimate is the
+  difference of the two rules' outputs.
+
+The panels are this engine's own: the reference's exact panel decisions depend on its
+evaluation order. Both engines meet rtol, so their results agree to about rtol, not bitwise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .errors import InvalidInputError
+from .size import MONODISPERSE, POWER_LAW, SEGMENT
+
+# QUADPACK's qk15 (Piessens, de Doncker-Kapenga, Ueberhuber and Kahaner, 1983): the Kronrod
+# abscissae in [0, 1] (descending), their weights, and the 7-point Gauss weights of the
+# abscissae 1, 3, 5 and 7 of the list.
+_XGK = np.array([0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                 0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                 0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                 0.207784955007898467600689403773245, 0.0])
+_WGK = np.array([0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                 0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                 0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                 0.204432940075298892414161999234649, 0.209482141084727828012999174891714])
+_WG = np.array([0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                0.381830050505118944950369775488975, 0.417959183673469387755102040816327])
+
+_INITIAL_PANELS = 4
+_MAX_ROUNDS = 64
+_GRID = 8193
+
+
+def kronrod15():
+    """(x (15,) ascending in [-1, 1], Kronrod weights (15,), Gauss weights (15,), 0 at the
+    Kronrod-only nodes)."""
+    x = np.concatenate([-_XGK[:7], [0.0], _XGK[6::-1]])
+    wk = np.concatenate([_WGK[:7], [_WGK[7]], _WGK[6::-1]])
+    wg = np.zeros(15)
+    for j, k in enumerate((1, 3, 5)):
+        wg[k] = wg[14 - k] = _WG[j]
+    wg[7] = _WG[3]
+    return x, wk, wg
+
+
+_X, _WK, _WGF = kronrod15()
+_GAUSS = np.flatnonzero(_WGF)
+
+
+def _dbl(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def size_range(kind: int, p) -> tuple[float, float]:
+    """The fixed path's ln r range of a distribution (scatrs_size_range)."""
+    q = np.ascontiguousarray(p, np.float64)
+    lo, hi = C.c_double(), C.c_double()
+    if _native.load().scatrs_size_range(int(kind), _dbl(q), C.byref(lo), C.byref(hi)) != 0:
+        raise InvalidInputError(f"invalid parameters {tuple(q)} of size distribution kind {kind}")
+    return lo.value, hi.value
+
+
+def log_density(kind: int, p, r) -> np.ndarray:
+    """ln u(r), the number density per unit radius up to a constant (scatrs_size_log_density)."""
+    q = np.ascontiguousarray(p, np.float64)
+    r = np.ascontiguousarray(r, np.float64).reshape(-1)
+    out = np.empty_like(r)
+    if _native.load().scatrs_size_log_density(int(kind), _dbl(q), r.size, _dbl(r),
+                                              _dbl(out)) != 0:
+        raise InvalidInputError(f"invalid parameters {tuple(q)} of size distribution kind {kind}")
+    return out
+
+
+def integration_range(kind: int, p, rtol: float) -> tuple[float, float, float]:
+    """(t_lo, t_hi, c): the ln r range whose area-weighted tails hold at most rtol/10 of the
+    area, and c, the largest ln of the density per ln r there (the weights' common scale)."""
+    lo, hi = size_range(kind, p)
+    if kind in (POWER_LAW, SEGMENT):
+        t = np.linspace(lo, hi, 257)
+        return lo, hi, float(np.max(log_density(kind, p, np.exp(t)) + t))
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    t = np.linspace(mid - 3.0 * half, mid + 3.0 * half, _GRID)
+    a = log_density(kind, p, np.exp(t)) + 3.0 * t  # ln of r^2 n(r) per ln r
+    f = np.exp(a - np.max(a))
+    cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(t))])
+    eps = 0.05 * rtol * cum[-1]  # rtol/10 in all, half per tail
+    i_lo = max(int(np.searchsorted(cum, eps, side="right")) - 1, 0)
+    i_hi = min(int(np.searchsorted(cum, cum[-1] - eps, side="left")), t.size - 1)
+    t_lo, t_hi = t[i_lo], t[i_hi]
+    if not t_hi > t_lo:
+        t_lo, t_hi = lo, hi
+    tt = np.linspace(t_lo, t_hi, 257)
+    return t_lo, t_hi, float(np.max(log_density(kind, p, np.exp(tt)) + tt))
+
+
+@dataclass
+class Task:
+    """One part at one wavelength: its distribution (kind, p), index and fractions, range and
+    panels, and its pass-1 state."""
+
+    kind: int
+    p: np.ndarray
+    wl: float
+    m: np.ndarray  # (L,) complex
+    fr: np.ndarray  # (L-1,)
+    t_lo: float = 0.0
+    t_hi: float = 0.0
+    c: float = 0.0
+    panels: list = field(default_factory=list)  # [a, b, K sums (5,) or None, G sums (5,)]
+    done: bool = False
+    converged: bool = False
+
+    @property
+    def layers(self) -> int:
+        return self.m.size
+
+
+def _nodes(task: Task, panels, idx):
+    """(r, W) of the panels' nodes idx of the rule (rows: the panels' nodes in order), with the
+    Kronrod weights at all 15 nodes or the Gauss weights at the 7."""
+    x = _X[idx]
+    w = _WK[idx] if idx.size == 15 else _WGF[idx]
+    rs, ws = [], []
+    for a, b in panels:
+        half, mid = 0.5 * (b - a), 0.5 * (a + b)
+        t = mid + half * x
+        r = np.exp(t)
+        u = np.exp(log_density(task.kind, task.p, r) + t - task.c)  # density per ln r, scaled
+        rs.append(r)
+        ws.append(w * half * u)
+    return np.concatenate(rs), np.concatenate(ws)
+
+
+def _call(tasks, rows, *, order=None, n_angles=2, mu=None, want_phase=False, normalized=False,
+          allow_unvalidated=False):
+    """One library call per (layers, nodes per row) over rows of (task index, r, W): the
+    outputs (scalars (rows, 11), greek, phase) in row order, and the rows' sums of W."""
+    import torch
+
+    from .ensemble import ensemble_nodes_device
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n = len(rows)
+    groups: dict[tuple[int, int], list[int]] = {}
+    for i, (ti, r, _) in enumerate(rows):
+        groups.setdefault((tasks[ti].layers, r.size), []).append(i)
+    sc = np.empty((n, 11))
+    gr = None if order is None else np.empty((n, 6, order + 1))
+    n_phase = n_angles if mu is None else int(np.size(mu))
+    ph = np.empty((n, 6, n_phase)) if want_phase else None
+    z = np.array([w.sum() for _, _, w in rows])
+    terms = 0
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    for (L, k), idx in groups.items():
+        nodes = np.stack([np.stack([rows[i][1], rows[i][2]], -1) for i in idx])  # (g, k, 2)
+        tk = [tasks[rows[i][0]] for i in idx]
+        wl = np.array([q.wl for q in tk])
+        m = np.stack([q.m for q in tk])  # (g, L) complex
+        fr = np.stack([q.fr for q in tk]) if L > 1 else None
+        s, g, p, nt = ensemble_nodes_device(
+            t(nodes), t(wl), t(m[:, 0] if L == 1 else m), None if fr is None else t(fr),
+            n_angles=n_angles, order=order, normalized=normalized,
+            allow_unvalidated=allow_unvalidated, want_phase=want_phase, mu=mu)
+        sc[idx] = s.cpu().numpy()
+        if gr is not None:
+            gr[idx] = g.cpu().numpy()
+        if ph is not None:
+            ph[idx] = p.cpu().numpy()
+        terms += nt
+    return sc, gr, ph, z, terms
+
+
+def _refined(sc, z):
+    """The refined sums [C_ext, C_sca, C_sca g, C_abs, C_back] of rows (normalized=False
+    scalars, which are averages: times the row's sum of weights); 0 for an empty row."""
+    s = np.stack([sc[:, 0], sc[:, 1], sc[:, 5] * sc[:, 1], sc[:, 2], sc[:, 3]], 1) * z[:, None]
+    s[z == 0.0] = 0.0
+    return s
+
+
+def _targets(total, rtol, refine_backscatter):
+    ext, sca, back = abs(total[0]), abs(total[1]), abs(total[4])
+    tgt = rtol * np.array([ext, sca, sca, ext, max(back, sca)])
+    if not refine_backscatter:
+        tgt[4] = np.inf
+    return tgt
+
+
+def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
+          allow_unvalidated: bool) -> int:
+    """Places every task's panels (see the module's notes); returns the Mie terms evaluated."""
+    terms = 0
+    for q in tasks:
+        if q.kind == MONODISPERSE:
+            q.done = q.converged = True
+            continue
+        q.t_lo, q.t_hi, q.c = integration_range(q.kind, q.p, rtol)
+        edges = np.linspace(q.t_lo, q.t_hi, _INITIAL_PANELS + 1)
+        q.panels = [[a, b, None, None] for a, b in zip(edges[:-1], edges[1:])]
+    extendable = lambda q: q.kind not in (POWER_LAW, SEGMENT)  # noqa: E731
+    for _ in range(_MAX_ROUNDS):
+        open_ = [(ti, pi) for ti, q in enumerate(tasks) if not q.done
+                 for pi, pn in enumerate(q.panels) if pn[2] is None]
+        if open_:
+            for idx, slot in ((np.arange(15), 2), (_GAUSS, 3)):
+                rows = []
+                for ti, pi in open_:
+                    r, w = _nodes(tasks[ti], [tasks[ti].panels[pi][:2]], idx)
+                    rows.append((ti, r, w))
+                sc, _, _, z, nt = _call(tasks, rows, allow_unvalidated=allow_unvalidated)
+                terms += nt
+                s = _refined(sc, z)
+                for (ti, pi), v in zip(open_, s):
+                    tasks[ti].panels[pi][slot] = v
+        active = False
+        for q in tasks:
+            if q.done:
+                continue
+            K = np.array([pn[2] for pn in q.panels])
+            E = np.abs(K - np.array([pn[3] for pn in q.panels]))
+            total = K.sum(0)
+            tgt = _targets(total, rtol, refine_backscatter)
+            fail = E.sum(0) > tgt
+            grow_lo = grow_hi = False
+            if extendable(q):
+                edge = 0.1 * tgt
+                grow_lo = bool(np.any(np.abs(K[0]) > edge))
+                grow_hi = bool(np.any(np.abs(K[-1]) > edge))
+            if not fail.any() and not grow_lo and not grow_hi:
+                q.done = q.converged = True
+                continue
+            cut = set()
+            if fail.any():
+                share = tgt / len(q.panels)
+                over = (E > share) & fail
+                cut = set(np.flatnonzero(over.any(1)))
+                cut.add(int(np.argmax((E / np.where(tgt > 0, tgt, 1.0))[:, fail].max(1))))
+            if len(q.panels) + len(cut) + grow_lo + grow_hi > max_panels:
+                q.done = True
+                continue
+            new = []
+            if grow_lo:
+                a, b = q.panels[0][:2]
+                new.append([2 * a - b, a, None, None])
+                q.t_lo = 2 * a - b
+            for pi, pn in enumerate(q.panels):
+                if pi in cut:
+                    mid = 0.5 * (pn[0] + pn[1])
+                    new += [[pn[0], mid, None, None], [mid, pn[1], None, None]]
+                else:
+                    new.append(pn)
+            if grow_hi:
+                a, b = q.panels[-1][:2]
+                new.append([b, 2 * b - a, None, None])
+                q.t_hi = 2 * b - a
+            q.panels = new
+            active = True
+        if not active:
+            break
+    for q in tasks:
+        q.done = True
+    return terms
+
+
+def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
+    """Every output of every task on its final panels (normalized=False), by the Kronrod and
+    by the Gauss rule: ((sc, gr, ph) Kronrod, (sc, gr, ph) Gauss, terms)."""
+    out = []
+    terms = 0
+    for idx in (np.arange(15), _GAUSS):
+        rows = []
+        for ti, q in enumerate(tasks):
+            if q.kind == MONODISPERSE:
+                rows.append((ti, np.array([q.p[0]]), np.array([1.0])))
+            else:
+                r, w = _nodes(q, [pn[:2] for pn in q.panels], idx)
+                rows.append((ti, r, w))
+        sc, gr, ph, _, nt = _call(tasks, rows, order=order, n_angles=n_angles, mu=mu,
+                                  want_phase=want_phase, normalized=False,
+                                  allow_unvalidated=allow_unvalidated)
+        out.append((sc, gr, ph))
+        terms += nt
+    return out[0], out[1], terms
+
+
+def greek_angles(tasks, order: int, n_angles: int | None) -> int:
+    """The angle rule of the Greek projection: the caller's, or the smallest exact for every
+    node, N_max + (order + 1)/2 cosines at the tasks' largest size parameter (at most 4096)."""
+    if n_angles is not None:
+        return int(n_angles)
+    lib = _native.load()
+    n_max = 1
+    for q in tasks:
+        r = q.p[0] if q.kind == MONODISPERSE else np.exp(q.t_hi)
+        n_max = max(n_max, int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl)))
+    return int(min(4096, max(2, n_max + (order + 2) // 2)))
+
+
+def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_panels: int,
+                   allow_unvalidated: bool, want_phase: bool, mu=None,
+                   refine_backscatter: bool = False):
+    """Each part (kind, p0, p1, p2, particle) at every wavelength by the adaptive engine with
+    normalized=False, as _parts_device returns them (tensors (parts, wavelengths, ...)), with
+    the Gauss rule's outputs, converged and panels (parts, wavelengths) and the ranges
+    (parts, wavelengths, 2) in r: (K outputs, G outputs, terms, converged, panels, ranges)."""
+    import torch
+
+    if not (1e-13 <= rtol <= 0.1):
+        raise InvalidInputError(f"rtol must be in [1e-13, 0.1], got {rtol}")
+    if max_panels < 1:
+        raise InvalidInputError("max_panels must be at least 1")
+    nw = wl.size
+    tasks = []
+    for kind, p0, p1, p2, particle in parts:
+        m = particle.indices(wl).reshape(particle.layers, -1)  # (L, nw)
+        fr = np.asarray(particle.fractions(), np.float64)
+        for i in range(nw):
+            tasks.append(Task(int(kind), np.array([p0, p1, p2], np.float64),
+                              float(wl.reshape(-1)[i]), m[:, i].copy(), fr))
+    terms = pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
+    na = n_angles
+    if order is not None:
+        na = greek_angles(tasks, order, n_angles)
+    elif na is None:
+        na = 2
+    K, G, t2 = pass2(tasks, order=order, n_angles=na, mu=mu, want_phase=want_phase,
+                     allow_unvalidated=allow_unvalidated)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    P = len(parts)
+
+    def shape(out):
+        sc, gr, ph = out
+        f = lambda a: None if a is None else torch.from_numpy(  # noqa: E731
+            a.reshape((P, nw) + a.shape[1:])).to(dev)
+        return f(sc), f(gr), f(ph)
+
+    conv = np.array([q.converged for q in tasks]).reshape(P, nw)
+    panels = np.array([max(len(q.panels), 1) for q in tasks]).reshape(P, nw)
+    ranges = np.array([(q.p[0], q.p[0]) if q.kind == MONODISPERSE
+                       else (np.exp(q.t_lo), np.exp(q.t_hi)) for q in tasks]).reshape(P, nw, 2)
+    return shape(K), shape(G), terms + t2, conv, panels, ranges, na

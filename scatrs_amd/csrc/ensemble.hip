@@ -7,26 +7,29 @@
 
 using namespace scatrs;
 
-extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
-                                     const double *d_p0, const double *d_p1, const double *d_p2,
-                                     const double *d_wavelength, int layers,
-                                     const double *d_fractions, const double *d_m,
-                                     int64_t n_radius, int64_t n_angles, int64_t order,
-                                     int normalized, int allow_unvalidated, double *d_scalars,
-                                     double *d_greek, double *d_phase, int64_t *total_terms) {
+// The forward pass of both entry points: the nodes from each ensemble's distribution (d_kind,
+// d_p0, d_p1, d_p2 on the Gauss-Legendre rule of n_radius nodes) or, with d_nodes, the caller's
+// n_radius (r, W) pairs per ensemble.
+static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                            const double *d_p0, const double *d_p1, const double *d_p2,
+                            const double *d_nodes, const double *d_wavelength, int layers,
+                            const double *d_fractions, const double *d_m, int64_t n_radius,
+                            int64_t n_angles, int64_t order, int normalized,
+                            int allow_unvalidated, double *d_scalars, double *d_greek,
+                            double *d_phase, int64_t *total_terms) {
     if (!ctx) return SCATRS_INVALID_INPUT;
     ctx->error.clear();
     if (total_terms) *total_terms = 0;
     int rc = ens_check_args(ctx, count, layers, n_radius, n_angles, d_greek != nullptr, order);
     if (rc) return rc;
     if (count == 0) return SCATRS_OK;
-    if (!d_kind || !d_p0 || !d_p1 || !d_wavelength || !d_m || !d_scalars ||
+    if ((!d_nodes && (!d_kind || !d_p0 || !d_p1)) || !d_wavelength || !d_m || !d_scalars ||
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
     EnsSetup S;
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_greek != nullptr, order, allow_unvalidated,
-                        ctx->angular_kernel, S)))
+                        ctx->angular_kernel, S, d_nodes)))
         return rc;
     if (total_terms) *total_terms = S.total_terms;
     hipStream_t s = ctx->stream;
@@ -89,4 +92,31 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
         e0 = e1;
     }
     return SCATRS_OK;
+}
+
+extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                                     const double *d_p0, const double *d_p1, const double *d_p2,
+                                     const double *d_wavelength, int layers,
+                                     const double *d_fractions, const double *d_m,
+                                     int64_t n_radius, int64_t n_angles, int64_t order,
+                                     int normalized, int allow_unvalidated, double *d_scalars,
+                                     double *d_greek, double *d_phase, int64_t *total_terms) {
+    return ensemble_forward(ctx, count, d_kind, d_p0, d_p1, d_p2, nullptr, d_wavelength, layers,
+                            d_fractions, d_m, n_radius, n_angles, order, normalized,
+                            allow_unvalidated, d_scalars, d_greek, d_phase, total_terms);
+}
+
+extern "C" int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
+                                     const double *d_nodes, const double *d_wavelength,
+                                     int layers, const double *d_fractions, const double *d_m,
+                                     int64_t n_angles, int64_t order, int normalized,
+                                     int allow_unvalidated, double *d_scalars, double *d_greek,
+                                     double *d_phase, int64_t *total_terms) {
+    if (ctx && count > 0 && !d_nodes) {
+        ctx->error.clear();
+        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    }
+    return ensemble_forward(ctx, count, nullptr, nullptr, nullptr, nullptr, d_nodes, d_wavelength,
+                            layers, d_fractions, d_m, n_nodes, n_angles, order, normalized,
+                            allow_unvalidated, d_scalars, d_greek, d_phase, total_terms);
 }

@@ -51,6 +51,7 @@ struct EnsArgs {
     const double *rnodes, *rweights;  // GL rule on [-1, 1]
     const double *p2;                 // third parameter (may be null: 0)
     double2 *geom;                    // per node (r, W), written by ens_prep
+    const double2 *user_rw;           // the caller's (r, W) per node, or null (from kind, p)
 };
 
 // Ensemble e's distribution parameters.
@@ -92,7 +93,8 @@ __device__ __forceinline__ NodeGeom node_stored(const EnsArgs &a, int64_t e, int
 }
 
 // Status codes of the ensemble prep, beyond kValid/kInvalid/kOutside: which input failed.
-enum : int { kBadDist = 1 << 4, kBadWl = 2 << 4, kBadM = 3 << 4, kBadFr = 4 << 4 };
+enum : int { kBadDist = 1 << 4, kBadWl = 2 << 4, kBadM = 3 << 4, kBadFr = 4 << 4,
+             kBadNode = 5 << 4 };
 
 // Checks of one layered node (layered/mod.rs:229-319, layered/domain.rs:53-95), x_l = f_l x.
 static __device__ int layered_code(double x, const double *fr, const double *m, int L, int allow,
@@ -137,18 +139,33 @@ static __global__ void __launch_bounds__(256) ens_prep(EnsArgs a, int64_t count,
     if (g < total) {
         const int64_t e = g / a.n_radius, i = g % a.n_radius;
         int code = kValid;
-        double pe[3];
-        ens_params(a, e, pe);
-        if (!dist_valid(a.kind[e], pe)) code = kInvalid | kBadDist;
-        else if (!(isfinite(a.wl[e]) && a.wl[e] > 0.0)) code = kInvalid | kBadWl;
+        if (!a.user_rw) {
+            double pe[3];
+            ens_params(a, e, pe);
+            if (!dist_valid(a.kind[e], pe)) code = kInvalid | kBadDist;
+        }
+        if (code != kValid) {
+        } else if (!(isfinite(a.wl[e]) && a.wl[e] > 0.0)) code = kInvalid | kBadWl;
         else if (a.layers > 1) {
             const double *fr = a.fr + e * (a.layers - 1);
             for (int l = 0; l < a.layers - 1; ++l)
                 if (!(fr[l] > 0.0 && fr[l] <= 1.0) || (l > 0 && fr[l] < fr[l - 1]))
                     code = kInvalid | kBadFr;
         }
+        if (code == kValid && a.user_rw) {
+            const double2 rw = a.user_rw[g];
+            if (!(isfinite(rw.x) && rw.x > 0.0 && isfinite(rw.y) && rw.y >= 0.0))
+                code = kInvalid | kBadNode;
+        }
         if (code == kValid) {
-            const NodeGeom geo = node_geometry(a, e, i);
+            NodeGeom geo;
+            if (a.user_rw) {
+                geo.r = a.user_rw[g].x;
+                geo.W = a.user_rw[g].y;
+                geo.x = 2.0 * kPi * geo.r / a.wl[e];
+            } else {
+                geo = node_geometry(a, e, i);
+            }
             a.geom[g] = make_double2(geo.r, geo.W);
             long long nn = 0;
             const int c = a.layers == 1
@@ -1138,10 +1155,12 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
     int32_t kind = 0;
     double p[3] = {0.0, 0.0, 0.0}, wl = 0;
     std::vector<double> fr(std::max(L - 1, 0)), m(2 * L);
-    (void)hipMemcpy(&kind, dev.kind + e, sizeof kind, hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&p[0], dev.p0 + e, sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipMemcpy(&p[1], dev.p1 + e, sizeof(double), hipMemcpyDeviceToHost);
-    if (dev.p2) (void)hipMemcpy(&p[2], dev.p2 + e, sizeof(double), hipMemcpyDeviceToHost);
+    if (!dev.user_rw) {
+        (void)hipMemcpy(&kind, dev.kind + e, sizeof kind, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&p[0], dev.p0 + e, sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&p[1], dev.p1 + e, sizeof(double), hipMemcpyDeviceToHost);
+        if (dev.p2) (void)hipMemcpy(&p[2], dev.p2 + e, sizeof(double), hipMemcpyDeviceToHost);
+    }
     const double p0 = p[0], p1 = p[1], p2 = p[2];
     (void)hipMemcpy(&wl, dev.wl + e, sizeof wl, hipMemcpyDeviceToHost);
     if (L > 1)
@@ -1150,7 +1169,9 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
     (void)hipMemcpy(m.data(), dev.m + e * 2 * L, m.size() * sizeof(double), hipMemcpyDeviceToHost);
     const std::string pre = "ensemble " + std::to_string(e) + ": ";
     *code_out = kInvalid;
-    if (!dist_valid(kind, p)) {
+    double2 rw = make_double2(0.0, 0.0);
+    if (dev.user_rw) (void)hipMemcpy(&rw, dev.user_rw + g, sizeof rw, hipMemcpyDeviceToHost);
+    if (!dev.user_rw && !dist_valid(kind, p)) {
         // The reference's messages (distribution.rs:99-185).
         const auto positive = [&](const char *name, double v) {
             return pre + name + " must be finite and > 0, got " + fmt_e(v);
@@ -1189,9 +1210,14 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
     for (int l = 0; l + 1 < L; ++l)
         if (!(fr[l] > 0.0 && fr[l] <= 1.0) || (l > 0 && fr[l] < fr[l - 1]))
             return pre + "the radius fractions must be nondecreasing in (0, 1]";
-    double t_lo, t_hi;
-    radius_range(kind, p, &t_lo, &t_hi);
-    const double r = std::exp(0.5 * (t_hi + t_lo) + 0.5 * (t_hi - t_lo) * rnodes[i]);
+    if (dev.user_rw && !(std::isfinite(rw.x) && rw.x > 0.0 && std::isfinite(rw.y) && rw.y >= 0.0))
+        return pre + "node " + std::to_string(i) +
+               ": the radius must be finite and > 0 and the weight finite and >= 0, got " +
+               fmt_e(rw.x) + " and " + fmt_e(rw.y);
+    double t_lo = 0.0, t_hi = 0.0;
+    if (!dev.user_rw) radius_range(kind, p, &t_lo, &t_hi);
+    const double r = dev.user_rw ? rw.x
+                                 : std::exp(0.5 * (t_hi + t_lo) + 0.5 * (t_hi - t_lo) * rnodes[i]);
     const double x = 2.0 * kPi * r / wl;
     const std::string at = pre + "at wavelength " + fmt_e(wl) + ", radius " + fmt_e(r) + ": ";
     std::string msg;
@@ -1250,7 +1276,7 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                      int layers,
                      const double *d_fractions, const double *d_m, int64_t n_radius,
                      int64_t n_angles, bool greek, int64_t order, int allow_unvalidated,
-                     int kernel, EnsSetup &S) {
+                     int kernel, EnsSetup &S, const double *d_nodes = nullptr) {
     int rc;
     if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
     hipStream_t s = ctx->stream;
@@ -1309,6 +1335,7 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                         "alloc")))
         return rc;
     S.a.geom = static_cast<double2 *>(ctx->buf[scatrs_ctx::kGeom]);
+    S.a.user_rw = reinterpret_cast<const double2 *>(d_nodes);
     ens_prep<<<(unsigned)pblocks, 256, 0, s>>>(S.a, count, allow_unvalidated, S.node_n, partial);
     reduce_status<<<1, 1024, 0, s>>>(partial, pblocks, st);
     ens_offsets<<<(unsigned)count, 256, 0, s>>>(S.node_n, n_radius, S.node_off, S.ens_terms);

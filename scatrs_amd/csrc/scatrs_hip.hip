@@ -11,6 +11,7 @@
 #include "../../include/scatrs_hip.h"
 #include "checks.h"
 #include "context.h"
+#include "ensemble_common.h"
 #include "mie_device.h"
 #include "sort.h"
 #include "util.h"
@@ -259,6 +260,20 @@ int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap) {
 }
 
 int64_t scatrs_default_n_max(double x) { return (int64_t)default_n_max(x); }
+
+int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi) {
+    if (!p || !t_lo || !t_hi || !scatrs::dist_valid(kind, p)) return SCATRS_INVALID_INPUT;
+    scatrs::radius_range(kind, p, t_lo, t_hi);
+    return SCATRS_OK;
+}
+
+int scatrs_size_log_density(int32_t kind, const double *p, int64_t n, const double *r,
+                            double *ln_u) {
+    if (!p || n < 0 || (n > 0 && (!r || !ln_u)) || !scatrs::dist_valid(kind, p))
+        return SCATRS_INVALID_INPUT;
+    for (int64_t i = 0; i < n; ++i) ln_u[i] = scatrs::log_density(kind, p, r[i], nullptr);
+    return SCATRS_OK;
+}
 
 int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
                             int64_t n_max, int allow_unvalidated, double *d_values,

@@ -24,7 +24,8 @@ from .size import SEGMENT, Distribution, MultiModal, Tabulated, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
-           "assemble", "mix_parts", "number_averages", "parameter_names"]
+           "ensemble_nodes_device",
+           "assemble", "mix_parts", "mix_units", "number_averages", "parameter_names"]
 
 SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
            "effective_radius", "mean_area", "mean_volume")
@@ -69,6 +70,47 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, p2=Non
             p(fr), p(args[4]), int(n_radius), int(n_angles), -1 if order is None else int(order),
             int(bool(normalized)), int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase),
             C.byref(terms))
+    finally:
+        if mu is not None:
+            ctx.set_phase_cosines(None)
+    ctx.check(rc)
+    return scalars, greek, phase, terms.value
+
+
+def ensemble_nodes_device(nodes, wavelength, m, fractions=None, *, n_angles: int = 128,
+                          order: int | None = 32, normalized: bool = True,
+                          allow_unvalidated: bool = False, want_phase: bool = True, mu=None):
+    """Ensembles on the caller's radius nodes (scatrs_ensemble_nodes): nodes float64 (count,
+    n_nodes, 2) of (r, W) pairs, W the quadrature weight times the number density per ln r;
+    wavelength, m, fractions and the outputs as ensemble_fixed_device's."""
+    import torch
+
+    dev = nodes.device
+    count, n_nodes = nodes.shape[0], nodes.shape[1]
+    layers = 1 if m.dim() == 1 else m.shape[1]
+    if nodes.shape != (count, n_nodes, 2):
+        raise InvalidInputError("nodes must have shape (count, n_nodes, 2)")
+    if layers > 1 and (fractions is None or fractions.shape != (count, layers - 1)):
+        raise InvalidInputError("layered spheres need fractions of shape (count, L - 1)")
+    nd = nodes.to(torch.float64).contiguous()
+    wl, mm = wavelength.contiguous(), m.contiguous()
+    fr = None if layers == 1 else fractions.contiguous()
+    scalars = torch.empty((count, 11), dtype=torch.float64, device=dev)
+    greek = None if order is None else torch.empty((count, 6, order + 1), dtype=torch.float64,
+                                                   device=dev)
+    n_phase = n_angles if mu is None else int(np.size(mu))
+    phase = torch.empty((count, 6, n_phase), dtype=torch.float64, device=dev) if want_phase \
+        else None
+    ctx = _native.context(dev.index)
+    terms = C.c_int64(0)
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+    if mu is not None:
+        ctx.set_phase_cosines(mu)
+    try:
+        rc = ctx.lib.scatrs_ensemble_nodes(
+            ctx.handle, count, n_nodes, p(nd), p(wl), layers, p(fr), p(mm), int(n_angles),
+            -1 if order is None else int(order), int(bool(normalized)),
+            int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase), C.byref(terms))
     finally:
         if mu is not None:
             ctx.set_phase_cosines(None)
@@ -358,6 +400,8 @@ class EnsembleResult:
     phase: EnsemblePhase | None = None
     wavelength: np.ndarray = field(default=None)
     terms: int = 0
+    converged: np.ndarray | None = None  # the adaptive engine's (rtol given), else None
+    panels: np.ndarray | None = None
 
 
 def _frozen(a: np.ndarray) -> np.ndarray:
@@ -365,7 +409,8 @@ def _frozen(a: np.ndarray) -> np.ndarray:
     return a
 
 
-def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int, mu=None):
+def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int, mu=None,
+            converged=None, panels=None):
     """EnsembleResult of outputs with rows in the C order of ``shape``; the phase matrix at the
     cosines mu, or at the n_angles Gauss-Legendre cosines."""
     s = sc.reshape(-1, 11).cpu().numpy()
@@ -384,7 +429,51 @@ def _result(sc, gr, ph, shape: tuple, wl: np.ndarray, n_angles: int, terms: int,
         a = ph.reshape(-1, 6, mu.size).cpu().numpy()
         p = EnsemblePhase(_frozen(mu), *(_frozen(a[:, k].reshape(shape + (mu.size,)))
                                          for k in range(6)))
-    return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()), terms=terms)
+    extra = {}
+    if converged is not None:
+        extra = dict(converged=_frozen(np.asarray(converged).reshape(shape)),
+                     panels=_frozen(np.asarray(panels).reshape(shape)))
+    return EnsembleResult(*fields, greek=g, phase=p, wavelength=_frozen(wl.copy()), terms=terms,
+                          **extra)
+
+
+def mix_units(scalars, greek, phase, unit, fractions, n_units: int, normalized: bool = True):
+    """Parts' outputs with normalized=False (scalars (parts, ..., 11), greek, phase or None)
+    mixed into their units (unit (parts,) indices in [0, n_units)) with the number fractions
+    (parts,), as mix_parts mixes one unit: (n_units, ..., 11), greek, phase."""
+    import torch
+
+    dev = scalars.device
+    u = torch.as_tensor(np.asarray(unit), device=dev)
+    f = torch.as_tensor(np.asarray(fractions, np.float64), device=dev)
+    f = f.reshape(f.shape + (1,) * (scalars.dim() - 1))
+    add = lambda a, w: torch.zeros((n_units,) + a.shape[1:], dtype=a.dtype,  # noqa: E731
+                                   device=dev).index_add(0, u, w * a)
+    avg = add(number_averages(scalars), f)
+    gr = None if greek is None else add(greek, f[..., None])
+    ph = None if phase is None else add(phase, f[..., None])
+    return assemble(avg, gr, ph, normalized)
+
+
+def _adaptive(parts, unit, fractions, n_units: int, wl: np.ndarray, *, greek, normalized,
+              n_angles, rtol, max_panels, allow_unvalidated, phase, mu, refine_backscatter):
+    """The units' outputs by the adaptive engine (adaptive.parts_adaptive): Kronrod and Gauss
+    mixtures ((n_units, nw, ...) each), terms, converged and panels (n_units, nw), the phase
+    rule's size."""
+    from .adaptive import parts_adaptive
+
+    K, G, terms, conv, panels, _, na = parts_adaptive(
+        parts, wl, order=greek, n_angles=n_angles, rtol=rtol, max_panels=max_panels,
+        allow_unvalidated=allow_unvalidated, want_phase=phase, mu=mu,
+        refine_backscatter=refine_backscatter)
+    k = mix_units(*K, unit, fractions, n_units, normalized)
+    g = mix_units(*G, unit, fractions, n_units, normalized)
+    cu = np.ones((n_units, wl.size), bool)
+    pu = np.zeros((n_units, wl.size), np.int64)
+    for i, ui in enumerate(unit):
+        cu[ui] &= conv[i]
+        pu[ui] += panels[i]
+    return k, g, terms, cu, pu, na
 
 
 def _part(size, particle):
@@ -435,14 +524,35 @@ class Ensemble:
         return np.broadcast(*self.size.params()).shape
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
-                n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
-                phase: bool = True, mu=None) -> EnsembleResult:
-        """Averages over the ensemble at each wavelength on the fixed discrete path; the phase
-        matrix at the cosines ``mu`` (the reference's mu argument) or at the n_angles
-        Gauss-Legendre cosines."""
+                n_radius: int = 128, n_angles: int | None = 128, allow_unvalidated: bool = False,
+                phase: bool = True, mu=None, rtol: float | None = None, max_panels: int = 4096,
+                refine_backscatter: bool = False) -> EnsembleResult:
+        """Averages over the ensemble at each wavelength on the fixed discrete path, or with
+        ``rtol`` by the adaptive engine (scatrs_amd.adaptive: Gauss-Kronrod panels refined to
+        rtol, at most max_panels per part; the result's converged and panels; n_angles=None
+        sizes the Greek projection's rule to the largest N); the phase matrix at the cosines
+        ``mu`` (the reference's mu argument) or at the n_angles Gauss-Legendre cosines."""
         import torch
 
         wl = np.asarray(wavelength, np.float64)
+        if rtol is not None:
+            if isinstance(self.size, (MultiModal, Tabulated)):
+                parts, fr = _parts_of(self.size, self.particle)
+                unit, grid = [0] * len(parts), ()
+            else:
+                grid = self.grid_shape
+                pg = [np.broadcast_to(v, grid).reshape(-1) for v in self.size.params()]
+                parts = [(self.size.kind, float(a), float(b), float(c), self.particle)
+                         for a, b, c in zip(*pg)]
+                unit, fr = list(range(len(parts))), [1.0] * len(parts)
+            k, _, terms, cu, pu, na = _adaptive(
+                parts, unit, fr, max(unit) + 1, wl, greek=greek, normalized=normalized,
+                n_angles=n_angles, rtol=rtol, max_panels=max_panels,
+                allow_unvalidated=allow_unvalidated, phase=phase, mu=mu,
+                refine_backscatter=refine_backscatter)
+            return _result(*k, grid + wl.shape, wl, na, terms, mu, cu, pu)
+        if n_angles is None:
+            raise InvalidInputError("the fixed path needs n_angles")
         if isinstance(self.size, (MultiModal, Tabulated)):
             parts, fr = _parts_of(self.size, self.particle)
             sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
@@ -513,10 +623,12 @@ class Mixture:
         object.__setattr__(self, "components", tuple(out))
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
-                n_radius: int = 128, n_angles: int = 128, allow_unvalidated: bool = False,
-                phase: bool = True, mu=None) -> EnsembleResult:
-        """Averages over the mixture at each wavelength on the fixed discrete path (the phase
-        matrix at the cosines ``mu`` when given)."""
+                n_radius: int = 128, n_angles: int | None = 128, allow_unvalidated: bool = False,
+                phase: bool = True, mu=None, rtol: float | None = None, max_panels: int = 4096,
+                refine_backscatter: bool = False) -> EnsembleResult:
+        """Averages over the mixture at each wavelength on the fixed discrete path, or with
+        ``rtol`` by the adaptive engine (as Ensemble.compute); the phase matrix at the cosines
+        ``mu`` when given."""
         import torch
 
         wl = np.asarray(wavelength, np.float64)
@@ -528,6 +640,15 @@ class Mixture:
             p, f = _parts_of(e.size, e.particle, w / total)
             parts += p
             fr += f
+        if rtol is not None:
+            k, _, terms, cu, pu, na = _adaptive(
+                parts, [0] * len(parts), fr, 1, wl, greek=greek, normalized=normalized,
+                n_angles=n_angles, rtol=rtol, max_panels=max_panels,
+                allow_unvalidated=allow_unvalidated, phase=phase, mu=mu,
+                refine_backscatter=refine_backscatter)
+            return _result(*k, wl.shape, wl, na, terms, mu, cu, pu)
+        if n_angles is None:
+            raise InvalidInputError("the fixed path needs n_angles")
         sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                           n_angles=n_angles, allow_unvalidated=allow_unvalidated,
                                           want_phase=phase, mu=mu)

@@ -35,7 +35,7 @@ def test_abi_version_and_host_truncation():
     from scatrs_amd import _native
 
     lib = _native.load()
-    assert lib.scatrs_abi_version() == 2
+    assert lib.scatrs_abi_version() == 3
     for x in (1e-6, 0.1, 1.0, 7.3, 100.0, 999.9, 2e4):
         assert lib.scatrs_default_n_max(x) == oracle.default_n_max(x)
 

@@ -1,0 +1,146 @@
+"""The adaptive engine (scatrs_amd.adaptive) and its node evaluator scatrs_ensemble_nodes on
+the GPU: the caller's nodes reproduce the fixed path; adaptive results against the oracle's
+converged integral over the same range (2048-8192 Gauss-Legendre nodes); refinement where
+the fixed rule fails, extension of the range, the panel limit, and the wire format's
+adaptive option."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from scatrs_amd.errors import InvalidInputError
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _fixed_nodes(kind, p, n):
+    """The fixed path's nodes (r, W) from the host helpers: Gauss-Legendre in ln r over the
+    fixed range, W = w half u r."""
+    from scatrs_amd.adaptive import log_density, size_range
+
+    lo, hi = size_range(kind, p)
+    x, w = np.polynomial.legendre.leggauss(n)
+    half, mid = 0.5 * (hi - lo), 0.5 * (hi + lo)
+    r = np.exp(mid + half * x)
+    lu = log_density(kind, p, r)
+    return np.stack([r, w * half * np.exp(lu - lu.max()) * r], -1)
+
+
+def test_caller_nodes_reproduce_the_fixed_path():
+    from scatrs_amd.ensemble import ensemble_fixed_device, ensemble_nodes_device
+
+    cases = [(0, (0.3, 1.6, 0.0), 0.55, [1.45 + 0.01j], None),
+             (1, (0.5, 0.2, 0.0), 0.9, [1.52 + 1e-3j], None),
+             (2, (2.0, 3.0, 1.2), 0.7, [1.6 + 0.1j, 1.35 + 1e-5j], [0.6]),
+             (3, (3.5, 0.1, 1.5), 1.1, [1.33 + 1e-4j], None)]
+    n, na, L = 48, 40, 12
+    for kind, p, wl, m, fr in cases:
+        nodes = _fixed_nodes(kind, p, n)[None]
+        mm = np.array(m)[None] if len(m) > 1 else np.array(m)
+        f = None if fr is None else _dev(np.array([fr]))
+        a = ensemble_nodes_device(_dev(nodes), _dev(np.array([wl])), _dev(mm), f, n_angles=na,
+                                  order=L)
+        b = ensemble_fixed_device(_dev(np.array([kind], np.int32)), _dev(np.array([p[0]])),
+                                  _dev(np.array([p[1]])), _dev(np.array([wl])), _dev(mm), f,
+                                  p2=_dev(np.array([p[2]])), n_radius=n, n_angles=na, order=L)
+        for x, y in zip(a[:3], b[:3]):
+            x, y = x.cpu().numpy(), y.cpu().numpy()
+            assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(y)), kind
+        assert a[3] == b[3]
+    with pytest.raises(InvalidInputError, match="node 3: the radius must be finite"):
+        bad = _fixed_nodes(0, (0.3, 1.6, 0.0), 8)[None].copy()
+        bad[0, 3, 0] = -1.0
+        ensemble_nodes_device(_dev(bad), _dev(np.array([0.5])), _dev(np.array([1.5 + 0j])))
+
+
+def _adaptive(kind, p, wl, m, rtol, **kw):
+    from scatrs_amd.adaptive import parts_adaptive
+    from scatrs_amd.particles import Sphere
+
+    K, G, terms, conv, panels, ranges, na = parts_adaptive(
+        [(kind, *p, Sphere(m))], np.asarray(wl, np.float64), order=kw.pop("order", 16),
+        n_angles=kw.pop("n_angles", None), rtol=rtol, max_panels=kw.pop("max_panels", 4096),
+        allow_unvalidated=False, want_phase=False, **kw)
+    return K[0][0].cpu().numpy(), K[1][0].cpu().numpy(), conv[0], panels[0], ranges[0], na
+
+
+@pytest.mark.parametrize("case", [
+    (0, (0.3, 1.6, 0.0), [0.55, 1.2], 1.5 + 0.05j, 1e-8),   # absorbing lognormal
+    (0, (0.08, 1.6, 0.0), [0.55], 1.43 + 1e-8j, 1e-6),      # resonant: the fixed rule fails
+    (0, (0.002, 2.0, 0.0), [1.0], 1.5 + 1e-3j, 1e-7),       # Rayleigh: C_sca ~ r^6 tail
+    (1, (0.6, 0.25, 0.0), [0.8], 1.38 + 2e-3j, 1e-7),       # gamma
+])
+def test_adaptive_matches_the_converged_integral_over_its_range(case):
+    """normalized=False averages against the oracle's fixed path with t_range set to the
+    engine's range and 4096 nodes (8192 for the resonant case), within a few rtol."""
+    from scatrs_amd.adaptive import integration_range
+
+    kind, p, wl, m, rtol = case
+    sc, gr, conv, panels, ranges, na = _adaptive(kind, p, wl, m, rtol)
+    assert conv.all(), panels
+    n_ref = 8192 if m.imag < 1e-6 else 4096
+    for i, w in enumerate(wl):
+        t = np.log(ranges[i])
+        ref = oracle.ensemble_fixed(kind, p[0], p[1], w, m, n_radius=n_ref, n_angles=na,
+                                    order=16, normalized=False, t_range=t, p2=p[2])
+        rs = ref["scalars"]
+        for k in (0, 1, 9, 10):  # C_ext, C_sca, <area>, <volume>
+            assert abs(sc[i, k] - rs[k]) <= 5 * rtol * abs(rs[k]), (k, sc[i, k], rs[k])
+        assert abs(sc[i, 5] * sc[i, 1] - rs[5] * rs[1]) <= 5 * rtol * abs(rs[1])
+        assert abs(sc[i, 2] - rs[2]) <= 5 * rtol * abs(rs[0])
+        a1 = ref["greek"][0]
+        assert np.max(np.abs(gr[i, 0] - a1)) <= 1e-5 * np.max(np.abs(a1))
+    if m.imag < 1e-6:
+        assert panels.max() > 4  # refined beyond the initial panels
+    if p[0] == 0.002:
+        # C_sca ~ r^6 outweighs the area: the range grew past the area rule's upper end
+        _, t_hi, _ = integration_range(kind, p, rtol)
+        assert np.log(ranges[0][1]) > t_hi + 1e-9
+
+
+def test_panel_limit_leaves_the_task_unconverged():
+    sc, _, conv, panels, _, _ = _adaptive(0, (0.08, 1.6, 0.0), [0.55], 1.43 + 1e-8j, 1e-10,
+                                          max_panels=6)
+    assert not conv.any() and panels.max() <= 6
+    assert np.all(np.isfinite(sc))
+
+
+def test_ensemble_mixture_and_wire_use_the_engine():
+    """Ensemble.compute(rtol=...) and the wire format's adaptive option give the same values;
+    a monodisperse law is the sphere; a mixture converges part by part."""
+    from scatrs_amd import _wire
+    from scatrs_amd.ensemble import Ensemble, Mixture
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import Gamma, LogNormal, Monodisperse
+
+    wl = np.array([0.5, 1.0])
+    m = np.array([1.45 + 0.01j, 1.44 + 0.02j])
+    r = Ensemble(Sphere(m), LogNormal(0.2, 1.5)).compute(wl, greek=8, rtol=1e-7, mu=[-1, 0, 1])
+    assert r.converged.all() and r.panels.min() >= 4
+    raw = _wire.ensemble_compute([(("lognormal", 0.2, 1.5), ("sphere", m), 1.0)], wl, False, 8,
+                                 False, np.array([-1.0, 0.0, 1.0]), True, [], 1e-7, 4096, False,
+                                 False, adaptive=True)
+    assert np.array_equal(raw["converged"], r.converged)
+    assert np.array_equal(raw["panels"], r.panels.astype(np.uint64))
+    assert np.allclose(raw["values"][0], r.c_ext, rtol=1e-14, atol=0)
+    assert np.allclose(raw["phase"][0], r.phase.f11, rtol=1e-14, atol=0)
+    assert np.all(raw["errors"][0] <= 1e-6 * raw["values"][0])
+    with pytest.raises(InvalidInputError, match="no derivatives"):
+        _wire.ensemble_compute([(("lognormal", 0.2, 1.5), ("sphere", m), 1.0)], wl, False, 8,
+                               False, np.zeros(0), True, [0], 1e-7, 4096, False, False,
+                               adaptive=True)
+    mono = Ensemble(Sphere(m), Monodisperse(0.4)).compute(wl, rtol=1e-6)
+    q, _ = oracle.mie_efficiencies_batch(2 * np.pi * 0.4 / wl, m)
+    assert np.allclose(mono.c_ext, np.pi * 0.16 * q[0], rtol=1e-12, atol=0)
+    mix = Mixture(((2.0, Ensemble(Sphere(m), LogNormal(0.2, 1.5))),
+                   (1.0, Ensemble(Sphere(m), Gamma(0.5, 0.1))))).compute(wl, greek=8, rtol=1e-6)
+    assert mix.converged.all() and np.all(mix.panels >= 8)
+    assert np.allclose(mix.greek.alpha1[:, 0], 1.0, rtol=0, atol=1e-14)

@@ -55,3 +55,45 @@ def test_gpu_entry_points_fail_loudly_without_a_gpu():
 
     with pytest.raises((NativeLibraryError, RuntimeError, AssertionError)):
         efficiencies(np.array([1.0]), np.array([1.5 + 0j]))
+
+
+def test_kronrod15_rule_is_qk15():
+    """The 7-point Gauss part equals numpy's Gauss-Legendre rule; the 15-point Kronrod rule
+    integrates x^k exactly for k <= 22 (degree 3 n + 1 for n = 7) and not for k = 24."""
+    from scatrs_amd.adaptive import kronrod15
+
+    x, wk, wg = kronrod15()
+    gx, gw = np.polynomial.legendre.leggauss(7)
+    sel = wg > 0
+    assert np.allclose(x[sel], gx, rtol=0, atol=1e-15)
+    assert np.allclose(wg[sel], gw, rtol=0, atol=1e-15)
+    assert np.all(np.diff(x) > 0) and np.allclose(x, -x[::-1], atol=0)
+    for k in range(23):
+        exact = 2.0 / (k + 1) if k % 2 == 0 else 0.0
+        assert abs(np.sum(wk * x ** k) - exact) < 1e-15, k
+    assert abs(np.sum(wk * x ** 24) - 2.0 / 25) > 1e-9
+
+
+def test_integration_range_cuts_the_area_tails_at_rtol_over_10():
+    """A lognormal's area-weighted law is a lognormal of median r_g exp(2 s^2): its range ends
+    where the normal tail holds rtol/20, within the grid's spacing; the density is the
+    closed form's up to a constant."""
+    from scipy.stats import norm
+
+    from scatrs_amd.adaptive import integration_range, log_density, size_range
+
+    r_g, sigma, rtol = 0.3, 1.6, 1e-6
+    s = np.log(sigma)
+    lo, hi = size_range(0, (r_g, sigma, 0.0))
+    assert np.isclose(lo, np.log(r_g) - 5 * s) and np.isclose(hi, np.log(r_g) + 5 * s)
+    t_lo, t_hi, _ = integration_range(0, (r_g, sigma, 0.0), rtol)
+    c = np.log(r_g) + 2 * s * s
+    z = norm.isf(rtol / 20)
+    step = 30 * s / 8192
+    assert abs(t_lo - (c - z * s)) <= 2 * step and abs(t_hi - (c + z * s)) <= 2 * step
+    r = np.array([0.05, 0.3, 1.2])
+    lu = log_density(0, (r_g, sigma, 0.0), r)
+    ref = -np.log(r / r_g) ** 2 / (2 * s * s) - np.log(r)
+    assert np.allclose(lu - lu[1], ref - ref[1], rtol=0, atol=1e-13)
+    with pytest.raises(InvalidInputError):
+        size_range(0, (0.3, 0.9, 0.0))
