@@ -21,7 +21,9 @@ panels; every evaluation is one batched library call over all open panels of all
   A task stops unconverged when the next bisection would pass ``max_panels``.
 * Pass 2: every output (the scalars, the Greek coefficients, the phase matrix) is evaluated on
   the final panels' Kronrod nodes, and again on their Gauss nodes; the error estimate is the
-  difference of the two rules' outputs.
+  difference of the two rules' outputs. The panels go in rows of 16, grouped by the angle rule
+  their Greek projection needs to be exact, N_max + (L+1)/2 cosines at the row's largest size
+  (the reference's bands, engine.rs:945-1017), and the rows are summed by their weights.
 
 The panels are this engine's own: the reference's exact panel decisions depend on its
 evaluation order. Both engines meet rtol, so their results agree to about rtol, not bitwise.
@@ -122,7 +124,8 @@ def integration_range(kind: int, p, rtol: float) -> tuple[float, float, float]:
 @dataclass
 class Task:
     """One part at one wavelength: its distribution (kind, p), index and fractions, range and
-    panels, and its pass-1 state."""
+    panels (edges a, b and the refined sums of each rule, NaN until evaluated), and its pass-1
+    state."""
 
     kind: int
     p: np.ndarray
@@ -132,7 +135,10 @@ class Task:
     t_lo: float = 0.0
     t_hi: float = 0.0
     c: float = 0.0
-    panels: list = field(default_factory=list)  # [a, b, K sums (5,) or None, G sums (5,)]
+    a: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    b: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    K: np.ndarray = field(default_factory=lambda: np.zeros((0, 5)))
+    G: np.ndarray = field(default_factory=lambda: np.zeros((0, 5)))
     done: bool = False
     converged: bool = False
 
@@ -140,58 +146,57 @@ class Task:
     def layers(self) -> int:
         return self.m.size
 
+    @property
+    def n_panels(self) -> int:
+        return max(self.a.size, 1)
 
-def _nodes(task: Task, panels, idx):
-    """(r, W) of the panels' nodes idx of the rule (rows: the panels' nodes in order), with the
+
+def _nodes(task: Task, a, b, idx):
+    """(r, W) (len(a), len(idx)) of the panels [a, b] at the nodes idx of the rule, with the
     Kronrod weights at all 15 nodes or the Gauss weights at the 7."""
     x = _X[idx]
     w = _WK[idx] if idx.size == 15 else _WGF[idx]
-    rs, ws = [], []
-    for a, b in panels:
-        half, mid = 0.5 * (b - a), 0.5 * (a + b)
-        t = mid + half * x
-        r = np.exp(t)
-        u = np.exp(log_density(task.kind, task.p, r) + t - task.c)  # density per ln r, scaled
-        rs.append(r)
-        ws.append(w * half * u)
-    return np.concatenate(rs), np.concatenate(ws)
+    half, mid = 0.5 * (b - a), 0.5 * (a + b)
+    t = mid[:, None] + half[:, None] * x[None, :]
+    r = np.exp(t)
+    u = np.exp(log_density(task.kind, task.p, r).reshape(r.shape) + t - task.c)
+    return r, w[None, :] * half[:, None] * u
 
 
-def _call(tasks, rows, *, order=None, n_angles=2, mu=None, want_phase=False, normalized=False,
-          allow_unvalidated=False):
-    """One library call per (layers, nodes per row) over rows of (task index, r, W): the
-    outputs (scalars (rows, 11), greek, phase) in row order, and the rows' sums of W."""
+def _call(tasks, ti, nodes, *, order=None, n_angles=2, mu=None, want_phase=False,
+          normalized=False, allow_unvalidated=False):
+    """The rows (task index ti (R,), nodes (R, k, 2) of (r, W)) by one library call per layer
+    count: (scalars (R, 11), greek, phase) in row order, the rows' sums of W, and the terms."""
     import torch
 
     from .ensemble import ensemble_nodes_device
 
     dev = torch.device("cuda", torch.cuda.current_device())
-    n = len(rows)
-    groups: dict[tuple[int, int], list[int]] = {}
-    for i, (ti, r, _) in enumerate(rows):
-        groups.setdefault((tasks[ti].layers, r.size), []).append(i)
-    sc = np.empty((n, 11))
-    gr = None if order is None else np.empty((n, 6, order + 1))
+    R = ti.size
+    layers = np.array([tasks[i].layers for i in ti])
+    sc = np.empty((R, 11))
+    gr = None if order is None else np.empty((R, 6, order + 1))
     n_phase = n_angles if mu is None else int(np.size(mu))
-    ph = np.empty((n, 6, n_phase)) if want_phase else None
-    z = np.array([w.sum() for _, _, w in rows])
+    ph = np.empty((R, 6, n_phase)) if want_phase else None
+    z = nodes[..., 1].sum(1)
     terms = 0
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
-    for (L, k), idx in groups.items():
-        nodes = np.stack([np.stack([rows[i][1], rows[i][2]], -1) for i in idx])  # (g, k, 2)
-        tk = [tasks[rows[i][0]] for i in idx]
-        wl = np.array([q.wl for q in tk])
-        m = np.stack([q.m for q in tk])  # (g, L) complex
-        fr = np.stack([q.fr for q in tk]) if L > 1 else None
+    wl_all = np.array([q.wl for q in tasks])
+    for L in np.unique(layers):
+        sel = np.flatnonzero(layers == L)
+        tk = ti[sel]
+        m = np.stack([tasks[i].m for i in tk])  # (g, L) complex
+        fr = np.stack([tasks[i].fr for i in tk]) if L > 1 else None
         s, g, p, nt = ensemble_nodes_device(
-            t(nodes), t(wl), t(m[:, 0] if L == 1 else m), None if fr is None else t(fr),
-            n_angles=n_angles, order=order, normalized=normalized,
-            allow_unvalidated=allow_unvalidated, want_phase=want_phase, mu=mu)
-        sc[idx] = s.cpu().numpy()
+            t(nodes[sel]), t(wl_all[tk]), t(m[:, 0] if L == 1 else m),
+            None if fr is None else t(fr), n_angles=n_angles, order=order,
+            normalized=normalized, allow_unvalidated=allow_unvalidated, want_phase=want_phase,
+            mu=mu)
+        sc[sel] = s.cpu().numpy()
         if gr is not None:
-            gr[idx] = g.cpu().numpy()
+            gr[sel] = g.cpu().numpy()
         if ph is not None:
-            ph[idx] = p.cpu().numpy()
+            ph[sel] = p.cpu().numpy()
         terms += nt
     return sc, gr, ph, z, terms
 
@@ -212,6 +217,10 @@ def _targets(total, rtol, refine_backscatter):
     return tgt
 
 
+def _nan(n):
+    return np.full((n, 5), np.nan)
+
+
 def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
           allow_unvalidated: bool) -> int:
     """Places every task's panels (see the module's notes); returns the Mie terms evaluated."""
@@ -222,64 +231,65 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             continue
         q.t_lo, q.t_hi, q.c = integration_range(q.kind, q.p, rtol)
         edges = np.linspace(q.t_lo, q.t_hi, _INITIAL_PANELS + 1)
-        q.panels = [[a, b, None, None] for a, b in zip(edges[:-1], edges[1:])]
-    extendable = lambda q: q.kind not in (POWER_LAW, SEGMENT)  # noqa: E731
+        q.a, q.b = edges[:-1].copy(), edges[1:].copy()
+        q.K, q.G = _nan(_INITIAL_PANELS), _nan(_INITIAL_PANELS)
     for _ in range(_MAX_ROUNDS):
-        open_ = [(ti, pi) for ti, q in enumerate(tasks) if not q.done
-                 for pi, pn in enumerate(q.panels) if pn[2] is None]
-        if open_:
-            for idx, slot in ((np.arange(15), 2), (_GAUSS, 3)):
-                rows = []
-                for ti, pi in open_:
-                    r, w = _nodes(tasks[ti], [tasks[ti].panels[pi][:2]], idx)
-                    rows.append((ti, r, w))
-                sc, _, _, z, nt = _call(tasks, rows, allow_unvalidated=allow_unvalidated)
+        todo = [(i, np.flatnonzero(np.isnan(q.K[:, 0]))) for i, q in enumerate(tasks)
+                if not q.done]
+        todo = [(i, j) for i, j in todo if j.size]
+        if todo:
+            ti = np.concatenate([np.full(j.size, i) for i, j in todo])
+            for idx, name in ((np.arange(15), "K"), (_GAUSS, "G")):
+                nodes = np.concatenate([np.stack(_nodes(tasks[i], tasks[i].a[j], tasks[i].b[j],
+                                                        idx), -1) for i, j in todo])
+                sc, _, _, z, nt = _call(tasks, ti, nodes, allow_unvalidated=allow_unvalidated)
                 terms += nt
-                s = _refined(sc, z)
-                for (ti, pi), v in zip(open_, s):
-                    tasks[ti].panels[pi][slot] = v
+                v = _refined(sc, z)
+                k = 0
+                for i, j in todo:
+                    getattr(tasks[i], name)[j] = v[k:k + j.size]
+                    k += j.size
         active = False
         for q in tasks:
             if q.done:
                 continue
-            K = np.array([pn[2] for pn in q.panels])
-            E = np.abs(K - np.array([pn[3] for pn in q.panels]))
-            total = K.sum(0)
-            tgt = _targets(total, rtol, refine_backscatter)
+            E = np.abs(q.K - q.G)
+            tgt = _targets(q.K.sum(0), rtol, refine_backscatter)
             fail = E.sum(0) > tgt
             grow_lo = grow_hi = False
-            if extendable(q):
+            if q.kind not in (POWER_LAW, SEGMENT):
                 edge = 0.1 * tgt
-                grow_lo = bool(np.any(np.abs(K[0]) > edge))
-                grow_hi = bool(np.any(np.abs(K[-1]) > edge))
+                grow_lo = bool(np.any(np.abs(q.K[0]) > edge))
+                grow_hi = bool(np.any(np.abs(q.K[-1]) > edge))
             if not fail.any() and not grow_lo and not grow_hi:
                 q.done = q.converged = True
                 continue
-            cut = set()
+            cut = np.zeros(q.a.size, bool)
             if fail.any():
-                share = tgt / len(q.panels)
-                over = (E > share) & fail
-                cut = set(np.flatnonzero(over.any(1)))
-                cut.add(int(np.argmax((E / np.where(tgt > 0, tgt, 1.0))[:, fail].max(1))))
-            if len(q.panels) + len(cut) + grow_lo + grow_hi > max_panels:
+                ratio = E[:, fail] / np.where(tgt[fail] > 0, tgt[fail], 1.0)
+                cut = (ratio > 1.0 / q.a.size).any(1)
+                cut[int(np.argmax(ratio.max(1)))] = True
+            if q.a.size + int(cut.sum()) + grow_lo + grow_hi > max_panels:
                 q.done = True
                 continue
-            new = []
+            mid = 0.5 * (q.a + q.b)
+            keep = ~cut
+            a = np.concatenate([q.a[keep], q.a[cut], mid[cut]])
+            b = np.concatenate([q.b[keep], mid[cut], q.b[cut]])
+            K = np.concatenate([q.K[keep], _nan(2 * int(cut.sum()))])
+            G = np.concatenate([q.G[keep], _nan(2 * int(cut.sum()))])
             if grow_lo:
-                a, b = q.panels[0][:2]
-                new.append([2 * a - b, a, None, None])
-                q.t_lo = 2 * a - b
-            for pi, pn in enumerate(q.panels):
-                if pi in cut:
-                    mid = 0.5 * (pn[0] + pn[1])
-                    new += [[pn[0], mid, None, None], [mid, pn[1], None, None]]
-                else:
-                    new.append(pn)
+                w = q.b[0] - q.a[0]
+                a, b = np.append(a, q.t_lo - w), np.append(b, q.t_lo)
+                K, G = np.concatenate([K, _nan(1)]), np.concatenate([G, _nan(1)])
+                q.t_lo -= w
             if grow_hi:
-                a, b = q.panels[-1][:2]
-                new.append([b, 2 * b - a, None, None])
-                q.t_hi = 2 * b - a
-            q.panels = new
+                w = q.b[-1] - q.a[-1]
+                a, b = np.append(a, q.t_hi), np.append(b, q.t_hi + w)
+                K, G = np.concatenate([K, _nan(1)]), np.concatenate([G, _nan(1)])
+                q.t_hi += w
+            order = np.argsort(a, kind="stable")
+            q.a, q.b, q.K, q.G = a[order], b[order], K[order], G[order]
             active = True
         if not active:
             break
@@ -288,38 +298,113 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
     return terms
 
 
+_ROW_PANELS = 16  # panels per pass-2 row
+
+
+def _rule_ladder(n: int) -> int:
+    """An angle rule's size: the next of 16, 20, 25, ... (steps of 1.25x), at most 4096."""
+    k = 16
+    while k < n:
+        k = int(np.ceil(k * 1.25))
+    return min(k, 4096)
+
+
+def _row_rules(tasks, rows, order, n_angles, single: bool):
+    """The Greek projection's rule of each pass-2 row: the caller's n_angles, or the smallest
+    exact for the row's nodes, N_max + (order + 1)/2 cosines (on a ladder, so that rows share
+    calls; one rule for all rows when ``single``, for the phase at the rule's cosines)."""
+    if order is None:
+        return np.full(len(rows), 2 if n_angles is None else int(n_angles))
+    if n_angles is not None:
+        return np.full(len(rows), int(n_angles))
+    lib = _native.load()
+    need = np.empty(len(rows), np.int64)
+    for k, (ti, lo, hi) in enumerate(rows):
+        q = tasks[ti]
+        r = q.p[0] if lo < 0 else np.exp(q.b[hi - 1])
+        n = int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl))
+        need[k] = _rule_ladder(n + (order + 2) // 2)
+    if single:
+        need[:] = need.max()
+    return need
+
+
+def _averages(sc):
+    """The linear averages [C_ext, C_sca, C_abs, C_back, C_sca g, r^2, r^3] of normalized=False
+    scalars (numpy)."""
+    return np.stack([sc[:, 0], sc[:, 1], sc[:, 2], sc[:, 3], sc[:, 5] * sc[:, 1],
+                     sc[:, 9] / np.pi, sc[:, 10] * (3.0 / (4.0 * np.pi))], 1)
+
+
+def _scalars(avg):
+    """normalized=False scalars from the linear averages (the inverse of _averages)."""
+    ext, sca, absb, back, scag, r2, r3 = avg.T
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.stack([ext, sca, absb, back, sca / ext, scag / sca, 4.0 * np.pi * ext / back,
+                         np.zeros_like(ext), r3 / r2, np.pi * r2, 4.0 / 3.0 * np.pi * r3], 1)
+
+
 def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
     """Every output of every task on its final panels (normalized=False), by the Kronrod and
-    by the Gauss rule: ((sc, gr, ph) Kronrod, (sc, gr, ph) Gauss, terms)."""
+    by the Gauss rule: ((sc, gr, ph) Kronrod, (sc, gr, ph) Gauss, terms, the phase rule's size).
+
+    A task's panels go in rows of 16 (the last padded with nodes of weight 0), so that every
+    call has many rows of one size; rows are grouped by their Greek rule (the smallest exact
+    one for their nodes unless the caller fixes n_angles), one call per rule. The rows'
+    unnormalized outputs are linear in their sums, so a task's are the rows' weighted by their
+    sums of weights."""
+    rows = []
+    for ti, q in enumerate(tasks):
+        if q.kind == MONODISPERSE:
+            rows.append((ti, -1, -1))
+            continue
+        for lo in range(0, q.a.size, _ROW_PANELS):
+            rows.append((ti, lo, min(lo + _ROW_PANELS, q.a.size)))
+    single = want_phase and mu is None
+    rules = _row_rules(tasks, rows, order, n_angles, single)
+    ti_rows = np.array([r[0] for r in rows])
+    T = len(tasks)
     out = []
     terms = 0
     for idx in (np.arange(15), _GAUSS):
-        rows = []
-        for ti, q in enumerate(tasks):
-            if q.kind == MONODISPERSE:
-                rows.append((ti, np.array([q.p[0]]), np.array([1.0])))
-            else:
-                r, w = _nodes(q, [pn[:2] for pn in q.panels], idx)
-                rows.append((ti, r, w))
-        sc, gr, ph, _, nt = _call(tasks, rows, order=order, n_angles=n_angles, mu=mu,
-                                  want_phase=want_phase, normalized=False,
-                                  allow_unvalidated=allow_unvalidated)
-        out.append((sc, gr, ph))
-        terms += nt
-    return out[0], out[1], terms
-
-
-def greek_angles(tasks, order: int, n_angles: int | None) -> int:
-    """The angle rule of the Greek projection: the caller's, or the smallest exact for every
-    node, N_max + (order + 1)/2 cosines at the tasks' largest size parameter (at most 4096)."""
-    if n_angles is not None:
-        return int(n_angles)
-    lib = _native.load()
-    n_max = 1
-    for q in tasks:
-        r = q.p[0] if q.kind == MONODISPERSE else np.exp(q.t_hi)
-        n_max = max(n_max, int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl)))
-    return int(min(4096, max(2, n_max + (order + 2) // 2)))
+        k = idx.size
+        nodes = np.zeros((len(rows), _ROW_PANELS * k, 2))
+        for j, (ti, lo, hi) in enumerate(rows):
+            q = tasks[ti]
+            if lo < 0:
+                nodes[j, :, 0] = q.p[0]
+                nodes[j, 0, 1] = 1.0
+                continue
+            r, w = _nodes(q, q.a[lo:hi], q.b[lo:hi], idx)
+            n = r.size
+            nodes[j, :n, 0], nodes[j, :n, 1] = r.reshape(-1), w.reshape(-1)
+            nodes[j, n:, 0] = r.reshape(-1)[-1]
+        z = nodes[..., 1].sum(1)
+        acc = np.zeros((T, 7))
+        gr = None if order is None else np.zeros((T, 6, order + 1))
+        n_phase = (int(rules[0]) if mu is None else int(np.size(mu))) if want_phase else 0
+        ph = np.zeros((T, 6, n_phase)) if want_phase else None
+        zt = np.zeros(T)
+        for na in np.unique(rules):
+            sel = np.flatnonzero(rules == na)
+            sc, g, p, _, nt = _call(tasks, ti_rows[sel], nodes[sel], order=order,
+                                    n_angles=int(na), mu=mu, want_phase=want_phase,
+                                    normalized=False, allow_unvalidated=allow_unvalidated)
+            terms += nt
+            zs = z[sel]
+            live = zs > 0.0
+            t, zl = ti_rows[sel][live], zs[live]
+            np.add.at(acc, t, zl[:, None] * _averages(sc[live]))
+            np.add.at(zt, t, zl)
+            if gr is not None:
+                np.add.at(gr, t, zl[:, None, None] * g[live])
+            if ph is not None:
+                np.add.at(ph, t, zl[:, None, None] * p[live])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            zi = 1.0 / zt
+        out.append((_scalars(acc * zi[:, None]), None if gr is None else gr * zi[:, None, None],
+                    None if ph is None else ph * zi[:, None, None]))
+    return out[0], out[1], terms, int(rules[0]) if len(rules) else 2
 
 
 def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_panels: int,
@@ -344,13 +429,8 @@ def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_p
             tasks.append(Task(int(kind), np.array([p0, p1, p2], np.float64),
                               float(wl.reshape(-1)[i]), m[:, i].copy(), fr))
     terms = pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
-    na = n_angles
-    if order is not None:
-        na = greek_angles(tasks, order, n_angles)
-    elif na is None:
-        na = 2
-    K, G, t2 = pass2(tasks, order=order, n_angles=na, mu=mu, want_phase=want_phase,
-                     allow_unvalidated=allow_unvalidated)
+    K, G, t2, na = pass2(tasks, order=order, n_angles=n_angles, mu=mu, want_phase=want_phase,
+                         allow_unvalidated=allow_unvalidated)
     dev = torch.device("cuda", torch.cuda.current_device())
     P = len(parts)
 
@@ -361,7 +441,7 @@ def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_p
         return f(sc), f(gr), f(ph)
 
     conv = np.array([q.converged for q in tasks]).reshape(P, nw)
-    panels = np.array([max(len(q.panels), 1) for q in tasks]).reshape(P, nw)
+    panels = np.array([q.n_panels for q in tasks]).reshape(P, nw)
     ranges = np.array([(q.p[0], q.p[0]) if q.kind == MONODISPERSE
                        else (np.exp(q.t_lo), np.exp(q.t_hi)) for q in tasks]).reshape(P, nw, 2)
     return shape(K), shape(G), terms + t2, conv, panels, ranges, na

@@ -1283,21 +1283,23 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     const int h = (int)((n_angles + 1) / 2);
     S.h = h;
     // The Gauss-Legendre rules (device copies in kMisc, kMisc2) and the Wigner d table (kMisc3)
-    // are kept in the context between calls of the same sizes.
-    if (ctx->rule_nr != n_radius || ctx->rule_na != n_angles) {
+    // are kept in the context between calls of the same sizes. The caller's nodes need no
+    // radius rule (a one-node stand-in: the host rule costs O(n^2)).
+    const int64_t nr_rule = d_nodes ? 1 : n_radius;
+    if (ctx->rule_nr != nr_rule || ctx->rule_na != n_angles) {
         ctx->rule_nr = ctx->rule_na = -1;
-        gauss_legendre(n_radius, ctx->rr_nodes, ctx->rr_weights);
+        gauss_legendre(nr_rule, ctx->rr_nodes, ctx->rr_weights);
         gauss_legendre(n_angles, ctx->ar_nodes, ctx->ar_weights);
-        std::vector<double> rtab(2 * n_radius);
+        std::vector<double> rtab(2 * nr_rule);
         std::copy(ctx->rr_nodes.begin(), ctx->rr_nodes.end(), rtab.begin());
-        std::copy(ctx->rr_weights.begin(), ctx->rr_weights.end(), rtab.begin() + n_radius);
+        std::copy(ctx->rr_weights.begin(), ctx->rr_weights.end(), rtab.begin() + nr_rule);
         std::vector<double> atab(n_angles + h);  // weights, then the non-negative cosines
         std::copy(ctx->ar_weights.begin(), ctx->ar_weights.end(), atab.begin());
         for (int jj = 0; jj < h; ++jj) atab[n_angles + jj] = ctx->ar_nodes[n_angles - h + jj];
         double *dr = nullptr, *da = nullptr;
         if ((rc = upload(ctx, scatrs_ctx::kMisc, rtab, &dr))) return rc;
         if ((rc = upload(ctx, scatrs_ctx::kMisc2, atab, &da))) return rc;
-        ctx->rule_nr = n_radius;
+        ctx->rule_nr = nr_rule;
         ctx->rule_na = n_angles;
         ctx->dtab_na = ctx->dtab_order = -1;
     }
@@ -1308,7 +1310,7 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     S.d_rtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc]);
     S.d_atab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc2]);
     S.a = EnsArgs{d_kind, d_p0, d_p1, d_wavelength, d_fractions, d_m, layers, n_radius,
-                  S.d_rtab, S.d_rtab + n_radius, d_p2};
+                  S.d_rtab, S.d_rtab + nr_rule, d_p2};
     // Per-node N and status.
     const int64_t total = count * n_radius;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, sizeof(Status) + 64), "alloc")))

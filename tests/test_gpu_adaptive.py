@@ -89,15 +89,18 @@ def test_adaptive_matches_the_converged_integral_over_its_range(case):
     n_ref = 8192 if m.imag < 1e-6 else 4096
     for i, w in enumerate(wl):
         t = np.log(ranges[i])
-        ref = oracle.ensemble_fixed(kind, p[0], p[1], w, m, n_radius=n_ref, n_angles=na,
+        # an angle rule exact for every node (the engine's rules are, band by band)
+        nq = oracle.default_n_max(2 * np.pi * ranges[i][1] / w) + 9
+        ref = oracle.ensemble_fixed(kind, p[0], p[1], w, m, n_radius=n_ref, n_angles=nq,
                                     order=16, normalized=False, t_range=t, p2=p[2])
         rs = ref["scalars"]
         for k in (0, 1, 9, 10):  # C_ext, C_sca, <area>, <volume>
             assert abs(sc[i, k] - rs[k]) <= 5 * rtol * abs(rs[k]), (k, sc[i, k], rs[k])
         assert abs(sc[i, 5] * sc[i, 1] - rs[5] * rs[1]) <= 5 * rtol * abs(rs[1])
         assert abs(sc[i, 2] - rs[2]) <= 5 * rtol * abs(rs[0])
-        a1 = ref["greek"][0]
-        assert np.max(np.abs(gr[i, 0] - a1)) <= 1e-5 * np.max(np.abs(a1))
+        for c in range(6):  # every Greek coefficient, exact projections on both sides
+            ref_c = ref["greek"][c]
+            assert np.max(np.abs(gr[i, c] - ref_c)) <= 20 * rtol * np.max(np.abs(ref["greek"][0]))
     if m.imag < 1e-6:
         assert panels.max() > 4  # refined beyond the initial panels
     if p[0] == 0.002:
