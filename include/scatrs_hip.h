@@ -174,11 +174,12 @@ int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes, const
                           double *d_phase, int64_t *total_terms);
 
 /* Host helpers of the adaptive engine, with the kernels' definitions: the fixed ln r range of
- * a distribution (p[3] as the SCATRS_* kinds list them), and ln u(r) of its number density per
- * unit radius, up to a constant, at n radii (distribution.rs:366-430; the density per ln r is
- * u r). SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. */
+ * a distribution (p[3] as the SCATRS_* kinds list them), and ln u(r) of the number density per
+ * unit radius, up to a constant, at n points, each with its own distribution (kind[i],
+ * p[3 i .. 3 i + 2]) and radius r[i] (distribution.rs:366-430; the density per ln r is u r).
+ * SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. */
 int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi);
-int scatrs_size_log_density(int32_t kind, const double *p, int64_t n, const double *r,
+int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
                             double *ln_u);
 
 #ifdef __cplusplus

@@ -89,8 +89,8 @@ def load() -> C.CDLL:
                                                 i32, vp, vp, vp, C.POINTER(i64)]
         if hasattr(L, "scatrs_size_range"):
             L.scatrs_size_range.argtypes = [i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
-            L.scatrs_size_log_density.argtypes = [i32, C.POINTER(dbl), i64, C.POINTER(dbl),
-                                                  C.POINTER(dbl)]
+            L.scatrs_size_log_density.argtypes = [i64, C.POINTER(C.c_int32), C.POINTER(dbl),
+                                                  C.POINTER(dbl), C.POINTER(dbl)]
         if L.scatrs_abi_version() != ABI_VERSION:
             raise NativeLibraryError("libscatrs_hip.so has a different ABI version")
         _lib = L

@@ -56,7 +56,7 @@ _WG = np.array([0.129484966168869693270611432679082, 0.2797053914892766679014677
 
 _INITIAL_PANELS = 4
 _MAX_ROUNDS = 64
-_GRID = 8193
+_GRID = 2049
 
 
 def kronrod15():
@@ -88,14 +88,17 @@ def size_range(kind: int, p) -> tuple[float, float]:
     return lo.value, hi.value
 
 
-def log_density(kind: int, p, r) -> np.ndarray:
-    """ln u(r), the number density per unit radius up to a constant (scatrs_size_log_density)."""
-    q = np.ascontiguousarray(p, np.float64)
+def log_density(kind, p, r) -> np.ndarray:
+    """ln u(r), the number density per unit radius up to a constant (scatrs_size_log_density):
+    kind a scalar or one per radius, p (3,) or (n, 3), r (n,)."""
     r = np.ascontiguousarray(r, np.float64).reshape(-1)
+    n = r.size
+    k = np.ascontiguousarray(np.broadcast_to(np.asarray(kind, np.int32), (n,)))
+    q = np.ascontiguousarray(np.broadcast_to(np.asarray(p, np.float64).reshape(-1, 3), (n, 3)))
     out = np.empty_like(r)
-    if _native.load().scatrs_size_log_density(int(kind), _dbl(q), r.size, _dbl(r),
-                                              _dbl(out)) != 0:
-        raise InvalidInputError(f"invalid parameters {tuple(q)} of size distribution kind {kind}")
+    if _native.load().scatrs_size_log_density(n, k.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              _dbl(q), _dbl(r), _dbl(out)) != 0:
+        raise InvalidInputError("invalid parameters of a size distribution")
     return out
 
 
@@ -151,15 +154,21 @@ class Task:
         return max(self.a.size, 1)
 
 
-def _nodes(task: Task, a, b, idx):
-    """(r, W) (len(a), len(idx)) of the panels [a, b] at the nodes idx of the rule, with the
-    Kronrod weights at all 15 nodes or the Gauss weights at the 7."""
+def _nodes(tasks, ti, a, b, idx):
+    """(r, W) (len(a), len(idx)) of the panels [a, b] (each of task ti) at the nodes idx of the
+    rule, with the Kronrod weights at all 15 nodes or the Gauss weights at the 7; a panel of
+    width 0 (padding) has weights 0. One density call for all panels."""
     x = _X[idx]
     w = _WK[idx] if idx.size == 15 else _WGF[idx]
     half, mid = 0.5 * (b - a), 0.5 * (a + b)
     t = mid[:, None] + half[:, None] * x[None, :]
     r = np.exp(t)
-    u = np.exp(log_density(task.kind, task.p, r).reshape(r.shape) + t - task.c)
+    kind = np.array([tasks[i].kind for i in ti], np.int32)
+    p = np.stack([tasks[i].p for i in ti]) if len(ti) else np.zeros((0, 3))
+    c = np.array([tasks[i].c for i in ti])
+    k = idx.size
+    lu = log_density(np.repeat(kind, k), np.repeat(p, k, axis=0), r.reshape(-1))
+    u = np.exp(lu.reshape(r.shape) + t - c[:, None])
     return r, w[None, :] * half[:, None] * u
 
 
@@ -239,9 +248,10 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
         todo = [(i, j) for i, j in todo if j.size]
         if todo:
             ti = np.concatenate([np.full(j.size, i) for i, j in todo])
+            pa = np.concatenate([tasks[i].a[j] for i, j in todo])
+            pb = np.concatenate([tasks[i].b[j] for i, j in todo])
             for idx, name in ((np.arange(15), "K"), (_GAUSS, "G")):
-                nodes = np.concatenate([np.stack(_nodes(tasks[i], tasks[i].a[j], tasks[i].b[j],
-                                                        idx), -1) for i, j in todo])
+                nodes = np.stack(_nodes(tasks, ti, pa, pb, idx), -1)
                 sc, _, _, z, nt = _call(tasks, ti, nodes, allow_unvalidated=allow_unvalidated)
                 terms += nt
                 v = _refined(sc, z)
@@ -366,19 +376,26 @@ def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
     T = len(tasks)
     out = []
     terms = 0
+    # Each row's panels, padded with panels of width 0 at its last edge (weights 0).
+    R = len(rows)
+    A = np.zeros((R, _ROW_PANELS))
+    Bd = np.zeros((R, _ROW_PANELS))
+    mono = np.zeros(R, bool)
+    for j, (ti, lo, hi) in enumerate(rows):
+        q = tasks[ti]
+        if lo < 0:
+            mono[j] = True
+            A[j] = Bd[j] = np.log(q.p[0])
+            continue
+        n = hi - lo
+        A[j, :n], Bd[j, :n] = q.a[lo:hi], q.b[lo:hi]
+        A[j, n:] = Bd[j, n:] = q.b[hi - 1]
     for idx in (np.arange(15), _GAUSS):
         k = idx.size
-        nodes = np.zeros((len(rows), _ROW_PANELS * k, 2))
-        for j, (ti, lo, hi) in enumerate(rows):
-            q = tasks[ti]
-            if lo < 0:
-                nodes[j, :, 0] = q.p[0]
-                nodes[j, 0, 1] = 1.0
-                continue
-            r, w = _nodes(q, q.a[lo:hi], q.b[lo:hi], idx)
-            n = r.size
-            nodes[j, :n, 0], nodes[j, :n, 1] = r.reshape(-1), w.reshape(-1)
-            nodes[j, n:, 0] = r.reshape(-1)[-1]
+        r, w = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1), idx)
+        nodes = np.stack([r.reshape(R, -1), w.reshape(R, -1)], -1)
+        nodes[mono, :, 1] = 0.0
+        nodes[mono, 0, 1] = 1.0  # a monodisperse law: its radius, weight 1
         z = nodes[..., 1].sum(1)
         acc = np.zeros((T, 7))
         gr = None if order is None else np.zeros((T, 6, order + 1))

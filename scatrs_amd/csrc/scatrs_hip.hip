@@ -267,11 +267,13 @@ int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi)
     return SCATRS_OK;
 }
 
-int scatrs_size_log_density(int32_t kind, const double *p, int64_t n, const double *r,
+int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
                             double *ln_u) {
-    if (!p || n < 0 || (n > 0 && (!r || !ln_u)) || !scatrs::dist_valid(kind, p))
-        return SCATRS_INVALID_INPUT;
-    for (int64_t i = 0; i < n; ++i) ln_u[i] = scatrs::log_density(kind, p, r[i], nullptr);
+    if (n < 0 || (n > 0 && (!kind || !p || !r || !ln_u))) return SCATRS_INVALID_INPUT;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!scatrs::dist_valid(kind[i], p + 3 * i)) return SCATRS_INVALID_INPUT;
+        ln_u[i] = scatrs::log_density(kind[i], p + 3 * i, r[i], nullptr);
+    }
     return SCATRS_OK;
 }
 

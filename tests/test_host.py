@@ -80,7 +80,7 @@ def test_integration_range_cuts_the_area_tails_at_rtol_over_10():
     closed form's up to a constant."""
     from scipy.stats import norm
 
-    from scatrs_amd.adaptive import integration_range, log_density, size_range
+    from scatrs_amd.adaptive import _GRID, integration_range, log_density, size_range
 
     r_g, sigma, rtol = 0.3, 1.6, 1e-6
     s = np.log(sigma)
@@ -89,7 +89,7 @@ def test_integration_range_cuts_the_area_tails_at_rtol_over_10():
     t_lo, t_hi, _ = integration_range(0, (r_g, sigma, 0.0), rtol)
     c = np.log(r_g) + 2 * s * s
     z = norm.isf(rtol / 20)
-    step = 30 * s / 8192
+    step = 30 * s / (_GRID - 1)  # the grid spans three times the fixed range, +-15 s
     assert abs(t_lo - (c - z * s)) <= 2 * step and abs(t_hi - (c + z * s)) <= 2 * step
     r = np.array([0.05, 0.3, 1.2])
     lu = log_density(0, (r_g, sigma, 0.0), r)
