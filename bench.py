@@ -421,6 +421,65 @@ def bench_distributions(args, torch):
                          "mixtures": cnt, "s_per_step": msec, "mixtures_per_s": cnt / msec}}
 
 
+def bench_adaptive(args, torch):
+    """The adaptive engine (scatrs_amd.adaptive) on the reference's own published ensemble
+    workloads, best of 3 calls each as the reference times them: a lognormal (r_g = 1 um,
+    sigma_g = 2, m = 1.33 + 1e-4 i) at rtol 1e-6 per wavelength, without and with the Greek
+    coefficients to 32 (docs/theory/ensembles.md:431-438); W1, the same lognormal at 0.3, 0.5
+    and 1 um in one call, Greek to 63, rtol 3e-4; W2, 50 lognormals x 20 wavelengths, m = 1.43
+    + 1e-8 i, Greek to 63, rtol 3e-4 (docs/validation/sasktran2.md:41-62). The published
+    numbers are the reference's on an Apple M4 Pro, all 12 cores."""
+    from scatrs_amd import _native
+    from scatrs_amd.ensemble import Ensemble
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import LogNormal
+
+    ctx = _native.context(0)
+
+    def best(ens, wl, **kw):
+        ens.compute(wl, **kw)
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = ens.compute(wl, **kw)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return min(times), r
+
+    e = Ensemble(Sphere(1.33 + 1e-4j), LogNormal(1.0, 2.0))
+    rg = np.logspace(np.log10(0.05), 0.0, 10)
+    RG, SG = np.meshgrid(rg, np.array([1.2, 1.4, 1.6, 1.8, 2.0]), indexing="ij")
+    e2 = Ensemble(Sphere(1.43 + 1e-8j), LogNormal(RG.reshape(-1), SG.reshape(-1)))
+    cases = []
+    for wl, pub_s, pub_g in ((1.0, 0.037, 0.093), (0.5, 0.11, 0.31), (0.3, 0.20, 0.90)):
+        cases.append((f"lognormal lambda={wl} rtol 1e-6", e, np.array([wl]),
+                      dict(rtol=1e-6, phase=False), pub_s))
+        cases.append((f"lognormal lambda={wl} rtol 1e-6 Greek to 32", e, np.array([wl]),
+                      dict(rtol=1e-6, greek=32, phase=False), pub_g))
+    cases.append(("W1: lognormal at 0.3, 0.5, 1 um, Greek to 63, rtol 3e-4", e,
+                  np.array([0.3, 0.5, 1.0]), dict(rtol=3e-4, greek=63, phase=False), 0.0838))
+    cases.append(("W2: 50 lognormals x 20 wavelengths, Greek to 63, rtol 3e-4", e2,
+                  np.linspace(0.28, 1.02, 20), dict(rtol=3e-4, greek=63, phase=False), 0.83))
+    out = []
+    for name, ens, wl, kw, pub in cases:
+        row = {"workload": name, "published_s_m4pro_12_cores": pub}
+        for kernel, key in ((0, "fp64"), (2, "fp16_split")):
+            ctx.set_angular_kernel(kernel)
+            try:
+                sec, r = best(ens, wl, **kw)
+            finally:
+                ctx.set_angular_kernel(0)
+            row[key] = {"s": sec, "speedup_vs_published": pub / sec}
+        row["panels"] = int(np.sum(r.panels))
+        row["converged"] = bool(np.all(r.converged))
+        log(f"adaptive {name}: {row['fp64']['s']:.3f} s FP64 ({pub / row['fp64']['s']:.2f}x the "
+            f"published {pub} s), {row['fp16_split']['s']:.3f} s fp16-split, {row['panels']} "
+            f"panels, converged {row['converged']}")
+        out.append(row)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,6 +525,7 @@ def main() -> int:
         res["adjoint"] = bench_adjoint(args, torch)
         res["layered"] = bench_layered(args, torch)
         res["distributions"] = bench_distributions(args, torch)
+        res["adaptive"] = bench_adaptive(args, torch)
     print(json.dumps(res), flush=True)
     return 0
 
