@@ -173,14 +173,30 @@ int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes, const
                           int allow_unvalidated, double *d_scalars, double *d_greek,
                           double *d_phase, int64_t *total_terms);
 
+/* Reverse mode of scatrs_ensemble_nodes, as scatrs_ensemble_fixed_vjp's with the range held
+ * fixed: the nodes stay where they are, and the parameters act through each node's ln W, whose
+ * derivatives d_dlnw (count, n_nodes, 3) the caller gives (d ln u(r) / d p at the node for
+ * W = w u(r) r; may be NULL: columns 0-2 zero). d_grad (count, P) as scatrs_ensemble_fixed_vjp's:
+ * columns 0-2 sum phi_i d ln W_i / d p over the nodes, phi_i the cotangent of ln W_i; then
+ * Re m, Im m per layer and the radius fractions. The adaptive engine's derivatives
+ * (engine.rs:1111-1123,1793-1804) on its final panels. */
+int scatrs_ensemble_nodes_vjp(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
+                              const double *d_nodes, const double *d_dlnw,
+                              const double *d_wavelength, int layers, const double *d_fractions,
+                              const double *d_m, int64_t n_angles, int64_t order, int normalized,
+                              int allow_unvalidated, const double *d_bar_scalars,
+                              const double *d_bar_greek, const double *d_bar_phase,
+                              double *d_grad, int64_t *total_terms);
+
 /* Host helpers of the adaptive engine, with the kernels' definitions: the fixed ln r range of
  * a distribution (p[3] as the SCATRS_* kinds list them), and ln u(r) of the number density per
  * unit radius, up to a constant, at n points, each with its own distribution (kind[i],
- * p[3 i .. 3 i + 2]) and radius r[i] (distribution.rs:366-430; the density per ln r is u r).
+ * p[3 i .. 3 i + 2]) and radius r[i] (distribution.rs:366-430; the density per ln r is u r),
+ * with d_ln_u (n, 3) not NULL also d ln u / d p (the reference radius held fixed).
  * SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. */
 int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi);
 int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
-                            double *ln_u);
+                            double *ln_u, double *d_ln_u);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,7 @@ SYMBOLS = (
     "scatrs_ensemble_fixed",
     "scatrs_ensemble_fixed_vjp",
     "scatrs_ensemble_nodes",
+    "scatrs_ensemble_nodes_vjp",
     "scatrs_size_range",
     "scatrs_size_log_density",
 )
@@ -90,7 +91,11 @@ def load() -> C.CDLL:
         if hasattr(L, "scatrs_size_range"):
             L.scatrs_size_range.argtypes = [i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
             L.scatrs_size_log_density.argtypes = [i64, C.POINTER(C.c_int32), C.POINTER(dbl),
-                                                  C.POINTER(dbl), C.POINTER(dbl)]
+                                                  C.POINTER(dbl), C.POINTER(dbl), vp]
+        if hasattr(L, "scatrs_ensemble_nodes_vjp"):
+            L.scatrs_ensemble_nodes_vjp.argtypes = [vp, i64, i64, vp, vp, vp, i32, vp, vp, i64,
+                                                    i64, i32, i32, vp, vp, vp, vp,
+                                                    C.POINTER(i64)]
         if L.scatrs_abi_version() != ABI_VERSION:
             raise NativeLibraryError("libscatrs_hip.so has a different ABI version")
         _lib = L

@@ -458,44 +458,79 @@ def _dict(V, E, D, DE, conv, counts, n_per_unit: int, W: int, n_mu: int):
     return out
 
 
-def _particle_object(comp):
-    """The component's particle as the package's Sphere or LayeredSphere, with one index per
-    wavelength."""
-    from .particles import LayeredSphere, Sphere
-
-    if comp.layers == 1:
-        return Sphere(comp.m[0])
-    return LayeredSphere(tuple(float(f) for f in comp.fr), [comp.m[l] for l in range(comp.layers)])
-
-
-def _run_adaptive(units, wl, *, order, mu, normalized, rtol, max_panels, refine_backscatter,
-                  n_angles, allow_unvalidated):
-    """Values and errors (U, n, K), converged (U, n), panels per unit (U,) by the adaptive
-    engine (scatrs_amd.adaptive)."""
+def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
+                  refine_backscatter, n_angles, allow_unvalidated):
+    """Values and errors (U, n, K), derivatives and their errors (P, K, n) for unit 0 (None
+    without keys), converged (U, n) and panels per (unit, wavelength) by the adaptive engine
+    (scatrs_amd.adaptive): pass 1 on the parameters' values, pass 2 differentiable
+    (adaptive.pass2_torch) by both rules, R replicas with one-hot cotangents as in _run."""
     import torch
 
-    from .ensemble import _adaptive, _parts_of
+    from . import adaptive
+    from .ensemble import assemble
 
-    parts, unit, frac = [], [], []
-    for u, comps in enumerate(units):
-        total = sum(c.weight for c in comps)
-        for c in comps:
-            if c.weight == 0.0:
-                continue
-            p, f = _parts_of(c.dist, _particle_object(c), c.weight / total)
-            parts += p
-            frac += f
-            unit += [u] * len(p)
-    k, g, _, conv, panels, _ = _adaptive(
-        parts, unit, frac, len(units), wl, greek=order, normalized=normalized,
-        n_angles=n_angles, rtol=rtol, max_panels=max_panels,
-        allow_unvalidated=allow_unvalidated, phase=mu is not None, mu=mu,
-        refine_backscatter=refine_backscatter)
-    V = torch.stack([_flat(k[0][u], None if k[1] is None else k[1][u],
-                           None if k[2] is None else k[2][u]) for u in range(len(units))])
-    Vg = torch.stack([_flat(g[0][u], None if g[1] is None else g[1][u],
-                            None if g[2] is None else g[2][u]) for u in range(len(units))])
-    return V, (V - Vg).abs(), torch.from_numpy(conv), panels
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n = wl.size
+    W = 0 if order is None else order + 1
+    n_mu = 0 if mu is None else mu.size
+    K = _N + 6 * W + 6 * n_mu
+    R = K if keys else 1
+    B = R * n
+    U = len(units)
+    with torch.enable_grad() if keys else torch.no_grad():
+        leaves = _Leaves(units, n, R, dev, bool(keys))
+        parts = _parts(units, leaves)
+        tasks, params = [], []
+        for (_, kind, p, m_re, m_im, fr, _, L) in parts:
+            p0 = p.detach().cpu().numpy()
+            m0 = (m_re.detach() + 1j * m_im.detach()).cpu().numpy()
+            f0 = fr.detach().cpu().numpy()
+            for i in range(n):
+                rows = torch.arange(R, device=dev) * n + i
+                tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
+                                           f0[i].copy()))
+                params.append((p[rows], m_re[rows], m_im[rows], fr[rows] if L > 1 else None))
+        adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
+        unit = torch.tensor([part[0] for part in parts], device=dev)
+        f = torch.stack([part[6] for part in parts])  # (parts, B)
+        P = len(parts)
+        out = []
+        for idx in (np.arange(15), adaptive._GAUSS):
+            avg, gr, ph, _, _ = adaptive.pass2_torch(
+                tasks, params, R=R, idx=idx, order=order, n_angles=n_angles, mu=mu,
+                want_phase=mu is not None, allow_unvalidated=allow_unvalidated)
+            # (P n, R, ...) -> (P, B, ...), b = r n + i
+            per = lambda a: None if a is None else a.reshape(  # noqa: E731
+                (P, n, R) + a.shape[2:]).transpose(1, 2).reshape((P, B) + a.shape[2:])
+            avg, gr, ph = per(avg), per(gr), per(ph)
+            mix = lambda a, w: torch.zeros((U,) + a.shape[1:], dtype=a.dtype,  # noqa: E731
+                                           device=dev).index_add(0, unit, w * a)
+            sc_u, gr_u, ph_u = assemble(mix(avg, f[..., None]),
+                                        None if gr is None else mix(gr, f[..., None, None]),
+                                        None if ph is None else mix(ph, f[..., None, None]),
+                                        normalized)
+            Y = [_flat(sc_u[u], None if gr_u is None else gr_u[u],
+                       None if ph_u is None else ph_u[u]) for u in range(U)]
+            D = None
+            if keys:
+                ts = leaves.tensors()
+                cot = torch.zeros((R, n, K), dtype=torch.float64, device=dev)
+                cot[torch.arange(K), :, torch.arange(K)] = 1.0
+                # both rules share the parameters' graph (the mixing fractions): keep it
+                gs = torch.autograd.grad(Y[0], ts, cot.reshape(B, K), allow_unused=True,
+                                         retain_graph=True)
+                grads = {id(t): g for t, g in zip(ts, gs)}
+                D = torch.stack([leaves.column(0, k, grads).reshape(R, n) for k in keys])
+            out.append((torch.stack([y.detach()[:n] for y in Y]), D))
+    (V, D), (Vg, Dg) = out
+    conv = np.ones((U, n), bool)
+    panels = np.zeros((U, n), np.int64)
+    for k, q in enumerate(tasks):
+        u, i = parts[k // n][0], k % n
+        conv[u, i] &= q.converged
+        panels[u, i] += q.n_panels
+    DE = None if D is None else (D - Dg).abs()
+    return V, (V - Vg).abs(), D, DE, torch.from_numpy(conv), panels
 
 
 def _mu(mu):
@@ -513,8 +548,9 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
     fixed path (``n_radius`` radius nodes per part, the Greek coefficients projected on
     ``n_angles`` Gauss-Legendre cosines, 128 by default) or, with ``adaptive``, by the adaptive
     engine (scatrs_amd.adaptive: rtol, max_panels and refine_backscatter as the reference
-    uses them; n_angles=None sizes the Greek projection's rule to the largest N; no
-    derivatives); ``derivatives`` are indices into ``ensemble_parameters(items, mixture)``."""
+    uses them; n_angles=None sizes each Greek projection's rule to its nodes' largest N;
+    derivatives on the final panels with the range held fixed); ``derivatives`` are indices
+    into ``ensemble_parameters(items, mixture)``."""
     comps = _components(items)
     wl = np.ascontiguousarray(np.asarray(wavelengths, np.float64).reshape(-1))
     mu = _mu(mu)
@@ -534,14 +570,11 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
     _check_options(rtol, max_panels, wl.size, n_mu, order, len(chosen))
     W = 0 if order is None else order + 1
     if adaptive:
-        if chosen:
-            raise InvalidInputError("the adaptive engine here computes no derivatives; the fixed "
-                                    "path (adaptive=False) does")
-        V, E, conv, panels = _run_adaptive(
-            [comps], wl, order=order, mu=mu, normalized=normalized, rtol=rtol,
+        V, E, D, DE, conv, panels = _run_adaptive(
+            [comps], wl, order=order, mu=mu, normalized=normalized, keys=chosen, rtol=rtol,
             max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
             allow_unvalidated=allow_unvalidated)
-        out = _dict(V, E, None, None, conv, np.zeros(1, np.int64), wl.size, W, n_mu)
+        out = _dict(V, E, D, DE, conv, np.zeros(1, np.int64), wl.size, W, n_mu)
         out["panels"] = panels.reshape(-1).astype(np.uint64)
         return out
     V, E, D, DE, conv, counts = _run(
@@ -577,8 +610,8 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
         return _dict(z, z, None, None, torch.zeros((0, wl.size), dtype=torch.bool),
                      np.zeros(0, np.int64), wl.size, W, n_mu)
     if adaptive:
-        V, E, conv, panels = _run_adaptive(
-            units, wl, order=order, mu=mu, normalized=normalized, rtol=rtol,
+        V, E, _, _, conv, panels = _run_adaptive(
+            units, wl, order=order, mu=mu, normalized=normalized, keys=[], rtol=rtol,
             max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
             allow_unvalidated=allow_unvalidated)
         out = _dict(V, E, None, None, conv, np.zeros(len(units), np.int64), wl.size, W, n_mu)

@@ -88,18 +88,21 @@ def size_range(kind: int, p) -> tuple[float, float]:
     return lo.value, hi.value
 
 
-def log_density(kind, p, r) -> np.ndarray:
+def log_density(kind, p, r, derivatives: bool = False):
     """ln u(r), the number density per unit radius up to a constant (scatrs_size_log_density):
-    kind a scalar or one per radius, p (3,) or (n, 3), r (n,)."""
+    kind a scalar or one per radius, p (3,) or (n, 3), r (n,); with ``derivatives`` also
+    d ln u / d p (n, 3)."""
     r = np.ascontiguousarray(r, np.float64).reshape(-1)
     n = r.size
     k = np.ascontiguousarray(np.broadcast_to(np.asarray(kind, np.int32), (n,)))
     q = np.ascontiguousarray(np.broadcast_to(np.asarray(p, np.float64).reshape(-1, 3), (n, 3)))
     out = np.empty_like(r)
-    if _native.load().scatrs_size_log_density(n, k.ctypes.data_as(C.POINTER(C.c_int32)),
-                                              _dbl(q), _dbl(r), _dbl(out)) != 0:
+    d = np.empty((n, 3)) if derivatives else None
+    if _native.load().scatrs_size_log_density(
+            n, k.ctypes.data_as(C.POINTER(C.c_int32)), _dbl(q), _dbl(r), _dbl(out),
+            None if d is None else C.c_void_p(d.ctypes.data)) != 0:
         raise InvalidInputError("invalid parameters of a size distribution")
-    return out
+    return (out, d) if derivatives else out
 
 
 def integration_range(kind: int, p, rtol: float) -> tuple[float, float, float]:
@@ -154,10 +157,11 @@ class Task:
         return max(self.a.size, 1)
 
 
-def _nodes(tasks, ti, a, b, idx):
+def _nodes(tasks, ti, a, b, idx, derivatives: bool = False):
     """(r, W) (len(a), len(idx)) of the panels [a, b] (each of task ti) at the nodes idx of the
     rule, with the Kronrod weights at all 15 nodes or the Gauss weights at the 7; a panel of
-    width 0 (padding) has weights 0. One density call for all panels."""
+    width 0 (padding) has weights 0. One density call for all panels. With ``derivatives``
+    also d ln W / d p (len(a), len(idx), 3), the range held fixed."""
     x = _X[idx]
     w = _WK[idx] if idx.size == 15 else _WGF[idx]
     half, mid = 0.5 * (b - a), 0.5 * (a + b)
@@ -167,8 +171,11 @@ def _nodes(tasks, ti, a, b, idx):
     p = np.stack([tasks[i].p for i in ti]) if len(ti) else np.zeros((0, 3))
     c = np.array([tasks[i].c for i in ti])
     k = idx.size
-    lu = log_density(np.repeat(kind, k), np.repeat(p, k, axis=0), r.reshape(-1))
+    out = log_density(np.repeat(kind, k), np.repeat(p, k, axis=0), r.reshape(-1), derivatives)
+    lu = out[0] if derivatives else out
     u = np.exp(lu.reshape(r.shape) + t - c[:, None])
+    if derivatives:
+        return r, w[None, :] * half[:, None] * u, out[1].reshape(r.shape + (3,))
     return r, w[None, :] * half[:, None] * u
 
 
@@ -422,6 +429,109 @@ def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
         out.append((_scalars(acc * zi[:, None]), None if gr is None else gr * zi[:, None, None],
                     None if ph is None else ph * zi[:, None, None]))
     return out[0], out[1], terms, int(rules[0]) if len(rules) else 2
+
+
+def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
+                allow_unvalidated):
+    """The Kronrod (idx all 15 nodes) or Gauss rule's pass 2, differentiable in every task's
+    parameters (torch.autograd): params[t] = (p (R, 3), m_real (R, L), m_imag (R, L),
+    fractions (R, L-1) or None), R replicas of task t's parameters. Returns the linear averages
+    [C_ext, C_sca, C_abs, C_back, C_sca g, r^2, r^3] (T, R, 7), greek (T, R, 6, order+1) and
+    phase (T, R, 6, n) (normalized=False; None when not asked for), and the terms.
+
+    The rows are pass2's. A row's sum of weights Z enters to first order in p, Z0 + (p - p0)
+    . sum W d ln W / d p, which has the right value and derivative at p0; a monodisperse task is
+    the fixed path's one node (its radius moves its node)."""
+    import torch
+
+    from .ensemble import ensemble_fixed, ensemble_nodes, number_averages
+
+    dev = params[0][0].device
+    T = len(tasks)
+    f64 = torch.float64
+    Wd = 0 if order is None else order + 1
+    single = want_phase and mu is None
+    rows = [(ti, lo, min(lo + _ROW_PANELS, q.a.size)) for ti, q in enumerate(tasks)
+            if q.kind != MONODISPERSE for lo in range(0, q.a.size, _ROW_PANELS)]
+    mono = [ti for ti, q in enumerate(tasks) if q.kind == MONODISPERSE]
+    rules = _row_rules(tasks, rows + [(t, -1, -1) for t in mono], order, n_angles, single)
+    n_phase = (int(rules[0]) if mu is None else int(np.size(mu))) if want_phase else 0
+    avg = torch.zeros((T * R, 7), dtype=f64, device=dev)
+    zt = torch.zeros(T * R, dtype=f64, device=dev)
+    gr = torch.zeros((T * R, 6, Wd), dtype=f64, device=dev) if order is not None else None
+    ph = torch.zeros((T * R, 6, n_phase), dtype=f64, device=dev) if want_phase else None
+    terms = 0
+    rep = torch.arange(R, device=dev)
+
+    def gather(tk, L):
+        """The parameters of tasks tk (each replicated R times, task-major)."""
+        pr = torch.cat([params[t][0] for t in tk])
+        mr = torch.cat([params[t][1] for t in tk])
+        mi = torch.cat([params[t][2] for t in tk])
+        fr = torch.cat([params[t][3] for t in tk]) if L > 1 else None
+        if L == 1:
+            mr, mi = mr[:, 0], mi[:, 0]
+        return pr, mr, mi, fr
+
+    def add(key, z, sc, g, p_):
+        nonlocal avg, zt, gr, ph
+        avg = avg.index_add(0, key, z[:, None] * number_averages(sc))
+        zt = zt.index_add(0, key, z)
+        if gr is not None:
+            gr = gr.index_add(0, key, z[:, None, None] * g)
+        if ph is not None:
+            ph = ph.index_add(0, key, z[:, None, None] * p_)
+
+    wl_all = np.array([q.wl for q in tasks])
+    layers = np.array([q.layers for q in tasks])
+    nr = len(rows)
+    if nr:
+        ti_rows = np.array([r[0] for r in rows])
+        A = np.zeros((nr, _ROW_PANELS))
+        Bd = np.zeros((nr, _ROW_PANELS))
+        for j, (ti, lo, hi) in enumerate(rows):
+            q = tasks[ti]
+            n = hi - lo
+            A[j, :n], Bd[j, :n] = q.a[lo:hi], q.b[lo:hi]
+            A[j, n:] = Bd[j, n:] = q.b[hi - 1]
+        r, w, d = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
+                         idx, derivatives=True)
+        nodes = np.stack([r.reshape(nr, -1), w.reshape(nr, -1)], -1)
+        dl = d.reshape(nr, -1, 3)
+        z0 = nodes[..., 1].sum(1)
+        zd = (nodes[..., 1:2] * dl).sum(1)
+        for na in np.unique(rules[:nr]):
+            for L in np.unique(layers[ti_rows]):
+                sel = np.flatnonzero((rules[:nr] == na) & (layers[ti_rows] == L) & (z0 > 0.0))
+                if not sel.size:
+                    continue
+                tk = ti_rows[sel]
+                pr, mr, mi, fr = gather(tk, L)
+                t = lambda a: torch.from_numpy(np.ascontiguousarray(  # noqa: E731
+                    np.repeat(a, R, axis=0))).to(dev)
+                sc, g, p_ = ensemble_nodes(pr, mr, mi, fr, t(nodes[sel]), t(dl[sel]),
+                                           t(wl_all[tk]), n_angles=int(na), order=order,
+                                           normalized=False,
+                                           allow_unvalidated=allow_unvalidated, mu=mu,
+                                           phase=want_phase)
+                z = t(z0[sel]) + ((pr - pr.detach()) * t(zd[sel])).sum(1)
+                key = (torch.from_numpy(tk).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
+                add(key, z, sc, g, p_)
+    for k, ti in enumerate(mono):
+        na = int(rules[nr + k])
+        L = tasks[ti].layers
+        pr, mr, mi, fr = gather([ti], L)
+        kind = torch.full((R,), MONODISPERSE, dtype=torch.int32, device=dev)
+        sc, g, p_ = ensemble_fixed(kind, pr[:, 0], pr[:, 1], torch.full(
+            (R,), tasks[ti].wl, dtype=f64, device=dev), mr, mi, fr, p2=pr[:, 2], n_radius=1,
+            n_angles=na, order=order, normalized=False, fixed_range=True,
+            allow_unvalidated=allow_unvalidated, mu=mu, phase=want_phase)
+        add(ti * R + rep, torch.ones(R, dtype=f64, device=dev), sc, g, p_)
+    avg = avg / zt[:, None]
+    gr = None if gr is None else gr / zt[:, None, None]
+    ph = None if ph is None else ph / zt[:, None, None]
+    shape = lambda a: None if a is None else a.reshape((T, R) + a.shape[1:])  # noqa: E731
+    return shape(avg), shape(gr), shape(ph), terms, (int(rules[0]) if len(rules) else 2)
 
 
 def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_panels: int,

@@ -52,6 +52,7 @@ struct EnsArgs {
     const double *p2;                 // third parameter (may be null: 0)
     double2 *geom;                    // per node (r, W), written by ens_prep
     const double2 *user_rw;           // the caller's (r, W) per node, or null (from kind, p)
+    const double *user_dlnw;          // with user_rw, the caller's d ln W / d p[3] per node
 };
 
 // Ensemble e's distribution parameters.

@@ -628,6 +628,13 @@ static __global__ __launch_bounds__(256) void ens_node_grads(
     const double *v = vals + g * kNodeVals;
     double phi = tabc[g] * ht[g];
     for (int k = 0; k < kNodeVals; ++k) phi += sb[k] * v[k];
+    if (a.user_rw) {
+        // The caller's nodes stay where they are: the parameters act through each node's
+        // ln W alone, whose derivatives the caller gives (phi is the cotangent of ln W).
+        const double *d = a.user_dlnw ? a.user_dlnw + (e * a.n_radius + i) * 3 : nullptr;
+        for (int th = 0; th < 3; ++th) out[th] = d ? phi * d[th] : 0.0;
+        return;
+    }
     const int kind = a.kind[e];
     double p[3];
     ens_params(a, e, p);
@@ -680,30 +687,31 @@ static void launch_nodes_vjp_ranked(hipStream_t s, const EnsArgs &a, int64_t e0,
 
 using namespace scatrs;
 
-extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
-                                         const double *d_p0, const double *d_p1,
-                                         const double *d_p2, const double *d_wavelength,
-                                         int layers,
-                                         const double *d_fractions, const double *d_m,
-                                         int64_t n_radius, int64_t n_angles, int64_t order,
-                                         int normalized, int fixed_range,
-                                         int allow_unvalidated, const double *d_bar_scalars,
-                                         const double *d_bar_greek, const double *d_bar_phase,
-                                         double *d_grad, int64_t *total_terms) {
+// The reverse pass of both entry points: nodes from the distributions, or with d_nodes the
+// caller's (r, W) pairs and d_dlnw their d ln W / d p[3] (may be null: zero columns 0-2).
+static int ensemble_reverse(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                            const double *d_p0, const double *d_p1, const double *d_p2,
+                            const double *d_nodes, const double *d_dlnw,
+                            const double *d_wavelength, int layers, const double *d_fractions,
+                            const double *d_m, int64_t n_radius, int64_t n_angles, int64_t order,
+                            int normalized, int fixed_range, int allow_unvalidated,
+                            const double *d_bar_scalars, const double *d_bar_greek,
+                            const double *d_bar_phase, double *d_grad, int64_t *total_terms) {
     if (!ctx) return SCATRS_INVALID_INPUT;
     ctx->error.clear();
     if (total_terms) *total_terms = 0;
     int rc = ens_check_args(ctx, count, layers, n_radius, n_angles, d_bar_greek != nullptr, order);
     if (rc) return rc;
     if (count == 0) return SCATRS_OK;
-    if (!d_kind || !d_p0 || !d_p1 || !d_wavelength || !d_m || !d_grad ||
+    if ((!d_nodes && (!d_kind || !d_p0 || !d_p1)) || !d_wavelength || !d_m || !d_grad ||
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
     EnsSetup S;
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_bar_greek != nullptr, order, allow_unvalidated,
-                        ctx->angular_kernel, S)))
+                        ctx->angular_kernel, S, d_nodes)))
         return rc;
+    S.a.user_dlnw = d_nodes ? d_dlnw : nullptr;
     // The reverse products read the FP64 coefficients: the node solves stage those, and with the
     // fp16-split kernel the forward products read an fp32 copy.
     S.stage32 = false;
@@ -828,4 +836,38 @@ extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const i
         e0 = e1;
     }
     return SCATRS_OK;
+}
+
+extern "C" int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                                         const double *d_p0, const double *d_p1,
+                                         const double *d_p2, const double *d_wavelength,
+                                         int layers,
+                                         const double *d_fractions, const double *d_m,
+                                         int64_t n_radius, int64_t n_angles, int64_t order,
+                                         int normalized, int fixed_range,
+                                         int allow_unvalidated, const double *d_bar_scalars,
+                                         const double *d_bar_greek, const double *d_bar_phase,
+                                         double *d_grad, int64_t *total_terms) {
+    return ensemble_reverse(ctx, count, d_kind, d_p0, d_p1, d_p2, nullptr, nullptr, d_wavelength,
+                            layers, d_fractions, d_m, n_radius, n_angles, order, normalized,
+                            fixed_range, allow_unvalidated, d_bar_scalars, d_bar_greek,
+                            d_bar_phase, d_grad, total_terms);
+}
+
+extern "C" int scatrs_ensemble_nodes_vjp(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
+                                         const double *d_nodes, const double *d_dlnw,
+                                         const double *d_wavelength, int layers,
+                                         const double *d_fractions, const double *d_m,
+                                         int64_t n_angles, int64_t order, int normalized,
+                                         int allow_unvalidated, const double *d_bar_scalars,
+                                         const double *d_bar_greek, const double *d_bar_phase,
+                                         double *d_grad, int64_t *total_terms) {
+    if (ctx && count > 0 && !d_nodes) {
+        ctx->error.clear();
+        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    }
+    return ensemble_reverse(ctx, count, nullptr, nullptr, nullptr, nullptr, d_nodes, d_dlnw,
+                            d_wavelength, layers, d_fractions, d_m, n_nodes, n_angles, order,
+                            normalized, 1, allow_unvalidated, d_bar_scalars, d_bar_greek,
+                            d_bar_phase, d_grad, total_terms);
 }

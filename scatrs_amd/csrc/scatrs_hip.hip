@@ -268,11 +268,11 @@ int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi)
 }
 
 int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
-                            double *ln_u) {
+                            double *ln_u, double *d_ln_u) {
     if (n < 0 || (n > 0 && (!kind || !p || !r || !ln_u))) return SCATRS_INVALID_INPUT;
     for (int64_t i = 0; i < n; ++i) {
         if (!scatrs::dist_valid(kind[i], p + 3 * i)) return SCATRS_INVALID_INPUT;
-        ln_u[i] = scatrs::log_density(kind[i], p + 3 * i, r[i], nullptr);
+        ln_u[i] = scatrs::log_density(kind[i], p + 3 * i, r[i], d_ln_u ? d_ln_u + 3 * i : nullptr);
     }
     return SCATRS_OK;
 }

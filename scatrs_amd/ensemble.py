@@ -24,7 +24,7 @@ from .size import SEGMENT, Distribution, MultiModal, Tabulated, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
-           "ensemble_nodes_device",
+           "ensemble_nodes", "ensemble_nodes_device", "ensemble_nodes_vjp_device",
            "assemble", "mix_parts", "mix_units", "number_averages", "parameter_names"]
 
 SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
@@ -116,6 +116,106 @@ def ensemble_nodes_device(nodes, wavelength, m, fractions=None, *, n_angles: int
             ctx.set_phase_cosines(None)
     ctx.check(rc)
     return scalars, greek, phase, terms.value
+
+
+def ensemble_nodes_vjp_device(nodes, dlnw, wavelength, m, fractions=None, *, bar_scalars=None,
+                              bar_greek=None, bar_phase=None, n_angles: int = 128,
+                              order: int | None = 32, normalized: bool = True,
+                              allow_unvalidated: bool = False, mu=None):
+    """Reverse mode of ensemble_nodes_device (scatrs_ensemble_nodes_vjp): dlnw (count, n_nodes,
+    3) the nodes' d ln W / d p, or None; the cotangents as ensemble_fixed_vjp_device's. Returns
+    the gradient (count, P) in parameter_names(L) order and the terms of one pass."""
+    import torch
+
+    dev = nodes.device
+    count, n_nodes = nodes.shape[0], nodes.shape[1]
+    layers = 1 if m.dim() == 1 else m.shape[1]
+    if layers > 1 and (fractions is None or fractions.shape != (count, layers - 1)):
+        raise InvalidInputError("layered spheres need fractions of shape (count, L - 1)")
+    if dlnw is not None and dlnw.shape != (count, n_nodes, 3):
+        raise InvalidInputError("dlnw must have shape (count, n_nodes, 3)")
+    c = lambda t: None if t is None else t.to(torch.float64).contiguous()  # noqa: E731
+    nd, dl = c(nodes), c(dlnw)
+    bs, bg, bp = c(bar_scalars), c(bar_greek), c(bar_phase)
+    fr = None if layers == 1 else fractions.contiguous()
+    P = 3 + 2 * layers + (layers - 1)
+    grad = torch.empty((count, P), dtype=torch.float64, device=dev)
+    ctx = _native.context(dev.index)
+    terms = C.c_int64(0)
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+    if mu is not None:
+        ctx.set_phase_cosines(mu)
+    try:
+        rc = ctx.lib.scatrs_ensemble_nodes_vjp(
+            ctx.handle, count, n_nodes, p(nd), p(dl), p(wavelength.contiguous()), layers, p(fr),
+            p(m.contiguous()), int(n_angles), -1 if order is None else int(order),
+            int(bool(normalized)), int(bool(allow_unvalidated)), p(bs), p(bg), p(bp), p(grad),
+            C.byref(terms))
+    finally:
+        if mu is not None:
+            ctx.set_phase_cosines(None)
+    ctx.check(rc)
+    return grad, terms.value
+
+
+def _nodes_function():
+    import torch
+
+    class EnsembleNodesFunction(torch.autograd.Function):
+        """(p (count, 3), m_real, m_imag, fractions) -> (scalars, greek, phase) on the caller's
+        nodes; the parameters act through the nodes' ln W (derivatives dlnw), the nodes stay."""
+
+        @staticmethod
+        def forward(ctx, p, m_real, m_imag, fractions, nodes, dlnw, wavelength, opts):
+            ctx.set_materialize_grads(False)
+            m = torch.complex(m_real, m_imag)
+            sc, gr, ph, _ = ensemble_nodes_device(
+                nodes, wavelength, m, fractions, n_angles=opts["n_angles"], order=opts["order"],
+                normalized=opts["normalized"], allow_unvalidated=opts["allow_unvalidated"],
+                want_phase=opts["phase"], mu=opts["mu"])
+            empty = torch.empty(0)
+            ctx.save_for_backward(m_real, m_imag, fractions if fractions is not None else empty,
+                                  nodes, dlnw, wavelength)
+            ctx.opts = opts
+            ctx.has_fr = fractions is not None
+            return sc, gr, ph
+
+        @staticmethod
+        def backward(ctx, g_sc, g_gr, g_ph):
+            m_real, m_imag, fr, nodes, dlnw, wl = ctx.saved_tensors
+            o = ctx.opts
+            m = torch.complex(m_real, m_imag)
+            layers = 1 if m.dim() == 1 else m.shape[1]
+            grad, _ = ensemble_nodes_vjp_device(
+                nodes, dlnw, wl, m, fr if ctx.has_fr else None, bar_scalars=g_sc, bar_greek=g_gr,
+                bar_phase=g_ph, n_angles=o["n_angles"], order=o["order"],
+                normalized=o["normalized"], allow_unvalidated=o["allow_unvalidated"],
+                mu=o["mu"])
+            gm_re = grad[:, 3:3 + 2 * layers:2].reshape(m_real.shape)
+            gm_im = grad[:, 4:4 + 2 * layers:2].reshape(m_imag.shape)
+            gfr = grad[:, 3 + 2 * layers:] if ctx.has_fr else None
+            return grad[:, :3], gm_re, gm_im, gfr, None, None, None, None
+
+    return EnsembleNodesFunction
+
+
+_NODES_FUNCTION = None
+
+
+def ensemble_nodes(p, m_real, m_imag, fractions, nodes, dlnw, wavelength, *, n_angles: int = 128,
+                   order: int | None = 32, normalized: bool = True,
+                   allow_unvalidated: bool = False, mu=None, phase: bool = True):
+    """Differentiable ensembles on the caller's nodes (torch.autograd): p (count, 3) the
+    parameters the nodes' weights depend on, through dlnw (count, n_nodes, 3) = d ln W / d p
+    (the nodes do not move); m_real, m_imag (count,) or (count, L); fractions (count, L-1) or
+    None; nodes (count, n_nodes, 2). Returns (scalars, greek, phase) as ensemble_nodes_device."""
+    global _NODES_FUNCTION
+    if _NODES_FUNCTION is None:
+        _NODES_FUNCTION = _nodes_function()
+    opts = dict(n_angles=int(n_angles), order=None if order is None else int(order),
+                normalized=bool(normalized), allow_unvalidated=bool(allow_unvalidated),
+                phase=bool(phase), mu=None if mu is None else np.array(mu, np.float64).reshape(-1))
+    return _NODES_FUNCTION.apply(p, m_real, m_imag, fractions, nodes, dlnw, wavelength, opts)
 
 
 def parameter_names(layers: int) -> list[str]:
