@@ -7,7 +7,9 @@ with the 15-point Kronrod rule and its embedded 7-point Gauss rule. The host pla
 panels; every evaluation is one batched library call over all open panels of all tasks.
 
 * Range: the ln r interval outside which the area-weighted law r^2 n(r) holds at most rtol/10
-  of its total, split evenly between the tails. The tails are found from the library's own
+  of its total, split evenly between the tails. Z and the moments <r^2>, <r^3> come from their
+  own integration over the number tail below and the volume tail above (moment_integrals), as
+  the reference's averages take them from its moment integration. The tails are found from the library's own
   log-density on a fine grid over three times the fixed path's range. A power law, a tabulated
   segment and a monodisperse law keep their supports.
 * Pass 1: the refined quantities are the reference's (engine.rs:620-690): <C_ext>, <C_sca>,
@@ -127,6 +129,54 @@ def integration_range(kind: int, p, rtol: float) -> tuple[float, float, float]:
     return t_lo, t_hi, float(np.max(log_density(kind, p, np.exp(tt)) + tt))
 
 
+def _tail_cuts(kind, p, eps, k_lo: int, k_hi: int):
+    """ln r cuts below which the law weighted by r^k_lo, and above which the law weighted by
+    r^k_hi, hold at most eps of their totals (on a fine grid over three times the fixed
+    path's range)."""
+    lo, hi = size_range(kind, p)
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    t = np.linspace(mid - 3.0 * half, mid + 3.0 * half, _GRID)
+    lu = log_density(kind, p, np.exp(t))
+    out = []
+    for k, upper in ((k_lo, False), (k_hi, True)):
+        a = lu + (k + 1.0) * t
+        f = np.exp(a - np.max(a))
+        cum = np.concatenate([[0.0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(t))])
+        if upper:
+            out.append(t[min(int(np.searchsorted(cum, cum[-1] * (1 - eps), side="left")),
+                             t.size - 1)])
+        else:
+            out.append(t[max(int(np.searchsorted(cum, eps * cum[-1], side="right")) - 1, 0)])
+    return out[0], out[1]
+
+
+_MOMENT_NODES = 1024
+
+
+def moment_integrals(kind: int, p, c: float, rtol: float):
+    """Z = int u dr and M_k = int u r^k dr (k = 2, 3) over the moment range, and their
+    derivatives with respect to p (3, 3) (rows Z, M2, M3), the density scaled by exp(-c) as
+    the nodes' weights are. The range runs from where the number tail below holds rtol/20 of
+    the number to where the volume tail above holds rtol/20 of the volume, so that Z, <r^2>
+    and <r^3> miss no more than the reference's moment integration (moments.rs:62-127); a
+    Gauss-Legendre rule of 1024 nodes in ln r. A power law or a tabulated segment: its
+    support."""
+    if kind in (POWER_LAW, SEGMENT):
+        t_lo, t_hi = size_range(kind, p)
+    else:
+        t_lo, t_hi = _tail_cuts(kind, p, 0.05 * rtol, 0, 3)
+    x, w = np.polynomial.legendre.leggauss(_MOMENT_NODES)
+    half, mid = 0.5 * (t_hi - t_lo), 0.5 * (t_hi + t_lo)
+    t = mid + half * x
+    lu, d = log_density(kind, p, np.exp(t), derivatives=True)
+    f = w * half * np.exp(lu + t - c)  # u r dt, scaled
+    r = np.exp(t)
+    vals = np.array([f.sum(), (f * r * r).sum(), (f * r ** 3).sum()])
+    der = np.stack([(f[:, None] * d).sum(0), ((f * r * r)[:, None] * d).sum(0),
+                    ((f * r ** 3)[:, None] * d).sum(0)])
+    return vals, der
+
+
 @dataclass
 class Task:
     """One part at one wavelength: its distribution (kind, p), index and fractions, range and
@@ -145,6 +195,8 @@ class Task:
     b: np.ndarray = field(default_factory=lambda: np.zeros(0))
     K: np.ndarray = field(default_factory=lambda: np.zeros((0, 5)))
     G: np.ndarray = field(default_factory=lambda: np.zeros((0, 5)))
+    moments: np.ndarray = field(default_factory=lambda: np.zeros(3))  # Z, M2, M3
+    d_moments: np.ndarray = field(default_factory=lambda: np.zeros((3, 3)))
     done: bool = False
     converged: bool = False
 
@@ -246,6 +298,7 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             q.done = q.converged = True
             continue
         q.t_lo, q.t_hi, q.c = integration_range(q.kind, q.p, rtol)
+        q.moments, q.d_moments = moment_integrals(q.kind, q.p, q.c, rtol)
         edges = np.linspace(q.t_lo, q.t_hi, _INITIAL_PANELS + 1)
         q.a, q.b = edges[:-1].copy(), edges[1:].copy()
         q.K, q.G = _nan(_INITIAL_PANELS), _nan(_INITIAL_PANELS)
@@ -424,8 +477,15 @@ def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
                 np.add.at(gr, t, zl[:, None, None] * g[live])
             if ph is not None:
                 np.add.at(ph, t, zl[:, None, None] * p[live])
+        # The averages divide by the moment integration's Z and take its <r^2>, <r^3>
+        # (engine.rs:1782-1836); a monodisperse law keeps its node's.
+        mom = np.array([q.moments for q in tasks])
+        single_node = np.array([q.kind == MONODISPERSE for q in tasks])
+        zm = np.where(single_node, zt, mom[:, 0])
+        acc[~single_node, 5] = mom[~single_node, 1]
+        acc[~single_node, 6] = mom[~single_node, 2]
         with np.errstate(divide="ignore", invalid="ignore"):
-            zi = 1.0 / zt
+            zi = 1.0 / zm
         out.append((_scalars(acc * zi[:, None]), None if gr is None else gr * zi[:, None, None],
                     None if ph is None else ph * zi[:, None, None]))
     return out[0], out[1], terms, int(rules[0]) if len(rules) else 2
@@ -527,9 +587,20 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             n_angles=na, order=order, normalized=False, fixed_range=True,
             allow_unvalidated=allow_unvalidated, mu=mu, phase=want_phase)
         add(ti * R + rep, torch.ones(R, dtype=f64, device=dev), sc, g, p_)
-    avg = avg / zt[:, None]
-    gr = None if gr is None else gr / zt[:, None, None]
-    ph = None if ph is None else ph / zt[:, None, None]
+    # The moment integration's Z, M2, M3 (first order in p: the right value and derivative at
+    # p0), in place of the nodes' own; a monodisperse law keeps its node's.
+    pall = torch.cat([params[t][0] for t in range(T)])  # (T R, 3)
+    dp = pall - pall.detach()
+    mom = torch.from_numpy(np.repeat(np.array([q.moments for q in tasks]), R, axis=0)).to(dev)
+    dmom = torch.from_numpy(np.repeat(np.array([q.d_moments for q in tasks]), R, axis=0)).to(dev)
+    mom = mom + (dmom * dp[:, None, :]).sum(-1)
+    single_node = torch.from_numpy(np.repeat(np.array([q.kind == MONODISPERSE for q in tasks]),
+                                             R)).to(dev)
+    zm = torch.where(single_node, zt, mom[:, 0])
+    avg = torch.cat([avg[:, :5], torch.where(single_node[:, None], avg[:, 5:], mom[:, 1:])], 1)
+    avg = avg / zm[:, None]
+    gr = None if gr is None else gr / zm[:, None, None]
+    ph = None if ph is None else ph / zm[:, None, None]
     shape = lambda a: None if a is None else a.reshape((T, R) + a.shape[1:])  # noqa: E731
     return shape(avg), shape(gr), shape(ph), terms, (int(rules[0]) if len(rules) else 2)
 

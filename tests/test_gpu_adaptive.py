@@ -72,6 +72,22 @@ def _adaptive(kind, p, wl, m, rtol, **kw):
     return K[0][0].cpu().numpy(), K[1][0].cpu().numpy(), conv[0], panels[0], ranges[0], na
 
 
+def _law(kind, p):
+    """The number fraction in [r_lo, r_hi] and <r^2>, <r^3> of a lognormal or gamma law in
+    closed form (distribution.rs:221-268)."""
+    from scipy.special import gammainc
+    from scipy.stats import norm
+
+    if kind == 0:
+        s = np.log(p[1])
+        frac = lambda lo, hi: norm.cdf((np.log(hi / p[0])) / s) - norm.cdf(  # noqa: E731
+            np.log(lo / p[0]) / s)
+        return frac, p[0] ** 2 * np.exp(2 * s * s), p[0] ** 3 * np.exp(4.5 * s * s)
+    k, th = (1 - 2 * p[1]) / p[1], p[0] * p[1]
+    frac = lambda lo, hi: gammainc(k, hi / th) - gammainc(k, lo / th)  # noqa: E731
+    return frac, th ** 2 * k * (k + 1), th ** 3 * k * (k + 1) * (k + 2)
+
+
 @pytest.mark.parametrize("case", [
     (0, (0.3, 1.6, 0.0), [0.55, 1.2], 1.5 + 0.05j, 1e-8),   # absorbing lognormal
     (0, (0.08, 1.6, 0.0), [0.55], 1.43 + 1e-8j, 1e-6),      # resonant: the fixed rule fails
@@ -80,26 +96,33 @@ def _adaptive(kind, p, wl, m, rtol, **kw):
 ])
 def test_adaptive_matches_the_converged_integral_over_its_range(case):
     """normalized=False averages against the oracle's fixed path with t_range set to the
-    engine's range and 4096 nodes (8192 for the resonant case), within a few rtol."""
+    engine's range and 4096 nodes (8192 for the resonant case): the oracle divides by the
+    number inside that range, the engine by the whole law's (its moment integration), so the
+    oracle's averages times the law's number fraction inside the range; <r^2> and <r^3>
+    against the closed forms. Within a few rtol."""
     from scatrs_amd.adaptive import integration_range
 
     kind, p, wl, m, rtol = case
     sc, gr, conv, panels, ranges, na = _adaptive(kind, p, wl, m, rtol)
     assert conv.all(), panels
     n_ref = 8192 if m.imag < 1e-6 else 4096
+    frac, r2, r3 = _law(kind, p)
     for i, w in enumerate(wl):
         t = np.log(ranges[i])
         # an angle rule exact for every node (the engine's rules are, band by band)
         nq = oracle.default_n_max(2 * np.pi * ranges[i][1] / w) + 9
         ref = oracle.ensemble_fixed(kind, p[0], p[1], w, m, n_radius=n_ref, n_angles=nq,
                                     order=16, normalized=False, t_range=t, p2=p[2])
-        rs = ref["scalars"]
-        for k in (0, 1, 9, 10):  # C_ext, C_sca, <area>, <volume>
+        f = frac(*ranges[i])
+        rs = ref["scalars"] * f
+        for k in (0, 1):  # C_ext, C_sca
             assert abs(sc[i, k] - rs[k]) <= 5 * rtol * abs(rs[k]), (k, sc[i, k], rs[k])
-        assert abs(sc[i, 5] * sc[i, 1] - rs[5] * rs[1]) <= 5 * rtol * abs(rs[1])
+        assert abs(sc[i, 5] * sc[i, 1] - rs[5] / f * rs[1]) <= 5 * rtol * abs(rs[1])
         assert abs(sc[i, 2] - rs[2]) <= 5 * rtol * abs(rs[0])
+        assert abs(sc[i, 9] - np.pi * r2) <= 5 * rtol * np.pi * r2
+        assert abs(sc[i, 10] - 4 * np.pi / 3 * r3) <= 5 * rtol * 4 * np.pi / 3 * r3
         for c in range(6):  # every Greek coefficient, exact projections on both sides
-            ref_c = ref["greek"][c]
+            ref_c = ref["greek"][c] * f
             assert np.max(np.abs(gr[i, c] - ref_c)) <= 20 * rtol * np.max(np.abs(ref["greek"][0]))
     if m.imag < 1e-6:
         assert panels.max() > 4  # refined beyond the initial panels
