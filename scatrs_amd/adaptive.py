@@ -35,6 +35,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
+from functools import lru_cache
 
 import numpy as np
 
@@ -153,6 +154,12 @@ def _tail_cuts(kind, p, eps, k_lo: int, k_hi: int):
 _MOMENT_NODES = 1024
 
 
+@lru_cache(maxsize=4)
+def _legendre(n: int):
+    """numpy's n-point Gauss-Legendre rule (an eigenvalue problem: computed once)."""
+    return np.polynomial.legendre.leggauss(n)
+
+
 def moment_integrals(kind: int, p, c: float, rtol: float):
     """Z = int u dr and M_k = int u r^k dr (k = 2, 3) over the moment range, and their
     derivatives with respect to p (3, 3) (rows Z, M2, M3), the density scaled by exp(-c) as
@@ -165,7 +172,7 @@ def moment_integrals(kind: int, p, c: float, rtol: float):
         t_lo, t_hi = size_range(kind, p)
     else:
         t_lo, t_hi = _tail_cuts(kind, p, 0.05 * rtol, 0, 3)
-    x, w = np.polynomial.legendre.leggauss(_MOMENT_NODES)
+    x, w = _legendre(_MOMENT_NODES)
     half, mid = 0.5 * (t_hi - t_lo), 0.5 * (t_hi + t_lo)
     t = mid + half * x
     lu, d = log_density(kind, p, np.exp(t), derivatives=True)
@@ -289,16 +296,74 @@ def _nan(n):
     return np.full((n, 5), np.nan)
 
 
+def _prepare(tasks, rtol: float) -> None:
+    """Every task's range, weight scale c and moment integrals, as integration_range and
+    moment_integrals give them one by one, with one density call per step for all tasks."""
+    live = [q for q in tasks if q.kind != MONODISPERSE]
+    if not live:
+        return
+    T = len(live)
+    kind = np.array([q.kind for q in live], np.int32)
+    p = np.stack([q.p for q in live])
+    lo, hi = np.array([size_range(q.kind, q.p) for q in live]).T
+    support = np.isin(kind, (POWER_LAW, SEGMENT))
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    span = np.where(support, 1.0, 3.0)
+    g = np.linspace(-1.0, 1.0, _GRID)
+    t = mid[:, None] + (span * half)[:, None] * g[None, :]
+    lu = log_density(np.repeat(kind, _GRID), np.repeat(p, _GRID, axis=0),
+                     np.exp(t).reshape(-1)).reshape(T, _GRID)
+    dt = np.diff(t, axis=1)
+
+    def cum(k):
+        a = lu + (k + 1.0) * t
+        f = np.exp(a - a.max(1, keepdims=True))
+        c = np.concatenate([np.zeros((T, 1)), np.cumsum(0.5 * (f[:, 1:] + f[:, :-1]) * dt, 1)], 1)
+        return c / c[:, -1:]
+
+    eps = 0.05 * rtol
+    rows = np.arange(T)
+
+    def lower(c):
+        return t[rows, np.maximum((c <= eps).sum(1) - 1, 0)]
+
+    def upper(c):
+        return t[rows, np.minimum(_GRID - (c >= 1.0 - eps).sum(1), _GRID - 1)]
+
+    area = cum(2)
+    t_lo = np.where(support, lo, lower(area))
+    t_hi = np.where(support, hi, upper(area))
+    bad = ~(t_hi > t_lo)
+    t_lo, t_hi = np.where(bad, lo, t_lo), np.where(bad, hi, t_hi)
+    m_lo = np.where(support, lo, lower(cum(0)))
+    m_hi = np.where(support, hi, upper(cum(3)))
+    inside = (t >= t_lo[:, None]) & (t <= t_hi[:, None])
+    c = np.where(inside, lu + t, -np.inf).max(1)
+    x, w = _legendre(_MOMENT_NODES)
+    mh, mm = 0.5 * (m_hi - m_lo), 0.5 * (m_hi + m_lo)
+    tm = mm[:, None] + mh[:, None] * x[None, :]
+    r = np.exp(tm)
+    lum, d = log_density(np.repeat(kind, _MOMENT_NODES), np.repeat(p, _MOMENT_NODES, axis=0),
+                         r.reshape(-1), derivatives=True)
+    f = w[None, :] * mh[:, None] * np.exp(lum.reshape(T, -1) + tm - c[:, None])
+    d = d.reshape(T, _MOMENT_NODES, 3)
+    fk = np.stack([f, f * r * r, f * r ** 3], 1)  # (T, 3, n)
+    mom = fk.sum(2)
+    dmom = np.einsum("tkn,tnp->tkp", fk, d)
+    for i, q in enumerate(live):
+        q.t_lo, q.t_hi, q.c = float(t_lo[i]), float(t_hi[i]), float(c[i])
+        q.moments, q.d_moments = mom[i], dmom[i]
+
+
 def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
           allow_unvalidated: bool) -> int:
     """Places every task's panels (see the module's notes); returns the Mie terms evaluated."""
     terms = 0
+    _prepare(tasks, rtol)
     for q in tasks:
         if q.kind == MONODISPERSE:
             q.done = q.converged = True
             continue
-        q.t_lo, q.t_hi, q.c = integration_range(q.kind, q.p, rtol)
-        q.moments, q.d_moments = moment_integrals(q.kind, q.p, q.c, rtol)
         edges = np.linspace(q.t_lo, q.t_hi, _INITIAL_PANELS + 1)
         q.a, q.b = edges[:-1].copy(), edges[1:].copy()
         q.K, q.G = _nan(_INITIAL_PANELS), _nan(_INITIAL_PANELS)
