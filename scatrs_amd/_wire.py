@@ -14,9 +14,11 @@ return the same dict of arrays, one row per wavelength (``ensemble_compute_batch
   with cosines mu (F11, F12, F22, F33, F34, F44);
 * ``converged`` bool (n) and ``panels`` uint64 (n).
 
-On the fixed path each part (a distribution, a multimodal law's mode, a tabulated law's
-segment) is one ``n_radius``-point Gauss-Legendre panel in ln r. What stands in for the
-reference's adaptive integration (engine.rs:307-943):
+By default they run the adaptive engine (scatrs_amd.adaptive), which takes rtol,
+max_panels and refine_backscatter as the reference does. Given ``n_radius``, they run the fixed
+path instead. There each part (a distribution, a multimodal law's mode, a tabulated law's
+segment) is one ``n_radius``-point Gauss-Legendre panel in ln r, and the following stand in for
+the reference's adaptive integration (engine.rs:307-943):
 
 * the error estimate of every output is its difference from the same computation on
   ``n_radius // 2`` nodes (the estimate of the coarser rule, as the reference's Kronrod minus
@@ -541,12 +543,13 @@ def _mu(mu):
 def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool, mu,
                      normalized: bool, derivatives, rtol: float, max_panels: int,
                      allow_unvalidated: bool, refine_backscatter: bool, *,
-                     n_radius: int = 128, n_angles: int | None = None,
-                     adaptive: bool = False) -> dict:
+                     n_radius: int | None = None, n_angles: int | None = None,
+                     adaptive: bool | None = None) -> dict:
     """The ensemble outputs of the components at each wavelength as the reference's
-    ``_scatrs.ensemble_compute`` returns them (scatrs-py/src/ensemble.rs:290-355), on the
-    fixed path (``n_radius`` radius nodes per part, the Greek coefficients projected on
-    ``n_angles`` Gauss-Legendre cosines, 128 by default) or, with ``adaptive``, by the adaptive
+    ``_scatrs.ensemble_compute`` returns them (scatrs-py/src/ensemble.rs:290-355): by the
+    adaptive engine unless ``n_radius`` is given (or ``adaptive`` is False), else on the
+    fixed path (``n_radius`` radius nodes per part, 128 without it, the Greek coefficients
+    projected on ``n_angles`` Gauss-Legendre cosines, 128 by default). The adaptive
     engine (scatrs_amd.adaptive: rtol, max_panels and refine_backscatter as the reference
     uses them; n_angles=None sizes each Greek projection's rule to its nodes' largest N;
     derivatives on the final panels with the range held fixed); ``derivatives`` are indices
@@ -569,6 +572,8 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
     n_mu = 0 if mu is None else mu.size
     _check_options(rtol, max_panels, wl.size, n_mu, order, len(chosen))
     W = 0 if order is None else order + 1
+    if adaptive is None:
+        adaptive = n_radius is None
     if adaptive:
         V, E, D, DE, conv, panels = _run_adaptive(
             [comps], wl, order=order, mu=mu, normalized=normalized, keys=chosen, rtol=rtol,
@@ -579,7 +584,7 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
         return out
     V, E, D, DE, conv, counts = _run(
         [comps], wl, order=order, mu=mu, normalized=normalized, keys=chosen, rtol=rtol,
-        refine_backscatter=refine_backscatter, n_radius=n_radius,
+        refine_backscatter=refine_backscatter, n_radius=128 if n_radius is None else n_radius,
         n_angles=128 if n_angles is None else n_angles, allow_unvalidated=allow_unvalidated)
     return _dict(V, E, D, DE, conv, counts, wl.size, W, n_mu)
 
@@ -587,8 +592,8 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
 def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders: bool, mu,
                            normalized: bool, rtol: float, max_panels: int,
                            allow_unvalidated: bool, refine_backscatter: bool, *,
-                           n_radius: int = 128, n_angles: int | None = None,
-                           adaptive: bool = False) -> dict:
+                           n_radius: int | None = None, n_angles: int | None = None,
+                           adaptive: bool | None = None) -> dict:
     """The outputs of each distribution (tagged tuples ``specs``) of one particle at each
     wavelength, without derivatives, one row per (distribution, wavelength),
     distribution-major, as the reference's ``_scatrs.ensemble_compute_batch``
@@ -603,6 +608,8 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
     n_mu = 0 if mu is None else mu.size
     _check_options(rtol, max_panels, len(units) * wl.size, n_mu, order, 0)
     W = 0 if order is None else order + 1
+    if adaptive is None:
+        adaptive = n_radius is None
     if not units:
         import torch
 
@@ -619,6 +626,6 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
         return out
     V, E, _, _, conv, counts = _run(
         units, wl, order=order, mu=mu, normalized=normalized, keys=[], rtol=rtol,
-        refine_backscatter=refine_backscatter, n_radius=n_radius,
+        refine_backscatter=refine_backscatter, n_radius=128 if n_radius is None else n_radius,
         n_angles=128 if n_angles is None else n_angles, allow_unvalidated=allow_unvalidated)
     return _dict(V, E, None, None, conv, counts, wl.size, W, n_mu)
