@@ -226,9 +226,10 @@ def _nodes(tasks, ti, a, b, idx, derivatives: bool = False):
     half, mid = 0.5 * (b - a), 0.5 * (a + b)
     t = mid[:, None] + half[:, None] * x[None, :]
     r = np.exp(t)
-    kind = np.array([tasks[i].kind for i in ti], np.int32)
-    p = np.stack([tasks[i].p for i in ti]) if len(ti) else np.zeros((0, 3))
-    c = np.array([tasks[i].c for i in ti])
+    ti = np.asarray(ti, np.int64)
+    kind = np.array([q.kind for q in tasks], np.int32)[ti]
+    p = np.array([q.p for q in tasks]).reshape(-1, 3)[ti]
+    c = np.array([q.c for q in tasks])[ti]
     k = idx.size
     out = log_density(np.repeat(kind, k), np.repeat(p, k, axis=0), r.reshape(-1), derivatives)
     lu = out[0] if derivatives else out
@@ -248,7 +249,8 @@ def _call(tasks, ti, nodes, *, order=None, n_angles=2, mu=None, want_phase=False
 
     dev = torch.device("cuda", torch.cuda.current_device())
     R = ti.size
-    layers = np.array([tasks[i].layers for i in ti])
+    layers_t = np.array([q.layers for q in tasks])
+    layers = layers_t[ti]
     sc = np.empty((R, 11))
     gr = None if order is None else np.empty((R, 6, order + 1))
     n_phase = n_angles if mu is None else int(np.size(mu))
@@ -260,8 +262,11 @@ def _call(tasks, ti, nodes, *, order=None, n_angles=2, mu=None, want_phase=False
     for L in np.unique(layers):
         sel = np.flatnonzero(layers == L)
         tk = ti[sel]
-        m = np.stack([tasks[i].m for i in tk])  # (g, L) complex
-        fr = np.stack([tasks[i].fr for i in tk]) if L > 1 else None
+        own = np.flatnonzero(layers_t == L)  # the tasks of L layers, then their rows
+        pos = np.full(len(tasks), -1)
+        pos[own] = np.arange(own.size)
+        m = np.stack([tasks[i].m for i in own])[pos[tk]]  # (g, L) complex
+        fr = np.stack([tasks[i].fr for i in own])[pos[tk]] if L > 1 else None
         s, g, p, nt = ensemble_nodes_device(
             t(nodes[sel]), t(wl_all[tk]), t(m[:, 0] if L == 1 else m),
             None if fr is None else t(fr), n_angles=n_angles, order=order,
@@ -375,12 +380,18 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             ti = np.concatenate([np.full(j.size, i) for i, j in todo])
             pa = np.concatenate([tasks[i].a[j] for i, j in todo])
             pb = np.concatenate([tasks[i].b[j] for i, j in todo])
-            for idx, name in ((np.arange(15), "K"), (_GAUSS, "G")):
-                nodes = np.stack(_nodes(tasks, ti, pa, pb, idx), -1)
-                sc, _, _, z, nt = _call(tasks, ti, nodes, allow_unvalidated=allow_unvalidated)
-                terms += nt
-                v = _refined(sc, z)
-                k = 0
+            # One call for both rules: the Gauss rows are the same 15 nodes with the Gauss
+            # weights (0 at the Kronrod-only nodes), which halves the calls of a round.
+            r, wk = _nodes(tasks, ti, pa, pb, np.arange(15))
+            wg = wk * np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
+            nodes = np.concatenate([np.stack([r, wk], -1), np.stack([r, wg], -1)])
+            sc, _, _, z, nt = _call(tasks, np.concatenate([ti, ti]), nodes,
+                                    allow_unvalidated=allow_unvalidated)
+            terms += nt
+            v = _refined(sc, z)
+            n_open = ti.size
+            for name, off in (("K", 0), ("G", n_open)):
+                k = off
                 for i, j in todo:
                     getattr(tasks[i], name)[j] = v[k:k + j.size]
                     k += j.size
