@@ -183,7 +183,8 @@ def _nodes_case():
 
 def test_caller_nodes_vjp_is_the_fixed_range_vjp():
     """scatrs_ensemble_nodes_vjp on the fixed path's nodes, with d ln W / d p from the density,
-    equals scatrs_ensemble_fixed_vjp with the range held fixed, every column."""
+    equals scatrs_ensemble_fixed_vjp with the range held fixed, every column (spheres, and
+    coated spheres with phase cotangents)."""
     from scatrs_amd.ensemble import ensemble_fixed_vjp_device, ensemble_nodes_vjp_device
 
     kind, p, r, W, d = _nodes_case()
@@ -202,6 +203,21 @@ def test_caller_nodes_vjp_is_the_fixed_range_vjp():
     g1, g2 = g1.cpu().numpy(), g2.cpu().numpy()
     assert np.max(np.abs(g1 - g2)) <= 1e-11 * np.max(np.abs(g2)), (g1, g2)
     assert t1 == t2
+    # coated spheres: every layer's m and the radius fraction
+    ml = np.array([[1.6 + 0.1j, 1.38 + 1e-4j]])
+    fr = _dev(np.array([[0.55]]))
+    bp = rng.normal(size=(1, 6, na))
+    g1, _ = ensemble_nodes_vjp_device(
+        _dev(np.stack([r, W], -1)[None]), _dev(d[None]), _dev(np.array([0.9])), _dev(ml), fr,
+        bar_scalars=_dev(bs), bar_greek=_dev(bg), bar_phase=_dev(bp), n_angles=na, order=L)
+    g2, _ = ensemble_fixed_vjp_device(
+        _dev(np.array([kind], np.int32)), _dev(np.array([p[0]])), _dev(np.array([p[1]])),
+        _dev(np.array([0.9])), _dev(ml), fr, p2=_dev(np.array([p[2]])), bar_scalars=_dev(bs),
+        bar_greek=_dev(bg), bar_phase=_dev(bp), n_radius=r.size, n_angles=na, order=L,
+        fixed_range=True)
+    g1, g2 = g1.cpu().numpy(), g2.cpu().numpy()
+    assert g1.shape == (1, 3 + 4 + 1)
+    assert np.max(np.abs(g1 - g2)) <= 1e-11 * np.max(np.abs(g2)), (g1, g2)
 
 
 def test_adaptive_derivative_columns_match_central_differences():
