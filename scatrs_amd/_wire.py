@@ -187,11 +187,9 @@ def ensemble_parameters(items, mixture: bool) -> list[str]:
 
 
 def _order(greek, all_orders: bool):
-    """The Greek order, or None (scatrs-py/src/ensemble.rs:371-381)."""
+    """The Greek order, "all", or None (scatrs-py/src/ensemble.rs:371-381)."""
     if all_orders:
-        raise InvalidInputError(
-            "all_orders needs the adaptive engine's band layout; the fixed path takes a Greek "
-            "order (greek=L)")
+        return "all"
     if greek is None:
         return None
     if greek < 0 or greek > MAX_GREEK_ORDER:
@@ -201,7 +199,9 @@ def _order(greek, all_orders: bool):
 
 def _check_options(rtol, max_panels, n, n_mu, order, p):
     """The reference's option checks (scatrs/src/ensemble/mod.rs:689-703) and its bound on
-    the outputs (scatrs-py/src/ensemble.rs:382-393)."""
+    the outputs (scatrs-py/src/ensemble.rs:382-393; every order: known after pass 1)."""
+    if order == "all":
+        order = None
     if not (1e-13 <= rtol <= 0.1):
         raise InvalidInputError(f"rtol must be in [1e-13, 0.1], got {rtol}")
     if max_panels < 1:
@@ -535,6 +535,57 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
     return V, (V - Vg).abs(), D, DE, torch.from_numpy(conv), panels
 
 
+def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidated):
+    """The reference's every order (GreekOrder::All, engine.rs:2034-2037): twice the largest N
+    at each (unit, wavelength)'s final nodes, from a pass 1 of its own, (U, n)."""
+    import torch
+
+    from . import _native, adaptive
+    from .size import MONODISPERSE
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n = wl.size
+    with torch.no_grad():
+        parts = _parts(units, _Leaves(units, n, 1, dev, False))
+        tasks = []
+        for (_, kind, p, m_re, m_im, fr, _, _) in parts:
+            p0 = p.cpu().numpy()
+            m0 = (m_re + 1j * m_im).cpu().numpy()
+            f0 = fr.cpu().numpy()
+            for i in range(n):
+                tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
+                                           f0[i].copy()))
+    adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
+    lib = _native.load()
+    out = np.zeros((len(units), n), np.int64)
+    for k, q in enumerate(tasks):
+        u, i = parts[k // n][0], k % n
+        if q.kind == MONODISPERSE:
+            r = q.p[0]
+        else:  # the last panel's outermost Kronrod node
+            r = np.exp(0.5 * (q.a[-1] + q.b[-1]) + 0.5 * (q.b[-1] - q.a[-1]) * adaptive._X[-1])
+        out[u, i] = max(out[u, i], 2 * int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl)))
+    return out
+
+
+def _clip_orders(V, E, D, DE, orders, W, n_mu):
+    """Zeros each (unit, wavelength)'s Greek coefficients above its own order (the reference
+    pads every result to the longest with zeros, scatrs-py/src/ensemble.rs:396-403)."""
+    l = np.arange(W)
+    for u in range(V.shape[0]):
+        for i in range(V.shape[1]):
+            cut = l > orders[u, i]
+            if not cut.any():
+                continue
+            for k in range(6):
+                cols = _N + k * W + np.flatnonzero(cut)
+                V[u, i, cols] = 0.0
+                E[u, i, cols] = 0.0
+                if D is not None and u == 0:
+                    D[:, cols, i] = 0.0
+                    DE[:, cols, i] = 0.0
+
+
 def _mu(mu):
     mu = np.ascontiguousarray(np.asarray(mu, np.float64).reshape(-1))
     return mu if mu.size else None
@@ -571,14 +622,24 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
     order = _order(greek, all_orders)
     n_mu = 0 if mu is None else mu.size
     _check_options(rtol, max_panels, wl.size, n_mu, order, len(chosen))
-    W = 0 if order is None else order + 1
     if adaptive is None:
         adaptive = n_radius is None
+    orders = None
+    if order == "all":
+        if not adaptive:
+            raise InvalidInputError("every Greek order (all_orders) needs the adaptive engine; "
+                                    "the fixed path takes greek=L")
+        orders = _all_orders([comps], wl, rtol, max_panels, refine_backscatter,
+                             allow_unvalidated)
+        order = int(orders.max())
+    W = 0 if order is None else order + 1
     if adaptive:
         V, E, D, DE, conv, panels = _run_adaptive(
             [comps], wl, order=order, mu=mu, normalized=normalized, keys=chosen, rtol=rtol,
             max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
             allow_unvalidated=allow_unvalidated)
+        if orders is not None:
+            _clip_orders(V, E, D, DE, orders, W, n_mu)
         out = _dict(V, E, D, DE, conv, np.zeros(1, np.int64), wl.size, W, n_mu)
         out["panels"] = panels.reshape(-1).astype(np.uint64)
         return out
@@ -607,9 +668,17 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
     order = _order(greek, all_orders)
     n_mu = 0 if mu is None else mu.size
     _check_options(rtol, max_panels, len(units) * wl.size, n_mu, order, 0)
-    W = 0 if order is None else order + 1
     if adaptive is None:
         adaptive = n_radius is None
+    orders = None
+    if order == "all":
+        if not adaptive:
+            raise InvalidInputError("every Greek order (all_orders) needs the adaptive engine; "
+                                    "the fixed path takes greek=L")
+        orders = _all_orders(units, wl, rtol, max_panels, refine_backscatter,
+                             allow_unvalidated) if units else np.zeros((0, wl.size), np.int64)
+        order = int(orders.max()) if orders.size else 0
+    W = 0 if order is None else order + 1
     if not units:
         import torch
 
@@ -621,6 +690,8 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
             units, wl, order=order, mu=mu, normalized=normalized, keys=[], rtol=rtol,
             max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
             allow_unvalidated=allow_unvalidated)
+        if orders is not None:
+            _clip_orders(V, E, None, None, orders, W, n_mu)
         out = _dict(V, E, None, None, conv, np.zeros(len(units), np.int64), wl.size, W, n_mu)
         out["panels"] = panels.reshape(-1).astype(np.uint64)
         return out

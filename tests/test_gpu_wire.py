@@ -193,3 +193,28 @@ def test_errors_are_the_two_rule_difference_and_set_converged():
                                     n_radius=8, n_angles=64)
     assert not coarse["converged"].any()
     assert np.all(coarse["errors"][0] > 1e-6 * coarse["values"][0])
+
+
+def test_every_greek_order_on_the_adaptive_engine():
+    """all_orders (the reference's GreekOrder::All, engine.rs:2034-2037): each wavelength's
+    coefficients up to twice the largest N of its final nodes, zero beyond (padded to the
+    longest); exact projections, so alpha1_0 = 1 and alpha1_1 = 3 g; the coefficients equal
+    the same call with that order given."""
+    from scatrs_amd import _wire
+
+    wl = np.array([0.45, 1.1])
+    items = [(("lognormal", 0.1, 1.5), ("sphere", np.full(2, 1.5 + 0.01j)), 1.0)]
+    raw = _wire.ensemble_compute(items, wl, False, None, True, np.zeros(0), True, [], *OPTS)
+    a1 = raw["greek"][0]
+    W = a1.shape[1]
+    lengths = [int(np.max(np.flatnonzero(a1[i] != 0.0))) + 1 for i in range(2)]
+    assert max(lengths) == W and lengths[0] != lengths[1]
+    for i in range(2):  # orders 0 .. 2 N, zero beyond
+        assert (lengths[i] - 1) % 2 == 0
+        assert np.all(raw["greek"][:, i, lengths[i]:] == 0.0)
+    assert np.allclose(a1[:, 0], 1.0, rtol=0, atol=1e-13)
+    assert np.allclose(a1[:, 1], 3 * raw["values"][5], rtol=1e-10, atol=0)
+    ref = _wire.ensemble_compute(items, wl, False, W - 1, False, np.zeros(0), True, [], *OPTS)
+    for i in range(2):
+        k = lengths[i]
+        assert np.allclose(raw["greek"][:, i, :k], ref["greek"][:, i, :k], rtol=0, atol=1e-13)

@@ -80,8 +80,9 @@ def test_inputs_are_checked_as_the_reference_checks_them():
         _compute(one, max_panels=0)
     with pytest.raises(InvalidInputError, match="at most 2097152"):
         _compute(one, greek=2 * (1 << 20) + 1)
-    with pytest.raises(InvalidInputError, match="all_orders"):
-        _compute(one, all_orders=True)
+    with pytest.raises(InvalidInputError, match="all_orders"):  # the fixed path's
+        _wire.ensemble_compute(one, WL4, False, None, True, np.zeros(0), True, [], 1e-6, 4096,
+                               False, False, n_radius=64)
     with pytest.raises(InvalidInputError, match="component 0 has 4 refractive indices for 3"):
         _compute(one, wl=WL4[:3])
     with pytest.raises(InvalidInputError, match="weight of component 1 must be finite"):
