@@ -290,3 +290,49 @@ def test_adaptive_layered_and_tabulated():
     assert a.converged.all() and np.all(a.panels >= 3 * 4)
     for k in ("c_ext", "c_sca", "g", "effective_radius", "mean_volume"):
         assert np.allclose(getattr(a, k), getattr(f, k), rtol=1e-7, atol=0), k
+
+
+def test_refine_backscatter_and_layered_derivatives():
+    """refine_backscatter adds C_back to the refined quantities (engine.rs:675-690): it takes
+    at least as many panels and its C_back error meets rtol against max(C_back, C_sca). A
+    coated-sphere gamma law's adaptive derivative columns (both layers' m, the radius
+    fraction, the effective variance) against central differences of the engine at rtol 1e-10."""
+    from scatrs_amd import _wire
+
+    wl = np.array([0.7])
+    sph = [(("lognormal", 0.4, 1.5), ("sphere", np.array([1.45 + 1e-4j])), 1.0)]
+    a = _wire.ensemble_compute(sph, wl, False, None, False, np.zeros(0), True, [], 1e-6, 4096,
+                               False, False)
+    b = _wire.ensemble_compute(sph, wl, False, None, False, np.zeros(0), True, [], 1e-6, 4096,
+                               False, True)
+    assert a["converged"].all() and b["converged"].all()
+    assert b["panels"][0] >= a["panels"][0]
+    assert b["errors"][3, 0] <= 1e-6 * max(b["values"][3, 0], b["values"][1, 0])
+
+    def items(f=0.6, m0=1.6 + 0.1j, m1=1.38 + 1e-3j, v=0.15):
+        m = np.array([[m0], [m1]])
+        return [(("gamma", 0.4, v), ("layered", np.array([f]), m), 1.0)]
+
+    names = _wire.ensemble_parameters(items(), False)
+    want = ["effective_variance", "layer0.m_real", "layer0.m_imag", "layer1.m_real",
+            "layer1.m_imag", "layer0.radius_fraction"]
+    raw = _wire.ensemble_compute(items(), wl, False, 4, False, np.zeros(0), True,
+                                 [names.index(n) for n in want], 1e-10, 4096, False, False)
+    assert raw["converged"].all()
+    base = dict(f=0.6, m0=1.6 + 0.1j, m1=1.38 + 1e-3j, v=0.15)
+    # steps of 1e-4 relative, 1e-5 in Im m (the shell's nearly linear absorption)
+    steps = [("v", 1e-4 * 0.15), ("m0", 1e-4 * 1.6), ("m0", 1e-5j), ("m1", 1e-4 * 1.38),
+             ("m1", 1e-5j), ("f", 1e-4 * 0.6)]
+    sel = [0, 1, 2, 4, 5]
+    for k, (key, h) in enumerate(steps):
+        up = _wire.ensemble_compute(items(**dict(base, **{key: base[key] + h})), wl, False, 4,
+                                    False, np.zeros(0), True, [], 1e-10, 4096, False, False)
+        dn = _wire.ensemble_compute(items(**dict(base, **{key: base[key] - h})), wl, False, 4,
+                                    False, np.zeros(0), True, [], 1e-10, 4096, False, False)
+        fd = (up["values"] - dn["values"]) / (2 * abs(h))
+        scale = np.max(np.abs(fd[sel]), 1, keepdims=True)
+        err = np.abs(raw["d"][k][sel] - fd[sel]) / scale
+        assert err.max() <= 2e-5, (want[k], err.max())
+        fdg = (up["greek"] - dn["greek"]) / (2 * abs(h))
+        errg = np.abs(raw["greek_d"][k] - fdg) / np.max(np.abs(fdg))
+        assert errg.max() <= 2e-5, (want[k], errg.max())
