@@ -22,7 +22,7 @@ the reference's adaptive integration (engine.rs:307-943):
 
 * the error estimate of every output is its difference from the same computation on
   ``n_radius // 2`` nodes (the estimate of the coarser rule, as the reference's Kronrod minus
-  Gauss difference is);
+  Gauss difference is; the adaptive engine's are the reference's bounds, _error_bounds);
 * ``converged`` applies the reference's refinement targets (engine.rs:620-690) to those
   differences part by part: C_ext, C_sca and C_abs (against C_ext's rtol), C_sca g against
   C_sca's, and with ``refine_backscatter`` C_back against the larger of C_back and C_sca;
@@ -497,10 +497,12 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
         f = torch.stack([part[6] for part in parts])  # (parts, B)
         P = len(parts)
         out = []
+        rows = []
         for idx in (np.arange(15), adaptive._GAUSS):
-            avg, gr, ph, _, _ = adaptive.pass2_torch(
+            avg, gr, ph, _, _, rw = adaptive.pass2_torch(
                 tasks, params, R=R, idx=idx, order=order, n_angles=n_angles, mu=mu,
                 want_phase=mu is not None, allow_unvalidated=allow_unvalidated)
+            rows.append(rw)
             # (P n, R, ...) -> (P, B, ...), b = r n + i
             per = lambda a: None if a is None else a.reshape(  # noqa: E731
                 (P, n, R) + a.shape[2:]).transpose(1, 2).reshape((P, B) + a.shape[2:])
@@ -524,7 +526,11 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                 grads = {id(t): g for t, g in zip(ts, gs)}
                 D = torch.stack([leaves.column(0, k, grads).reshape(R, n) for k in keys])
             out.append((torch.stack([y.detach()[:n] for y in Y]), D))
+            if len(out) == 1:
+                kept = (avg.detach(), None if gr is None else gr.detach(),
+                        None if ph is None else ph.detach(), f.detach())
     (V, D), (Vg, Dg) = out
+    E = _error_bounds(rows, kept, unit, U, P, n, R, normalized)
     conv = np.ones((U, n), bool)
     panels = np.zeros((U, n), np.int64)
     for k, q in enumerate(tasks):
@@ -532,7 +538,70 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
         conv[u, i] &= q.converged
         panels[u, i] += q.n_panels
     DE = None if D is None else (D - Dg).abs()
-    return V, (V - Vg).abs(), D, DE, torch.from_numpy(conv), panels
+    return V, E, D, DE, torch.from_numpy(conv), panels
+
+
+def _error_bounds(rows, kept, unit, U, P, n, R, normalized):
+    """The values' error bounds as the reference forms them (lin.rs:7-68): each linear sum's
+    error is the sum over pass-2 rows of |Kronrod - Gauss| (engine.rs:1697-1763), over the
+    moment integration's Z; a mixture adds its parts' errors times |fraction|; a ratio a/b
+    takes (e_a + |a/b| e_b)/|b| (ssa, g, the lidar ratio, the normalized Greek coefficients and
+    phase matrix). (U, n, K), replica 0."""
+    import math
+
+    import torch
+
+    (rk, rg), (avg, gr, ph, f) = rows, kept
+    T = P * n
+    dev = avg.device
+
+    def task_err(a, b):
+        if a is None:
+            return None
+        d = (a - b).abs()
+        return torch.zeros((T * R,) + d.shape[1:], dtype=d.dtype, device=dev).index_add(
+            0, rk["key"], d)
+
+    z = rk["z"]
+    e_lin = task_err(rk["lin"], rg["lin"]) / z[:, None]
+    e_gr = None if rk["gr"] is None else task_err(rk["gr"], rg["gr"]) / z[:, None, None]
+    e_ph = None if rk["ph"] is None else task_err(rk["ph"], rg["ph"]) / z[:, None, None]
+    # (T R, ...) -> (P, n, ...) at replica 0
+    first = lambda a: None if a is None else a.reshape((P, n, R) + a.shape[1:])[:, :, 0]  # noqa
+    e_lin, e_gr, e_ph = first(e_lin), first(e_gr), first(e_ph)
+    e_lin[..., 5:] = 0.0  # <r^2>, <r^3>: the moment integration's, errors below rtol/10
+    vals = lambda a: None if a is None else a.reshape((P, R, n) + a.shape[2:])[:, 0]  # noqa
+    fa = f.reshape(P, R, n)[:, 0].abs()
+    mix = lambda a, w: torch.zeros((U,) + a.shape[1:], dtype=a.dtype,  # noqa: E731
+                                   device=dev).index_add(0, unit, w * a)
+    av = mix(vals(avg), f.reshape(P, R, n)[:, 0][..., None])
+    ea = mix(e_lin, fa[..., None])
+    ext, sca, absb, back, scag = (av[..., k] for k in range(5))
+    e_ext, e_sca, e_abs, e_back, e_scag = (ea[..., k] for k in range(5))
+
+    def div(va, ea_, vb, eb):
+        q = va / vb
+        return (ea_ + q.abs() * eb) / vb.abs()
+
+    zero = torch.zeros_like(ext)
+    es = torch.stack([e_ext, e_sca, e_abs, e_back, div(sca, e_sca, ext, e_ext),
+                      div(scag, e_scag, sca, e_sca), 4.0 * math.pi * div(ext, e_ext, back, e_back),
+                      zero, zero, zero, zero], -1)
+    cols = [es]
+    if gr is not None:
+        g_v = mix(vals(gr), f.reshape(P, R, n)[:, 0][..., None, None])
+        g_e = mix(e_gr, fa[..., None, None])
+        if normalized:
+            a0, e0 = g_v[..., 0:1, 0:1], g_e[..., 0:1, 0:1]
+            g_e = div(g_v, g_e, a0, e0)
+        cols.append(g_e.reshape(g_e.shape[:2] + (-1,)))
+    if ph is not None:
+        p_v = mix(vals(ph), f.reshape(P, R, n)[:, 0][..., None, None])
+        p_e = mix(e_ph, fa[..., None, None])
+        if normalized:
+            p_e = div(p_v, p_e, sca[..., None, None], e_sca[..., None, None])
+        cols.append(p_e.reshape(p_e.shape[:2] + (-1,)))
+    return torch.cat(cols, -1)
 
 
 def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidated):

@@ -573,7 +573,9 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     parameters (torch.autograd): params[t] = (p (R, 3), m_real (R, L), m_imag (R, L),
     fractions (R, L-1) or None), R replicas of task t's parameters. Returns the linear averages
     [C_ext, C_sca, C_abs, C_back, C_sca g, r^2, r^3] (T, R, 7), greek (T, R, 6, order+1) and
-    phase (T, R, 6, n) (normalized=False; None when not asked for), and the terms.
+    phase (T, R, 6, n) (normalized=False; None when not asked for), the terms, the phase
+    rule's size, and each row's sums (detached: key = task x R + replica, lin (.., 7), gr, ph,
+    and z, every task-replica's Z), for the error bounds.
 
     The rows are pass2's. A row's sum of weights Z enters to first order in p, Z0 + (p - p0)
     . sum W d ln W / d p, which has the right value and derivative at p0; a monodisperse task is
@@ -609,14 +611,29 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             mr, mi = mr[:, 0], mi[:, 0]
         return pr, mr, mi, fr
 
-    def add(key, z, sc, g, p_):
+    # Each row's sums (detached), at its place among all rows x replicas, for the error bounds.
+    n_rows = len(rows) + len(mono)
+    row_lin = torch.zeros((n_rows * R, 7), dtype=f64, device=dev)
+    row_gr = torch.zeros((n_rows * R, 6, Wd), dtype=f64, device=dev) if gr is not None else None
+    row_ph = torch.zeros((n_rows * R, 6, n_phase), dtype=f64, device=dev) if ph is not None \
+        else None
+    row_key = torch.zeros(n_rows * R, dtype=torch.int64, device=dev)
+
+    def add(key, z, sc, g, p_, place):
         nonlocal avg, zt, gr, ph
-        avg = avg.index_add(0, key, z[:, None] * number_averages(sc))
+        lin = z[:, None] * number_averages(sc)
+        avg = avg.index_add(0, key, lin)
         zt = zt.index_add(0, key, z)
+        row_lin[place] = lin.detach()
+        row_key[place] = key
         if gr is not None:
-            gr = gr.index_add(0, key, z[:, None, None] * g)
+            gz = z[:, None, None] * g
+            gr = gr.index_add(0, key, gz)
+            row_gr[place] = gz.detach()
         if ph is not None:
-            ph = ph.index_add(0, key, z[:, None, None] * p_)
+            pz = z[:, None, None] * p_
+            ph = ph.index_add(0, key, pz)
+            row_ph[place] = pz.detach()
 
     wl_all = np.array([q.wl for q in tasks])
     layers = np.array([q.layers for q in tasks])
@@ -652,7 +669,8 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
                                            phase=want_phase)
                 z = t(z0[sel]) + ((pr - pr.detach()) * t(zd[sel])).sum(1)
                 key = (torch.from_numpy(tk).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
-                add(key, z, sc, g, p_)
+                place = (torch.from_numpy(sel).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
+                add(key, z, sc, g, p_, place)
     for k, ti in enumerate(mono):
         na = int(rules[nr + k])
         L = tasks[ti].layers
@@ -662,7 +680,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             (R,), tasks[ti].wl, dtype=f64, device=dev), mr, mi, fr, p2=pr[:, 2], n_radius=1,
             n_angles=na, order=order, normalized=False, fixed_range=True,
             allow_unvalidated=allow_unvalidated, mu=mu, phase=want_phase)
-        add(ti * R + rep, torch.ones(R, dtype=f64, device=dev), sc, g, p_)
+        add(ti * R + rep, torch.ones(R, dtype=f64, device=dev), sc, g, p_, (nr + k) * R + rep)
     # The moment integration's Z, M2, M3 (first order in p: the right value and derivative at
     # p0), in place of the nodes' own; a monodisperse law keeps its node's.
     pall = torch.cat([params[t][0] for t in range(T)])  # (T R, 3)
@@ -678,7 +696,9 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     gr = None if gr is None else gr / zm[:, None, None]
     ph = None if ph is None else ph / zm[:, None, None]
     shape = lambda a: None if a is None else a.reshape((T, R) + a.shape[1:])  # noqa: E731
-    return shape(avg), shape(gr), shape(ph), terms, (int(rules[0]) if len(rules) else 2)
+    rows_out = dict(key=row_key, lin=row_lin, gr=row_gr, ph=row_ph, z=zm.detach())
+    return (shape(avg), shape(gr), shape(ph), terms, (int(rules[0]) if len(rules) else 2),
+            rows_out)
 
 
 def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_panels: int,

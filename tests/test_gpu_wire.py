@@ -218,3 +218,22 @@ def test_every_greek_order_on_the_adaptive_engine():
     for i in range(2):
         k = lengths[i]
         assert np.allclose(raw["greek"][:, i, :k], ref["greek"][:, i, :k], rtol=0, atol=1e-13)
+
+
+def test_adaptive_errors_follow_the_references_propagation():
+    """The adaptive engine's errors are the reference's bounds (lin.rs:24-44): the sums' errors
+    add |Kronrod - Gauss| over the rows, and a ratio a/b takes (e_a + |a/b| e_b)/|b| (ssa =
+    C_sca/C_ext, lidar ratio = 4 pi C_ext/C_back); the refined quantities' bounds meet rtol."""
+    from scatrs_amd import _wire
+
+    wl = np.array([0.5, 0.9])
+    items = [(("lognormal", 0.3, 1.7), ("sphere", np.full(2, 1.45 + 2e-3j)), 1.0)]
+    raw = _wire.ensemble_compute(items, wl, False, 6, False, np.zeros(0), True, [], 1e-6, 4096,
+                                 False, False)
+    v, e = raw["values"], raw["errors"]
+    assert np.all(e[0] <= 1e-6 * v[0]) and np.all(e[1] <= 1e-6 * v[1])
+    assert np.all(e[[0, 1, 2, 3]] > 0.0)
+    assert np.allclose(e[4], (e[1] + v[4] * e[0]) / v[0], rtol=1e-12, atol=0)
+    assert np.allclose(e[6], 4 * np.pi * (e[0] + v[6] / (4 * np.pi) * e[3]) / v[3], rtol=1e-12,
+                       atol=0)
+    assert np.all(e[8:11] == 0.0) and np.all(e[7] == 0.0)
