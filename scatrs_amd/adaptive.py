@@ -362,84 +362,135 @@ def _prepare(tasks, rtol: float) -> None:
 
 def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
           allow_unvalidated: bool) -> int:
-    """Places every task's panels (see the module's notes); returns the Mie terms evaluated."""
+    """Places every task's panels (see the module's notes); returns the Mie terms evaluated.
+
+    The panels of all tasks are one table sorted by (task, ln r); every round's decisions are
+    segment reductions over it."""
     terms = 0
     _prepare(tasks, rtol)
+    live = np.array([q.kind != MONODISPERSE for q in tasks])
     for q in tasks:
         if q.kind == MONODISPERSE:
             q.done = q.converged = True
-            continue
-        edges = np.linspace(q.t_lo, q.t_hi, _INITIAL_PANELS + 1)
-        q.a, q.b = edges[:-1].copy(), edges[1:].copy()
-        q.K, q.G = _nan(_INITIAL_PANELS), _nan(_INITIAL_PANELS)
+    ids = np.flatnonzero(live)
+    T = len(tasks)
+    if not ids.size:
+        return terms
+    kinds = np.array([q.kind for q in tasks])
+    fixed_support = np.isin(kinds, (POWER_LAW, SEGMENT))
+    t_lo = np.array([q.t_lo for q in tasks])
+    t_hi = np.array([q.t_hi for q in tasks])
+    g = np.linspace(0.0, 1.0, _INITIAL_PANELS + 1)
+    edges = t_lo[ids, None] + (t_hi - t_lo)[ids, None] * g[None, :]
+    task = np.repeat(ids, _INITIAL_PANELS)
+    a = edges[:, :-1].reshape(-1)
+    b = edges[:, 1:].reshape(-1)
+    K = np.full((a.size, 5), np.nan)
+    G = np.full((a.size, 5), np.nan)
+    done = ~live
+    conv = ~live
+    fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
     for _ in range(_MAX_ROUNDS):
-        todo = [(i, np.flatnonzero(np.isnan(q.K[:, 0]))) for i, q in enumerate(tasks)
-                if not q.done]
-        todo = [(i, j) for i, j in todo if j.size]
-        if todo:
-            ti = np.concatenate([np.full(j.size, i) for i, j in todo])
-            pa = np.concatenate([tasks[i].a[j] for i, j in todo])
-            pb = np.concatenate([tasks[i].b[j] for i, j in todo])
+        todo = np.flatnonzero(np.isnan(K[:, 0]) & ~done[task])
+        if todo.size:
+            ti = task[todo]
+            r, wk = _nodes(tasks, ti, a[todo], b[todo], np.arange(15))
             # One call for both rules: the Gauss rows are the same 15 nodes with the Gauss
             # weights (0 at the Kronrod-only nodes), which halves the calls of a round.
-            r, wk = _nodes(tasks, ti, pa, pb, np.arange(15))
-            wg = wk * np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
-            nodes = np.concatenate([np.stack([r, wk], -1), np.stack([r, wg], -1)])
+            nodes = np.concatenate([np.stack([r, wk], -1), np.stack([r, wk * fold], -1)])
             sc, _, _, z, nt = _call(tasks, np.concatenate([ti, ti]), nodes,
                                     allow_unvalidated=allow_unvalidated)
             terms += nt
             v = _refined(sc, z)
-            n_open = ti.size
-            for name, off in (("K", 0), ("G", n_open)):
-                k = off
-                for i, j in todo:
-                    getattr(tasks[i], name)[j] = v[k:k + j.size]
-                    k += j.size
-        active = False
-        for q in tasks:
-            if q.done:
-                continue
-            E = np.abs(q.K - q.G)
-            tgt = _targets(q.K.sum(0), rtol, refine_backscatter)
-            fail = E.sum(0) > tgt
-            grow_lo = grow_hi = False
-            if q.kind not in (POWER_LAW, SEGMENT):
-                edge = 0.1 * tgt
-                grow_lo = bool(np.any(np.abs(q.K[0]) > edge))
-                grow_hi = bool(np.any(np.abs(q.K[-1]) > edge))
-            if not fail.any() and not grow_lo and not grow_hi:
-                q.done = q.converged = True
-                continue
-            cut = np.zeros(q.a.size, bool)
-            if fail.any():
-                ratio = E[:, fail] / np.where(tgt[fail] > 0, tgt[fail], 1.0)
-                cut = (ratio > 1.0 / q.a.size).any(1)
-                cut[int(np.argmax(ratio.max(1)))] = True
-            if q.a.size + int(cut.sum()) + grow_lo + grow_hi > max_panels:
-                q.done = True
-                continue
-            mid = 0.5 * (q.a + q.b)
-            keep = ~cut
-            a = np.concatenate([q.a[keep], q.a[cut], mid[cut]])
-            b = np.concatenate([q.b[keep], mid[cut], q.b[cut]])
-            K = np.concatenate([q.K[keep], _nan(2 * int(cut.sum()))])
-            G = np.concatenate([q.G[keep], _nan(2 * int(cut.sum()))])
-            if grow_lo:
-                w = q.b[0] - q.a[0]
-                a, b = np.append(a, q.t_lo - w), np.append(b, q.t_lo)
-                K, G = np.concatenate([K, _nan(1)]), np.concatenate([G, _nan(1)])
-                q.t_lo -= w
-            if grow_hi:
-                w = q.b[-1] - q.a[-1]
-                a, b = np.append(a, q.t_hi), np.append(b, q.t_hi + w)
-                K, G = np.concatenate([K, _nan(1)]), np.concatenate([G, _nan(1)])
-                q.t_hi += w
-            order = np.argsort(a, kind="stable")
-            q.a, q.b, q.K, q.G = a[order], b[order], K[order], G[order]
-            active = True
-        if not active:
+            K[todo], G[todo] = v[:todo.size], v[todo.size:]
+        act = ~done[task]
+        if not act.any():
             break
-    for q in tasks:
+        # Per task: totals, error sums and targets (engine.rs:620-690).
+        E = np.abs(K - G)
+        tot = np.zeros((T, 5))
+        esum = np.zeros((T, 5))
+        np.add.at(tot, task[act], K[act])
+        np.add.at(esum, task[act], E[act])
+        sca = np.abs(tot[:, 1])
+        tgt = rtol * np.stack([np.abs(tot[:, 0]), sca, sca, np.abs(tot[:, 0]),
+                               np.maximum(np.abs(tot[:, 4]), sca)], 1)
+        if not refine_backscatter:
+            tgt[:, 4] = np.inf
+        fail = esum > tgt
+        n_pan = np.bincount(task, minlength=T)
+        first = np.ones(task.size, bool)
+        first[1:] = task[1:] != task[:-1]
+        last = np.ones(task.size, bool)
+        last[:-1] = task[1:] != task[:-1]
+        edge_big = (np.abs(K) > 0.1 * tgt[task]).any(1) & ~fixed_support[task]
+        grow_lo = np.zeros(T, bool)
+        grow_hi = np.zeros(T, bool)
+        grow_lo[task[first & edge_big]] = True
+        grow_hi[task[last & edge_big]] = True
+        ok = ~fail.any(1) & ~grow_lo & ~grow_hi
+        newly = ok & ~done
+        conv |= newly
+        done |= newly
+        # Panels to bisect: error over the target's share, and each failing task's worst.
+        tg = np.where(tgt > 0, tgt, 1.0)
+        ratio = np.where(fail[task], E / tg[task], 0.0).max(1)
+        cut = (ratio > 1.0 / n_pan[task]) & ~done[task]
+        worst = np.full(T, -1.0)
+        np.maximum.at(worst, task, ratio)
+        has_fail = fail.any(1) & ~done
+        pick = has_fail[task] & (ratio == worst[task])
+        firsts = np.flatnonzero(pick)
+        seen = np.zeros(T, bool)
+        for i in firsts:  # the first panel at each failing task's largest ratio
+            if not seen[task[i]]:
+                seen[task[i]] = True
+                cut[i] = True
+        n_cut = np.bincount(task[cut], minlength=T)
+        over = (n_pan + n_cut + grow_lo + grow_hi > max_panels) & ~done
+        done |= over
+        cut &= ~done[task]
+        grow_lo &= ~done
+        grow_hi &= ~done
+        if not (cut.any() or grow_lo.any() or grow_hi.any()):
+            if done[ids].all():
+                break
+            continue
+        mid = 0.5 * (a + b)
+        keep = ~cut
+        na = [a[keep], a[cut], mid[cut]]
+        nb = [b[keep], mid[cut], b[cut]]
+        nt_ = [task[keep], task[cut], task[cut]]
+        nk = [K[keep], np.full((2 * cut.sum(), 5), np.nan)]
+        ng = [G[keep], np.full((2 * cut.sum(), 5), np.nan)]
+        lo_t = np.flatnonzero(grow_lo)
+        hi_t = np.flatnonzero(grow_hi)
+        if lo_t.size:
+            w = (b - a)[first][np.searchsorted(task[first], lo_t)]
+            na.append(t_lo[lo_t] - w)
+            nb.append(t_lo[lo_t].copy())
+            nt_.append(lo_t)
+            t_lo[lo_t] -= w
+        if hi_t.size:
+            w = (b - a)[last][np.searchsorted(task[last], hi_t)]
+            na.append(t_hi[hi_t].copy())
+            nb.append(t_hi[hi_t] + w)
+            nt_.append(hi_t)
+            t_hi[hi_t] += w
+        grown = lo_t.size + hi_t.size
+        nk.append(np.full((grown, 5), np.nan))
+        ng.append(np.full((grown, 5), np.nan))
+        a, b, task = np.concatenate(na), np.concatenate(nb), np.concatenate(nt_)
+        K, G = np.concatenate(nk), np.concatenate(ng)
+        order = np.lexsort((a, task))
+        a, b, task, K, G = a[order], b[order], task[order], K[order], G[order]
+    bounds = np.searchsorted(task, np.arange(T + 1))
+    for i in ids:
+        q = tasks[i]
+        lo, hi = bounds[i], bounds[i + 1]
+        q.a, q.b, q.K, q.G = a[lo:hi].copy(), b[lo:hi].copy(), K[lo:hi].copy(), G[lo:hi].copy()
+        q.t_lo, q.t_hi = float(t_lo[i]), float(t_hi[i])
+        q.converged = bool(conv[i])
         q.done = True
     return terms
 
