@@ -428,7 +428,8 @@ def bench_adaptive(args, torch):
     coefficients to 32 (docs/theory/ensembles.md:431-438); W1, the same lognormal at 0.3, 0.5
     and 1 um in one call, Greek to 63, rtol 3e-4; W2, 50 lognormals x 20 wavelengths, m = 1.43
     + 1e-8 i, Greek to 63, rtol 3e-4 (docs/validation/sasktran2.md:41-62). The published
-    numbers are the reference's on an Apple M4 Pro, all 12 cores."""
+    numbers are the reference's on an Apple M4 Pro, all 12 cores. The Greek coefficients are
+    projected exactly, as the reference's are: each pass-2 row on N_max + (L+1)/2 cosines."""
     from scatrs_amd import _native
     from scatrs_amd.ensemble import Ensemble
     from scatrs_amd.particles import Sphere
@@ -454,13 +455,15 @@ def bench_adaptive(args, torch):
     cases = []
     for wl, pub_s, pub_g in ((1.0, 0.037, 0.093), (0.5, 0.11, 0.31), (0.3, 0.20, 0.90)):
         cases.append((f"lognormal lambda={wl} rtol 1e-6", e, np.array([wl]),
-                      dict(rtol=1e-6, phase=False), pub_s))
+                      dict(rtol=1e-6, phase=False, n_angles=None), pub_s))
         cases.append((f"lognormal lambda={wl} rtol 1e-6 Greek to 32", e, np.array([wl]),
-                      dict(rtol=1e-6, greek=32, phase=False), pub_g))
+                      dict(rtol=1e-6, greek=32, phase=False, n_angles=None), pub_g))
     cases.append(("W1: lognormal at 0.3, 0.5, 1 um, Greek to 63, rtol 3e-4", e,
-                  np.array([0.3, 0.5, 1.0]), dict(rtol=3e-4, greek=63, phase=False), 0.0838))
+                  np.array([0.3, 0.5, 1.0]), dict(rtol=3e-4, greek=63, phase=False,
+                                                  n_angles=None), 0.0838))
     cases.append(("W2: 50 lognormals x 20 wavelengths, Greek to 63, rtol 3e-4", e2,
-                  np.linspace(0.28, 1.02, 20), dict(rtol=3e-4, greek=63, phase=False), 0.83))
+                  np.linspace(0.28, 1.02, 20), dict(rtol=3e-4, greek=63, phase=False,
+                                                    n_angles=None), 0.83))
     out = []
     for name, ens, wl, kw, pub in cases:
         row = {"workload": name, "published_s_m4pro_12_cores": pub}

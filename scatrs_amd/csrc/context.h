@@ -44,10 +44,23 @@ struct scatrs_ctx {
     void *buf[kNumBufs] = {};
     size_t cap[kNumBufs] = {};
     void *host_pinned = nullptr;  // small pinned status block
-    // Ensemble setup kept between calls of the same sizes: the Gauss-Legendre rules in radius
-    // and angle (device copies in kMisc, kMisc2) and the Wigner d table (kMisc3).
-    int64_t rule_nr = -1, rule_na = -1, dtab_na = -1, dtab_order = -1;
-    std::vector<double> rr_nodes, rr_weights, ar_nodes, ar_weights;
+    // Ensemble setup kept between calls: the Gauss-Legendre rule in radius (device copy in
+    // kMisc) of the last size, and the angle rules of the last sizes, each with its device
+    // table (weights, then the non-negative cosines) and its Wigner d table for one order.
+    int64_t rule_nr = -1;
+    std::vector<double> rr_nodes, rr_weights;
+    struct AngleRule {
+        int64_t n = 0;
+        std::vector<double> nodes, weights;
+        double *d_atab = nullptr;
+        int64_t order = -1;
+        double *d_dtab = nullptr;
+        size_t dtab_cap = 0;
+        uint64_t used = 0;
+    };
+    static constexpr int kMaxAngleRules = 64;
+    std::vector<AngleRule> angle_rules;
+    uint64_t use_clock = 0;
     bool profiling = false;
     int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2: fp16-split MFMA
     std::vector<double> phase_mu;  // the caller's phase cosines (empty: the angle rule's)

@@ -1272,6 +1272,46 @@ struct EnsSetup {
     bool stage32 = false;       // the node solves stage the fp32 form (kernel 2, forward calls)
 };
 
+// The angle rule of n cosines from the context's cache, computed and uploaded on a miss (the
+// least recently used of kMaxAngleRules entries makes room).
+static int angle_rule(scatrs_ctx *ctx, int64_t n, scatrs_ctx::AngleRule **out) {
+    int rc;
+    ++ctx->use_clock;
+    for (auto &r : ctx->angle_rules)
+        if (r.n == n) {
+            r.used = ctx->use_clock;
+            *out = &r;
+            return SCATRS_OK;
+        }
+    if ((int)ctx->angle_rules.size() >= scatrs_ctx::kMaxAngleRules) {
+        auto old = std::min_element(ctx->angle_rules.begin(), ctx->angle_rules.end(),
+                                    [](const auto &x, const auto &y) { return x.used < y.used; });
+        if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"))) return rc;
+        if (old->d_atab) (void)hipFree(old->d_atab);
+        if (old->d_dtab) (void)hipFree(old->d_dtab);
+        ctx->angle_rules.erase(old);
+    }
+    scatrs_ctx::AngleRule r;
+    r.n = n;
+    r.used = ctx->use_clock;
+    gauss_legendre(n, r.nodes, r.weights);
+    const int h = (int)((n + 1) / 2);
+    std::vector<double> atab(n + h);  // weights, then the non-negative cosines
+    std::copy(r.weights.begin(), r.weights.end(), atab.begin());
+    for (int jj = 0; jj < h; ++jj) atab[n + jj] = r.nodes[n - h + jj];
+    if ((rc = check_hip(ctx, hipMalloc(&r.d_atab, atab.size() * sizeof(double)), "alloc")))
+        return rc;
+    if ((rc = check_hip(ctx, hipMemcpyAsync(r.d_atab, atab.data(), atab.size() * sizeof(double),
+                                            hipMemcpyHostToDevice, ctx->stream),
+                        "upload"))) {
+        (void)hipFree(r.d_atab);
+        return rc;
+    }
+    ctx->angle_rules.push_back(std::move(r));
+    *out = &ctx->angle_rules.back();
+    return SCATRS_OK;
+}
+
 static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, const double *d_p0,
                      const double *d_p1, const double *d_p2, const double *d_wavelength,
                      int layers,
@@ -1283,33 +1323,28 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     hipStream_t s = ctx->stream;
     const int h = (int)((n_angles + 1) / 2);
     S.h = h;
-    // The Gauss-Legendre rules (device copies in kMisc, kMisc2) and the Wigner d table (kMisc3)
-    // are kept in the context between calls of the same sizes. The caller's nodes need no
-    // radius rule (a one-node stand-in: the host rule costs O(n^2)).
+    // The Gauss-Legendre rule in radius (device copy in kMisc) is kept in the context for the
+    // last size, the angle rules for the last kMaxAngleRules sizes (the host rules cost
+    // O(n^2)). The caller's nodes need no radius rule (a one-node stand-in).
     const int64_t nr_rule = d_nodes ? 1 : n_radius;
-    if (ctx->rule_nr != nr_rule || ctx->rule_na != n_angles) {
-        ctx->rule_nr = ctx->rule_na = -1;
+    if (ctx->rule_nr != nr_rule) {
+        ctx->rule_nr = -1;
         gauss_legendre(nr_rule, ctx->rr_nodes, ctx->rr_weights);
-        gauss_legendre(n_angles, ctx->ar_nodes, ctx->ar_weights);
         std::vector<double> rtab(2 * nr_rule);
         std::copy(ctx->rr_nodes.begin(), ctx->rr_nodes.end(), rtab.begin());
         std::copy(ctx->rr_weights.begin(), ctx->rr_weights.end(), rtab.begin() + nr_rule);
-        std::vector<double> atab(n_angles + h);  // weights, then the non-negative cosines
-        std::copy(ctx->ar_weights.begin(), ctx->ar_weights.end(), atab.begin());
-        for (int jj = 0; jj < h; ++jj) atab[n_angles + jj] = ctx->ar_nodes[n_angles - h + jj];
-        double *dr = nullptr, *da = nullptr;
+        double *dr = nullptr;
         if ((rc = upload(ctx, scatrs_ctx::kMisc, rtab, &dr))) return rc;
-        if ((rc = upload(ctx, scatrs_ctx::kMisc2, atab, &da))) return rc;
         ctx->rule_nr = nr_rule;
-        ctx->rule_na = n_angles;
-        ctx->dtab_na = ctx->dtab_order = -1;
     }
+    scatrs_ctx::AngleRule *A = nullptr;
+    if ((rc = angle_rule(ctx, n_angles, &A))) return rc;
     S.rr.nodes = ctx->rr_nodes;
     S.rr.weights = ctx->rr_weights;
-    S.ar.nodes = ctx->ar_nodes;
-    S.ar.weights = ctx->ar_weights;
+    S.ar.nodes = A->nodes;
+    S.ar.weights = A->weights;
     S.d_rtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc]);
-    S.d_atab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc2]);
+    S.d_atab = A->d_atab;
     S.a = EnsArgs{d_kind, d_p0, d_p1, d_wavelength, d_fractions, d_m, layers, n_radius,
                   S.d_rtab, S.d_rtab + nr_rule, d_p2};
     // Per-node N and status.
@@ -1383,8 +1418,8 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                                                                      S.texp, S.tab16);
     }
     if (greek) {
-        if (ctx->dtab_na != n_angles || ctx->dtab_order != order) {
-            ctx->dtab_na = ctx->dtab_order = -1;
+        if (A->order != order) {
+            A->order = -1;
             const int64_t len = order + 1;
             // Both layouts: angle-major for the Greek projection, order-major (after it) for
             // its transpose in reverse mode.
@@ -1398,12 +1433,24 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                     for (int64_t l = 0; l < len; ++l)
                         dt[cells + ((size_t)p * len + l) * n_angles + j] = row[l];
                 }
-            double *dd = nullptr;
-            if ((rc = upload(ctx, scatrs_ctx::kMisc3, dt, &dd))) return rc;
-            ctx->dtab_na = n_angles;
-            ctx->dtab_order = order;
+            const size_t bytes = dt.size() * sizeof(double);
+            if (A->dtab_cap < bytes) {
+                if (A->d_dtab) {
+                    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "sync"))) return rc;
+                    (void)hipFree(A->d_dtab);
+                    A->d_dtab = nullptr;
+                    A->dtab_cap = 0;
+                }
+                if ((rc = check_hip(ctx, hipMalloc(&A->d_dtab, bytes), "alloc"))) return rc;
+                A->dtab_cap = bytes;
+            }
+            if ((rc = check_hip(ctx, hipMemcpyAsync(A->d_dtab, dt.data(), bytes,
+                                                    hipMemcpyHostToDevice, s),
+                                "upload")))
+                return rc;
+            A->order = order;
         }
-        S.d_dtab = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc3]);
+        S.d_dtab = A->d_dtab;
         S.d_dtab_t = S.d_dtab + 4 * n_angles * (order + 1);
     }
     return SCATRS_OK;

@@ -177,6 +177,10 @@ int scatrs_destroy(scatrs_ctx *ctx) {
     else (void)hipDeviceSynchronize();
     for (int i = 0; i < scatrs_ctx::kNumBufs; ++i)
         if (ctx->buf[i]) (void)hipFree(ctx->buf[i]);
+    for (auto &r : ctx->angle_rules) {
+        if (r.d_atab) (void)hipFree(r.d_atab);
+        if (r.d_dtab) (void)hipFree(r.d_dtab);
+    }
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
     for (int i = 0; i < scatrs_ctx::kMaxEvents; ++i) {
         if (ctx->ev_begin[i]) (void)hipEventDestroy(ctx->ev_begin[i]);

@@ -624,14 +624,15 @@ class Ensemble:
         return np.broadcast(*self.size.params()).shape
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
-                n_radius: int = 128, n_angles: int | None = 128, allow_unvalidated: bool = False,
+                n_radius: int = 128, n_angles: int | None = None, allow_unvalidated: bool = False,
                 phase: bool = True, mu=None, rtol: float | None = None, max_panels: int = 4096,
                 refine_backscatter: bool = False) -> EnsembleResult:
         """Averages over the ensemble at each wavelength on the fixed discrete path, or with
         ``rtol`` by the adaptive engine (scatrs_amd.adaptive: Gauss-Kronrod panels refined to
-        rtol, at most max_panels per part; the result's converged and panels; n_angles=None
-        sizes the Greek projection's rule to the largest N); the phase matrix at the cosines
-        ``mu`` (the reference's mu argument) or at the n_angles Gauss-Legendre cosines."""
+        rtol, at most max_panels per part; the result's converged and panels; by default each
+        Greek projection exact for its nodes, on N_max + (L+1)/2 cosines, as the reference's);
+        the phase matrix at the cosines ``mu`` (the reference's mu argument) or at the
+        n_angles Gauss-Legendre cosines (128 on the fixed path by default)."""
         import torch
 
         wl = np.asarray(wavelength, np.float64)
@@ -652,7 +653,7 @@ class Ensemble:
                 refine_backscatter=refine_backscatter)
             return _result(*k, grid + wl.shape, wl, na, terms, mu, cu, pu)
         if n_angles is None:
-            raise InvalidInputError("the fixed path needs n_angles")
+            n_angles = 128
         if isinstance(self.size, (MultiModal, Tabulated)):
             parts, fr = _parts_of(self.size, self.particle)
             sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
@@ -723,7 +724,7 @@ class Mixture:
         object.__setattr__(self, "components", tuple(out))
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
-                n_radius: int = 128, n_angles: int | None = 128, allow_unvalidated: bool = False,
+                n_radius: int = 128, n_angles: int | None = None, allow_unvalidated: bool = False,
                 phase: bool = True, mu=None, rtol: float | None = None, max_panels: int = 4096,
                 refine_backscatter: bool = False) -> EnsembleResult:
         """Averages over the mixture at each wavelength on the fixed discrete path, or with
@@ -748,7 +749,7 @@ class Mixture:
                 refine_backscatter=refine_backscatter)
             return _result(*k, wl.shape, wl, na, terms, mu, cu, pu)
         if n_angles is None:
-            raise InvalidInputError("the fixed path needs n_angles")
+            n_angles = 128
         sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,
                                           n_angles=n_angles, allow_unvalidated=allow_unvalidated,
                                           want_phase=phase, mu=mu)
