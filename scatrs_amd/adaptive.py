@@ -22,8 +22,8 @@ panels; every evaluation is one batched library call over all open panels of all
     always the worst one.
   A task stops unconverged when the next bisection would pass ``max_panels``.
 * Pass 2: every output (the scalars, the Greek coefficients, the phase matrix) is evaluated on
-  the final panels' Kronrod nodes, and again on their Gauss nodes; the error estimate is the
-  difference of the two rules' outputs. The panels go in rows of 16, grouped by the angle rule
+  the final panels' Kronrod nodes, and with the Gauss weights on the same nodes (the same
+  call); the error estimate comes from the two rules' outputs. The panels go in rows of 16, grouped by the angle rule
   their Greek projection needs to be exact, N_max + (L+1)/2 cosines at the row's largest size
   (the reference's bands, engine.rs:945-1017), and the rows are summed by their weights.
 
@@ -577,44 +577,55 @@ def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
         n = hi - lo
         A[j, :n], Bd[j, :n] = q.a[lo:hi], q.b[lo:hi]
         A[j, n:] = Bd[j, n:] = q.b[hi - 1]
-    for idx in (np.arange(15), _GAUSS):
-        k = idx.size
-        r, w = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1), idx)
+    # Both rules in one call per angle rule: the Gauss rows are the same 15 nodes per panel with
+    # the Gauss weights (0 at the Kronrod-only nodes).
+    r, wk = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
+                   np.arange(15))
+    fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
+    both = []
+    for w in (wk, wk * fold):
         nodes = np.stack([r.reshape(R, -1), w.reshape(R, -1)], -1)
         nodes[mono, :, 1] = 0.0
         nodes[mono, 0, 1] = 1.0  # a monodisperse law: its radius, weight 1
-        z = nodes[..., 1].sum(1)
-        acc = np.zeros((T, 7))
-        gr = None if order is None else np.zeros((T, 6, order + 1))
-        n_phase = (int(rules[0]) if mu is None else int(np.size(mu))) if want_phase else 0
-        ph = np.zeros((T, 6, n_phase)) if want_phase else None
-        zt = np.zeros(T)
-        for na in np.unique(rules):
-            sel = np.flatnonzero(rules == na)
-            sc, g, p, _, nt = _call(tasks, ti_rows[sel], nodes[sel], order=order,
-                                    n_angles=int(na), mu=mu, want_phase=want_phase,
-                                    normalized=False, allow_unvalidated=allow_unvalidated)
-            terms += nt
-            zs = z[sel]
+        both.append(nodes)
+    n_phase = (int(rules[0]) if mu is None else int(np.size(mu))) if want_phase else 0
+    acc = np.zeros((2, T, 7))
+    zt = np.zeros((2, T))
+    gr = None if order is None else np.zeros((2, T, 6, order + 1))
+    ph = np.zeros((2, T, 6, n_phase)) if want_phase else None
+    for na in np.unique(rules):
+        sel = np.flatnonzero(rules == na)
+        sc, g, p, _, nt = _call(tasks, np.concatenate([ti_rows[sel], ti_rows[sel]]),
+                                np.concatenate([both[0][sel], both[1][sel]]), order=order,
+                                n_angles=int(na), mu=mu, want_phase=want_phase,
+                                normalized=False, allow_unvalidated=allow_unvalidated)
+        terms += nt
+        for k in range(2):
+            rows_k = slice(k * sel.size, (k + 1) * sel.size)
+            zs = both[k][sel][..., 1].sum(1)
             live = zs > 0.0
             t, zl = ti_rows[sel][live], zs[live]
-            np.add.at(acc, t, zl[:, None] * _averages(sc[live]))
-            np.add.at(zt, t, zl)
+            np.add.at(acc[k], t, zl[:, None] * _averages(sc[rows_k][live]))
+            np.add.at(zt[k], t, zl)
             if gr is not None:
-                np.add.at(gr, t, zl[:, None, None] * g[live])
+                np.add.at(gr[k], t, zl[:, None, None] * g[rows_k][live])
             if ph is not None:
-                np.add.at(ph, t, zl[:, None, None] * p[live])
-        # The averages divide by the moment integration's Z and take its <r^2>, <r^3>
-        # (engine.rs:1782-1836); a monodisperse law keeps its node's.
-        mom = np.array([q.moments for q in tasks])
-        single_node = np.array([q.kind == MONODISPERSE for q in tasks])
-        zm = np.where(single_node, zt, mom[:, 0])
-        acc[~single_node, 5] = mom[~single_node, 1]
-        acc[~single_node, 6] = mom[~single_node, 2]
+                np.add.at(ph[k], t, zl[:, None, None] * p[rows_k][live])
+    # The averages divide by the moment integration's Z and take its <r^2>, <r^3>
+    # (engine.rs:1782-1836); a monodisperse law keeps its node's.
+    mom = np.array([q.moments for q in tasks])
+    single_node = np.array([q.kind == MONODISPERSE for q in tasks])
+    out = []
+    for k in range(2):
+        zm = np.where(single_node, zt[k], mom[:, 0])
+        a_k = acc[k]
+        a_k[~single_node, 5] = mom[~single_node, 1]
+        a_k[~single_node, 6] = mom[~single_node, 2]
         with np.errstate(divide="ignore", invalid="ignore"):
             zi = 1.0 / zm
-        out.append((_scalars(acc * zi[:, None]), None if gr is None else gr * zi[:, None, None],
-                    None if ph is None else ph * zi[:, None, None]))
+        out.append((_scalars(a_k * zi[:, None]),
+                    None if gr is None else gr[k] * zi[:, None, None],
+                    None if ph is None else ph[k] * zi[:, None, None]))
     return out[0], out[1], terms, int(rules[0]) if len(rules) else 2
 
 
