@@ -38,6 +38,8 @@ every (output, parameter) pair. The size range is held fixed, as the reference d
 
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 
 from .errors import InvalidInputError
@@ -498,10 +500,18 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
         P = len(parts)
         out = []
         rows = []
-        for idx in (np.arange(15), adaptive._GAUSS):
-            avg, gr, ph, _, _, rw = adaptive.pass2_torch(
+        if keys:  # one differentiable pass per rule (their reverse passes stay apart)
+            passes = [adaptive.pass2_torch(
                 tasks, params, R=R, idx=idx, order=order, n_angles=n_angles, mu=mu,
                 want_phase=mu is not None, allow_unvalidated=allow_unvalidated)
+                for idx in (np.arange(15), adaptive._GAUSS)]
+        else:  # both rules in one call per angle rule: the tasks twice, Gauss weights second
+            passes = _split_rules(adaptive.pass2_torch(
+                tasks + [copy.copy(q) for q in tasks], params + params, R=R,
+                idx=np.arange(15), order=order, n_angles=n_angles, mu=mu,
+                want_phase=mu is not None, allow_unvalidated=allow_unvalidated,
+                gauss_tasks=np.repeat([False, True], len(tasks))), len(tasks), R)
+        for avg, gr, ph, _, _, rw in passes:
             rows.append(rw)
             # (P n, R, ...) -> (P, B, ...), b = r n + i
             per = lambda a: None if a is None else a.reshape(  # noqa: E731
@@ -539,6 +549,33 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
         panels[u, i] += q.n_panels
     DE = None if D is None else (D - Dg).abs()
     return V, E, D, DE, torch.from_numpy(conv), panels
+
+
+def _split_rules(res, T: int, R: int):
+    """The Kronrod and Gauss halves of pass2_torch over a doubled task list (tasks, then their
+    Gauss copies): outputs by task, and the rows' sums aligned by place. The rows are [the
+    originals' panel rows, the copies' in the same order, the originals' monodisperse rows,
+    the copies'], each x R; a copy's key is its original's."""
+    import torch
+
+    avg, gr, ph, terms, na, rw = res
+    half = lambda a, k: None if a is None else a[k * T:(k + 1) * T]  # noqa: E731
+    nr, nm = rw["n_rows"] // 2, rw["n_mono"] // 2
+    dev = rw["key"].device
+    out = []
+    for k in range(2):
+        sel = torch.cat([torch.arange((k * nr) * R, (k * nr + nr) * R, device=dev),
+                         torch.arange((2 * nr + k * nm) * R, (2 * nr + k * nm + nm) * R,
+                                      device=dev)])
+        key = rw["key"][sel]
+        if k:
+            key = torch.where(key >= T * R, key - T * R, key)
+        d = {"key": key, "lin": rw["lin"][sel],
+             "gr": None if rw["gr"] is None else rw["gr"][sel],
+             "ph": None if rw["ph"] is None else rw["ph"][sel],
+             "z": rw["z"][k * T * R:(k + 1) * T * R]}
+        out.append((half(avg, k), half(gr, k), half(ph, k), terms, na, d))
+    return out
 
 
 def _error_bounds(rows, kept, unit, U, P, n, R, normalized):

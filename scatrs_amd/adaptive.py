@@ -630,7 +630,7 @@ def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
 
 
 def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
-                allow_unvalidated):
+                allow_unvalidated, gauss_tasks=None):
     """The Kronrod (idx all 15 nodes) or Gauss rule's pass 2, differentiable in every task's
     parameters (torch.autograd): params[t] = (p (R, 3), m_real (R, L), m_imag (R, L),
     fractions (R, L-1) or None), R replicas of task t's parameters. Returns the linear averages
@@ -641,7 +641,9 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
 
     The rows are pass2's. A row's sum of weights Z enters to first order in p, Z0 + (p - p0)
     . sum W d ln W / d p, which has the right value and derivative at p0; a monodisperse task is
-    the fixed path's one node (its radius moves its node)."""
+    the fixed path's one node (its radius moves its node). With gauss_tasks (bool per task) and
+    all 15 nodes, those tasks take the Gauss weights on the same nodes (0 at the Kronrod-only
+    ones): a task list doubled this way gets both rules from one call per angle rule."""
     import torch
 
     from .ensemble import ensemble_fixed, ensemble_nodes, number_averages
@@ -711,6 +713,10 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             A[j, n:] = Bd[j, n:] = q.b[hi - 1]
         r, w, d = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
                          idx, derivatives=True)
+        if gauss_tasks is not None:
+            fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
+            g_rows = np.repeat(np.asarray(gauss_tasks)[ti_rows], _ROW_PANELS)
+            w = np.where(g_rows[:, None], w * fold, w)
         nodes = np.stack([r.reshape(nr, -1), w.reshape(nr, -1)], -1)
         dl = d.reshape(nr, -1, 3)
         z0 = nodes[..., 1].sum(1)
@@ -758,7 +764,8 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     gr = None if gr is None else gr / zm[:, None, None]
     ph = None if ph is None else ph / zm[:, None, None]
     shape = lambda a: None if a is None else a.reshape((T, R) + a.shape[1:])  # noqa: E731
-    rows_out = dict(key=row_key, lin=row_lin, gr=row_gr, ph=row_ph, z=zm.detach())
+    rows_out = dict(key=row_key, lin=row_lin, gr=row_gr, ph=row_ph, z=zm.detach(),
+                    n_rows=len(rows), n_mono=len(mono))
     return (shape(avg), shape(gr), shape(ph), terms, (int(rules[0]) if len(rules) else 2),
             rows_out)
 
