@@ -500,8 +500,18 @@ class EnsembleResult:
     phase: EnsemblePhase | None = None
     wavelength: np.ndarray = field(default=None)
     terms: int = 0
-    converged: np.ndarray | None = None  # the adaptive engine's (rtol given), else None
+    converged: np.ndarray | None = None  # the adaptive engine's (and the fixed grid's checks)
     panels: np.ndarray | None = None
+    error: "EnsembleResult | None" = None  # the outputs' error estimates, same layout
+    params: tuple = ()  # the parameters of d(), in the reference's names
+    _d: tuple = ()
+
+    def d(self, name: str) -> "EnsembleResult":
+        """The derivatives of every output with respect to the parameter ``name`` (the
+        reference's EnsembleResult.d), their error estimates in ``.error``."""
+        if name not in self.params:
+            raise KeyError(f"no derivative with respect to {name!r}; computed: {self.params}")
+        return self._d[self.params.index(name)]
 
 
 def _frozen(a: np.ndarray) -> np.ndarray:
@@ -555,27 +565,6 @@ def mix_units(scalars, greek, phase, unit, fractions, n_units: int, normalized: 
     return assemble(avg, gr, ph, normalized)
 
 
-def _adaptive(parts, unit, fractions, n_units: int, wl: np.ndarray, *, greek, normalized,
-              n_angles, rtol, max_panels, allow_unvalidated, phase, mu, refine_backscatter):
-    """The units' outputs by the adaptive engine (adaptive.parts_adaptive): Kronrod and Gauss
-    mixtures ((n_units, nw, ...) each), terms, converged and panels (n_units, nw), the phase
-    rule's size."""
-    from .adaptive import parts_adaptive
-
-    K, G, terms, conv, panels, _, na = parts_adaptive(
-        parts, wl, order=greek, n_angles=n_angles, rtol=rtol, max_panels=max_panels,
-        allow_unvalidated=allow_unvalidated, want_phase=phase, mu=mu,
-        refine_backscatter=refine_backscatter)
-    k = mix_units(*K, unit, fractions, n_units, normalized)
-    g = mix_units(*G, unit, fractions, n_units, normalized)
-    cu = np.ones((n_units, wl.size), bool)
-    pu = np.zeros((n_units, wl.size), np.int64)
-    for i, ui in enumerate(unit):
-        cu[ui] &= conv[i]
-        pu[ui] += panels[i]
-    return k, g, terms, cu, pu, na
-
-
 def _part(size, particle):
     """(kind, p0, p1, p2, particle) of a distribution with scalar parameters."""
     return (size.kind, *(float(v) for v in size.params()), particle)
@@ -597,6 +586,83 @@ def _parts_of(size, particle, weight: float = 1.0):
         return ([(SEGMENT, r0, r1, tilt, particle) for r0, r1, tilt in segs],
                 list(weight * w / w.sum()))
     return [_part(size, particle)], [weight]
+
+
+def _spec(size):
+    """The reference's tagged tuple of a distribution with scalar parameters
+    (scatrs-py/src/ensemble.rs:31-77)."""
+    from .size import Gamma, LogNormal, ModifiedGamma, Monodisperse, PowerLaw
+
+    f = float
+    if isinstance(size, LogNormal):
+        return ("lognormal", f(size.median_radius), f(size.width))
+    if isinstance(size, Gamma):
+        return ("gamma", f(size.effective_radius), f(size.effective_variance))
+    if isinstance(size, ModifiedGamma):
+        return ("modified_gamma", f(size.alpha), f(size.b), f(size.gamma))
+    if isinstance(size, PowerLaw):
+        return ("power_law", f(size.exponent), f(size.min_radius), f(size.max_radius))
+    if isinstance(size, Monodisperse):
+        return ("monodisperse", f(size.radius))
+    if isinstance(size, Tabulated):
+        return ("tabulated", np.array(size.radius), np.array(size.density))
+    if isinstance(size, MultiModal):
+        return ("multimodal", [(float(w), _spec(d)) for w, d in size.modes])
+    raise TypeError(f"no spec for {type(size).__name__}")
+
+
+def _particle_spec(particle, wl: np.ndarray):
+    """The reference's tagged tuple of a particle at the wavelengths
+    (scatrs-py/src/ensemble.rs:79-135)."""
+    m = particle.indices(wl).reshape(particle.layers, -1)
+    if particle.layers == 1:
+        return ("sphere", np.ascontiguousarray(m[0]))
+    return ("layered", np.asarray(particle.fractions(), np.float64), np.ascontiguousarray(m))
+
+
+def _chosen(names: list, derivatives) -> list:
+    """The indices of the requested parameters: all with True, none with False, or by name."""
+    if derivatives is True:
+        return list(range(len(names)))
+    if derivatives is False or derivatives is None:
+        return []
+    if isinstance(derivatives, (str, bytes)):
+        raise TypeError("derivatives must be a bool or a sequence of parameter names")
+    out = []
+    for n in derivatives:
+        if n not in names:
+            raise InvalidInputError(f"no parameter {n}; the parameters are {names}")
+        out.append(names.index(n))
+    return out
+
+
+def _from_wire(raw, shape: tuple, wl: np.ndarray, mu, names: list, chosen: list, terms=0):
+    """An EnsembleResult (with its errors, derivatives, converged and panels) from the wire
+    format's dict, rows in the C order of ``shape``."""
+    mu = None if mu is None else _frozen(np.array(mu, np.float64).reshape(-1))
+
+    def outputs(v, g, p, **extra):
+        fields = [_frozen(np.ascontiguousarray(v[k]).reshape(shape)) for k in range(11)]
+        gk = None if g is None else EnsembleGreek(*(_frozen(np.ascontiguousarray(g[k]).reshape(
+            shape + (g.shape[-1],))) for k in range(6)))
+        pk = None if p is None else EnsemblePhase(mu, *(_frozen(np.ascontiguousarray(
+            p[k]).reshape(shape + (p.shape[-1],))) for k in range(6)))
+        return EnsembleResult(*fields, greek=gk, phase=pk, wavelength=_frozen(wl.copy()),
+                              **extra)
+
+    gv, ge = raw.get("greek"), raw.get("greek_errors")
+    pv, pe = raw.get("phase"), raw.get("phase_errors")
+    err = outputs(raw["errors"], ge, pe)
+    ds = []
+    for k in range(len(chosen)):
+        de = outputs(raw["d_errors"][k], None if ge is None else raw["greek_d_errors"][k],
+                     None if pe is None else raw["phase_d_errors"][k])
+        ds.append(outputs(raw["d"][k], None if gv is None else raw["greek_d"][k],
+                          None if pv is None else raw["phase_d"][k], error=de))
+    return outputs(raw["values"], gv, pv, terms=terms,
+                   converged=_frozen(raw["converged"].reshape(shape)),
+                   panels=_frozen(raw["panels"].reshape(shape)), error=err,
+                   params=tuple(names[i] for i in chosen), _d=tuple(ds))
 
 
 @dataclass(frozen=True)
@@ -624,34 +690,53 @@ class Ensemble:
         return np.broadcast(*self.size.params()).shape
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
-                n_radius: int = 128, n_angles: int | None = None, allow_unvalidated: bool = False,
-                phase: bool = True, mu=None, rtol: float | None = None, max_panels: int = 4096,
-                refine_backscatter: bool = False) -> EnsembleResult:
-        """Averages over the ensemble at each wavelength on the fixed discrete path, or with
-        ``rtol`` by the adaptive engine (scatrs_amd.adaptive: Gauss-Kronrod panels refined to
-        rtol, at most max_panels per part; the result's converged and panels; by default each
-        Greek projection exact for its nodes, on N_max + (L+1)/2 cosines, as the reference's);
-        the phase matrix at the cosines ``mu`` (the reference's mu argument) or at the
-        n_angles Gauss-Legendre cosines (128 on the fixed path by default)."""
+                mu=None, derivatives=False, rtol: float = 1e-6, max_panels: int = 4096,
+                allow_unvalidated: bool = False, refine_backscatter: bool = False,
+                all_orders: bool = False, n_radius: int | None = None,
+                n_angles: int | None = None, phase: bool = True) -> EnsembleResult:
+        """Averages over the ensemble at each wavelength, as the reference's
+        Ensemble.compute (python/scatrs/ensemble.py:484-591): by the adaptive engine
+        (scatrs_amd.adaptive: Gauss-Kronrod panels refined to rtol, at most max_panels per
+        part; each Greek projection exact for its nodes) through the wire format, with the
+        result's error, d(name), converged and panels; the phase matrix at the cosines ``mu``.
+        With ``n_radius``, on the fixed discrete path instead (n_radius Gauss-Legendre nodes;
+        without derivatives, the phase matrix at ``mu`` or at the n_angles Gauss-Legendre
+        cosines, 128 by default). ``derivatives``: True, or the parameters' names."""
         import torch
 
+        from . import _wire
+
         wl = np.asarray(wavelength, np.float64)
-        if rtol is not None:
-            if isinstance(self.size, (MultiModal, Tabulated)):
-                parts, fr = _parts_of(self.size, self.particle)
-                unit, grid = [0] * len(parts), ()
-            else:
-                grid = self.grid_shape
-                pg = [np.broadcast_to(v, grid).reshape(-1) for v in self.size.params()]
-                parts = [(self.size.kind, float(a), float(b), float(c), self.particle)
-                         for a, b, c in zip(*pg)]
-                unit, fr = list(range(len(parts))), [1.0] * len(parts)
-            k, _, terms, cu, pu, na = _adaptive(
-                parts, unit, fr, max(unit) + 1, wl, greek=greek, normalized=normalized,
-                n_angles=n_angles, rtol=rtol, max_panels=max_panels,
-                allow_unvalidated=allow_unvalidated, phase=phase, mu=mu,
-                refine_backscatter=refine_backscatter)
-            return _result(*k, grid + wl.shape, wl, na, terms, mu, cu, pu)
+        if n_radius is None or derivatives:
+            grid = self.grid_shape
+            rt = dict(rtol=rtol, max_panels=max_panels, allow_unvalidated=allow_unvalidated,
+                      refine_backscatter=refine_backscatter, n_radius=n_radius,
+                      n_angles=n_angles)
+            muv = np.zeros(0) if mu is None else np.asarray(mu, np.float64).reshape(-1)
+            pspec = _particle_spec(self.particle, wl)
+            if grid == ():
+                items = [(_spec(self.size), pspec, 1.0)]
+                names = _wire.ensemble_parameters(items, False)
+                chosen = _chosen(names, derivatives)
+                raw = _wire.ensemble_compute(
+                    items, wl, False, greek, all_orders, muv, normalized, chosen,
+                    rt["rtol"], rt["max_panels"], rt["allow_unvalidated"],
+                    rt["refine_backscatter"], n_radius=n_radius, n_angles=n_angles)
+                return _from_wire(raw, wl.shape, wl, mu, names, chosen)
+            if derivatives:
+                raise InvalidInputError("derivatives need a distribution with scalar parameters "
+                                        "(a grid is computed without them)")
+            import dataclasses
+
+            pg = [np.broadcast_to(v, grid).reshape(-1) for v in self.size.params()]
+            n_args = len(dataclasses.fields(self.size))  # in the order of params()
+            specs = [_spec(type(self.size)(*(float(pg[k][i]) for k in range(n_args))))
+                     for i in range(pg[0].size)]
+            raw = _wire.ensemble_compute_batch(
+                specs, pspec, wl, greek, all_orders, muv, normalized, rt["rtol"],
+                rt["max_panels"], rt["allow_unvalidated"], rt["refine_backscatter"],
+                n_radius=n_radius, n_angles=n_angles)
+            return _from_wire(raw, grid + wl.shape, wl, mu, [], [])
         if n_angles is None:
             n_angles = 128
         if isinstance(self.size, (MultiModal, Tabulated)):
@@ -724,15 +809,29 @@ class Mixture:
         object.__setattr__(self, "components", tuple(out))
 
     def compute(self, wavelength, *, greek: int | None = None, normalized: bool = True,
-                n_radius: int = 128, n_angles: int | None = None, allow_unvalidated: bool = False,
-                phase: bool = True, mu=None, rtol: float | None = None, max_panels: int = 4096,
-                refine_backscatter: bool = False) -> EnsembleResult:
-        """Averages over the mixture at each wavelength on the fixed discrete path, or with
-        ``rtol`` by the adaptive engine (as Ensemble.compute); the phase matrix at the cosines
-        ``mu`` when given."""
+                mu=None, derivatives=False, rtol: float = 1e-6, max_panels: int = 4096,
+                allow_unvalidated: bool = False, refine_backscatter: bool = False,
+                all_orders: bool = False, n_radius: int | None = None,
+                n_angles: int | None = None, phase: bool = True) -> EnsembleResult:
+        """Averages over the mixture at each wavelength, as Ensemble.compute: by the adaptive
+        engine through the wire format (with error, d(name) in the mixture's parameter names
+        c{i}.*, converged, panels), or on the fixed discrete path with ``n_radius``."""
         import torch
 
+        from . import _wire
+
         wl = np.asarray(wavelength, np.float64)
+        if n_radius is None or derivatives:
+            items = [(_spec(e.size), _particle_spec(e.particle, wl), w)
+                     for w, e in self.components]
+            names = _wire.ensemble_parameters(items, True)
+            chosen = _chosen(names, derivatives)
+            muv = np.zeros(0) if mu is None else np.asarray(mu, np.float64).reshape(-1)
+            raw = _wire.ensemble_compute(items, wl, True, greek, all_orders, muv, normalized,
+                                         chosen, rtol, max_panels, allow_unvalidated,
+                                         refine_backscatter, n_radius=n_radius,
+                                         n_angles=n_angles)
+            return _from_wire(raw, wl.shape, wl, mu, names, chosen)
         total = sum(w for w, _ in self.components)
         parts, fr = [], []
         for w, e in self.components:
@@ -741,13 +840,6 @@ class Mixture:
             p, f = _parts_of(e.size, e.particle, w / total)
             parts += p
             fr += f
-        if rtol is not None:
-            k, _, terms, cu, pu, na = _adaptive(
-                parts, [0] * len(parts), fr, 1, wl, greek=greek, normalized=normalized,
-                n_angles=n_angles, rtol=rtol, max_panels=max_panels,
-                allow_unvalidated=allow_unvalidated, phase=phase, mu=mu,
-                refine_backscatter=refine_backscatter)
-            return _result(*k, wl.shape, wl, na, terms, mu, cu, pu)
         if n_angles is None:
             n_angles = 128
         sc, gr, ph, terms = _parts_device(parts, wl, order=greek, n_radius=n_radius,

@@ -336,3 +336,36 @@ def test_refine_backscatter_and_layered_derivatives():
         fdg = (up["greek"] - dn["greek"]) / (2 * abs(h))
         errg = np.abs(raw["greek_d"][k] - fdg) / np.max(np.abs(fdg))
         assert errg.max() <= 2e-5, (want[k], errg.max())
+
+
+def test_ensemble_api_errors_derivatives_and_grids():
+    """Ensemble.compute as the reference's: the adaptive engine by default, result.error,
+    result.d(name) (equal to the wire format's columns), a grid of distributions through the
+    batch call, and the fixed grid with n_radius."""
+    from scatrs_amd import _wire
+    from scatrs_amd.ensemble import Ensemble
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import LogNormal
+
+    wl = np.array([0.5, 0.8])
+    m = 1.45 + 0.01j
+    r = Ensemble(Sphere(m), LogNormal(0.2, 1.5)).compute(wl, greek=4,
+                                                         derivatives=["width", "m_imag"])
+    assert r.params == ("width", "m_imag") and r.converged.all()
+    assert np.all(r.error.c_ext <= 1e-6 * r.c_ext) and r.error.c_ext.shape == wl.shape
+    raw = _wire.ensemble_compute([(("lognormal", 0.2, 1.5), ("sphere", np.full(2, m)), 1.0)],
+                                 wl, False, 4, False, np.zeros(0), True, [1, 3], 1e-6, 4096,
+                                 False, False)
+    assert np.array_equal(r.d("width").c_ext, raw["d"][0][0])
+    assert np.array_equal(r.d("m_imag").greek.alpha1, raw["greek_d"][1][0])
+    assert r.d("width").error.c_ext.shape == wl.shape
+    with pytest.raises(KeyError):
+        r.d("median_radius")
+    with pytest.raises(InvalidInputError, match="no parameter"):
+        Ensemble(Sphere(m), LogNormal(0.2, 1.5)).compute(wl, derivatives=["radius"])
+    grid = Ensemble(Sphere(m), LogNormal(np.array([0.1, 0.2, 0.4]), 1.5)).compute(wl, greek=4)
+    assert grid.c_ext.shape == (3, 2) and grid.greek.alpha1.shape[:2] == (3, 2)
+    assert grid.converged.shape == (3, 2) and grid.converged.all()
+    assert np.allclose(grid.c_ext[1], r.c_ext, rtol=1e-13, atol=0)
+    fixed = Ensemble(Sphere(m), LogNormal(0.2, 1.5)).compute(wl, greek=4, n_radius=256)
+    assert fixed.error is None and np.allclose(fixed.c_ext, r.c_ext, rtol=1e-4, atol=0)
