@@ -59,7 +59,7 @@ _WG = np.array([0.129484966168869693270611432679082, 0.2797053914892766679014677
 
 _INITIAL_PANELS = 4
 _MAX_ROUNDS = 64
-_GRID = 2049
+_GRID = 1025
 
 
 def kronrod15():
@@ -498,12 +498,21 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
 _ROW_PANELS = 16  # panels per pass-2 row
 
 
-def _rule_ladder(n: int) -> int:
-    """An angle rule's size: the next of 16, 20, 25, ... (steps of 1.25x), at most 4096."""
-    k = 16
-    while k < n:
-        k = int(np.ceil(k * 1.25))
-    return min(k, 4096)
+def _ladder_sizes():
+    out = [16]
+    while out[-1] < 4096:
+        out.append(int(np.ceil(out[-1] * 1.25)))
+    return np.minimum(np.array(out), 4096)
+
+
+_LADDER = _ladder_sizes()
+
+
+def _rule_ladder(n):
+    """An angle rule's size: the next of 16, 20, 25, ... (steps of 1.25x), at most 4096 (n a
+    number or an array)."""
+    i = np.minimum(np.searchsorted(_LADDER, n, side="left"), _LADDER.size - 1)
+    return _LADDER[i]
 
 
 def _row_rules(tasks, rows, order, n_angles, single: bool):
@@ -519,8 +528,8 @@ def _row_rules(tasks, rows, order, n_angles, single: bool):
     for k, (ti, lo, hi) in enumerate(rows):
         q = tasks[ti]
         r = q.p[0] if lo < 0 else np.exp(q.b[hi - 1])
-        n = int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl))
-        need[k] = _rule_ladder(n + (order + 2) // 2)
+        need[k] = int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl)) + (order + 2) // 2
+    need = _rule_ladder(need).astype(np.int64)
     if single:
         need[:] = need.max()
     return need
@@ -711,8 +720,11 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             n = hi - lo
             A[j, :n], Bd[j, :n] = q.a[lo:hi], q.b[lo:hi]
             A[j, n:] = Bd[j, n:] = q.b[hi - 1]
-        r, w, d = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
-                         idx, derivatives=True)
+        grad = torch.is_grad_enabled()  # without gradients the weights' derivatives go unused
+        out_n = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
+                       idx, derivatives=grad)
+        r, w = out_n[0], out_n[1]
+        d = out_n[2] if grad else np.zeros(r.shape + (3,))
         if gauss_tasks is not None:
             fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
             g_rows = np.repeat(np.asarray(gauss_tasks)[ti_rows], _ROW_PANELS)
@@ -730,12 +742,15 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
                 pr, mr, mi, fr = gather(tk, L)
                 t = lambda a: torch.from_numpy(np.ascontiguousarray(  # noqa: E731
                     np.repeat(a, R, axis=0))).to(dev)
-                sc, g, p_ = ensemble_nodes(pr, mr, mi, fr, t(nodes[sel]), t(dl[sel]),
+                sc, g, p_ = ensemble_nodes(pr, mr, mi, fr, t(nodes[sel]),
+                                           t(dl[sel]) if grad else None,
                                            t(wl_all[tk]), n_angles=int(na), order=order,
                                            normalized=False,
                                            allow_unvalidated=allow_unvalidated, mu=mu,
                                            phase=want_phase)
-                z = t(z0[sel]) + ((pr - pr.detach()) * t(zd[sel])).sum(1)
+                z = t(z0[sel])
+                if grad:
+                    z = z + ((pr - pr.detach()) * t(zd[sel])).sum(1)
                 key = (torch.from_numpy(tk).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
                 place = (torch.from_numpy(sel).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
                 add(key, z, sc, g, p_, place)
