@@ -289,18 +289,6 @@ def _refined(sc, z):
     return s
 
 
-def _targets(total, rtol, refine_backscatter):
-    ext, sca, back = abs(total[0]), abs(total[1]), abs(total[4])
-    tgt = rtol * np.array([ext, sca, sca, ext, max(back, sca)])
-    if not refine_backscatter:
-        tgt[4] = np.inf
-    return tgt
-
-
-def _nan(n):
-    return np.full((n, 5), np.nan)
-
-
 def _prepare(tasks, rtol: float) -> None:
     """Every task's range, weight scale c and moment integrals, as integration_range and
     moment_integrals give them one by one, with one density call per step for all tasks."""
