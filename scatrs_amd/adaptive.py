@@ -735,7 +735,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
                                            t(wl_all[tk]), n_angles=int(na), order=order,
                                            normalized=False,
                                            allow_unvalidated=allow_unvalidated, mu=mu,
-                                           phase=want_phase)
+                                           phase=want_phase, replicas=R)
                 z = t(z0[sel])
                 if grad:
                     z = z + ((pr - pr.detach()) * t(zd[sel])).sum(1)

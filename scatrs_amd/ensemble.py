@@ -169,10 +169,15 @@ def _nodes_function():
         def forward(ctx, p, m_real, m_imag, fractions, nodes, dlnw, wavelength, opts):
             ctx.set_materialize_grads(False)
             m = torch.complex(m_real, m_imag)
+            k = opts["replicas"]  # rows come in runs of k equal replicas: solve each run once
             sc, gr, ph, _ = ensemble_nodes_device(
-                nodes, wavelength, m, fractions, n_angles=opts["n_angles"], order=opts["order"],
+                nodes[::k], wavelength[::k], m[::k], None if fractions is None else
+                fractions[::k], n_angles=opts["n_angles"], order=opts["order"],
                 normalized=opts["normalized"], allow_unvalidated=opts["allow_unvalidated"],
                 want_phase=opts["phase"], mu=opts["mu"])
+            if k > 1:
+                rep = lambda a: None if a is None else a.repeat_interleave(k, 0)  # noqa: E731
+                sc, gr, ph = rep(sc), rep(gr), rep(ph)
             empty = torch.empty(0)
             ctx.save_for_backward(m_real, m_imag, fractions if fractions is not None else empty,
                                   nodes, dlnw, wavelength)
@@ -204,17 +209,22 @@ _NODES_FUNCTION = None
 
 def ensemble_nodes(p, m_real, m_imag, fractions, nodes, dlnw, wavelength, *, n_angles: int = 128,
                    order: int | None = 32, normalized: bool = True,
-                   allow_unvalidated: bool = False, mu=None, phase: bool = True):
+                   allow_unvalidated: bool = False, mu=None, phase: bool = True,
+                   replicas: int = 1):
     """Differentiable ensembles on the caller's nodes (torch.autograd): p (count, 3) the
     parameters the nodes' weights depend on, through dlnw (count, n_nodes, 3) = d ln W / d p
     (the nodes do not move); m_real, m_imag (count,) or (count, L); fractions (count, L-1) or
-    None; nodes (count, n_nodes, 2). Returns (scalars, greek, phase) as ensemble_nodes_device."""
+    None; nodes (count, n_nodes, 2). Returns (scalars, greek, phase) as ensemble_nodes_device.
+    With ``replicas`` = k, the rows come in runs of k equal copies (the same nodes and
+    parameter values, each its own leaf for its own cotangent): the forward pass solves each
+    run once."""
     global _NODES_FUNCTION
     if _NODES_FUNCTION is None:
         _NODES_FUNCTION = _nodes_function()
     opts = dict(n_angles=int(n_angles), order=None if order is None else int(order),
                 normalized=bool(normalized), allow_unvalidated=bool(allow_unvalidated),
-                phase=bool(phase), mu=None if mu is None else np.array(mu, np.float64).reshape(-1))
+                phase=bool(phase), mu=None if mu is None else np.array(mu, np.float64).reshape(-1),
+                replicas=int(replicas))
     return _NODES_FUNCTION.apply(p, m_real, m_imag, fractions, nodes, dlnw, wavelength, opts)
 
 
