@@ -97,3 +97,80 @@ def test_integration_range_cuts_the_area_tails_at_rtol_over_10():
     assert np.allclose(lu - lu[1], ref - ref[1], rtol=0, atol=1e-13)
     with pytest.raises(InvalidInputError):
         size_range(0, (0.3, 0.9, 0.0))
+
+
+def _stub_call(q_of_r):
+    """A stand-in for the device call of the adaptive engine: each row's normalized=False
+    scalars as averages of closed-form per-node quantities q_of_r(r) -> (C_ext, C_sca, C_abs,
+    C_back, g) over the row's weights."""
+
+    def call(tasks, ti, nodes, **kw):
+        r, w = nodes[..., 0], nodes[..., 1]
+        z = w.sum(1)
+        ext, sca, absb, back, g = q_of_r(r)
+        avg = lambda q: (w * q).sum(1) / np.where(z > 0, z, 1.0)  # noqa: E731
+        sc = np.zeros((r.shape[0], 11))
+        sc[:, 0], sc[:, 1], sc[:, 2], sc[:, 3] = avg(ext), avg(sca), avg(absb), avg(back)
+        sc[:, 5] = avg(sca * g) / np.where(avg(sca) > 0, avg(sca), 1.0)
+        return sc, None, None, z, int(r.size)
+
+    return call
+
+
+def test_adaptive_pass1_converges_to_the_closed_form(monkeypatch):
+    """Pass 1's panel logic on the CPU (the device call replaced by closed-form per-node
+    quantities): for a lognormal with C_ext = pi r^2 and C_sca = 0.9 pi r^2 (1 + 0.3 sin(40 ln
+    r)) the engine's <C_ext> meets rtol against the closed form r_g^2 exp(2 s^2) pi over the
+    whole law (its moment integration's Z), refines where the integrand oscillates, and stops
+    unconverged at max_panels."""
+    from scatrs_amd import adaptive
+
+    def q(r):
+        ext = np.pi * r * r
+        sca = 0.9 * ext * (1.0 + 0.3 * np.sin(40.0 * np.log(r)))
+        return ext, sca, ext - sca, 0.1 * ext, np.full_like(r, 0.5)
+
+    monkeypatch.setattr(adaptive, "_call", _stub_call(q))
+    r_g, sigma, rtol = 0.3, 1.8, 1e-8
+    s = np.log(sigma)
+
+    def task():
+        return adaptive.Task(0, np.array([r_g, sigma, 0.0]), 0.5, np.array([1.5 + 0j]),
+                             np.zeros(0))
+
+    t = task()
+    adaptive.pass1([t], rtol, 4096, False, False)
+    assert t.converged and t.a.size > 4 * 4
+    assert np.all(np.diff(t.a) > 0) and np.allclose(t.a[1:], t.b[:-1], rtol=0, atol=1e-15)
+    s_ext = np.nansum(t.K[:, 0])  # C_ext sum over the panels, in the weights' scale
+    exact = np.pi * r_g ** 2 * np.exp(2 * s * s)
+    assert abs(s_ext / t.moments[0] - exact) <= 5 * rtol * exact
+    err = np.abs(t.K - t.G).sum(0)
+    assert err[0] <= rtol * abs(t.K[:, 0].sum()) and err[1] <= rtol * abs(t.K[:, 1].sum())
+    small = task()
+    adaptive.pass1([small], rtol, 8, False, False)
+    assert not small.converged and small.a.size <= 8
+
+
+def test_adaptive_pass1_grows_the_range_for_steep_integrands(monkeypatch):
+    """With C_sca ~ r^6 (the Rayleigh law) the outer panel holds more than rtol/10 of it at the
+    area rule's upper end: the range grows until the r^6 law's tail is below rtol/10 of its
+    closed-form total r_g^6 exp(18 s^2); a gamma task in the same batch converges beside it."""
+    from scatrs_amd import adaptive
+
+    def q(r):
+        ext = r ** 6 + 0.01 * r * r
+        return ext, r ** 6, 0.01 * r * r, r ** 6, np.zeros_like(r)
+
+    monkeypatch.setattr(adaptive, "_call", _stub_call(q))
+    r_g, sigma, rtol = 0.01, 2.0, 1e-7
+    s = np.log(sigma)
+    t = adaptive.Task(0, np.array([r_g, sigma, 0.0]), 1.0, np.array([1.5 + 0j]), np.zeros(0))
+    g = adaptive.Task(1, np.array([0.5, 0.2, 0.0]), 1.0, np.array([1.5 + 0j]), np.zeros(0))
+    _, t_hi, _ = adaptive.integration_range(0, t.p, rtol)
+    adaptive.pass1([t, g], rtol, 4096, False, False)
+    assert t.converged and g.converged
+    assert t.t_hi > t_hi + 1e-9
+    sca = np.nansum(t.K[:, 1]) / t.moments[0]
+    exact = r_g ** 6 * np.exp(18 * s * s)
+    assert abs(sca - exact) <= 5 * rtol * exact
