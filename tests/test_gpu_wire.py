@@ -237,3 +237,35 @@ def test_adaptive_errors_follow_the_references_propagation():
     assert np.allclose(e[6], 4 * np.pi * (e[0] + v[6] / (4 * np.pi) * e[3]) / v[3], rtol=1e-12,
                        atol=0)
     assert np.all(e[8:11] == 0.0) and np.all(e[7] == 0.0)
+
+
+def test_adaptive_mixture_of_layer_counts_with_derivatives():
+    """A mixture of coated spheres (gamma law) and spheres (lognormal) on the adaptive engine,
+    rows of both layer counts in each call: values equal Mixture.compute's, and the coated
+    component's shell Im m and the lognormal's width derivatives match central differences
+    of the engine (rtol 1e-10)."""
+    from scatrs_amd import _wire
+
+    wl = np.array([0.6])
+
+    def items(mi=1e-3, width=1.6):
+        coated = ("layered", np.array([0.6]), np.array([[1.6 + 0.1j], [1.4 + 1j * mi]]))
+        return [(("gamma", 0.3, 0.12), coated, 1.0),
+                (("lognormal", 0.25, width), ("sphere", np.array([1.5 + 0.01j])), 2.0)]
+
+    names = _wire.ensemble_parameters(items(), True)
+    want = ["c0.layer1.m_imag", "c1.width"]
+    run = lambda its, d: _wire.ensemble_compute(its, wl, True, 6, False, np.zeros(0),  # noqa
+                                                True, d, 1e-10, 4096, False, False)
+    raw = run(items(), [names.index(n) for n in want])
+    assert raw["converged"].all()
+    for k, (key, base, h) in enumerate([("mi", 1e-3, 1e-6), ("width", 1.6, 1.6e-4)]):
+        up = run(items(**{key: base + h}), [])
+        dn = run(items(**{key: base - h}), [])
+        fd = (up["values"] - dn["values"]) / (2 * h)
+        sel = [0, 1, 2, 4, 5]
+        scale = np.max(np.abs(fd[sel]), 1, keepdims=True)
+        err = np.abs(raw["d"][k][sel] - fd[sel]) / scale
+        assert err.max() <= 2e-5, (want[k], err.max())
+        fdg = (up["greek"] - dn["greek"]) / (2 * h)
+        assert np.max(np.abs(raw["greek_d"][k] - fdg)) <= 2e-5 * np.max(np.abs(fdg))
