@@ -195,6 +195,10 @@ int scatrs_ensemble_nodes_vjp(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
  * with d_ln_u (n, 3) not NULL also d ln u / d p (the reference radius held fixed).
  * SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. */
 int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi);
+/* The reference's message for invalid parameters p[3] of a distribution (distribution.rs:
+ * 99-185) into buf (cap bytes, NUL-terminated): SCATRS_INVALID_INPUT with the message, or
+ * SCATRS_OK and "" for valid ones. */
+int scatrs_size_message(int32_t kind, const double *p, char *buf, int64_t cap);
 int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
                             double *ln_u, double *d_ln_u);
 

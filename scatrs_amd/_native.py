@@ -43,6 +43,7 @@ SYMBOLS = (
     "scatrs_ensemble_nodes_vjp",
     "scatrs_size_range",
     "scatrs_size_log_density",
+    "scatrs_size_message",
 )
 
 
@@ -92,6 +93,8 @@ def load() -> C.CDLL:
             L.scatrs_size_range.argtypes = [i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
             L.scatrs_size_log_density.argtypes = [i64, C.POINTER(C.c_int32), C.POINTER(dbl),
                                                   C.POINTER(dbl), C.POINTER(dbl), vp]
+        if hasattr(L, "scatrs_size_message"):
+            L.scatrs_size_message.argtypes = [i32, C.POINTER(dbl), C.c_char_p, i64]
         if hasattr(L, "scatrs_ensemble_nodes_vjp"):
             L.scatrs_ensemble_nodes_vjp.argtypes = [vp, i64, i64, vp, vp, vp, i32, vp, vp, i64,
                                                     i64, i32, i32, vp, vp, vp, vp,

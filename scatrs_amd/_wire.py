@@ -123,11 +123,28 @@ def _components(items):
     return [_Component(_distribution(s), *_particle(p), w) for s, p, w in items]
 
 
+def _check_distribution(dist) -> None:
+    """The reference's parameter checks of a distribution and its modes
+    (distribution.rs:99-185), before any device work."""
+    from .adaptive import check_parameters
+
+    if isinstance(dist, MultiModal):
+        for j, (_, d) in enumerate(dist.modes):
+            try:
+                _check_distribution(d)
+            except InvalidInputError as e:
+                raise InvalidInputError(f"mode {j}: {e}") from None
+    elif not isinstance(dist, Tabulated):
+        check_parameters(dist.kind, np.array(dist.params(), np.float64))
+
+
 def _validate(comps, n: int) -> None:
-    """The reference's checks of the components (scatrs/src/ensemble/mod.rs:745-780): one
-    refractive index per wavelength, finite weights >= 0 with a positive sum."""
+    """The reference's checks of the components (scatrs/src/ensemble/mod.rs:745-780): valid
+    distributions, one refractive index per wavelength, finite weights >= 0 with a positive
+    sum."""
     total = 0.0
     for i, c in enumerate(comps):
+        _check_distribution(c.dist)
         if c.m.shape[1] != n:
             raise InvalidInputError(
                 f"component {i} has {c.m.shape[1]} refractive indices for {n} wavelengths")
@@ -769,8 +786,8 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
     wl = np.ascontiguousarray(np.asarray(wavelengths, np.float64).reshape(-1))
     mu = _mu(mu)
     units = [[_Component(_distribution(s), m, fr, 1.0)] for s in specs]
-    if units:
-        _validate(units[0], wl.size)
+    for u in units:
+        _validate(u, wl.size)
     order = _order(greek, all_orders)
     n_mu = 0 if mu is None else mu.size
     _check_options(rtol, max_panels, len(units) * wl.size, n_mu, order, 0)

@@ -82,11 +82,21 @@ def _dbl(a):
     return np.ascontiguousarray(a, np.float64).ctypes.data_as(C.POINTER(C.c_double))
 
 
+def check_parameters(kind: int, p) -> None:
+    """InvalidInputError with the reference's message for invalid distribution parameters
+    (scatrs_size_message)."""
+    q = np.ascontiguousarray(p, np.float64)
+    buf = C.create_string_buffer(512)
+    if _native.load().scatrs_size_message(int(kind), _dbl(q), buf, 512) != 0:
+        raise InvalidInputError(buf.value.decode())
+
+
 def size_range(kind: int, p) -> tuple[float, float]:
     """The fixed path's ln r range of a distribution (scatrs_size_range)."""
     q = np.ascontiguousarray(p, np.float64)
     lo, hi = C.c_double(), C.c_double()
     if _native.load().scatrs_size_range(int(kind), _dbl(q), C.byref(lo), C.byref(hi)) != 0:
+        check_parameters(kind, q)
         raise InvalidInputError(f"invalid parameters {tuple(q)} of size distribution kind {kind}")
     return lo.value, hi.value
 

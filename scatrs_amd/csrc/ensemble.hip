@@ -3,6 +3,8 @@
 // the phase matrix at the n_angles Gauss-Legendre cosines by the parity-split angular
 // products, the quadrature-weighted reduction, and the Greek projection. The kernels live in
 // ensemble_kernels.h.
+#include <cstring>
+
 #include "ensemble_kernels.h"
 
 using namespace scatrs;
@@ -119,4 +121,13 @@ extern "C" int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_n
     return ensemble_forward(ctx, count, nullptr, nullptr, nullptr, nullptr, d_nodes, d_wavelength,
                             layers, d_fractions, d_m, n_nodes, n_angles, order, normalized,
                             allow_unvalidated, d_scalars, d_greek, d_phase, total_terms);
+}
+
+extern "C" int scatrs_size_message(int32_t kind, const double *p, char *buf, int64_t cap) {
+    if (!p || !buf || cap < 1) return SCATRS_INVALID_INPUT;
+    const std::string msg = dist_message(kind, p);
+    const size_t n = std::min<size_t>(msg.size(), (size_t)cap - 1);
+    std::memcpy(buf, msg.data(), n);
+    buf[n] = '\0';
+    return msg.empty() ? SCATRS_OK : SCATRS_INVALID_INPUT;
 }

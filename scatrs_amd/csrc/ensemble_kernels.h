@@ -1149,6 +1149,44 @@ static int upload(scatrs_ctx *ctx, int which, const std::vector<T> &v, T **out) 
                      "upload");
 }
 
+// The reference's message for invalid distribution parameters (distribution.rs:99-185), or
+// "" when they are valid.
+static std::string dist_message(int kind, const double *p) {
+    if (dist_valid(kind, p)) return "";
+    const double p0 = p[0], p1 = p[1], p2 = p[2];
+    const auto positive = [&](const char *name, double v) {
+        return std::string(name) + " must be finite and > 0, got " + fmt_e(v);
+    };
+    const bool ok0 = std::isfinite(p0) && p0 > 0.0;
+    if (kind == kLogNormal) {
+        if (!ok0) return positive("median_radius", p0);
+        return "the lognormal width must be finite and > 1, got " + fmt_e(p1);
+    }
+    if (kind == kGamma) {
+        if (!ok0) return positive("effective_radius", p0);
+        return "effective_variance must be in (0, 0.5), got " + fmt_e(p1);
+    }
+    if (kind == kModifiedGamma) {
+        if (!(std::isfinite(p0) && p0 > -1.0))
+            return "alpha must be finite and > -1, got " + fmt_e(p0);
+        if (!(std::isfinite(p1) && p1 > 0.0)) return positive("b", p1);
+        return positive("gamma", p2);
+    }
+    if (kind == kPowerLaw) {
+        if (!std::isfinite(p0)) return "exponent must be finite, got " + fmt_e(p0);
+        if (!(std::isfinite(p1) && p1 > 0.0)) return positive("min_radius", p1);
+        if (!(std::isfinite(p2) && p2 > 0.0)) return positive("max_radius", p2);
+        return "min_radius must be < max_radius, got " + fmt_e(p1) + " and " + fmt_e(p2);
+    }
+    if (kind == kMonodisperse) return positive("radius", p0);
+    if (kind == kSegment) {
+        if (!(std::isfinite(p0) && p0 > 0.0)) return positive("radius", p0);
+        if (!(std::isfinite(p1) && p1 > p0)) return "tabulated radii must increase";
+        return "a tabulated segment's tilt must be in [0, 1], got " + fmt_e(p2);
+    }
+    return "unknown distribution kind " + std::to_string(kind);
+}
+
 static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &dev, int64_t n_radius,
                              const std::vector<double> &rnodes, int allow, int *code_out) {
     const int64_t e = g / n_radius, i = g % n_radius;
@@ -1172,40 +1210,7 @@ static std::string ensemble_message(scatrs_ctx *ctx, int64_t g, const EnsArgs &d
     *code_out = kInvalid;
     double2 rw = make_double2(0.0, 0.0);
     if (dev.user_rw) (void)hipMemcpy(&rw, dev.user_rw + g, sizeof rw, hipMemcpyDeviceToHost);
-    if (!dev.user_rw && !dist_valid(kind, p)) {
-        // The reference's messages (distribution.rs:99-185).
-        const auto positive = [&](const char *name, double v) {
-            return pre + name + " must be finite and > 0, got " + fmt_e(v);
-        };
-        const bool ok0 = std::isfinite(p0) && p0 > 0.0;
-        if (kind == kLogNormal) {
-            if (!ok0) return positive("median_radius", p0);
-            return pre + "the lognormal width must be finite and > 1, got " + fmt_e(p1);
-        }
-        if (kind == kGamma) {
-            if (!ok0) return positive("effective_radius", p0);
-            return pre + "effective_variance must be in (0, 0.5), got " + fmt_e(p1);
-        }
-        if (kind == kModifiedGamma) {
-            if (!(std::isfinite(p0) && p0 > -1.0))
-                return pre + "alpha must be finite and > -1, got " + fmt_e(p0);
-            if (!(std::isfinite(p1) && p1 > 0.0)) return positive("b", p1);
-            return positive("gamma", p2);
-        }
-        if (kind == kPowerLaw) {
-            if (!std::isfinite(p0)) return pre + "exponent must be finite, got " + fmt_e(p0);
-            if (!(std::isfinite(p1) && p1 > 0.0)) return positive("min_radius", p1);
-            if (!(std::isfinite(p2) && p2 > 0.0)) return positive("max_radius", p2);
-            return pre + "min_radius must be < max_radius, got " + fmt_e(p1) + " and " + fmt_e(p2);
-        }
-        if (kind == kMonodisperse) return positive("radius", p0);
-        if (kind == kSegment) {
-            if (!(std::isfinite(p0) && p0 > 0.0)) return positive("radius", p0);
-            if (!(std::isfinite(p1) && p1 > p0)) return pre + "tabulated radii must increase";
-            return pre + "a tabulated segment's tilt must be in [0, 1], got " + fmt_e(p2);
-        }
-        return pre + "unknown distribution kind " + std::to_string(kind);
-    }
+    if (!dev.user_rw && !dist_valid(kind, p)) return pre + dist_message(kind, p);
     if (!(std::isfinite(wl) && wl > 0.0))
         return pre + "the wavelength must be finite and > 0, got " + fmt_e(wl);
     for (int l = 0; l + 1 < L; ++l)

@@ -269,3 +269,15 @@ def test_adaptive_mixture_of_layer_counts_with_derivatives():
         assert err.max() <= 2e-5, (want[k], err.max())
         fdg = (up["greek"] - dn["greek"]) / (2 * h)
         assert np.max(np.abs(raw["greek_d"][k] - fdg)) <= 2e-5 * np.max(np.abs(fdg))
+
+
+def test_adaptive_path_reports_sizes_outside_the_validated_domain():
+    """A law whose range reaches x > 1e4 (layered/domain and mie/domain limits) raises
+    OutsideValidatedDomainError with the reference's wavelength-and-radius context."""
+    from scatrs_amd import _wire
+    from scatrs_amd.errors import OutsideValidatedDomainError
+
+    items = [(("lognormal", 200.0, 2.0), ("sphere", np.array([1.5 + 0.01j])), 1.0)]
+    with pytest.raises(OutsideValidatedDomainError, match="at wavelength"):
+        _wire.ensemble_compute(items, np.array([0.3]), False, None, False, np.zeros(0), True,
+                               [], 1e-6, 4096, False, False)

@@ -96,3 +96,20 @@ def test_inputs_are_checked_as_the_reference_checks_them():
         _wire.ensemble_compute_batch([("lognormal", 0.1, 1.5)],
                                      ("layered", np.array([0.2, 0.5]), np.ones((2, 4), complex)),
                                      WL4, 8, False, np.zeros(0), True, 1e-6, 4096, False, False)
+
+
+def test_distribution_parameters_are_checked_before_any_device_work():
+    """The reference's messages for invalid parameters (distribution.rs:99-185), from the
+    library's host helper, for single laws, a multimodal law's mode and a batch."""
+    with pytest.raises(InvalidInputError, match="the lognormal width must be finite and > 1"):
+        _compute([(("lognormal", 0.1, 0.9), ("sphere", M4), 1.0)])
+    with pytest.raises(InvalidInputError, match="effective_variance must be in"):
+        _compute([(("gamma", 0.5, 0.7), ("sphere", M4), 1.0)])
+    with pytest.raises(InvalidInputError, match="mode 1: alpha must be finite and > -1"):
+        bi = ("multimodal", [(1.0, ("lognormal", 0.1, 1.5)),
+                             (1.0, ("modified_gamma", -2.0, 1.0, 1.0))])
+        _compute([(bi, ("sphere", M4), 1.0)])
+    with pytest.raises(InvalidInputError, match="min_radius must be < max_radius"):
+        _wire.ensemble_compute_batch([("lognormal", 0.1, 1.5), ("power_law", 3.0, 1.0, 0.5)],
+                                     ("sphere", M4), WL4, None, False, np.zeros(0), True, 1e-6,
+                                     4096, False, False)
