@@ -389,10 +389,10 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
     conv = ~live
     fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
     for _ in range(_MAX_ROUNDS):
-        todo = np.flatnonzero(np.isnan(K[:, 0]) & ~done[task])
-        if todo.size:
-            ti = task[todo]
-            r, wk = _nodes(tasks, ti, a[todo], b[todo], np.arange(15))
+        pending = np.flatnonzero(np.isnan(K[:, 0]) & ~done[task])
+        if pending.size:
+            ti = task[pending]
+            r, wk = _nodes(tasks, ti, a[pending], b[pending], np.arange(15))
             # One call for both rules: the Gauss rows are the same 15 nodes with the Gauss
             # weights (0 at the Kronrod-only nodes), which halves the calls of a round.
             nodes = np.concatenate([np.stack([r, wk], -1), np.stack([r, wk * fold], -1)])
@@ -400,7 +400,7 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
                                     allow_unvalidated=allow_unvalidated)
             terms += nt
             v = _refined(sc, z)
-            K[todo], G[todo] = v[:todo.size], v[todo.size:]
+            K[pending], G[pending] = v[:pending.size], v[pending.size:]
         act = ~done[task]
         if not act.any():
             break
