@@ -281,3 +281,24 @@ def test_adaptive_path_reports_sizes_outside_the_validated_domain():
     with pytest.raises(OutsideValidatedDomainError, match="at wavelength"):
         _wire.ensemble_compute(items, np.array([0.3]), False, None, False, np.zeros(0), True,
                                [], 1e-6, 4096, False, False)
+
+
+def test_adaptive_unnormalized_outputs():
+    """normalized=False on the adaptive engine (the reference's Normalization::Unnormalized):
+    the same scalars; the Greek coefficients and the phase matrix times <C_sca>, so that
+    alpha1_0 = <C_sca> (the projection is exact)."""
+    from scatrs_amd import _wire
+
+    wl = np.array([0.6, 1.0])
+    items = [(("gamma", 0.4, 0.15), ("sphere", np.full(2, 1.45 + 0.01j)), 1.0)]
+    mu = np.array([-0.5, 0.0, 0.7])
+    a = _wire.ensemble_compute(items, wl, False, 8, False, mu, True, [], 1e-7, 4096, False,
+                               False)
+    b = _wire.ensemble_compute(items, wl, False, 8, False, mu, False, [], 1e-7, 4096, False,
+                               False)
+    assert np.array_equal(a["panels"], b["panels"])
+    assert np.allclose(a["values"], b["values"], rtol=1e-14, atol=0)
+    sca = a["values"][1]
+    assert np.allclose(b["greek"][0, :, 0], sca, rtol=1e-12, atol=0)
+    assert np.allclose(b["greek"], a["greek"] * sca[None, :, None], rtol=1e-12, atol=1e-300)
+    assert np.allclose(b["phase"], a["phase"] * sca[None, :, None], rtol=1e-12, atol=0)
