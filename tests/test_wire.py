@@ -113,3 +113,57 @@ def test_distribution_parameters_are_checked_before_any_device_work():
         _wire.ensemble_compute_batch([("lognormal", 0.1, 1.5), ("power_law", 3.0, 1.0, 0.5)],
                                      ("sphere", M4), WL4, None, False, np.zeros(0), True, 1e-6,
                                      4096, False, False)
+
+
+def test_split_rules_pairs_the_gauss_copies_with_their_originals():
+    """_split_rules on a doubled task list (T tasks, then their Gauss copies) with R replicas:
+    panel rows [originals, copies] then monodisperse rows [originals, copies]; each half keeps
+    its rows in order, the copies' keys mapped back to their originals', and its tasks' Z."""
+    torch = pytest.importorskip("torch")
+    T, R, nr, nm = 3, 2, 4, 1  # per half: 4 panel rows, 1 monodisperse row
+    n_all = 2 * nr + 2 * nm
+    place_task = [0, 1, 1, 2] + [3, 4, 4, 5] + [2] + [5]  # tasks of the rows, in place order
+    key = torch.tensor([t * R + r for t in place_task for r in range(R)])
+    lin = torch.arange(n_all * R * 7, dtype=torch.float64).reshape(n_all * R, 7)
+    z = torch.arange(2 * T * R, dtype=torch.float64)
+    rw = dict(key=key, lin=lin, gr=None, ph=None, z=z, n_rows=2 * nr, n_mono=2 * nm)
+    avg = torch.arange(2 * T * R * 7, dtype=torch.float64).reshape(2 * T, R, 7)
+    (ka, _, _, _, _, kd), (ga, _, _, _, _, gd) = _wire._split_rules(
+        (avg, None, None, 0, 16, rw), T, R)
+    assert torch.equal(ka, avg[:T]) and torch.equal(ga, avg[T:])
+    assert torch.equal(kd["key"], gd["key"])  # the copies' keys are their originals'
+    assert kd["lin"].shape == gd["lin"].shape == ((nr + nm) * R, 7)
+    assert torch.equal(kd["lin"][:nr * R], lin[:nr * R])
+    assert torch.equal(gd["lin"][:nr * R], lin[nr * R:2 * nr * R])
+    assert torch.equal(gd["lin"][nr * R:], lin[(2 * nr + nm) * R:])
+    assert torch.equal(kd["z"], z[:T * R]) and torch.equal(gd["z"], z[T * R:])
+
+
+def test_results_from_the_wire_dict():
+    """Ensemble results built from the wire format's dict: shapes of the grid and
+    wavelengths, read-only arrays, error, d(name) with its own error, converged and panels."""
+    from scatrs_amd.ensemble import _from_wire
+
+    n, P, W, nmu = 4, 2, 3, 5
+    rng = np.random.default_rng(0)
+    raw = {"values": rng.random((11, n)), "errors": rng.random((11, n)),
+           "d": rng.random((P, 11, n)), "d_errors": rng.random((P, 11, n)),
+           "greek": rng.random((6, n, W)), "greek_errors": rng.random((6, n, W)),
+           "greek_d": rng.random((P, 6, n, W)), "greek_d_errors": rng.random((P, 6, n, W)),
+           "phase": rng.random((6, n, nmu)), "phase_errors": rng.random((6, n, nmu)),
+           "phase_d": rng.random((P, 6, n, nmu)), "phase_d_errors": rng.random((P, 6, n, nmu)),
+           "converged": np.array([True, False, True, True]),
+           "panels": np.arange(n, dtype=np.uint64)}
+    wl = np.array([[0.4, 0.5], [0.6, 0.7]])
+    mu = np.linspace(-1, 1, nmu)
+    r = _from_wire(raw, wl.shape, wl, mu, ["a", "b", "c"], [2, 0])
+    assert r.params == ("c", "a") and r.c_ext.shape == (2, 2)
+    assert np.array_equal(r.ssa.reshape(-1), raw["values"][4])
+    assert np.array_equal(r.error.g.reshape(-1), raw["errors"][5])
+    assert r.greek.alpha2.shape == (2, 2, W) and r.phase.f34.shape == (2, 2, nmu)
+    assert np.array_equal(r.d("a").c_sca.reshape(-1), raw["d"][1][1])
+    assert np.array_equal(r.d("c").error.greek.beta1.reshape(-1, W), raw["greek_d_errors"][0][4])
+    assert np.array_equal(r.converged.reshape(-1), raw["converged"])
+    assert not r.c_ext.flags.writeable
+    with pytest.raises(KeyError):
+        r.d("b")
