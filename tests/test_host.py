@@ -174,3 +174,50 @@ def test_adaptive_pass1_grows_the_range_for_steep_integrands(monkeypatch):
     sca = np.nansum(t.K[:, 1]) / t.moments[0]
     exact = r_g ** 6 * np.exp(18 * s * s)
     assert abs(sca - exact) <= 5 * rtol * exact
+
+
+def test_adaptive_pass2_rows_recombine_to_the_panel_sums(monkeypatch):
+    """pass2_torch on the CPU with the device call replaced by closed-form per-node quantities
+    (C_ext = pi r^2, alpha_l = r^l): the rows of 16 panels, grouped by their angle rules and
+    recombined by their weights, give each task's sums over all its panels divided by the
+    moment integration's Z, for every replica."""
+    torch = pytest.importorskip("torch")
+    from scatrs_amd import adaptive, ensemble
+
+    def device(nodes, wavelength, m, fractions=None, *, n_angles, order, normalized,
+               allow_unvalidated, want_phase, mu):
+        r, w = nodes[..., 0], nodes[..., 1]
+        z = w.sum(1)
+        sc = torch.zeros((r.shape[0], 11), dtype=torch.float64)
+        sc[:, 0] = (w * np.pi * r * r).sum(1) / z
+        sc[:, 1] = 0.5 * sc[:, 0]
+        sc[:, 9] = sc[:, 0]  # pi <r^2>
+        gr = torch.stack([torch.stack([(w * r ** l).sum(1) / z for l in range(order + 1)], 1)
+                          for _ in range(6)], 1)
+        return sc, gr, None, 0
+
+    monkeypatch.setattr(ensemble, "ensemble_nodes_device", device)
+    tasks = []
+    for p, wl in (((0.3, 1.6, 0.0), 0.5), ((0.5, 0.2, 0.0), 0.9)):
+        kind = 0 if p[1] > 1 else 1
+        q = adaptive.Task(kind, np.array(p), wl, np.array([1.5 + 0j]), np.zeros(0))
+        q.t_lo, q.t_hi, q.c = adaptive.integration_range(kind, q.p, 1e-8)
+        q.moments, q.d_moments = adaptive.moment_integrals(kind, q.p, q.c, 1e-8)
+        edges = np.linspace(q.t_lo, q.t_hi, 38)  # 37 panels: rows of 16, 16 and 5
+        q.a, q.b = edges[:-1], edges[1:]
+        tasks.append(q)
+    R, order = 2, 3
+    params = [(torch.tensor(np.tile(q.p, (R, 1))), torch.full((R, 1), 1.5, dtype=torch.float64),
+               torch.zeros((R, 1), dtype=torch.float64), None) for q in tasks]
+    avg, gr, ph, _, _, rows = adaptive.pass2_torch(
+        tasks, params, R=R, idx=np.arange(15), order=order, n_angles=None, mu=None,
+        want_phase=False, allow_unvalidated=False)
+    assert ph is None and avg.shape == (2, R, 7) and gr.shape == (2, R, 6, order + 1)
+    for t, q in enumerate(tasks):
+        r, w = adaptive._nodes(tasks, np.full(q.a.size, t), q.a, q.b, np.arange(15))
+        zm = q.moments[0]
+        for k in range(R):
+            assert np.isclose(avg[t, k, 0].item(), (w * np.pi * r * r).sum() / zm, rtol=1e-13)
+            for l in range(order + 1):
+                assert np.isclose(gr[t, k, 2, l].item(), (w * r ** l).sum() / zm, rtol=1e-13)
+    assert rows["n_rows"] == 6 and rows["n_mono"] == 0
