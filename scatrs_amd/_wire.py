@@ -506,11 +506,11 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
             p0 = p.detach().cpu().numpy()
             m0 = (m_re.detach() + 1j * m_im.detach()).cpu().numpy()
             f0 = fr.detach().cpu().numpy()
+            assert p.shape[0] == B  # rows b = r n + i: wavelength i's replicas are [i::n]
             for i in range(n):
-                rows = torch.arange(R, device=dev) * n + i
                 tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
                                            f0[i].copy()))
-                params.append((p[rows], m_re[rows], m_im[rows], fr[rows] if L > 1 else None))
+                params.append((p[i::n], m_re[i::n], m_im[i::n], fr[i::n] if L > 1 else None))
         adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
         unit = torch.tensor([part[0] for part in parts], device=dev)
         f = torch.stack([part[6] for part in parts])  # (parts, B)

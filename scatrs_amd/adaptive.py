@@ -305,10 +305,16 @@ def _prepare(tasks, rtol: float) -> None:
     live = [q for q in tasks if q.kind != MONODISPERSE]
     if not live:
         return
-    T = len(live)
-    kind = np.array([q.kind for q in live], np.int32)
-    p = np.stack([q.p for q in live])
-    lo, hi = np.array([size_range(q.kind, q.p) for q in live]).T
+    # Ranges and moments depend on the law alone: once per distinct (kind, p), bit for bit
+    # (a sweep over wavelengths shares them).
+    key = np.concatenate([np.array([[q.kind] for q in live], np.float64),
+                          np.stack([q.p for q in live])], 1)
+    _, first, inv = np.unique(np.ascontiguousarray(key).view(np.dtype((np.void, 32))).ravel(),
+                              return_index=True, return_inverse=True)
+    T = first.size
+    kind = key[first, 0].astype(np.int32)
+    p = key[first, 1:]
+    lo, hi = np.array([size_range(k, q) for k, q in zip(kind, p)]).T
     support = np.isin(kind, (POWER_LAW, SEGMENT))
     mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
     span = np.where(support, 1.0, 3.0)
@@ -353,9 +359,9 @@ def _prepare(tasks, rtol: float) -> None:
     fk = np.stack([f, f * r * r, f * r ** 3], 1)  # (T, 3, n)
     mom = fk.sum(2)
     dmom = np.einsum("tkn,tnp->tkp", fk, d)
-    for i, q in enumerate(live):
+    for q, i in zip(live, inv.reshape(-1)):
         q.t_lo, q.t_hi, q.c = float(t_lo[i]), float(t_hi[i]), float(c[i])
-        q.moments, q.d_moments = mom[i], dmom[i]
+        q.moments, q.d_moments = mom[i].copy(), dmom[i].copy()
 
 
 def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
@@ -406,10 +412,9 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             break
         # Per task: totals, error sums and targets (engine.rs:620-690).
         E = np.abs(K - G)
-        tot = np.zeros((T, 5))
-        esum = np.zeros((T, 5))
-        np.add.at(tot, task[act], K[act])
-        np.add.at(esum, task[act], E[act])
+        ta = task[act]
+        tot = np.stack([np.bincount(ta, K[act, j], minlength=T) for j in range(5)], 1)
+        esum = np.stack([np.bincount(ta, E[act, j], minlength=T) for j in range(5)], 1)
         sca = np.abs(tot[:, 1])
         tgt = rtol * np.stack([np.abs(tot[:, 0]), sca, sca, np.abs(tot[:, 0]),
                                np.maximum(np.abs(tot[:, 4]), sca)], 1)
@@ -435,15 +440,12 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
         ratio = np.where(fail[task], E / tg[task], 0.0).max(1)
         cut = (ratio > 1.0 / n_pan[task]) & ~done[task]
         worst = np.full(T, -1.0)
-        np.maximum.at(worst, task, ratio)
+        starts = np.flatnonzero(first)  # the table is sorted by task
+        worst[task[starts]] = np.maximum.reduceat(ratio, starts)
         has_fail = fail.any(1) & ~done
-        pick = has_fail[task] & (ratio == worst[task])
-        firsts = np.flatnonzero(pick)
-        seen = np.zeros(T, bool)
-        for i in firsts:  # the first panel at each failing task's largest ratio
-            if not seen[task[i]]:
-                seen[task[i]] = True
-                cut[i] = True
+        pick = np.flatnonzero(has_fail[task] & (ratio == worst[task]))
+        # the first panel at each failing task's largest ratio
+        cut[pick[np.unique(task[pick], return_index=True)[1]]] = True
         n_cut = np.bincount(task[cut], minlength=T)
         over = (n_pan + n_cut + grow_lo + grow_hi > max_panels) & ~done
         done |= over

@@ -51,7 +51,8 @@ def build(verbose: bool = False) -> Path:
         for f in [pool.submit(_run, c, verbose) for c in cmds]:
             f.result()
     tmp = OUT.with_suffix(".so.tmp")
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs], verbose)
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *objs],
+         verbose)
     os.replace(tmp, OUT)
     return OUT
 

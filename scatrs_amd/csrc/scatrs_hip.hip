@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/scatrs_hip.h"
 #include "checks.h"
@@ -274,10 +276,34 @@ int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi)
 int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
                             double *ln_u, double *d_ln_u) {
     if (n < 0 || (n > 0 && (!kind || !p || !r || !ln_u))) return SCATRS_INVALID_INPUT;
-    for (int64_t i = 0; i < n; ++i) {
-        if (!scatrs::dist_valid(kind[i], p + 3 * i)) return SCATRS_INVALID_INPUT;
-        ln_u[i] = scatrs::log_density(kind[i], p + 3 * i, r[i], d_ln_u ? d_ln_u + 3 * i : nullptr);
+    auto run = [=](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+            if (!scatrs::dist_valid(kind[i], p + 3 * i)) return false;
+            ln_u[i] = scatrs::log_density(kind[i], p + 3 * i, r[i], d_ln_u ? d_ln_u + 3 * i : nullptr);
+        }
+        return true;
+    };
+    // The radii are independent: slices of at least 32768 on up to 16 host threads (the
+    // adaptive engine asks for millions at once). A thread that cannot start leaves its slice
+    // to the calling thread.
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<int64_t>({(n + 32767) / 32768, (int64_t)hw, 16});
+    if (nt <= 1) return run(0, n) ? SCATRS_OK : SCATRS_INVALID_INPUT;
+    std::vector<std::thread> pool;
+    std::vector<char> ok(nt, 1), started(nt, 0);
+    for (int t = 1; t < nt; ++t) {
+        try {
+            pool.emplace_back([&ok, &run, t, nt, n] { ok[t] = run(t * n / nt, (t + 1) * n / nt); });
+            started[t] = 1;
+        } catch (...) {
+        }
     }
+    ok[0] = run(0, n / nt);
+    for (int t = 1; t < nt; ++t)
+        if (!started[t]) ok[t] = run(t * n / nt, (t + 1) * n / nt);
+    for (auto &th : pool) th.join();
+    for (char v : ok)
+        if (!v) return SCATRS_INVALID_INPUT;
     return SCATRS_OK;
 }
 

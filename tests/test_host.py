@@ -221,3 +221,29 @@ def test_adaptive_pass2_rows_recombine_to_the_panel_sums(monkeypatch):
             for l in range(order + 1):
                 assert np.isclose(gr[t, k, 2, l].item(), (w * r ** l).sum() / zm, rtol=1e-13)
     assert rows["n_rows"] == 6 and rows["n_mono"] == 0
+
+
+def test_log_density_on_host_threads_equals_the_serial_loop():
+    """scatrs_size_log_density splits a long call over host threads: every radius gives the
+    same bits as in calls short enough for one thread, and an invalid parameter row in the last
+    slice still fails the call."""
+    from scatrs_amd.adaptive import log_density
+
+    rng = np.random.default_rng(3)
+    n = 300_000
+    kind = rng.integers(0, 3, n).astype(np.int32)
+    p = np.stack([rng.uniform(0.1, 1.0, n), rng.uniform(1.2, 2.0, n), np.zeros(n)], 1)
+    gam = kind != 0
+    p[gam, 1] = rng.uniform(0.05, 0.3, gam.sum())  # effective variance
+    p[kind == 2, 2] = rng.uniform(0.5, 2.0, (kind == 2).sum())
+    p[kind == 2, 1] = rng.uniform(1.0, 3.0, (kind == 2).sum())
+    r = np.exp(rng.uniform(-4.0, 1.0, n))
+    lu, d = log_density(kind, p, r, derivatives=True)
+    for lo in range(0, n, 30_000):
+        s = slice(lo, lo + 30_000)
+        a, b = log_density(kind[s], p[s], r[s], derivatives=True)
+        assert np.array_equal(a, lu[s]) and np.array_equal(b, d[s])
+    p[-1, 1] = 0.5  # a lognormal width <= 1 or an effective variance >= 0.5
+    kind[-1] = 0
+    with pytest.raises(InvalidInputError):
+        log_density(kind, p, r)
