@@ -638,6 +638,28 @@ def pass2(tasks, *, order, n_angles, mu, want_phase, allow_unvalidated):
     return out[0], out[1], terms, int(rules[0]) if len(rules) else 2
 
 
+def _row_panels(tasks):
+    """The pass-2 rows of every task with panels, in task order: the task of each row and the
+    [a, b] of its up to _ROW_PANELS panels (nr, _ROW_PANELS), a short last row padded with
+    panels of width 0 at the task's upper end."""
+    own = np.array([t for t, q in enumerate(tasks) if q.kind != MONODISPERSE and q.a.size],
+                   np.int64)
+    if not own.size:
+        return own, np.zeros((0, _ROW_PANELS)), np.zeros((0, _ROW_PANELS))
+    cnt = np.array([tasks[t].a.size for t in own], np.int64)
+    n_rows = -(-cnt // _ROW_PANELS)
+    first_row = np.cumsum(n_rows) - n_rows
+    j = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)  # panel within its task
+    row = np.repeat(first_row, cnt) + j // _ROW_PANELS
+    col = j % _ROW_PANELS
+    pad = np.repeat(np.array([tasks[t].b[-1] for t in own]), n_rows)
+    A = np.repeat(pad[:, None], _ROW_PANELS, 1)
+    Bd = A.copy()
+    A[row, col] = np.concatenate([tasks[t].a for t in own])
+    Bd[row, col] = np.concatenate([tasks[t].b for t in own])
+    return np.repeat(own, n_rows), A, Bd
+
+
 def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
                 allow_unvalidated, gauss_tasks=None):
     """The Kronrod (idx all 15 nodes) or Gauss rule's pass 2, differentiable in every task's
@@ -712,14 +734,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     layers = np.array([q.layers for q in tasks])
     nr = len(rows)
     if nr:
-        ti_rows = np.array([r[0] for r in rows])
-        A = np.zeros((nr, _ROW_PANELS))
-        Bd = np.zeros((nr, _ROW_PANELS))
-        for j, (ti, lo, hi) in enumerate(rows):
-            q = tasks[ti]
-            n = hi - lo
-            A[j, :n], Bd[j, :n] = q.a[lo:hi], q.b[lo:hi]
-            A[j, n:] = Bd[j, n:] = q.b[hi - 1]
+        ti_rows, A, Bd = _row_panels(tasks)
         grad = torch.is_grad_enabled()  # without gradients the weights' derivatives go unused
         out_n = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
                        idx, derivatives=grad)
