@@ -394,6 +394,7 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
     done = ~live
     conv = ~live
     fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
+    finished = []
     for _ in range(_MAX_ROUNDS):
         pending = np.flatnonzero(np.isnan(K[:, 0]) & ~done[task])
         if pending.size:
@@ -456,6 +457,11 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             if done[ids].all():
                 break
             continue
+        fin = done[task]  # finished tasks' panels leave the table: later rounds skip them
+        if fin.any():
+            finished.append((a[fin], b[fin], task[fin], K[fin], G[fin]))
+            a, b, task, K, G, cut, first, last = (
+                x[~fin] for x in (a, b, task, K, G, cut, first, last))
         mid = 0.5 * (a + b)
         keep = ~cut
         na = [a[keep], a[cut], mid[cut]]
@@ -482,6 +488,11 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
         ng.append(np.full((grown, 5), np.nan))
         a, b, task = np.concatenate(na), np.concatenate(nb), np.concatenate(nt_)
         K, G = np.concatenate(nk), np.concatenate(ng)
+        order = np.lexsort((a, task))
+        a, b, task, K, G = a[order], b[order], task[order], K[order], G[order]
+    if finished:
+        a, b, task, K, G = (np.concatenate([f[j] for f in finished] + [x])
+                            for j, x in enumerate((a, b, task, K, G)))
         order = np.lexsort((a, task))
         a, b, task, K, G = a[order], b[order], task[order], K[order], G[order]
     bounds = np.searchsorted(task, np.arange(T + 1))
