@@ -193,7 +193,9 @@ int scatrs_ensemble_nodes_vjp(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
  * unit radius, up to a constant, at n points, each with its own distribution (kind[i],
  * p[3 i .. 3 i + 2]) and radius r[i] (distribution.rs:366-430; the density per ln r is u r),
  * with d_ln_u (n, 3) not NULL also d ln u / d p (the reference radius held fixed).
- * SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. */
+ * SCATRS_INVALID_INPUT for invalid parameters. No context, no device work. A log-density
+ * call of more than 32768 points runs on up to 16 host threads that it starts and joins
+ * before it returns. The results do not depend on the thread count. */
 int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi);
 /* The reference's message for invalid parameters p[3] of a distribution (distribution.rs:
  * 99-185) into buf (cap bytes, NUL-terminated): SCATRS_INVALID_INPUT with the message, or
