@@ -36,6 +36,10 @@ HBM_PEAK_GBS = 8000.0
 MIE_BATCH_TRAFFIC_BYTES = 1.207e9
 
 
+# The ensemble legs after the sphere batch, in the order they run.
+LEGS = ("ensembles", "adjoint", "layered", "distributions", "adaptive")
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -144,6 +148,9 @@ def bench_ensembles(args, torch):
     from scatrs_amd.ensemble import ensemble_fixed_device
 
     kind, p0, p1, wl, m = workloads.lognormals()
+    if args.ens_dists < 4096:
+        keep = args.ens_dists * 64
+        kind, p0, p1, wl, m = kind[:keep], p0[:keep], p1[:keep], wl[:keep], m[:keep]
     nr, na, order = 512, 1024, 256
     h = (na + 1) // 2
     ctx = _native.context(0)
@@ -495,6 +502,12 @@ def main() -> int:
                     help="HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc")
     ap.add_argument("--no-ensembles", action="store_true",
                     help="only the sphere batch (skip the ensemble, adjoint and layered lines)")
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma-separated ensemble legs to run after the sphere batch, from "
+                         + ", ".join(LEGS) + " (or 'none')")
+    ap.add_argument("--ens-dists", type=int, default=4096,
+                    help="distributions of the 4096 x 64 lognormal workload to run (a prefix; "
+                         "for profiling runs)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip timing the plain FP64 angular kernel")
     ap.add_argument("--ens-steps", type=int, default=1)
@@ -523,12 +536,14 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"copy calibration: {args.copy_calibration} MiB x 3")
     res = bench_spheres(args, torch)
-    if not args.no_ensembles:
-        res["ensembles"] = bench_ensembles(args, torch)
-        res["adjoint"] = bench_adjoint(args, torch)
-        res["layered"] = bench_layered(args, torch)
-        res["distributions"] = bench_distributions(args, torch)
-        res["adaptive"] = bench_adaptive(args, torch)
+    legs = [] if args.no_ensembles or args.legs == "none" else args.legs.split(",")
+    for leg in legs:
+        if leg not in LEGS:
+            print(f"unknown leg {leg!r}; the legs are {', '.join(LEGS)}", file=sys.stderr)
+            return 2
+    for leg in LEGS:
+        if leg in legs:
+            res[leg] = globals()["bench_" + leg](args, torch)
     print(json.dumps(res), flush=True)
     return 0
 
