@@ -324,23 +324,56 @@ def bench_layered(args, torch):
         bg = _dev(torch, rng.normal(size=(kind.size, 6, order + 1)))
 
         def step():
-            _, _, _, terms = ensemble_fixed_device(*t, tfr, n_radius=nr, n_angles=na, order=order,
-                                                   want_phase=False)
+            sc, gr, _, terms = ensemble_fixed_device(*t, tfr, n_radius=nr, n_angles=na,
+                                                     order=order, want_phase=False)
             g, _ = ensemble_fixed_vjp_device(*t, tfr, bar_scalars=bs, bar_greek=bg, n_radius=nr,
                                              n_angles=na, order=order)
-            return terms
+            return terms, sc, gr, g
 
-        secs = {}
+        secs, outs = {}, {}
         for kernel in (2, 0):  # fp16-split forward tables, then FP64 throughout
             ctx.set_angular_kernel(kernel)
             try:
                 step()
-                secs[kernel], _, terms = _timed(torch, ctx, step, 1)
+                secs[kernel], _, (terms, *outs[kernel]) = _timed(torch, ctx, step, 1)
             finally:
                 ctx.set_angular_kernel(0)
+        # The timed outputs: finite everywhere, and one sampled row against the oracle (forward
+        # and the VJP against the oracle's Jacobian contracted with the same cotangents).
+        sc, gr, g = outs[2]
+        finite = all(bool(torch.isfinite(v).all().item()) for v in (sc, gr, g, *outs[0]))
+        assert finite, f"non-finite layered outputs at L={L}"
+        check = None
+        if not args.no_check:
+            import oracle
+
+            e = int(np.random.default_rng(60 + L).integers(kind.size))
+            ref = oracle.ensemble_fixed(1, p0[e], p1[e], wl[e], m[e], fractions=fr[e],
+                                        n_radius=nr, n_angles=na, order=order)
+            s_e, g_e = sc[e].cpu().numpy(), gr[e].cpu().numpy()
+            k = [0, 1, 4, 5]
+            J = oracle.ensemble_fixed_jacobian(1, p0[e], p1[e], wl[e], m[e], fr[e], nr, na,
+                                               order)
+            bs_e, bg_e = bs[e].cpu().numpy(), bg[e].cpu().numpy()
+            terms_j = np.concatenate([J["d_scalars"] * bs_e[None, :],
+                                      (J["d_greek"] * bg_e[None]).reshape(
+                                          J["d_greek"].shape[0], -1)], axis=1)
+            scale = np.abs(terms_j).sum(axis=1)
+            gerr = np.abs(g[e].cpu().numpy() - terms_j.sum(axis=1)) / np.where(scale > 0, scale,
+                                                                              1.0)
+            check = {"row": e,
+                     "c_ext_c_sca_ssa_g": float(np.max(np.abs(s_e[k] - ref["scalars"][k]) /
+                                                       np.abs(ref["scalars"][k]))),
+                     "alpha1": float(np.max(np.abs(g_e[0] - ref["greek"][0])) /
+                                     np.max(np.abs(ref["greek"][0]))),
+                     "vjp_vs_oracle_jacobian": float(gerr.max())}
+            assert check["c_ext_c_sca_ssa_g"] < 1e-6 and check["alpha1"] < 1e-6, check
+            assert check["vjp_vs_oracle_jacobian"] < 1e-5, check
+            log(f"layered L={L} row {e} vs oracle: {check}")
+        del outs, sc, gr, g
         sec = secs[2]
         per_l[str(L)] = {"ensembles": int(kind.size), "s": sec, "s_fp64_matrix_cores": secs[0],
-                         "terms": int(terms)}
+                         "terms": int(terms), "finite": finite, "check_vs_oracle": check}
         total_s += sec
         total64 += secs[0]
         total_e += kind.size
