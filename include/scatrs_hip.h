@@ -91,6 +91,22 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
                             int64_t n_max, int allow_unvalidated, double *d_values,
                             double *d_jacobian, int64_t *total_terms);
 
+/* scatrs_mie_efficiencies without any device allocation: the same arguments, kernels and
+ * results, with every buffer beyond the inputs and outputs (status, N histogram, keys,
+ * permutation, the persistent wavefronts' slabs) carved from a 64 MiB array that the library
+ * holds in each device's memory from its load until the process ends. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The array's address on a device is
+ * looked up when the first context on that device is created (scatrs_create).
+ *   len <= 2^20 spheres, else SCATRS_INVALID_INPUT; spheres whose N needs more slab memory than
+ *   the array holds for 4 wavefronts, SCATRS_INVALID_INPUT (about N > 20 000).
+ * Every call on a device shares that device's array: calls that overlap on one device (two
+ * streams, two threads) race and must be serialized by the caller. Synchronizes once, like
+ * scatrs_mie_efficiencies. */
+int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const double *d_m,
+                                    int64_t len, int64_t n_max, int allow_unvalidated,
+                                    double *d_values, double *d_jacobian, int64_t *total_terms);
+
 /* Size distributions (scatrs/src/ensemble/distribution.rs:15-68) and their parameters. */
 #define SCATRS_LOGNORMAL 0      /* p0 = median radius r_g, p1 = width sigma_g */
 #define SCATRS_GAMMA 1          /* p0 = effective radius, p1 = effective variance */

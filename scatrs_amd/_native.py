@@ -37,6 +37,7 @@ SYMBOLS = (
     "scatrs_set_phase_cosines",
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
+    "scatrs_mie_efficiencies_noalloc",
     "scatrs_ensemble_fixed",
     "scatrs_ensemble_fixed_vjp",
     "scatrs_ensemble_nodes",
@@ -78,6 +79,8 @@ def load() -> C.CDLL:
         L.scatrs_default_n_max.restype = i64
         L.scatrs_mie_efficiencies.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
                                               C.POINTER(i64)]
+        L.scatrs_mie_efficiencies_noalloc.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
+                                                      C.POINTER(i64)]
         if hasattr(L, "scatrs_ensemble_fixed"):
             L.scatrs_ensemble_fixed.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp, i64,
                                                 i64, i64, i32, i32, vp, vp, vp,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -142,6 +143,141 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     }
 }
 
+
+// ---- the batched-sphere path, shared by the allocating and the no-allocation entry points ----
+
+constexpr size_t kStatusBytes = sizeof(Status) + 64;  // status block, then the work counter
+constexpr size_t kHistBytes = 2 * kBuckets * sizeof(unsigned);
+
+// The device memory of one sphere batch beyond its inputs and outputs.
+struct MieWork {
+    void *status = nullptr;  // Status, then the persistent kernel's work counter
+    unsigned *hist = nullptr;
+    int *keys = nullptr;
+    int *perm = nullptr;
+    Partial *partial = nullptr;
+    void *slab = nullptr;  // the wavefronts' checkpoint and u_k(x) slabs
+};
+
+// Bytes of one wavefront's slab for spheres of up to nmax terms.
+size_t mie_slab_per_wave(int64_t nmax) {
+    const int64_t rows = nmax + 2;
+    const int64_t rows_ck = nmax / kZBlock + 2;
+    return (size_t)64 * (rows_ck * sizeof(cx) + rows * sizeof(double));
+}
+
+// Argument checks common to both entry points.
+int mie_begin(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+              double *d_values, int64_t *total_terms) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->error.clear();
+    if (total_terms) *total_terms = 0;
+    if (len < 0) return fail(ctx, SCATRS_INVALID_INPUT, "the number of spheres must be >= 0");
+    if (len == 0) return SCATRS_OK;
+    if (len > (int64_t)INT32_MAX)
+        return fail(ctx, SCATRS_INVALID_INPUT, "at most 2^31 - 1 spheres per call");
+    if (!d_x || !d_m || !d_values)
+        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    return check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
+}
+
+// The checks of every sphere and the N histogram; waits for the status (the one
+// synchronization of a call) and maps the first failing element to the reference's message.
+int mie_validate(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double *d_m,
+                 int64_t len, int64_t n_max, int allow_unvalidated, int64_t *nmax_out,
+                 int64_t *total_terms) {
+    int rc;
+    const int threads = 256;
+    const int64_t blocks = (len + threads - 1) / threads;
+    Status *st = static_cast<Status *>(w.status);
+    unsigned long long *chunk_counter = reinterpret_cast<unsigned long long *>(
+        static_cast<char *>(w.status) + sizeof(Status));
+    hipStream_t s = ctx->stream;
+    fill_u64<<<1, 1, 0, s>>>(chunk_counter, 0ull, 1);
+    fill_u32<<<(2 * kBuckets + 255) / 256, 256, 0, s>>>(w.hist, 0u, 2 * kBuckets);
+    mie_prep<<<(unsigned)blocks, threads, 0, s>>>(d_x, d_m, len, n_max, allow_unvalidated, w.keys,
+                                                    w.partial);
+    reduce_status<<<1, 1024, 0, s>>>(w.partial, blocks, st);
+    histogram_keys<<<(unsigned)((len + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(w.keys, len,
+                                                                                   w.hist);
+    Status *hst = static_cast<Status *>(ctx->host_pinned);
+    if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
+                        "status copy")))
+        return rc;
+    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "prep"))) return rc;
+    const unsigned long long bad = std::min(hst->first_bad, hst->first_outside);
+    if (bad != ~0ull) {
+        double h[3];
+        (void)hipMemcpy(&h[0], d_x + bad, sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&h[1], d_m + 2 * bad, 2 * sizeof(double), hipMemcpyDeviceToHost);
+        std::string msg;
+        const int code = sphere_message(h[0], h[1], h[2], n_max, allow_unvalidated, &msg);
+        return fail(ctx, code == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
+                    "element " + std::to_string(bad) + ": " + msg);
+    }
+    *nmax_out = hst->n_max;
+    if (total_terms) *total_terms = (int64_t)hst->total_terms;
+    return SCATRS_OK;
+}
+
+// The N-ranked permutation and the persistent solve on `waves` wavefronts (a multiple of 4)
+// whose slabs start at w.slab.
+int mie_solve(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double *d_m,
+              int64_t len, int64_t n_max, int64_t nmax, int64_t waves, double *d_values,
+              double *d_jacobian) {
+    hipStream_t s = ctx->stream;
+    sort_by_keys(w.keys, w.hist, w.perm, len, s);
+    const int64_t rows = nmax + 2;
+    const int64_t rows_ck = nmax / kZBlock + 2;
+    unsigned long long *chunk_counter = reinterpret_cast<unsigned long long *>(
+        static_cast<char *>(w.status) + sizeof(Status));
+    cx *ck = static_cast<cx *>(w.slab);
+    double *ux = reinterpret_cast<double *>(ck + (size_t)waves * rows_ck * 64);
+    mark_begin(ctx);
+    if (d_jacobian)
+        mie_batch<true><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, w.perm,
+                                                             chunk_counter, ck, ux, rows_ck,
+                                                             rows, d_values, d_jacobian);
+    else
+        mie_batch<false><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, w.perm,
+                                                              chunk_counter, ck, ux, rows_ck,
+                                                              rows, d_values, nullptr);
+    mark_end(ctx);
+    return check_hip(ctx, hipGetLastError(), "mie_batch launch");
+}
+
+// ---- the no-allocation scratch ----
+
+// Device memory of scatrs_mie_efficiencies_noalloc: every call on a device carves its status,
+// histogram, keys, permutation, partials and slabs from this array, which the device holds
+// from the library's load until the process ends. Calls that overlap on one device race on
+// it. In host code the name is only a handle for hipGetSymbolAddress: the device address of
+// each device's copy comes from scratch_device_ptr.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int64_t kNoAllocMaxSpheres = int64_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+void *scratch_ptr[kMaxDevices] = {};
+bool scratch_tried[kMaxDevices] = {};
+
+// Looks up the scratch array's address on `device` (current) once per device.
+void scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (scratch_tried[device]) return;
+    scratch_tried[device] = true;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) scratch_ptr[device] = p;
+}
+
+void *scratch_device_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    return scratch_ptr[device];
+}
+
 }  // namespace scatrs
 
 using namespace scatrs;
@@ -157,6 +293,7 @@ int scatrs_create(int device, void *stream, scatrs_ctx **out) {
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
         return SCATRS_DEVICE_ERROR;
     if (hipSetDevice(device) != hipSuccess) return SCATRS_DEVICE_ERROR;
+    scratch_init(device);
     scatrs_ctx *ctx = new scatrs_ctx();
     ctx->device = device;
     ctx->stream = static_cast<hipStream_t>(stream);
@@ -310,86 +447,90 @@ int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, con
 int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
                             int64_t n_max, int allow_unvalidated, double *d_values,
                             double *d_jacobian, int64_t *total_terms) {
-    if (!ctx) return SCATRS_INVALID_INPUT;
-    ctx->error.clear();
-    if (total_terms) *total_terms = 0;
-    if (len < 0) return fail(ctx, SCATRS_INVALID_INPUT, "the number of spheres must be >= 0");
-    if (len == 0) return SCATRS_OK;
-    if (len > (int64_t)INT32_MAX)
-        return fail(ctx, SCATRS_INVALID_INPUT, "at most 2^31 - 1 spheres per call");
-    if (!d_x || !d_m || !d_values)
-        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
-    int rc;
-    if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, sizeof(Status) + 64), "alloc")))
+    int rc = mie_begin(ctx, d_x, d_m, len, d_values, total_terms);
+    if (rc != SCATRS_OK || len == 0) return rc;
+    const int64_t blocks = (len + 255) / 256;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, kStatusBytes), "alloc")))
         return rc;
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHist, 2 * kBuckets * sizeof(unsigned)), "alloc")))
-        return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHist, kHistBytes), "alloc"))) return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kKeys, (size_t)len * sizeof(int)), "alloc")))
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kPerm, (size_t)len * sizeof(int)), "alloc")))
         return rc;
-    const int threads = 256;
-    const int64_t blocks = (len + threads - 1) / threads;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kNodes, (size_t)blocks * sizeof(Partial)),
                         "alloc")))
         return rc;
-    Partial *partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kNodes]);
-    Status *st = static_cast<Status *>(ctx->buf[scatrs_ctx::kCounters]);
-    unsigned long long *chunk_counter =
-        reinterpret_cast<unsigned long long *>(static_cast<char *>(ctx->buf[scatrs_ctx::kCounters]) + sizeof(Status));
-    unsigned *hist = static_cast<unsigned *>(ctx->buf[scatrs_ctx::kHist]);
-    int *keys = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
-    int *perm = static_cast<int *>(ctx->buf[scatrs_ctx::kPerm]);
-    hipStream_t s = ctx->stream;
-    fill_u64<<<1, 1, 0, s>>>(chunk_counter, 0ull, 1);
-    fill_u32<<<(2 * kBuckets + 255) / 256, 256, 0, s>>>(hist, 0u, 2 * kBuckets);
-    mie_prep<<<(unsigned)blocks, threads, 0, s>>>(d_x, d_m, len, n_max, allow_unvalidated, keys,
-                                                    partial);
-    reduce_status<<<1, 1024, 0, s>>>(partial, blocks, st);
-    histogram_keys<<<(unsigned)((len + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(keys, len,
-                                                                                   hist);
-    Status *hst = static_cast<Status *>(ctx->host_pinned);
-    if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
-                        "status copy")))
+    MieWork w;
+    w.status = ctx->buf[scatrs_ctx::kCounters];
+    w.hist = static_cast<unsigned *>(ctx->buf[scatrs_ctx::kHist]);
+    w.keys = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
+    w.perm = static_cast<int *>(ctx->buf[scatrs_ctx::kPerm]);
+    w.partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kNodes]);
+    int64_t nmax = 0;
+    if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
         return rc;
-    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "prep"))) return rc;
-    const unsigned long long bad = std::min(hst->first_bad, hst->first_outside);
-    if (bad != ~0ull) {
-        double h[3];
-        (void)hipMemcpy(&h[0], d_x + bad, sizeof(double), hipMemcpyDeviceToHost);
-        (void)hipMemcpy(&h[1], d_m + 2 * bad, 2 * sizeof(double), hipMemcpyDeviceToHost);
-        std::string msg;
-        const int code = sphere_message(h[0], h[1], h[2], n_max, allow_unvalidated, &msg);
-        return fail(ctx, code == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
-                    "element " + std::to_string(bad) + ": " + msg);
-    }
-    const int64_t nmax = hst->n_max;
-    if (total_terms) *total_terms = (int64_t)hst->total_terms;
-    sort_by_keys(keys, hist, perm, len, s);
     // Persistent wavefronts: 8 per CU (two 4-wave blocks share the CU's LDS tiles), fewer if
     // the slabs would pass 8 GiB.
-    const int64_t rows = nmax + 2;
-    const int64_t rows_ck = nmax / kZBlock + 2;
-    const size_t per_wave = (size_t)64 * (rows_ck * sizeof(cx) + rows * sizeof(double));
+    const size_t per_wave = mie_slab_per_wave(nmax);
     int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (len + 63) / 64);
     waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((8ull << 30) / per_wave)));
     waves = (waves + 3) / 4 * 4;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves), "scratch")))
         return rc;
-    cx *ck = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
-    double *ux = reinterpret_cast<double *>(ck + (size_t)waves * rows_ck * 64);
-    mark_begin(ctx);
-    if (d_jacobian)
-        mie_batch<true><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, perm,
-                                                             chunk_counter, ck, ux, rows_ck,
-                                                             rows, d_values, d_jacobian);
-    else
-        mie_batch<false><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, perm,
-                                                              chunk_counter, ck, ux, rows_ck,
-                                                              rows, d_values, nullptr);
-    mark_end(ctx);
-    return check_hip(ctx, hipGetLastError(), "mie_batch launch");
+    w.slab = ctx->buf[scatrs_ctx::kScratch];
+    return mie_solve(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jacobian);
+}
+
+int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const double *d_m,
+                                    int64_t len, int64_t n_max, int allow_unvalidated,
+                                    double *d_values, double *d_jacobian, int64_t *total_terms) {
+    int rc = mie_begin(ctx, d_x, d_m, len, d_values, total_terms);
+    if (rc != SCATRS_OK || len == 0) return rc;
+    if (len > kNoAllocMaxSpheres)
+        return fail(ctx, SCATRS_INVALID_INPUT,
+                    "the no-allocation entry point takes at most " +
+                        std::to_string(kNoAllocMaxSpheres) + " spheres per call, got " +
+                        std::to_string(len));
+    char *base = static_cast<char *>(scratch_device_ptr(ctx->device));
+    if (!base)
+        return fail(ctx, SCATRS_DEVICE_ERROR,
+                    "the no-allocation scratch of device " + std::to_string(ctx->device) +
+                        " was not set up (scatrs_create could not find its address)");
+    // The scratch array, carved in 256-byte aligned pieces: status, histogram, keys,
+    // permutation, per-block partials, then the wavefronts' slabs in what is left.
+    const int64_t blocks = (len + 255) / 256;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base + off;
+        off += (bytes + 255) & ~size_t(255);
+        return p;
+    };
+    MieWork w;
+    w.status = take(kStatusBytes);
+    w.hist = reinterpret_cast<unsigned *>(take(kHistBytes));
+    w.keys = reinterpret_cast<int *>(take((size_t)len * sizeof(int)));
+    w.perm = reinterpret_cast<int *>(take((size_t)len * sizeof(int)));
+    w.partial = reinterpret_cast<Partial *>(take((size_t)blocks * sizeof(Partial)));
+    w.slab = base + off;
+    const size_t slab_bytes = kScratchBytes - off;
+    int64_t nmax = 0;
+    if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
+        return rc;
+    // As many persistent wavefronts as the scratch holds slabs for, up to 8 per CU, in whole
+    // blocks of 4.
+    const size_t per_wave = mie_slab_per_wave(nmax);
+    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (len + 63) / 64);
+    waves = (waves + 3) / 4 * 4;
+    waves = std::min<int64_t>(waves, (int64_t)(slab_bytes / per_wave));
+    waves = waves / 4 * 4;
+    if (waves < 4) {
+        if (total_terms) *total_terms = 0;
+        return fail(ctx, SCATRS_INVALID_INPUT,
+                    "N = " + std::to_string(nmax) +
+                        " terms need more slab memory than the no-allocation scratch holds (" +
+                        std::to_string(kScratchBytes >> 20) + " MiB)");
+    }
+    return mie_solve(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jacobian);
 }
 
 }  // extern "C"
