@@ -61,7 +61,7 @@ __device__ __forceinline__ double step_real(int k, double inv_rho, double u) {
     const double c = inv_rho * (double)(2 * k + 1);
     double den = c - u;
     if (den == 0.0) den = kEps * (fabs(c) + fabs(u));
-    return 1.0 / den;
+    return recip(den);
 }
 
 // u_k(m x) for k = 1..n_top+1 into col[k * stride] (scatrs/src/mie/kernel.rs:159-205).
@@ -115,9 +115,11 @@ struct ZColumn {
     int stride;
 };
 
-struct ZCheckpoint {
+template <int B>
+struct ZCheckpointT {
     static constexpr bool kBlocked = true;
-    const cx *ckpt;     // ckpt[j * stride]: u at index 2 + (j + 1) kZBlock
+    static constexpr int kBlock = B;
+    const cx *ckpt;     // ckpt[j * stride]: u at index 2 + (j + 1) B
     int stride;
     cx *lds;            // the lane's tile: element i at lds[i * 64]
     cx inv_z;           // 1/(mx), complex chain
@@ -128,8 +130,8 @@ struct ZCheckpoint {
     // u_i for i in [2 + j B, 2 + j B + B - 1] into lds[(i - 2 - j B) * 64], regenerated
     // downward from the checkpoint above the block (or from 0 above the start index).
     __device__ __forceinline__ void regen(int j) const {
-        const long long lo = 2 + (long long)j * kZBlock;
-        const long long top = lo + kZBlock - 1;
+        const long long lo = 2 + (long long)j * B;
+        const long long top = lo + B - 1;
         const long long hi = top < start ? top : start;
         if (real_chain) {
             double u = (top + 1 <= start) ? ckpt[(int64_t)j * stride].re : 0.0;
@@ -147,19 +149,22 @@ struct ZCheckpoint {
     }
 };
 
+using ZCheckpoint = ZCheckpointT<kZBlock>;
+
 // The downward pass of u_k(m x) from the start index, keeping only the checkpoints of
-// ZCheckpoint (indices 2 + (j + 1) kZBlock for the blocks that hold orders 1..n).
-__device__ __forceinline__ ZCheckpoint ratio_chain_ckpt(cx m, double x, int n, cx *ckpt,
-                                                       int stride, cx *lds) {
-    ZCheckpoint zs;
+// ZCheckpointT<B> (indices 2 + (j + 1) B for the blocks that hold orders 1..n).
+template <int B = kZBlock>
+__device__ __forceinline__ ZCheckpointT<B> ratio_chain_ckpt(cx m, double x, int n, cx *ckpt,
+                                                           int stride, cx *lds) {
+    ZCheckpointT<B> zs;
     const cx z = m * x;
     zs.ckpt = ckpt;
     zs.stride = stride;
     zs.lds = lds;
     zs.real_chain = m.im == 0.0;
-    const int jmax = (n - 1) / kZBlock;  // block of the last index used, n + 1
-    const long long cmax = 2 + (long long)(jmax + 1) * kZBlock;
-    const long long c0 = 2 + kZBlock;
+    const int jmax = (n - 1) / B;  // block of the last index used, n + 1
+    const long long cmax = 2 + (long long)(jmax + 1) * B;
+    const long long c0 = 2 + B;
     if (zs.real_chain) {
         zs.inv_rho = 1.0 / z.re;
         zs.inv_z = real(0.0);
@@ -167,7 +172,7 @@ __device__ __forceinline__ ZCheckpoint ratio_chain_ckpt(cx m, double x, int n, c
         double u = 0.0;
         for (long long k = zs.start; k >= c0; --k) {
             u = step_real((int)k, zs.inv_rho, u);
-            if (k <= cmax && (k - 2) % kZBlock == 0) ckpt[((k - 2) / kZBlock - 1) * stride] = real(u);
+            if (k <= cmax && (k - 2) % B == 0) ckpt[((k - 2) / B - 1) * stride] = real(u);
         }
     } else {
         zs.inv_z = inv(z);
@@ -176,7 +181,7 @@ __device__ __forceinline__ ZCheckpoint ratio_chain_ckpt(cx m, double x, int n, c
         cx u = real(0.0);
         for (long long k = zs.start; k >= c0; --k) {
             u = step_complex((int)k, zs.inv_z, u);
-            if (k <= cmax && (k - 2) % kZBlock == 0) ckpt[((k - 2) / kZBlock - 1) * stride] = u;
+            if (k <= cmax && (k - 2) % B == 0) ckpt[((k - 2) / B - 1) * stride] = u;
         }
     }
     return zs;
@@ -213,7 +218,7 @@ __device__ __forceinline__ double absorption_term(cx f, cx df, cx l, double n2) 
     return -((df.im - 2.0 * f.im * l.re) * n2);
 }
 
-// Whether a sink wants a call at the end of each block of kZBlock orders (block_end(k0, kend)).
+// Whether a sink wants a call at the end of each block of orders (block_end(k0, kend)).
 template <class S, class = void>
 struct has_block_end : std::false_type {};
 template <class S>
@@ -379,10 +384,12 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
         }
     };
     if constexpr (ZSrc::kBlocked) {
-        for (int j = 0; 1 + j * kZBlock <= n; ++j) {
+        constexpr int B = ZSrc::kBlock;
+        for (int j = 0; 1 + j * B <= n; ++j) {
             zs.regen(j);
-            const int k0 = 1 + j * kZBlock;
-            const int kend = n < k0 + kZBlock - 1 ? n : k0 + kZBlock - 1;
+            const int k0 = 1 + j * B;
+            const int kend = n < k0 + B - 1 ? n : k0 + B - 1;
+#pragma unroll 1
             for (int k = k0; k <= kend; ++k) order_step(k, zs.lds[(k - k0) * 64]);
             // A sink may use the block's tile slots, free once their order is done.
             if constexpr (has_block_end<Sink>::value) sink.block_end(k0, kend);
@@ -420,28 +427,35 @@ struct EfficiencySink {
     cx da_prev[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     cx db_prev[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 
+    // 1/k of the coming order: each order takes one reciprocal, 1/(k + 1), and keeps it for the
+    // next, for its pair and own weights pair(k - 1) = (k - 1)(k + 1)/k and
+    // own(k) = (2k + 1)/(k (k + 1)).
+    double rk = 1.0;
+
     __device__ void order(int k, cx a, cx b) {
         const double w = (double)(2 * k + 1), kf = (double)k;
+        const double rk1 = recip(kf + 1.0);
         sca += w * (norm2(a) + norm2(b));
         const cx t = (a - b) * w;
         back = (k % 2 == 0) ? back + t : back - t;
         if (k > 1) {
             const double km = kf - 1.0;
-            g += km * (km + 2.0) * recip(km + 1.0) *
-                 (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
+            g += (km * (km + 2.0)) * rk * (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
         }
-        g += w * recip(kf * (kf + 1.0)) * re_mul_conj(a, b);
+        g += w * (rk * rk1) * re_mul_conj(a, b);
         a_prev = a;
         b_prev = b;
+        rk = rk1;
     }
 
     __device__ void order_d(int k, cx a, cx b, cx da_dx, cx da_dm, cx db_dx, cx db_dm) {
         const cx dak[3] = {da_dx, da_dm, times_i(da_dm)};
         const cx dbk[3] = {db_dx, db_dm, times_i(db_dm)};
         const double w = (double)(2 * k + 1), kf = (double)k;
-        const double own = w * recip(kf * (kf + 1.0));
+        const double rk1 = recip(kf + 1.0);
+        const double own = w * (rk * rk1);
         const double km = kf - 1.0;
-        const double pair = km * (km + 2.0) * recip(km + 1.0);
+        const double pair = (km * (km + 2.0)) * rk;
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             d_sca[p] += w * 2.0 * (re_mul_conj(dak[p], a) + re_mul_conj(dbk[p], b));

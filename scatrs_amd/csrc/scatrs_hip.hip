@@ -97,17 +97,24 @@ __global__ void __launch_bounds__(256) mie_prep(const double *x, const double *m
     block_partial(n, ok, bad, outside, partial);
 }
 
+// Orders per checkpoint block of the sphere batch (its LDS tile is 64 lanes x this many
+// complex values per wavefront) and its persistent wavefronts per CU: 8 KiB tiles let 16
+// wavefronts (4 per SIMD) share a CU's LDS.
+constexpr int kSphereBlock = 8;
+constexpr int kSphereWavesPerCU = 16;
+
 // The persistent solve: each wavefront takes 64 spheres at a time, largest N first, from a
 // shared counter; each lane solves one sphere. The downward u_k(mx) pass keeps one checkpoint
-// per 16 indices in the wavefront's global slab; the order loop regenerates each block of 16
-// into the wavefront's LDS tile. u_k(x) above floor(x) (a few dozen orders) stays in the slab.
+// per kSphereBlock indices in the wavefront's global slab; the order loop regenerates each
+// block into the wavefront's LDS tile. u_k(x) above floor(x) (a few dozen orders) stays in the
+// slab.
 template <bool DERIVS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mie_batch(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DERIVS ? 2 : 4))) void mie_batch(
     const double *__restrict__ x, const double *__restrict__ m, int64_t len, int64_t n_fixed,
     const int *__restrict__ perm, unsigned long long *chunk_counter, cx *ck_base,
     double *ux_base, int64_t rows_ck, int64_t rows, double *__restrict__ values,
     double *__restrict__ jac) {
-    __shared__ cx ztile[4][kZBlock * 64];
+    __shared__ cx ztile[4][kSphereBlock * 64];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     cx *ck = ck_base + wave * rows_ck * 64 + lane;
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             const double xi = x[i];
             const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
             const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
-            const ZCheckpoint zs = ratio_chain_ckpt(mi, xi, n, ck, 64, tile);
+            const auto zs = ratio_chain_ckpt<kSphereBlock>(mi, xi, n, ck, 64, tile);
             x_chain(xi, n, upward_orders(xi, n), ux, 64);
             EfficiencySink<DERIVS> s;
             NoOuter none;
@@ -162,7 +169,7 @@ struct MieWork {
 // Bytes of one wavefront's slab for spheres of up to nmax terms.
 size_t mie_slab_per_wave(int64_t nmax) {
     const int64_t rows = nmax + 2;
-    const int64_t rows_ck = nmax / kZBlock + 2;
+    const int64_t rows_ck = nmax / kSphereBlock + 2;
     return (size_t)64 * (rows_ck * sizeof(cx) + rows * sizeof(double));
 }
 
@@ -228,7 +235,7 @@ int mie_solve(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double
     hipStream_t s = ctx->stream;
     sort_by_keys(w.keys, w.hist, w.perm, len, s);
     const int64_t rows = nmax + 2;
-    const int64_t rows_ck = nmax / kZBlock + 2;
+    const int64_t rows_ck = nmax / kSphereBlock + 2;
     unsigned long long *chunk_counter = reinterpret_cast<unsigned long long *>(
         static_cast<char *>(w.status) + sizeof(Status));
     cx *ck = static_cast<cx *>(w.slab);
@@ -469,10 +476,10 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
     int64_t nmax = 0;
     if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
         return rc;
-    // Persistent wavefronts: 8 per CU (two 4-wave blocks share the CU's LDS tiles), fewer if
-    // the slabs would pass 8 GiB.
+    // Persistent wavefronts: kSphereWavesPerCU per CU (4-wave blocks share the CU's LDS
+    // tiles), fewer if the slabs would pass 8 GiB.
     const size_t per_wave = mie_slab_per_wave(nmax);
-    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (len + 63) / 64);
+    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * kSphereWavesPerCU, (len + 63) / 64);
     waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((8ull << 30) / per_wave)));
     waves = (waves + 3) / 4 * 4;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves), "scratch")))
@@ -516,10 +523,10 @@ int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const do
     int64_t nmax = 0;
     if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
         return rc;
-    // As many persistent wavefronts as the scratch holds slabs for, up to 8 per CU, in whole
-    // blocks of 4.
+    // As many persistent wavefronts as the scratch holds slabs for, up to kSphereWavesPerCU
+    // per CU, in whole blocks of 4.
     const size_t per_wave = mie_slab_per_wave(nmax);
-    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (len + 63) / 64);
+    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * kSphereWavesPerCU, (len + 63) / 64);
     waves = (waves + 3) / 4 * 4;
     waves = std::min<int64_t>(waves, (int64_t)(slab_bytes / per_wave));
     waves = waves / 4 * 4;
