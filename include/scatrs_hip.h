@@ -107,6 +107,59 @@ int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const do
                                     int64_t len, int64_t n_max, int allow_unvalidated,
                                     double *d_values, double *d_jacobian, int64_t *total_terms);
 
+/* The number of terms of each sphere without solving (host outputs, may be NULL): the largest
+ * N, for sizing coefficient outputs, and the sum of N. The checks of Mie::compute; the
+ * reference's mie::n_terms (scatrs/src/mie/mod.rs:126-145), as the sizing pass of
+ * _scatrs.mie_coefficients uses it (scatrs-py/src/mie.rs:386-396). Synchronizes. */
+int scatrs_mie_terms(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+                     int64_t n_max, int allow_unvalidated, int64_t *max_terms,
+                     int64_t *total_terms);
+
+/* The Mie coefficients of `len` spheres: d_terms (len) each sphere's N; d_a, d_b (len, width)
+ * complex128 a_n, b_n at [i, n - 1], zero beyond each sphere's N (width >= the largest N, from
+ * scatrs_mie_terms, else SCATRS_INVALID_INPUT); with d_derivatives != NULL, (4, len, width)
+ * complex128 [da/dx, da/dm, db/dx, db/dm] (the complex derivative d/dm; d/d(Im m) = i d/dm).
+ * Replaces _scatrs.mie_coefficients (scatrs-py/src/mie.rs:376-442) over Mie::compute /
+ * compute_with_derivatives and MieSolution::a, b, derivatives (scatrs/src/mie/mod.rs:226-251,
+ * 356-380). Synchronizes once (validation). */
+int scatrs_mie_coefficients(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+                            int64_t n_max, int allow_unvalidated, int64_t width,
+                            int64_t *d_terms, double *d_a, double *d_b, double *d_derivatives,
+                            int64_t *total_terms);
+
+/* Layered spheres: `count` particles of `layers` layers (1 <= layers <= 8), d_x (count, layers)
+ * the interface size parameters core first (the outer one last, nondecreasing), d_m
+ * (count, layers) complex128 the layers' indices, row-major like the PyO3 arrays
+ * (scatrs-py/src/layered.rs:1-7). */
+
+/* The largest and the total number of terms (host outputs, may be NULL) of layered spheres,
+ * after the checks of Layered::compute (derivatives != 0: of compute_with_derivatives, whose
+ * validated domain ends at x_L = 1e3); the reference's layered::n_terms
+ * (scatrs/src/layered/mod.rs:217-226). Synchronizes. */
+int scatrs_layered_terms(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t count,
+                         int layers, int64_t n_max, int allow_unvalidated, int derivatives,
+                         int64_t *max_terms, int64_t *total_terms);
+
+/* Efficiencies [Q_ext, Q_sca, Q_abs, Q_back, g] (5, count) of layered spheres per unit outer
+ * cross section, and with d_jacobian != NULL their derivatives (layers, 3, 5, count) with
+ * respect to each layer's x_l, Re m_l, Im m_l. Replaces _scatrs.layered_efficiencies
+ * (scatrs-py/src/layered.rs:123-193) over Layered::compute / compute_with_derivatives,
+ * efficiencies and efficiencies_jacobian (scatrs/src/layered/mod.rs:137-214,412-440).
+ * Synchronizes once (validation). */
+int scatrs_layered_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m,
+                                int64_t count, int layers, int64_t n_max, int allow_unvalidated,
+                                double *d_values, double *d_jacobian, int64_t *total_terms);
+
+/* The Mie coefficients of layered spheres, as scatrs_mie_coefficients: d_terms (count), d_a,
+ * d_b (count, width), and with d_derivatives != NULL (layers, 4, count, width)
+ * [da/dx_l, da/dm_l, db/dx_l, db/dm_l] per layer. Replaces _scatrs.layered_coefficients
+ * (scatrs-py/src/layered.rs:195-267) over LayeredSolution::a, b, derivatives
+ * (scatrs/src/layered/mod.rs:341-380). Synchronizes once (validation). */
+int scatrs_layered_coefficients(scatrs_ctx *ctx, const double *d_x, const double *d_m,
+                                int64_t count, int layers, int64_t n_max, int allow_unvalidated,
+                                int64_t width, int64_t *d_terms, double *d_a, double *d_b,
+                                double *d_derivatives, int64_t *total_terms);
+
 /* Size distributions (scatrs/src/ensemble/distribution.rs:15-68) and their parameters. */
 #define SCATRS_LOGNORMAL 0      /* p0 = median radius r_g, p1 = width sigma_g */
 #define SCATRS_GAMMA 1          /* p0 = effective radius, p1 = effective variance */

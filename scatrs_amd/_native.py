@@ -38,6 +38,11 @@ SYMBOLS = (
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
     "scatrs_mie_efficiencies_noalloc",
+    "scatrs_mie_terms",
+    "scatrs_mie_coefficients",
+    "scatrs_layered_terms",
+    "scatrs_layered_efficiencies",
+    "scatrs_layered_coefficients",
     "scatrs_ensemble_fixed",
     "scatrs_ensemble_fixed_vjp",
     "scatrs_ensemble_nodes",
@@ -81,6 +86,15 @@ def load() -> C.CDLL:
                                               C.POINTER(i64)]
         L.scatrs_mie_efficiencies_noalloc.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
                                                       C.POINTER(i64)]
+        L.scatrs_mie_terms.argtypes = [vp, vp, vp, i64, i64, i32, C.POINTER(i64), C.POINTER(i64)]
+        L.scatrs_mie_coefficients.argtypes = [vp, vp, vp, i64, i64, i32, i64, vp, vp, vp, vp,
+                                              C.POINTER(i64)]
+        L.scatrs_layered_terms.argtypes = [vp, vp, vp, i64, i32, i64, i32, i32, C.POINTER(i64),
+                                           C.POINTER(i64)]
+        L.scatrs_layered_efficiencies.argtypes = [vp, vp, vp, i64, i32, i64, i32, vp, vp,
+                                                  C.POINTER(i64)]
+        L.scatrs_layered_coefficients.argtypes = [vp, vp, vp, i64, i32, i64, i32, i64, vp, vp, vp,
+                                                  vp, C.POINTER(i64)]
         if hasattr(L, "scatrs_ensemble_fixed"):
             L.scatrs_ensemble_fixed.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp, i64,
                                                 i64, i64, i32, i32, vp, vp, vp,

@@ -14,7 +14,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libscatrs_hip.so"
-SOURCES = ["scatrs_hip.hip", "ensemble.hip", "ensemble_vjp.hip"]
+SOURCES = ["scatrs_hip.hip", "ensemble.hip", "ensemble_vjp.hip", "single.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",

@@ -299,20 +299,21 @@ struct LayeredOuter {
     __device__ OuterDerivs derivs(int) { return od; }
 };
 
-// Solves one layered sphere of outer size parameter x (x_l = fr[l] x, indices mm interleaved
-// re, im per layer) with n orders, feeding sink as mie_orders does. The columns of the lane's
-// slab are filled first.
+// Solves one layered sphere of interface size parameters xl[0..L-1] (the outer one last;
+// indices mm interleaved re, im per layer) with n orders, feeding sink as mie_orders does. The
+// columns of the lane's slab are filled first.
 template <int L, bool DERIVS, class Sink>
-__device__ inline void layered_solve(double x, const double *fr, const double *mm, int n,
-                                     cx *slab, int64_t rows, Sink &sink,
-                                     LayeredOuter<L, DERIVS> *keep = nullptr) {
+__device__ inline void layered_solve_xl(const double *xl, const double *mm, int n, cx *slab,
+                                        int64_t rows, Sink &sink,
+                                        LayeredOuter<L, DERIVS> *keep = nullptr) {
     const int64_t cs = rows * 64;
     LayeredOuter<L, DERIVS> o;
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-        o.xl[l] = l < L - 1 ? fr[l] * x : x;
+        o.xl[l] = xl[l];
         o.m[l] = mk(mm[2 * l], mm[2 * l + 1]);
     }
+    const double x = o.xl[L - 1];
     o.slab = slab;
     o.cs = cs;
     double *ux = real_column(slab, (2 * L - 1) * cs);
@@ -332,6 +333,18 @@ __device__ inline void layered_solve(double x, const double *fr, const double *m
     for (int l = 0; l < L; ++l) lossless = lossless && o.m[l].im == 0.0;
     if (lossless) sink.zero_absorption();
     if (keep) *keep = o;
+}
+
+// Solves one layered sphere of outer size parameter x (x_l = fr[l] x, indices mm interleaved
+// re, im per layer) with n orders, as layered_solve_xl.
+template <int L, bool DERIVS, class Sink>
+__device__ inline void layered_solve(double x, const double *fr, const double *mm, int n,
+                                     cx *slab, int64_t rows, Sink &sink,
+                                     LayeredOuter<L, DERIVS> *keep = nullptr) {
+    double xl[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) xl[l] = l < L - 1 ? fr[l] * x : x;
+    layered_solve_xl<L, DERIVS>(xl, mm, n, slab, rows, sink, keep);
 }
 
 }  // namespace scatrs

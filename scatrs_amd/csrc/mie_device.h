@@ -413,49 +413,19 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
     sink.done(abs_sum, jac0, jac1, jac2);
 }
 
-// The efficiency series (scatrs/src/mie/efficiencies.rs:64-187) accumulated order by order:
-// Q_sca = (2/x^2) sca, Q_back = |back|^2 / x^2, g = 2 G / sca, and with derivatives the sums
-// of efficiencies.rs:89-146 for d/dx, d/dRe m, d/dIm m.
-template <bool DERIVS>
-struct EfficiencySink {
-    double sca = 0.0, g = 0.0, abs_sum = 0.0;
-    cx back = {0.0, 0.0};
-    cx a_prev = {0.0, 0.0}, b_prev = {0.0, 0.0};
-    double jac_abs[3] = {0.0, 0.0, 0.0};
+// The derivative sums of the efficiency series along one particle parameter group (x, Re m,
+// Im m of one layer; efficiencies.rs:89-146): order k adds its coefficients' derivatives
+// dak[p], dbk[p] with w = 2k + 1, own = (2k + 1)/(k (k + 1)), pair = (k - 1)(k + 1)/k and the
+// previous order's coefficients a_prev, b_prev.
+struct DerivSums {
     double d_sca[3] = {0.0, 0.0, 0.0}, d_g[3] = {0.0, 0.0, 0.0};
+    double jac_abs[3] = {0.0, 0.0, 0.0};
     cx d_back[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     cx da_prev[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     cx db_prev[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 
-    // 1/k of the coming order: each order takes one reciprocal, 1/(k + 1), and keeps it for the
-    // next, for its pair and own weights pair(k - 1) = (k - 1)(k + 1)/k and
-    // own(k) = (2k + 1)/(k (k + 1)).
-    double rk = 1.0;
-
-    __device__ void order(int k, cx a, cx b) {
-        const double w = (double)(2 * k + 1), kf = (double)k;
-        const double rk1 = recip(kf + 1.0);
-        sca += w * (norm2(a) + norm2(b));
-        const cx t = (a - b) * w;
-        back = (k % 2 == 0) ? back + t : back - t;
-        if (k > 1) {
-            const double km = kf - 1.0;
-            g += (km * (km + 2.0)) * rk * (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
-        }
-        g += w * (rk * rk1) * re_mul_conj(a, b);
-        a_prev = a;
-        b_prev = b;
-        rk = rk1;
-    }
-
-    __device__ void order_d(int k, cx a, cx b, cx da_dx, cx da_dm, cx db_dx, cx db_dm) {
-        const cx dak[3] = {da_dx, da_dm, times_i(da_dm)};
-        const cx dbk[3] = {db_dx, db_dm, times_i(db_dm)};
-        const double w = (double)(2 * k + 1), kf = (double)k;
-        const double rk1 = recip(kf + 1.0);
-        const double own = w * (rk * rk1);
-        const double km = kf - 1.0;
-        const double pair = (km * (km + 2.0)) * rk;
+    __device__ void add(int k, double w, double own, double pair, cx a, cx b, cx a_prev,
+                        cx b_prev, const cx dak[3], const cx dbk[3]) {
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
             d_sca[p] += w * 2.0 * (re_mul_conj(dak[p], a) + re_mul_conj(dbk[p], b));
@@ -469,6 +439,74 @@ struct EfficiencySink {
             da_prev[p] = dak[p];
             db_prev[p] = dbk[p];
         }
+    }
+
+    // The Jacobian [d/dx, d/dRe m, d/dIm m][Q_ext..g] (efficiencies.rs:192-230) from the values
+    // q and the value sums sca, back; `outer` adds the 1/x^2 prefactor's term to d/dx (the
+    // outer layer's size parameter sets the efficiencies' normalization).
+    __device__ void jacobian(double x, const double q[5], double sca, cx back, bool outer,
+                             double out[15]) const {
+#pragma clang fp contract(off)
+        const double inv_x = 1.0 / x, inv_x2 = inv_x * inv_x, two_x2 = 2.0 * inv_x2;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const double xt = (p == 0 && outer) ? 2.0 * inv_x : 0.0;
+            const double q_sca = two_x2 * d_sca[p] - xt * q[1];
+            const double q_abs = two_x2 * jac_abs[p] - xt * q[2];
+            const double q_back = 2.0 * re_mul_conj(d_back[p], back) * inv_x2 - xt * q[3];
+            const double gg = sca == 0.0 ? 0.0 : (2.0 * d_g[p] - q[4] * d_sca[p]) / sca;
+            out[p * 5 + 0] = q_sca + q_abs;
+            out[p * 5 + 1] = q_sca;
+            out[p * 5 + 2] = q_abs;
+            out[p * 5 + 3] = q_back;
+            out[p * 5 + 4] = gg;
+        }
+    }
+};
+
+// The efficiency series (scatrs/src/mie/efficiencies.rs:64-187) accumulated order by order:
+// Q_sca = (2/x^2) sca, Q_back = |back|^2 / x^2, g = 2 G / sca, and with derivatives the sums
+// of efficiencies.rs:89-146 for d/dx, d/dRe m, d/dIm m.
+template <bool DERIVS>
+struct EfficiencySink {
+    double sca = 0.0, g = 0.0, abs_sum = 0.0;
+    cx back = {0.0, 0.0};
+    cx a_prev = {0.0, 0.0}, b_prev = {0.0, 0.0};
+    DerivSums ds;
+    // 1/k of the coming order: each order takes one reciprocal, 1/(k + 1), and keeps it for the
+    // next, for its pair and own weights pair(k - 1) = (k - 1)(k + 1)/k and
+    // own(k) = (2k + 1)/(k (k + 1)).
+    double rk = 1.0;
+
+    // The weights of order k: own(k) and pair(k - 1), and 1/(k + 1).
+    __device__ void weights(int k, double *own, double *pair, double *rk1) const {
+        const double w = (double)(2 * k + 1), kf = (double)k;
+        *rk1 = recip(kf + 1.0);
+        *own = w * (rk * *rk1);
+        const double km = kf - 1.0;
+        *pair = (km * (km + 2.0)) * rk;
+    }
+
+    __device__ void order(int k, cx a, cx b) {
+        const double w = (double)(2 * k + 1);
+        double own, pair, rk1;
+        weights(k, &own, &pair, &rk1);
+        sca += w * (norm2(a) + norm2(b));
+        const cx t = (a - b) * w;
+        back = (k % 2 == 0) ? back + t : back - t;
+        if (k > 1) g += pair * (re_mul_conj(a_prev, a) + re_mul_conj(b_prev, b));
+        g += own * re_mul_conj(a, b);
+        a_prev = a;
+        b_prev = b;
+        rk = rk1;
+    }
+
+    __device__ void order_d(int k, cx a, cx b, cx da_dx, cx da_dm, cx db_dx, cx db_dm) {
+        const cx dak[3] = {da_dx, da_dm, times_i(da_dm)};
+        const cx dbk[3] = {db_dx, db_dm, times_i(db_dm)};
+        double own, pair, rk1;
+        weights(k, &own, &pair, &rk1);
+        ds.add(k, (double)(2 * k + 1), own, pair, a, b, a_prev, b_prev, dak, dbk);
         order(k, a, b);
     }
 
@@ -479,15 +517,15 @@ struct EfficiencySink {
     // (layered/kernel.rs:453-464).
     __device__ void zero_absorption() {
         abs_sum = 0.0;
-        jac_abs[0] = 0.0;
-        jac_abs[1] = 0.0;
+        ds.jac_abs[0] = 0.0;
+        ds.jac_abs[1] = 0.0;
     }
 
     __device__ void done(double abs_s, double j0, double j1, double j2) {
         abs_sum = abs_s;
-        jac_abs[0] = j0;
-        jac_abs[1] = j1;
-        jac_abs[2] = j2;
+        ds.jac_abs[0] = j0;
+        ds.jac_abs[1] = j1;
+        ds.jac_abs[2] = j2;
     }
 
     // [Q_ext, Q_sca, Q_abs, Q_back, g] (efficiencies.rs:169-187).
@@ -505,21 +543,7 @@ struct EfficiencySink {
 
     // The Jacobian [d/dx, d/dRe m, d/dIm m][Q_ext..g] (efficiencies.rs:192-230).
     __device__ void jacobian(double x, const double q[5], bool outer, double out[15]) const {
-#pragma clang fp contract(off)
-        const double inv_x = 1.0 / x, inv_x2 = inv_x * inv_x, two_x2 = 2.0 * inv_x2;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) {
-            const double xt = (p == 0 && outer) ? 2.0 * inv_x : 0.0;
-            const double q_sca = two_x2 * d_sca[p] - xt * q[1];
-            const double q_abs = two_x2 * jac_abs[p] - xt * q[2];
-            const double q_back = 2.0 * re_mul_conj(d_back[p], back) * inv_x2 - xt * q[3];
-            const double gg = sca == 0.0 ? 0.0 : (2.0 * d_g[p] - q[4] * d_sca[p]) / sca;
-            out[p * 5 + 0] = q_sca + q_abs;
-            out[p * 5 + 1] = q_sca;
-            out[p * 5 + 2] = q_abs;
-            out[p * 5 + 3] = q_back;
-            out[p * 5 + 4] = gg;
-        }
+        ds.jacobian(x, q, sca, back, outer, out);
     }
 };
 

@@ -1,8 +1,11 @@
-"""Batched homogeneous spheres on the GPU: the efficiencies and their Jacobian.
+"""Batched homogeneous spheres on the GPU: the efficiencies and their Jacobian, and the Mie
+coefficients a_n, b_n and their derivatives.
 
-Mirrors ``scatrs.mie.efficiencies`` (python/scatrs/mie.py:390-416) and the wire layout of
-``_scatrs.mie_efficiencies`` (scatrs-py/src/mie.rs:303-364): values (5, len) in the order
-Q_ext, Q_sca, Q_abs, Q_back, g and the Jacobian (3, 5, len) with respect to x, Re m, Im m.
+Mirrors ``scatrs.mie.efficiencies`` and ``scatrs.mie.coefficients`` (python/scatrs/mie.py:
+390-446) and the wire layouts of ``_scatrs.mie_efficiencies`` and ``_scatrs.mie_coefficients``
+(scatrs-py/src/mie.rs:303-442): values (5, len) in the order Q_ext, Q_sca, Q_abs, Q_back, g and
+the Jacobian (3, 5, len) with respect to x, Re m, Im m; coefficients (len, N) zero-padded to
+the largest N, derivatives (4, len, N) [da/dx, da/dm, db/dx, db/dm].
 """
 
 from __future__ import annotations
@@ -15,7 +18,8 @@ import numpy as np
 
 from . import _native
 
-__all__ = ["MieEfficiencies", "default_n_max", "efficiencies", "efficiencies_device"]
+__all__ = ["MieCoefficients", "MieEfficiencies", "coefficients", "coefficients_device",
+           "default_n_max", "efficiencies", "efficiencies_device"]
 
 _PARAMS = ("x", "m_real", "m_imag")
 
@@ -45,6 +49,31 @@ class MieEfficiencies:
             raise ValueError(f"no derivative with respect to {param!r}; computed: {self.params}")
         rows = self._jacobian[_PARAMS.index(param)]
         return MieEfficiencies(*rows)
+
+
+@dataclass(frozen=True, eq=False)
+class MieCoefficients:
+    """a_n, b_n of spheres, arrays of shape S + (N,) with N the largest number of terms
+    (``a[..., n - 1]`` is a_n, zero beyond each sphere's ``n_terms``); ``d`` gives their
+    derivatives (python/scatrs/mie.py:173-212)."""
+
+    a: np.ndarray
+    b: np.ndarray
+    n_terms: np.ndarray
+    params: tuple[str, ...] = ()
+    _derivatives: np.ndarray | None = field(default=None, repr=False)
+
+    def d(self, param: str) -> "MieCoefficients":
+        """The derivatives with respect to ``param``: "x", "m_real", "m_imag", or "m" for the
+        complex derivative d/dm (equal to d/d(Re m); d/d(Im m) is i d/dm)."""
+        if param not in self.params and not (param == "m" and self.params):
+            raise ValueError(f"no derivative with respect to {param!r}; computed: {self.params}")
+        da_dx, da_dm, db_dx, db_dm = self._derivatives
+        if param == "x":
+            return MieCoefficients(da_dx, db_dx, self.n_terms)
+        if param == "m_imag":
+            return MieCoefficients(1j * da_dm, 1j * db_dm, self.n_terms)
+        return MieCoefficients(da_dm, db_dm, self.n_terms)
 
 
 def _parameters(derivatives: bool | Sequence[str]) -> tuple[str, ...]:
@@ -103,3 +132,67 @@ def efficiencies(x, m, *, n_max: int | None = None, derivatives: bool | Sequence
     fields = [v[k].reshape(shape) for k in range(5)]
     j = None if jac is None else jac.cpu().numpy().reshape((3, 5, *shape))
     return MieEfficiencies(*fields, params=params, _jacobian=j)
+
+
+def _device_pair(x, m):
+    import torch
+
+    if x.dtype != torch.float64 or m.dtype != torch.complex128:
+        raise TypeError("x must be float64 and m complex128")
+    if x.device.type != "cuda" or m.device != x.device:
+        raise ValueError("x and m must be on the same GPU")
+    return x.contiguous(), m.contiguous()
+
+
+def coefficients_device(x, m, *, n_max: int | None = None, derivatives: bool = False,
+                        allow_unvalidated: bool = False):
+    """Device-resident form: ``x`` float64 (len,), ``m`` complex128 (len,) CUDA tensors; returns
+    (n_terms int64 (len,), a, b complex128 (len, N), derivatives complex128 (4, len, N) or
+    None), N the largest number of terms (scatrs_mie_terms, then scatrs_mie_coefficients)."""
+    import torch
+
+    x, m = _device_pair(x, m)
+    x, m = x.reshape(-1), m.reshape(-1)
+    if m.numel() != x.numel():
+        raise ValueError("x and m must have the same number of elements")
+    n = x.numel()
+    ctx = _native.context(x.device.index)
+    nm = 0 if n_max is None else int(n_max)
+    width, total = C.c_int64(0), C.c_int64(0)
+    ctx.check(ctx.lib.scatrs_mie_terms(ctx.handle, C.c_void_p(x.data_ptr()),
+                                       C.c_void_p(m.data_ptr()), n, nm, int(bool(allow_unvalidated)),
+                                       C.byref(width), C.byref(total)))
+    w = max(width.value, 1)
+    dev = x.device
+    terms = torch.empty(n, dtype=torch.int64, device=dev)
+    a = torch.empty((n, w), dtype=torch.complex128, device=dev)
+    b = torch.empty((n, w), dtype=torch.complex128, device=dev)
+    d = torch.empty((4, n, w), dtype=torch.complex128, device=dev) if derivatives else None
+    ctx.check(ctx.lib.scatrs_mie_coefficients(
+        ctx.handle, C.c_void_p(x.data_ptr()), C.c_void_p(m.data_ptr()), n, nm,
+        int(bool(allow_unvalidated)), w, C.c_void_p(terms.data_ptr()), C.c_void_p(a.data_ptr()),
+        C.c_void_p(b.data_ptr()), C.c_void_p(d.data_ptr()) if d is not None else None,
+        C.byref(total)))
+    return terms, a, b, d
+
+
+def coefficients(x, m, *, n_max: int | None = None, derivatives: bool | Sequence[str] = False,
+                 allow_unvalidated: bool = False) -> MieCoefficients:
+    """Mie coefficients a_n and b_n of spheres (host arrays in, host arrays out), shape
+    S + (N,) zero-padded to the largest number of terms N (python/scatrs/mie.py:418-446)."""
+    import torch
+
+    params = _parameters(derivatives)
+    xb, mb = np.broadcast_arrays(np.asarray(x, dtype=np.float64),
+                                 np.asarray(m, dtype=np.complex128))
+    shape = xb.shape
+    dev = torch.device("cuda", torch.cuda.current_device())
+    xt = torch.from_numpy(np.ascontiguousarray(xb).reshape(-1)).to(dev)
+    mt = torch.from_numpy(np.ascontiguousarray(mb).reshape(-1)).to(dev)
+    terms, a, b, d = coefficients_device(xt, mt, n_max=n_max, derivatives=bool(params),
+                                         allow_unvalidated=allow_unvalidated)
+    w = a.shape[1]
+    dd = None if d is None else d.cpu().numpy().reshape((4, *shape, w))
+    return MieCoefficients(a.cpu().numpy().reshape((*shape, w)),
+                           b.cpu().numpy().reshape((*shape, w)),
+                           terms.cpu().numpy().reshape(shape), params, dd)

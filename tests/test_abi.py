@@ -47,3 +47,15 @@ def test_context_rejects_a_missing_device():
     h = C.c_void_p()
     assert lib.scatrs_create(1 << 20, None, C.byref(h)) != 0
     assert not h.value
+
+
+def test_rust_binding_declares_every_symbol():
+    """rust/scatrs-hip-sys/src/lib.rs declares exactly the symbols of include/scatrs_hip.h,
+    and maps every status (3 -> NotConverged) in its check."""
+    text = (ROOT / "rust" / "scatrs-hip-sys" / "src" / "lib.rs").read_text()
+    rust = set(re.findall(r"pub fn (scatrs_\w+)\(", text))
+    assert rust == set(_declared())
+    for code, variant in (("SCATRS_INVALID_INPUT", "InvalidInput"),
+                          ("SCATRS_OUTSIDE_VALIDATED_DOMAIN", "OutsideValidatedDomain"),
+                          ("SCATRS_NOT_CONVERGED", "NotConverged")):
+        assert re.search(code + r" => Err\(Error::" + variant + r"\(", text), code

@@ -63,3 +63,54 @@ def efficiency_conditions(q: np.ndarray, jac: np.ndarray, x: float, m: complex) 
         / sizes[k]
         for k in range(5)
     ]
+
+
+# ---- layered spheres (scatrs/src/layered/domain.rs:96-160; tests/common/layered.rs:132-215) ----
+
+LAYERED_CONSTANT = {
+    "coefficients": 32.0,
+    "q_ext": 32.0,
+    "q_sca": 16.0,
+    "q_abs": 256.0,
+    "q_back": 64.0,
+    "g": 16.0,
+    "amplitudes": 128.0,
+    "coefficient_derivatives": 1024.0,
+    "jacobian": 256.0,
+}
+
+
+def layered_kappa(x_outer: float, m) -> float:
+    """(1 + max_l max(1, |m_l|) x_L) max_l max(1, |m_l|/|m_l - 1|) over the layers whose index
+    is not exactly 1 (infinite when every layer has m = 1)."""
+    ms = np.atleast_1d(np.asarray(m, np.complex128))
+    big = max(1.0, float(np.max(np.abs(ms))))
+    ones = [max(abs(v) / abs(v - 1.0), 1.0) for v in ms if v != 1.0]
+    near = max(ones) if ones else float("inf")
+    return (1.0 + big * x_outer) * near
+
+
+def layered_tolerance(kind: str, x_outer: float, m, cond: float) -> float:
+    """C (kappa' + cond) 2^-52 with the layered kappa (layered/domain.rs:115-132)."""
+    k = layered_kappa(x_outer, m)
+    ms = np.atleast_1d(np.asarray(m, np.complex128))
+    big = max(1.0, float(np.max(np.abs(ms))))
+    near = max(k / (1.0 + big * x_outer), 1.0)
+    prior = k * near if kind == "jacobian" else k
+    return LAYERED_CONSTANT[kind] * (prior + cond) * EPS
+
+
+def layered_in_domain(x, m, derivatives: bool = False) -> bool:
+    """The reference's validated domain of layered spheres (layered/domain.rs:53-95); one
+    layer is a sphere, with the sphere's domain (mie/domain.rs:123-140)."""
+    x = np.atleast_1d(np.asarray(x, np.float64))
+    ms = np.atleast_1d(np.asarray(m, np.complex128))
+    if x.size == 1:
+        v = ms[0]
+        return (1e-6 <= x[0] <= 2e4 and 0.0 < v.real <= 10.0 and 0.0 <= v.imag <= 10.0
+                and abs(v) >= 0.1)
+    outer = x[-1]
+    ok = (x.size <= 16 and 1e-6 <= outer <= 1e4
+          and all(abs(v) * outer <= 3e4 and 0.0 < v.real <= 10.0 and 0.0 <= v.imag <= 10.0
+                  and abs(v) >= 0.1 for v in ms))
+    return ok and (not derivatives or outer <= 1e3)
