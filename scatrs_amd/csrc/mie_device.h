@@ -64,6 +64,32 @@ __device__ __forceinline__ double step_real(int k, double inv_rho, double u) {
     return recip(den);
 }
 
+// The plain-path check of a block of complex reciprocals: the smallest and largest squared
+// modulus and the smallest |Im| of the arguments. inv takes its one-division form exactly when
+// 1e-300 < |z|^2 < 1e300 and Im z != 0 (and step_complex keeps its denominator when it is not
+// 0), so a block whose arguments all pass computes, without any branch, what inv would.
+struct PlainCheck {
+    double lo = 1e308, hi = 0.0, im = 1e308;
+    __device__ __forceinline__ bool ok() const { return lo > 1e-300 && hi < 1e300 && im != 0.0; }
+};
+
+// 1/z by the one-division form of inv, its argument noted in pc.
+__device__ __forceinline__ cx inv_plain(cx z, PlainCheck &pc) {
+    const double c = z.re, d = z.im;
+    const double s = c * c + d * d;
+    pc.lo = fmin(pc.lo, s);
+    pc.hi = fmax(pc.hi, s);
+    pc.im = fmin(pc.im, fabs(d));
+    const double t = recip(s);
+    return cx{c * t, -(d * t)};
+}
+
+// step_complex on the plain path (its result is step_complex's when pc.ok() after the block).
+__device__ __forceinline__ cx step_complex_plain(int k, cx inv_z, cx u, PlainCheck &pc) {
+    const cx c = inv_z * (double)(2 * k + 1);
+    return inv_plain(c - u, pc);
+}
+
 // u_k(m x) for k = 1..n_top+1 into col[k * stride] (scatrs/src/mie/kernel.rs:159-205).
 __device__ __forceinline__ void ratio_chain(cx m, double x, int n_top, cx *col, int stride) {
     const cx z = m * x;
@@ -108,6 +134,9 @@ __device__ __forceinline__ void x_chain(double x, int n, int n_s, double *col, i
 //    into the wavefront's LDS tile by the same downward steps before using it, so the full
 //    column never goes through HBM.
 constexpr int kZBlock = 16;
+// Orders per checkpoint block of the single-particle batches (the sphere batch and the layered
+// entry points' one-layer case share it, so that their results are the same bit for bit).
+constexpr int kSphereBlock = 8;
 
 struct ZColumn {
     static constexpr bool kBlocked = false;
@@ -139,8 +168,26 @@ struct ZCheckpointT {
                 u = step_real((int)i, inv_rho, u);
                 lds[(i - lo) * 64] = real(u);
             }
+        } else if (top + 1 <= start) {
+            // A full block below the start: B steps without branches, redone on the checked
+            // path in the rare lane whose arguments left the plain range.
+            const cx u0 = ckpt[(int64_t)j * stride];
+            cx u = u0;
+            PlainCheck pc;
+#pragma unroll
+            for (int q = B - 1; q >= 0; --q) {
+                u = step_complex_plain((int)lo + q, inv_z, u, pc);
+                lds[q * 64] = u;
+            }
+            if (!pc.ok()) {
+                u = u0;
+                for (int q = B - 1; q >= 0; --q) {
+                    u = step_complex((int)lo + q, inv_z, u);
+                    lds[q * 64] = u;
+                }
+            }
         } else {
-            cx u = (top + 1 <= start) ? ckpt[(int64_t)j * stride] : real(0.0);
+            cx u = real(0.0);
             for (long long i = hi; i >= lo; --i) {
                 u = step_complex((int)i, inv_z, u);
                 lds[(i - lo) * 64] = u;
@@ -179,9 +226,32 @@ __device__ __forceinline__ ZCheckpointT<B> ratio_chain_ckpt(cx m, double x, int 
         zs.inv_rho = 0.0;
         zs.start = start_index(n + 1, cabs(z));
         cx u = real(0.0);
-        for (long long k = zs.start; k >= c0; --k) {
+        long long k = zs.start;
+        // Down to the first checkpoint index (k = 2 mod B), then whole blocks k .. k - B + 1
+        // whose first step gives a checkpoint, without branches, each redone on the checked
+        // path in the rare lane whose arguments left the plain range.
+        for (; k >= c0 && (k - 2) % B != 0; --k) u = step_complex((int)k, zs.inv_z, u);
+        for (; k > c0; k -= B) {
+            const cx u0 = u;
+            PlainCheck pc;
+            cx first;
+#pragma unroll
+            for (int q = 0; q < B; ++q) {
+                u = step_complex_plain((int)(k - q), zs.inv_z, u, pc);
+                if (q == 0) first = u;
+            }
+            if (!pc.ok()) {
+                u = u0;
+                for (int q = 0; q < B; ++q) {
+                    u = step_complex((int)(k - q), zs.inv_z, u);
+                    if (q == 0) first = u;
+                }
+            }
+            if (k <= cmax) ckpt[((k - 2) / B - 1) * stride] = first;
+        }
+        if (k == c0) {
             u = step_complex((int)k, zs.inv_z, u);
-            if (k <= cmax && (k - 2) % B == 0) ckpt[((k - 2) / B - 1) * stride] = u;
+            ckpt[0] = u;
         }
     }
     return zs;
@@ -217,6 +287,14 @@ struct NoOuter {
 __device__ __forceinline__ double absorption_term(cx f, cx df, cx l, double n2) {
     return -((df.im - 2.0 * f.im * l.re) * n2);
 }
+
+// Whether a sink takes the branch-free blocks of orders (a sink with little state, which the
+// block saves for its redo: kPlainBlocks = true).
+template <class S, class = void>
+struct plain_blocks : std::false_type {};
+template <class S>
+struct plain_blocks<S, std::void_t<decltype(S::kPlainBlocks)>>
+    : std::integral_constant<bool, S::kPlainBlocks> {};
 
 // Whether a sink wants a call at the end of each block of orders (block_end(k0, kend)).
 template <class S, class = void>
@@ -255,8 +333,11 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
     const bool m_is_one = (m.re == 1.0 && m.im == 0.0);
     double abs_sum = 0.0;
     double jac0 = 0.0, jac1 = 0.0, jac2 = 0.0;
-    // One order of the loop, with u_{k+1}(mx) in registers.
-    auto order_step = [&](int k, const cx u_z1) {
+    // One order of the loop, with u_{k+1}(mx) in registers. The plain form (fast = true, for
+    // spheres without derivatives, orders below n_s) takes the upward branch without testing
+    // for it and inverts on the plain path of inv, noting its arguments in pc.
+    auto order_step_t = [&](int k, const cx u_z1, auto fast, PlainCheck &pc) {
+        constexpr bool FAST = decltype(fast)::value;
         const double kf = (double)k;
         const double k_x = kf * inv_x;
 
@@ -273,7 +354,7 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
         const cx big_a = d_m + k_x;
         const cx big_b = md + k_x;
         cx an, bn, inv_da, inv_db, g_a, g_b, h_a, h_b;
-        if (k <= n_s) {
+        if (FAST || k <= n_s) {
             const double c = (double)(2 * k - 1) * inv_x;
             const double psi_next = c * psi - psi_prev;
             psi_prev = psi;
@@ -283,8 +364,14 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             chi = chi_next;
             const cx xi = mk(psi, -chi);
             const cx xi_prev = mk(psi_prev, -chi_prev);
-            inv_da = inv(big_a * xi - xi_prev);
-            inv_db = inv(big_b * xi - xi_prev);
+            if constexpr (FAST) {
+                inv_da = inv_plain(big_a * xi - xi_prev, pc);
+                inv_db = inv_plain(big_b * xi - xi_prev, pc);
+            } else {
+                (void)pc;
+                inv_da = inv(big_a * xi - xi_prev);
+                inv_db = inv(big_b * xi - xi_prev);
+            }
             if constexpr (Outer::kLayered) {
                 if (!side.direct_a) {
                     const cx own = m_is_one ? real(0.0) : ((d * inv_m + k_x) * psi - psi_prev);
@@ -307,7 +394,7 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             g_b = xi * inv_db;
             h_a = xi_prev * inv_da;
             h_b = xi_prev * inv_db;
-            if (k == n_s) {
+            if (!FAST && k == n_s) {
                 inv_xi = inv(xi);
                 t = real(psi) * inv_xi;
                 r = xi_prev * inv_xi;
@@ -383,12 +470,41 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             if constexpr (Outer::kLayered) sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md, outer);
         }
     };
+    auto order_step = [&](int k, const cx u_z1) {
+        PlainCheck pc;
+        order_step_t(k, u_z1, std::false_type{}, pc);
+    };
     if constexpr (ZSrc::kBlocked) {
         constexpr int B = ZSrc::kBlock;
         for (int j = 0; 1 + j * B <= n; ++j) {
             zs.regen(j);
             const int k0 = 1 + j * B;
             const int kend = n < k0 + B - 1 ? n : k0 + B - 1;
+            if constexpr (!DERIVS && !Outer::kLayered && plain_blocks<Sink>::value) {
+                // A block wholly below n_s: B orders without branches, redone on the checked
+                // path (from the saved state) in the rare lane whose inverses left the plain
+                // range.
+                if (k0 + B - 1 < n_s) {
+                    const double s_psi = psi, s_psi_prev = psi_prev, s_chi = chi,
+                                 s_chi_prev = chi_prev, s_abs = abs_sum;
+                    const Sink s_sink = sink;
+                    PlainCheck pc;
+#pragma unroll
+                    for (int q = 0; q < B; ++q)
+                        order_step_t(k0 + q, zs.lds[q * 64], std::true_type{}, pc);
+                    if (!pc.ok()) {
+                        psi = s_psi;
+                        psi_prev = s_psi_prev;
+                        chi = s_chi;
+                        chi_prev = s_chi_prev;
+                        abs_sum = s_abs;
+                        sink = s_sink;
+                        for (int q = 0; q < B; ++q) order_step(k0 + q, zs.lds[q * 64]);
+                    }
+                    if constexpr (has_block_end<Sink>::value) sink.block_end(k0, k0 + B - 1);
+                    continue;
+                }
+            }
 #pragma unroll 1
             for (int k = k0; k <= kend; ++k) order_step(k, zs.lds[(k - k0) * 64]);
             // A sink may use the block's tile slots, free once their order is done.
@@ -469,6 +585,7 @@ struct DerivSums {
 // of efficiencies.rs:89-146 for d/dx, d/dRe m, d/dIm m.
 template <bool DERIVS>
 struct EfficiencySink {
+    static constexpr bool kPlainBlocks = !DERIVS;
     double sca = 0.0, g = 0.0, abs_sum = 0.0;
     cx back = {0.0, 0.0};
     cx a_prev = {0.0, 0.0}, b_prev = {0.0, 0.0};

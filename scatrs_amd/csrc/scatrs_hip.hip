@@ -97,11 +97,10 @@ __global__ void __launch_bounds__(256) mie_prep(const double *x, const double *m
     block_partial(n, ok, bad, outside, partial);
 }
 
-// Orders per checkpoint block of the sphere batch (its LDS tile is 64 lanes x this many
-// complex values per wavefront) and its persistent wavefronts per CU: 8 KiB tiles let 16
-// wavefronts (4 per SIMD) share a CU's LDS.
-constexpr int kSphereBlock = 8;
-constexpr int kSphereWavesPerCU = 16;
+// Persistent wavefronts per CU of the sphere batch (two per SIMD: its order blocks run without
+// branches and take the registers of two wavefronts each; its LDS tiles hold kSphereBlock
+// orders of 64 lanes, 8 KiB per wavefront).
+constexpr int kSphereWavesPerCU = 8;
 
 // The persistent solve: each wavefront takes 64 spheres at a time, largest N first, from a
 // shared counter; each lane solves one sphere. The downward u_k(mx) pass keeps one checkpoint
@@ -109,7 +108,7 @@ constexpr int kSphereWavesPerCU = 16;
 // block into the wavefront's LDS tile. u_k(x) above floor(x) (a few dozen orders) stays in the
 // slab.
 template <bool DERIVS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DERIVS ? 2 : 4))) void mie_batch(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mie_batch(
     const double *__restrict__ x, const double *__restrict__ m, int64_t len, int64_t n_fixed,
     const int *__restrict__ perm, unsigned long long *chunk_counter, cx *ck_base,
     double *ux_base, int64_t rows_ck, int64_t rows, double *__restrict__ values,

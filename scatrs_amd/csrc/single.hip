@@ -193,7 +193,7 @@ struct LayeredEffSink {
 // Slab elements of one wavefront: spheres keep the checkpoint column and u_k(x) (as the
 // ensemble node solves do); layered spheres keep 2 L - 1 full complex ratio columns and u_k(x).
 __host__ __device__ inline int64_t single_slab_elems(int layers, int64_t rows) {
-    if (layers == 1) return 64 * ((rows - 2) / kZBlock + 2 + (rows + 1) / 2);
+    if (layers == 1) return 64 * ((rows - 2) / kSphereBlock + 2 + (rows + 1) / 2);
     return rows * 64 * 2 * layers;
 }
 
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void single_solve(const double *__restrict__ x
                                                     const int *__restrict__ nterms, int64_t start,
                                                     int64_t end, cx *scratch, int64_t rows,
                                                     SingleOut out) {
-    __shared__ cx ztile[4][L == 1 ? kZBlock * 64 : 1];
+    __shared__ cx ztile[4][L == 1 ? kSphereBlock * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     cx *slab = scratch + wave * single_slab_elems(L, rows) + lane;
@@ -219,8 +219,8 @@ __global__ __launch_bounds__(256) void single_solve(const double *__restrict__ x
         if constexpr (L == 1) {
             const cx mm = mk(mi[0], mi[1]);
             cx *tile = &ztile[threadIdx.x >> 6][lane];
-            double *ux = real_column(slab, ((rows - 2) / kZBlock + 2) * 64);
-            const ZCheckpoint zs = ratio_chain_ckpt(mm, xi[0], n, slab, 64, tile);
+            double *ux = real_column(slab, ((rows - 2) / kSphereBlock + 2) * 64);
+            const auto zs = ratio_chain_ckpt<kSphereBlock>(mm, xi[0], n, slab, 64, tile);
             x_chain(xi[0], n, upward_orders(xi[0], n), ux, 64);
             NoOuter none;
             mie_orders<DERIVS>(xi[0], mm, n, zs, ux, 64, none, sink);
