@@ -71,6 +71,7 @@ __device__ __forceinline__ double step_real(int k, double inv_rho, double u) {
 struct PlainCheck {
     double lo = 1e308, hi = 0.0, im = 1e308;
     __device__ __forceinline__ bool ok() const { return lo > 1e-300 && hi < 1e300 && im != 0.0; }
+    __device__ __forceinline__ bool ok_pair() const { return lo > 1e-150 && hi < 1e150; }
 };
 
 // 1/z by the one-division form of inv, its argument noted in pc.
@@ -82,6 +83,20 @@ __device__ __forceinline__ cx inv_plain(cx z, PlainCheck &pc) {
     pc.im = fmin(pc.im, fabs(d));
     const double t = recip(s);
     return cx{c * t, -(d * t)};
+}
+
+// 1/za and 1/zb from one reciprocal, 1/(|za|^2 |zb|^2), for two independent inverses of the
+// same order: within a few ulps of inv's wherever both squared moduli lie in (1e-150, 1e150),
+// which the block checks afterwards from pc (pc.ok_pair()).
+__device__ __forceinline__ void inv_plain_pair(cx za, cx zb, PlainCheck &pc, cx &ia, cx &ib) {
+    const double sa = za.re * za.re + za.im * za.im;
+    const double sb = zb.re * zb.re + zb.im * zb.im;
+    pc.lo = fmin(pc.lo, fmin(sa, sb));
+    pc.hi = fmax(pc.hi, fmax(sa, sb));
+    const double t = recip(sa * sb);
+    const double ta = sb * t, tb = sa * t;
+    ia = cx{za.re * ta, -(za.im * ta)};
+    ib = cx{zb.re * tb, -(zb.im * tb)};
 }
 
 // step_complex on the plain path (its result is step_complex's when pc.ok() after the block).
@@ -365,8 +380,7 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             const cx xi = mk(psi, -chi);
             const cx xi_prev = mk(psi_prev, -chi_prev);
             if constexpr (FAST) {
-                inv_da = inv_plain(big_a * xi - xi_prev, pc);
-                inv_db = inv_plain(big_b * xi - xi_prev, pc);
+                inv_plain_pair(big_a * xi - xi_prev, big_b * xi - xi_prev, pc, inv_da, inv_db);
             } else {
                 (void)pc;
                 inv_da = inv(big_a * xi - xi_prev);
@@ -492,7 +506,7 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
 #pragma unroll
                     for (int q = 0; q < B; ++q)
                         order_step_t(k0 + q, zs.lds[q * 64], std::true_type{}, pc);
-                    if (!pc.ok()) {
+                    if (!pc.ok_pair()) {
                         psi = s_psi;
                         psi_prev = s_psi_prev;
                         chi = s_chi;
