@@ -37,7 +37,7 @@ MIE_BATCH_TRAFFIC_BYTES = 1.207e9
 
 
 # The ensemble legs after the sphere batch, in the order they run.
-LEGS = ("ensembles", "adjoint", "layered", "distributions", "adaptive")
+LEGS = ("c1", "ensembles", "adjoint", "layered", "distributions", "adaptive")
 
 
 def log(msg: str) -> None:
@@ -138,6 +138,77 @@ def _timed(torch, ctx, fn, steps):
     kt = ctx.kernel_times()
     ctx.set_profiling(False)
     return (t1 - t0) / steps, kt, out
+
+
+def bench_c1(args, torch):
+    """The reference's CPU-runnable case: a sulfate lognormal (r_g = 0.08 um, sigma_g = 1.6,
+    m = 1.43 + 1e-8 i) at 0.35, 0.55 and 1.0 um, 128 radius nodes, Greek to 32 on a 64-cosine
+    rule (exact for its N <= 33). Latency-bound: ensembles/s of back-to-back eager calls and of
+    replays of the call captured as a HIP graph, and the oracle on the host."""
+    from scatrs_amd import workloads
+    from scatrs_amd.ensemble import EnsembleGraph, ensemble_fixed_device
+
+    kind, p0, p1, wl, m = workloads.sulfate()
+    t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
+    kw = dict(n_radius=128, n_angles=64, order=32, want_phase=False)
+    n = args.c1_repeats
+    for _ in range(5):
+        sc, gr, _, _ = ensemble_fixed_device(*t, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        sc, gr, _, _ = ensemble_fixed_device(*t, **kw)
+    torch.cuda.synchronize()
+    eager = (time.perf_counter() - t0) / n
+    g = EnsembleGraph(*t, **kw)
+    for _ in range(5):
+        g.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        g.launch()
+    torch.cuda.synchronize()
+    replay = (time.perf_counter() - t0) / n
+    g.status()
+    same = bool(torch.equal(g.scalars, sc) and torch.equal(g.greek, gr))
+    g.close()
+    log(f"C1: eager {1e6 * eager:.1f} us/call, graph replay {1e6 * replay:.1f} us/call "
+        f"({3 / replay:.0f} ensembles/s), replay equals eager: {same}")
+    res = {"workload": "sulfate lognormal r_g 0.08 um, sigma_g 1.6, m 1.43+1e-8i at 0.35, 0.55, "
+                       "1.0 um; 128 radius nodes, Greek to 32 (64-cosine rule)",
+           "ensembles": 3, "repeats": n, "eager_us_per_call": 1e6 * eager,
+           "graph_us_per_call": 1e6 * replay, "ensembles_per_s": 3 / replay,
+           "eager_ensembles_per_s": 3 / eager, "graph_equals_eager": same}
+    if not args.no_check:
+        import oracle
+
+        sch, grh = sc.cpu().numpy(), gr.cpu().numpy()
+        es, eg = 0.0, 0.0
+        for e in range(3):
+            ref = oracle.ensemble_fixed(0, p0[e], p1[e], wl[e], m[e], n_radius=128, n_angles=64,
+                                        order=32)
+            k = [0, 1, 4, 5]
+            es = max(es, float(np.max(np.abs(sch[e][k] - ref["scalars"][k]) /
+                                      np.abs(ref["scalars"][k]))))
+            eg = max(eg, float(np.max(np.abs(grh[e] - ref["greek"])) /
+                               np.max(np.abs(ref["greek"][0]))))
+        res["max_rel_err_vs_oracle"] = {"c_ext_c_sca_ssa_g": es, "greek": eg}
+    if not args.no_cpu_baseline:
+        import oracle
+
+        base = {}
+        for th in sorted({1, cpu_threads()}):
+            reps, c0 = 0, time.perf_counter()
+            while time.perf_counter() - c0 < 2.0:
+                oracle.ensemble_batch(kind, p0, p1, wl, m, 128, 64, 32, threads=th)
+                reps += 1
+            base[th] = 3 * reps / (time.perf_counter() - c0)
+        th = max(base)
+        res["cpu_baseline"] = {"value": base[th], "unit": "ensembles/s", "cores": th,
+                               "kind": "port", "one_thread": base[1],
+                               "sample": "the 3 ensembles, repeated for 2 s per thread count"}
+        log(f"C1 cpu baseline: {base[th]:.1f} ensembles/s on {th} threads, {base[1]:.1f} on 1")
+    return res
 
 
 def bench_ensembles(args, torch):
@@ -544,6 +615,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true",
                     help="skip timing the plain FP64 angular kernel")
     ap.add_argument("--ens-steps", type=int, default=1)
+    ap.add_argument("--c1-repeats", type=int, default=2000,
+                    help="calls timed per mode of the sulfate case")
     ap.add_argument("--ens-cpu-sample", type=int, default=32)
     ap.add_argument("--adj-cpu-sample", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0,

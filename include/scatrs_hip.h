@@ -229,6 +229,34 @@ int scatrs_ensemble_fixed_vjp(scatrs_ctx *ctx, int64_t count, const int32_t *d_k
                               const double *d_bar_greek, const double *d_bar_phase,
                               double *d_grad, int64_t *total_terms);
 
+/* A forward ensemble call captured as a HIP graph, for replaying the same batch without
+ * launch or host overhead (a latency-bound call such as three ensembles of 128 nodes issues
+ * about twenty kernels and synchronizes twice). No reference counterpart: the reference has no
+ * device. scatrs_ensemble_fixed_graph takes scatrs_ensemble_fixed's arguments, runs the call
+ * once (the outputs are written, errors are reported as scatrs_ensemble_fixed's), then captures
+ * the same call's kernels, with every host-side decision (sizes, chunks, launch plans) taken
+ * from that run, into *out. The graph owns the device buffers the call used (the context
+ * allocates afresh for later calls), so that other calls on the context cannot disturb it.
+ * scatrs_graph_launch replays the kernels on `stream` (a hipStream_t; NULL: the null stream),
+ * reading the input buffers and writing the output buffers of the capture: inputs changed in
+ * place are recomputed as long as every radius node keeps its number of terms N; otherwise the
+ * replay keeps the captured N (so that every launch stays within its memory) and
+ * scatrs_graph_status (synchronizes `stream`) returns SCATRS_INVALID_INPUT with a message in
+ * scatrs_graph_last_error. A graph is not thread-safe; replays on one graph must not overlap.
+ * scatrs_graph_destroy frees it (synchronizes the device). */
+typedef struct scatrs_graph scatrs_graph;
+int scatrs_ensemble_fixed_graph(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                                const double *d_p0, const double *d_p1, const double *d_p2,
+                                const double *d_wavelength, int layers,
+                                const double *d_fractions, const double *d_m, int64_t n_radius,
+                                int64_t n_angles, int64_t order, int normalized,
+                                int allow_unvalidated, double *d_scalars, double *d_greek,
+                                double *d_phase, int64_t *total_terms, scatrs_graph **out);
+int scatrs_graph_launch(scatrs_graph *g, void *stream);
+int scatrs_graph_status(scatrs_graph *g, void *stream);
+const char *scatrs_graph_last_error(const scatrs_graph *g);
+int scatrs_graph_destroy(scatrs_graph *g);
+
 /* The ensembles of scatrs_ensemble_fixed on the caller's radius nodes: d_nodes (count, n_nodes,
  * 2) the (r, W) pairs of each ensemble's quadrature (r > 0, W >= 0; W the quadrature weight
  * times the number density per ln r, so that an average is sum W q / sum W), every other

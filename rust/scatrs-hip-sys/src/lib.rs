@@ -42,6 +42,12 @@ pub mod ffi {
         _p: [u8; 0],
     }
 
+    /// An opaque captured ensemble call (scatrs_ensemble_fixed_graph).
+    #[repr(C)]
+    pub struct ScatrsGraph {
+        _p: [u8; 0],
+    }
+
     extern "C" {
         pub fn scatrs_abi_version() -> c_int;
         pub fn scatrs_create(device: c_int, stream: *mut c_void, out: *mut *mut ScatrsCtx)
@@ -93,6 +99,16 @@ pub mod ffi {
             n_angles: i64, order: i64, normalized: c_int, fixed_range: c_int,
             allow_unvalidated: c_int, d_bar_scalars: *const f64, d_bar_greek: *const f64,
             d_bar_phase: *const f64, d_grad: *mut f64, total_terms: *mut i64) -> c_int;
+        pub fn scatrs_ensemble_fixed_graph(ctx: *mut ScatrsCtx, count: i64, d_kind: *const i32,
+            d_p0: *const f64, d_p1: *const f64, d_p2: *const f64, d_wavelength: *const f64,
+            layers: c_int, d_fractions: *const f64, d_m: *const f64, n_radius: i64,
+            n_angles: i64, order: i64, normalized: c_int, allow_unvalidated: c_int,
+            d_scalars: *mut f64, d_greek: *mut f64, d_phase: *mut f64, total_terms: *mut i64,
+            out: *mut *mut ScatrsGraph) -> c_int;
+        pub fn scatrs_graph_launch(g: *mut ScatrsGraph, stream: *mut c_void) -> c_int;
+        pub fn scatrs_graph_status(g: *mut ScatrsGraph, stream: *mut c_void) -> c_int;
+        pub fn scatrs_graph_last_error(g: *const ScatrsGraph) -> *const c_char;
+        pub fn scatrs_graph_destroy(g: *mut ScatrsGraph) -> c_int;
         pub fn scatrs_ensemble_nodes(ctx: *mut ScatrsCtx, count: i64, n_nodes: i64,
             d_nodes: *const f64, d_wavelength: *const f64, layers: c_int,
             d_fractions: *const f64, d_m: *const f64, n_angles: i64, order: i64,

@@ -45,6 +45,11 @@ SYMBOLS = (
     "scatrs_layered_coefficients",
     "scatrs_ensemble_fixed",
     "scatrs_ensemble_fixed_vjp",
+    "scatrs_ensemble_fixed_graph",
+    "scatrs_graph_launch",
+    "scatrs_graph_status",
+    "scatrs_graph_last_error",
+    "scatrs_graph_destroy",
     "scatrs_ensemble_nodes",
     "scatrs_ensemble_nodes_vjp",
     "scatrs_size_range",
@@ -99,6 +104,14 @@ def load() -> C.CDLL:
             L.scatrs_ensemble_fixed.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp, i64,
                                                 i64, i64, i32, i32, vp, vp, vp,
                                                 C.POINTER(i64)]
+        L.scatrs_ensemble_fixed_graph.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp, i64,
+                                                  i64, i64, i32, i32, vp, vp, vp,
+                                                  C.POINTER(i64), C.POINTER(vp)]
+        L.scatrs_graph_launch.argtypes = [vp, vp]
+        L.scatrs_graph_status.argtypes = [vp, vp]
+        L.scatrs_graph_last_error.argtypes = [vp]
+        L.scatrs_graph_last_error.restype = C.c_char_p
+        L.scatrs_graph_destroy.argtypes = [vp]
         if hasattr(L, "scatrs_ensemble_fixed_vjp"):
             L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp,
                                                     i64, i64, i64, i32, i32, i32, vp, vp, vp, vp,

@@ -67,6 +67,24 @@ struct scatrs_ctx {
     std::vector<double> phase_amu;  // their distinct |mu|, ascending
     std::vector<int> phase_idx;     // each cosine's place among -|mu|, +|mu| (2 h of them)
     std::vector<int> phase_qptr, phase_qlist;  // the cosines at each place, gathered
+    // Record and replay of a forward ensemble call for HIP graph capture
+    // (scatrs_ensemble_fixed_graph): the first run records every value the host reads back
+    // (each ensemble's number of terms, the largest N, the chunking, each chunk's node-launch
+    // plan); the captured run takes them from the record instead, so it issues no host
+    // synchronization, copy to the host or allocation.
+    struct Replay {
+        int mode = 0;  // 0 off, 1 recording, 2 capturing from the record
+        std::vector<int64_t> terms;
+        int n_max = 0;
+        int64_t total_terms = 0;
+        int64_t max_terms = 0;
+        std::vector<std::vector<int64_t>> plans;  // per chunk: (start, end, rows, waves) ...
+        std::vector<size_t> scratch;             // per chunk: the node launches' slab bytes
+        size_t next_plan = 0;
+        int *node_n_rec = nullptr;  // device: every node's N at recording, for the guard
+        int *guard = nullptr;       // device: set when a replay's inputs change some node's N
+    };
+    Replay replay;
     static constexpr int kMaxEvents = 1024;
     hipEvent_t ev_begin[kMaxEvents] = {}, ev_end[kMaxEvents] = {};
     int ev_count = 0;

@@ -4,6 +4,8 @@
 // products, the quadrature-weighted reduction, and the Greek projection. The kernels live in
 // ensemble_kernels.h.
 #include <cstring>
+#include <string>
+#include <vector>
 
 #include "ensemble_kernels.h"
 
@@ -42,7 +44,11 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
     PhaseMu U;
     if (user_mu && (rc = phase_mu_setup(ctx, S, U))) return rc;
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B, 16 B in fp32).
-    const int64_t max_terms = chunk_terms_limit(4, S.stage32 ? sizeof(float4) : 2 * sizeof(cx));
+    auto &rp = ctx->replay;
+    const int64_t max_terms =
+        rp.mode == 2 ? rp.max_terms
+                     : chunk_terms_limit(4, S.stage32 ? sizeof(float4) : 2 * sizeof(cx));
+    if (rp.mode == 1) rp.max_terms = max_terms;
     int64_t e0 = 0;
     while (e0 < count) {
         int64_t chunk_terms = 0;
@@ -106,6 +112,167 @@ extern "C" int scatrs_ensemble_fixed(scatrs_ctx *ctx, int64_t count, const int32
     return ensemble_forward(ctx, count, d_kind, d_p0, d_p1, d_p2, nullptr, d_wavelength, layers,
                             d_fractions, d_m, n_radius, n_angles, order, normalized,
                             allow_unvalidated, d_scalars, d_greek, d_phase, total_terms);
+}
+
+// A captured forward ensemble call: the HIP graph of its kernels, which own the device buffers
+// they use (taken from the context when the graph was made).
+struct scatrs_graph {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> bufs;                    // the context's buffers and angle-rule tables
+    int *node_n_rec = nullptr;                   // the recorded N per node, then the guard flag
+    int *guard = nullptr;
+    int *host_guard = nullptr;                   // pinned: the guard read back
+    std::string error;
+};
+
+namespace {
+
+// Moves every device buffer of the context into the graph, so that later calls on the context
+// cannot free or grow memory the graph's kernels use (the context allocates afresh).
+void take_buffers(scatrs_ctx *ctx, scatrs_graph *g) {
+    for (int i = 0; i < scatrs_ctx::kNumBufs; ++i) {
+        if (ctx->buf[i]) g->bufs.push_back(ctx->buf[i]);
+        ctx->buf[i] = nullptr;
+        ctx->cap[i] = 0;
+    }
+    for (auto &r : ctx->angle_rules) {
+        if (r.d_atab) g->bufs.push_back(r.d_atab);
+        if (r.d_dtab) g->bufs.push_back(r.d_dtab);
+    }
+    ctx->angle_rules.clear();
+    ctx->rule_nr = -1;
+}
+
+void free_graph(scatrs_graph *g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (void *p : g->bufs) (void)hipFree(p);
+    if (g->node_n_rec) (void)hipFree(g->node_n_rec);
+    if (g->host_guard) (void)hipHostFree(g->host_guard);
+    delete g;
+}
+
+}  // namespace
+
+extern "C" int scatrs_ensemble_fixed_graph(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind,
+                                           const double *d_p0, const double *d_p1,
+                                           const double *d_p2, const double *d_wavelength,
+                                           int layers, const double *d_fractions,
+                                           const double *d_m, int64_t n_radius,
+                                           int64_t n_angles, int64_t order, int normalized,
+                                           int allow_unvalidated, double *d_scalars,
+                                           double *d_greek, double *d_phase,
+                                           int64_t *total_terms, scatrs_graph **out) {
+    if (!ctx || !out) return SCATRS_INVALID_INPUT;
+    *out = nullptr;
+    if (count < 1) {
+        ctx->error.clear();
+        return fail(ctx, SCATRS_INVALID_INPUT, "a captured call needs at least one ensemble");
+    }
+    if (ctx->profiling)
+        return fail(ctx, SCATRS_INVALID_INPUT, "turn profiling off before capturing a call");
+    auto &rp = ctx->replay;
+    rp = scatrs_ctx::Replay{};
+    // 1. The call as usual, recording what the host reads back.
+    rp.mode = 1;
+    int rc = scatrs_ensemble_fixed(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers,
+                                   d_fractions, d_m, n_radius, n_angles, order, normalized,
+                                   allow_unvalidated, d_scalars, d_greek, d_phase, total_terms);
+    scatrs_graph *g = new scatrs_graph();
+    g->device = ctx->device;
+    g->node_n_rec = rp.node_n_rec;
+    g->guard = rp.guard;
+    if (rc == SCATRS_OK) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (rc == SCATRS_OK)
+        rc = check_hip(ctx, hipHostMalloc(&g->host_guard, sizeof(int), hipHostMallocDefault),
+                       "alloc");
+    // 2. The same call captured from the record: kernels only.
+    hipStream_t s = ctx->stream;
+    hipStream_t own = nullptr;
+    if (rc == SCATRS_OK && !s) {
+        // The null stream cannot be captured: capture on a stream of our own.
+        rc = check_hip(ctx, hipStreamCreateWithFlags(&own, hipStreamNonBlocking), "stream");
+        ctx->stream = own;
+    }
+    if (rc == SCATRS_OK) {
+        rp.mode = 2;
+        rp.next_plan = 0;
+        rc = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+                       "capture");
+        if (rc == SCATRS_OK) {
+            int rc2 = scatrs_ensemble_fixed(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength,
+                                            layers, d_fractions, d_m, n_radius, n_angles, order,
+                                            normalized, allow_unvalidated, d_scalars, d_greek,
+                                            d_phase, nullptr);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+            g->graph = graph;
+            rc = rc2 ? rc2 : check_hip(ctx, e, "capture");
+        }
+        if (rc == SCATRS_OK)
+            rc = check_hip(ctx, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0),
+                           "graph instantiate");
+    }
+    if (own) {
+        (void)hipStreamSynchronize(own);
+        (void)hipStreamDestroy(own);
+        ctx->stream = nullptr;
+    }
+    rp.mode = 0;
+    rp.node_n_rec = nullptr;
+    rp.guard = nullptr;
+    if (rc != SCATRS_OK) {
+        free_graph(g);
+        return rc;
+    }
+    take_buffers(ctx, g);
+    *out = g;
+    return SCATRS_OK;
+}
+
+extern "C" int scatrs_graph_launch(scatrs_graph *g, void *stream) {
+    if (!g || !g->exec) return SCATRS_INVALID_INPUT;
+    if (hipSetDevice(g->device) != hipSuccess) return SCATRS_DEVICE_ERROR;
+    const hipError_t e = hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        g->error = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+        return SCATRS_DEVICE_ERROR;
+    }
+    return SCATRS_OK;
+}
+
+extern "C" int scatrs_graph_status(scatrs_graph *g, void *stream) {
+    if (!g) return SCATRS_INVALID_INPUT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (hipMemcpyAsync(g->host_guard, g->guard, sizeof(int), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        g->error = "could not read the graph's status";
+        return SCATRS_DEVICE_ERROR;
+    }
+    if (*g->host_guard) {
+        g->error =
+            "the inputs changed the number of terms of a radius node since the call was "
+            "captured: the replay kept the captured truncations and its results are invalid; "
+            "capture the call again";
+        return SCATRS_INVALID_INPUT;
+    }
+    g->error.clear();
+    return SCATRS_OK;
+}
+
+extern "C" const char *scatrs_graph_last_error(const scatrs_graph *g) {
+    return g ? g->error.c_str() : "";
+}
+
+extern "C" int scatrs_graph_destroy(scatrs_graph *g) {
+    if (!g) return SCATRS_OK;
+    (void)hipSetDevice(g->device);
+    (void)hipDeviceSynchronize();
+    free_graph(g);
+    return SCATRS_OK;
 }
 
 extern "C" int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,

@@ -184,6 +184,17 @@ static __global__ void __launch_bounds__(256) ens_prep(EnsArgs a, int64_t count,
     block_partial(n, ok, bad, outside, partial);
 }
 
+// The replay guard of a captured call: every node keeps its recorded N (the launches were
+// planned for it), and *guard is set if the inputs changed any node's N.
+static __global__ void __launch_bounds__(256) plan_guard(int *node_n, const int *node_n_rec,
+                                                         int64_t total, int *guard) {
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g < total && node_n[g] != node_n_rec[g]) {
+        node_n[g] = node_n_rec[g];
+        *guard = 1;
+    }
+}
+
 // Exclusive scan of N over each ensemble's nodes: node_off (terms) and ens_terms.
 static __global__ void __launch_bounds__(256) ens_offsets(const int *node_n, int64_t n_radius,
                                                    int64_t *node_off, int64_t *ens_terms) {
@@ -522,14 +533,19 @@ constexpr int kGroupNodes = 16;
 constexpr int kMfmaOrders = 32;
 constexpr int kSliceAngles = 128;
 
+// With splits > 1 (small batches, whose ensembles and cosine slices alone do not fill the
+// chip) the node groups of an ensemble are dealt over `splits` blocks, split sp taking groups
+// sp, sp + splits, ...; each writes its partial table to tab_out + sp * (ne 4 n_angles), and
+// ens_split_reduce adds the splits in order.
 static __global__ __launch_bounds__(512) void ens_angular_mfma(
-    int64_t e0, int64_t n_radius, int n_angles, int h, int slices, const int *node_n,
-    const int64_t *node_off, const int64_t *ens_base, const cx *staging, const double *tabc,
-    const double2 *table, double *tab_out) {
+    int64_t e0, int64_t n_radius, int n_angles, int h, int slices, int splits, int64_t ne,
+    const int *node_n, const int64_t *node_off, const int64_t *ens_base, const cx *staging,
+    const double *tabc, const double2 *table, double *tab_out) {
     __shared__ cx lds_a[kGroupNodes][kMfmaOrders];
     __shared__ cx lds_b[kGroupNodes][kMfmaOrders];
-    const int64_t eb = blockIdx.x / slices;
-    const int slice = blockIdx.x % slices;
+    const int sp = blockIdx.x % splits;
+    const int64_t eb = (blockIdx.x / splits) / slices;
+    const int slice = (blockIdx.x / splits) % slices;
     const int64_t e = e0 + eb;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
@@ -541,7 +557,8 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
     const int64_t *noff = node_off + e * n_radius;
     const int64_t gbase = eb * n_radius;
     double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][v]: this lane's element, rows kq+4v
-    for (int64_t g0 = 0; g0 < n_radius; g0 += kGroupNodes) {
+    for (int64_t g0 = (int64_t)sp * kGroupNodes; g0 < n_radius;
+         g0 += (int64_t)splits * kGroupNodes) {
         int nmax = 0;
         for (int q = 0; q < kGroupNodes; ++q) {
             const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
@@ -632,7 +649,7 @@ static __global__ __launch_bounds__(512) void ens_angular_mfma(
             tp[sgn][v] = x;
         }
     if (qin == 0) {
-        double *out = tab_out + eb * 4 * (int64_t)n_angles;
+        double *out = tab_out + ((int64_t)sp * ne + eb) * 4 * (int64_t)n_angles;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16 (i % 4), element i / 4.
@@ -680,6 +697,16 @@ static int table_exponent(int n_max) {
 
 // tab16[(2 t + q) * h * npad + jj * npad + n - 1]: slice q of table t (G1, G2) times 2^texp;
 // zero beyond n_max.
+// The split partial tables of ens_angular_mfma added in split order: out[i] = sum_sp part[sp][i].
+static __global__ void ens_split_reduce(const double *part, int splits, int64_t cells,
+                                        double *out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    double acc = part[i];
+    for (int sp = 1; sp < splits; ++sp) acc += part[(int64_t)sp * cells + i];
+    out[i] = acc;
+}
+
 static __global__ void pt_table_f16(const double2 *table, int h, int n_max, int npad, int texp,
                                     _Float16 *tab16) {
     const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1144,6 +1171,7 @@ static int upload(scatrs_ctx *ctx, int which, const std::vector<T> &v, T **out) 
     int rc = check_hip(ctx, ensure(ctx, which, v.size() * sizeof(T)), "alloc");
     if (rc) return rc;
     *out = static_cast<T *>(ctx->buf[which]);
+    if (ctx->replay.mode == 2) return SCATRS_OK;  // uploaded by the recording run
     return check_hip(ctx, hipMemcpyAsync(*out, v.data(), v.size() * sizeof(T),
                                          hipMemcpyHostToDevice, ctx->stream),
                      "upload");
@@ -1380,29 +1408,59 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
     S.a.geom = static_cast<double2 *>(ctx->buf[scatrs_ctx::kGeom]);
     S.a.user_rw = reinterpret_cast<const double2 *>(d_nodes);
     ens_prep<<<(unsigned)pblocks, 256, 0, s>>>(S.a, count, allow_unvalidated, S.node_n, partial);
+    auto &rp = ctx->replay;
+    if (rp.mode == 2) {
+        // A replay computes with the recorded N at every node (the launches were planned for
+        // them) and flags any node whose N its inputs changed.
+        fill_u32<<<1, 1, 0, s>>>(reinterpret_cast<unsigned *>(rp.guard), 0u, 1);
+        plan_guard<<<(unsigned)pblocks, 256, 0, s>>>(S.node_n, rp.node_n_rec, total, rp.guard);
+    }
     reduce_status<<<1, 1024, 0, s>>>(partial, pblocks, st);
     ens_offsets<<<(unsigned)count, 256, 0, s>>>(S.node_n, n_radius, S.node_off, S.ens_terms);
-    Status *hst = static_cast<Status *>(ctx->host_pinned);
-    if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
-                        "status copy")))
-        return rc;
-    S.terms.assign(count, 0);
-    S.base.assign(count, 0);
-    if ((rc = check_hip(ctx, hipMemcpyAsync(S.terms.data(), S.ens_terms, count * sizeof(int64_t),
-                                            hipMemcpyDeviceToHost, s),
-                        "terms copy")))
-        return rc;
-    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "ensemble prep"))) return rc;
-    const unsigned long long bad = std::min(hst->first_bad, hst->first_outside);
-    if (bad != ~0ull) {
-        int code = kInvalid;
-        const std::string msg = ensemble_message(ctx, (int64_t)bad, S.a, n_radius, S.rr.nodes,
-                                                 allow_unvalidated, &code);
-        return fail(ctx, code == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
-                    msg);
+    if (rp.mode == 2) {
+        S.terms = rp.terms;
+        S.base.assign(count, 0);
+        S.total_terms = rp.total_terms;
+        S.n_max = rp.n_max;
     }
-    S.total_terms = (int64_t)hst->total_terms;
-    S.n_max = hst->n_max;
+    Status *hst = static_cast<Status *>(ctx->host_pinned);
+    // The status and each ensemble's number of terms come back to the host (not in a replay).
+    if (rp.mode != 2) {
+        if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
+                            "status copy")))
+            return rc;
+        S.terms.assign(count, 0);
+        S.base.assign(count, 0);
+        if ((rc = check_hip(ctx, hipMemcpyAsync(S.terms.data(), S.ens_terms, count * sizeof(int64_t),
+                                                hipMemcpyDeviceToHost, s),
+                            "terms copy")))
+            return rc;
+        if ((rc = check_hip(ctx, hipStreamSynchronize(s), "ensemble prep"))) return rc;
+        const unsigned long long bad = std::min(hst->first_bad, hst->first_outside);
+        if (bad != ~0ull) {
+            int code = kInvalid;
+            const std::string msg = ensemble_message(ctx, (int64_t)bad, S.a, n_radius, S.rr.nodes,
+                                                     allow_unvalidated, &code);
+            return fail(ctx, code == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
+                        msg);
+        }
+        S.total_terms = (int64_t)hst->total_terms;
+        S.n_max = hst->n_max;
+        if (rp.mode == 1) {
+            rp.terms = S.terms;
+            rp.total_terms = S.total_terms;
+            rp.n_max = S.n_max;
+            if ((rc = check_hip(ctx, hipMalloc(&rp.node_n_rec, (size_t)total * sizeof(int) + 256),
+                                "alloc")))
+                return rc;
+            rp.guard = reinterpret_cast<int *>(reinterpret_cast<char *>(rp.node_n_rec) +
+                                               (size_t)total * sizeof(int));
+            if ((rc = check_hip(ctx, hipMemcpyAsync(rp.node_n_rec, S.node_n, (size_t)total * sizeof(int),
+                                                    hipMemcpyDeviceToDevice, s),
+                                "record")))
+                return rc;
+        }
+    }
     // The w pi_n, w tau_n table at the non-negative cosines up to the largest N.
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables, (size_t)S.n_max * h * sizeof(double2)),
                         "table")))
@@ -1528,6 +1586,16 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
     }
 }
 
+// Keeps a chunk's node-launch plan for the replay (recording run only).
+static void record_plan(scatrs_ctx *ctx, const ChunkBufs &B, size_t scratch_bytes) {
+    auto &rp = ctx->replay;
+    if (rp.mode != 1) return;
+    std::vector<int64_t> q;
+    for (const NodeLaunch &l : B.plan) q.insert(q.end(), {l.start, l.end, l.rows, l.waves});
+    rp.plans.push_back(std::move(q));
+    rp.scratch.push_back(scratch_bytes);
+}
+
 // Ranks the chunk's nodes by N, largest first, into B.order, and plans the node launches of
 // B.plan over that ranking. Spheres: one persistent launch, slabs sized by the chunk's largest
 // N, as many wavefronts as kSphereSlabBudget holds (at most 8 per CU). Layered spheres:
@@ -1552,6 +1620,17 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
     histogram_keys<<<(unsigned)((nodes + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(keys, nodes,
                                                                                     hist);
     sort_by_keys(keys, hist, B.order, nodes, s);
+    auto &rp = ctx->replay;
+    if (rp.mode == 2) {
+        if (rp.next_plan >= rp.plans.size())
+            return fail(ctx, SCATRS_DEVICE_ERROR, "the graph capture met more chunks than recorded");
+        const std::vector<int64_t> &q = rp.plans[rp.next_plan];
+        B.plan.clear();
+        for (size_t i = 0; i + 3 < q.size(); i += 4)
+            B.plan.push_back(NodeLaunch{q[i], q[i + 1], q[i + 2], q[i + 3]});
+        *scratch_bytes = rp.scratch[rp.next_plan++];
+        return SCATRS_OK;
+    }
     std::vector<unsigned> h(kBuckets);
     if ((rc = check_hip(ctx, hipMemcpyAsync(h.data(), hist, kBuckets * sizeof(unsigned),
                                             hipMemcpyDeviceToHost, s),
@@ -1572,6 +1651,7 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         waves = (waves + 3) / 4 * 4;
         B.plan.push_back(NodeLaunch{0, nodes, rows, waves});
         *scratch_bytes = per_wave * (size_t)waves;
+        record_plan(ctx, B, *scratch_bytes);
         return SCATRS_OK;
     }
     int64_t pos = 0, below = 0;  // below: ranked nodes in the buckets before b
@@ -1589,6 +1669,7 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         *scratch_bytes = std::max(*scratch_bytes, per_wave * (size_t)((end - pos + 255) / 256 * 4));
         pos = end;
     }
+    record_plan(ctx, B, *scratch_bytes);
     return SCATRS_OK;
 }
 
@@ -1618,8 +1699,10 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
         return rc;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, scratch_bytes), "scratch")))
         return rc;
-    // ens_base holds the staging offsets of this chunk's ensembles, relative to e0's.
-    if ((rc = check_hip(ctx, hipMemcpyAsync(S.ens_base + e0, S.base.data() + e0, ne * sizeof(int64_t),
+    // ens_base holds the staging offsets of this chunk's ensembles, relative to e0's (a replay
+    // finds them where the recording run put them).
+    if (ctx->replay.mode != 2 &&
+        (rc = check_hip(ctx, hipMemcpyAsync(S.ens_base + e0, S.base.data() + e0, ne * sizeof(int64_t),
                                             hipMemcpyHostToDevice, s),
                         "base copy")))
         return rc;
@@ -1680,9 +1763,26 @@ static int angular_products(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne
             B.staging32, B.tabc, B.xexp, tab16, tab_out);
     } else {
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
-        ens_angular_mfma<<<(unsigned)(ne * slices), 512, 0, s>>>(
-            e0, n_radius, n_out, h, slices, S.node_n, S.node_off, S.ens_base, B.staging, B.tabc,
-            table, tab_out);
+        // Small batches: node groups over more blocks, up to about two per CU in all.
+        const int64_t groups = (n_radius + kGroupNodes - 1) / kGroupNodes;
+        const int64_t want = (2 * (int64_t)ctx->cus + ne * slices - 1) / (ne * slices);
+        const int splits = (int)std::max<int64_t>(1, std::min<int64_t>({groups, want, 16}));
+        const int64_t cells = ne * 4 * (int64_t)n_out;
+        double *dst = tab_out;
+        if (splits > 1) {
+            int rc;
+            if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kMisc3,
+                                            (size_t)splits * cells * sizeof(double)),
+                                "split tables")))
+                return rc;
+            dst = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc3]);
+        }
+        ens_angular_mfma<<<(unsigned)(ne * slices * splits), 512, 0, s>>>(
+            e0, n_radius, n_out, h, slices, splits, ne, S.node_n, S.node_off, S.ens_base,
+            B.staging, B.tabc, table, dst);
+        if (splits > 1)
+            ens_split_reduce<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(dst, splits, cells,
+                                                                           tab_out);
     }
     mark_end(ctx, s);
     return check_hip(ctx, hipGetLastError(), "ensemble launch");
@@ -1728,7 +1828,8 @@ static int phase_mu_setup(scatrs_ctx *ctx, const EnsSetup &S, PhaseMu &U) {
         const std::vector<int> &src;
     } copies[3] = {{U.idx, ctx->phase_idx}, {U.q_ptr, ctx->phase_qptr}, {U.q_list, ctx->phase_qlist}};
     for (const auto &c : copies)
-        if ((rc = check_hip(ctx, hipMemcpyAsync(c.dst, c.src.data(), c.src.size() * sizeof(int),
+        if (ctx->replay.mode != 2 &&
+            (rc = check_hip(ctx, hipMemcpyAsync(c.dst, c.src.data(), c.src.size() * sizeof(int),
                                                 hipMemcpyHostToDevice, s),
                             "upload")))
             return rc;

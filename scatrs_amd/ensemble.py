@@ -22,7 +22,7 @@ from .errors import InvalidInputError
 from .particles import LayeredSphere, Sphere
 from .size import SEGMENT, Distribution, MultiModal, Tabulated, check
 
-__all__ = ["SCALARS", "Ensemble", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
+__all__ = ["SCALARS", "Ensemble", "EnsembleGraph", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
            "ensemble_nodes", "ensemble_nodes_device", "ensemble_nodes_vjp_device",
            "assemble", "mix_parts", "mix_units", "number_averages", "parameter_names"]
@@ -75,6 +75,81 @@ def ensemble_fixed_device(kind, p0, p1, wavelength, m, fractions=None, *, p2=Non
             ctx.set_phase_cosines(None)
     ctx.check(rc)
     return scalars, greek, phase, terms.value
+
+
+class EnsembleGraph:
+    """A forward ensemble batch captured as a HIP graph (scatrs_ensemble_fixed_graph): made
+    from the arguments of ensemble_fixed_device, whose outputs it computes once; ``launch()``
+    recomputes them into the same output tensors by replaying the captured kernels (no host
+    synchronization, no per-kernel launch cost), reading the same input tensors, which may be
+    changed in place as long as no radius node changes its number of terms (``status()``
+    reports it). The graph keeps its inputs and outputs alive."""
+
+    def __init__(self, kind, p0, p1, wavelength, m, fractions=None, *, p2=None,
+                 n_radius: int = 128, n_angles: int = 128, order: int | None = 32,
+                 normalized: bool = True, allow_unvalidated: bool = False,
+                 want_phase: bool = True):
+        import torch
+
+        dev = p0.device
+        count = p0.numel()
+        layers = 1 if m.dim() == 1 else m.shape[1]
+        if layers > 1 and (fractions is None or fractions.shape != (count, layers - 1)):
+            raise InvalidInputError("layered spheres need fractions of shape (count, L - 1)")
+        self.inputs = [kind.to(torch.int32).contiguous(), p0.contiguous(), p1.contiguous(),
+                       wavelength.contiguous(), m.contiguous(),
+                       None if layers == 1 else fractions.contiguous(),
+                       None if p2 is None else p2.to(torch.float64).contiguous()]
+        self.scalars = torch.empty((count, 11), dtype=torch.float64, device=dev)
+        self.greek = None if order is None else torch.empty((count, 6, order + 1),
+                                                            dtype=torch.float64, device=dev)
+        self.phase = (torch.empty((count, 6, n_angles), dtype=torch.float64, device=dev)
+                      if want_phase else None)
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+        self.ctx = _native.context(dev.index)
+        k, q0, q1, wl, mm, fr, q2 = self.inputs
+        terms = C.c_int64(0)
+        handle = C.c_void_p()
+        self.ctx.check(self.ctx.lib.scatrs_ensemble_fixed_graph(
+            self.ctx.handle, count, ptr(k), ptr(q0), ptr(q1), ptr(q2), ptr(wl), layers, ptr(fr),
+            ptr(mm), n_radius, n_angles, -1 if order is None else order, int(bool(normalized)),
+            int(bool(allow_unvalidated)), ptr(self.scalars), ptr(self.greek), ptr(self.phase),
+            C.byref(terms), C.byref(handle)))
+        self.handle = handle
+        self.terms = terms.value
+        self.lib = self.ctx.lib
+
+    def launch(self, stream=None) -> None:
+        """Replays the call on ``stream`` (default: torch's current stream)."""
+        import torch
+
+        s = torch.cuda.current_stream(self.scalars.device) if stream is None else stream
+        rc = self.lib.scatrs_graph_launch(self.handle, C.c_void_p(s.cuda_stream))
+        if rc != _native.OK:
+            raise _native.NativeLibraryError(self.lib.scatrs_graph_last_error(self.handle).decode())
+
+    def status(self, stream=None) -> None:
+        """Waits for the replays on ``stream`` and raises InvalidInputError if an input changed a
+        node's number of terms since the capture."""
+        import torch
+
+        s = torch.cuda.current_stream(self.scalars.device) if stream is None else stream
+        rc = self.lib.scatrs_graph_status(self.handle, C.c_void_p(s.cuda_stream))
+        if rc == _native.INVALID_INPUT:
+            raise InvalidInputError(self.lib.scatrs_graph_last_error(self.handle).decode())
+        if rc != _native.OK:
+            raise _native.NativeLibraryError(self.lib.scatrs_graph_last_error(self.handle).decode())
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.scatrs_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ensemble_nodes_device(nodes, wavelength, m, fractions=None, *, n_angles: int = 128,
