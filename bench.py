@@ -60,6 +60,7 @@ def bench_spheres(args, torch):
     mt = torch.from_numpy(m).to(dev)
     ctx = _native.context(0)
     log(f"spheres: {x.size}, terms {terms}, algorithmic flops {flops:.4e}")
+    tuned = _tune(args, torch, ctx, lambda: efficiencies_device(xt, mt), ["sphere_waves"])
     for _ in range(args.warmup):
         efficiencies_device(xt, mt)
     torch.cuda.synchronize()
@@ -117,7 +118,24 @@ def bench_spheres(args, torch):
                    "(all five efficiencies computed)", "spheres": 1 << 20, "terms": terms,
                    "seed": workloads.SEED},
         "roofline": roof, "cpu_baseline": cpu, "max_rel_err_vs_oracle": check,
+        "tuning": tuned,
     }
+
+
+def _tune(args, torch, ctx, run, params, repeats=5):
+    """The project's autotuner (scatrs_amd.tune) on this workload: the chosen launch parameters
+    stay applied on the context; returns them with the candidates' times."""
+    from scatrs_amd import tune
+
+    if args.no_tune:
+        return None
+    res = tune.tune(run, lambda name, v: ctx.set_tuning(tune.KEYS[name], v), params,
+                    repeats=repeats, sync=torch.cuda.synchronize)
+    log("tuned " + ", ".join(f"{k} = {v}" for k, v in res["best"].items()) + " (" +
+        "; ".join(f"{k}: " + ", ".join(f"{c} {1e3 * t:.3f} ms" for c, t in row.items())
+                  for k, row in res["timings"].items()) + ")")
+    return {"best": res["best"],
+            "ms": {k: {str(c): 1e3 * t for c, t in row.items()} for k, row in res["timings"].items()}}
 
 
 def _dev(torch, a):
@@ -145,13 +163,16 @@ def bench_c1(args, torch):
     m = 1.43 + 1e-8 i) at 0.35, 0.55 and 1.0 um, 128 radius nodes, Greek to 32 on a 64-cosine
     rule (exact for its N <= 33). Latency-bound: ensembles/s of back-to-back eager calls and of
     replays of the call captured as a HIP graph, and the oracle on the host."""
-    from scatrs_amd import workloads
+    from scatrs_amd import _native, workloads
     from scatrs_amd.ensemble import EnsembleGraph, ensemble_fixed_device
 
     kind, p0, p1, wl, m = workloads.sulfate()
     t = [_dev(torch, v) for v in (kind, p0, p1, wl, m)]
     kw = dict(n_radius=128, n_angles=64, order=32, want_phase=False)
     n = args.c1_repeats
+    ctx = _native.context(0)
+    tuned = _tune(args, torch, ctx, lambda: ensemble_fixed_device(*t, **kw), ["split_blocks"],
+                  repeats=20)
     for _ in range(5):
         sc, gr, _, _ = ensemble_fixed_device(*t, **kw)
     torch.cuda.synchronize()
@@ -178,7 +199,7 @@ def bench_c1(args, torch):
                        "1.0 um; 128 radius nodes, Greek to 32 (64-cosine rule)",
            "ensembles": 3, "repeats": n, "eager_us_per_call": 1e6 * eager,
            "graph_us_per_call": 1e6 * replay, "ensembles_per_s": 3 / replay,
-           "eager_ensembles_per_s": 3 / eager, "graph_equals_eager": same}
+           "eager_ensembles_per_s": 3 / eager, "graph_equals_eager": same, "tuning": tuned}
     if not args.no_check:
         import oracle
 
@@ -231,6 +252,7 @@ def bench_ensembles(args, torch):
     # The fused path: fp16-split matrix-core angular products (kernel 2).
     ctx.set_angular_kernel(2)
     try:
+        tuned = _tune(args, torch, ctx, fn, ["chunk_share", "node_waves"], repeats=1)
         fn()
         sec, kt, out = _timed(torch, ctx, fn, args.ens_steps)
     finally:
@@ -244,7 +266,7 @@ def bench_ensembles(args, torch):
     res = {"workload": "4096 lognormals (r_g log-uniform [0.01, 5] um, sigma_g U[1.2, 2.5], m as "
                        "the sphere batch) x 64 wavelengths log-spaced 0.3-2.5 um; 512 radius nodes, "
                        "1024 angles, Greek to 256, F11..F44",
-           "ensembles": int(kind.size), "terms": int(terms),
+           "ensembles": int(kind.size), "terms": int(terms), "tuning": tuned,
            "ensembles_per_s": kind.size / sec, "s_per_step": sec, "steps": args.ens_steps,
            "node_ms_per_step": node_ms, "angular_ms_per_step": ang_ms}
     ach = flops / (ang_ms * 1e-3) / 1e12
@@ -602,6 +624,8 @@ def main() -> int:
     ap.add_argument("--workload", default="spheres", choices=["spheres"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default launch parameters (no autotuning)")
     ap.add_argument("--traffic", type=float, default=MIE_BATCH_TRAFFIC_BYTES,
                     help="HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc")
     ap.add_argument("--no-ensembles", action="store_true",

@@ -66,6 +66,24 @@ int scatrs_kernel_times(scatrs_ctx *ctx, double *ms_out, int cap);
  * No reference counterpart. */
 int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
 
+/* Launch parameters of a context, for the project's autotuner (scatrs_amd/tune.py), which
+ * times candidate values on the caller's workload and keeps the fastest; no reference
+ * counterpart. Results do not depend on them, except SCATRS_TUNE_SPLIT_BLOCKS, which changes
+ * the order in which a small batch's angular products are summed (rounding only).
+ *   SCATRS_TUNE_SPHERE_WAVES: persistent wavefronts per CU of the sphere batch (1-32, default 8);
+ *   SCATRS_TUNE_NODE_WAVES: the same for the ensembles' sphere node solves (1-32, default 8);
+ *   SCATRS_TUNE_CHUNK_SHARE: a chunk of ensembles stages its coefficients in 1/share of the free
+ *     device memory (1-64, default 4; at most 64 GiB, at least 1 GiB);
+ *   SCATRS_TUNE_SPLIT_BLOCKS: blocks per CU that a small batch's FP64 matrix-core angular
+ *     products aim for by dealing node groups over blocks (1-64, default 2).
+ * scatrs_get_tuning returns the value, or -1 for an unknown key. */
+#define SCATRS_TUNE_SPHERE_WAVES 0
+#define SCATRS_TUNE_NODE_WAVES 1
+#define SCATRS_TUNE_CHUNK_SHARE 2
+#define SCATRS_TUNE_SPLIT_BLOCKS 3
+int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value);
+int64_t scatrs_get_tuning(const scatrs_ctx *ctx, int key);
+
 /* The cosines at which scatrs_ensemble_fixed writes the phase matrix: mu (host, n_mu values in
  * [-1, 1], at most 65536; n_mu = 0 restores the default, the n_angles Gauss-Legendre cosines);
  * d_phase is then (count, 6, n_mu). The angular products run once more at these cosines (with

@@ -25,6 +25,12 @@ pub const SCATRS_DEVICE_ERROR: c_int = 4;
 /// The ABI version this crate was written against (`scatrs_abi_version()`).
 pub const SCATRS_ABI_VERSION: c_int = 3;
 
+/// Launch-parameter keys of scatrs_set_tuning.
+pub const SCATRS_TUNE_SPHERE_WAVES: c_int = 0;
+pub const SCATRS_TUNE_NODE_WAVES: c_int = 1;
+pub const SCATRS_TUNE_CHUNK_SHARE: c_int = 2;
+pub const SCATRS_TUNE_SPLIT_BLOCKS: c_int = 3;
+
 /// Distribution kinds of the ensemble entry points (`scatrs/src/ensemble/distribution.rs:15-68`).
 pub const SCATRS_LOGNORMAL: i32 = 0;
 pub const SCATRS_GAMMA: i32 = 1;
@@ -59,6 +65,8 @@ pub mod ffi {
         /// ms_out: host array of `cap` doubles.
         pub fn scatrs_kernel_times(ctx: *mut ScatrsCtx, ms_out: *mut f64, cap: c_int) -> c_int;
         pub fn scatrs_set_angular_kernel(ctx: *mut ScatrsCtx, kind: c_int) -> c_int;
+        pub fn scatrs_set_tuning(ctx: *mut ScatrsCtx, key: c_int, value: i64) -> c_int;
+        pub fn scatrs_get_tuning(ctx: *const ScatrsCtx, key: c_int) -> i64;
         /// mu: host array of n_mu cosines.
         pub fn scatrs_set_phase_cosines(ctx: *mut ScatrsCtx, mu: *const f64, n_mu: i64) -> c_int;
         pub fn scatrs_default_n_max(x: f64) -> i64;

@@ -34,6 +34,8 @@ SYMBOLS = (
     "scatrs_set_profiling",
     "scatrs_kernel_times",
     "scatrs_set_angular_kernel",
+    "scatrs_set_tuning",
+    "scatrs_get_tuning",
     "scatrs_set_phase_cosines",
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
@@ -84,6 +86,9 @@ def load() -> C.CDLL:
         L.scatrs_set_profiling.argtypes = [vp, i32]
         L.scatrs_kernel_times.argtypes = [vp, C.POINTER(dbl), i32]
         L.scatrs_set_angular_kernel.argtypes = [vp, i32]
+        L.scatrs_set_tuning.argtypes = [vp, i32, i64]
+        L.scatrs_get_tuning.argtypes = [vp, i32]
+        L.scatrs_get_tuning.restype = i64
         L.scatrs_set_phase_cosines.argtypes = [vp, vp, i64]
         L.scatrs_default_n_max.argtypes = [dbl]
         L.scatrs_default_n_max.restype = i64
@@ -166,6 +171,13 @@ class Context:
         a = np.ascontiguousarray(mu, np.float64).reshape(-1)
         self.check(self.lib.scatrs_set_phase_cosines(
             self.handle, a.ctypes.data_as(C.c_void_p), a.size))
+
+    def set_tuning(self, key: int, value: int) -> None:
+        """A launch parameter of the context (scatrs_set_tuning)."""
+        self.check(self.lib.scatrs_set_tuning(self.handle, int(key), int(value)))
+
+    def get_tuning(self, key: int) -> int:
+        return int(self.lib.scatrs_get_tuning(self.handle, int(key)))
 
     def set_angular_kernel(self, kind: int) -> None:
         """0: FP64 matrix cores (default); 1: the plain FP64 vector kernel; 2: fp16 matrix

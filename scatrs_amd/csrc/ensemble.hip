@@ -47,7 +47,8 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
     auto &rp = ctx->replay;
     const int64_t max_terms =
         rp.mode == 2 ? rp.max_terms
-                     : chunk_terms_limit(4, S.stage32 ? sizeof(float4) : 2 * sizeof(cx));
+                     : chunk_terms_limit(ctx->tune[SCATRS_TUNE_CHUNK_SHARE],
+                                         S.stage32 ? sizeof(float4) : 2 * sizeof(cx));
     if (rp.mode == 1) rp.max_terms = max_terms;
     int64_t e0 = 0;
     while (e0 < count) {

@@ -1646,7 +1646,8 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         const int64_t n = b0 == 0 ? S.n_max : std::max(kBuckets - 2 - b0, 0);
         const int64_t rows = n + 2;
         const size_t per_wave = (size_t)slab_elems(1, rows) * sizeof(cx);
-        int64_t waves = std::min<int64_t>((int64_t)ctx->cus * 8, (nodes + 63) / 64);
+        int64_t waves = std::min<int64_t>((int64_t)ctx->cus * ctx->tune[SCATRS_TUNE_NODE_WAVES],
+                                          (nodes + 63) / 64);
         waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)(kSphereSlabBudget / per_wave)));
         waves = (waves + 3) / 4 * 4;
         B.plan.push_back(NodeLaunch{0, nodes, rows, waves});
@@ -1765,7 +1766,9 @@ static int angular_products(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne
         const int slices = (h + kSliceAngles - 1) / kSliceAngles;
         // Small batches: node groups over more blocks, up to about two per CU in all.
         const int64_t groups = (n_radius + kGroupNodes - 1) / kGroupNodes;
-        const int64_t want = (2 * (int64_t)ctx->cus + ne * slices - 1) / (ne * slices);
+        const int64_t want =
+            (ctx->tune[SCATRS_TUNE_SPLIT_BLOCKS] * (int64_t)ctx->cus + ne * slices - 1) /
+            (ne * slices);
         const int splits = (int)std::max<int64_t>(1, std::min<int64_t>({groups, want, 16}));
         const int64_t cells = ne * 4 * (int64_t)n_out;
         double *dst = tab_out;

@@ -97,11 +97,6 @@ __global__ void __launch_bounds__(256) mie_prep(const double *x, const double *m
     block_partial(n, ok, bad, outside, partial);
 }
 
-// Persistent wavefronts per CU of the sphere batch (two per SIMD: its order blocks run without
-// branches and take the registers of two wavefronts each; its LDS tiles hold kSphereBlock
-// orders of 64 lanes, 8 KiB per wavefront).
-constexpr int kSphereWavesPerCU = 8;
-
 // The persistent solve: each wavefront takes 64 spheres at a time, largest N first, from a
 // shared counter; each lane solves one sphere. The downward u_k(mx) pass keeps one checkpoint
 // per kSphereBlock indices in the wavefront's global slab; the order loop regenerates each
@@ -350,6 +345,26 @@ int scatrs_set_profiling(scatrs_ctx *ctx, int on) {
     return SCATRS_OK;
 }
 
+int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value) {
+    if (!ctx) return SCATRS_INVALID_INPUT;
+    ctx->error.clear();
+    static const int64_t lo[4] = {1, 1, 1, 1}, hi[4] = {32, 32, 64, 64};
+    if (key < 0 || key > 3)
+        return fail(ctx, SCATRS_INVALID_INPUT, "unknown tuning key " + std::to_string(key));
+    if (value < lo[key] || value > hi[key])
+        return fail(ctx, SCATRS_INVALID_INPUT,
+                    "tuning key " + std::to_string(key) + " takes values in [" +
+                        std::to_string(lo[key]) + ", " + std::to_string(hi[key]) + "], got " +
+                        std::to_string(value));
+    ctx->tune[key] = (int)value;
+    return SCATRS_OK;
+}
+
+int64_t scatrs_get_tuning(const scatrs_ctx *ctx, int key) {
+    if (!ctx || key < 0 || key > 3) return -1;
+    return ctx->tune[key];
+}
+
 int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind) {
     if (!ctx) return SCATRS_INVALID_INPUT;
     if (kind < 0 || kind > 2)
@@ -475,10 +490,12 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
     int64_t nmax = 0;
     if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
         return rc;
-    // Persistent wavefronts: kSphereWavesPerCU per CU (4-wave blocks share the CU's LDS
-    // tiles), fewer if the slabs would pass 8 GiB.
+    // Persistent wavefronts: the tuned number per CU (default 8: two per SIMD, which the
+    // kernel's registers allow; 4-wave blocks share the CU's LDS tiles of kSphereBlock orders),
+    // fewer if the slabs would pass 8 GiB.
     const size_t per_wave = mie_slab_per_wave(nmax);
-    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * kSphereWavesPerCU, (len + 63) / 64);
+    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * ctx->tune[SCATRS_TUNE_SPHERE_WAVES],
+                                      (len + 63) / 64);
     waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((8ull << 30) / per_wave)));
     waves = (waves + 3) / 4 * 4;
     if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves), "scratch")))
@@ -522,10 +539,11 @@ int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const do
     int64_t nmax = 0;
     if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
         return rc;
-    // As many persistent wavefronts as the scratch holds slabs for, up to kSphereWavesPerCU
-    // per CU, in whole blocks of 4.
+    // As many persistent wavefronts as the scratch holds slabs for, up to the tuned number per
+    // CU, in whole blocks of 4.
     const size_t per_wave = mie_slab_per_wave(nmax);
-    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * kSphereWavesPerCU, (len + 63) / 64);
+    int64_t waves = std::min<int64_t>((int64_t)ctx->cus * ctx->tune[SCATRS_TUNE_SPHERE_WAVES],
+                                      (len + 63) / 64);
     waves = (waves + 3) / 4 * 4;
     waves = std::min<int64_t>(waves, (int64_t)(slab_bytes / per_wave));
     waves = waves / 4 * 4;
