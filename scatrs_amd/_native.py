@@ -73,6 +73,11 @@ def load() -> C.CDLL:
                 f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`"
                 " or `python -m scatrs_amd.build`")
         try:
+            # torch's HIP runtime first: the library then binds to it, where loading the
+            # library first would bring a second HIP and HSA runtime into the process, and the
+            # one initialized second finds no device
+            import torch  # noqa: F401
+
             L = C.CDLL(str(LIB_PATH))
         except OSError as e:
             raise NativeLibraryError(f"cannot load {LIB_PATH}: {e}") from e

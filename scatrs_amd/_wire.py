@@ -42,6 +42,7 @@ import copy
 
 import numpy as np
 
+from .ensemble import segment_sum
 from .errors import InvalidInputError
 from .size import (SEGMENT, Gamma, LogNormal, ModifiedGamma, Monodisperse, MultiModal,
                    PowerLaw, Tabulated)
@@ -287,7 +288,7 @@ class _Leaves:
 
 def _parts(units, leaves: _Leaves):
     """Every part (unit, kind, p (B, 3), m_real, m_imag, fractions, number fraction (B,),
-    layers) of the units: a component's distribution, a multimodal law's modes
+    layers, (component, mode)) of the units: a component's distribution, a multimodal law's modes
     (engine.rs:143-160) or a tabulated law's segments, with the number fractions of the
     reference's mixing (engine.rs:2131-2133): the component's weight over the unit's total
     times the mode's weight over its component's total, times a segment's share of its
@@ -312,10 +313,11 @@ def _parts(units, leaves: _Leaves):
                     for (r0, r1, tilt), ws in zip(segs, w):
                         p = torch.tensor([r0, r1, tilt], dtype=torch.float64,
                                          device=f.device).expand(f.shape[0], 3)
-                        out.append((u, SEGMENT, p, m_re, m_im, fr, f * ws, comp.layers))
+                        out.append((u, SEGMENT, p, m_re, m_im, fr, f * ws, comp.layers,
+                                    (c, j)))
                 else:
                     out.append((u, d.kind, leaves.dist[u, c, j], m_re, m_im, fr, f,
-                                comp.layers))
+                                comp.layers, (c, j)))
     return out
 
 
@@ -361,18 +363,15 @@ def _evaluate(units, wl: np.ndarray, *, R: int, grad: bool, order, mu, normalize
     unit = torch.tensor([part[0] for part in parts], device=dev)
     f = torch.stack([part[6] for part in parts])  # (parts, B)
     sc_parts = torch.stack(sc_of)
-    avg = torch.zeros((U, B, 7), dtype=torch.float64, device=dev).index_add(
-        0, unit, f[..., None] * number_averages(sc_parts))
+    avg = segment_sum(unit, f[..., None] * number_averages(sc_parts), U)
     gr = None
     if order is not None:
         g = torch.stack(gr_of)
-        gr = torch.zeros((U,) + g.shape[1:], dtype=torch.float64, device=dev).index_add(
-            0, unit, f[..., None, None] * g)
+        gr = segment_sum(unit, f[..., None, None] * g, U)
     ph = None
     if phase:
         q = torch.stack(ph_of)
-        ph = torch.zeros((U,) + q.shape[1:], dtype=torch.float64, device=dev).index_add(
-            0, unit, f[..., None, None] * q)
+        ph = segment_sum(unit, f[..., None, None] * q, U)
     sc, gr, ph = assemble(avg, gr, ph, normalized)
     return leaves, sc, gr, ph, sc_parts.detach(), [part[0] for part in parts]
 
@@ -479,6 +478,27 @@ def _dict(V, E, D, DE, conv, counts, n_per_unit: int, W: int, n_mu: int):
     return out
 
 
+def _refined_derivatives(keys, cj, layers: int):
+    """The derivatives pass 1 refines for the part (component, mode) ``cj`` (engine.rs:158-178):
+    its distribution's requested parameters (indices into its p) and its particle's requested
+    directions (2 l, 2 l + 1: Re m, Im m of layer l; 2 L + j: radius fraction j). Weights are
+    only integrated."""
+    c, j = cj
+    theta, dirs = [], []
+    for (kc, kj, what, i) in keys or ():
+        if kc != c:
+            continue
+        if what == "dist" and kj == j:
+            theta.append(i)
+        elif what == "m_real":
+            dirs.append(2 * i)
+        elif what == "m_imag":
+            dirs.append(2 * i + 1)
+        elif what == "fraction":
+            dirs.append(2 * layers + i)
+    return tuple(theta), tuple(sorted(dirs))
+
+
 def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                   refine_backscatter, n_angles, allow_unvalidated):
     """Values and errors (U, n, K), derivatives and their errors (P, K, n) for unit 0 (None
@@ -502,14 +522,15 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
         leaves = _Leaves(units, n, R, dev, bool(keys))
         parts = _parts(units, leaves)
         tasks, params = [], []
-        for (_, kind, p, m_re, m_im, fr, _, L) in parts:
+        for (u, kind, p, m_re, m_im, fr, _, L, cj) in parts:
             p0 = p.detach().cpu().numpy()
             m0 = (m_re.detach() + 1j * m_im.detach()).cpu().numpy()
             f0 = fr.detach().cpu().numpy()
+            theta, dirs = _refined_derivatives(keys if u == 0 else (), cj, L)
             assert p.shape[0] == B  # rows b = r n + i: wavelength i's replicas are [i::n]
             for i in range(n):
                 tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
-                                           f0[i].copy()))
+                                           f0[i].copy(), theta=theta, dirs=dirs))
                 params.append((p[i::n], m_re[i::n], m_im[i::n], fr[i::n] if L > 1 else None))
         adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
         unit = torch.tensor([part[0] for part in parts], device=dev)
@@ -534,8 +555,7 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
             per = lambda a: None if a is None else a.reshape(  # noqa: E731
                 (P, n, R) + a.shape[2:]).transpose(1, 2).reshape((P, B) + a.shape[2:])
             avg, gr, ph = per(avg), per(gr), per(ph)
-            mix = lambda a, w: torch.zeros((U,) + a.shape[1:], dtype=a.dtype,  # noqa: E731
-                                           device=dev).index_add(0, unit, w * a)
+            mix = lambda a, w: segment_sum(unit, w * a, U)  # noqa: E731
             sc_u, gr_u, ph_u = assemble(mix(avg, f[..., None]),
                                         None if gr is None else mix(gr, f[..., None, None]),
                                         None if ph is None else mix(ph, f[..., None, None]),
@@ -613,8 +633,7 @@ def _error_bounds(rows, kept, unit, U, P, n, R, normalized):
         if a is None:
             return None
         d = (a - b).abs()
-        return torch.zeros((T * R,) + d.shape[1:], dtype=d.dtype, device=dev).index_add(
-            0, rk["key"], d)
+        return segment_sum(rk["key"], d, T * R)
 
     z = rk["z"]
     e_lin = task_err(rk["lin"], rg["lin"]) / z[:, None]
@@ -626,8 +645,7 @@ def _error_bounds(rows, kept, unit, U, P, n, R, normalized):
     e_lin[..., 5:] = 0.0  # <r^2>, <r^3>: the moment integration's, errors below rtol/10
     vals = lambda a: None if a is None else a.reshape((P, R, n) + a.shape[2:])[:, 0]  # noqa
     fa = f.reshape(P, R, n)[:, 0].abs()
-    mix = lambda a, w: torch.zeros((U,) + a.shape[1:], dtype=a.dtype,  # noqa: E731
-                                   device=dev).index_add(0, unit, w * a)
+    mix = lambda a, w: segment_sum(unit, w * a, U)  # noqa: E731
     av = mix(vals(avg), f.reshape(P, R, n)[:, 0][..., None])
     ea = mix(e_lin, fa[..., None])
     ext, sca, absb, back, scag = (av[..., k] for k in range(5))
@@ -658,7 +676,7 @@ def _error_bounds(rows, kept, unit, U, P, n, R, normalized):
     return torch.cat(cols, -1)
 
 
-def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidated):
+def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidated, keys=()):
     """The reference's every order (GreekOrder::All, engine.rs:2034-2037): twice the largest N
     at each (unit, wavelength)'s final nodes, from a pass 1 of its own, (U, n)."""
     import torch
@@ -671,13 +689,14 @@ def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidat
     with torch.no_grad():
         parts = _parts(units, _Leaves(units, n, 1, dev, False))
         tasks = []
-        for (_, kind, p, m_re, m_im, fr, _, _) in parts:
+        for (u, kind, p, m_re, m_im, fr, _, L, cj) in parts:
             p0 = p.cpu().numpy()
             m0 = (m_re + 1j * m_im).cpu().numpy()
             f0 = fr.cpu().numpy()
+            theta, dirs = _refined_derivatives(keys if u == 0 else (), cj, L)
             for i in range(n):
                 tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
-                                           f0[i].copy()))
+                                           f0[i].copy(), theta=theta, dirs=dirs))
     adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
     lib = _native.load()
     out = np.zeros((len(units), n), np.int64)
@@ -753,7 +772,7 @@ def ensemble_compute(items, wavelengths, mixture: bool, greek, all_orders: bool,
             raise InvalidInputError("every Greek order (all_orders) needs the adaptive engine; "
                                     "the fixed path takes greek=L")
         orders = _all_orders([comps], wl, rtol, max_panels, refine_backscatter,
-                             allow_unvalidated)
+                             allow_unvalidated, chosen)
         order = int(orders.max())
     W = 0 if order is None else order + 1
     if adaptive:

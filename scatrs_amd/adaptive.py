@@ -34,6 +34,7 @@ evaluation order. Both engines meet rtol, so their results agree to about rtol, 
 from __future__ import annotations
 
 import ctypes as C
+import re
 from dataclasses import dataclass, field
 from functools import lru_cache
 
@@ -216,6 +217,10 @@ class Task:
     d_moments: np.ndarray = field(default_factory=lambda: np.zeros((3, 3)))
     done: bool = False
     converged: bool = False
+    # The derivatives pass 1 refines: the distribution parameters (indices into p) and the
+    # particle's directions (2 l, 2 l + 1: Re m, Im m of layer l; 2 L + j: radius fraction j).
+    theta: tuple = ()
+    dirs: tuple = ()
 
     @property
     def layers(self) -> int:
@@ -364,12 +369,135 @@ def _prepare(tasks, rtol: float) -> None:
         q.moments, q.d_moments = mom[i].copy(), dmom[i].copy()
 
 
+# The refined quantities of a block (engine.rs:27-51): C_ext, C_sca, C_sca g, C_abs, C_back, and
+# the quantity whose scale each takes within its block (C_abs: C_ext's, the accuracy of
+# C_ext - C_sca).
+_Q = 5
+_SCALE = np.array([0, 1, 2, 0, 4])
+MIN_WIDTH = 1e-10  # panels narrower than this (in ln r) are not split (engine.rs:62)
+_ELEMENT = re.compile(r"element (\d+): (.*)", re.S)
+
+
+def _node_values(tasks, ti, r, allow_unvalidated: bool):
+    """The pass-1 quantities of single particles (engine.rs:379-465): at each node (radius r[i]
+    of task ti[i]'s particle at its wavelength) the cross sections [C_ext, C_sca, C_sca g,
+    C_abs, C_back] (n, 5), pi r^2 times the efficiencies, and their derivatives along each
+    task's requested directions (n, D, 5) in the order of its ``dirs`` (D the most any of these
+    tasks asks for, zeros past a task's own; None when none asks), and the terms. One library
+    call per layer count: the sphere batch (scatrs_mie_efficiencies) for spheres,
+    scatrs_layered_efficiencies for layered spheres. An invalid node raises the library's error
+    with the reference's wavelength-and-radius context (engine.rs:124-131)."""
+    import torch
+
+    from . import layered, mie
+    from .errors import OutsideValidatedDomainError
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    ti = np.asarray(ti, np.int64)
+    n = r.size
+    D = max((len(tasks[k].dirs) for k in np.unique(ti)), default=0)
+    Q = np.empty((n, _Q))
+    dQ = np.zeros((n, D, _Q)) if D else None
+    wl = np.array([q.wl for q in tasks])[ti]
+    x = 2.0 * np.pi * r / wl
+    area = np.pi * r * r
+    layers = np.array([q.layers for q in tasks])[ti]
+    terms = 0
+    for L in np.unique(layers):
+        sel = np.flatnonzero(layers == L)
+        tk = ti[sel]
+        own = np.unique(tk)
+        want = any(len(tasks[k].dirs) for k in own)
+        pos = np.full(len(tasks), -1)
+        pos[own] = np.arange(own.size)
+        m = np.stack([tasks[k].m for k in own])[pos[tk]]  # (g, L)
+        try:
+            if L == 1:
+                v, jac, nt = mie.efficiencies_device(t(x[sel]), t(m[:, 0]), derivatives=want,
+                                                     allow_unvalidated=allow_unvalidated)
+            else:
+                fr = np.stack([tasks[k].fr for k in own])[pos[tk]]  # (g, L-1)
+                xl = np.concatenate([fr * x[sel, None], x[sel, None]], 1)
+                v, jac, nt = layered.efficiencies_device(t(xl), t(m), derivatives=want,
+                                                         allow_unvalidated=allow_unvalidated)
+        except (InvalidInputError, OutsideValidatedDomainError) as e:
+            hit = _ELEMENT.fullmatch(str(e))
+            if hit is None:
+                raise
+            i = sel[int(hit[1])]
+            raise type(e)(f"at wavelength {wl[i]:e}, radius {r[i]:e}: {hit[2]}") from None
+        terms += nt
+        v = v.cpu().numpy()  # q_ext, q_sca, q_abs, q_back, g
+        a = area[sel]
+        Q[sel] = (a * np.stack([v[0], v[1], v[1] * v[4], v[2], v[3]])).T
+        if not want:
+            continue
+        j = jac.cpu().numpy()
+        # every direction's Jacobian of (q_ext, q_sca, q_abs, q_back, g): (g, 2 L + L - 1, 5)
+        if L == 1:
+            dirs_all = np.stack([j[1], j[2]])
+        else:
+            ms = [j[l, 1 + h] for l in range(L) for h in (0, 1)]
+            fs = [x[sel] * j[f, 0] for f in range(L - 1)]  # x_L d/dx_f at a fixed x_L
+            dirs_all = np.stack(ms + fs)
+        dirs_all = dirs_all.transpose(2, 0, 1)
+        # the slots of each row's task
+        idx = np.full((own.size, D), -1)
+        for k, q in enumerate(own):
+            idx[k, :len(tasks[q].dirs)] = tasks[q].dirs
+        idx = idx[pos[tk]]  # (g, D)
+        rows = np.arange(sel.size)[:, None]
+        d = dirs_all[rows, np.maximum(idx, 0)]  # (g, D, 5)
+        d = np.where(idx[..., None] >= 0, d, 0.0)
+        sca, g = v[1][:, None], v[4][:, None]
+        dQ[sel] = a[:, None, None] * np.stack(
+            [d[..., 0], d[..., 1], d[..., 1] * g + sca * d[..., 4], d[..., 2], d[..., 3]], -1)
+    return Q, dQ, terms
+
+
+def _targets(tot, asum, rtol: float, D: int, H: int, n_dir, n_theta, refine_backscatter: bool):
+    """The targets (T, nq) and the end targets (T, nq) of every refined quantity from the
+    tasks' Kronrod totals ``tot`` and their sums of |integrand| ``asum`` (engine.rs:620-716):
+    the values to rtol of |C_ext| (C_ext, C_abs), |C_sca| (C_sca, C_sca g) or max(|C_back|,
+    |C_sca|) (C_back); a derivative along the particle's directions, or a distribution
+    parameter's derivative, to rtol of the integral of its absolute integrand (C_abs's: of
+    C_ext's), inf for the blocks a task does not ask for; C_back in every block only with
+    ``refine_backscatter``. An end is negligible below a tenth of its target, a distribution
+    parameter's below a tenth of rtol times its own total, floored at 1e-3 of its absolute
+    integral."""
+    T = tot.shape[0]
+    sca = np.abs(tot[:, 1])
+    tgt = np.full(tot.shape, np.inf)
+    tgt[:, :_Q] = rtol * np.stack([np.abs(tot[:, 0]), sca, sca, np.abs(tot[:, 0]),
+                                   np.maximum(np.abs(tot[:, 4]), sca)], 1)
+    end = tgt.copy()
+    for blk in range(1, 1 + D + H):
+        o = blk * _Q
+        used = n_dir > blk - 1 if blk <= D else n_theta > blk - 1 - D
+        scale = asum[:, o + _SCALE]
+        t = np.where(used[:, None], rtol * scale, np.inf)
+        tgt[:, o:o + _Q] = t
+        if blk <= D:
+            end[:, o:o + _Q] = t
+        else:
+            end[:, o:o + _Q] = np.where(used[:, None], rtol * np.maximum(
+                np.abs(tot[:, o:o + _Q]), 1e-3 * scale), np.inf)
+    if not refine_backscatter:
+        tgt[:, _Q - 1::_Q] = np.inf
+        end[:, _Q - 1::_Q] = np.inf
+    assert tgt.shape == (T, _Q * (1 + D + H))
+    return tgt, end
+
+
 def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
           allow_unvalidated: bool) -> int:
     """Places every task's panels (see the module's notes); returns the Mie terms evaluated.
 
     The panels of all tasks are one table sorted by (task, ln r); every round's decisions are
-    segment reductions over it."""
+    segment reductions over it. Each panel's 15 nodes are solved once (the Gauss rule takes 7
+    of them); its sums per refined quantity by both rules, and the Kronrod sums of the
+    absolute integrands, are formed on the host (engine.rs:482-566)."""
     terms = 0
     _prepare(tasks, rtol)
     live = np.array([q.kind != MONODISPERSE for q in tasks])
@@ -382,6 +510,13 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
         return terms
     kinds = np.array([q.kind for q in tasks])
     fixed_support = np.isin(kinds, (POWER_LAW, SEGMENT))
+    n_dir = np.array([len(q.dirs) if lv else 0 for q, lv in zip(tasks, live)])
+    n_theta = np.array([len(q.theta) if lv else 0 for q, lv in zip(tasks, live)])
+    D, H = int(n_dir.max()), int(n_theta.max())
+    theta = np.full((T, max(H, 1)), -1)
+    for i in ids:
+        theta[i, :n_theta[i]] = tasks[i].theta
+    nq = _Q * (1 + D + H)
     t_lo = np.array([q.t_lo for q in tasks])
     t_hi = np.array([q.t_hi for q in tasks])
     g = np.linspace(0.0, 1.0, _INITIAL_PANELS + 1)
@@ -389,86 +524,96 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
     task = np.repeat(ids, _INITIAL_PANELS)
     a = edges[:, :-1].reshape(-1)
     b = edges[:, 1:].reshape(-1)
-    K = np.full((a.size, 5), np.nan)
-    G = np.full((a.size, 5), np.nan)
+    K, G, A = (np.full((a.size, nq), np.nan) for _ in range(3))
     done = ~live
     conv = ~live
     fold = np.where(_WGF > 0, _WGF / _WK, 0.0)[None, :]
     finished = []
-    for _ in range(_MAX_ROUNDS):
+    for rnd in range(_MAX_ROUNDS):
         pending = np.flatnonzero(np.isnan(K[:, 0]) & ~done[task])
         if pending.size:
             ti = task[pending]
-            r, wk = _nodes(tasks, ti, a[pending], b[pending], np.arange(15))
-            # One call for both rules: the Gauss rows are the same 15 nodes with the Gauss
-            # weights (0 at the Kronrod-only nodes), which halves the calls of a round.
-            nodes = np.concatenate([np.stack([r, wk], -1), np.stack([r, wk * fold], -1)])
-            sc, _, _, z, nt = _call(tasks, np.concatenate([ti, ti]), nodes,
-                                    allow_unvalidated=allow_unvalidated)
+            n = pending.size
+            out = _nodes(tasks, ti, a[pending], b[pending], np.arange(15), derivatives=H > 0)
+            r, wk = out[0], out[1]
+            q, dq, nt = _node_values(tasks, np.repeat(ti, 15), r.reshape(-1),
+                                     allow_unvalidated)
             terms += nt
-            v = _refined(sc, z)
-            K[pending], G[pending] = v[:pending.size], v[pending.size:]
+            q = q.reshape(n, 15, _Q)
+            f = [q]
+            if D:  # these panels' tasks may ask for fewer directions than the most asked
+                d = np.zeros((n * 15, D, _Q))
+                if dq is not None:
+                    d[:, :dq.shape[1]] = dq
+                f.append(d.reshape(n, 15, D * _Q))
+            for h in range(H):  # d u / d theta = u d ln u / d theta
+                k = theta[ti, h]
+                dl = out[2][np.arange(n)[:, None], np.arange(15)[None, :], np.maximum(k, 0)[:, None]]
+                f.append(q * np.where(k >= 0, 1.0, 0.0)[:, None, None] * dl[..., None])
+            f = np.concatenate(f, -1)  # (n, 15, nq): the integrands per unit weight
+            K[pending] = np.einsum("nk,nkq->nq", wk, f)
+            G[pending] = np.einsum("nk,nkq->nq", wk * fold, f)
+            A[pending] = np.einsum("nk,nkq->nq", wk, np.abs(f))
         act = ~done[task]
         if not act.any():
             break
-        # Per task: totals, error sums and targets (engine.rs:620-690).
+        # Per task: totals, error sums and targets (engine.rs:620-716).
         E = np.abs(K - G)
         ta = task[act]
-        tot = np.stack([np.bincount(ta, K[act, j], minlength=T) for j in range(5)], 1)
-        esum = np.stack([np.bincount(ta, E[act, j], minlength=T) for j in range(5)], 1)
-        sca = np.abs(tot[:, 1])
-        tgt = rtol * np.stack([np.abs(tot[:, 0]), sca, sca, np.abs(tot[:, 0]),
-                               np.maximum(np.abs(tot[:, 4]), sca)], 1)
-        if not refine_backscatter:
-            tgt[:, 4] = np.inf
+        seg = lambda v: np.stack([np.bincount(ta, v[act, j], minlength=T)  # noqa: E731
+                                  for j in range(nq)], 1)
+        tot, esum, asum = seg(K), seg(E), seg(A)
+        tgt, end_tgt = _targets(tot, asum, rtol, D, H, n_dir, n_theta, refine_backscatter)
         fail = esum > tgt
         n_pan = np.bincount(task, minlength=T)
         first = np.ones(task.size, bool)
         first[1:] = task[1:] != task[:-1]
         last = np.ones(task.size, bool)
         last[:-1] = task[1:] != task[:-1]
-        edge_big = (np.abs(K) > 0.1 * tgt[task]).any(1) & ~fixed_support[task]
+        edge_big = (np.abs(K) > 0.1 * end_tgt[task]).any(1) & ~fixed_support[task]
         grow_lo = np.zeros(T, bool)
         grow_hi = np.zeros(T, bool)
         grow_lo[task[first & edge_big]] = True
         grow_hi[task[last & edge_big]] = True
-        ok = ~fail.any(1) & ~grow_lo & ~grow_hi
-        newly = ok & ~done
+        has_fail = fail.any(1) & ~done
+        wants = (has_fail | grow_lo | grow_hi) & ~done
+        newly = ~wants & ~done
         conv |= newly
         done |= newly
-        # Panels to bisect: error over the target's share, and each failing task's worst.
-        tg = np.where(tgt > 0, tgt, 1.0)
-        ratio = np.where(fail[task], E / tg[task], 0.0).max(1)
-        cut = (ratio > 1.0 / n_pan[task]) & ~done[task]
-        worst = np.full(T, -1.0)
-        starts = np.flatnonzero(first)  # the table is sorted by task
-        worst[task[starts]] = np.maximum.reduceat(ratio, starts)
-        has_fail = fail.any(1) & ~done
-        pick = np.flatnonzero(has_fail[task] & (ratio == worst[task]))
-        # the first panel at each failing task's largest ratio
-        cut[pick[np.unique(task[pick], return_index=True)[1]]] = True
-        n_cut = np.bincount(task[cut], minlength=T)
-        over = (n_pan + n_cut + grow_lo + grow_hi > max_panels) & ~done
-        done |= over
-        cut &= ~done[task]
         grow_lo &= ~done
         grow_hi &= ~done
-        if not (cut.any() or grow_lo.any() or grow_hi.any()):
-            if done[ids].all():
-                break
-            continue
+        # Within max_panels: the range extensions first, then the panels whose error exceeds
+        # their share of a missed target, worst first (engine.rs:881-909); none narrower than
+        # MIN_WIDTH.
+        room = max_panels - n_pan
+        grow_lo &= room >= 1
+        room = room - grow_lo
+        grow_hi &= room >= 1
+        room = room - grow_hi
+        tg = np.where(tgt > 0, tgt, np.finfo(float).tiny)
+        ratio = np.where(fail[task], E / tg[task], 0.0).max(1)
+        cand = np.flatnonzero((ratio * n_pan[task] > 1.0) & ~done[task] & (b - a > MIN_WIDTH))
+        order = cand[np.lexsort((-ratio[cand], task[cand]))]  # by task, worst first
+        ot = task[order]
+        rank = np.arange(order.size) - np.searchsorted(ot, ot)
+        cut = np.zeros(task.size, bool)
+        cut[order[rank < room[ot]]] = True
+        n_cut = np.bincount(task[cut], minlength=T)
+        stuck = wants & (n_cut == 0) & ~grow_lo & ~grow_hi  # out of panels: unconverged
+        done |= stuck
+        if not (cut.any() or grow_lo.any() or grow_hi.any()) or rnd == _MAX_ROUNDS - 1:
+            break  # after the last round the open tasks stay unconverged, every panel evaluated
         fin = done[task]  # finished tasks' panels leave the table: later rounds skip them
         if fin.any():
-            finished.append((a[fin], b[fin], task[fin], K[fin], G[fin]))
-            a, b, task, K, G, cut, first, last = (
-                x[~fin] for x in (a, b, task, K, G, cut, first, last))
+            finished.append((a[fin], b[fin], task[fin], K[fin], G[fin], A[fin]))
+            a, b, task, K, G, A, cut, first, last = (
+                x[~fin] for x in (a, b, task, K, G, A, cut, first, last))
         mid = 0.5 * (a + b)
         keep = ~cut
+        nc = int(cut.sum())
         na = [a[keep], a[cut], mid[cut]]
         nb = [b[keep], mid[cut], b[cut]]
         nt_ = [task[keep], task[cut], task[cut]]
-        nk = [K[keep], np.full((2 * cut.sum(), 5), np.nan)]
-        ng = [G[keep], np.full((2 * cut.sum(), 5), np.nan)]
         lo_t = np.flatnonzero(grow_lo)
         hi_t = np.flatnonzero(grow_hi)
         if lo_t.size:
@@ -483,16 +628,14 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             nb.append(t_hi[hi_t] + w)
             nt_.append(hi_t)
             t_hi[hi_t] += w
-        grown = lo_t.size + hi_t.size
-        nk.append(np.full((grown, 5), np.nan))
-        ng.append(np.full((grown, 5), np.nan))
+        fresh = np.full((2 * nc + lo_t.size + hi_t.size, nq), np.nan)
+        K, G, A = (np.concatenate([v[keep], fresh]) for v in (K, G, A))
         a, b, task = np.concatenate(na), np.concatenate(nb), np.concatenate(nt_)
-        K, G = np.concatenate(nk), np.concatenate(ng)
         order = np.lexsort((a, task))
-        a, b, task, K, G = a[order], b[order], task[order], K[order], G[order]
+        a, b, task, K, G, A = (v[order] for v in (a, b, task, K, G, A))
     if finished:
-        a, b, task, K, G = (np.concatenate([f[j] for f in finished] + [x])
-                            for j, x in enumerate((a, b, task, K, G)))
+        a, b, task, K, G, A = (np.concatenate([f[j] for f in finished] + [x])
+                               for j, x in enumerate((a, b, task, K, G, A)))
         order = np.lexsort((a, task))
         a, b, task, K, G = a[order], b[order], task[order], K[order], G[order]
     bounds = np.searchsorted(task, np.arange(T + 1))
@@ -688,7 +831,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     ones): a task list doubled this way gets both rules from one call per angle rule."""
     import torch
 
-    from .ensemble import ensemble_fixed, ensemble_nodes, number_averages
+    from .ensemble import ensemble_fixed, ensemble_nodes, number_averages, segment_sum
 
     dev = params[0][0].device
     T = len(tasks)
@@ -728,17 +871,17 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     def add(key, z, sc, g, p_, place):
         nonlocal avg, zt, gr, ph
         lin = z[:, None] * number_averages(sc)
-        avg = avg.index_add(0, key, lin)
-        zt = zt.index_add(0, key, z)
+        avg = avg + segment_sum(key, lin, T * R)
+        zt = zt + segment_sum(key, z, T * R)
         row_lin[place] = lin.detach()
         row_key[place] = key
         if gr is not None:
             gz = z[:, None, None] * g
-            gr = gr.index_add(0, key, gz)
+            gr = gr + segment_sum(key, gz, T * R)
             row_gr[place] = gz.detach()
         if ph is not None:
             pz = z[:, None, None] * p_
-            ph = ph.index_add(0, key, pz)
+            ph = ph + segment_sum(key, pz, T * R)
             row_ph[place] = pz.detach()
 
     wl_all = np.array([q.wl for q in tasks])

@@ -443,6 +443,32 @@ def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *, 
     return _FUNCTION.apply(p0, p1, p2, m_real, m_imag, fractions, kind, wavelength, opts)
 
 
+def segment_sum(index, src, n: int):
+    """(n, ...) the sums of the rows of ``src`` by ``index`` (a tensor of row numbers in
+    [0, n)), bit for bit the same on every call: the GPU's index_add adds a row's terms by
+    atomics in any order, so its sums may differ in the last bit from call to call. Here the
+    rows are sorted by index (stably), laid out one segment per row of a padded table, and
+    summed along it; every scatter writes each place once. Differentiable."""
+    import torch
+
+    out = src.new_zeros((n,) + tuple(src.shape[1:]))
+    if src.shape[0] == 0:
+        return out
+    index = torch.as_tensor(index, device=src.device)
+    order = torch.argsort(index, stable=True)
+    ks = index[order]
+    uk, counts = torch.unique_consecutive(ks, return_counts=True)
+    width = int(counts.max())
+    if width == 1:
+        return out.index_add(0, ks, src[order])
+    start = torch.cumsum(counts, 0) - counts
+    seg = torch.repeat_interleave(torch.arange(uk.numel(), device=src.device), counts)
+    pos = torch.arange(ks.numel(), device=src.device) - start[seg]
+    table = src.new_zeros((uk.numel(), width) + tuple(src.shape[1:]))
+    table = table.index_put((seg, pos), src[order])
+    return out.index_add(0, uk, table.sum(1))
+
+
 def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
     """A mixture of parts, as the reference's engine mixes them
     (scatrs/src/ensemble/engine.rs:2120-2150): each part's number averages, taken from its
@@ -642,8 +668,7 @@ def mix_units(scalars, greek, phase, unit, fractions, n_units: int, normalized: 
     u = torch.as_tensor(np.asarray(unit), device=dev)
     f = torch.as_tensor(np.asarray(fractions, np.float64), device=dev)
     f = f.reshape(f.shape + (1,) * (scalars.dim() - 1))
-    add = lambda a, w: torch.zeros((n_units,) + a.shape[1:], dtype=a.dtype,  # noqa: E731
-                                   device=dev).index_add(0, u, w * a)
+    add = lambda a, w: segment_sum(u, w * a, n_units)  # noqa: E731
     avg = add(number_averages(scalars), f)
     gr = None if greek is None else add(greek, f[..., None])
     ph = None if phase is None else add(phase, f[..., None])

@@ -99,20 +99,23 @@ def test_integration_range_cuts_the_area_tails_at_rtol_over_10():
         size_range(0, (0.3, 0.9, 0.0))
 
 
-def _stub_call(q_of_r):
-    """A stand-in for the device call of the adaptive engine: each row's normalized=False
-    scalars as averages of closed-form per-node quantities q_of_r(r) -> (C_ext, C_sca, C_abs,
-    C_back, g) over the row's weights."""
+def _stub_nodes(q_of_r, dq_of_r=None):
+    """A stand-in for pass 1's device call (adaptive._node_values): closed-form per-node
+    quantities q_of_r(r) -> (C_ext, C_sca, C_abs, C_back, g), and with dq_of_r(r) -> the same
+    five along one direction of the particle, their derivatives."""
 
-    def call(tasks, ti, nodes, **kw):
-        r, w = nodes[..., 0], nodes[..., 1]
-        z = w.sum(1)
+    def call(tasks, ti, r, allow_unvalidated):
         ext, sca, absb, back, g = q_of_r(r)
-        avg = lambda q: (w * q).sum(1) / np.where(z > 0, z, 1.0)  # noqa: E731
-        sc = np.zeros((r.shape[0], 11))
-        sc[:, 0], sc[:, 1], sc[:, 2], sc[:, 3] = avg(ext), avg(sca), avg(absb), avg(back)
-        sc[:, 5] = avg(sca * g) / np.where(avg(sca) > 0, avg(sca), 1.0)
-        return sc, None, None, z, int(r.size)
+        Q = np.stack([ext, sca, sca * g, absb, back], 1)
+        D = max((len(tasks[k].dirs) for k in np.unique(ti)), default=0)
+        dQ = None
+        if D:
+            de, ds, da, db, dg = dq_of_r(r)
+            one = np.stack([de, ds, ds * g + sca * dg, da, db], 1)
+            has = np.array([len(tasks[k].dirs) > 0 for k in ti])
+            dQ = np.zeros((r.size, D, 5))
+            dQ[:, 0] = np.where(has[:, None], one, 0.0)
+        return Q, dQ, int(r.size)
 
     return call
 
@@ -130,7 +133,7 @@ def test_adaptive_pass1_converges_to_the_closed_form(monkeypatch):
         sca = 0.9 * ext * (1.0 + 0.3 * np.sin(40.0 * np.log(r)))
         return ext, sca, ext - sca, 0.1 * ext, np.full_like(r, 0.5)
 
-    monkeypatch.setattr(adaptive, "_call", _stub_call(q))
+    monkeypatch.setattr(adaptive, "_node_values", _stub_nodes(q))
     r_g, sigma, rtol = 0.3, 1.8, 1e-8
     s = np.log(sigma)
 
@@ -142,14 +145,17 @@ def test_adaptive_pass1_converges_to_the_closed_form(monkeypatch):
     adaptive.pass1([t], rtol, 4096, False, False)
     assert t.converged and t.a.size > 4 * 4
     assert np.all(np.diff(t.a) > 0) and np.allclose(t.a[1:], t.b[:-1], rtol=0, atol=1e-15)
-    s_ext = np.nansum(t.K[:, 0])  # C_ext sum over the panels, in the weights' scale
+    s_ext = np.sum(t.K[:, 0])  # C_ext sum over the panels, in the weights' scale
     exact = np.pi * r_g ** 2 * np.exp(2 * s * s)
     assert abs(s_ext / t.moments[0] - exact) <= 5 * rtol * exact
     err = np.abs(t.K - t.G).sum(0)
     assert err[0] <= rtol * abs(t.K[:, 0].sum()) and err[1] <= rtol * abs(t.K[:, 1].sum())
-    small = task()
-    adaptive.pass1([small], rtol, 8, False, False)
-    assert not small.converged and small.a.size <= 8
+    assert np.all(np.isfinite(t.K)) and np.all(np.isfinite(t.G))
+    for cap in (8, 11):  # the splits fill the budget exactly, worst first
+        small = task()
+        adaptive.pass1([small], rtol, cap, False, False)
+        assert not small.converged and small.a.size == cap
+        assert np.all(np.isfinite(small.K))
 
 
 def test_adaptive_pass1_grows_the_range_for_steep_integrands(monkeypatch):
@@ -162,7 +168,7 @@ def test_adaptive_pass1_grows_the_range_for_steep_integrands(monkeypatch):
         ext = r ** 6 + 0.01 * r * r
         return ext, r ** 6, 0.01 * r * r, r ** 6, np.zeros_like(r)
 
-    monkeypatch.setattr(adaptive, "_call", _stub_call(q))
+    monkeypatch.setattr(adaptive, "_node_values", _stub_nodes(q))
     r_g, sigma, rtol = 0.01, 2.0, 1e-7
     s = np.log(sigma)
     t = adaptive.Task(0, np.array([r_g, sigma, 0.0]), 1.0, np.array([1.5 + 0j]), np.zeros(0))
@@ -171,9 +177,84 @@ def test_adaptive_pass1_grows_the_range_for_steep_integrands(monkeypatch):
     adaptive.pass1([t, g], rtol, 4096, False, False)
     assert t.converged and g.converged
     assert t.t_hi > t_hi + 1e-9
-    sca = np.nansum(t.K[:, 1]) / t.moments[0]
+    sca = np.sum(t.K[:, 1]) / t.moments[0]
     exact = r_g ** 6 * np.exp(18 * s * s)
     assert abs(sca - exact) <= 5 * rtol * exact
+
+
+def test_adaptive_pass1_refines_the_requested_derivatives(monkeypatch):
+    """Derivative targets (engine.rs:620-716) on the CPU: a direction of the particle whose
+    derivative has Lorentzian spikes of width 0.03 in ln r (as d C / d Im m has at the
+    resonances of a nonabsorbing sphere) cannot be resolved within 64 panels, so the task ends
+    unconverged at exactly 64 panels, while the same values without that derivative converge
+    on far fewer; a pass whose budget suffices meets the derivative's target, rtol times the
+    integral of its absolute integrand."""
+    from scatrs_amd import adaptive
+
+    centers = np.linspace(np.log(0.05), np.log(3.0), 25)
+
+    def q(r):
+        ext = np.pi * r * r
+        return ext, 0.9 * ext, 0.1 * ext, 0.05 * ext, np.full_like(r, 0.6)
+
+    def dq(r):
+        t = np.log(r)[:, None]
+        spikes = (0.03 ** 2 / ((t - centers[None, :]) ** 2 + 0.03 ** 2)).sum(1)
+        d = np.pi * r * r * spikes
+        return d, 0.5 * d, 0.5 * d, 0.1 * d, np.zeros_like(r)
+
+    monkeypatch.setattr(adaptive, "_node_values", _stub_nodes(q, dq))
+    rtol = 1e-6
+
+    def task(dirs=()):
+        return adaptive.Task(0, np.array([0.3, 1.8, 0.0]), 0.5, np.array([1.33 + 0j]),
+                             np.zeros(0), dirs=dirs)
+
+    plain, wanted = task(), task(dirs=(1,))
+    adaptive.pass1([plain, wanted], rtol, 64, False, False)
+    assert plain.converged and plain.a.size < 32
+    assert not wanted.converged and wanted.a.size == 64
+    big = task(dirs=(1,))
+    adaptive.pass1([big], rtol, 4096, False, False)
+    assert big.converged and big.a.size > 64
+    blk = slice(5, 10)  # the direction's block: C_ext, C_sca, C_sca g, C_abs, C_back
+    err = np.abs(big.K[:, blk] - big.G[:, blk]).sum(0)
+    # its absolute integrands are the integrands themselves here (positive spikes)
+    absint = np.abs(big.K[:, blk]).sum(0)
+    assert np.all(err[:4] <= rtol * absint[[0, 1, 2, 0]])
+
+
+def test_adaptive_pass1_distribution_parameter_blocks(monkeypatch):
+    """The distribution-parameter blocks of pass 1 (engine.rs:528-545): with C_ext = pi r^2 the
+    lognormal's integral of u C_ext has the closed form pi r0 sqrt(2 pi) r_g^2 exp(2 s^2) (u at
+    its fixed reference radius r0 = r_g, s = ln sigma_g), so the blocks of d/d r_g and d/d
+    sigma_g are that integral times 2 / r_g and 4 s / sigma_g; both are refined to rtol of
+    their absolute integrals and take at least the panels of the values alone."""
+    from scatrs_amd import adaptive
+
+    def q(r):
+        ext = np.pi * r * r
+        sca = 0.9 * ext * (1.0 + 0.3 * np.sin(12.0 * np.log(r)))
+        return ext, sca, ext - sca, 0.1 * ext, np.full_like(r, 0.5)
+
+    monkeypatch.setattr(adaptive, "_node_values", _stub_nodes(q))
+    r_g, sigma, rtol = 0.3, 1.8, 1e-9
+    s = np.log(sigma)
+    plain = adaptive.Task(0, np.array([r_g, sigma, 0.0]), 0.5, np.array([1.5 + 0j]), np.zeros(0))
+    t = adaptive.Task(0, np.array([r_g, sigma, 0.0]), 0.5, np.array([1.5 + 0j]), np.zeros(0),
+                      theta=(0, 1))
+    adaptive.pass1([plain, t], rtol, 4096, False, False)
+    assert plain.converged and t.converged and t.a.size >= plain.a.size
+    base = t.K[:, 0].sum()
+    d_rg, d_sigma = t.K[:, 5].sum(), t.K[:, 10].sum()
+    # the range holds all but ~rtol/10 of the area; the derivative blocks' heavier tails
+    # (z^2 u) lose a few times that
+    assert abs(d_rg / base - 2.0 / r_g) <= 10 * rtol * 2.0 / r_g
+    assert abs(d_sigma / base - 4.0 * s / sigma) <= 10 * rtol * 4.0 * s / sigma
+    for o in (5, 10):
+        err = np.abs(t.K[:, o:o + 4] - t.G[:, o:o + 4]).sum(0)
+        absint = np.abs(t.K[:, o:o + 4]).sum(0)  # a lower bound of the absolute integrals
+        assert np.all(err <= rtol * absint[[0, 1, 2, 0]] * 1.0001)
 
 
 def test_adaptive_pass2_rows_recombine_to_the_panel_sums(monkeypatch):
@@ -247,3 +328,26 @@ def test_log_density_on_host_threads_equals_the_serial_loop():
     kind[-1] = 0
     with pytest.raises(InvalidInputError):
         log_density(kind, p, r)
+
+
+def test_segment_sum_matches_index_add_and_its_gradient():
+    """ensemble.segment_sum, the deterministic stand-in for index_add: the same sums (to
+    rounding, in another order), unique indices exact, empty input, and the gradient a gather
+    of the output's."""
+    import torch
+
+    from scatrs_amd.ensemble import segment_sum
+
+    rng = np.random.default_rng(0)
+    idx = torch.from_numpy(rng.integers(0, 7, 50))
+    src = torch.from_numpy(rng.normal(size=(50, 3, 2))).requires_grad_(True)
+    out = segment_sum(idx, src, 9)
+    ref = torch.zeros(9, 3, 2, dtype=torch.float64).index_add(0, idx, src.detach())
+    assert torch.allclose(out, ref, rtol=1e-14, atol=1e-14)
+    assert torch.all(out[7:] == 0.0)
+    w = torch.arange(54.0, dtype=torch.float64).reshape(9, 3, 2)
+    (g,) = torch.autograd.grad((out * w).sum(), src)
+    assert torch.equal(g, w[idx])
+    one = segment_sum(torch.tensor([2, 0]), torch.tensor([1.5, 2.5], dtype=torch.float64), 3)
+    assert one.tolist() == [2.5, 0.0, 1.5]
+    assert segment_sum(torch.zeros(0, dtype=torch.int64), torch.ones(0), 2).tolist() == [0, 0]
