@@ -318,6 +318,12 @@ int scatrs_size_range(int32_t kind, const double *p, double *t_lo, double *t_hi)
 int scatrs_size_message(int32_t kind, const double *p, char *buf, int64_t cap);
 int scatrs_size_log_density(int64_t n, const int32_t *kind, const double *p, const double *r,
                             double *ln_u, double *d_ln_u);
+/* The n-point Gauss-Legendre rule on [-1, 1] the kernels use (radius nodes in ln r, the
+ * angular rule, and the cosines of the phase matrix at n_angles), nodes ascending and exactly
+ * symmetric, into host arrays of n doubles; 1 <= n <= 65536, the largest rule the library
+ * builds. Replaces scatrs.special.gauss_legendre (python/scatrs/special.py:31-68,
+ * scatrs/src/special/gauss_legendre.rs:210-216) for the rules of this path. Host only. */
+int scatrs_gauss_legendre(int64_t n, double *nodes, double *weights);
 
 #ifdef __cplusplus
 }

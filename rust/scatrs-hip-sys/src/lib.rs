@@ -136,6 +136,8 @@ pub mod ffi {
             -> c_int;
         pub fn scatrs_size_log_density(n: i64, kind: *const i32, p: *const f64, r: *const f64,
             ln_u: *mut f64, d_ln_u: *mut f64) -> c_int;
+        /// The library's n-point Gauss-Legendre rule (1 <= n <= 65536) into host arrays.
+        pub fn scatrs_gauss_legendre(n: i64, nodes: *mut f64, weights: *mut f64) -> c_int;
     }
 }
 

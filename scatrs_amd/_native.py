@@ -56,6 +56,7 @@ SYMBOLS = (
     "scatrs_ensemble_nodes_vjp",
     "scatrs_size_range",
     "scatrs_size_log_density",
+    "scatrs_gauss_legendre",
     "scatrs_size_message",
 )
 
@@ -133,6 +134,7 @@ def load() -> C.CDLL:
             L.scatrs_size_range.argtypes = [i32, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
             L.scatrs_size_log_density.argtypes = [i64, C.POINTER(C.c_int32), C.POINTER(dbl),
                                                   C.POINTER(dbl), C.POINTER(dbl), vp]
+        L.scatrs_gauss_legendre.argtypes = [i64, C.POINTER(dbl), C.POINTER(dbl)]
         if hasattr(L, "scatrs_size_message"):
             L.scatrs_size_message.argtypes = [i32, C.POINTER(dbl), C.c_char_p, i64]
         if hasattr(L, "scatrs_ensemble_nodes_vjp"):

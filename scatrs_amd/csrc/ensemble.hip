@@ -291,6 +291,15 @@ extern "C" int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_n
                             allow_unvalidated, d_scalars, d_greek, d_phase, total_terms);
 }
 
+extern "C" int scatrs_gauss_legendre(int64_t n, double *nodes, double *weights) {
+    if (n < 1 || n > 65536 || !nodes || !weights) return SCATRS_INVALID_INPUT;
+    std::vector<double> x, w;
+    gauss_legendre(n, x, w);
+    std::memcpy(nodes, x.data(), n * sizeof(double));
+    std::memcpy(weights, w.data(), n * sizeof(double));
+    return SCATRS_OK;
+}
+
 extern "C" int scatrs_size_message(int32_t kind, const double *p, char *buf, int64_t cap) {
     if (!p || !buf || cap < 1) return SCATRS_INVALID_INPUT;
     const std::string msg = dist_message(kind, p);

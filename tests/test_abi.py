@@ -59,3 +59,25 @@ def test_rust_binding_declares_every_symbol():
                           ("SCATRS_OUTSIDE_VALIDATED_DOMAIN", "OutsideValidatedDomain"),
                           ("SCATRS_NOT_CONVERGED", "NotConverged")):
         assert re.search(code + r" => Err\(Error::" + variant + r"\(", text), code
+
+
+def test_library_gauss_legendre_rule():
+    """scatrs_gauss_legendre, the rule the kernels use (host only): nodes as numpy's
+    Golub-Welsch rule's to 2^-52 (whose weights are the less accurate ones: 3e-11 relative at
+    n = 200), exactly symmetric, and the weights pinned by exactness: x^k integrated to
+    2/(k+1) for even k up to 2n - 2; n outside [1, 65536] rejected."""
+    import pytest
+
+    from scatrs_amd.errors import InvalidInputError
+    from scatrs_amd.quadrature import gauss_legendre
+
+    for n in (1, 2, 7, 64, 200):
+        x, w = gauss_legendre(n)
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.max(np.abs(x - xr)) <= 4e-16
+        assert np.array_equal(x, -x[::-1]) and np.array_equal(w, w[::-1])
+        for k in sorted({0, min(2, 2 * n - 2), n // 2 * 2, 2 * n - 2}):
+            assert abs((w * x ** k).sum() - 2.0 / (k + 1)) <= 1e-14, (n, k)
+    for bad in (0, 65537):
+        with pytest.raises(InvalidInputError):
+            gauss_legendre(bad)
