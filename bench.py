@@ -33,7 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of mie_batch on this workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # two rocprofv3 --pmc runs, scaled by what the counters read for a 1 GiB device copy
 # (profiles/mie_batch_spheres_2p20.md).
-MIE_BATCH_TRAFFIC_BYTES = 1.207e9
+MIE_BATCH_TRAFFIC_BYTES = 1.627e9
+# mie_batch on the same workload under rocprofv3 --kernel-trace --stats
+# (profiles/mie_batch_spheres_2p20.md): the average duration of its launches.
+MIE_BATCH_ROCPROF = {"kernel_ms": 1.0118, "source": "profiles/mie_batch_spheres_2p20.md"}
 
 
 # The ensemble legs after the sphere batch, in the order they run.
@@ -47,6 +50,21 @@ def log(msg: str) -> None:
 def cpu_threads() -> int:
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return max(1, min(16, n or 1))
+
+
+def host_info() -> dict:
+    """The host the CPU baselines ran on: logical CPUs, those this process may use, the model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"nproc": os.cpu_count(), "usable": usable, "cpu_model": model}
 
 
 def bench_spheres(args, torch):
@@ -100,14 +118,26 @@ def bench_spheres(args, torch):
         c0 = time.perf_counter()
         oracle.mie_efficiencies_batch(x, m, threads=th)
         c1 = time.perf_counter()
+        # one thread on every 16th sphere (the same mix of sizes)
+        sub = slice(0, None, 16)
+        sub_terms, _ = workloads.mie_cost(x[sub], m[sub])
+        c2 = time.perf_counter()
+        oracle.mie_efficiencies_batch(x[sub], m[sub], threads=1)
+        c3 = time.perf_counter()
         cpu = {"value": terms / (c1 - c0), "unit": "terms/s", "cores": th, "kind": "port",
-               "sample": f"all 2^20 spheres of the workload, one pass ({c1 - c0:.2f} s)"}
-        log(f"cpu baseline: {cpu['value']:.4e} terms/s on {th} threads")
-    roof = {"bound": "mfma", "pipe": "fp64 (vector; MI355X's FP64 vector and matrix peaks "
-            "are both 78.6 TFLOP/s)", "kernel": "mie_batch",
+               "sample": f"all 2^20 spheres of the workload, one pass ({c1 - c0:.2f} s); one "
+                         f"thread: every 16th sphere ({c3 - c2:.2f} s)",
+               "one_thread": sub_terms / (c3 - c2), "host": host_info()}
+        log(f"cpu baseline: {cpu['value']:.4e} terms/s on {th} threads, "
+            f"{cpu['one_thread']:.4e} on 1")
+    roof = {"bound": "fp64-valu", "pipe": "FP64 vector ALU (MI355X's FP64 vector peak, "
+            "78.6 TFLOP/s)", "kernel": "mie_batch",
             "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": args.traffic,
-            "kernel_ms": k_ms, "flops_per_launch": flops}
+            "kernel_ms": k_ms, "flops_per_launch": flops,
+            "rocprof": dict(MIE_BATCH_ROCPROF,
+                            frac=flops / (MIE_BATCH_ROCPROF["kernel_ms"] * 1e-3) / 1e12
+                            / FP64_PEAK_TFLOPS)}
     return {
         "metric": METRIC, "value": value, "unit": "terms/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -226,7 +256,7 @@ def bench_c1(args, torch):
             base[th] = 3 * reps / (time.perf_counter() - c0)
         th = max(base)
         res["cpu_baseline"] = {"value": base[th], "unit": "ensembles/s", "cores": th,
-                               "kind": "port", "one_thread": base[1],
+                               "kind": "port", "one_thread": base[1], "host": host_info(),
                                "sample": "the 3 ensembles, repeated for 2 s per thread count"}
         log(f"C1 cpu baseline: {base[th]:.1f} ensembles/s on {th} threads, {base[1]:.1f} on 1")
     return res
@@ -329,10 +359,19 @@ def bench_ensembles(args, torch):
         oracle.ensemble_batch(kind[sel], p0[sel], p1[sel], wl[sel], m[sel], nr, na, order,
                               threads=th)
         c1 = time.perf_counter()
+        one = sel[::4]
+        c2 = time.perf_counter()
+        oracle.ensemble_batch(kind[one], p0[one], p1[one], wl[one], m[one], nr, na, order,
+                              threads=1)
+        c3 = time.perf_counter()
         res["cpu_baseline"] = {"value": sel.size / (c1 - c0), "unit": "ensembles/s",
                                "cores": th, "kind": "port",
-                               "sample": f"{sel.size} of the ensembles, evenly spaced ({c1 - c0:.1f} s)"}
-        log(f"cpu baseline: {sel.size / (c1 - c0):.2f} ensembles/s on {th} threads")
+                               "sample": f"{sel.size} of the ensembles, evenly spaced "
+                                         f"({c1 - c0:.1f} s); one thread: every 4th of them "
+                                         f"({c3 - c2:.1f} s)",
+                               "one_thread": one.size / (c3 - c2), "host": host_info()}
+        log(f"cpu baseline: {sel.size / (c1 - c0):.2f} ensembles/s on {th} threads, "
+            f"{one.size / (c3 - c2):.2f} on 1")
     return res
 
 
@@ -393,7 +432,7 @@ def bench_adjoint(args, torch):
             oracle.ensemble_fixed_jacobian(0, p0[e], p1[e], wl[e], m[e], None, nr, na, order)
         c1 = time.perf_counter()
         res["cpu_baseline"] = {"value": n / (c1 - c0), "unit": "ensembles/s", "cores": 1,
-                               "kind": "port",
+                               "kind": "port", "host": host_info(),
                                "sample": f"forward-mode Jacobian (all 4 parameters) of {n} evenly "
                                          f"spaced ensembles, one thread ({c1 - c0:.1f} s)"}
     return res
@@ -675,6 +714,12 @@ def main() -> int:
         if leg in legs:
             res[leg] = globals()["bench_" + leg](args, torch)
     print(json.dumps(res), flush=True)
+    # Leave nothing behind: the device idle and the library's contexts released before the
+    # interpreter exits (bench.py itself starts no other process or thread).
+    from scatrs_amd import _native
+
+    torch.cuda.synchronize()
+    _native.release_contexts()
     return 0
 
 

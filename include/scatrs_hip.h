@@ -70,7 +70,7 @@ int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
  * times candidate values on the caller's workload and keeps the fastest; no reference
  * counterpart. Results do not depend on them, except SCATRS_TUNE_SPLIT_BLOCKS, which changes
  * the order in which a small batch's angular products are summed (rounding only).
- *   SCATRS_TUNE_SPHERE_WAVES: persistent wavefronts per CU of the sphere batch (1-32, default 8);
+ *   SCATRS_TUNE_SPHERE_WAVES: persistent wavefronts per CU of the sphere batch (1-32, default 16);
  *   SCATRS_TUNE_NODE_WAVES: the same for the ensembles' sphere node solves (1-32, default 8);
  *   SCATRS_TUNE_CHUNK_SHARE: a chunk of ensembles stages its coefficients in 1/share of the free
  *     device memory (1-64, default 4; at most 64 GiB, at least 1 GiB);

@@ -214,6 +214,17 @@ class Context:
             pass
 
 
+def release_contexts() -> None:
+    """Destroys this thread's contexts (each waits for its device work first); the next
+    context() call creates a new one."""
+    ctxs = getattr(_tls, "ctxs", None) or {}
+    _tls.ctxs = {}
+    for ctx in ctxs.values():
+        h, ctx.handle = ctx.handle, None
+        if h:
+            ctx.lib.scatrs_destroy(h)
+
+
 def context(device: int | None = None) -> Context:
     """This thread's context on `device` (default: torch's current device), bound to torch's
     current stream."""

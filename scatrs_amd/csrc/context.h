@@ -65,7 +65,7 @@ struct scatrs_ctx {
     // persistent wavefronts per CU of the sphere batch and of the ensembles' sphere node solves,
     // the share of free device memory one chunk of ensembles stages (1/share), and the blocks per
     // CU a small batch's angular products aim for when they split node groups.
-    int tune[4] = {8, 8, 4, 2};
+    int tune[4] = {16, 8, 4, 2};
     bool profiling = false;
     int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2: fp16-split MFMA
     std::vector<double> phase_mu;  // the caller's phase cosines (empty: the angle rule's)

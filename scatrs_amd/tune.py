@@ -24,9 +24,9 @@ from pathlib import Path
 from typing import Callable, Sequence
 
 KEYS = {"sphere_waves": 0, "node_waves": 1, "chunk_share": 2, "split_blocks": 3}
-DEFAULTS = {"sphere_waves": 8, "node_waves": 8, "chunk_share": 4, "split_blocks": 2}
+DEFAULTS = {"sphere_waves": 16, "node_waves": 8, "chunk_share": 4, "split_blocks": 2}
 CANDIDATES = {
-    "sphere_waves": (4, 6, 8, 12, 16),
+    "sphere_waves": (4, 6, 8, 12, 16, 20),
     "node_waves": (4, 6, 8, 12),
     "chunk_share": (2, 3, 4, 6),
     "split_blocks": (1, 2, 4, 8),

@@ -312,8 +312,9 @@ def test_phase_at_the_callers_cosines(kernel):
     finally:
         ctx.set_angular_kernel(0)
     assert torch.equal(s0, s1) and torch.equal(g0, g1)
-    # numpy's rule labels the outputs; the library's differs from it in the last bits only
-    assert float((f0 - f2).abs().max() / f0[:, 0].abs().max()) <= 1e-12
+    # the library's own cosines label the outputs: at them the caller's-cosine path gives the
+    # rule's phase matrix, its angular tables built apart (rounding only)
+    assert float((f0 - f2).abs().max() / f0[:, 0].abs().max()) <= 1e-13
     f1 = f1.cpu().numpy()
     tol = 1e-6 if kernel == 2 else 1e-10
     for e in range(3):
