@@ -9,7 +9,7 @@ from scatrs_amd import tune
 def test_coordinate_search_picks_each_minimum_and_keeps_it_applied():
     state = dict(tune.DEFAULTS)
     now = [0.0]
-    cost = {"sphere_waves": {4: 3.0, 6: 1.0, 8: 2.0, 12: 2.5, 16: 4.0},
+    cost = {"sphere_waves": {4: 3.0, 6: 1.0, 8: 2.0, 12: 2.5, 16: 4.0, 20: 4.5},
             "chunk_share": {2: 0.5, 3: 0.7, 4: 0.9, 6: 1.1}}
 
     def run():
