@@ -500,7 +500,7 @@ def _refined_derivatives(keys, cj, layers: int):
 
 
 def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
-                  refine_backscatter, n_angles, allow_unvalidated):
+                  refine_backscatter, n_angles, allow_unvalidated, shared: bool = False):
     """Values and errors (U, n, K), derivatives and their errors (P, K, n) for unit 0 (None
     without keys), converged (U, n) and panels per (unit, wavelength) by the adaptive engine
     (scatrs_amd.adaptive): pass 1 on the parameters' values, pass 2 differentiable
@@ -532,7 +532,16 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                 tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
                                            f0[i].copy(), theta=theta, dirs=dirs))
                 params.append((p[i::n], m_re[i::n], m_im[i::n], fr[i::n] if L > 1 else None))
-        adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
+        if shared and not keys:
+            # one particle for every unit: at each wavelength all parts share their panels and
+            # nodes (the reference's batch, engine.rs:257-296); task k is wavelength k % n
+            from . import shared as sh
+
+            groups = [list(range(i, len(tasks), n)) for i in range(n)]
+            sh.pass1_groups(tasks, groups, rtol, max_panels, refine_backscatter,
+                            allow_unvalidated)
+        else:
+            adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
         unit = torch.tensor([part[0] for part in parts], device=dev)
         f = torch.stack([part[6] for part in parts])  # (parts, B)
         P = len(parts)
@@ -543,6 +552,10 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                 tasks, params, R=R, idx=idx, order=order, n_angles=n_angles, mu=mu,
                 want_phase=mu is not None, allow_unvalidated=allow_unvalidated)
                 for idx in (np.arange(15), adaptive._GAUSS)]
+        elif shared:  # every shared node solved once; both rules from its values
+            passes = sh.pass2_groups(tasks, order=order, n_angles=n_angles, mu=mu,
+                                     want_phase=mu is not None,
+                                     allow_unvalidated=allow_unvalidated)
         else:  # both rules in one call per angle rule: the tasks twice, Gauss weights second
             passes = _split_rules(adaptive.pass2_torch(
                 tasks + [copy.copy(q) for q in tasks], params + params, R=R,
@@ -676,7 +689,8 @@ def _error_bounds(rows, kept, unit, U, P, n, R, normalized):
     return torch.cat(cols, -1)
 
 
-def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidated, keys=()):
+def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidated, keys=(),
+                shared: bool = False):
     """The reference's every order (GreekOrder::All, engine.rs:2034-2037): twice the largest N
     at each (unit, wavelength)'s final nodes, from a pass 1 of its own, (U, n)."""
     import torch
@@ -697,7 +711,13 @@ def _all_orders(units, wl, rtol, max_panels, refine_backscatter, allow_unvalidat
             for i in range(n):
                 tasks.append(adaptive.Task(int(kind), p0[i].copy(), float(wl[i]), m0[i].copy(),
                                            f0[i].copy(), theta=theta, dirs=dirs))
-    adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
+    if shared and not keys:
+        from .shared import pass1_groups
+
+        pass1_groups(tasks, [list(range(i, len(tasks), n)) for i in range(n)], rtol,
+                     max_panels, refine_backscatter, allow_unvalidated)
+    else:
+        adaptive.pass1(tasks, rtol, max_panels, refine_backscatter, allow_unvalidated)
     lib = _native.load()
     out = np.zeros((len(units), n), np.int64)
     for k, q in enumerate(tasks):
@@ -796,11 +816,17 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
                            normalized: bool, rtol: float, max_panels: int,
                            allow_unvalidated: bool, refine_backscatter: bool, *,
                            n_radius: int | None = None, n_angles: int | None = None,
-                           adaptive: bool | None = None) -> dict:
+                           adaptive: bool | None = None, shared_panels: bool = False) -> dict:
     """The outputs of each distribution (tagged tuples ``specs``) of one particle at each
     wavelength, without derivatives, one row per (distribution, wavelength),
     distribution-major, as the reference's ``_scatrs.ensemble_compute_batch``
-    (scatrs-py/src/ensemble.rs:480-504); every distribution in the one library call."""
+    (scatrs-py/src/ensemble.rs:480-504); every distribution in the one library call.
+
+    ``shared_panels`` (adaptive engine): at each wavelength the distributions share one panel
+    set and each node is solved once for all of them, as the reference's batch does
+    (engine.rs:257-296; scatrs_amd.shared). By default each distribution gets its own panels,
+    with the results of its single call: on the GPU the node solves are cheap and the shared
+    set's finer union panels cost more host work per distribution than they save."""
     m, fr = _particle(particle_spec)
     wl = np.ascontiguousarray(np.asarray(wavelengths, np.float64).reshape(-1))
     mu = _mu(mu)
@@ -818,7 +844,8 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
             raise InvalidInputError("every Greek order (all_orders) needs the adaptive engine; "
                                     "the fixed path takes greek=L")
         orders = _all_orders(units, wl, rtol, max_panels, refine_backscatter,
-                             allow_unvalidated) if units else np.zeros((0, wl.size), np.int64)
+                             allow_unvalidated, shared=shared_panels) if units \
+            else np.zeros((0, wl.size), np.int64)
         order = int(orders.max()) if orders.size else 0
     W = 0 if order is None else order + 1
     if not units:
@@ -831,7 +858,7 @@ def ensemble_compute_batch(specs, particle_spec, wavelengths, greek, all_orders:
         V, E, _, _, conv, panels = _run_adaptive(
             units, wl, order=order, mu=mu, normalized=normalized, keys=[], rtol=rtol,
             max_panels=max_panels, refine_backscatter=refine_backscatter, n_angles=n_angles,
-            allow_unvalidated=allow_unvalidated)
+            allow_unvalidated=allow_unvalidated, shared=shared_panels)
         if orders is not None:
             _clip_orders(V, E, None, None, orders, W, n_mu)
         out = _dict(V, E, None, None, conv, np.zeros(len(units), np.int64), wl.size, W, n_mu)

@@ -3,15 +3,19 @@
 // and uploaded; the device kernels only read them.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <thread>
 #include <vector>
 
 namespace scatrs {
 
 // n-point Gauss-Legendre rule on [-1, 1], nodes ascending and exactly symmetric
 // (scatrs/src/special/gauss_legendre.rs:210-216 gives the same rule): Newton's method on the
-// three-term recurrence of P_n in long double from Tricomi's initial guesses.
+// three-term recurrence of P_n in long double from Tricomi's initial guesses. Each node costs
+// O(n); rules of 512 points or more are computed on up to 16 host threads (started and joined
+// here), each taking every t-th node, with the same arithmetic per node whatever the count.
 inline void gauss_legendre(int64_t n, std::vector<double> &nodes, std::vector<double> &weights) {
     nodes.assign(n, 0.0);
     weights.assign(n, 0.0);
@@ -30,7 +34,7 @@ inline void gauss_legendre(int64_t n, std::vector<double> &nodes, std::vector<do
         p = p1;
         dp = (long double)n * (x * p1 - p0) / (x * x - 1.0L);
     };
-    for (int64_t i = 0; i < (n + 1) / 2; ++i) {
+    auto node = [&](int64_t i) {
         long double x = cosl(pi * ((long double)i + 0.75L) / ((long double)n + 0.5L));
         long double p = 0, dp = 1;
         for (int it = 0; it < 100; ++it) {
@@ -46,7 +50,20 @@ inline void gauss_legendre(int64_t n, std::vector<double> &nodes, std::vector<do
         nodes[i] = -(double)x;
         weights[n - 1 - i] = (double)w;
         weights[i] = (double)w;
+    };
+    const int64_t half = (n + 1) / 2;
+    const int64_t threads =
+        n < 512 ? 1 : std::min<int64_t>({16, half, std::max(1u, std::thread::hardware_concurrency())});
+    if (threads <= 1) {
+        for (int64_t i = 0; i < half; ++i) node(i);
+        return;
     }
+    std::vector<std::thread> pool;
+    for (int64_t t = 0; t < threads; ++t)
+        pool.emplace_back([&node, t, threads, half] {
+            for (int64_t i = t; i < half; i += threads) node(i);
+        });
+    for (auto &th : pool) th.join();
 }
 
 // d^l_{mn}(x) for l = 0..lmax into out (scatrs/src/special/wigner.rs:152-182), for the small

@@ -351,3 +351,73 @@ def test_segment_sum_matches_index_add_and_its_gradient():
     one = segment_sum(torch.tensor([2, 0]), torch.tensor([1.5, 2.5], dtype=torch.float64), 3)
     assert one.tolist() == [2.5, 0.0, 1.5]
     assert segment_sum(torch.zeros(0, dtype=torch.int64), torch.ones(0), 2).tolist() == [0, 0]
+
+
+def test_shared_panels_integrate_every_member(monkeypatch):
+    """Shared pass-1 panels (scatrs_amd.shared, engine.rs:257-296) on the CPU with closed-form
+    per-node quantities: three lognormals of one particle at one wavelength share one panel set;
+    each member meets rtol against its own closed form <C_ext> = pi r_g^2 exp(2 s^2), every
+    member's panels are panels of the group, and the nodes solved are fewer than three separate
+    runs solve. A panel limit leaves the members unconverged within it."""
+    from scatrs_amd import adaptive, shared
+
+    def q(r):
+        ext = np.pi * r * r
+        sca = 0.9 * ext * (1.0 + 0.3 * np.sin(40.0 * np.log(r)))
+        return ext, sca, ext - sca, 0.1 * ext, np.full_like(r, 0.5)
+
+    monkeypatch.setattr(adaptive, "_node_values", _stub_nodes(q))
+    rtol = 1e-8
+    laws = [(0.3, 1.8), (0.5, 1.4), (0.12, 2.2)]
+
+    def tasks():
+        return [adaptive.Task(0, np.array([rg, sg, 0.0]), 0.5, np.array([1.5 + 0j]), np.zeros(0))
+                for rg, sg in laws]
+
+    ts = tasks()
+    n_shared = shared.pass1_groups(ts, [[0, 1, 2]], rtol, 4096, False, False)
+    edges = set()
+    for t, (rg, sg) in zip(ts, laws):
+        assert t.converged
+        assert np.all(np.isfinite(t.K)) and np.all(np.diff(t.a) > 0)
+        s = np.log(sg)
+        exact = np.pi * rg ** 2 * np.exp(2 * s * s)
+        assert abs(t.K[:, 0].sum() / t.moments[0] - exact) <= 5 * rtol * exact
+        err = np.abs(t.K - t.G).sum(0)
+        assert err[0] <= rtol * abs(t.K[:, 0].sum()) and err[1] <= rtol * abs(t.K[:, 1].sum())
+        edges |= set(zip(t.a.tolist(), t.b.tolist()))
+    # one panel set: any two members' panels either coincide or do not overlap
+    e = sorted(edges)
+    for (a0, b0), (a1, b1) in zip(e[:-1], e[1:]):
+        assert (a0, b0) == (a1, b1) or b0 <= a1
+    alone = tasks()
+    n_alone = adaptive.pass1(alone, rtol, 4096, False, False)
+    assert n_shared < n_alone
+    capped = tasks()
+    shared.pass1_groups(capped, [[0, 1, 2]], rtol, 24, False, False)
+    assert not all(t.converged for t in capped)
+    assert len({(x, y) for t in capped for x, y in zip(t.a, t.b)}) <= 24
+
+
+def test_shared_panels_extend_a_member_with_a_steep_integrand(monkeypatch):
+    """With C_sca ~ r^6 a member's range grows past its area rule's end inside the group (onto
+    the group's panels, or new ones), until the r^6 tail is below rtol/10; the closed form
+    r_g^6 exp(18 s^2) is met; a gamma member beside it converges too."""
+    from scatrs_amd import adaptive, shared
+
+    def q(r):
+        ext = r ** 6 + 0.01 * r * r
+        return ext, r ** 6, 0.01 * r * r, r ** 6, np.zeros_like(r)
+
+    monkeypatch.setattr(adaptive, "_node_values", _stub_nodes(q))
+    r_g, sigma, rtol = 0.01, 2.0, 1e-7
+    s = np.log(sigma)
+    t = adaptive.Task(0, np.array([r_g, sigma, 0.0]), 1.0, np.array([1.5 + 0j]), np.zeros(0))
+    g = adaptive.Task(1, np.array([0.5, 0.2, 0.0]), 1.0, np.array([1.5 + 0j]), np.zeros(0))
+    _, t_hi, _ = adaptive.integration_range(0, t.p, rtol)
+    shared.pass1_groups([t, g], [[0, 1]], rtol, 4096, False, False)
+    assert t.converged and g.converged
+    assert t.t_hi > t_hi + 1e-9
+    sca = t.K[:, 1].sum() / t.moments[0]
+    exact = r_g ** 6 * np.exp(18 * s * s)
+    assert abs(sca - exact) <= 5 * rtol * exact
