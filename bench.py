@@ -315,7 +315,9 @@ def bench_ensembles(args, torch):
                                 "angular_tflops": flops / (ang64 * 1e-3) / 1e12,
                                 "angular_frac_of_fp64_peak": flops / (ang64 * 1e-3) / 1e12 /
                                 FP64_PEAK_TFLOPS}
-    log(f"FP64 matrix cores: {sec64:.3f} s/step, angular {ang64:.0f} ms")
+    res["speedup_vs_fp64_matrix_cores"] = sec64 / sec
+    log(f"FP64 matrix cores: {sec64:.3f} s/step, angular {ang64:.0f} ms; fused is "
+        f"{sec64 / sec:.2f}x")
     # alpha1 of every ensemble, fp16-split against FP64 matrix cores (on the device)
     d_all = float(((out[1][:, 0] - out64[1][:, 0]).abs().amax(dim=1) /
                    out64[1][:, 0].abs().amax(dim=1)).max().item())
@@ -327,8 +329,12 @@ def bench_ensembles(args, torch):
             sec_p, kt_p, _ = _timed(torch, ctx, fn, 1)
         finally:
             ctx.set_angular_kernel(0)
+        ang_p = float(np.sum(kt_p[1::2]))
         res["plain_fp64"] = {"ensembles_per_s": kind.size / sec_p, "s_per_step": sec_p,
-                             "angular_ms_per_step": float(np.sum(kt_p[1::2]))}
+                             "angular_ms_per_step": ang_p,
+                             "angular_tflops": flops / (ang_p * 1e-3) / 1e12,
+                             "angular_frac_of_fp64_peak": flops / (ang_p * 1e-3) / 1e12 /
+                             FP64_PEAK_TFLOPS}
         res["speedup_vs_plain"] = sec_p / sec
         log(f"plain FP64: {sec_p:.3f} s/step, fused speedup {sec_p / sec:.2f}x")
     if not args.no_check:
@@ -422,6 +428,29 @@ def bench_adjoint(args, torch):
            "speedup_vs_fp64_matrix_cores": sec64 / sec,
            "grad_max_diff_vs_fp64": gdiff,
            "finite": bool(torch.isfinite(g).all().item())}
+    # The reverse products' roofline: one reverse pass alone, its marked regions per chunk being
+    # the forward nodes and tables it recomputes, the reverse angular products, then the
+    # reverse node solves; 32 h N flops per node (the forward S1, S2 again and the transposed
+    # products) on the FP64 matrix cores.
+    sc, gr, _, terms = ensemble_fixed_device(*t, n_radius=nr, n_angles=na, order=order,
+                                             want_phase=False)
+    _, bs, bg = workloads.adjoint_loss(sc, gr, tt)
+    ctx.set_angular_kernel(2)
+    try:
+        _, kt_r, _ = _timed(torch, ctx, lambda: ensemble_fixed_vjp_device(
+            *t, bar_scalars=bs, bar_greek=bg, n_radius=nr, n_angles=na, order=order), 1)
+    finally:
+        ctx.set_angular_kernel(0)
+    rev_ms = float(np.sum(kt_r[2::4]))
+    h = (na + 1) // 2
+    ach = 32.0 * h * terms / (rev_ms * 1e-3) / 1e12
+    res["roofline_reverse"] = {"bound": "mfma", "pipe": "FP64 matrix cores "
+                               "(v_mfma_f64_16x16x4f64)", "kernel": "ens_angular_vjp",
+                               "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": ach / FP64_PEAK_TFLOPS, "reverse_ms_per_step": rev_ms,
+                               "flops_per_step": 32.0 * h * terms}
+    log(f"reverse products: {rev_ms:.0f} ms, {ach:.1f} TFLOP/s ({ach / FP64_PEAK_TFLOPS:.3f} "
+        f"of FP64 peak)")
     if not args.no_cpu_baseline:
         import oracle
 
