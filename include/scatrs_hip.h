@@ -288,6 +288,23 @@ int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes, const
                           int allow_unvalidated, double *d_scalars, double *d_greek,
                           double *d_phase, int64_t *total_terms);
 
+/* Forward mode of scatrs_ensemble_nodes along one of the particle's directions (direction 2 l,
+ * 2 l + 1: Re m, Im m of layer l, core first; 2 L + j: radius fraction j as x_L d/dx_j at a
+ * fixed outer size): the same call (normalized = 0) with every node's coefficients moved to
+ * c + sign eps dc/d(direction), eps = 1/(1 + x) at the node's outer size parameter x, and its
+ * weight W scaled by 1/(2 eps). Every output's weighted sum is linear or quadratic in the
+ * coefficients, so the sign = +1 call's sums (average times the row's scaled weight) minus the
+ * sign = -1 call's are the sums' derivative along the direction, exactly up to rounding; the
+ * C_abs column moves with C_ext - C_sca (the absorption sum is not moved). Replaces the
+ * reference's forward-mode derivative columns (ensemble/engine.rs:1111-1123, particle.rs:221-249)
+ * for the particle's directions: P + 1 evaluations instead of one reverse pass per output. */
+int scatrs_ensemble_nodes_tangent(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
+                                  const double *d_nodes, const double *d_wavelength, int layers,
+                                  const double *d_fractions, const double *d_m, int64_t n_angles,
+                                  int64_t order, int direction, int sign, int allow_unvalidated,
+                                  double *d_scalars, double *d_greek, double *d_phase,
+                                  int64_t *total_terms);
+
 /* Reverse mode of scatrs_ensemble_nodes, as scatrs_ensemble_fixed_vjp's with the range held
  * fixed: the nodes stay where they are, and the parameters act through each node's ln W, whose
  * derivatives d_dlnw (count, n_nodes, 3) the caller gives (d ln u(r) / d p at the node for

@@ -122,6 +122,11 @@ pub mod ffi {
             d_fractions: *const f64, d_m: *const f64, n_angles: i64, order: i64,
             normalized: c_int, allow_unvalidated: c_int, d_scalars: *mut f64,
             d_greek: *mut f64, d_phase: *mut f64, total_terms: *mut i64) -> c_int;
+        pub fn scatrs_ensemble_nodes_tangent(ctx: *mut ScatrsCtx, count: i64, n_nodes: i64,
+            d_nodes: *const f64, d_wavelength: *const f64, layers: c_int,
+            d_fractions: *const f64, d_m: *const f64, n_angles: i64, order: i64,
+            direction: c_int, sign: c_int, allow_unvalidated: c_int, d_scalars: *mut f64,
+            d_greek: *mut f64, d_phase: *mut f64, total_terms: *mut i64) -> c_int;
         pub fn scatrs_ensemble_nodes_vjp(ctx: *mut ScatrsCtx, count: i64, n_nodes: i64,
             d_nodes: *const f64, d_dlnw: *const f64, d_wavelength: *const f64, layers: c_int,
             d_fractions: *const f64, d_m: *const f64, n_angles: i64, order: i64,

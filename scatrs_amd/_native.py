@@ -53,6 +53,7 @@ SYMBOLS = (
     "scatrs_graph_last_error",
     "scatrs_graph_destroy",
     "scatrs_ensemble_nodes",
+    "scatrs_ensemble_nodes_tangent",
     "scatrs_ensemble_nodes_vjp",
     "scatrs_size_range",
     "scatrs_size_log_density",
@@ -127,6 +128,8 @@ def load() -> C.CDLL:
             L.scatrs_ensemble_fixed_vjp.argtypes = [vp, i64, vp, vp, vp, vp, vp, i32, vp, vp,
                                                     i64, i64, i64, i32, i32, i32, vp, vp, vp, vp,
                                                     C.POINTER(i64)]
+        L.scatrs_ensemble_nodes_tangent.argtypes = [vp, i64, i64, vp, vp, i32, vp, vp, i64, i64,
+                                                    i32, i32, i32, vp, vp, vp, C.POINTER(i64)]
         if hasattr(L, "scatrs_ensemble_nodes"):
             L.scatrs_ensemble_nodes.argtypes = [vp, i64, i64, vp, vp, i32, vp, vp, i64, i64, i32,
                                                 i32, vp, vp, vp, C.POINTER(i64)]

@@ -815,7 +815,7 @@ def _row_panels(tasks):
 
 
 def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
-                allow_unvalidated, gauss_tasks=None):
+                allow_unvalidated, gauss_tasks=None, derivatives=None):
     """The Kronrod (idx all 15 nodes) or Gauss rule's pass 2, differentiable in every task's
     parameters (torch.autograd): params[t] = (p (R, 3), m_real (R, L), m_imag (R, L),
     fractions (R, L-1) or None), R replicas of task t's parameters. Returns the linear averages
@@ -828,7 +828,9 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     . sum W d ln W / d p, which has the right value and derivative at p0; a monodisperse task is
     the fixed path's one node (its radius moves its node). With gauss_tasks (bool per task) and
     all 15 nodes, those tasks take the Gauss weights on the same nodes (0 at the Kronrod-only
-    ones): a task list doubled this way gets both rules from one call per angle rule."""
+    ones): a task list doubled this way gets both rules from one call per angle rule. The
+    weights' derivatives d ln W / d p are formed when ``derivatives`` (default: when gradients
+    are enabled; forward mode asks for them explicitly)."""
     import torch
 
     from .ensemble import ensemble_fixed, ensemble_nodes, number_averages, segment_sum
@@ -889,7 +891,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     nr = len(rows)
     if nr:
         ti_rows, A, Bd = _row_panels(tasks)
-        grad = torch.is_grad_enabled()  # without gradients the weights' derivatives go unused
+        grad = torch.is_grad_enabled() if derivatives is None else bool(derivatives)
         out_n = _nodes(tasks, np.repeat(ti_rows, _ROW_PANELS), A.reshape(-1), Bd.reshape(-1),
                        idx, derivatives=grad)
         r, w = out_n[0], out_n[1]

@@ -20,12 +20,17 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
                             const double *d_fractions, const double *d_m, int64_t n_radius,
                             int64_t n_angles, int64_t order, int normalized,
                             int allow_unvalidated, double *d_scalars, double *d_greek,
-                            double *d_phase, int64_t *total_terms) {
+                            double *d_phase, int64_t *total_terms, int tan_dir = -1,
+                            double tan_sign = 0.0) {
     if (!ctx) return SCATRS_INVALID_INPUT;
     ctx->error.clear();
     if (total_terms) *total_terms = 0;
     int rc = ens_check_args(ctx, count, layers, n_radius, n_angles, d_greek != nullptr, order);
     if (rc) return rc;
+    if (tan_dir >= 0 && (tan_dir >= 3 * layers - 1 || (tan_sign != 1.0 && tan_sign != -1.0)))
+        return fail(ctx, SCATRS_INVALID_INPUT,
+                    "the tangent direction must be in [0, " + std::to_string(3 * layers - 1) +
+                        ") and its sign +1 or -1");
     if (count == 0) return SCATRS_OK;
     if ((!d_nodes && (!d_kind || !d_p0 || !d_p1)) || !d_wavelength || !d_m || !d_scalars ||
         (layers > 1 && !d_fractions))
@@ -36,6 +41,8 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
                         ctx->angular_kernel, S, d_nodes)))
         return rc;
     if (total_terms) *total_terms = S.total_terms;
+    S.a.tan_dir = tan_dir;
+    S.a.tan_sign = tan_sign;
     hipStream_t s = ctx->stream;
     // The phase at the caller's cosines (scatrs_set_phase_cosines): the angular products once
     // more, at the distinct |mu| as the non-negative cosines of a symmetric set of 2 h; the
@@ -298,6 +305,27 @@ extern "C" int scatrs_gauss_legendre(int64_t n, double *nodes, double *weights) 
     std::memcpy(nodes, x.data(), n * sizeof(double));
     std::memcpy(weights, w.data(), n * sizeof(double));
     return SCATRS_OK;
+}
+
+extern "C" int scatrs_ensemble_nodes_tangent(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
+                                             const double *d_nodes, const double *d_wavelength,
+                                             int layers, const double *d_fractions,
+                                             const double *d_m, int64_t n_angles, int64_t order,
+                                             int direction, int sign, int allow_unvalidated,
+                                             double *d_scalars, double *d_greek, double *d_phase,
+                                             int64_t *total_terms) {
+    if (ctx && count > 0 && !d_nodes) {
+        ctx->error.clear();
+        return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
+    }
+    if (ctx && direction < 0) {
+        ctx->error.clear();
+        return fail(ctx, SCATRS_INVALID_INPUT, "the tangent direction must be >= 0");
+    }
+    return ensemble_forward(ctx, count, nullptr, nullptr, nullptr, nullptr, d_nodes, d_wavelength,
+                            layers, d_fractions, d_m, n_nodes, n_angles, order, 0,
+                            allow_unvalidated, d_scalars, d_greek, d_phase, total_terms,
+                            direction, (double)sign);
 }
 
 extern "C" int scatrs_size_message(int32_t kind, const double *p, char *buf, int64_t cap) {

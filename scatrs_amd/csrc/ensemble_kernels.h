@@ -53,6 +53,10 @@ struct EnsArgs {
     double2 *geom;                    // per node (r, W), written by ens_prep
     const double2 *user_rw;           // the caller's (r, W) per node, or null (from kind, p)
     const double *user_dlnw;          // with user_rw, the caller's d ln W / d p[3] per node
+    // Tangent mode (scatrs_ensemble_nodes_tangent): the particle direction whose derivative is
+    // taken (-1: none) and the sign of the step (TangentStagingSink).
+    int tan_dir = -1;
+    double tan_sign = 0.0;
 };
 
 // Ensemble e's distribution parameters.
@@ -289,6 +293,108 @@ struct StagingSink {
     __device__ void zero_absorption() { eff.zero_absorption(); }
 };
 
+// Tangent mode: a node's coefficients moved along one of the particle's directions,
+// c' = c + sign eps dc/d(theta) with eps = 1 / (1 + x), staged as StagingSink stages c (FP64 or
+// fp32), with the efficiency sums of c'. Every output of an ensemble is linear or quadratic in
+// its nodes' coefficients (S1, S2 linear; F, C_sca, C_sca g, C_back quadratic), so
+// [q(c + eps dc) - q(c - eps dc)] / (2 eps) is dq exactly (to rounding): two calls, sign +1
+// and -1, with the node weights scaled by 1 / (2 eps), give the derivative of the weighted
+// sums as their difference. The absorption sum (C_ext = C_sca + C_abs, efficiencies.rs:169-187)
+// moves by sign eps times its derivative along the direction, from the absorption identity's
+// Jacobian: the outer layer's from the solve's (done), an inner layer's summed order by order
+// from its sensitivities (stored), as the reverse mode's AdjointSink forms them. Directions as
+// the reference's (ensemble/particle.rs:221-249): 2 l, 2 l + 1 are Re m, Im m of layer l
+// (d/d Im m = i d/dm), 2 L + j the radius fraction j (x_L d/dx_j at a fixed x_L).
+template <int L>
+struct TangentStagingSink {
+    cx *ab_out;
+    float4 *ab32;  // not null: the fp32 form
+    EfficiencySink<false> eff;
+    double amax = 0.0;
+    int dir;
+    double step;  // sign eps
+    double x;     // the outer size parameter (radius-fraction directions)
+    double dabs = 0.0;  // the absorption sum's derivative along the direction (inner layers:
+                        // summed over the orders; after done: the direction's)
+    cx pa, pb;    // the order's moved coefficients before the inner layers' part
+    __device__ void put(int k, cx a, cx b) {
+        if (ab32) {
+            ab32[k - 1] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+        } else {
+            ab_out[2 * (k - 1)] = a;
+            ab_out[2 * (k - 1) + 1] = b;
+        }
+        amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
+        eff.order(k, a, b);
+    }
+    __device__ void order(int k, cx a, cx b) { put(k, a, b); }
+    __device__ void order_d(int k, cx a, cx b, cx, cx dam, cx, cx dbm) {
+        cx ta = real(0.0), tb = real(0.0);
+        if (dir == 2 * (L - 1)) {
+            ta = dam;
+            tb = dbm;
+        } else if (dir == 2 * (L - 1) + 1) {
+            ta = times_i(dam);
+            tb = times_i(dbm);
+        }
+        pa = a + ta * step;
+        pb = b + tb * step;
+        if constexpr (L == 1) put(k, pa, pb);
+    }
+    template <class O>
+    __device__ void stored(int k, cx inv_da, cx inv_db, cx g_a, cx g_b, cx d_m, cx md,
+                           const O &o) {
+        if constexpr (L > 1) {
+            // the inner layer (or interface) the direction moves: l < L - 1
+            const int l = dir < 2 * L ? dir / 2 : dir - 2 * L;
+            if (l < L - 1) {
+                const cx mL = o.m[L - 1];
+                const cx inv_mL = inv(mL);
+                const bool frac = dir >= 2 * L;
+                const int c = frac ? 0 : 1;  // dh columns: [d/dx_l, d/dm_l] for a, then for b
+                cx ba = o.dh[l][c] * inv_mL, bb = mL * o.dh[l][2 + c];
+                if (!frac && (dir & 1)) {  // d/d Im m = i d/dm
+                    ba = times_i(ba);
+                    bb = times_i(bb);
+                }
+                cx da = -(times_i(ba) * (inv_da * inv_da));
+                cx db = -(times_i(bb) * (inv_db * inv_db));
+                double ab = (double)(2 * k + 1) *
+                            (absorption_term(d_m, ba, ba * g_a, norm2(inv_da)) +
+                             absorption_term(md, bb, bb * g_b, norm2(inv_db)));
+                if (frac) {
+                    da = da * x;
+                    db = db * x;
+                    ab *= x;
+                }
+                dabs += ab;
+                pa = pa + da * step;
+                pb = pb + db * step;
+            }
+            put(k, pa, pb);
+        } else {
+            (void)k, (void)inv_da, (void)inv_db, (void)g_a, (void)g_b, (void)d_m, (void)md,
+                (void)o;
+        }
+    }
+    // The absorption sum's derivative along the direction: the outer layer's from the solve.
+    __device__ double absorption_derivative(double j1, double j2) const {
+        if (dir == 2 * (L - 1)) return j1;
+        if (dir == 2 * (L - 1) + 1) return j2;
+        return dabs;
+    }
+    __device__ void done(double s, double j0, double j1, double j2) {
+        dabs = absorption_derivative(j1, j2);
+        eff.done(s + step * dabs, j0, j1, j2);
+    }
+    // A lossless layered sphere absorbs nothing and stays lossless as any x_l or Re m_l changes
+    // (layered/kernel.rs:453-464); along an Im m direction its absorption moves from 0.
+    __device__ void zero_absorption() {
+        eff.zero_absorption();
+        if (dir < 2 * L && (dir & 1)) eff.abs_sum = step * dabs;
+    }
+};
+
 // The exponent s that brings a node's largest coefficient component amax into [2^9, 2^10) by
 // 2^s (|a_n|, |b_n| <= 1), within [0, 100]: the fp16-split angular products scale the node's
 // coefficients by 2^s, so that their slices keep fp16's full precision (those of small
@@ -356,6 +462,54 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     if (back_out) back_out[g] = sink.eff.back;
 }
 
+// forward_node in tangent mode (a.tan_dir >= 0): the solve with derivatives, the coefficients
+// moved along the direction (TangentStagingSink), the node's weight times 1 / (2 eps).
+template <int L>
+__device__ __forceinline__ void tangent_node(const EnsArgs &a, int64_t e0, int64_t g,
+                                             const int *node_n, const int64_t *node_off,
+                                             const int64_t *ens_base, cx *slab, int64_t rows,
+                                             cx *tile, cx *staging, int stage32, double *vals,
+                                             double *tabc, int *xexp) {
+    const int64_t e = e0 + g / a.n_radius, i = g % a.n_radius;
+    const int64_t gi = e * a.n_radius + i;
+    const NodeGeom geo = node_stored(a, e, i);
+    const int n = node_n[gi];
+    const int64_t off = ens_base[e] + node_off[gi];
+    const double eps = 1.0 / (1.0 + geo.x);
+    TangentStagingSink<L> sink;
+    sink.ab_out = staging + 2 * off;
+    sink.ab32 = stage32 ? reinterpret_cast<float4 *>(staging) + off : nullptr;
+    sink.dir = a.tan_dir;
+    sink.step = a.tan_sign * eps;
+    sink.x = geo.x;
+    if constexpr (L == 1) {
+        const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
+        double *ux = real_column(slab, sphere_ckpt_rows(rows) * 64);
+        const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
+        x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
+        NoOuter none;
+        mie_orders<true>(geo.x, m, n, zs, ux, 64, none, sink);
+    } else {
+        (void)tile;
+        layered_solve<L, true>(geo.x, a.fr + e * (L - 1), a.m + 2 * e * L, n, slab, rows, sink);
+    }
+    double q[5];
+    sink.eff.values(geo.x, q);
+    const double area = kPi * geo.r * geo.r;
+    const double W = geo.W / (2.0 * eps);
+    double *v = vals + g * kNodeVals;
+    v[0] = W;
+    v[1] = W * (area * q[0]);
+    v[2] = W * (area * q[1]);
+    v[3] = W * (area * q[1] * q[4]);
+    v[4] = W * (area * q[2]);
+    v[5] = W * (area * q[3]);
+    v[6] = W * (geo.r * geo.r);
+    v[7] = W * (geo.r * geo.r * geo.r);
+    tabc[g] = W * (area * (4.0 / (geo.x * geo.x)));
+    xexp[g] = coefficient_exponent(sink.amax);
+}
+
 // Spheres: persistent; each wavefront takes the next 64 ranked nodes from the counter, so the
 // nodes of largest N start first and the rest fill in behind them, and the 64 lanes of a
 // wavefront solve nodes of about the same N. Slabs of `rows` rows (the chunk's largest N).
@@ -379,6 +533,50 @@ static __global__ __launch_bounds__(256) void ens_nodes_persist(
             forward_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile,
                             staging, stage32, vals, tabc, xexp, back_out);
     }
+}
+
+// Tangent mode, spheres: persistent over the ranked nodes, as ens_nodes_persist.
+static __global__ __launch_bounds__(256) void ens_nodes_tangent_persist(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
+    const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, cx *scratch,
+    int64_t rows, cx *staging, int stage32, double *vals, double *tabc, int *xexp) {
+    __shared__ cx ztile[4][kZBlock * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *tile = &ztile[threadIdx.x >> 6][lane];
+    cx *slab = scratch + wave * slab_elems(1, rows) + lane;
+    while (true) {
+        unsigned long long chunk = 0;
+        if (lane == 0) chunk = atomicAdd(counter, 1ull);
+        chunk = __shfl(chunk, 0, 64);
+        if ((int64_t)chunk * 64 >= chunk_nodes) break;
+        const int64_t idx = (int64_t)chunk * 64 + lane;
+        if (idx < chunk_nodes)
+            tangent_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile,
+                            staging, stage32, vals, tabc, xexp);
+    }
+}
+
+// Tangent mode, layered spheres: one launch covers the ranked nodes [start, end), one group of
+// 64 per wavefront and no work loop (slabs sized by the launch's first node), as
+// ens_nodes_ranked.
+template <int L>
+__global__ __launch_bounds__(256) void ens_nodes_tangent(EnsArgs a, int64_t e0, const int *order,
+                                                         int64_t start, int64_t end,
+                                                         const int *node_n,
+                                                         const int64_t *node_off,
+                                                         const int64_t *ens_base, cx *scratch,
+                                                         int64_t rows, cx *staging, int stage32,
+                                                         double *vals, double *tabc, int *xexp) {
+    __shared__ cx ztile[4][L == 1 ? kZBlock * 64 : 1];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    cx *slab = scratch + wave * slab_elems(L, rows) + lane;
+    cx *tile = L == 1 ? &ztile[threadIdx.x >> 6][lane] : nullptr;
+    const int64_t idx = start + wave * 64 + lane;
+    if (idx < end)
+        tangent_node<L>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile, staging,
+                        stage32, vals, tabc, xexp);
 }
 
 // Layered spheres: one launch covers the ranked nodes [start, end), one group of 64 per
@@ -1573,6 +1771,19 @@ template <int L>
 static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, const EnsSetup &S,
                                 cx *scratch, const ChunkBufs &B, cx *back_out) {
     for (const NodeLaunch &q : B.plan) {
+        if (a.tan_dir >= 0) {
+            if constexpr (L == 1) {
+                fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+                ens_nodes_tangent_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
+                    a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
+                    q.rows, B.staging, S.stage32, B.vals, B.tabc, B.xexp);
+            } else {
+                ens_nodes_tangent<L><<<(unsigned)((q.end - q.start + 255) / 256), 256, 0, s>>>(
+                    a, e0, B.order, q.start, q.end, S.node_n, S.node_off, S.ens_base, scratch,
+                    q.rows, B.staging, S.stage32, B.vals, B.tabc, B.xexp);
+            }
+            continue;
+        }
         if constexpr (L == 1) {
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(

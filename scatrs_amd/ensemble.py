@@ -24,7 +24,8 @@ from .size import SEGMENT, Distribution, MultiModal, Tabulated, check
 
 __all__ = ["SCALARS", "Ensemble", "EnsembleGraph", "EnsembleGreek", "EnsemblePhase", "EnsembleResult",
            "Mixture", "ensemble_fixed", "ensemble_fixed_device", "ensemble_fixed_vjp_device",
-           "ensemble_nodes", "ensemble_nodes_device", "ensemble_nodes_vjp_device",
+           "ensemble_nodes", "ensemble_nodes_device", "ensemble_nodes_tangent_device",
+           "ensemble_nodes_vjp_device",
            "assemble", "mix_parts", "mix_units", "number_averages", "parameter_names"]
 
 SCALARS = ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "depolarization",
@@ -193,6 +194,136 @@ def ensemble_nodes_device(nodes, wavelength, m, fractions=None, *, n_angles: int
     return scalars, greek, phase, terms.value
 
 
+def ensemble_nodes_tangent_device(nodes, wavelength, m, fractions=None, *, direction: int,
+                                  sign: int, n_angles: int = 128, order: int | None = 32,
+                                  allow_unvalidated: bool = False, want_phase: bool = True,
+                                  mu=None):
+    """scatrs_ensemble_nodes_tangent: ensemble_nodes_device (normalized=False) with every node's
+    coefficients moved by sign eps dc along the particle's ``direction`` (2 l, 2 l + 1: Re m,
+    Im m of layer l; 2 L + j: radius fraction j), eps = 1/(1 + x), and its weight scaled by
+    1/(2 eps). Returns (scalars, greek, phase, terms): averages over the scaled weights."""
+    import torch
+
+    dev = nodes.device
+    count, n_nodes = nodes.shape[0], nodes.shape[1]
+    layers = 1 if m.dim() == 1 else m.shape[1]
+    nd = nodes.to(torch.float64).contiguous()
+    wl, mm = wavelength.contiguous(), m.contiguous()
+    fr = None if layers == 1 else fractions.contiguous()
+    scalars = torch.empty((count, 11), dtype=torch.float64, device=dev)
+    greek = None if order is None else torch.empty((count, 6, order + 1), dtype=torch.float64,
+                                                   device=dev)
+    n_phase = n_angles if mu is None else int(np.size(mu))
+    phase = torch.empty((count, 6, n_phase), dtype=torch.float64, device=dev) if want_phase \
+        else None
+    ctx = _native.context(dev.index)
+    terms = C.c_int64(0)
+    p = lambda t: None if t is None else C.c_void_p(t.data_ptr())  # noqa: E731
+    if mu is not None:
+        ctx.set_phase_cosines(mu)
+    try:
+        rc = ctx.lib.scatrs_ensemble_nodes_tangent(
+            ctx.handle, count, n_nodes, p(nd), p(wl), layers, p(fr), p(mm), int(n_angles),
+            -1 if order is None else int(order), int(direction), int(sign),
+            int(bool(allow_unvalidated)), p(scalars), p(greek), p(phase), C.byref(terms))
+    finally:
+        if mu is not None:
+            ctx.set_phase_cosines(None)
+    ctx.check(rc)
+    return scalars, greek, phase, terms.value
+
+
+def _scalars_tangent(avg, davg):
+    """The tangent of assemble's scalars (normalized=False) at the averages avg (..., 7) along
+    davg: d(a / b) = (da - (a / b) db) / b for the ratios, depolarization 0."""
+    import torch
+
+    ext, sca, absb, back, scag, r2, r3 = avg.unbind(-1)
+    de, ds, da, db, dg, d2, d3 = davg.unbind(-1)
+    q = lambda a, b, dta, dtb: (dta - (a / b) * dtb) / b  # noqa: E731
+    return torch.stack([de, ds, da, db, q(sca, ext, ds, de), q(scag, sca, dg, ds),
+                        4.0 * math.pi * q(ext, back, de, db), torch.zeros_like(de),
+                        q(r3, r2, d3, d2), math.pi * d2, 4.0 / 3.0 * math.pi * d3], dim=-1)
+
+
+def _nodes_jvp(o, sc, gr, ph, m_real, m_imag, fractions, nodes, dlnw, wavelength, dp, dm_re,
+               dm_im, dfr):
+    """Tangents of (scalars, greek, phase) of ensemble_nodes (normalized=False; rows in runs of
+    o["replicas"] = k equal replicas, each with its own tangent) along the input tangents.
+
+    The weights' part: one more call on the rows whose d ln W . dp is not zero, with the weights
+    W (s + c) > 0, s = d ln W . dp, c = 1 + max |s| per row; its sums minus c times the primal's
+    are sum W s F. Each particle direction's part (2 l, 2 l + 1: Re m, Im m of layer l; 2 L + j:
+    radius fraction j): two tangent calls (scatrs_ensemble_nodes_tangent, sign +1 and -1) on one
+    row of every run with a nonzero tangent, whose sums' difference is sum W dF per unit step,
+    times each row's tangent. Averages move by (dS - A dZ) / Z."""
+    import torch
+
+    k = o["replicas"]
+    W = nodes[..., 1]
+    z = W.sum(1)
+    m = torch.complex(m_real, m_imag)
+    layers = 1 if m.dim() == 1 else m.shape[1]
+    fr = fractions if layers > 1 else None
+    W6 = 6 * (gr.shape[-1] if gr is not None else 0)
+    nph = ph.shape[-1] if ph is not None else 0
+    sub = lambda a, rows: None if a is None else a[rows]  # noqa: E731
+
+    def flat(s, g, p):
+        cols = [number_averages(s)]
+        if g is not None:
+            cols.append(g.reshape(s.shape[0], -1))
+        if p is not None:
+            cols.append(p.reshape(s.shape[0], -1))
+        return torch.cat(cols, 1)
+
+    A = flat(sc, gr, ph)  # averages of the linear quantities over W
+    dS = torch.zeros_like(A)  # tangents of their weighted sums
+    dZ = torch.zeros_like(z)
+    kw = dict(n_angles=o["n_angles"], order=o["order"], allow_unvalidated=o["allow_unvalidated"],
+              want_phase=o["phase"], mu=o["mu"])
+    if dp is not None and dlnw is not None:
+        sw = (dlnw * dp[:, None, :]).sum(-1)
+        rows = torch.nonzero((sw != 0).any(1)).flatten()
+        if rows.numel():
+            s_r, w_r = sw[rows], W[rows]
+            c = s_r.abs().amax(1) + 1.0
+            ww = w_r * (s_r + c[:, None])
+            nw = torch.stack([nodes[rows, :, 0], ww], -1)
+            s2, g2, p2, _ = ensemble_nodes_device(nw, wavelength[rows], m[rows], sub(fr, rows),
+                                                  normalized=False, **kw)
+            dS[rows] += flat(s2, g2, p2) * ww.sum(1)[:, None] - c[:, None] * A[rows] * \
+                z[rows, None]
+            dZ[rows] += (w_r * s_r).sum(1)
+    dirs = []
+    for l in range(layers):
+        for part, dm in ((0, dm_re), (1, dm_im)):
+            if dm is not None:
+                dirs.append((2 * l + part, dm if dm.dim() == 1 else dm[:, l]))
+    if dfr is not None and layers > 1:
+        for j in range(layers - 1):
+            dirs.append((2 * layers + j, dfr[:, j]))
+    x = 2.0 * math.pi * nodes[..., 0] / wavelength[:, None]
+    zs = (W * (1.0 + x) * 0.5).sum(1)  # the tangent calls' sums of weights
+    span = torch.arange(k, device=nodes.device)
+    for d, t in dirs:
+        runs = torch.nonzero((t != 0).reshape(-1, k).any(1)).flatten()
+        if not runs.numel():
+            continue
+        u = runs * k  # a run's rows share their nodes and particle
+        one = lambda sign: flat(*ensemble_nodes_tangent_device(  # noqa: E731
+            nodes[u], wavelength[u], m[u], sub(fr, u), direction=d, sign=sign, **kw)[:3])
+        dd = (one(1) - one(-1)) * zs[u, None]
+        rr = (u[:, None] + span[None, :]).reshape(-1)
+        dS[rr] += t[rr, None] * dd.repeat_interleave(k, 0)
+    dA = (dS - A * dZ[:, None]) / z[:, None]
+    dsc = _scalars_tangent(A[:, :7], dA[:, :7])
+    dgr = dA[:, 7:7 + W6].reshape(gr.shape).contiguous() if gr is not None else None
+    dph = dA[:, 7 + W6:7 + W6 + 6 * nph].reshape(ph.shape).contiguous() if ph is not None \
+        else None
+    return dsc, dgr, dph
+
+
 def ensemble_nodes_vjp_device(nodes, dlnw, wavelength, m, fractions=None, *, bar_scalars=None,
                               bar_greek=None, bar_phase=None, n_angles: int = 128,
                               order: int | None = 32, normalized: bool = True,
@@ -256,9 +387,24 @@ def _nodes_function():
             empty = torch.empty(0)
             ctx.save_for_backward(m_real, m_imag, fractions if fractions is not None else empty,
                                   nodes, dlnw, wavelength)
+            ctx.save_for_forward(m_real, m_imag, fractions if fractions is not None else empty,
+                                 nodes, dlnw if dlnw is not None else empty, wavelength)
+            ctx.fwd_out = (sc, gr, ph)
             ctx.opts = opts
             ctx.has_fr = fractions is not None
+            ctx.has_dlnw = dlnw is not None
             return sc, gr, ph
+
+        @staticmethod
+        def jvp(ctx, dp, dm_re, dm_im, dfr, dnodes, ddlnw, dwl, dopts):
+            m_real, m_imag, fr, nodes, dlnw, wl = ctx.saved_tensors
+            o = ctx.opts
+            if o["normalized"]:
+                raise NotImplementedError("forward mode needs normalized=False")
+            sc, gr, ph = ctx.fwd_out
+            return _nodes_jvp(o, sc, gr, ph, m_real, m_imag, fr if ctx.has_fr else None, nodes,
+                              dlnw if ctx.has_dlnw else None, wl, dp, dm_re, dm_im,
+                              dfr if ctx.has_fr else None)
 
         @staticmethod
         def backward(ctx, g_sc, g_gr, g_ph):

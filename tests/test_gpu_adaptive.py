@@ -239,6 +239,104 @@ def test_caller_nodes_vjp_is_the_fixed_range_vjp():
     assert np.max(np.abs(g1 - g2)) <= 1e-11 * np.max(np.abs(g2)), (g1, g2)
 
 
+@pytest.mark.parametrize("layers", [1, 2, 3])
+def test_caller_nodes_forward_mode_is_the_transposed_vjp(layers):
+    """Forward mode of ensemble_nodes (scatrs_ensemble_nodes_tangent for the particle's
+    directions, a reweighted call for the distribution's) against the reverse mode
+    (scatrs_ensemble_nodes_vjp) by the transposition identity c . (J e_j) = (J^T c) . e_j, for
+    every parameter e_j alone (the distribution's three, Re m and Im m of each layer, the radius
+    fractions) with random cotangents over the scalars, Greek coefficients and phase matrix;
+    and replicas with different tangents in one call."""
+    from torch.autograd import forward_ad
+
+    from scatrs_amd.ensemble import ensemble_nodes, ensemble_nodes_vjp_device
+
+    kind, p, r, W, d = _nodes_case()
+    rng = np.random.default_rng(11 + layers)
+    L, na, wl = 10, 32, 0.8
+    ms = [1.6 + 0.1j, 1.45 + 0.02j, 1.38 + 1e-3j][-layers:]
+    m = np.array([ms]) if layers > 1 else np.array([ms[0]])
+    fr = _dev(np.array([[0.35, 0.7][-(layers - 1):]])) if layers > 1 else None
+    bs, bg, bp = rng.normal(size=(1, 11)), rng.normal(size=(1, 6, L + 1)), rng.normal(
+        size=(1, 6, na))
+    bs[0, 7] = 0.0
+    nodes, dl, wlt, md = (_dev(np.stack([r, W], -1)[None]), _dev(d[None]),
+                          _dev(np.array([wl])), _dev(m))
+    g, _ = ensemble_nodes_vjp_device(nodes, dl, wlt, md, fr, bar_scalars=_dev(bs),
+                                     bar_greek=_dev(bg), bar_phase=_dev(bp), n_angles=na,
+                                     order=L, normalized=False)
+    g = g.cpu().numpy()[0]
+    P = 3 + 2 * layers + (layers - 1)
+    assert g.shape == (P,)
+    p0 = _dev(np.array([p]))
+    mr, mi = md.real.contiguous(), md.imag.contiguous()
+
+    def tangent(j, k=1):
+        """Outputs' tangents along parameter e_j (rows repeated k times, row 0 seeded)."""
+        e = np.zeros(P)
+        e[j] = 1.0
+        rep = lambda a: None if a is None else a.repeat_interleave(k, 0)  # noqa: E731
+        seeds = [e[:3], e[3:3 + 2 * layers:2], e[4:4 + 2 * layers:2], e[3 + 2 * layers:]]
+        t = [np.zeros((k,) + a.shape) for a in seeds]
+        for a, s in zip(t, seeds):
+            a[0] = s
+        if layers == 1:
+            t[1], t[2] = t[1][:, 0], t[2][:, 0]
+        with forward_ad.dual_level():
+            dual = lambda a, s: forward_ad.make_dual(rep(a), _dev(s))  # noqa: E731
+            out = ensemble_nodes(
+                dual(p0, t[0]), dual(mr, t[1]), dual(mi, t[2]),
+                None if fr is None else dual(fr, t[3]), rep(nodes), rep(dl), rep(wlt),
+                n_angles=na, order=L, normalized=False, replicas=k)
+            return [forward_ad.unpack_dual(o).tangent.cpu().numpy() for o in out]
+
+    scale = np.abs(g).max()
+    for j in range(P):
+        ts, tg, tp = tangent(j)
+        lhs = (bs * ts).sum() + (bg * tg).sum() + (bp * tp).sum()
+        assert abs(lhs - g[j]) <= 1e-9 * scale, (j, lhs, g[j])
+    # replicas: row 0 carries the tangent, row 1 none
+    ts, tg, tp = tangent(P - 1, k=2)
+    lhs = (bs * ts[:1]).sum() + (bg * tg[:1]).sum() + (bp * tp[:1]).sum()
+    assert abs(lhs - g[P - 1]) <= 1e-9 * scale
+    assert np.all(ts[1] == 0) and np.all(tg[1] == 0) and np.all(tp[1] == 0)
+
+
+def test_adaptive_forward_and_reverse_derivatives_agree(monkeypatch):
+    """The wire's derivative columns in forward mode (one replica per parameter) and in reverse
+    mode (one replica per output component) on the same panels: a two-component mixture with a
+    coated sphere, the phase matrix at three cosines and Greek coefficients; they agree to
+    1e-9 of each column's scale."""
+    import functools
+
+    from scatrs_amd import _wire
+
+    wl = np.array([0.55, 0.9])
+    items = [(("lognormal", 0.25, 1.7), ("sphere", np.full(2, 1.5 + 0.01j)), 1.5),
+             (("gamma", 0.4, 0.12), ("layered", np.array([0.6]),
+                                     np.array([[1.6 + 0.1j] * 2, [1.38 + 1e-3j] * 2])), 1.0)]
+    names = _wire.ensemble_parameters(items, True)
+    idx = list(range(len(names)))
+
+    def run():
+        return _wire.ensemble_compute(items, wl, True, 5, False, np.array([-1.0, 0.2, 1.0]),
+                                      True, idx, 1e-7, 4096, False, False)
+
+    fwd = run()
+    monkeypatch.setattr(_wire, "_run_adaptive",
+                        functools.partial(_wire._run_adaptive, forward=False))
+    rev = run()
+    # the same panels; the values' calls group the rows differently (one call for both rules)
+    assert np.array_equal(fwd["panels"], rev["panels"])
+    assert np.allclose(fwd["values"], rev["values"], rtol=1e-12, atol=0)
+    for key in ("d", "greek_d", "phase_d"):
+        a, b = fwd[key], rev[key]
+        assert a.shape == b.shape == (len(names),) + a.shape[1:]
+        for j in range(len(names)):
+            scale = max(np.abs(b[j]).max(), 1e-300)
+            assert np.abs(a[j] - b[j]).max() <= 1e-9 * scale, (key, names[j])
+
+
 def test_adaptive_derivative_columns_match_central_differences():
     """The wire's adaptive option with derivatives: d and greek_d of a lognormal's median
     radius, width, Re m and Im m, and of a mixture's weight, against central differences of the

@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 
@@ -351,6 +353,149 @@ def test_segment_sum_matches_index_add_and_its_gradient():
     one = segment_sum(torch.tensor([2, 0]), torch.tensor([1.5, 2.5], dtype=torch.float64), 3)
     assert one.tolist() == [2.5, 0.0, 1.5]
     assert segment_sum(torch.zeros(0, dtype=torch.int64), torch.ones(0), 2).tolist() == [0, 0]
+
+
+def _toy_ensembles(order, mu):
+    """CPU stand-ins for ensemble_nodes_device and ensemble_nodes_tangent_device on a toy
+    particle: two complex coefficients per node, analytic in each layer's m and linear in the
+    core fraction, with C_ext linear and C_sca, C_sca g, C_back, the Greek rows and the phase
+    rows quadratic in them (as Mie sums are), C_abs = C_ext - C_sca; the tangent call moves
+    every sum with the coefficients, as the library's does (its absorption sum by the absorption
+    identity's derivative)."""
+    import torch
+
+    from scatrs_amd.ensemble import assemble
+
+    mu = torch.as_tensor(mu, dtype=torch.float64)
+    ell = torch.arange(order + 1, dtype=torch.float64)
+
+    def coeffs(x, m, fr):
+        ms = m[:, -1:]
+        core = 1.0 + 0.2 * (m[:, :1] - ms) * (fr[:, :1] if m.shape[1] > 1 else 0.0)
+        base = (ms * ms - 1.0) / (ms * ms + 2.0)
+        c1 = base * x ** 3 * core * (1.0 + 0.1j * x)
+        c2 = ms * x * torch.exp(-x / 10.0) * (0.3 + 0.05j) * core
+        return c1, c2
+
+    def sums(nodes, wl, c1, c2, wscale):
+        r, W = nodes[..., 0], nodes[..., 1] * wscale
+        area = math.pi * r * r
+        ext = area * (c1 + c2).real
+        sca = area * (c1.abs() ** 2 + c2.abs() ** 2)
+        absb = area * ((c1 + c2).real - c1.abs() ** 2 - c2.abs() ** 2)
+        back = area * (c1 - c2).abs() ** 2
+        scag = area * (c1 * c2.conj()).real
+        lin = torch.stack([ext, sca, absb, back, scag, r * r, r ** 3], -1)
+        pair = torch.stack([c1.abs() ** 2, c2.abs() ** 2, (c1 * c2.conj()).real,
+                            (c1 * c2.conj()).imag, (c1 + c2).abs() ** 2, (c1 - 0.5 * c2).real ** 2],
+                           -1) * area[..., None]
+        gr = pair[..., :, None] / (1.0 + ell)  # (C, N, 6, order + 1)
+        ph = (c1[..., None, None] + mu * c2[..., None, None]).abs() ** 2 * (1.0 + torch.arange(
+            6.0, dtype=torch.float64))[:, None] * area[..., None, None]
+        z = W.sum(1)
+        avg = (W[..., None] * lin).sum(1) / z[:, None]
+        g = (W[..., None, None] * gr).sum(1) / z[:, None, None]
+        p = (W[..., None, None] * ph).sum(1) / z[:, None, None]
+        return assemble(avg, g, p, normalized=False)
+
+    def split(m, fr):
+        mm = m if m.dim() == 2 else m[:, None]
+        f = fr if fr is not None else torch.zeros((mm.shape[0], 0), dtype=torch.float64)
+        return mm, f
+
+    def forward(nodes, wl, m, fr=None, **kw):
+        mm, f = split(m, fr)
+        x = 2.0 * math.pi * nodes[..., 0] / wl[:, None]
+        c1, c2 = coeffs(x, mm, f)
+        sc, g, p = sums(nodes, wl, c1, c2, 1.0)
+        return sc, g, p, 0
+
+    def tangent(nodes, wl, m, fr=None, *, direction, sign, **kw):
+        mm, f = split(m, fr)
+        L = mm.shape[1]
+        x = 2.0 * math.pi * nodes[..., 0] / wl[:, None]
+        em, ef = torch.zeros_like(mm), torch.zeros_like(f)
+        if direction < 2 * L:
+            em[:, direction // 2] = 1.0 if direction % 2 == 0 else 1j
+        else:
+            ef[:, direction - 2 * L] = 1.0
+        c0 = coeffs(x, mm, f)
+        hh = 1e-6  # dc by a central difference (the toy is smooth; its error is ~1e-12)
+        cp, cm = coeffs(x, mm + hh * em, f + hh * ef), coeffs(x, mm - hh * em, f - hh * ef)
+        dc = [(a - b) / (2 * hh) for a, b in zip(cp, cm)]
+        step = sign / (1.0 + x)
+        c1, c2 = (c + step * d for c, d in zip(c0, dc))
+        sc, g, p = sums(nodes, wl, c1, c2, (1.0 + x) / 2.0)
+        return sc, g, p, 0
+
+    return forward, tangent
+
+
+@pytest.mark.parametrize("layers", [1, 2])
+def test_nodes_forward_mode_matches_central_differences(monkeypatch, layers):
+    """ensemble_nodes' forward mode (the host side: weights' part from a shifted positive
+    weighting, particle directions from the library's sign +-1 tangent calls, replica runs
+    each with its own tangent) on toy per-node coefficients: every output's tangent within
+    1e-7 of a central difference of the same toy's averages, for a distribution direction, the
+    particle's directions (Re m, Im m per layer, the core fraction) and a mix."""
+    import torch
+    from torch.autograd import forward_ad
+
+    from scatrs_amd import ensemble
+
+    order, mu = 3, [-1.0, -0.2, 0.5, 1.0]
+    fwd, tan = _toy_ensembles(order, mu)
+    monkeypatch.setattr(ensemble, "ensemble_nodes_device", fwd)
+    monkeypatch.setattr(ensemble, "ensemble_nodes_tangent_device", tan)
+    rng = np.random.default_rng(3)
+    C, N, k = 3, 9, 4
+    f64 = torch.float64
+    r = torch.from_numpy(np.sort(rng.uniform(0.05, 1.5, (C, N)), 1))
+    W0 = torch.from_numpy(rng.uniform(0.1, 1.0, (C, N)))
+    dlnw = torch.from_numpy(rng.normal(size=(C, N, 3)))
+    wl = torch.tensor([0.4, 0.55, 1.0], dtype=f64)
+    m = torch.from_numpy(rng.uniform(1.3, 1.6, (C, layers)) + 1j * rng.uniform(1e-3, 0.1,
+                                                                               (C, layers)))
+    fr = torch.from_numpy(rng.uniform(0.3, 0.8, (C, 1))) if layers > 1 else None
+    p0 = torch.from_numpy(rng.normal(size=(C, 3)))
+    rep = lambda a: None if a is None else a.repeat_interleave(k, 0)  # noqa: E731
+    nodes = torch.stack([r, W0], -1)
+    # replica tangents: a distribution direction, Re m (outer layer), Im m (core), a mix
+    dp = torch.zeros((C * k, 3), dtype=f64)
+    dmr = torch.zeros((C * k, layers), dtype=f64)
+    dmi = torch.zeros((C * k, layers), dtype=f64)
+    dfr = torch.zeros((C * k, 1), dtype=f64) if layers > 1 else None
+    dp[0::k] = torch.tensor([1.0, -0.5, 0.25], dtype=f64)
+    dmr[1::k, -1] = 1.0
+    dmi[2::k, 0] = 1.0
+    dp[3::k, 1] = 0.3
+    dmr[3::k, 0] = -0.7
+    dmi[3::k, -1] = 0.4
+    if layers > 1:
+        dfr[3::k, 0] = 1.3
+    squeeze = (lambda a: a[:, 0]) if layers == 1 else (lambda a: a)  # noqa: E731
+    with forward_ad.dual_level():
+        dual = lambda a, t: forward_ad.make_dual(a, t)  # noqa: E731
+        out = ensemble.ensemble_nodes(
+            dual(rep(p0), dp), dual(squeeze(rep(m.real)), squeeze(dmr)),
+            dual(squeeze(rep(m.imag)), squeeze(dmi)),
+            None if fr is None else dual(rep(fr), dfr), rep(nodes), rep(dlnw), rep(wl),
+            order=order, mu=mu, normalized=False, replicas=k)
+        tg = [forward_ad.unpack_dual(o).tangent for o in out]
+    h = 1e-5
+
+    def at(s):
+        w = W0.repeat_interleave(k, 0) * torch.exp(s * (rep(dlnw) * dp[:, None, :]).sum(-1))
+        mm = rep(m) + s * (dmr + 1j * dmi)
+        ff = None if fr is None else rep(fr) + s * dfr
+        return fwd(torch.stack([rep(r), w], -1), rep(wl), mm, ff)[:3]
+
+    plus, minus = at(h), at(-h)
+    for t, a, b in zip(tg, plus, minus):
+        fd = (a - b) / (2 * h)
+        scale = fd.abs().amax() + 1e-300
+        assert torch.allclose(t, fd, rtol=1e-7, atol=1e-7 * float(scale)), (t - fd).abs().max()
+    assert torch.all(tg[0][:, 7] == 0) and torch.all(tg[0][1::k, 8:] == 0)
 
 
 def test_shared_panels_integrate_every_member(monkeypatch):
