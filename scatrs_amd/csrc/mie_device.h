@@ -20,6 +20,7 @@ constexpr double kEps = 2.220446049250313080847e-16;  // 2^-52
 constexpr double kBits = 52.0;
 constexpr int kMargin = 30;  // ceil(bits / 2) + 4 (mie/kernel.rs:82-84)
 
+
 // N = ceil(x + 6 x^(1/3) + 3) (scatrs/src/mie/mod.rs:122-124).
 SCATRS_HD long long default_n_max(double x) {
     const double n = ceil(x + 6.0 * cbrt(x) + 3.0);
@@ -88,7 +89,9 @@ __device__ __forceinline__ cx inv_plain(cx z, PlainCheck &pc) {
 // 1/za and 1/zb from one reciprocal, 1/(|za|^2 |zb|^2), for two independent inverses of the
 // same order: within a few ulps of inv's wherever both squared moduli lie in (1e-150, 1e150),
 // which the block checks afterwards from pc (pc.ok_pair()).
-__device__ __forceinline__ void inv_plain_pair(cx za, cx zb, PlainCheck &pc, cx &ia, cx &ib) {
+// na, nb receive 1/|za|^2 and 1/|zb|^2.
+__device__ __forceinline__ void inv_plain_pair(cx za, cx zb, PlainCheck &pc, cx &ia, cx &ib,
+                                               double &na, double &nb) {
     const double sa = za.re * za.re + za.im * za.im;
     const double sb = zb.re * zb.re + zb.im * zb.im;
     pc.lo = fmin(pc.lo, fmin(sa, sb));
@@ -97,6 +100,8 @@ __device__ __forceinline__ void inv_plain_pair(cx za, cx zb, PlainCheck &pc, cx 
     const double ta = sb * t, tb = sa * t;
     ia = cx{za.re * ta, -(za.im * ta)};
     ib = cx{zb.re * tb, -(zb.im * tb)};
+    na = ta;
+    nb = tb;
 }
 
 // step_complex on the plain path (its result is step_complex's when pc.ok() after the block).
@@ -155,13 +160,19 @@ constexpr int kSphereBlock = 8;
 
 struct ZColumn {
     static constexpr bool kBlocked = false;
+    static constexpr bool kPlain = false;
     const cx *uz;
     int stride;
 };
 
-template <int B>
+// PLAIN = true: every full block (of the downward pass, of the regeneration and of the orders
+// below floor(x)) runs on the plain path without a redo, its arguments noted in pc (the
+// downward steps) and pc_pair (the orders' paired inverses); the caller redoes the sphere on the
+// checked path (PLAIN = false) in the rare lane whose checks fail.
+template <int B, bool PLAIN = false>
 struct ZCheckpointT {
     static constexpr bool kBlocked = true;
+    static constexpr bool kPlain = PLAIN;
     static constexpr int kBlock = B;
     const cx *ckpt;     // ckpt[j * stride]: u at index 2 + (j + 1) B
     int stride;
@@ -170,6 +181,10 @@ struct ZCheckpointT {
     double inv_rho;     // 1/Re(mx), real chain
     bool real_chain;
     long long start;
+    mutable PlainCheck pc, pc_pair;
+
+    // Whether every plain step of the sphere stayed in the plain range (PLAIN only).
+    __device__ __forceinline__ bool plain_ok() const { return pc.ok() && pc_pair.ok_pair(); }
 
     // u_i for i in [2 + j B, 2 + j B + B - 1] into lds[(i - 2 - j B) * 64], regenerated
     // downward from the checkpoint above the block (or from 0 above the start index).
@@ -188,6 +203,14 @@ struct ZCheckpointT {
             // path in the rare lane whose arguments left the plain range.
             const cx u0 = ckpt[(int64_t)j * stride];
             cx u = u0;
+            if constexpr (PLAIN) {
+#pragma unroll
+                for (int q = B - 1; q >= 0; --q) {
+                    u = step_complex_plain((int)lo + q, inv_z, u, pc);
+                    lds[q * 64] = u;
+                }
+                return;
+            }
             PlainCheck pc;
 #pragma unroll
             for (int q = B - 1; q >= 0; --q) {
@@ -215,10 +238,10 @@ using ZCheckpoint = ZCheckpointT<kZBlock>;
 
 // The downward pass of u_k(m x) from the start index, keeping only the checkpoints of
 // ZCheckpointT<B> (indices 2 + (j + 1) B for the blocks that hold orders 1..n).
-template <int B = kZBlock>
-__device__ __forceinline__ ZCheckpointT<B> ratio_chain_ckpt(cx m, double x, int n, cx *ckpt,
-                                                           int stride, cx *lds) {
-    ZCheckpointT<B> zs;
+template <int B = kZBlock, bool PLAIN = false>
+__device__ __forceinline__ ZCheckpointT<B, PLAIN> ratio_chain_ckpt(cx m, double x, int n,
+                                                                  cx *ckpt, int stride, cx *lds) {
+    ZCheckpointT<B, PLAIN> zs;
     const cx z = m * x;
     zs.ckpt = ckpt;
     zs.stride = stride;
@@ -246,6 +269,17 @@ __device__ __forceinline__ ZCheckpointT<B> ratio_chain_ckpt(cx m, double x, int 
         // whose first step gives a checkpoint, without branches, each redone on the checked
         // path in the rare lane whose arguments left the plain range.
         for (; k >= c0 && (k - 2) % B != 0; --k) u = step_complex((int)k, zs.inv_z, u);
+        if constexpr (PLAIN) {
+            for (; k > c0; k -= B) {
+                cx first;
+#pragma unroll
+                for (int q = 0; q < B; ++q) {
+                    u = step_complex_plain((int)(k - q), zs.inv_z, u, zs.pc);
+                    if (q == 0) first = u;
+                }
+                if (k <= cmax) ckpt[((k - 2) / B - 1) * stride] = first;
+            }
+        }
         for (; k > c0; k -= B) {
             const cx u0 = u;
             PlainCheck pc;
@@ -348,11 +382,15 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
     const bool m_is_one = (m.re == 1.0 && m.im == 0.0);
     double abs_sum = 0.0;
     double jac0 = 0.0, jac1 = 0.0, jac2 = 0.0;
-    // One order of the loop, with u_{k+1}(mx) in registers. The plain form (fast = true, for
-    // spheres without derivatives, orders below n_s) takes the upward branch without testing
-    // for it and inverts on the plain path of inv, noting its arguments in pc.
-    auto order_step_t = [&](int k, const cx u_z1, auto fast, PlainCheck &pc) {
-        constexpr bool FAST = decltype(fast)::value;
+    // One order of the loop, with u_{k+1}(mx) in registers. The plain forms (for spheres
+    // without derivatives) take their branch without testing for it and invert on the plain
+    // path of inv, noting the paired inverses' arguments in pc and the single ones' in pcr:
+    // mode 1 for orders up to n_s (the upward branch), mode 2 for orders above it (the ratio
+    // region); mode 0 is the checked form of either.
+    auto order_step_t = [&](int k, const cx u_z1, auto mode, PlainCheck &pc, PlainCheck &pcr) {
+        constexpr int MODE = decltype(mode)::value;
+        constexpr bool FAST = MODE == 1;
+        const bool upward = MODE == 1 ? true : (MODE == 2 ? false : k <= n_s);
         const double kf = (double)k;
         const double k_x = kf * inv_x;
 
@@ -369,7 +407,8 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
         const cx big_a = d_m + k_x;
         const cx big_b = md + k_x;
         cx an, bn, inv_da, inv_db, g_a, g_b, h_a, h_b;
-        if (FAST || k <= n_s) {
+        double na = 0.0, nb = 0.0;
+        if (upward) {
             const double c = (double)(2 * k - 1) * inv_x;
             const double psi_next = c * psi - psi_prev;
             psi_prev = psi;
@@ -380,9 +419,11 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             const cx xi = mk(psi, -chi);
             const cx xi_prev = mk(psi_prev, -chi_prev);
             if constexpr (FAST) {
-                inv_plain_pair(big_a * xi - xi_prev, big_b * xi - xi_prev, pc, inv_da, inv_db);
+                inv_plain_pair(big_a * xi - xi_prev, big_b * xi - xi_prev, pc, inv_da, inv_db,
+                               na, nb);
             } else {
                 (void)pc;
+                (void)pcr;
                 inv_da = inv(big_a * xi - xi_prev);
                 inv_db = inv(big_b * xi - xi_prev);
             }
@@ -408,13 +449,16 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             g_b = xi * inv_db;
             h_a = xi_prev * inv_da;
             h_b = xi_prev * inv_db;
-            if (!FAST && k == n_s) {
+            if (MODE == 0 && k == n_s) {
                 inv_xi = inv(xi);
                 t = real(psi) * inv_xi;
                 r = xi_prev * inv_xi;
             }
         } else {
-            r = inv((double)(2 * k - 1) * inv_x - r);
+            if constexpr (MODE == 2)
+                r = inv_plain((double)(2 * k - 1) * inv_x - r, pcr);
+            else
+                r = inv((double)(2 * k - 1) * inv_x - r);
             t = (t * r) * ux[k * stride];
             inv_xi = inv_xi * r;
             const double ux1 = ux[(k + 1) * stride];
@@ -425,8 +469,14 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
                 num_a = side.direct_a ? d_m - d_x : num_a + side.va * inv_m;
                 num_b = side.direct_b ? md - d_x : num_b + m * side.vb;
             }
-            const cx ia = inv(big_a - r);
-            const cx ib = inv(big_b - r);
+            cx ia, ib;
+            if constexpr (MODE == 2) {
+                double ta, tb;
+                inv_plain_pair(big_a - r, big_b - r, pc, ia, ib, ta, tb);
+            } else {
+                ia = inv(big_a - r);
+                ib = inv(big_b - r);
+            }
             an = (t * num_a) * ia;
             bn = (t * num_b) * ib;
             inv_da = inv_xi * ia;
@@ -436,7 +486,11 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             h_a = r * ia;
             h_b = r * ib;
         }
-        const double na = norm2(inv_da), nb = norm2(inv_db);
+        // |1/Delta|^2: from the paired reciprocal on the plain upward path
+        if constexpr (!FAST) {
+            na = norm2(inv_da);
+            nb = norm2(inv_db);
+        }
         const double weight = (double)(2 * k + 1);
         abs_sum += weight * (-(d_m.im * na) - md.im * nb);
         if constexpr (!DERIVS) {
@@ -486,7 +540,7 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
     };
     auto order_step = [&](int k, const cx u_z1) {
         PlainCheck pc;
-        order_step_t(k, u_z1, std::false_type{}, pc);
+        order_step_t(k, u_z1, std::integral_constant<int, 0>{}, pc, pc);
     };
     if constexpr (ZSrc::kBlocked) {
         constexpr int B = ZSrc::kBlock;
@@ -498,14 +552,35 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
                 // A block wholly below n_s: B orders without branches, redone on the checked
                 // path (from the saved state) in the rare lane whose inverses left the plain
                 // range.
-                if (k0 + B - 1 < n_s) {
+                if constexpr (ZSrc::kPlain) {
+                    if (k0 + B - 1 < n_s) {
+#pragma unroll
+                        for (int q = 0; q < B; ++q) {
+                            order_step_t(k0 + q, zs.lds[q * 64], std::integral_constant<int, 1>{},
+                                         zs.pc_pair, zs.pc);
+                        }
+                        if constexpr (has_block_end<Sink>::value) sink.block_end(k0, k0 + B - 1);
+                        continue;
+                    }
+                    // A full block above n_s (the ratio region).
+                    if (k0 > n_s && kend == k0 + B - 1) {
+#pragma unroll
+                        for (int q = 0; q < B; ++q) {
+                            order_step_t(k0 + q, zs.lds[q * 64], std::integral_constant<int, 2>{},
+                                         zs.pc_pair, zs.pc);
+                        }
+                        if constexpr (has_block_end<Sink>::value) sink.block_end(k0, kend);
+                        continue;
+                    }
+                } else if (k0 + B - 1 < n_s) {
                     const double s_psi = psi, s_psi_prev = psi_prev, s_chi = chi,
                                  s_chi_prev = chi_prev, s_abs = abs_sum;
                     const Sink s_sink = sink;
                     PlainCheck pc;
 #pragma unroll
                     for (int q = 0; q < B; ++q)
-                        order_step_t(k0 + q, zs.lds[q * 64], std::true_type{}, pc);
+                        order_step_t(k0 + q, zs.lds[q * 64], std::integral_constant<int, 1>{}, pc,
+                                     pc);
                     if (!pc.ok_pair()) {
                         psi = s_psi;
                         psi_prev = s_psi_prev;
@@ -604,6 +679,7 @@ struct EfficiencySink {
     cx back = {0.0, 0.0};
     cx a_prev = {0.0, 0.0}, b_prev = {0.0, 0.0};
     DerivSums ds;
+
     // 1/k of the coming order: each order takes one reciprocal, 1/(k + 1), and keeps it for the
     // next, for its pair and own weights pair(k - 1) = (k - 1)(k + 1)/k and
     // own(k) = (2k + 1)/(k (k + 1)).
@@ -677,5 +753,44 @@ struct EfficiencySink {
         ds.jacobian(x, q, sca, back, outer, out);
     }
 };
+
+// The sink of a sphere's checked redo: its orders below floor(x) take the checked path one by
+// one, without the branch-free blocks.
+template <bool DERIVS>
+struct CheckedEfficiencySink : EfficiencySink<DERIVS> {
+    static constexpr bool kPlainBlocks = false;
+};
+
+// One sphere's efficiencies (and Jacobian): the downward u_k(mx) pass keeps one checkpoint per
+// kSphereBlock indices in ck (lane-strided by 64), the order loop regenerates each block into
+// the lane's LDS tile, u_k(x) above floor(x) lives in ux. Without derivatives the whole sphere
+// first runs on the plain path (no per-block state kept for a redo) and, in the rare lane whose
+// inverses left the plain range, again on the checked path.
+template <int B, bool DERIVS>
+__device__ __forceinline__ void sphere_efficiencies(double x, cx m, int n, cx *ck, double *ux,
+                                                    cx *tile, double q[5], double *jac15) {
+    x_chain(x, n, upward_orders(x, n), ux, 64);
+    NoOuter none;
+    if constexpr (!DERIVS) {
+        (void)jac15;
+        EfficiencySink<false> s;
+        const auto zs = ratio_chain_ckpt<B, true>(m, x, n, ck, 64, tile);
+        mie_orders<false>(x, m, n, zs, ux, 64, none, s);
+        if (zs.plain_ok()) {
+            s.values(x, q);
+            return;
+        }
+        CheckedEfficiencySink<false> c;
+        const auto zc = ratio_chain_ckpt<B>(m, x, n, ck, 64, tile);
+        mie_orders<false>(x, m, n, zc, ux, 64, none, c);
+        c.values(x, q);
+    } else {
+        EfficiencySink<true> s;
+        const auto zs = ratio_chain_ckpt<B>(m, x, n, ck, 64, tile);
+        mie_orders<true>(x, m, n, zs, ux, 64, none, s);
+        s.values(x, q);
+        s.jacobian(x, q, true, jac15);
+    }
+}
 
 }  // namespace scatrs

@@ -125,18 +125,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             const double xi = x[i];
             const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
             const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
-            const auto zs = ratio_chain_ckpt<kSphereBlock>(mi, xi, n, ck, 64, tile);
-            x_chain(xi, n, upward_orders(xi, n), ux, 64);
-            EfficiencySink<DERIVS> s;
-            NoOuter none;
-            mie_orders<DERIVS>(xi, mi, n, zs, ux, 64, none, s);
-            double q[5];
-            s.values(xi, q);
+            double q[5], j[15];
+            sphere_efficiencies<kSphereBlock, DERIVS>(xi, mi, n, ck, ux, tile, q, j);
 #pragma unroll
             for (int k = 0; k < 5; ++k) values[k * len + i] = q[k];
             if constexpr (DERIVS) {
-                double j[15];
-                s.jacobian(xi, q, true, j);
 #pragma unroll
                 for (int k = 0; k < 15; ++k) jac[k * len + i] = j[k];
             }

@@ -236,15 +236,14 @@ __global__ __launch_bounds__(256) void single_solve(const double *__restrict__ x
         sink.d = DERIVS ? out.d + i * out.width : nullptr;
         solve(sink);
     } else if constexpr (L == 1) {
-        EfficiencySink<DERIVS> sink;
-        solve(sink);
-        double q[5];
-        sink.values(xi[0], q);
+        // the sphere batch's own solve, so that one layer gives the sphere bit for bit
+        double q[5], j[15];
+        sphere_efficiencies<kSphereBlock, DERIVS>(
+            xi[0], mk(mi[0], mi[1]), n, slab, real_column(slab, ((rows - 2) / kSphereBlock + 2) * 64),
+            &ztile[threadIdx.x >> 6][lane], q, j);
 #pragma unroll
         for (int k = 0; k < 5; ++k) out.values[k * count + i] = q[k];
         if constexpr (DERIVS) {
-            double j[15];
-            sink.jacobian(xi[0], q, true, j);
 #pragma unroll
             for (int k = 0; k < 15; ++k) out.jac[k * count + i] = j[k];
         }
