@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of mie_batch on this workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 from
 # two rocprofv3 --pmc runs, scaled by what the counters read for a 1 GiB device copy
 # (profiles/mie_batch_spheres_2p20.md).
-MIE_BATCH_TRAFFIC_BYTES = 1.627e9
+MIE_BATCH_TRAFFIC_BYTES = 1.602e9
 # mie_batch on the same workload under rocprofv3 --kernel-trace --stats
 # (profiles/mie_batch_spheres_2p20.md): the average duration of its launches.
-MIE_BATCH_ROCPROF = {"kernel_ms": 1.0118, "source": "profiles/mie_batch_spheres_2p20.md"}
+MIE_BATCH_ROCPROF = {"kernel_ms": 0.9629, "source": "profiles/mie_batch_spheres_2p20.md"}
 
 
 # The ensemble legs after the sphere batch, in the order they run.

@@ -75,12 +75,17 @@ int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
  *   SCATRS_TUNE_CHUNK_SHARE: a chunk of ensembles stages its coefficients in 1/share of the free
  *     device memory (1-64, default 4; at most 64 GiB, at least 1 GiB);
  *   SCATRS_TUNE_SPLIT_BLOCKS: blocks per CU that a small batch's FP64 matrix-core angular
- *     products aim for by dealing node groups over blocks (1-64, default 2).
+ *     products aim for by dealing node groups over blocks (1-64, default 2);
+ *   SCATRS_TUNE_SPHERE_PLAIN: 1 (default) solves each sphere of the sphere batch first on the
+ *     branch-free plain path and redoes it on the checked path (the reference's branch per
+ *     complex inverse) only where an argument left the plain range; 0 takes the checked path for
+ *     every sphere (same results to rounding; for tests of that path).
  * scatrs_get_tuning returns the value, or -1 for an unknown key. */
 #define SCATRS_TUNE_SPHERE_WAVES 0
 #define SCATRS_TUNE_NODE_WAVES 1
 #define SCATRS_TUNE_CHUNK_SHARE 2
 #define SCATRS_TUNE_SPLIT_BLOCKS 3
+#define SCATRS_TUNE_SPHERE_PLAIN 4
 int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value);
 int64_t scatrs_get_tuning(const scatrs_ctx *ctx, int key);
 

@@ -763,25 +763,29 @@ struct CheckedEfficiencySink : EfficiencySink<DERIVS> {
 
 // One sphere's efficiencies (and Jacobian): the downward u_k(mx) pass keeps one checkpoint per
 // kSphereBlock indices in ck (lane-strided by 64), the order loop regenerates each block into
-// the lane's LDS tile, u_k(x) above floor(x) lives in ux. Without derivatives the whole sphere
-// first runs on the plain path (no per-block state kept for a redo) and, in the rare lane whose
-// inverses left the plain range, again on the checked path.
+// the lane's LDS tile, u_k(x) above floor(x) lives in ux. Without derivatives (and with
+// `plain`) the whole sphere first runs on the plain path (no per-block state kept for a redo)
+// and, in the rare lane whose inverses left the plain range, again on the checked path.
 template <int B, bool DERIVS>
 __device__ __forceinline__ void sphere_efficiencies(double x, cx m, int n, cx *ck, double *ux,
-                                                    cx *tile, double q[5], double *jac15) {
+                                                    cx *tile, bool plain, double q[5],
+                                                    double *jac15) {
     x_chain(x, n, upward_orders(x, n), ux, 64);
     NoOuter none;
     if constexpr (!DERIVS) {
         (void)jac15;
-        EfficiencySink<false> s;
-        const auto zs = ratio_chain_ckpt<B, true>(m, x, n, ck, 64, tile);
-        mie_orders<false>(x, m, n, zs, ux, 64, none, s);
-        if (zs.plain_ok()) {
-            s.values(x, q);
-            return;
+        if (plain) {
+            EfficiencySink<false> s;
+            const auto zs = ratio_chain_ckpt<B, true>(m, x, n, ck, 64, tile);
+            mie_orders<false>(x, m, n, zs, ux, 64, none, s);
+            if (zs.plain_ok()) {
+                s.values(x, q);
+                return;
+            }
         }
         CheckedEfficiencySink<false> c;
         const auto zc = ratio_chain_ckpt<B>(m, x, n, ck, 64, tile);
+        (void)plain;
         mie_orders<false>(x, m, n, zc, ux, 64, none, c);
         c.values(x, q);
     } else {

@@ -106,7 +106,7 @@ template <bool DERIVS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mie_batch(
     const double *__restrict__ x, const double *__restrict__ m, int64_t len, int64_t n_fixed,
     const int *__restrict__ perm, unsigned long long *chunk_counter, cx *ck_base,
-    double *ux_base, int64_t rows_ck, int64_t rows, double *__restrict__ values,
+    double *ux_base, int64_t rows_ck, int64_t rows, int plain, double *__restrict__ values,
     double *__restrict__ jac) {
     __shared__ cx ztile[4][kSphereBlock * 64];
     const int lane = threadIdx.x & 63;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
             const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
             double q[5], j[15];
-            sphere_efficiencies<kSphereBlock, DERIVS>(xi, mi, n, ck, ux, tile, q, j);
+            sphere_efficiencies<kSphereBlock, DERIVS>(xi, mi, n, ck, ux, tile, plain != 0, q, j);
 #pragma unroll
             for (int k = 0; k < 5; ++k) values[k * len + i] = q[k];
             if constexpr (DERIVS) {
@@ -231,11 +231,14 @@ int mie_solve(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double
     if (d_jacobian)
         mie_batch<true><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, w.perm,
                                                              chunk_counter, ck, ux, rows_ck,
-                                                             rows, d_values, d_jacobian);
+                                                             rows, ctx->tune[SCATRS_TUNE_SPHERE_PLAIN],
+                                                             d_values, d_jacobian);
     else
         mie_batch<false><<<(unsigned)(waves / 4), 256, 0, s>>>(d_x, d_m, len, n_max, w.perm,
                                                               chunk_counter, ck, ux, rows_ck,
-                                                              rows, d_values, nullptr);
+                                                              rows,
+                                                              ctx->tune[SCATRS_TUNE_SPHERE_PLAIN],
+                                                              d_values, nullptr);
     mark_end(ctx);
     return check_hip(ctx, hipGetLastError(), "mie_batch launch");
 }
@@ -341,8 +344,8 @@ int scatrs_set_profiling(scatrs_ctx *ctx, int on) {
 int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value) {
     if (!ctx) return SCATRS_INVALID_INPUT;
     ctx->error.clear();
-    static const int64_t lo[4] = {1, 1, 1, 1}, hi[4] = {32, 32, 64, 64};
-    if (key < 0 || key > 3)
+    static const int64_t lo[5] = {1, 1, 1, 1, 0}, hi[5] = {32, 32, 64, 64, 1};
+    if (key < 0 || key > 4)
         return fail(ctx, SCATRS_INVALID_INPUT, "unknown tuning key " + std::to_string(key));
     if (value < lo[key] || value > hi[key])
         return fail(ctx, SCATRS_INVALID_INPUT,
@@ -354,7 +357,7 @@ int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value) {
 }
 
 int64_t scatrs_get_tuning(const scatrs_ctx *ctx, int key) {
-    if (!ctx || key < 0 || key > 3) return -1;
+    if (!ctx || key < 0 || key > 4) return -1;
     return ctx->tune[key];
 }
 

@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void single_solve(const double *__restrict__ x
         double q[5], j[15];
         sphere_efficiencies<kSphereBlock, DERIVS>(
             xi[0], mk(mi[0], mi[1]), n, slab, real_column(slab, ((rows - 2) / kSphereBlock + 2) * 64),
-            &ztile[threadIdx.x >> 6][lane], q, j);
+            &ztile[threadIdx.x >> 6][lane], true, q, j);
 #pragma unroll
         for (int k = 0; k < 5; ++k) out.values[k * count + i] = q[k];
         if constexpr (DERIVS) {

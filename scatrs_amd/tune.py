@@ -7,7 +7,9 @@ The parameters (keys of ``KEYS``):
   * ``node_waves``: persistent wavefronts per CU of the ensembles' sphere node solves;
   * ``chunk_share``: a chunk of ensembles stages its coefficients in 1/share of the free device
     memory (larger chunks wait less often for their slowest node, smaller ones leave memory);
-  * ``split_blocks``: blocks per CU a small batch's angular products aim for.
+  * ``split_blocks``: blocks per CU a small batch's angular products aim for;
+  * ``sphere_plain`` (not tuned: a switch for tests): 0 runs every sphere of the sphere batch on
+    the checked path instead of trying the branch-free plain path first.
 
 Results do not depend on the first three; the fourth changes a small batch's summation order
 (rounding only). Tuning coordinate by coordinate: each parameter in turn over its candidates,
@@ -23,8 +25,10 @@ import time
 from pathlib import Path
 from typing import Callable, Sequence
 
-KEYS = {"sphere_waves": 0, "node_waves": 1, "chunk_share": 2, "split_blocks": 3}
-DEFAULTS = {"sphere_waves": 16, "node_waves": 8, "chunk_share": 4, "split_blocks": 2}
+KEYS = {"sphere_waves": 0, "node_waves": 1, "chunk_share": 2, "split_blocks": 3,
+        "sphere_plain": 4}
+DEFAULTS = {"sphere_waves": 16, "node_waves": 8, "chunk_share": 4, "split_blocks": 2,
+            "sphere_plain": 1}
 CANDIDATES = {
     "sphere_waves": (4, 6, 8, 12, 16, 20),
     "node_waves": (4, 6, 8, 12),

@@ -92,6 +92,38 @@ def test_parity_with_oracle_on_batched_spheres():
     assert worst < 1e-7
 
 
+@pytest.mark.parametrize("plain", [1, 0])
+def test_values_without_derivatives_on_both_paths(plain):
+    """The values-only batch (the branch-free plain path tried first, and with
+    SCATRS_TUNE_SPHERE_PLAIN = 0 the checked path for every sphere) within 2x the reference's
+    declared tolerance of the oracle, and the two paths within a few ulps of each other."""
+    from scatrs_amd import _native, tune
+
+    x, m = _seeded(20000, seed=11)
+    ref, jref = oracle.mie_efficiencies_batch(x, m, derivatives=True)
+    ctx = _native.context(0)
+    try:
+        ctx.set_tuning(tune.KEYS["sphere_plain"], plain)
+        assert ctx.get_tuning(tune.KEYS["sphere_plain"]) == plain
+        got = _stack(_gpu(x, m))
+        ctx.set_tuning(tune.KEYS["sphere_plain"], 1 - plain)
+        other = _stack(_gpu(x, m))
+    finally:
+        ctx.set_tuning(tune.KEYS["sphere_plain"], 1)
+    for i in range(x.size):
+        q = ref[:, i]
+        sizes = efficiency_sizes(q, x[i])
+        conds = efficiency_conditions(q, jref[:, :, i], x[i], m[i])
+        for k in range(5):
+            tol = 2.0 * tolerance("efficiencies", x[i], m[i], conds[k])
+            assert abs(got[k, i] - q[k]) / sizes[k] <= tol, (i, k, x[i], m[i], got[k, i], q[k])
+    # Q_ext, Q_sca and g of the two paths: the same recurrences, rounded differently only where
+    # the plain path forms the two inverses of an order from one reciprocal.
+    rel = np.abs(got - other)[[0, 1, 4]] / np.abs(other[[0, 1, 4]])
+    assert rel.max() < 1e-12, rel.max()
+    assert np.all(got[0] == got[1] + got[2])
+
+
 def test_real_index_gives_zero_absorption():
     x = np.array([0.5, 3.0, 40.0, 700.0])
     m = np.array([1.33, 1.5, 1.7, 1.45], dtype=np.complex128)
