@@ -300,7 +300,9 @@ int scatrs_ensemble_nodes(scatrs_ctx *ctx, int64_t count, int64_t n_nodes, const
  * weight W scaled by 1/(2 eps). Every output's weighted sum is linear or quadratic in the
  * coefficients, so the sign = +1 call's sums (average times the row's scaled weight) minus the
  * sign = -1 call's are the sums' derivative along the direction, exactly up to rounding; the
- * C_abs column moves with C_ext - C_sca (the absorption sum is not moved). Replaces the
+ * absorption sum moves by sign eps times its derivative along the direction (the absorption
+ * identity's Jacobian: the outer layer's from the solve, an inner layer's summed over the orders
+ * from its sensitivities), so the C_abs and C_ext columns are exact too. Replaces the
  * reference's forward-mode derivative columns (ensemble/engine.rs:1111-1123, particle.rs:221-249)
  * for the particle's directions: P + 1 evaluations instead of one reverse pass per output. */
 int scatrs_ensemble_nodes_tangent(scatrs_ctx *ctx, int64_t count, int64_t n_nodes,
