@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/scatrs_hip.h"
@@ -259,21 +260,25 @@ __host__ __device__ inline int64_t slab_elems(int layers, int64_t rows) {
 // Writes a node's coefficients to the staging buffer, interleaved: in FP64 (a_1, b_1, a_2, b_2,
 // ...: one 32-byte stream per lane, whose cache lines fill every four orders) or, for the
 // fp16-split angular products, in fp32 (one float4 (a.re, a.im, b.re, b.im) per order); and
-// accumulates its efficiency sums.
+// accumulates its efficiency sums. MODE (fixed per launch, so that the order loop carries no
+// branch on it): kStageF64; kStageF32 (one float4 per order straight to the buffer); kStageTile
+// (spheres' fp32 form: each order's float4 goes to the lane's slot of the order loop's LDS tile,
+// its u_{k+1}(mx) consumed by then, and the block's orders go out together at its end, 16 B x 16
+// contiguous per lane, so that the staging buffer receives whole cache lines).
+constexpr int kStageF64 = 0, kStageF32 = 1, kStageTile = 2;
+
+template <int MODE>
 struct StagingSink {
     cx *ab_out;
-    float4 *ab32;       // not null: the fp32 form
-    // Spheres' fp32 form: each order's float4 goes to the lane's slot of the order loop's LDS
-    // tile (its u_{k+1}(mx) is consumed by then) and the block's orders go out together at its
-    // end, 16 B x 16 contiguous per lane, so the staging buffer receives whole cache lines.
+    float4 *ab32;
     float4 *tile32 = nullptr;
     EfficiencySink<false> eff;
     double amax = 0.0;  // the largest |Re|, |Im| of the node's coefficients
     __device__ void order(int k, cx a, cx b) {
-        if (tile32) {
+        if constexpr (MODE == kStageTile) {
             tile32[((k - 1) % kZBlock) * 64] =
                 make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
-        } else if (ab32) {
+        } else if constexpr (MODE == kStageF32) {
             ab32[k - 1] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
         } else {
             ab_out[2 * (k - 1)] = a;
@@ -283,8 +288,11 @@ struct StagingSink {
         eff.order(k, a, b);
     }
     __device__ void block_end(int k0, int kend) {
-        if (!tile32) return;
-        for (int k = k0; k <= kend; ++k) ab32[k - 1] = tile32[(k - k0) * 64];
+        if constexpr (MODE == kStageTile) {
+            for (int k = k0; k <= kend; ++k) ab32[k - 1] = tile32[(k - k0) * 64];
+        } else {
+            (void)k0, (void)kend;
+        }
     }
     __device__ void order_d(int, cx, cx, cx, cx, cx, cx) {}
     template <class O>
@@ -425,27 +433,52 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     const NodeGeom geo = node_stored(a, e, i);
     const int n = node_n[gi];
     const int64_t off = ens_base[e] + node_off[gi];
-    StagingSink sink;
-    sink.ab_out = staging + 2 * off;
-    sink.ab32 = stage32 ? reinterpret_cast<float4 *>(staging) + off : nullptr;
-    if constexpr (L == 1) {
-        if (stage32) sink.tile32 = reinterpret_cast<float4 *>(tile);
-    }
-    if constexpr (L == 1) {
-        const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
-        // Checkpoints in the first column, u_k(x) in the second, the block being used in the
-        // wavefront's LDS tile.
-        double *ux = real_column(slab, sphere_ckpt_rows(rows) * 64);
-        const ZCheckpoint zs = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
-        x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
-        NoOuter none;
-        mie_orders<false>(geo.x, m, n, zs, ux, 64, none, sink);
+    auto solve = [&](auto &sink) __attribute__((always_inline)) {
+        using Sink = std::remove_reference_t<decltype(sink)>;
+        sink.ab_out = staging + 2 * off;
+        sink.ab32 = reinterpret_cast<float4 *>(staging) + off;
+        sink.tile32 = reinterpret_cast<float4 *>(tile);
+        if constexpr (L == 1) {
+            const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
+            // Checkpoints in the first column, u_k(x) in the second, the block being used in
+            // the wavefront's LDS tile. The node first runs whole on the plain path (branch-free
+            // blocks, no per-block state for a redo) and, in the rare lane whose inverses left
+            // the plain range, again on the checked path, restaging its coefficients.
+            double *ux = real_column(slab, sphere_ckpt_rows(rows) * 64);
+            x_chain(geo.x, n, upward_orders(geo.x, n), ux, 64);
+            NoOuter none;
+            const Sink fresh = sink;
+            bool ok;
+            {
+                const auto zs = ratio_chain_ckpt<kZBlock, true>(m, geo.x, n, slab, 64, tile);
+                mie_orders<false>(geo.x, m, n, zs, ux, 64, none, sink);
+                ok = zs.plain_ok();
+            }
+            if (!ok) {
+                sink = fresh;
+                const ZCheckpoint zc = ratio_chain_ckpt(m, geo.x, n, slab, 64, tile);
+                mie_orders<false>(geo.x, m, n, zc, ux, 64, none, sink);
+            }
+        } else {
+            (void)tile;
+            layered_solve<L, false>(geo.x, a.fr + e * (L - 1), a.m + 2 * e * L, n, slab, rows,
+                                    sink);
+        }
+        return sink.amax;
+    };
+    EfficiencySink<false> eff;
+    double amax;
+    if (stage32) {
+        StagingSink<L == 1 ? kStageTile : kStageF32> sink;
+        amax = solve(sink);
+        eff = sink.eff;
     } else {
-        (void)tile;
-        layered_solve<L, false>(geo.x, a.fr + e * (L - 1), a.m + 2 * e * L, n, slab, rows, sink);
+        StagingSink<kStageF64> sink;
+        amax = solve(sink);
+        eff = sink.eff;
     }
     double q[5];
-    sink.eff.values(geo.x, q);
+    eff.values(geo.x, q);
     const double area = kPi * geo.r * geo.r;
     const double W = geo.W;
     double *v = vals + g * kNodeVals;
@@ -458,8 +491,8 @@ __device__ __forceinline__ void forward_node(const EnsArgs &a, int64_t e0, int64
     v[6] = W * (geo.r * geo.r);
     v[7] = W * (geo.r * geo.r * geo.r);
     tabc[g] = W * (area * (4.0 / (geo.x * geo.x)));
-    xexp[g] = coefficient_exponent(sink.amax);
-    if (back_out) back_out[g] = sink.eff.back;
+    xexp[g] = coefficient_exponent(amax);
+    if (back_out) back_out[g] = eff.back;
 }
 
 // forward_node in tangent mode (a.tan_dir >= 0): the solve with derivatives, the coefficients
@@ -513,7 +546,7 @@ __device__ __forceinline__ void tangent_node(const EnsArgs &a, int64_t e0, int64
 // Spheres: persistent; each wavefront takes the next 64 ranked nodes from the counter, so the
 // nodes of largest N start first and the rest fill in behind them, and the 64 lanes of a
 // wavefront solve nodes of about the same N. Slabs of `rows` rows (the chunk's largest N).
-static __global__ __launch_bounds__(256) void ens_nodes_persist(
+static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void ens_nodes_persist(
     EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, cx *scratch,
     int64_t rows, cx *staging, int stage32, double *vals, double *tabc, int *xexp,

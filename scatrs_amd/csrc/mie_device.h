@@ -548,26 +548,33 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
             zs.regen(j);
             const int k0 = 1 + j * B;
             const int kend = n < k0 + B - 1 ? n : k0 + B - 1;
-            if constexpr (!DERIVS && !Outer::kLayered && plain_blocks<Sink>::value) {
+            if constexpr (!DERIVS && !Outer::kLayered &&
+                          (ZSrc::kPlain || plain_blocks<Sink>::value)) {
                 // A block wholly below n_s: B orders without branches, redone on the checked
                 // path (from the saved state) in the rare lane whose inverses left the plain
                 // range.
                 if constexpr (ZSrc::kPlain) {
+                    // Unrolled 8 orders at a time (blocks of 16 in two halves).
+                    constexpr int U = B < 8 ? B : 8;
                     if (k0 + B - 1 < n_s) {
+#pragma unroll 1
+                        for (int h = 0; h < B; h += U) {
 #pragma unroll
-                        for (int q = 0; q < B; ++q) {
-                            order_step_t(k0 + q, zs.lds[q * 64], std::integral_constant<int, 1>{},
-                                         zs.pc_pair, zs.pc);
+                            for (int q = h; q < h + U; ++q)
+                                order_step_t(k0 + q, zs.lds[q * 64],
+                                             std::integral_constant<int, 1>{}, zs.pc_pair, zs.pc);
                         }
                         if constexpr (has_block_end<Sink>::value) sink.block_end(k0, k0 + B - 1);
                         continue;
                     }
                     // A full block above n_s (the ratio region).
                     if (k0 > n_s && kend == k0 + B - 1) {
+#pragma unroll 1
+                        for (int h = 0; h < B; h += U) {
 #pragma unroll
-                        for (int q = 0; q < B; ++q) {
-                            order_step_t(k0 + q, zs.lds[q * 64], std::integral_constant<int, 2>{},
-                                         zs.pc_pair, zs.pc);
+                            for (int q = h; q < h + U; ++q)
+                                order_step_t(k0 + q, zs.lds[q * 64],
+                                             std::integral_constant<int, 2>{}, zs.pc_pair, zs.pc);
                         }
                         if constexpr (has_block_end<Sink>::value) sink.block_end(k0, kend);
                         continue;
