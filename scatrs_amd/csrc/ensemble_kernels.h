@@ -748,16 +748,17 @@ static __global__ __launch_bounds__(512) void ens_angular(int64_t e0, int64_t n_
 }
 
 // The angular products on FP64 matrix cores (v_mfma_f64_16x16x4f64), as GEMMs per group of
-// 16 radius nodes: C1 = G1 X and C2 = G2 X with G1, G2 (angles x orders) the w pi_n, w tau_n
-// table by parity and X (orders x 64) the nodes' u, v (a_n, b_n by parity) split into re, im:
-// column 4q + c of a 16-column tile is node q's (u.re, u.im, v.re, v.im)[c]. A workgroup is
-// one ensemble and 128 non-negative cosines (8 wavefronts x 16); the coefficients of the
-// group are staged in LDS 32 orders at a time. After a group's K loop each lane holds, for 4
-// cosines and its (node, component), C1 and C2; the quad of lanes of one node exchanges the
-// four components, forms S1(+-mu) = A +- B, S2(+-mu) = C +- D (A, C = G1 u, G1 v; B, D =
-// G2 v, G2 u), and the lane of component c keeps F11, F12, F33 or F34 (c = 0..3), times the
-// node's factor, summed over the groups in registers; the four node lanes are added at the
-// end, always in the same order.
+// 16 radius nodes: C1^T = X^T G1^T and C2^T = X^T G2^T with G1, G2 (angles x orders) the
+// w pi_n, w tau_n table by parity and X (orders x 64) the nodes' u, v (a_n, b_n by parity)
+// split into re, im. The coefficients are the A operand: row 4c + q of a 16-row tile is
+// component c of node q, (u.re, u.im, v.re, v.im)[c]; the table is the B operand, a
+// wavefront's 16 columns its cosines. v_mfma_f64_16x16x4f64 leaves D[i][j] in lane
+// j + 16 (i % 4), element i / 4, so each lane receives the four components of node q = lane / 16
+// of the tile at its cosine j = lane % 16, forms S1(+-mu) = A +- B, S2(+-mu) = C +- D (A, C =
+// G1 u, G1 v; B, D = G2 v, G2 u) and F11, F12, F33, F34 itself, times the node's factor, summed
+// over the groups in registers; the four node lanes of a cosine are added at the end, always in
+// the same order. A workgroup is one ensemble and 128 non-negative cosines (8 wavefronts x 16);
+// the coefficients of a group are staged in LDS 32 orders at a time.
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGroupNodes = 16;
@@ -768,128 +769,199 @@ constexpr int kSliceAngles = 128;
 // chip) the node groups of an ensemble are dealt over `splits` blocks, split sp taking groups
 // sp, sp + splits, ...; each writes its partial table to tab_out + sp * (ne 4 n_angles), and
 // ens_split_reduce adds the splits in order.
+// With lds_nodes the ensemble's node list sits in dynamic LDS (mfma_lds_bytes): N and the staging
+// offset of each node, then each group's largest N, so that moving to the next item issues no
+// dependent global loads; without it (more than kMfmaLdsNodes nodes) they come from memory.
+constexpr int64_t kMfmaLdsNodes = 8192;
+
+__host__ __device__ inline size_t mfma_lds_bytes(int64_t n_radius) {
+    return (size_t)(2 * n_radius + (n_radius + kGroupNodes - 1) / kGroupNodes) * sizeof(int);
+}
+
 static __global__ __launch_bounds__(512) void ens_angular_mfma(
     int64_t e0, int64_t n_radius, int n_angles, int h, int slices, int splits, int64_t ne,
     const int *node_n, const int64_t *node_off, const int64_t *ens_base, const cx *staging,
-    const double *tabc, const double2 *table, double *tab_out) {
+    const double *tabc, const double2 *table, double *tab_out, int lds_nodes) {
     __shared__ cx lds_a[kGroupNodes][kMfmaOrders];
     __shared__ cx lds_b[kGroupNodes][kMfmaOrders];
+    extern __shared__ int mf_node[];  // [n_radius] N, [n_radius] offsets, [groups] largest N
     const int sp = blockIdx.x % splits;
     const int64_t eb = (blockIdx.x / splits) / slices;
     const int slice = (blockIdx.x / splits) % slices;
     const int64_t e = e0 + eb;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int row = lane & 15, kq = lane >> 4;  // A: angle row, order; B: column, order
-    const int comp = lane & 3, qin = (lane & 15) >> 2;
-    const int jj = slice * kSliceAngles + wave * 16 + row;  // this lane's A-operand cosine
+    const int row = lane & 15, kq = lane >> 4;  // A: row 4c + q, order; B: cosine, order
+    const int comp = row >> 2, qin = row & 3;   // the A row's component and node of the tile
+    const int jj = slice * kSliceAngles + wave * 16 + row;  // this lane's B-operand cosine
     const int64_t base = ens_base[e];
     const int *nn = node_n + e * n_radius;
     const int64_t *noff = node_off + e * n_radius;
     const int64_t gbase = eb * n_radius;
-    double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [sign][v]: this lane's element, rows kq+4v
-    for (int64_t g0 = (int64_t)sp * kGroupNodes; g0 < n_radius;
-         g0 += (int64_t)splits * kGroupNodes) {
-        int nmax = 0;
+    // [sign][F11, F12, F33, F34] at cosine jj of the lane's nodes (4 tt + kq of each group)
+    double tp[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    // Software pipeline over the items (group g0, 32-order chunk kc) of this block: while an
+    // item's products run, the next item's staged coefficients (this thread's (node, order)
+    // pair) and its eight table steps load into registers, so that no load waits in the
+    // product loop; the products and their order are those of the unpipelined loop.
+    const int sq = t / kMfmaOrders, sk = t % kMfmaOrders;  // this thread's staging pair
+    const int64_t gstep = (int64_t)splits * kGroupNodes;
+    const int nr = (int)n_radius;
+    if (lds_nodes) {
+        const int groups = (nr + kGroupNodes - 1) / kGroupNodes;
+        for (int i = t; i < groups; i += blockDim.x) mf_node[2 * nr + i] = 0;
+        for (int i = t; i < nr; i += blockDim.x) {
+            mf_node[i] = nn[i];
+            mf_node[nr + i] = (int)noff[i];
+        }
+        __syncthreads();
+        for (int i = t; i < nr; i += blockDim.x)
+            atomicMax(&mf_node[2 * nr + i / kGroupNodes], mf_node[i]);
+        __syncthreads();
+    }
+    auto node_nq = [&](int64_t i) { return lds_nodes ? mf_node[i] : nn[i]; };
+    auto node_of = [&](int64_t i) -> int64_t { return lds_nodes ? (int64_t)mf_node[nr + i] : noff[i]; };
+    auto group_nmax = [&](int64_t g0) {
+        if (lds_nodes) return mf_node[2 * nr + (int)(g0 / kGroupNodes)];
+        int nm = 0;
         for (int q = 0; q < kGroupNodes; ++q) {
             const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-            nmax = nq > nmax ? nq : nmax;
+            nm = nq > nm ? nq : nm;
         }
-        mfma_d4 acc1[4], acc2[4];
+        return nm;
+    };
+    // The item's coefficient pair (zero beyond its node's N) and table steps (zero beyond the
+    // group's largest N or the cosines).
+    auto load = [&](int64_t g0, int kc, int nm, cx &va, cx &vb, double2 (&tb)[8])
+                    __attribute__((always_inline)) {
+        const int n = kc + sk;
+        const int nq = g0 + sq < n_radius ? node_nq(g0 + sq) : 0;
+        va = real(0.0);
+        vb = real(0.0);
+        if (n < nq) {
+            const cx *c = staging + 2 * (base + node_of(g0 + sq));
+            va = c[2 * n];
+            vb = c[2 * n + 1];
+        }
+        // Loaded from a clamped address in every lane (the load count per item is fixed, so the
+        // product loop's waits cover exactly the previous item's loads), masked after arrival.
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const int nt = kc + 4 * st + kq + 1;
+            const bool ok = jj < h && nt <= nm;
+            const double2 v = table[(size_t)(ok ? nt - 1 : 0) * h + (jj < h ? jj : 0)];
+            tb[st] = ok ? v : make_double2(0.0, 0.0);
+        }
+    };
+    int64_t g0 = (int64_t)sp * kGroupNodes;
+    if (g0 >= n_radius) g0 = n_radius;  // no group for this split: straight to the output
+    int nmax = g0 < n_radius ? group_nmax(g0) : 0;
+    int kc = 0;
+    cx pa = real(0.0), pb = real(0.0);
+    double2 tn[8];
+    if (g0 < n_radius) load(g0, 0, nmax, pa, pb, tn);
+    mfma_d4 acc1[4], acc2[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+        acc1[tt] = mfma_d4{0, 0, 0, 0};
+        acc2[tt] = mfma_d4{0, 0, 0, 0};
+    }
+    while (g0 < n_radius) {
+        __syncthreads();  // the previous item's products are done with the LDS tile
+        if (t < kGroupNodes * kMfmaOrders) {
+            lds_a[sq][sk] = pa;
+            lds_b[sq][sk] = pb;
+        }
+        double2 tc[8];
+#pragma unroll
+        for (int st = 0; st < 8; ++st) tc[st] = tn[st];
+        __syncthreads();
+        // The next item, and its loads.
+        int64_t g1 = g0;
+        int kc1 = kc + kMfmaOrders, nmax1 = nmax;
+        if (kc1 >= nmax) {
+            g1 = g0 + gstep;
+            kc1 = 0;
+            nmax1 = g1 < n_radius ? group_nmax(g1) : 0;
+        }
+        if (g1 < n_radius) load(g1, kc1, nmax1, pa, pb, tn);
+        const int kend = nmax - kc < kMfmaOrders ? nmax - kc : kMfmaOrders;
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const int k4 = 4 * st;
+            if (k4 < kend) {
+                const int kk = k4 + kq;              // order index in the chunk
+                const bool odd = (kc + kk + 1) & 1;  // the order's parity
+                const double2 pt = tc[st];
+                const double g1v = odd ? pt.x : pt.y;
+                const double g2v = odd ? pt.y : pt.x;
+                // The lane's A element, component comp of (u, v) = (a_n, b_n) at odd n, (b_n, a_n)
+                // at even n, read from its own address (no per-lane branch).
+                const double *src = reinterpret_cast<const double *>(
+                                        (comp < 2) == odd ? &lds_a[0][0] : &lds_b[0][0]) +
+                                    (comp & 1);
+#pragma unroll
+                for (int tt = 0; tt < 4; ++tt) {
+                    const int q = 4 * tt + qin;
+                    const double val = src[2 * (q * kMfmaOrders + kk)];
+                    acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(val, g1v, acc1[tt], 0, 0, 0);
+                    acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(val, g2v, acc2[tt], 0, 0, 0);
+                }
+            }
+        }
+        if (g1 == g0) {
+            kc = kc1;
+            continue;
+        }
+        // Epilogue of the group: node 4 tt + kq at cosine jj, all four components in the lane.
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            const int64_t node = g0 + 4 * tt + kq;
+            const double cfac = node < n_radius ? tabc[gbase + node] : 0.0;
+            const cx A = mk(acc1[tt][0], acc1[tt][1]), C = mk(acc1[tt][2], acc1[tt][3]);
+            const cx D = mk(acc2[tt][0], acc2[tt][1]), B = mk(acc2[tt][2], acc2[tt][3]);
+#pragma unroll
+            for (int sgn = 0; sgn < 2; ++sgn) {
+                const cx s1 = sgn == 0 ? A + B : A - B;
+                const cx s2 = sgn == 0 ? C + D : C - D;
+                const cx pp = s2 * conj(s1);
+                tp[sgn][0] += cfac * (0.5 * (norm2(s1) + norm2(s2)));
+                tp[sgn][1] += cfac * (0.5 * (norm2(s2) - norm2(s1)));
+                tp[sgn][2] += cfac * pp.re;
+                tp[sgn][3] += cfac * pp.im;
+            }
+        }
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
             acc1[tt] = mfma_d4{0, 0, 0, 0};
             acc2[tt] = mfma_d4{0, 0, 0, 0};
         }
-        for (int kc = 0; kc < nmax; kc += kMfmaOrders) {
-            __syncthreads();
-            for (int idx = t; idx < kGroupNodes * kMfmaOrders; idx += blockDim.x) {
-                const int q = idx / kMfmaOrders, k = idx % kMfmaOrders;
-                const int n = kc + k;
-                const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-                cx va = real(0.0), vb = real(0.0);
-                if (n < nq) {
-                    const cx *c = staging + 2 * (base + noff[g0 + q]);
-                    va = c[2 * n];
-                    vb = c[2 * n + 1];
-                }
-                lds_a[q][k] = va;
-                lds_b[q][k] = vb;
-            }
-            __syncthreads();
-            const int kend = nmax - kc < kMfmaOrders ? nmax - kc : kMfmaOrders;
-            for (int k4 = 0; k4 < kend; k4 += 4) {
-                const int kk = k4 + kq;              // order index in the chunk
-                const int n = kc + kk + 1;           // the order
-                double2 pt = make_double2(0.0, 0.0);
-                if (jj < h && n <= nmax) pt = table[(size_t)(n - 1) * h + jj];
-                const bool odd = n & 1;
-                const double g1 = odd ? pt.x : pt.y;
-                const double g2 = odd ? pt.y : pt.x;
-#pragma unroll
-                for (int tt = 0; tt < 4; ++tt) {
-                    const int q = 4 * tt + qin;
-                    const cx a = lds_a[q][kk], b = lds_b[q][kk];
-                    const cx u = odd ? a : b, v = odd ? b : a;
-                    const double val = comp == 0 ? u.re : comp == 1 ? u.im : comp == 2 ? v.re : v.im;
-                    acc1[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g1, val, acc1[tt], 0, 0, 0);
-                    acc2[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(g2, val, acc2[tt], 0, 0, 0);
-                }
-            }
-        }
-        // Epilogue of the group: the lane's column is node 4 tt + qin, component comp.
-#pragma unroll
-        for (int tt = 0; tt < 4; ++tt) {
-            const int64_t node = g0 + 4 * tt + qin;
-            const double cfac = node < n_radius ? tabc[gbase + node] : 0.0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double c1[4], c2[4];
-                const double x1 = acc1[tt][v], x2 = acc2[tt][v];
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    c1[comp ^ d] = d == 0 ? x1 : __shfl_xor(x1, d, 64);
-                    c2[comp ^ d] = d == 0 ? x2 : __shfl_xor(x2, d, 64);
-                }
-                const cx A = mk(c1[0], c1[1]), C = mk(c1[2], c1[3]);
-                const cx D = mk(c2[0], c2[1]), B = mk(c2[2], c2[3]);
-#pragma unroll
-                for (int sgn = 0; sgn < 2; ++sgn) {
-                    const cx s1 = sgn == 0 ? A + B : A - B;
-                    const cx s2 = sgn == 0 ? C + D : C - D;
-                    double f;
-                    if (comp == 0) f = 0.5 * (norm2(s1) + norm2(s2));
-                    else if (comp == 1) f = 0.5 * (norm2(s2) - norm2(s1));
-                    else {
-                        const cx pp = s2 * conj(s1);
-                        f = comp == 2 ? pp.re : pp.im;
-                    }
-                    tp[sgn][v] += cfac * f;
-                }
-            }
-        }
+        g0 = g1;
+        kc = kc1;
+        nmax = nmax1;
     }
-    // Add the four node lanes (qin = 0..3) of each element, in a fixed order.
+    // Add the four node lanes (kq = 0..3) of each cosine, in a fixed order.
 #pragma unroll
     for (int sgn = 0; sgn < 2; ++sgn)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            double x = tp[sgn][v];
-            x += __shfl_xor(x, 4, 64);
-            x += __shfl_xor(x, 8, 64);
-            tp[sgn][v] = x;
+        for (int k = 0; k < 4; ++k) {
+            double x = tp[sgn][k];
+            x += __shfl_xor(x, 16, 64);
+            x += __shfl_xor(x, 32, 64);
+            tp[sgn][k] = x;
         }
-    if (qin == 0) {
+    if (jj < h) {
+        // Lane kq writes component kq at +mu and -mu.
         double *out = tab_out + ((int64_t)sp * ne + eb) * 4 * (int64_t)n_angles;
+        const int jp = n_angles - h + jj, jm = n_angles - 1 - jp;
+        double vp = tp[0][0], vm = tp[1][0];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            // v_mfma_f64_16x16x4f64 leaves D[i][j] in lane j + 16 (i % 4), element i / 4.
-            const int jr = slice * kSliceAngles + wave * 16 + kq + 4 * v;  // output cosine
-            if (jr >= h) continue;
-            const int jp = n_angles - h + jr, jm = n_angles - 1 - jp;
-            out[comp * n_angles + jp] = tp[0][v];
-            if (jm != jp) out[comp * n_angles + jm] = tp[1][v];
+        for (int k = 1; k < 4; ++k) {
+            vp = kq == k ? tp[0][k] : vp;
+            vm = kq == k ? tp[1][k] : vm;
         }
+        out[kq * n_angles + jp] = vp;
+        if (jm != jp) out[kq * n_angles + jm] = vm;
     }
 }
 
@@ -2024,9 +2096,11 @@ static int angular_products(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne
                 return rc;
             dst = static_cast<double *>(ctx->buf[scatrs_ctx::kMisc3]);
         }
-        ens_angular_mfma<<<(unsigned)(ne * slices * splits), 512, 0, s>>>(
+        const int lds_nodes = n_radius <= kMfmaLdsNodes;
+        ens_angular_mfma<<<(unsigned)(ne * slices * splits), 512,
+                           lds_nodes ? mfma_lds_bytes(n_radius) : 0, s>>>(
             e0, n_radius, n_out, h, slices, splits, ne, S.node_n, S.node_off, S.ens_base,
-            B.staging, B.tabc, table, dst);
+            B.staging, B.tabc, table, dst, lds_nodes);
         if (splits > 1)
             ens_split_reduce<<<(unsigned)((cells + 255) / 256), 256, 0, s>>>(dst, splits, cells,
                                                                            tab_out);
