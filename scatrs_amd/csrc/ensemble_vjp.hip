@@ -301,6 +301,7 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
     const int lane = t & 63, wave = t >> 6;
     const int row = lane & 15, kq = lane >> 4;
     const int comp = lane & 3, qin = (lane & 15) >> 2;  // transposed products' D column
+    const int sq = t / kVjpOrders, sk = t % kVjpOrders;  // this thread's staging pair
     double *slab = slabs + (size_t)blockIdx.x * 2 * kVjpCols * h;
     const int64_t groups = (n_radius + kVjpNodes - 1) / kVjpNodes;
     const int64_t items = ne * groups;
@@ -331,33 +332,51 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                 acc1[tt] = mfma_d4{0, 0, 0, 0};
                 acc2[tt] = mfma_d4{0, 0, 0, 0};
             }
+            // Pipelined: the next chunk's coefficient pair of this thread (node sq, order sk)
+            // loads while a chunk's products run, and the table two steps ahead of its use
+            // (from a clamped address, masked after arrival); the products and their order are
+            // those of the unpipelined loop.
+            cx pa = real(0.0), pb = real(0.0);
+            auto load_pair = [&](int kc) __attribute__((always_inline)) {
+                const int n = kc + sk;
+                const int nq = g0 + sq < n_radius ? nn[g0 + sq] : 0;
+                pa = real(0.0);
+                pb = real(0.0);
+                if (n < nq) {
+                    const cx *c = staging + 2 * (base + noff[g0 + sq]);
+                    pa = c[2 * n];
+                    pb = c[2 * n + 1];
+                }
+            };
+            auto tab = [&](int kc, int st) __attribute__((always_inline)) {
+                const int n = kc + 4 * st + kq + 1;
+                const bool ok = jj < h && n <= nmax;
+                const double2 v = table[(size_t)(ok ? n - 1 : 0) * h + (jj < h ? jj : 0)];
+                return ok ? v : make_double2(0.0, 0.0);
+            };
+            if (nmax > 0) load_pair(0);
             for (int kc = 0; kc < nmax; kc += kVjpOrders) {
                 __syncthreads();
-                for (int idx = t; idx < kVjpNodes * kVjpOrders; idx += blockDim.x) {
-                    const int q = idx / kVjpOrders, k = idx % kVjpOrders;
-                    const int n = kc + k;
-                    const int nq = g0 + q < n_radius ? nn[g0 + q] : 0;
-                    cx va = real(0.0), vb = real(0.0);
-                    if (n < nq) {
-                        const cx *c = staging + 2 * (base + noff[g0 + q]);
-                        va = c[2 * n];
-                        vb = c[2 * n + 1];
-                    }
+                {
+                    const int n = kc + sk;
                     const bool odd = (n + 1) & 1;
-                    const cx u = odd ? va : vb, v = odd ? vb : va;
-                    double *col = &lds_v[q / 4][q % 4][k];
+                    const cx u = odd ? pa : pb, v = odd ? pb : pa;
+                    double *col = &lds_v[sq / 4][sq % 4][sk];
                     col[0] = u.re;
                     col[4 * (kVjpOrders + 1)] = u.im;
                     col[8 * (kVjpOrders + 1)] = v.re;
                     col[12 * (kVjpOrders + 1)] = v.im;
                 }
                 __syncthreads();
+                if (kc + kVjpOrders < nmax) load_pair(kc + kVjpOrders);
                 const int kend = nmax - kc < kVjpOrders ? nmax - kc : kVjpOrders;
+                double2 r0 = tab(kc, 0), r1 = tab(kc, 1);
                 for (int k4 = 0; k4 < kend; k4 += 4) {
                     const int kk = k4 + kq;
                     const int n = kc + kk + 1;
-                    double2 pt = make_double2(0.0, 0.0);
-                    if (jj < h && n <= nmax) pt = table[(size_t)(n - 1) * h + jj];
+                    const double2 pt = r0;
+                    r0 = r1;
+                    r1 = tab(kc, k4 / 4 + 2);
                     const bool odd = n & 1;
                     const double g1 = odd ? pt.x : pt.y;
                     const double g2 = odd ? pt.y : pt.x;
@@ -441,19 +460,48 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                 d1[tt] = mfma_d4{0, 0, 0, 0};
                 d2[tt] = mfma_d4{0, 0, 0, 0};
             }
+            // Pipelined as the forward products: the next panel chunk (this thread's kPanelVals
+            // values) loads while a chunk's products run, the table two steps ahead.
+            constexpr int kPanelVals = 2 * kVjpAngles * kVjpCols / 512;
+            double pv[kPanelVals];
+            auto load_panel = [&](int jc) __attribute__((always_inline)) {
+#pragma unroll
+                for (int r = 0; r < kPanelVals; ++r) {
+                    const int idx = t + 512 * r;
+                    const int s = idx / (kVjpAngles * kVjpCols), k = (idx / kVjpCols) % kVjpAngles,
+                              col = idx % kVjpCols;
+                    const int j = jc + k;
+                    pv[r] = j < h ? slab[((size_t)s * h + j) * kVjpCols + col] : 0.0;
+                }
+            };
+            auto tabr = [&](int jc, int st) __attribute__((always_inline)) {
+                const int jb = jc + 4 * st + kq;
+                const bool ok = n_row <= nmax && jb < h;
+                const double2 v = table[(size_t)(ok ? n_row - 1 : 0) * h + (ok ? jb : 0)];
+                return ok ? v : make_double2(0.0, 0.0);
+            };
+            load_panel(0);
             for (int jc = 0; jc < h; jc += kVjpAngles) {
                 __syncthreads();
-                for (int idx = t; idx < 2 * kVjpAngles * kVjpCols; idx += blockDim.x) {
-                    const int s = idx / (kVjpAngles * kVjpCols), k = (idx / kVjpCols) % kVjpAngles, col = idx % kVjpCols;
-                    const int j = jc + k;
-                    lds_g[s][k][col] = j < h ? slab[((size_t)s * h + j) * kVjpCols + col] : 0.0;
+#pragma unroll
+                for (int r = 0; r < kPanelVals; ++r) {
+                    const int idx = t + 512 * r;
+                    const int s = idx / (kVjpAngles * kVjpCols), k = (idx / kVjpCols) % kVjpAngles,
+                              col = idx % kVjpCols;
+                    lds_g[s][k][col] = pv[r];
                 }
                 __syncthreads();
+                if (jc + kVjpAngles < h) load_panel(jc + kVjpAngles);
                 const int kend = !active ? 0 : h - jc < kVjpAngles ? h - jc : kVjpAngles;
+                double2 r0 = make_double2(0.0, 0.0), r1 = r0;
+                if (kend > 0) {
+                    r0 = tabr(jc, 0);
+                    r1 = tabr(jc, 1);
+                }
                 for (int k4 = 0; k4 < kend; k4 += 4) {
-                    const int jb = jc + k4 + kq;
-                    double2 pt = make_double2(0.0, 0.0);
-                    if (n_row <= nmax && jb < h) pt = table[(size_t)(n_row - 1) * h + jb];
+                    const double2 pt = r0;
+                    r0 = r1;
+                    r1 = tabr(jc, k4 / 4 + 2);
                     const double g1 = odd_row ? pt.x : pt.y;
                     const double g2 = odd_row ? pt.y : pt.x;
 #pragma unroll
