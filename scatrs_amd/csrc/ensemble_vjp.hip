@@ -449,11 +449,20 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
             double *o = ht + eb * n_radius + g0 + t;
             *o = accumulate ? *o + s : s;
         }
-        // 3: the transposed products, 128 orders (16 per wave) at a time.
+        // 3: the transposed products, 128 orders at a time in ob <= 8 blocks of 16. Wavefront
+        // w takes order block w % ob and angle split w / ob, for the splits = 8, 4, 2 or 1
+        // (ob = 1, 2, 3-4, 5-8) that the 8 wavefronts hold: each split takes a contiguous run of
+        // every panel chunk's angle steps, and the splits' partial sums are added in split
+        // order through LDS, so that a small group's orders do not leave wavefronts idle.
         for (int nb = 0; nb < nmax; nb += kVjpOrderRows) {
-            const int n_row = nb + wave * 16 + row + 1;  // this lane's A-operand order
+            const int rem = nmax - nb;
+            const int ob = rem >= kVjpOrderRows ? 8 : (rem + 15) / 16;
+            const int splits = ob == 1 ? 8 : ob == 2 ? 4 : ob <= 4 ? 2 : 1;
+            const int obi = wave % ob, sp = wave / ob;
+            const bool active = sp < splits;
+            const int sp_angles = kVjpAngles / splits;   // a split's angles of each chunk
+            const int n_row = nb + obi * 16 + row + 1;  // this lane's A-operand order
             const bool odd_row = n_row & 1;
-            const bool active = nb + wave * 16 < nmax;   // the wavefront has orders to form
             mfma_d4 d1[kVjpTiles], d2[kVjpTiles];
 #pragma unroll
             for (int tt = 0; tt < kVjpTiles; ++tt) {
@@ -492,13 +501,15 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                 }
                 __syncthreads();
                 if (jc + kVjpAngles < h) load_panel(jc + kVjpAngles);
-                const int kend = !active ? 0 : h - jc < kVjpAngles ? h - jc : kVjpAngles;
+                const int cend = h - jc < kVjpAngles ? h - jc : kVjpAngles;
+                const int kbeg = sp * sp_angles;
+                const int kend = !active ? 0 : (kbeg + sp_angles < cend ? kbeg + sp_angles : cend);
                 double2 r0 = make_double2(0.0, 0.0), r1 = r0;
-                if (kend > 0) {
-                    r0 = tabr(jc, 0);
-                    r1 = tabr(jc, 1);
+                if (kbeg < kend) {
+                    r0 = tabr(jc, kbeg / 4);
+                    r1 = tabr(jc, kbeg / 4 + 1);
                 }
-                for (int k4 = 0; k4 < kend; k4 += 4) {
+                for (int k4 = kbeg; k4 < kend; k4 += 4) {
                     const double2 pt = r0;
                     r0 = r1;
                     r1 = tabr(jc, k4 / 4 + 2);
@@ -513,6 +524,34 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                     }
                 }
             }
+            double dv[kVjpTiles][4];
+#pragma unroll
+            for (int tt = 0; tt < kVjpTiles; ++tt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dv[tt][v] = d1[tt][v] + d2[tt][v];
+            if (splits > 1) {
+                // The splits' partial sums through the panel tile (8 x 8 x 64 doubles).
+                double *red = &lds_g[0][0][0];
+                __syncthreads();  // every wavefront is done with the last panel chunk
+                if (active)
+#pragma unroll
+                    for (int tt = 0; tt < kVjpTiles; ++tt)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            red[((wave * kVjpTiles + tt) * 4 + v) * 64 + lane] = dv[tt][v];
+                __syncthreads();
+                if (active && sp == 0)
+#pragma unroll
+                    for (int tt = 0; tt < kVjpTiles; ++tt)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            double x = dv[tt][v];
+                            for (int q = 1; q < splits; ++q)
+                                x += red[(((q * ob + obi) * kVjpTiles + tt) * 4 + v) * 64 + lane];
+                            dv[tt][v] = x;
+                        }
+            }
+            if (!active || sp != 0) continue;
 #pragma unroll
             for (int tt = 0; tt < kVjpTiles; ++tt) {
                 const int64_t node = g0 + 4 * tt + qin;
@@ -521,9 +560,9 @@ static __global__ __launch_bounds__(512, 4) void ens_angular_vjp(
                 cx *cb = abar + 2 * (base + noff[node]);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int n = nb + wave * 16 + kq + 4 * v + 1;
+                    const int n = nb + obi * 16 + kq + 4 * v + 1;
                     if (n > nq) continue;
-                    const double val = d1[tt][v] + d2[tt][v];
+                    const double val = dv[tt][v];
                     const bool to_a = (comp < 2) == (bool)(n & 1);  // u is a_n at odd n
                     double *dst = reinterpret_cast<double *>(cb + (to_a ? 0 : nq) + n - 1);
                     dst[comp & 1] = accumulate ? dst[comp & 1] + val : val;
