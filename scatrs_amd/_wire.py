@@ -38,8 +38,6 @@ every (output, parameter) pair. The size range is held fixed, as the reference d
 
 from __future__ import annotations
 
-import copy
-
 import numpy as np
 
 from .ensemble import segment_sum
@@ -243,26 +241,38 @@ class _Leaves:
         B = R * n
         self.n = n
         col = np.tile(np.arange(n), R)
-        t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64,  # noqa: E731
-                                   device=dev).requires_grad_(grad)
         self.dist, self.m_real, self.m_imag, self.fraction = {}, {}, {}, {}
         self.weight, self.mode_weight = {}, {}
+        specs = []  # (table, key, host array): all go to the device in one transfer
+
+        def t(table, key, a):
+            specs.append((table, key, np.ascontiguousarray(a, dtype=np.float64)))
+
         for u, comps in enumerate(units):
             for c, comp in enumerate(comps):
                 mb = comp.m[:, col].T  # (B, L)
-                self.m_real[u, c] = t(mb.real)
-                self.m_imag[u, c] = t(mb.imag)
-                self.fraction[u, c] = t(np.tile(comp.fr, (B, 1)))
-                self.weight[u, c] = t(np.full(B, comp.weight))
+                t(self.m_real, (u, c), mb.real)
+                t(self.m_imag, (u, c), mb.imag)
+                t(self.fraction, (u, c), np.tile(comp.fr, (B, 1)))
+                t(self.weight, (u, c), np.full(B, comp.weight))
                 d = comp.dist
                 if isinstance(d, MultiModal):
-                    self.mode_weight[u, c] = t(np.tile([w for w, _ in d.modes], (B, 1)))
+                    t(self.mode_weight, (u, c), np.tile([w for w, _ in d.modes], (B, 1)))
                     for j, (_, dj) in enumerate(d.modes):
                         if not isinstance(dj, Tabulated):
-                            self.dist[u, c, j] = t(np.tile(np.array(dj.params(), np.float64),
-                                                           (B, 1)))
+                            t(self.dist, (u, c, j),
+                              np.tile(np.array(dj.params(), np.float64), (B, 1)))
                 elif not isinstance(d, Tabulated):
-                    self.dist[u, c, None] = t(np.tile(np.array(d.params(), np.float64), (B, 1)))
+                    t(self.dist, (u, c, None),
+                      np.tile(np.array(d.params(), np.float64), (B, 1)))
+        flat = np.concatenate([a.reshape(-1) for _, _, a in specs]) if specs else np.zeros(0)
+        # Leaves of reverse mode: views of one tensor that requires grad, each a graph node of
+        # its own (its gradient is the slice's).
+        base = torch.from_numpy(flat).to(dev).requires_grad_(grad)
+        off = 0
+        for table, key, a in specs:
+            table[key] = base[off:off + a.size].view(a.shape)
+            off += a.size
 
     def tensors(self):
         return [*self.dist.values(), *self.m_real.values(), *self.m_imag.values(),
@@ -603,7 +613,7 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                 passes = sh.pass2_groups(tasks, **kw)
             else:  # both rules in one call per angle rule: the tasks twice, Gauss weights second
                 passes = _split_rules(adaptive.pass2_torch(
-                    tasks + [copy.copy(q) for q in tasks], params + params, R=R,
+                    tasks + [q.shallow() for q in tasks], params + params, R=R,
                     idx=np.arange(15), gauss_tasks=np.repeat([False, True], len(tasks)),
                     derivatives=fwd, **kw), len(tasks), R)
             for avg, gr, ph, _, _, rw in passes:

@@ -230,6 +230,12 @@ class Task:
     def n_panels(self) -> int:
         return max(self.a.size, 1)
 
+    def shallow(self) -> "Task":
+        """A shallow copy (its arrays shared), cheaper than copy.copy on a dataclass."""
+        c = object.__new__(Task)
+        c.__dict__.update(self.__dict__)
+        return c
+
 
 def _nodes(tasks, ti, a, b, idx, derivatives: bool = False):
     """(r, W) (len(a), len(idx)) of the panels [a, b] (each of task ti) at the nodes idx of the
@@ -904,27 +910,40 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
         dl = d.reshape(nr, -1, 3)
         z0 = nodes[..., 1].sum(1)
         zd = (nodes[..., 1:2] * dl).sum(1)
+        # The rows' arrays go to the device once; each group of rows (one angle rule and layer
+        # count) takes its rows there, by one transfer of all the groups' row lists.
+        groups = []
         for na in np.unique(rules[:nr]):
             for L in np.unique(layers[ti_rows]):
                 sel = np.flatnonzero((rules[:nr] == na) & (layers[ti_rows] == L) & (z0 > 0.0))
-                if not sel.size:
-                    continue
-                tk = ti_rows[sel]
-                pr, mr, mi, fr = gather(tk, L)
-                t = lambda a: torch.from_numpy(np.ascontiguousarray(  # noqa: E731
-                    np.repeat(a, R, axis=0))).to(dev)
-                sc, g, p_ = ensemble_nodes(pr, mr, mi, fr, t(nodes[sel]),
-                                           t(dl[sel]) if grad else None,
-                                           t(wl_all[tk]), n_angles=int(na), order=order,
-                                           normalized=False,
-                                           allow_unvalidated=allow_unvalidated, mu=mu,
-                                           phase=want_phase, replicas=R)
-                z = t(z0[sel])
-                if grad:
-                    z = z + ((pr - pr.detach()) * t(zd[sel])).sum(1)
-                key = (torch.from_numpy(tk).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
-                place = (torch.from_numpy(sel).to(dev)[:, None] * R + rep[None, :]).reshape(-1)
-                add(key, z, sc, g, p_, place)
+                if sel.size:
+                    groups.append((int(na), int(L), sel))
+        if groups:
+            to_dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+            nodes_d, z0_d = to_dev(nodes), to_dev(z0)
+            dl_d, zd_d = (to_dev(dl), to_dev(zd)) if grad else (None, None)
+            wl_d, ti_d = to_dev(wl_all), to_dev(ti_rows.astype(np.int64))
+            sel_d = to_dev(np.concatenate([g_[2] for g_ in groups]).astype(np.int64))
+        pos = 0
+        for na, L, sel in groups:
+            tk = ti_rows[sel]
+            pr, mr, mi, fr = gather(tk, L)
+            sd = sel_d[pos:pos + sel.size]
+            pos += sel.size
+            rep_rows = lambda a: a[sd].repeat_interleave(R, dim=0)  # noqa: E731
+            tk_d = ti_d[sd]
+            sc, g, p_ = ensemble_nodes(pr, mr, mi, fr, rep_rows(nodes_d),
+                                       rep_rows(dl_d) if grad else None,
+                                       wl_d[tk_d].repeat_interleave(R, dim=0), n_angles=na,
+                                       order=order, normalized=False,
+                                       allow_unvalidated=allow_unvalidated, mu=mu,
+                                       phase=want_phase, replicas=R)
+            z = rep_rows(z0_d)
+            if grad:
+                z = z + ((pr - pr.detach()) * rep_rows(zd_d)).sum(1)
+            key = (tk_d[:, None] * R + rep[None, :]).reshape(-1)
+            place = (sd[:, None] * R + rep[None, :]).reshape(-1)
+            add(key, z, sc, g, p_, place)
     for k, ti in enumerate(mono):
         na = int(rules[nr + k])
         L = tasks[ti].layers
