@@ -44,8 +44,10 @@ static __global__ void __launch_bounds__(1024) scan_buckets(const unsigned *hist
     }
 }
 
-// Elements per thread of the histogram and scatter kernels (256 threads per block).
-constexpr int kSortPerThread = 4;
+// Elements per thread of the histogram and scatter kernels (256 threads per block): 4096 per
+// block, so that each block's pass over the 4096 LDS buckets (zeroing, and one global atomic
+// per non-empty bucket) serves as many elements as it has buckets.
+constexpr int kSortPerThread = 16;
 constexpr int kSortBlock = 256 * kSortPerThread;
 
 // Block-local histogram in LDS, then one global atomic per non-empty bucket of the block.
