@@ -175,8 +175,9 @@ int mie_begin(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len
     return check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice");
 }
 
-// The checks of every sphere and the N histogram; waits for the status (the one
-// synchronization of a call) and maps the first failing element to the reference's message.
+// The checks of every sphere, the N histogram and the N-ranked permutation; waits for the
+// status (the one synchronization of a call) and maps the first failing element to the
+// reference's message.
 int mie_validate(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double *d_m,
                  int64_t len, int64_t n_max, int allow_unvalidated, int64_t *nmax_out,
                  int64_t *total_terms) {
@@ -194,6 +195,9 @@ int mie_validate(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const dou
     reduce_status<<<1, 1024, 0, s>>>(w.partial, blocks, st);
     histogram_keys<<<(unsigned)((len + kSortBlock - 1) / kSortBlock), 256, 0, s>>>(w.keys, len,
                                                                                    w.hist);
+    // The N-ranked permutation needs only the keys: it runs while the host waits for the status
+    // (and is unused if a sphere fails its checks).
+    sort_by_keys(w.keys, w.hist, w.perm, len, s);
     Status *hst = static_cast<Status *>(ctx->host_pinned);
     if ((rc = check_hip(ctx, hipMemcpyAsync(hst, st, sizeof(Status), hipMemcpyDeviceToHost, s),
                         "status copy")))
@@ -214,13 +218,12 @@ int mie_validate(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const dou
     return SCATRS_OK;
 }
 
-// The N-ranked permutation and the persistent solve on `waves` wavefronts (a multiple of 4)
-// whose slabs start at w.slab.
+// The persistent solve over the N-ranked permutation (mie_validate's) on `waves` wavefronts (a
+// multiple of 4) whose slabs start at w.slab.
 int mie_solve(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double *d_m,
               int64_t len, int64_t n_max, int64_t nmax, int64_t waves, double *d_values,
               double *d_jacobian) {
     hipStream_t s = ctx->stream;
-    sort_by_keys(w.keys, w.hist, w.perm, len, s);
     const int64_t rows = nmax + 2;
     const int64_t rows_ck = nmax / kSphereBlock + 2;
     unsigned long long *chunk_counter = reinterpret_cast<unsigned long long *>(
