@@ -166,6 +166,31 @@ def test_fp16_split_angular_kernel_within_1e6_of_the_oracle():
         assert ga <= 1e-6 and gall <= 1e-6 and pe <= 1e-6, (e, ga, gall, pe)
 
 
+def test_fp64_matrix_core_kernel_beyond_its_lds_node_list():
+    """The FP64 matrix-core angular kernel keeps an ensemble's node list in LDS up to 8192
+    radius nodes and reads it from memory beyond: at 8200 nodes (a broad lognormal, so that
+    every group of nodes has terms) its tables, Greek coefficients and phase matrix equal the
+    plain FP64 vector kernel's to rounding, and at 8192 nodes (in LDS) likewise."""
+    from scatrs_amd import _native
+
+    ctx = _native.context(0)
+    kind, p0, p1 = np.zeros(2, np.int32), np.array([0.3, 1.0]), np.array([1.8, 1.5])
+    wl, m = np.array([0.55, 1.0]), np.array([1.45 + 1e-3j, 1.33 + 1e-6j])
+    for nr in (8192, 8200):
+        out = {}
+        for kernel in (0, 1):
+            ctx.set_angular_kernel(kernel)
+            try:
+                out[kernel] = _gpu(kind, p0, p1, wl, m, n_radius=nr, n_angles=96, order=40)
+            finally:
+                ctx.set_angular_kernel(0)
+        (s0, g0, f0, _), (s1, g1, f1, _) = out[0], out[1]
+        assert np.max(np.abs(s0 - s1) / np.maximum(np.abs(s1), 1e-300)) <= 1e-13, nr
+        for e in range(2):
+            assert np.max(np.abs(g0[e] - g1[e])) / np.max(np.abs(g1[e][0])) <= 1e-12, (nr, e)
+            assert np.max(np.abs(f0[e] - f1[e])) / np.max(np.abs(f1[e][0])) <= 1e-12, (nr, e)
+
+
 def test_mixture_and_multimodal_match_the_oracle():
     """Mixture (a lognormal of spheres, a gamma law of coated spheres, a bimodal lognormal of
     spheres) and a MultiModal Ensemble through the GPU path (one library call per layer count,
