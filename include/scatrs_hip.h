@@ -79,13 +79,20 @@ int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
  *   SCATRS_TUNE_SPHERE_PLAIN: 1 (default) solves each sphere of the sphere batch first on the
  *     branch-free plain path and redoes it on the checked path (the reference's branch per
  *     complex inverse) only where an argument left the plain range; 0 takes the checked path for
- *     every sphere (same results to rounding; for tests of that path).
+ *     every sphere (same results to rounding; for tests of that path);
+ *   SCATRS_TUNE_SPHERE_WAVE: when a values-only sphere batch is solved one sphere per wavefront
+ *     (each sphere's recurrences split over the 64 lanes and joined by scans) instead of one per
+ *     lane: 0 never, 1 always, f >= 2 (default 8) when its total terms times f are fewer than
+ *     the lanes of the GPU times its largest N, i.e. when one sphere per lane would leave most
+ *     lanes idle while the largest sphere's serial recurrences run (0-4096; same results to
+ *     rounding).
  * scatrs_get_tuning returns the value, or -1 for an unknown key. */
 #define SCATRS_TUNE_SPHERE_WAVES 0
 #define SCATRS_TUNE_NODE_WAVES 1
 #define SCATRS_TUNE_CHUNK_SHARE 2
 #define SCATRS_TUNE_SPLIT_BLOCKS 3
 #define SCATRS_TUNE_SPHERE_PLAIN 4
+#define SCATRS_TUNE_SPHERE_WAVE 5
 int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value);
 int64_t scatrs_get_tuning(const scatrs_ctx *ctx, int key);
 
@@ -109,6 +116,9 @@ int64_t scatrs_default_n_max(double x);
  *   d_x[len], d_m[2*len] (re, im interleaved: complex128), n_max <= 0 for the default
  *   truncation; d_values (5, len) row-major; d_jacobian (3, 5, len) or NULL.
  *   total_terms (host, may be NULL): sum over spheres of N, the (particle, order) terms.
+ * A values-only batch too small to keep the GPU's lanes busy (SCATRS_TUNE_SPHERE_WAVE) is solved
+ * one sphere per wavefront, its recurrences split over the lanes; larger batches and the
+ * Jacobian one sphere per lane. Both give the reference's results to rounding.
  * Synchronizes once (validation and sizing); the solve itself is queued on the stream. */
 int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
                             int64_t n_max, int allow_unvalidated, double *d_values,
@@ -116,7 +126,8 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
 
 /* scatrs_mie_efficiencies without any device allocation: the same arguments, kernels and
  * results, with every buffer beyond the inputs and outputs (status, N histogram, keys,
- * permutation, the persistent wavefronts' slabs) carved from a 64 MiB array that the library
+ * permutation, the persistent wavefronts' slabs or, for a batch solved one sphere per
+ * wavefront, their columns) carved from a 64 MiB array that the library
  * holds in each device's memory from its load until the process ends. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). The array's address on a device is

@@ -65,8 +65,8 @@ struct scatrs_ctx {
     // persistent wavefronts per CU of the sphere batch and of the ensembles' sphere node solves,
     // the share of free device memory one chunk of ensembles stages (1/share), and the blocks per
     // CU a small batch's angular products aim for when they split node groups; whether the sphere
-    // batch tries the plain path first.
-    int tune[5] = {16, 8, 4, 2, 1};
+    // batch tries the plain path first; when a sphere batch is solved one sphere per wavefront.
+    int tune[6] = {16, 8, 4, 2, 1, 8};
     bool profiling = false;
     int angular_kernel = 0;  // 0: FP64 MFMA, 1: FP64 vector (plain), 2: fp16-split MFMA
     std::vector<double> phase_mu;  // the caller's phase cosines (empty: the angle rule's)

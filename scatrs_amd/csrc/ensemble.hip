@@ -36,9 +36,12 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
         (layers > 1 && !d_fractions))
         return fail(ctx, SCATRS_INVALID_INPUT, "null input or output pointer");
     EnsSetup S;
+    // The angle rule's products (and so its table) feed the Greek coefficients and, without the
+    // caller's cosines, the phase matrix.
+    const bool rule_pass = d_greek || (d_phase && ctx->phase_mu.empty());
     if ((rc = ens_setup(ctx, count, d_kind, d_p0, d_p1, d_p2, d_wavelength, layers, d_fractions, d_m,
                         n_radius, n_angles, d_greek != nullptr, order, allow_unvalidated,
-                        ctx->angular_kernel, S, d_nodes)))
+                        ctx->angular_kernel, S, d_nodes, rule_pass)))
         return rc;
     if (total_terms) *total_terms = S.total_terms;
     S.a.tan_dir = tan_dir;
@@ -48,6 +51,9 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
     // more, at the distinct |mu| as the non-negative cosines of a symmetric set of 2 h; the
     // Greek coefficients keep the angle rule's products (computed only when asked for).
     const bool user_mu = d_phase && !ctx->phase_mu.empty();
+    // Without angular products the node solves need stage nothing (small launches then solve
+    // one node per wavefront).
+    S.values_only = !rule_pass && !user_mu;
     PhaseMu U;
     if (user_mu && (rc = phase_mu_setup(ctx, S, U))) return rc;
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B, 16 B in fp32).
@@ -65,10 +71,7 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
         ChunkBufs B;
         if ((rc = forward_nodes(ctx, S, e0, ne, chunk_terms, n_radius, n_angles, B, nullptr, s)))
             return rc;
-        // The angle rule's products feed the Greek coefficients and, without the caller's
-        // cosines, the phase matrix.
-        if ((d_greek || (d_phase && !user_mu)) &&
-            (rc = forward_angular(ctx, S, e0, ne, n_radius, n_angles, B, s)))
+        if (rule_pass && (rc = forward_angular(ctx, S, e0, ne, n_radius, n_angles, B, s)))
             return rc;
         double *tab_u = nullptr, *fac_u = nullptr;
         if (user_mu) {

@@ -34,6 +34,7 @@
 #include "ensemble_common.h"
 #include "layered_device.h"
 #include "mie_device.h"
+#include "mie_wave.h"
 #include "quadrature.h"
 #include "sort.h"
 #include "util.h"
@@ -565,6 +566,49 @@ static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
         if (idx < chunk_nodes)
             forward_node<1>(a, e0, order[idx], node_n, node_off, ens_base, slab, rows, tile,
                             staging, stage32, vals, tabc, xexp, back_out);
+    }
+}
+
+// The node solves of a small values-only launch (no angular products, so nothing staged), one
+// node per wavefront (blocks of one wavefront) from the shared counter over the ranked nodes:
+// the node's values as forward_node writes them, its coefficient exponent 0.
+static __global__ __launch_bounds__(64) void ens_nodes_wave(
+    EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
+    unsigned long long *counter, char *cols, int64_t rows, int64_t wave_bytes, double *vals,
+    double *tabc, int *xexp) {
+    const int lane = threadIdx.x;
+    char *base = cols + (int64_t)blockIdx.x * wave_bytes;
+    cx *uz = reinterpret_cast<cx *>(base);
+    double *ux = reinterpret_cast<double *>(base + rows * (int64_t)sizeof(cx));
+    while (true) {
+        unsigned long long idx = 0;
+        if (lane == 0) idx = atomicAdd(counter, 1ull);
+        idx = __shfl(idx, 0, 64);
+        if ((int64_t)idx >= chunk_nodes) break;
+        const int64_t g = order[idx];
+        const int64_t e = e0 + g / a.n_radius, i = g % a.n_radius;
+        const NodeGeom geo = node_stored(a, e, i);
+        const int n = node_n[e * a.n_radius + i];
+        const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
+        double q[5];
+        sphere_efficiencies_wave(geo.x, m, n, uz, ux, lane, q);
+        if (lane == 0) {
+            const double area = kPi * geo.r * geo.r;
+            const double W = geo.W;
+            double *v = vals + g * kNodeVals;
+            v[0] = W;
+            v[1] = W * (area * q[0]);
+            v[2] = W * (area * q[1]);
+            v[3] = W * (area * q[1] * q[4]);
+            v[4] = W * (area * q[2]);
+            v[5] = W * (area * q[3]);
+            v[6] = W * (geo.r * geo.r);
+            v[7] = W * (geo.r * geo.r * geo.r);
+            tabc[g] = W * (area * (4.0 / (geo.x * geo.x)));
+            xexp[g] = 0;
+        }
+        // The columns are rewritten by the next node only after every lane has read them.
+        __syncthreads();
     }
 }
 
@@ -1606,6 +1650,7 @@ struct EnsSetup {
     int texp = 0;               // its scale 2^texp
     int kernel = 0;             // the angular kernel of the call
     bool stage32 = false;       // the node solves stage the fp32 form (kernel 2, forward calls)
+    bool values_only = false;   // a forward call without angular products: no staging needed
 };
 
 // The angle rule of n cosines from the context's cache, computed and uploaded on a miss (the
@@ -1653,7 +1698,8 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                      int layers,
                      const double *d_fractions, const double *d_m, int64_t n_radius,
                      int64_t n_angles, bool greek, int64_t order, int allow_unvalidated,
-                     int kernel, EnsSetup &S, const double *d_nodes = nullptr) {
+                     int kernel, EnsSetup &S, const double *d_nodes = nullptr,
+                     bool rule_tables = true) {
     int rc;
     if ((rc = check_hip(ctx, hipSetDevice(ctx->device), "hipSetDevice"))) return rc;
     hipStream_t s = ctx->stream;
@@ -1764,17 +1810,23 @@ static int ens_setup(scatrs_ctx *ctx, int64_t count, const int32_t *d_kind, cons
                 return rc;
         }
     }
-    // The w pi_n, w tau_n table at the non-negative cosines up to the largest N.
-    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables, (size_t)S.n_max * h * sizeof(double2)),
-                        "table")))
-        return rc;
-    S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
-    pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
+    // The w pi_n, w tau_n table at the non-negative cosines up to the largest N (not made when
+    // the call takes no angular products at the rule's cosines: rule_tables false).
+    if (rule_tables) {
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables,
+                                        (size_t)S.n_max * h * sizeof(double2)),
+                            "table")))
+            return rc;
+        S.table = static_cast<double2 *>(ctx->buf[scatrs_ctx::kTables]);
+        pt_table<<<(h + 63) / 64, 64, 0, s>>>(S.d_atab + n_angles, h, S.n_max, S.table);
+    }
     S.kernel = kernel == 2 && n_radius > kF16MaxNodes ? 0 : kernel;
     S.stage32 = S.kernel == 2;
     if (S.kernel == 2) {
         S.npad = (S.n_max + kF16Orders - 1) / kF16Orders * kF16Orders;
         S.texp = table_exponent(S.n_max);
+    }
+    if (S.kernel == 2 && rule_tables) {
         const size_t cells = (size_t)h * S.npad;
         if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kTables16, 4 * cells * sizeof(_Float16)),
                             "table")))
@@ -1891,6 +1943,12 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
         }
         if constexpr (L == 1) {
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
+            if (q.waves < 0) {
+                ens_nodes_wave<<<(unsigned)(-q.waves), 64, 0, s>>>(
+                    a, e0, q.end, B.order, S.node_n, S.counter, reinterpret_cast<char *>(scratch),
+                    q.rows, mie_wave_bytes(q.rows - 2), B.vals, B.tabc, B.xexp);
+                continue;
+            }
             ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
                 a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter, scratch,
                 q.rows, B.staging, S.stage32, B.vals, B.tabc, B.xexp, back_out);
@@ -1920,7 +1978,7 @@ static void record_plan(scatrs_ctx *ctx, const ChunkBufs &B, size_t scratch_byte
 // shrink the launches of the many small ones. *scratch_bytes receives the slab bytes of the
 // largest launch. Synchronizes: the histogram of N comes back to the host.
 static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t nodes,
-                      ChunkBufs &B, size_t *scratch_bytes, hipStream_t s) {
+                      int64_t chunk_terms, ChunkBufs &B, size_t *scratch_bytes, hipStream_t s) {
     int rc;
     const int kord = scatrs_ctx::kOrder;
     if ((rc = check_hip(ctx, ensure(ctx, kord,
@@ -1961,6 +2019,16 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         // Bucket 0 holds every N above kBuckets - 2: its rows come from the largest N.
         const int64_t n = b0 == 0 ? S.n_max : std::max(kBuckets - 2 - b0, 0);
         const int64_t rows = n + 2;
+        if (S.values_only && S.a.tan_dir < 0 &&
+            wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, chunk_terms, n)) {
+            // One node per wavefront (ens_nodes_wave), two wavefronts per SIMD at most: the
+            // plan's negative count marks it.
+            const int64_t waves = std::min<int64_t>(nodes, (int64_t)ctx->cus * 8);
+            B.plan.push_back(NodeLaunch{0, nodes, rows, -waves});
+            *scratch_bytes = (size_t)(mie_wave_bytes(n) * waves);
+            record_plan(ctx, B, *scratch_bytes);
+            return SCATRS_OK;
+        }
         const size_t per_wave = (size_t)slab_elems(1, rows) * sizeof(cx);
         int64_t waves = std::min<int64_t>((int64_t)ctx->cus * ctx->tune[SCATRS_TUNE_NODE_WAVES],
                                           (nodes + 63) / 64);
@@ -2003,7 +2071,7 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     const int kstg = scatrs_ctx::kStaging, knod = scatrs_ctx::kNodes, kacc = scatrs_ctx::kAccum;
     const size_t term_bytes = S.stage32 ? sizeof(float4) : 2 * sizeof(cx);
     size_t scratch_bytes = 0;
-    if ((rc = plan_nodes(ctx, S, e0, nodes, B, &scratch_bytes, s))) return rc;
+    if ((rc = plan_nodes(ctx, S, e0, nodes, chunk_terms, B, &scratch_bytes, s))) return rc;
     if ((rc = check_hip(ctx, ensure(ctx, kstg, (size_t)std::max<int64_t>(chunk_terms, 1) * term_bytes),
                         "staging")))
         return rc;

@@ -351,16 +351,26 @@ struct has_block_end : std::false_type {};
 template <class S>
 struct has_block_end<S, std::void_t<decltype(&S::block_end)>> : std::true_type {};
 
+// A range of orders k0..k1 of a sphere and the recurrence state on entry to order k0:
+// psi = psi_{k0-1}(x), psi_prev = psi_{k0-2}(x), chi and chi_prev likewise (orders k0 <= n_s),
+// and above n_s the ratio region's t, 1/xi and r of order k0 - 1.
+struct OrderStart {
+    int k0, k1;
+    double psi, psi_prev, chi, chi_prev;
+    cx t, inv_xi, r;
+};
+
 // The upward order loop (scatrs/src/mie/kernel.rs:325-559) after ratio_chain and x_chain.
 // Calls sink.order(k, a, b) (or sink.order_d with the four derivatives) for k = 1..n and
 // sink.done(abs_sum, jac). With a layered Outer source the per-order sides replace D_n(mx);
 // with derivatives the sink then also receives, after order_d of the same order,
 // sink.stored(k, inv_da, inv_db, g_a, g_b, d_m, md, outer), outer holding that order's
-// inner-layer sensitivities.
+// inner-layer sensitivities. With `start` (an unblocked source only) the loop runs over
+// start->k0..k1 from start's state, and abs_sum covers those orders.
 template <bool DERIVS, class ZSrc, class Outer, class Sink>
 __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs,
                                            const double *ux, int stride, Outer &outer,
-                                           Sink &sink) {
+                                           Sink &sink, const OrderStart *start = nullptr) {
     const cx z = m * x;
     const int n_s = upward_orders(x, n);
     const double inv_x = 1.0 / x;
@@ -378,6 +388,15 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
         t = real(sin_x) * inv(mk(sin_x, -cos_x));
         inv_xi = mk(sin_x, cos_x);
         r = mk(0.0, 1.0);
+    }
+    if (start) {
+        psi = start->psi;
+        psi_prev = start->psi_prev;
+        chi = start->chi;
+        chi_prev = start->chi_prev;
+        t = start->t;
+        inv_xi = start->inv_xi;
+        r = start->r;
     }
     const bool m_is_one = (m.re == 1.0 && m.im == 0.0);
     double abs_sum = 0.0;
@@ -611,16 +630,17 @@ __device__ __forceinline__ void mie_orders(double x, cx m, int n, const ZSrc &zs
         // has a whole order of arithmetic to arrive and stays in its registers.
         const cx *uz = zs.uz;
         const int zst = zs.stride;
-        cx pa = uz[2 * zst];
-        cx pb = n >= 2 ? uz[3 * zst] : real(0.0);
-        int k = 1;
-        for (; k + 1 <= n; k += 2) {
+        const int ka = start ? start->k0 : 1, kb = start ? start->k1 : n;
+        cx pa = ka <= kb ? uz[(ka + 1) * zst] : real(0.0);
+        cx pb = ka + 1 <= kb ? uz[(ka + 2) * zst] : real(0.0);
+        int k = ka;
+        for (; k + 1 <= kb; k += 2) {
             order_step(k, pa);
-            if (k + 3 <= n + 1) pa = uz[(k + 3) * zst];
+            if (k + 3 <= kb + 1) pa = uz[(k + 3) * zst];
             order_step(k + 1, pb);
-            if (k + 4 <= n + 1) pb = uz[(k + 4) * zst];
+            if (k + 4 <= kb + 1) pb = uz[(k + 4) * zst];
         }
-        if (k == n) order_step(k, pa);
+        if (k == kb) order_step(k, pa);
     }
     sink.done(abs_sum, jac0, jac1, jac2);
 }

@@ -16,6 +16,7 @@
 #include "context.h"
 #include "ensemble_common.h"
 #include "mie_device.h"
+#include "mie_wave.h"
 #include "sort.h"
 #include "util.h"
 
@@ -137,6 +138,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
     }
 }
 
+// The solve of a small batch, one sphere per wavefront (blocks of one wavefront), largest N
+// first from the shared counter: the wavefront's columns (wave_bytes each from cols) hold its
+// sphere's u_k(m x) (rows complex) and u_k(x) (rows doubles).
+__global__ __launch_bounds__(64) void mie_wave(const double *__restrict__ x,
+                                               const double *__restrict__ m, int64_t len,
+                                               int64_t n_fixed, const int *__restrict__ perm,
+                                               unsigned long long *counter, char *cols,
+                                               int64_t rows, int64_t wave_bytes,
+                                               double *__restrict__ values) {
+    const int lane = threadIdx.x;
+    char *base = cols + (int64_t)blockIdx.x * wave_bytes;
+    cx *uz = reinterpret_cast<cx *>(base);
+    double *ux = reinterpret_cast<double *>(base + rows * (int64_t)sizeof(cx));
+    while (true) {
+        unsigned long long idx = 0;
+        if (lane == 0) idx = atomicAdd(counter, 1ull);
+        idx = __shfl(idx, 0, 64);
+        if ((int64_t)idx >= len) break;
+        const int i = perm[idx];
+        const double xi = x[i];
+        const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
+        const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
+        double q[5];
+        sphere_efficiencies_wave(xi, mi, n, uz, ux, lane, q);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) values[k * len + i] = q[k];
+        }
+        // The columns are rewritten by the next sphere only after every lane has read them.
+        __syncthreads();
+    }
+}
+
+// Whether a batch takes mie_wave: values only, and the rule of wave_rule.
+bool use_wave(const scatrs_ctx *ctx, bool derivatives, int64_t terms, int64_t nmax) {
+    return !derivatives && wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, terms, nmax);
+}
 
 // ---- the batched-sphere path, shared by the allocating and the no-allocation entry points ----
 
@@ -246,6 +284,20 @@ int mie_solve(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double
     return check_hip(ctx, hipGetLastError(), "mie_batch launch");
 }
 
+// The wavefront-per-sphere solve over the N-ranked permutation on `waves` wavefronts whose
+// columns (mie_wave_bytes(nmax) each) start at w.slab.
+int mie_solve_wave(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double *d_m,
+                   int64_t len, int64_t n_max, int64_t nmax, int64_t waves, double *d_values) {
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(
+        static_cast<char *>(w.status) + sizeof(Status));
+    mark_begin(ctx);
+    mie_wave<<<(unsigned)waves, 64, 0, ctx->stream>>>(d_x, d_m, len, n_max, w.perm, counter,
+                                                      static_cast<char *>(w.slab), nmax + 2,
+                                                      mie_wave_bytes(nmax), d_values);
+    mark_end(ctx);
+    return check_hip(ctx, hipGetLastError(), "mie_wave launch");
+}
+
 // ---- the no-allocation scratch ----
 
 // Device memory of scatrs_mie_efficiencies_noalloc: every call on a device carves its status,
@@ -347,8 +399,8 @@ int scatrs_set_profiling(scatrs_ctx *ctx, int on) {
 int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value) {
     if (!ctx) return SCATRS_INVALID_INPUT;
     ctx->error.clear();
-    static const int64_t lo[5] = {1, 1, 1, 1, 0}, hi[5] = {32, 32, 64, 64, 1};
-    if (key < 0 || key > 4)
+    static const int64_t lo[6] = {1, 1, 1, 1, 0, 0}, hi[6] = {32, 32, 64, 64, 1, 4096};
+    if (key < 0 || key > 5)
         return fail(ctx, SCATRS_INVALID_INPUT, "unknown tuning key " + std::to_string(key));
     if (value < lo[key] || value > hi[key])
         return fail(ctx, SCATRS_INVALID_INPUT,
@@ -360,7 +412,7 @@ int scatrs_set_tuning(scatrs_ctx *ctx, int key, int64_t value) {
 }
 
 int64_t scatrs_get_tuning(const scatrs_ctx *ctx, int key) {
-    if (!ctx || key < 0 || key > 4) return -1;
+    if (!ctx || key < 0 || key > 5) return -1;
     return ctx->tune[key];
 }
 
@@ -486,9 +538,19 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
     w.keys = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
     w.perm = static_cast<int *>(ctx->buf[scatrs_ctx::kPerm]);
     w.partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kNodes]);
-    int64_t nmax = 0;
-    if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
+    int64_t nmax = 0, terms = 0;
+    if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, &terms)))
         return rc;
+    if (total_terms) *total_terms = terms;
+    if (use_wave(ctx, d_jacobian != nullptr, terms, nmax)) {
+        const int64_t waves = std::min<int64_t>(len, (int64_t)ctx->cus * 8);
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch,
+                                        (size_t)(mie_wave_bytes(nmax) * waves)),
+                            "scratch")))
+            return rc;
+        w.slab = ctx->buf[scatrs_ctx::kScratch];
+        return mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values);
+    }
     // Persistent wavefronts: the tuned number per CU (default 8: two per SIMD, which the
     // kernel's registers allow; 4-wave blocks share the CU's LDS tiles of kSphereBlock orders),
     // fewer if the slabs would pass 8 GiB.
@@ -535,9 +597,24 @@ int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const do
     w.partial = reinterpret_cast<Partial *>(take((size_t)blocks * sizeof(Partial)));
     w.slab = base + off;
     const size_t slab_bytes = kScratchBytes - off;
-    int64_t nmax = 0;
-    if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, total_terms)))
+    int64_t nmax = 0, terms = 0;
+    if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, &terms)))
         return rc;
+    if (total_terms) *total_terms = terms;
+    if (use_wave(ctx, d_jacobian != nullptr, terms, nmax)) {
+        // As many wavefronts as the scratch holds columns for, up to two per SIMD.
+        const int64_t wb = mie_wave_bytes(nmax);
+        const int64_t waves = std::min<int64_t>({len, (int64_t)ctx->cus * 8,
+                                                 (int64_t)(slab_bytes / (size_t)wb)});
+        if (waves < 1) {
+            if (total_terms) *total_terms = 0;
+            return fail(ctx, SCATRS_INVALID_INPUT,
+                        "N = " + std::to_string(nmax) +
+                            " terms need more column memory than the no-allocation scratch "
+                            "holds (" + std::to_string(kScratchBytes >> 20) + " MiB)");
+        }
+        return mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values);
+    }
     // As many persistent wavefronts as the scratch holds slabs for, up to the tuned number per
     // CU, in whole blocks of 4.
     const size_t per_wave = mie_slab_per_wave(nmax);

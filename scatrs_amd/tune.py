@@ -9,7 +9,10 @@ The parameters (keys of ``KEYS``):
     memory (larger chunks wait less often for their slowest node, smaller ones leave memory);
   * ``split_blocks``: blocks per CU a small batch's angular products aim for;
   * ``sphere_plain`` (not tuned: a switch for tests): 0 runs every sphere of the sphere batch on
-    the checked path instead of trying the branch-free plain path first.
+    the checked path instead of trying the branch-free plain path first;
+  * ``sphere_wave`` (not tuned: set by a rule of the batch's size): when a values-only sphere
+    batch takes one wavefront per sphere (0 never, 1 always, f >= 2 when its terms times f are
+    fewer than the GPU's lanes times its largest N).
 
 Results do not depend on the first three; the fourth changes a small batch's summation order
 (rounding only). Tuning coordinate by coordinate: each parameter in turn over its candidates,
@@ -26,9 +29,9 @@ from pathlib import Path
 from typing import Callable, Sequence
 
 KEYS = {"sphere_waves": 0, "node_waves": 1, "chunk_share": 2, "split_blocks": 3,
-        "sphere_plain": 4}
+        "sphere_plain": 4, "sphere_wave": 5}
 DEFAULTS = {"sphere_waves": 16, "node_waves": 8, "chunk_share": 4, "split_blocks": 2,
-            "sphere_plain": 1}
+            "sphere_plain": 1, "sphere_wave": 8}
 CANDIDATES = {
     "sphere_waves": (4, 6, 8, 12, 16, 20),
     "node_waves": (4, 6, 8, 12),

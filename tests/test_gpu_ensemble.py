@@ -71,6 +71,38 @@ def test_random_lognormals_match_oracle():
         _check(sc[e], gr[e], ph[e], ref, 1e-9)
 
 
+def test_scalars_only_calls_solve_one_node_per_wavefront():
+    """A call without phase or Greek outputs takes no angular products; its node solves then run
+    one node per wavefront when the launch is small (SCATRS_TUNE_SPHERE_WAVE = 1 forces it, 0
+    keeps one node per lane): both match the oracle's scalars and each other to rounding."""
+    from scatrs_amd import _native, tune
+
+    rng = np.random.default_rng(23)
+    n = 16
+    p0 = np.exp(rng.uniform(np.log(0.01), np.log(2.0), n))
+    p1 = rng.uniform(1.2, 2.0, n)
+    wl = np.exp(rng.uniform(np.log(0.3), np.log(2.5), n))
+    m = rng.uniform(1.3, 1.7, n) + 1j * np.exp(rng.uniform(np.log(1e-8), np.log(1e-1), n))
+    ctx = _native.context(0)
+    out = {}
+    try:
+        for mode in (1, 0):
+            ctx.set_tuning(tune.KEYS["sphere_wave"], mode)
+            out[mode] = _gpu(np.zeros(n), p0, p1, wl, m, n_radius=96, n_angles=16, order=None,
+                             want_phase=False)
+    finally:
+        ctx.set_tuning(tune.KEYS["sphere_wave"], tune.DEFAULTS["sphere_wave"])
+    sw, sl = out[1][0], out[0][0]
+    assert out[1][1] is None and out[1][2] is None
+    for e in range(n):
+        ref = oracle.ensemble_fixed(0, p0[e], p1[e], wl[e], m[e], n_radius=96, n_angles=16,
+                                    order=-1)
+        _check(sw[e], None, None, ref, 1e-9)
+    rel = np.abs(sw - sl) / np.maximum(np.abs(sl), 1e-300)
+    rel[:, 7] = np.abs(sw[:, 7] - sl[:, 7])
+    assert rel.max() < 1e-11, rel.max()
+
+
 def test_gamma_and_unnormalized_match_oracle():
     p0 = np.array([0.2, 0.5, 1.1])
     p1 = np.array([0.05, 0.2, 0.38])

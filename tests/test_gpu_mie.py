@@ -94,15 +94,17 @@ def test_parity_with_oracle_on_batched_spheres():
 
 @pytest.mark.parametrize("plain", [1, 0])
 def test_values_without_derivatives_on_both_paths(plain):
-    """The values-only batch (the branch-free plain path tried first, and with
-    SCATRS_TUNE_SPHERE_PLAIN = 0 the checked path for every sphere) within 2x the reference's
-    declared tolerance of the oracle, and the two paths within a few ulps of each other."""
+    """The values-only batch on the one-sphere-per-lane kernel (SCATRS_TUNE_SPHERE_WAVE = 0; the
+    branch-free plain path tried first, and with SCATRS_TUNE_SPHERE_PLAIN = 0 the checked path
+    for every sphere) within 2x the reference's declared tolerance of the oracle, and the two
+    paths within a few ulps of each other."""
     from scatrs_amd import _native, tune
 
     x, m = _seeded(20000, seed=11)
     ref, jref = oracle.mie_efficiencies_batch(x, m, derivatives=True)
     ctx = _native.context(0)
     try:
+        ctx.set_tuning(tune.KEYS["sphere_wave"], 0)
         ctx.set_tuning(tune.KEYS["sphere_plain"], plain)
         assert ctx.get_tuning(tune.KEYS["sphere_plain"]) == plain
         got = _stack(_gpu(x, m))
@@ -110,6 +112,7 @@ def test_values_without_derivatives_on_both_paths(plain):
         other = _stack(_gpu(x, m))
     finally:
         ctx.set_tuning(tune.KEYS["sphere_plain"], 1)
+        ctx.set_tuning(tune.KEYS["sphere_wave"], tune.DEFAULTS["sphere_wave"])
     for i in range(x.size):
         q = ref[:, i]
         sizes = efficiency_sizes(q, x[i])
@@ -122,6 +125,57 @@ def test_values_without_derivatives_on_both_paths(plain):
     rel = np.abs(got - other)[[0, 1, 4]] / np.abs(other[[0, 1, 4]])
     assert rel.max() < 1e-12, rel.max()
     assert np.all(got[0] == got[1] + got[2])
+
+
+def _with_tuning(key, value, fn):
+    from scatrs_amd import _native, tune
+
+    ctx = _native.context(0)
+    old = ctx.get_tuning(tune.KEYS[key])
+    ctx.set_tuning(tune.KEYS[key], value)
+    try:
+        return fn()
+    finally:
+        ctx.set_tuning(tune.KEYS[key], old)
+
+
+def test_wavefront_per_sphere_path():
+    """The values-only batch solved one sphere per wavefront (each sphere's recurrences split over
+    the 64 lanes and joined by scans; SCATRS_TUNE_SPHERE_WAVE = 1 forces it) within 2x the
+    reference's declared tolerance of the oracle and within rounding of the one-sphere-per-lane
+    path, on the seeded batch and on spheres below x = 1, with a real index, and with N fixed
+    below and above 64 terms."""
+    x, m = _seeded(20000, seed=5)
+    x = np.concatenate([x, [1e-3, 0.05, 0.7, 0.999, 1.0, 1.5, 2500.0, 9000.0]])
+    m = np.concatenate([m, [1.5 + 0.1j, 1.33 + 0j, 2.0 + 1e-3j, 1.1 + 1e-6j, 1.5 + 0j,
+                            1.33 + 1e-4j, 1.5 + 0.01j, 1.33 + 1e-8j]])
+    ref, jref = oracle.mie_efficiencies_batch(x, m, derivatives=True)
+    got = _stack(_with_tuning("sphere_wave", 1, lambda: _gpu(x, m)))
+    lane = _stack(_with_tuning("sphere_wave", 0, lambda: _gpu(x, m)))
+    worst = 0.0
+    for i in range(x.size):
+        q = ref[:, i]
+        sizes = efficiency_sizes(q, x[i])
+        conds = efficiency_conditions(q, jref[:, :, i], x[i], m[i])
+        for k in range(5):
+            tol = 2.0 * tolerance("efficiencies", x[i], m[i], conds[k])
+            err = abs(got[k, i] - q[k]) / sizes[k]
+            worst = max(worst, err / tol)
+            assert err <= tol, (i, k, x[i], m[i], got[k, i], q[k])
+    rel = np.abs(got - lane)[[0, 1, 4]] / np.abs(lane[[0, 1, 4]])
+    print("worst error / 2x declared tolerance:", worst, "largest difference from the lane path:",
+          rel.max())
+    assert rel.max() < 1e-10, rel.max()
+    assert np.all(got[0] == got[1] + got[2])
+    for xs, ms, n in ((3.0, 1.5 + 0.01j, 20), (30.0, 1.33 + 1e-6j, 60), (150.0, 1.7 + 0.1j, 200)):
+        w = _stack(_with_tuning("sphere_wave", 1,
+                                lambda: _gpu(np.array([xs]), np.array([ms]), n_max=n)))
+        s_ = oracle.mie_solve(xs, ms, n=n)
+        assert np.allclose(w[:, 0], oracle.efficiencies(xs, s_), rtol=1e-11, atol=0)
+    one = _stack(_with_tuning("sphere_wave", 1, lambda: _gpu(np.array([700.0]),
+                                                             np.array([1.5 + 1e-3j]))))
+    r1, _ = oracle.mie_efficiencies_batch(np.array([700.0]), np.array([1.5 + 1e-3j]))
+    assert np.allclose(one[[0, 1, 4]], r1[[0, 1, 4]], rtol=1e-11, atol=0)
 
 
 def test_real_index_gives_zero_absorption():
