@@ -54,6 +54,7 @@ static int ensemble_forward(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
     // Without angular products the node solves need stage nothing (small launches then solve
     // one node per wavefront).
     S.values_only = !rule_pass && !user_mu;
+    S.forward_only = true;
     PhaseMu U;
     if (user_mu && (rc = phase_mu_setup(ctx, S, U))) return rc;
     // Chunks of ensembles bounded by the staging buffer (a, b per term: 32 B, 16 B in fp32).

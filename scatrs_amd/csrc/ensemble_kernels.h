@@ -569,13 +569,16 @@ static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     }
 }
 
-// The node solves of a small values-only launch (no angular products, so nothing staged), one
-// node per wavefront (blocks of one wavefront) from the shared counter over the ranked nodes:
-// the node's values as forward_node writes them, its coefficient exponent 0.
+// The node solves of a small launch (spheres), one node per wavefront (blocks of one
+// wavefront) from the shared counter over the ranked nodes: what forward_node writes (staging,
+// values, table factor, coefficient exponent, backscatter sum), its orders split over the
+// lanes (sphere_efficiencies_wave). staging == nullptr: nothing staged (a call without angular
+// products).
 static __global__ __launch_bounds__(64) void ens_nodes_wave(
     EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
-    unsigned long long *counter, char *cols, int64_t rows, int64_t wave_bytes, double *vals,
-    double *tabc, int *xexp) {
+    const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, char *cols,
+    int64_t rows, int64_t wave_bytes, cx *staging, int stage32, double *vals, double *tabc,
+    int *xexp, cx *back_out) {
     const int lane = threadIdx.x;
     char *base = cols + (int64_t)blockIdx.x * wave_bytes;
     cx *uz = reinterpret_cast<cx *>(base);
@@ -590,8 +593,20 @@ static __global__ __launch_bounds__(64) void ens_nodes_wave(
         const NodeGeom geo = node_stored(a, e, i);
         const int n = node_n[e * a.n_radius + i];
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
-        double q[5];
-        sphere_efficiencies_wave(geo.x, m, n, uz, ux, lane, q);
+        double q[5], amax = 0.0;
+        cx back;
+        if (!staging) {
+            sphere_efficiencies_wave(geo.x, m, n, uz, ux, lane, q, NoStage{}, &amax, &back);
+        } else {
+            const int64_t off = ens_base[e] + node_off[e * a.n_radius + i];
+            if (stage32)
+                sphere_efficiencies_wave(geo.x, m, n, uz, ux, lane, q,
+                                         StageF32{reinterpret_cast<float4 *>(staging) + off},
+                                         &amax, &back);
+            else
+                sphere_efficiencies_wave(geo.x, m, n, uz, ux, lane, q, StageF64{staging + 2 * off},
+                                         &amax, &back);
+        }
         if (lane == 0) {
             const double area = kPi * geo.r * geo.r;
             const double W = geo.W;
@@ -605,7 +620,8 @@ static __global__ __launch_bounds__(64) void ens_nodes_wave(
             v[6] = W * (geo.r * geo.r);
             v[7] = W * (geo.r * geo.r * geo.r);
             tabc[g] = W * (area * (4.0 / (geo.x * geo.x)));
-            xexp[g] = 0;
+            xexp[g] = coefficient_exponent(amax);
+            if (back_out) back_out[g] = back;
         }
         // The columns are rewritten by the next node only after every lane has read them.
         __syncthreads();
@@ -1651,6 +1667,10 @@ struct EnsSetup {
     int kernel = 0;             // the angular kernel of the call
     bool stage32 = false;       // the node solves stage the fp32 form (kernel 2, forward calls)
     bool values_only = false;   // a forward call without angular products: no staging needed
+    // A forward call (scatrs_ensemble_fixed / _nodes): its node plan serves only its own node
+    // launches, so small launches may solve one node per wavefront. The reverse pass reuses its
+    // forward's plan for the reverse node kernels, which need the slab layout.
+    bool forward_only = false;
 };
 
 // The angle rule of n cosines from the context's cache, computed and uploaded on a miss (the
@@ -1945,8 +1965,10 @@ static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, con
             fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
             if (q.waves < 0) {
                 ens_nodes_wave<<<(unsigned)(-q.waves), 64, 0, s>>>(
-                    a, e0, q.end, B.order, S.node_n, S.counter, reinterpret_cast<char *>(scratch),
-                    q.rows, mie_wave_bytes(q.rows - 2), B.vals, B.tabc, B.xexp);
+                    a, e0, q.end, B.order, S.node_n, S.node_off, S.ens_base, S.counter,
+                    reinterpret_cast<char *>(scratch), q.rows, mie_wave_bytes(q.rows - 2),
+                    S.values_only ? nullptr : B.staging, S.stage32, B.vals, B.tabc, B.xexp,
+                    back_out);
                 continue;
             }
             ens_nodes_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
@@ -2019,7 +2041,7 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         // Bucket 0 holds every N above kBuckets - 2: its rows come from the largest N.
         const int64_t n = b0 == 0 ? S.n_max : std::max(kBuckets - 2 - b0, 0);
         const int64_t rows = n + 2;
-        if (S.values_only && S.a.tan_dir < 0 &&
+        if (S.forward_only && S.a.tan_dir < 0 &&
             wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, chunk_terms, n)) {
             // One node per wavefront (ens_nodes_wave), two wavefronts per SIMD at most: the
             // plan's negative count marks it.

@@ -898,7 +898,13 @@ static int ensemble_reverse(scatrs_ctx *ctx, int64_t count, const int32_t *d_kin
         } else {
             fill_f64<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(ht, 0.0, nodes);
         }
-        // The forward chunk's ranking and launch plan, and its scratch, serve the reverse solves.
+        // The forward chunk's ranking and launch plan, and its scratch, serve the reverse solves:
+        // the plan must be the slab layout's (a forward call's plan may instead solve small
+        // launches one node per wavefront, which the reverse kernels cannot follow).
+        for (const NodeLaunch &q : B.plan)
+            if (q.waves < 0 || (layers == 1 && q.waves < 4))
+                return fail(ctx, SCATRS_DEVICE_ERROR,
+                            "internal: the reverse node solves met a launch plan without slabs");
         cx *scratch = static_cast<cx *>(ctx->buf[scatrs_ctx::kScratch]);
         const EnsArgs &a = S.a;
         mark_begin(ctx);

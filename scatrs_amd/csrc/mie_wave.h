@@ -151,16 +151,42 @@ __device__ __forceinline__ void chain_down(int top, int bottom, int keep, Coef c
     }
 }
 
+// What a lane does with each of its orders' coefficients besides the efficiency sums: nothing,
+// or write them to a node's staging rows (FP64 a_1, b_1, a_2, ...; or one fp32 float4 (a.re,
+// a.im, b.re, b.im) per order), as the ensembles' node solves stage them.
+struct NoStage {
+    __device__ void operator()(int, cx, cx) const {}
+};
+struct StageF64 {
+    cx *ab;
+    __device__ void operator()(int k, cx a, cx b) const {
+        ab[2 * (k - 1)] = a;
+        ab[2 * (k - 1) + 1] = b;
+    }
+};
+struct StageF32 {
+    float4 *ab;
+    __device__ void operator()(int k, cx a, cx b) const {
+        ab[k - 1] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+    }
+};
+
 // The efficiency sink of one lane's orders: also keeps the first order's coefficients, for the
-// pair term across the boundary with the previous lane's last order.
+// pair term across the boundary with the previous lane's last order, stages each order's
+// coefficients, and keeps their largest |Re|, |Im|.
+template <class Stage>
 struct WaveEfficiencySink : EfficiencySink<false> {
+    Stage stage;
     int k_first = 0;
     cx a_first = {0.0, 0.0}, b_first = {0.0, 0.0};
+    double amax = 0.0;
     __device__ void order(int k, cx a, cx b) {
         if (k == k_first) {
             a_first = a;
             b_first = b;
         }
+        stage(k, a, b);
+        amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
         EfficiencySink<false>::order(k, a, b);
     }
 };
@@ -171,11 +197,23 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
 // One sphere's efficiencies on the whole wavefront (every lane calls it with the same sphere;
 // lane 0's q holds the result). uz (n + 2 complex) and ux (n + 2 doubles) are the wavefront's
-// columns, the u_k(m x) and u_k(x) chains indexed by k.
+// columns, the u_k(m x) and u_k(x) chains indexed by k. Each order's coefficients also go to
+// `stage`; *amax receives the largest |Re|, |Im| of them and *back the backscatter sum
+// sum (2n+1) (-1)^n (a_n - b_n) (every lane).
+template <class Stage = NoStage>
 __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, cx *uz,
-                                                         double *ux, int lane, double q[5]) {
+                                                         double *ux, int lane, double q[5],
+                                                         Stage stage = Stage{},
+                                                         double *amax = nullptr,
+                                                         cx *back = nullptr) {
     const int n_s = upward_orders(x, n);
     // u_k(x) above the upward orders (x_chain's range), kept in ux.
     if (n_s < n) {
@@ -272,7 +310,8 @@ __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, 
     st.inv_xi = ixi0 * ex;
     st.r = kr0 == k0 ? r_in : r0;
 
-    WaveEfficiencySink s;
+    WaveEfficiencySink<Stage> s;
+    s.stage = stage;
     s.k_first = k0;
     s.rk = k0 > 1 ? recip((double)k0) : 1.0;
     NoOuter none;
@@ -291,6 +330,8 @@ __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, 
     tot.abs_sum = wave_sum(s.abs_sum);
     tot.back = cx{wave_sum(s.back.re), wave_sum(s.back.im)};
     tot.values(x, q);
+    if (amax) *amax = wave_max(s.amax);
+    if (back) *back = tot.back;
 }
 
 // Bytes of one wavefront's columns (u_k(m x) complex, then u_k(x)) for spheres of up to nmax

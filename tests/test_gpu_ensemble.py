@@ -103,6 +103,43 @@ def test_scalars_only_calls_solve_one_node_per_wavefront():
     assert rel.max() < 1e-11, rel.max()
 
 
+@pytest.mark.parametrize("kernel", [0, 2])
+def test_node_solves_one_per_wavefront_stage_the_coefficients(kernel):
+    """With phase and Greek outputs the node solves of a small launch, one node per wavefront,
+    also stage each node's coefficients (FP64 for the FP64 matrix-core kernel, fp32 for the
+    fp16-split one) and its coefficient exponent: the outputs match the one-node-per-lane
+    solves' (to rounding in FP64; to the split products' 2^-22 with kernel 2) and the oracle."""
+    from scatrs_amd import _native, tune
+
+    rng = np.random.default_rng(29)
+    n = 8
+    p0 = np.exp(rng.uniform(np.log(0.01), np.log(1.0), n))
+    p1 = rng.uniform(1.2, 2.0, n)
+    wl = np.exp(rng.uniform(np.log(0.3), np.log(2.5), n))
+    m = rng.uniform(1.3, 1.7, n) + 1j * np.exp(rng.uniform(np.log(1e-8), np.log(1e-1), n))
+    ctx = _native.context(0)
+    out = {}
+    try:
+        ctx.set_angular_kernel(kernel)
+        for mode in (1, 0):
+            ctx.set_tuning(tune.KEYS["sphere_wave"], mode)
+            out[mode] = _gpu(np.zeros(n), p0, p1, wl, m, n_radius=64, n_angles=96, order=32)
+    finally:
+        ctx.set_tuning(tune.KEYS["sphere_wave"], tune.DEFAULTS["sphere_wave"])
+        ctx.set_angular_kernel(0)
+    tol = 1e-11 if kernel == 0 else 1e-6
+    (sw, gw, pw, _), (sl, gl, pl, _) = out[1], out[0]
+    rel = np.abs(sw - sl) / np.maximum(np.abs(sl), 1e-300)
+    rel[:, 7] = np.abs(sw[:, 7] - sl[:, 7])
+    assert rel.max() < 1e-11, rel.max()
+    for e in range(n):
+        assert np.max(np.abs(gw[e] - gl[e])) / np.max(np.abs(gl[e, 0])) < tol
+        assert np.max(np.abs(pw[e] - pl[e])) / np.max(np.abs(pl[e, 0])) < tol
+        ref = oracle.ensemble_fixed(0, p0[e], p1[e], wl[e], m[e], n_radius=64, n_angles=96,
+                                    order=32)
+        _check(sw[e], gw[e], pw[e], ref, 1e-9 if kernel == 0 else 2e-6)
+
+
 def test_gamma_and_unnormalized_match_oracle():
     p0 = np.array([0.2, 0.5, 1.1])
     p1 = np.array([0.05, 0.2, 0.38])
