@@ -40,7 +40,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .ensemble import segment_sum
+from .ensemble import segment_plan, segment_sum
 from .errors import InvalidInputError
 from .size import (SEGMENT, Gamma, LogNormal, ModifiedGamma, Monodisperse, MultiModal,
                    PowerLaw, Tabulated)
@@ -395,7 +395,7 @@ def _evaluate(units, wl: np.ndarray, *, R: int, grad: bool, order, mu, normalize
             gr_of[i] = None if gr is None else gr[q * B:(q + 1) * B]
             ph_of[i] = None if ph is None else ph[q * B:(q + 1) * B]
     U = len(units)
-    unit = torch.tensor([part[0] for part in parts], device=dev)
+    unit = segment_plan(np.array([part[0] for part in parts], np.int64), dev)
     f = torch.stack([part[6] for part in parts])  # (parts, B)
     sc_parts = torch.stack(sc_of)
     avg = segment_sum(unit, f[..., None] * number_averages(sc_parts), U)
@@ -602,7 +602,7 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                 parts = _parts(units, leaves)
             params = [(p[i::n], m_re[i::n], m_im[i::n], fr[i::n] if L > 1 else None)
                       for (_, _, p, m_re, m_im, fr, _, L, _) in parts for i in range(n)]
-            unit = torch.tensor([part[0] for part in parts], device=dev)
+            unit = segment_plan(np.array([part[0] for part in parts], np.int64), dev)
             f = torch.stack([part[6] for part in parts])  # (parts, B)
             kw = dict(order=order, n_angles=n_angles, mu=mu, want_phase=mu is not None,
                       allow_unvalidated=allow_unvalidated)
@@ -699,12 +699,13 @@ def _error_bounds(rows, kept, unit, U, P, n, R, normalized):
     (rk, rg), (avg, gr, ph, f) = rows, kept
     T = P * n
     dev = avg.device
+    kplan = segment_plan(rk["key"], dev)
 
     def task_err(a, b):
         if a is None:
             return None
         d = (a - b).abs()
-        return segment_sum(rk["key"], d, T * R)
+        return segment_sum(kplan, d, T * R)
 
     z = rk["z"]
     e_lin = task_err(rk["lin"], rg["lin"]) / z[:, None]

@@ -839,7 +839,8 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     are enabled; forward mode asks for them explicitly)."""
     import torch
 
-    from .ensemble import ensemble_fixed, ensemble_nodes, number_averages, segment_sum
+    from .ensemble import (ensemble_fixed, ensemble_nodes, number_averages, segment_plan,
+                           segment_sum)
 
     dev = params[0][0].device
     T = len(tasks)
@@ -877,19 +878,21 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     row_key = torch.zeros(n_rows * R, dtype=torch.int64, device=dev)
 
     def add(key, z, sc, g, p_, place):
+        """Adds rows with task-replica keys ``key`` (host array) at row places ``place``."""
         nonlocal avg, zt, gr, ph
+        plan = segment_plan(key, dev)
         lin = z[:, None] * number_averages(sc)
-        avg = avg + segment_sum(key, lin, T * R)
-        zt = zt + segment_sum(key, z, T * R)
+        avg = avg + segment_sum(plan, lin, T * R)
+        zt = zt + segment_sum(plan, z, T * R)
         row_lin[place] = lin.detach()
-        row_key[place] = key
+        row_key[place] = plan.index
         if gr is not None:
             gz = z[:, None, None] * g
-            gr = gr + segment_sum(key, gz, T * R)
+            gr = gr + segment_sum(plan, gz, T * R)
             row_gr[place] = gz.detach()
         if ph is not None:
             pz = z[:, None, None] * p_
-            ph = ph + segment_sum(key, pz, T * R)
+            ph = ph + segment_sum(plan, pz, T * R)
             row_ph[place] = pz.detach()
 
     wl_all = np.array([q.wl for q in tasks])
@@ -941,7 +944,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             z = rep_rows(z0_d)
             if grad:
                 z = z + ((pr - pr.detach()) * rep_rows(zd_d)).sum(1)
-            key = (tk_d[:, None] * R + rep[None, :]).reshape(-1)
+            key = (tk[:, None] * R + np.arange(R)[None, :]).reshape(-1)
             place = (sd[:, None] * R + rep[None, :]).reshape(-1)
             add(key, z, sc, g, p_, place)
     for k, ti in enumerate(mono):
@@ -953,7 +956,8 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             (R,), tasks[ti].wl, dtype=f64, device=dev), mr, mi, fr, p2=pr[:, 2], n_radius=1,
             n_angles=na, order=order, normalized=False, fixed_range=True,
             allow_unvalidated=allow_unvalidated, mu=mu, phase=want_phase)
-        add(ti * R + rep, torch.ones(R, dtype=f64, device=dev), sc, g, p_, (nr + k) * R + rep)
+        add(ti * R + np.arange(R), torch.ones(R, dtype=f64, device=dev), sc, g, p_,
+            (nr + k) * R + rep)
     # The moment integration's Z, M2, M3 (first order in p: the right value and derivative at
     # p0), in place of the nodes' own; a monodisperse law keeps its node's.
     pall = torch.cat([params[t][0] for t in range(T)])  # (T R, 3)

@@ -1999,8 +1999,12 @@ static void record_plan(scatrs_ctx *ctx, const ChunkBufs &B, size_t scratch_byte
 // wavefronts as kSlabBudget holds (at most kMaxNodeWaves), so that a few nodes of large N do not
 // shrink the launches of the many small ones. *scratch_bytes receives the slab bytes of the
 // largest launch. Synchronizes: the histogram of N comes back to the host.
+// The node launches of a chunk (nodes ranked by N on the device). Spheres: one persistent
+// launch whose slabs are sized by the chunk's largest N, from the N histogram or, for a chunk
+// that is the whole call (`whole`), the call's largest N without reading the histogram back.
 static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t nodes,
-                      int64_t chunk_terms, ChunkBufs &B, size_t *scratch_bytes, hipStream_t s) {
+                      int64_t chunk_terms, bool whole, ChunkBufs &B, size_t *scratch_bytes,
+                      hipStream_t s) {
     int rc;
     const int kord = scatrs_ctx::kOrder;
     if ((rc = check_hip(ctx, ensure(ctx, kord,
@@ -2028,18 +2032,22 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         return SCATRS_OK;
     }
     std::vector<unsigned> h(kBuckets);
-    if ((rc = check_hip(ctx, hipMemcpyAsync(h.data(), hist, kBuckets * sizeof(unsigned),
-                                            hipMemcpyDeviceToHost, s),
-                        "histogram copy")))
-        return rc;
-    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "node ranking"))) return rc;
+    const bool sphere_whole = S.a.layers == 1 && whole;
+    if (!sphere_whole) {
+        if ((rc = check_hip(ctx, hipMemcpyAsync(h.data(), hist, kBuckets * sizeof(unsigned),
+                                                hipMemcpyDeviceToHost, s),
+                            "histogram copy")))
+            return rc;
+        if ((rc = check_hip(ctx, hipStreamSynchronize(s), "node ranking"))) return rc;
+    }
     B.plan.clear();
     *scratch_bytes = 0;
     if (S.a.layers == 1) {
         int b0 = 0;
-        while (b0 < kBuckets - 1 && h[b0] == 0) ++b0;
-        // Bucket 0 holds every N above kBuckets - 2: its rows come from the largest N.
-        const int64_t n = b0 == 0 ? S.n_max : std::max(kBuckets - 2 - b0, 0);
+        while (!sphere_whole && b0 < kBuckets - 1 && h[b0] == 0) ++b0;
+        // Bucket 0 holds every N above kBuckets - 2: its rows come from the largest N (the
+        // first nonempty bucket's N is the chunk's largest, the call's for a whole call).
+        const int64_t n = (sphere_whole || b0 == 0) ? S.n_max : std::max(kBuckets - 2 - b0, 0);
         const int64_t rows = n + 2;
         if (S.forward_only && S.a.tan_dir < 0 &&
             wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, chunk_terms, n)) {
@@ -2093,7 +2101,8 @@ static int forward_nodes(scatrs_ctx *ctx, EnsSetup &S, int64_t e0, int64_t ne,
     const int kstg = scatrs_ctx::kStaging, knod = scatrs_ctx::kNodes, kacc = scatrs_ctx::kAccum;
     const size_t term_bytes = S.stage32 ? sizeof(float4) : 2 * sizeof(cx);
     size_t scratch_bytes = 0;
-    if ((rc = plan_nodes(ctx, S, e0, nodes, chunk_terms, B, &scratch_bytes, s))) return rc;
+    const bool whole = e0 == 0 && ne == (int64_t)S.terms.size();
+    if ((rc = plan_nodes(ctx, S, e0, nodes, chunk_terms, whole, B, &scratch_bytes, s))) return rc;
     if ((rc = check_hip(ctx, ensure(ctx, kstg, (size_t)std::max<int64_t>(chunk_terms, 1) * term_bytes),
                         "staging")))
         return rc;

@@ -589,30 +589,63 @@ def ensemble_fixed(kind, p0, p1, wavelength, m_real, m_imag, fractions=None, *, 
     return _FUNCTION.apply(p0, p1, p2, m_real, m_imag, fractions, kind, wavelength, opts)
 
 
-def segment_sum(index, src, n: int):
-    """(n, ...) the sums of the rows of ``src`` by ``index`` (a tensor of row numbers in
-    [0, n)), bit for bit the same on every call: the GPU's index_add adds a row's terms by
-    atomics in any order, so its sums may differ in the last bit from call to call. Here the
-    rows are sorted by index (stably), laid out one segment per row of a padded table, and
-    summed along it; every scatter writes each place once. Differentiable."""
+class SegmentPlan:
+    """The layout segment_sum sums by, for one index vector (``index``, on the device): the
+    rows' stable order by index, the sorted indices, the distinct indices, the longest segment,
+    and each sorted row's segment and place in it."""
+
+    __slots__ = ("index", "order", "ks", "uk", "width", "seg", "pos")
+
+    def __init__(self, index, order, ks, uk, width, seg, pos):
+        self.index, self.order, self.ks, self.uk = index, order, ks, uk
+        self.width, self.seg, self.pos = width, seg, pos
+
+
+def segment_plan(index, device):
+    """The SegmentPlan of ``index``: from a host array (numpy) without any device
+    synchronization (its five arrays go to ``device`` in one transfer), or from a tensor."""
     import torch
 
-    out = src.new_zeros((n,) + tuple(src.shape[1:]))
-    if src.shape[0] == 0:
-        return out
-    index = torch.as_tensor(index, device=src.device)
+    if isinstance(index, np.ndarray):
+        idx = np.ascontiguousarray(index, np.int64).reshape(-1)
+        order = np.argsort(idx, kind="stable")
+        ks = idx[order]
+        uk, counts = np.unique(ks, return_counts=True)
+        width = int(counts.max()) if counts.size else 0
+        start = np.cumsum(counts) - counts
+        seg = np.repeat(np.arange(uk.size), counts)
+        pos = np.arange(ks.size) - start[seg]
+        flat = torch.from_numpy(np.concatenate([idx, order, ks, uk, seg, pos])).to(device)
+        n, u = idx.size, uk.size
+        return SegmentPlan(flat[:n], flat[n:2 * n], flat[2 * n:3 * n], flat[3 * n:3 * n + u],
+                           width, flat[3 * n + u:4 * n + u], flat[4 * n + u:])
+    index = torch.as_tensor(index, device=device)
     order = torch.argsort(index, stable=True)
     ks = index[order]
     uk, counts = torch.unique_consecutive(ks, return_counts=True)
-    width = int(counts.max())
-    if width == 1:
-        return out.index_add(0, ks, src[order])
+    width = int(counts.max()) if counts.numel() else 0
     start = torch.cumsum(counts, 0) - counts
-    seg = torch.repeat_interleave(torch.arange(uk.numel(), device=src.device), counts)
-    pos = torch.arange(ks.numel(), device=src.device) - start[seg]
-    table = src.new_zeros((uk.numel(), width) + tuple(src.shape[1:]))
-    table = table.index_put((seg, pos), src[order])
-    return out.index_add(0, uk, table.sum(1))
+    seg = torch.repeat_interleave(torch.arange(uk.numel(), device=device), counts)
+    pos = torch.arange(ks.numel(), device=device) - start[seg]
+    return SegmentPlan(index, order, ks, uk, width, seg, pos)
+
+
+def segment_sum(index, src, n: int):
+    """(n, ...) the sums of the rows of ``src`` by ``index`` (a tensor or host array of row
+    numbers in [0, n), or its SegmentPlan), bit for bit the same on every call: the GPU's
+    index_add adds a row's terms by atomics in any order, so its sums may differ in the last bit
+    from call to call. Here the rows are sorted by index (stably), laid out one segment per row
+    of a padded table, and summed along it; every scatter writes each place once.
+    Differentiable."""
+    out = src.new_zeros((n,) + tuple(src.shape[1:]))
+    if src.shape[0] == 0:
+        return out
+    p = index if isinstance(index, SegmentPlan) else segment_plan(index, src.device)
+    if p.width == 1:
+        return out.index_add(0, p.ks, src[p.order])
+    table = src.new_zeros((p.uk.numel(), p.width) + tuple(src.shape[1:]))
+    table = table.index_put((p.seg, p.pos), src[p.order])
+    return out.index_add(0, p.uk, table.sum(1))
 
 
 def mix_parts(scalars, greek, phase, fractions, normalized: bool = True):
