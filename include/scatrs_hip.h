@@ -124,6 +124,19 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
                             int64_t n_max, int allow_unvalidated, double *d_values,
                             double *d_jacobian, int64_t *total_terms);
 
+/* scatrs_mie_efficiencies on host arrays, as _scatrs.mie_efficiencies takes them
+ * (scatrs-py/src/mie.rs:303-364: numpy x and m in, numpy values and Jacobian out): x[len],
+ * m[2*len], values (5, len), jacobian (3, 5, len) or NULL, all host memory. Up to 65536
+ * spheres the checks, each sphere's N and the largest-first ranking are done on the host (the
+ * first failing element reported as scatrs_mie_efficiencies reports it) and go to the device
+ * with the inputs in one copy through the context's pinned staging: one synchronization, no
+ * validation launches; larger batches are checked and ranked on the device. The same kernels
+ * solve and the outputs come back in one copy. Results are those of scatrs_mie_efficiencies on
+ * the same inputs. */
+int scatrs_mie_efficiencies_host(scatrs_ctx *ctx, const double *x, const double *m, int64_t len,
+                                 int64_t n_max, int allow_unvalidated, double *values,
+                                 double *jacobian, int64_t *total_terms);
+
 /* scatrs_mie_efficiencies without any device allocation: the same arguments, kernels and
  * results, with every buffer beyond the inputs and outputs (status, N histogram, keys,
  * permutation, the persistent wavefronts' slabs or, for a batch solved one sphere per

@@ -77,6 +77,9 @@ pub mod ffi {
         pub fn scatrs_mie_efficiencies_noalloc(ctx: *mut ScatrsCtx, d_x: *const f64,
             d_m: *const f64, len: i64, n_max: i64, allow_unvalidated: c_int,
             d_values: *mut f64, d_jacobian: *mut f64, total_terms: *mut i64) -> c_int;
+        pub fn scatrs_mie_efficiencies_host(ctx: *mut ScatrsCtx, x: *const f64, m: *const f64,
+            len: i64, n_max: i64, allow_unvalidated: c_int, values: *mut f64,
+            jacobian: *mut f64, total_terms: *mut i64) -> c_int;
         pub fn scatrs_mie_terms(ctx: *mut ScatrsCtx, d_x: *const f64, d_m: *const f64, len: i64,
             n_max: i64, allow_unvalidated: c_int, max_terms: *mut i64, total_terms: *mut i64)
             -> c_int;
@@ -259,6 +262,32 @@ impl Context {
         let rc = ffi::scatrs_mie_efficiencies(self.raw, d_x, d_m, len, n_max.unwrap_or(0),
                                               allow_unvalidated as c_int, d_values, d_jacobian,
                                               &mut terms);
+        self.check(rc).map(|_| terms)
+    }
+
+    /// [`Context::mie_efficiencies`] on host slices, as `_scatrs.mie_efficiencies` takes numpy
+    /// arrays: x (len), m (len complex as re, im pairs), values (5 len), jacobian (15 len) if
+    /// given. One synchronization; the checks and the ranking run on the host.
+    pub fn mie_efficiencies_host(&mut self, x: &[f64], m: &[f64], n_max: Option<i64>,
+                                 allow_unvalidated: bool, values: &mut [f64],
+                                 jacobian: Option<&mut [f64]>) -> Result<i64> {
+        let len = x.len();
+        assert!(m.len() == 2 * len && values.len() == 5 * len);
+        let jac = match jacobian {
+            Some(j) => {
+                assert!(j.len() == 15 * len);
+                j.as_mut_ptr()
+            }
+            None => std::ptr::null_mut(),
+        };
+        let mut terms = 0i64;
+        // Safety: the slices hold the sizes asserted above; the library reads and writes only
+        // those elements, before it returns.
+        let rc = unsafe {
+            ffi::scatrs_mie_efficiencies_host(self.raw, x.as_ptr(), m.as_ptr(), len as i64,
+                                              n_max.unwrap_or(0), allow_unvalidated as c_int,
+                                              values.as_mut_ptr(), jac, &mut terms)
+        };
         self.check(rc).map(|_| terms)
     }
 

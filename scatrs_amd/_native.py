@@ -40,6 +40,7 @@ SYMBOLS = (
     "scatrs_default_n_max",
     "scatrs_mie_efficiencies",
     "scatrs_mie_efficiencies_noalloc",
+    "scatrs_mie_efficiencies_host",
     "scatrs_mie_terms",
     "scatrs_mie_coefficients",
     "scatrs_layered_terms",
@@ -103,6 +104,8 @@ def load() -> C.CDLL:
                                               C.POINTER(i64)]
         L.scatrs_mie_efficiencies_noalloc.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
                                                       C.POINTER(i64)]
+        L.scatrs_mie_efficiencies_host.argtypes = [vp, vp, vp, i64, i64, i32, vp, vp,
+                                                   C.POINTER(i64)]
         L.scatrs_mie_terms.argtypes = [vp, vp, vp, i64, i64, i32, C.POINTER(i64), C.POINTER(i64)]
         L.scatrs_mie_coefficients.argtypes = [vp, vp, vp, i64, i64, i32, i64, vp, vp, vp, vp,
                                               C.POINTER(i64)]

@@ -420,8 +420,8 @@ def _node_values(tasks, ti, r, allow_unvalidated: bool):
         m = np.stack([tasks[k].m for k in own])[pos[tk]]  # (g, L)
         try:
             if L == 1:
-                v, jac, nt = mie.efficiencies_device(t(x[sel]), t(m[:, 0]), derivatives=want,
-                                                     allow_unvalidated=allow_unvalidated)
+                v, jac, nt = mie.efficiencies_host(x[sel], m[:, 0], derivatives=want,
+                                                   allow_unvalidated=allow_unvalidated)
             else:
                 fr = np.stack([tasks[k].fr for k in own])[pos[tk]]  # (g, L-1)
                 xl = np.concatenate([fr * x[sel, None], x[sel, None]], 1)
@@ -434,12 +434,14 @@ def _node_values(tasks, ti, r, allow_unvalidated: bool):
             i = sel[int(hit[1])]
             raise type(e)(f"at wavelength {wl[i]:e}, radius {r[i]:e}: {hit[2]}") from None
         terms += nt
-        v = v.cpu().numpy()  # q_ext, q_sca, q_abs, q_back, g
+        if L != 1:
+            v = v.cpu().numpy()  # q_ext, q_sca, q_abs, q_back, g
+            jac = None if jac is None else jac.cpu().numpy()
         a = area[sel]
         Q[sel] = (a * np.stack([v[0], v[1], v[1] * v[4], v[2], v[3]])).T
         if not want:
             continue
-        j = jac.cpu().numpy()
+        j = jac
         # every direction's Jacobian of (q_ext, q_sca, q_abs, q_back, g): (g, 2 L + L - 1, 5)
         if L == 1:
             dirs_all = np.stack([j[1], j[2]])

@@ -39,11 +39,14 @@ struct scatrs_ctx {
         kTablesMu,    // ... the w pi_n, w tau_n table at |mu| (and its fp16 slices)
         kAccumMu,     // ... the summed tables of a chunk, and each ensemble's phase factor
         kGeom,        // ensembles: every node's radius and quadrature weight (ens_prep)
+        kHostIO,      // host-array sphere batch: its inputs and ranking, then its outputs
         kNumBufs
     };
     void *buf[kNumBufs] = {};
     size_t cap[kNumBufs] = {};
     void *host_pinned = nullptr;  // small pinned status block
+    void *host_io = nullptr;      // pinned staging of the host-array sphere batch (grow-only)
+    size_t host_io_cap = 0;
     // Ensemble setup kept between calls: the Gauss-Legendre rule in radius (device copy in
     // kMisc) of the last size, and the angle rules of the last sizes, each with its device
     // table (weights, then the non-negative cosines) and its Wigner d table for one order.

@@ -373,6 +373,7 @@ int scatrs_destroy(scatrs_ctx *ctx) {
         if (r.d_dtab) (void)hipFree(r.d_dtab);
     }
     if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+    if (ctx->host_io) (void)hipHostFree(ctx->host_io);
     for (int i = 0; i < scatrs_ctx::kMaxEvents; ++i) {
         if (ctx->ev_begin[i]) (void)hipEventDestroy(ctx->ev_begin[i]);
         if (ctx->ev_end[i]) (void)hipEventDestroy(ctx->ev_end[i]);
@@ -563,6 +564,130 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
         return rc;
     w.slab = ctx->buf[scatrs_ctx::kScratch];
     return mie_solve(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jacobian);
+}
+
+int scatrs_mie_efficiencies_host(scatrs_ctx *ctx, const double *x, const double *m, int64_t len,
+                                 int64_t n_max, int allow_unvalidated, double *values,
+                                 double *jacobian, int64_t *total_terms) {
+    int rc = mie_begin(ctx, x, m, len, values, total_terms);
+    if (rc != SCATRS_OK || len == 0) return rc;
+    // Up to kHostChecks spheres: the checks of every sphere, its N and the largest-first
+    // ranking on the host (the first failing element reported as the device path reports it);
+    // larger batches are checked and ranked on the device, as scatrs_mie_efficiencies does.
+    constexpr int64_t kHostChecks = 65536;
+    const bool host_checks = len <= kHostChecks;
+    std::vector<int> nn(host_checks ? (size_t)len : 0);
+    std::vector<unsigned> count(kBuckets + 1, 0u);
+    int64_t nmax = 0, terms = 0;
+    for (int64_t i = 0; host_checks && i < len; ++i) {
+        long long n = 0;
+        const int code = sphere_code(x[i], m[2 * i], m[2 * i + 1], n_max, allow_unvalidated, &n);
+        if (code != kValid) {
+            std::string msg;
+            const int c = sphere_message(x[i], m[2 * i], m[2 * i + 1], n_max, allow_unvalidated,
+                                         &msg);
+            return fail(ctx, c == kOutside ? SCATRS_OUTSIDE_VALIDATED_DOMAIN : SCATRS_INVALID_INPUT,
+                        "element " + std::to_string(i) + ": " + msg);
+        }
+        nn[(size_t)i] = (int)n;
+        nmax = std::max<int64_t>(nmax, n);
+        terms += n;
+        ++count[(size_t)bucket_of((int)n) + 1];
+    }
+    for (int b = 0; b < kBuckets; ++b) count[(size_t)b + 1] += count[(size_t)b];
+    // Pinned staging: x, m (interleaved), then the ranking; the outputs come back through it.
+    const bool derivs = jacobian != nullptr;
+    const size_t in_bytes = (size_t)len * 3 * sizeof(double) + (size_t)len * sizeof(int);
+    const size_t out_bytes = (size_t)len * (derivs ? 20 : 5) * sizeof(double);
+    const size_t io_bytes = std::max(in_bytes, out_bytes);
+    if (ctx->host_io_cap < io_bytes) {
+        if (ctx->host_io) {
+            if ((rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "host staging")))
+                return rc;
+            (void)hipHostFree(ctx->host_io);
+            ctx->host_io = nullptr;
+            ctx->host_io_cap = 0;
+        }
+        const size_t want = std::max(io_bytes, ctx->host_io_cap + ctx->host_io_cap / 4);
+        if ((rc = check_hip(ctx, hipHostMalloc(&ctx->host_io, want, hipHostMallocDefault),
+                            "host staging")))
+            return rc;
+        ctx->host_io_cap = want;
+    }
+    char *hio = static_cast<char *>(ctx->host_io);
+    std::memcpy(hio, x, (size_t)len * sizeof(double));
+    std::memcpy(hio + (size_t)len * sizeof(double), m, (size_t)len * 2 * sizeof(double));
+    int *hperm = reinterpret_cast<int *>(hio + (size_t)len * 3 * sizeof(double));
+    if (host_checks)
+        for (int64_t i = 0; i < len; ++i) hperm[count[(size_t)bucket_of(nn[(size_t)i])]++] = (int)i;
+    // Device: the inputs and ranking, then the outputs.
+    const size_t out_off = (in_bytes + 255) & ~size_t(255);
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHostIO, out_off + out_bytes), "alloc")))
+        return rc;
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kCounters, kStatusBytes), "alloc")))
+        return rc;
+    char *dio = static_cast<char *>(ctx->buf[scatrs_ctx::kHostIO]);
+    const double *d_x = reinterpret_cast<const double *>(dio);
+    const double *d_m = d_x + len;
+    double *d_values = reinterpret_cast<double *>(dio + out_off);
+    double *d_jac = derivs ? d_values + 5 * len : nullptr;
+    hipStream_t s = ctx->stream;
+    if ((rc = check_hip(ctx, hipMemcpyAsync(dio, hio, host_checks ? in_bytes : 3 * (size_t)len *
+                                                                             sizeof(double),
+                                            hipMemcpyHostToDevice, s),
+                        "input copy")))
+        return rc;
+    MieWork w;
+    w.status = ctx->buf[scatrs_ctx::kCounters];
+    w.perm = reinterpret_cast<int *>(dio + (size_t)len * 3 * sizeof(double));
+    if (host_checks) {
+        unsigned long long *counter = reinterpret_cast<unsigned long long *>(
+            static_cast<char *>(w.status) + sizeof(Status));
+        fill_u64<<<1, 1, 0, s>>>(counter, 0ull, 1);
+    } else {
+        const int64_t blocks = (len + 255) / 256;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kHist, kHistBytes), "alloc"))) return rc;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kKeys, (size_t)len * sizeof(int)), "alloc")))
+            return rc;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kNodes, (size_t)blocks * sizeof(Partial)),
+                            "alloc")))
+            return rc;
+        w.hist = static_cast<unsigned *>(ctx->buf[scatrs_ctx::kHist]);
+        w.keys = static_cast<int *>(ctx->buf[scatrs_ctx::kKeys]);
+        w.partial = static_cast<Partial *>(ctx->buf[scatrs_ctx::kNodes]);
+        if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, &terms)))
+            return rc;
+    }
+    if (use_wave(ctx, derivs, terms, nmax)) {
+        const int64_t waves = std::min<int64_t>(len, (int64_t)ctx->cus * 8);
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch,
+                                        (size_t)(mie_wave_bytes(nmax) * waves)),
+                            "scratch")))
+            return rc;
+        w.slab = ctx->buf[scatrs_ctx::kScratch];
+        rc = mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values);
+    } else {
+        const size_t per_wave = mie_slab_per_wave(nmax);
+        int64_t waves = std::min<int64_t>((int64_t)ctx->cus * ctx->tune[SCATRS_TUNE_SPHERE_WAVES],
+                                          (len + 63) / 64);
+        waves = std::min<int64_t>(waves, std::max<int64_t>(4, (int64_t)((8ull << 30) / per_wave)));
+        waves = (waves + 3) / 4 * 4;
+        if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch, per_wave * (size_t)waves),
+                            "scratch")))
+            return rc;
+        w.slab = ctx->buf[scatrs_ctx::kScratch];
+        rc = mie_solve(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jac);
+    }
+    if (rc) return rc;
+    if ((rc = check_hip(ctx, hipMemcpyAsync(hio, d_values, out_bytes, hipMemcpyDeviceToHost, s),
+                        "output copy")))
+        return rc;
+    if ((rc = check_hip(ctx, hipStreamSynchronize(s), "sphere batch"))) return rc;
+    std::memcpy(values, hio, (size_t)len * 5 * sizeof(double));
+    if (derivs) std::memcpy(jacobian, hio + (size_t)len * 5 * sizeof(double),
+                            (size_t)len * 15 * sizeof(double));
+    if (total_terms) *total_terms = terms;
+    return SCATRS_OK;
 }
 
 int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const double *d_m,

@@ -19,7 +19,7 @@ import numpy as np
 from . import _native
 
 __all__ = ["MieCoefficients", "MieEfficiencies", "coefficients", "coefficients_device",
-           "default_n_max", "efficiencies", "efficiencies_device"]
+           "default_n_max", "efficiencies", "efficiencies_device", "efficiencies_host"]
 
 _PARAMS = ("x", "m_real", "m_imag")
 
@@ -114,23 +114,42 @@ def efficiencies_device(x, m, *, n_max: int | None = None, derivatives: bool = F
     return values, jac, terms.value
 
 
+def efficiencies_host(x, m, *, n_max: int | None = None, derivatives: bool = False,
+                      allow_unvalidated: bool = False):
+    """Host-array form (scatrs_mie_efficiencies_host): ``x`` float64 (len,), ``m`` complex128
+    (len,) numpy arrays; returns (values (5, len), jacobian (3, 5, len) or None, total terms)
+    as numpy arrays, with one host synchronization and no validation launches."""
+    import torch
+
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    m = np.ascontiguousarray(m, np.complex128).reshape(-1)
+    if m.size != x.size:
+        raise ValueError("x and m must have the same number of elements")
+    n = x.size
+    ctx = _native.context(torch.cuda.current_device())
+    values = np.empty((5, n))
+    jac = np.empty((3, 5, n)) if derivatives else None
+    terms = C.c_int64(0)
+    rc = ctx.lib.scatrs_mie_efficiencies_host(
+        ctx.handle, C.c_void_p(x.ctypes.data), C.c_void_p(m.ctypes.data), n,
+        0 if n_max is None else int(n_max), int(bool(allow_unvalidated)),
+        C.c_void_p(values.ctypes.data), C.c_void_p(jac.ctypes.data) if jac is not None else None,
+        C.byref(terms))
+    ctx.check(rc)
+    return values, jac, terms.value
+
+
 def efficiencies(x, m, *, n_max: int | None = None, derivatives: bool | Sequence[str] = False,
                  allow_unvalidated: bool = False) -> MieEfficiencies:
     """Efficiencies and asymmetry parameter of spheres (host arrays in, host arrays out)."""
-    import torch
-
     params = _parameters(derivatives)
     xb, mb = np.broadcast_arrays(np.asarray(x, dtype=np.float64),
                                  np.asarray(m, dtype=np.complex128))
     shape = xb.shape
-    dev = torch.device("cuda", torch.cuda.current_device())
-    xt = torch.from_numpy(np.ascontiguousarray(xb).reshape(-1)).to(dev)
-    mt = torch.from_numpy(np.ascontiguousarray(mb).reshape(-1)).to(dev)
-    values, jac, _ = efficiencies_device(xt, mt, n_max=n_max, derivatives=bool(params),
-                                         allow_unvalidated=allow_unvalidated)
-    v = values.cpu().numpy()
+    v, jac, _ = efficiencies_host(xb, mb, n_max=n_max, derivatives=bool(params),
+                                  allow_unvalidated=allow_unvalidated)
     fields = [v[k].reshape(shape) for k in range(5)]
-    j = None if jac is None else jac.cpu().numpy().reshape((3, 5, *shape))
+    j = None if jac is None else jac.reshape((3, 5, *shape))
     return MieEfficiencies(*fields, params=params, _jacobian=j)
 
 

@@ -178,6 +178,30 @@ def test_wavefront_per_sphere_path():
     assert np.allclose(one[[0, 1, 4]], r1[[0, 1, 4]], rtol=1e-11, atol=0)
 
 
+@pytest.mark.parametrize("n,derivatives", [(3000, False), (3000, True), (300_000, False)])
+def test_host_array_entry_equals_the_device_entry(n, derivatives):
+    """scatrs_mie_efficiencies_host (checks, N and ranking on the host, one copy each way)
+    gives scatrs_mie_efficiencies' results bit for bit: a small batch (one sphere per
+    wavefront), its Jacobian (one per lane) and a large batch (one per lane)."""
+    from scatrs_amd.mie import efficiencies_device, efficiencies_host
+
+    x, m = _seeded(n, seed=17)
+    v, j, t = efficiencies_host(x, m, derivatives=derivatives)
+    dev = torch.device("cuda", 0)
+    vd, jd, td = efficiencies_device(torch.from_numpy(x).to(dev), torch.from_numpy(m).to(dev),
+                                     derivatives=derivatives)
+    assert t == td > 0
+    assert np.array_equal(v, vd.cpu().numpy())
+    if derivatives:
+        assert np.array_equal(j, jd.cpu().numpy())
+    from scatrs_amd import InvalidInputError
+
+    xb = x[:10].copy()
+    xb[7] = -1.0
+    with pytest.raises(InvalidInputError, match="element 7: the size parameter"):
+        efficiencies_host(xb, m[:10])
+
+
 def test_real_index_gives_zero_absorption():
     x = np.array([0.5, 3.0, 40.0, 700.0])
     m = np.array([1.33, 1.5, 1.7, 1.45], dtype=np.complex128)
