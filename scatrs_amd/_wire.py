@@ -611,9 +611,10 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                           for idx in (np.arange(15), adaptive._GAUSS)]
             elif shared and not keys:  # every shared node solved once; both rules from it
                 passes = sh.pass2_groups(tasks, **kw)
-            else:  # both rules in one call per angle rule: the tasks twice, Gauss weights second
+            else:  # both rules in one call per angle rule: the tasks twice (the same objects:
+                # pass 2 only reads them), Gauss weights second
                 passes = _split_rules(adaptive.pass2_torch(
-                    tasks + [q.shallow() for q in tasks], params + params, R=R,
+                    tasks + tasks, params + params, R=R,
                     idx=np.arange(15), gauss_tasks=np.repeat([False, True], len(tasks)),
                     derivatives=fwd, **kw), len(tasks), R)
             for avg, gr, ph, _, _, rw in passes:

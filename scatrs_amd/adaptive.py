@@ -230,14 +230,22 @@ class Task:
     def n_panels(self) -> int:
         return max(self.a.size, 1)
 
-    def shallow(self) -> "Task":
-        """A shallow copy (its arrays shared), cheaper than copy.copy on a dataclass."""
-        c = object.__new__(Task)
-        c.__dict__.update(self.__dict__)
-        return c
+
+class _TaskTable:
+    """The per-task attributes the rounds of pass 1 read, as arrays indexed by task (built once
+    per pass, after the ranges and scales are set): kind, p (T, 3), c, wavelength, layers,
+    the number of refined directions."""
+
+    def __init__(self, tasks):
+        self.kind = np.array([q.kind for q in tasks], np.int32)
+        self.p = np.array([q.p for q in tasks], np.float64).reshape(-1, 3)
+        self.c = np.array([q.c for q in tasks], np.float64)
+        self.wl = np.array([q.wl for q in tasks], np.float64)
+        self.layers = np.array([q.m.size for q in tasks], np.int64)
+        self.n_dirs = np.array([len(q.dirs) for q in tasks], np.int64)
 
 
-def _nodes(tasks, ti, a, b, idx, derivatives: bool = False):
+def _nodes(tasks, ti, a, b, idx, derivatives: bool = False, table: _TaskTable | None = None):
     """(r, W) (len(a), len(idx)) of the panels [a, b] (each of task ti) at the nodes idx of the
     rule, with the Kronrod weights at all 15 nodes or the Gauss weights at the 7; a panel of
     width 0 (padding) has weights 0. One density call for all panels. With ``derivatives``
@@ -248,9 +256,10 @@ def _nodes(tasks, ti, a, b, idx, derivatives: bool = False):
     t = mid[:, None] + half[:, None] * x[None, :]
     r = np.exp(t)
     ti = np.asarray(ti, np.int64)
-    kind = np.array([q.kind for q in tasks], np.int32)[ti]
-    p = np.array([q.p for q in tasks]).reshape(-1, 3)[ti]
-    c = np.array([q.c for q in tasks])[ti]
+    tab = table if table is not None else _TaskTable(tasks)
+    kind = tab.kind[ti]
+    p = tab.p[ti]
+    c = tab.c[ti]
     k = idx.size
     out = log_density(np.repeat(kind, k), np.repeat(p, k, axis=0), r.reshape(-1), derivatives)
     lu = out[0] if derivatives else out
@@ -384,7 +393,7 @@ MIN_WIDTH = 1e-10  # panels narrower than this (in ln r) are not split (engine.r
 _ELEMENT = re.compile(r"element (\d+): (.*)", re.S)
 
 
-def _node_values(tasks, ti, r, allow_unvalidated: bool):
+def _node_values(tasks, ti, r, allow_unvalidated: bool, table: _TaskTable | None = None):
     """The pass-1 quantities of single particles (engine.rs:379-465): at each node (radius r[i]
     of task ti[i]'s particle at its wavelength) the cross sections [C_ext, C_sca, C_sca g,
     C_abs, C_back] (n, 5), pi r^2 times the efficiencies, and their derivatives along each
@@ -401,20 +410,21 @@ def _node_values(tasks, ti, r, allow_unvalidated: bool):
     dev = torch.device("cuda", torch.cuda.current_device())
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
     ti = np.asarray(ti, np.int64)
+    tab = table if table is not None else _TaskTable(tasks)
     n = r.size
-    D = max((len(tasks[k].dirs) for k in np.unique(ti)), default=0)
+    D = int(tab.n_dirs[ti].max()) if ti.size else 0
     Q = np.empty((n, _Q))
     dQ = np.zeros((n, D, _Q)) if D else None
-    wl = np.array([q.wl for q in tasks])[ti]
+    wl = tab.wl[ti]
     x = 2.0 * np.pi * r / wl
     area = np.pi * r * r
-    layers = np.array([q.layers for q in tasks])[ti]
+    layers = tab.layers[ti]
     terms = 0
     for L in np.unique(layers):
         sel = np.flatnonzero(layers == L)
         tk = ti[sel]
         own = np.unique(tk)
-        want = any(len(tasks[k].dirs) for k in own)
+        want = bool(tab.n_dirs[own].any())
         pos = np.full(len(tasks), -1)
         pos[own] = np.arange(own.size)
         m = np.stack([tasks[k].m for k in own])[pos[tk]]  # (g, L)
@@ -508,6 +518,7 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
     absolute integrands, are formed on the host (engine.rs:482-566)."""
     terms = 0
     _prepare(tasks, rtol)
+    table = _TaskTable(tasks)
     live = np.array([q.kind != MONODISPERSE for q in tasks])
     for q in tasks:
         if q.kind == MONODISPERSE:
@@ -542,10 +553,11 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
         if pending.size:
             ti = task[pending]
             n = pending.size
-            out = _nodes(tasks, ti, a[pending], b[pending], np.arange(15), derivatives=H > 0)
+            out = _nodes(tasks, ti, a[pending], b[pending], np.arange(15), derivatives=H > 0,
+                         table=table)
             r, wk = out[0], out[1]
             q, dq, nt = _node_values(tasks, np.repeat(ti, 15), r.reshape(-1),
-                                     allow_unvalidated)
+                                     allow_unvalidated, table=table)
             terms += nt
             q = q.reshape(n, 15, _Q)
             f = [q]
@@ -568,8 +580,9 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
         # Per task: totals, error sums and targets (engine.rs:620-716).
         E = np.abs(K - G)
         ta = task[act]
-        seg = lambda v: np.stack([np.bincount(ta, v[act, j], minlength=T)  # noqa: E731
-                                  for j in range(nq)], 1)
+        cols = (ta[:, None] * nq + np.arange(nq)[None, :]).reshape(-1)
+        seg = lambda v: np.bincount(cols, v[act].reshape(-1),  # noqa: E731
+                                    minlength=T * nq).reshape(T, nq)
         tot, esum, asum = seg(K), seg(E), seg(A)
         tgt, end_tgt = _targets(tot, asum, rtol, D, H, n_dir, n_theta, refine_backscatter)
         fail = esum > tgt

@@ -106,7 +106,7 @@ def _stub_nodes(q_of_r, dq_of_r=None):
     quantities q_of_r(r) -> (C_ext, C_sca, C_abs, C_back, g), and with dq_of_r(r) -> the same
     five along one direction of the particle, their derivatives."""
 
-    def call(tasks, ti, r, allow_unvalidated):
+    def call(tasks, ti, r, allow_unvalidated, table=None):
         ext, sca, absb, back, g = q_of_r(r)
         Q = np.stack([ext, sca, sca * g, absb, back], 1)
         D = max((len(tasks[k].dirs) for k in np.unique(ti)), default=0)
