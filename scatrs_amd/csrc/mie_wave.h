@@ -171,14 +171,16 @@ struct StageF32 {
     }
 };
 
-// The efficiency sink of one lane's orders: also keeps the first order's coefficients, for the
-// pair term across the boundary with the previous lane's last order, stages each order's
-// coefficients, and keeps their largest |Re|, |Im|.
-template <class Stage>
-struct WaveEfficiencySink : EfficiencySink<false> {
+// The efficiency sink of one lane's orders: also keeps the first order's coefficients (and
+// with DERIVS their derivatives), for the pair terms across the boundary with the previous
+// lane's last order, stages each order's coefficients, and keeps their largest |Re|, |Im|.
+template <bool DERIVS, class Stage>
+struct WaveEfficiencySink : EfficiencySink<DERIVS> {
+    using Base = EfficiencySink<DERIVS>;
     Stage stage;
     int k_first = 0;
     cx a_first = {0.0, 0.0}, b_first = {0.0, 0.0};
+    cx da_first[2] = {{0.0, 0.0}, {0.0, 0.0}}, db_first[2] = {{0.0, 0.0}, {0.0, 0.0}};
     double amax = 0.0;
     __device__ void order(int k, cx a, cx b) {
         if (k == k_first) {
@@ -187,7 +189,19 @@ struct WaveEfficiencySink : EfficiencySink<false> {
         }
         stage(k, a, b);
         amax = fmax(amax, fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im))));
-        EfficiencySink<false>::order(k, a, b);
+        Base::order(k, a, b);
+    }
+    // d/dx and d/dm of a, b (d/d Im m = i d/dm).
+    __device__ void order_d(int k, cx a, cx b, cx da_dx, cx da_dm, cx db_dx, cx db_dm) {
+        if (k == k_first) {
+            a_first = a;
+            b_first = b;
+            da_first[0] = da_dx;
+            da_first[1] = da_dm;
+            db_first[0] = db_dx;
+            db_first[1] = db_dm;
+        }
+        Base::order_d(k, a, b, da_dx, da_dm, db_dx, db_dm);
     }
 };
 
@@ -204,16 +218,18 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // One sphere's efficiencies on the whole wavefront (every lane calls it with the same sphere;
-// lane 0's q holds the result). uz (n + 2 complex) and ux (n + 2 doubles) are the wavefront's
-// columns, the u_k(m x) and u_k(x) chains indexed by k. Each order's coefficients also go to
-// `stage`; *amax receives the largest |Re|, |Im| of them and *back the backscatter sum
+// lane 0's q holds the result), with DERIVS also their Jacobian jac15 (sphere_efficiencies'
+// layout). uz (n + 2 complex) and ux (n + 2 doubles) are the wavefront's columns, the u_k(m x)
+// and u_k(x) chains indexed by k. Each order's coefficients also go to `stage` (values only);
+// *amax receives the largest |Re|, |Im| of them and *back the backscatter sum
 // sum (2n+1) (-1)^n (a_n - b_n) (every lane).
-template <class Stage = NoStage>
+template <bool DERIVS = false, class Stage = NoStage>
 __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, cx *uz,
                                                          double *ux, int lane, double q[5],
                                                          Stage stage = Stage{},
                                                          double *amax = nullptr,
-                                                         cx *back = nullptr) {
+                                                         cx *back = nullptr,
+                                                         double *jac15 = nullptr) {
     const int n_s = upward_orders(x, n);
     // u_k(x) above the upward orders (x_chain's range), kept in ux.
     if (n_s < n) {
@@ -310,26 +326,44 @@ __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, 
     st.inv_xi = ixi0 * ex;
     st.r = kr0 == k0 ? r_in : r0;
 
-    WaveEfficiencySink<Stage> s;
+    WaveEfficiencySink<DERIVS, Stage> s;
     s.stage = stage;
     s.k_first = k0;
     s.rk = k0 > 1 ? recip((double)k0) : 1.0;
     NoOuter none;
-    if (k0 <= k1) mie_orders<false>(x, m, n, ZColumn{uz, 1}, ux, 1, none, s, &st);
-    // The pair term across the boundary with the next lane's first order k1 + 1.
-    const cx an = shfl_v(s.a_first, (lane + 1) & 63), bn = shfl_v(s.b_first, (lane + 1) & 63);
+    if (k0 <= k1) mie_orders<DERIVS>(x, m, n, ZColumn{uz, 1}, ux, 1, none, s, &st);
+    // The pair terms across the boundary with the next lane's first order k1 + 1.
+    const int next = (lane + 1) & 63;
+    const cx an = shfl_v(s.a_first, next), bn = shfl_v(s.b_first, next);
+    const bool joined = k0 <= k1 && k1 < n;
+    const double kf = (double)k1;
+    const double pair = joined ? (kf * (kf + 2.0)) * recip(kf + 1.0) : 0.0;
     double g = s.g;
-    if (k0 <= k1 && k1 < n) {
-        const double kf = (double)k1;
-        const double pair = (kf * (kf + 2.0)) * recip(kf + 1.0);
-        g += pair * (re_mul_conj(s.a_prev, an) + re_mul_conj(s.b_prev, bn));
-    }
-    EfficiencySink<false> tot;
+    if (joined) g += pair * (re_mul_conj(s.a_prev, an) + re_mul_conj(s.b_prev, bn));
+    EfficiencySink<DERIVS> tot;
     tot.sca = wave_sum(s.sca);
     tot.g = wave_sum(g);
     tot.abs_sum = wave_sum(s.abs_sum);
     tot.back = cx{wave_sum(s.back.re), wave_sum(s.back.im)};
     tot.values(x, q);
+    if constexpr (DERIVS) {
+        const cx dan[3] = {shfl_v(s.da_first[0], next), shfl_v(s.da_first[1], next),
+                           times_i(shfl_v(s.da_first[1], next))};
+        const cx dbn[3] = {shfl_v(s.db_first[0], next), shfl_v(s.db_first[1], next),
+                           times_i(shfl_v(s.db_first[1], next))};
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            double dg = s.ds.d_g[p];
+            if (joined)
+                dg += pair * (re_mul_conj(s.ds.da_prev[p], an) + re_mul_conj(s.a_prev, dan[p]) +
+                              re_mul_conj(s.ds.db_prev[p], bn) + re_mul_conj(s.b_prev, dbn[p]));
+            tot.ds.d_g[p] = wave_sum(dg);
+            tot.ds.d_sca[p] = wave_sum(s.ds.d_sca[p]);
+            tot.ds.jac_abs[p] = wave_sum(s.ds.jac_abs[p]);
+            tot.ds.d_back[p] = cx{wave_sum(s.ds.d_back[p].re), wave_sum(s.ds.d_back[p].im)};
+        }
+        tot.jacobian(x, q, true, jac15);
+    }
     if (amax) *amax = wave_max(s.amax);
     if (back) *back = tot.back;
 }

@@ -140,13 +140,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 
 // The solve of a small batch, one sphere per wavefront (blocks of one wavefront), largest N
 // first from the shared counter: the wavefront's columns (wave_bytes each from cols) hold its
-// sphere's u_k(m x) (rows complex) and u_k(x) (rows doubles).
+// sphere's u_k(m x) (rows complex) and u_k(x) (rows doubles). DERIVS: the Jacobian too.
+template <bool DERIVS>
 __global__ __launch_bounds__(64) void mie_wave(const double *__restrict__ x,
                                                const double *__restrict__ m, int64_t len,
                                                int64_t n_fixed, const int *__restrict__ perm,
                                                unsigned long long *counter, char *cols,
                                                int64_t rows, int64_t wave_bytes,
-                                               double *__restrict__ values) {
+                                               double *__restrict__ values,
+                                               double *__restrict__ jac) {
     const int lane = threadIdx.x;
     char *base = cols + (int64_t)blockIdx.x * wave_bytes;
     cx *uz = reinterpret_cast<cx *>(base);
@@ -160,20 +162,25 @@ __global__ __launch_bounds__(64) void mie_wave(const double *__restrict__ x,
         const double xi = x[i];
         const cx mi = mk(m[2 * (int64_t)i], m[2 * (int64_t)i + 1]);
         const int n = (int)(n_fixed > 0 ? n_fixed : default_n_max(xi));
-        double q[5];
-        sphere_efficiencies_wave(xi, mi, n, uz, ux, lane, q);
+        double q[5], j[15];
+        sphere_efficiencies_wave<DERIVS>(xi, mi, n, uz, ux, lane, q, NoStage{}, nullptr, nullptr,
+                                         j);
         if (lane == 0) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) values[k * len + i] = q[k];
+            if constexpr (DERIVS) {
+#pragma unroll
+                for (int k = 0; k < 15; ++k) jac[k * len + i] = j[k];
+            }
         }
         // The columns are rewritten by the next sphere only after every lane has read them.
         __syncthreads();
     }
 }
 
-// Whether a batch takes mie_wave: values only, and the rule of wave_rule.
-bool use_wave(const scatrs_ctx *ctx, bool derivatives, int64_t terms, int64_t nmax) {
-    return !derivatives && wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, terms, nmax);
+// Whether a batch takes mie_wave (with or without the Jacobian): the rule of wave_rule.
+bool use_wave(const scatrs_ctx *ctx, int64_t terms, int64_t nmax) {
+    return wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, terms, nmax);
 }
 
 // ---- the batched-sphere path, shared by the allocating and the no-allocation entry points ----
@@ -287,15 +294,47 @@ int mie_solve(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double
 // The wavefront-per-sphere solve over the N-ranked permutation on `waves` wavefronts whose
 // columns (mie_wave_bytes(nmax) each) start at w.slab.
 int mie_solve_wave(scatrs_ctx *ctx, const MieWork &w, const double *d_x, const double *d_m,
-                   int64_t len, int64_t n_max, int64_t nmax, int64_t waves, double *d_values) {
+                   int64_t len, int64_t n_max, int64_t nmax, int64_t waves, double *d_values,
+                   double *d_jacobian) {
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(
         static_cast<char *>(w.status) + sizeof(Status));
+    char *cols = static_cast<char *>(w.slab);
     mark_begin(ctx);
-    mie_wave<<<(unsigned)waves, 64, 0, ctx->stream>>>(d_x, d_m, len, n_max, w.perm, counter,
-                                                      static_cast<char *>(w.slab), nmax + 2,
-                                                      mie_wave_bytes(nmax), d_values);
+    if (d_jacobian)
+        mie_wave<true><<<(unsigned)waves, 64, 0, ctx->stream>>>(
+            d_x, d_m, len, n_max, w.perm, counter, cols, nmax + 2, mie_wave_bytes(nmax), d_values,
+            d_jacobian);
+    else
+        mie_wave<false><<<(unsigned)waves, 64, 0, ctx->stream>>>(
+            d_x, d_m, len, n_max, w.perm, counter, cols, nmax + 2, mie_wave_bytes(nmax), d_values,
+            nullptr);
     mark_end(ctx);
     return check_hip(ctx, hipGetLastError(), "mie_wave launch");
+}
+
+// The wavefront-per-sphere solve of another entry point's checked and ranked batch (the layered
+// entry points' one-layer case, which gives the sphere batch's results bit for bit): *taken says
+// whether the sphere batch's rule takes it, and then the solve is queued. The batch's status
+// block (kCounters) holds the work counter, perm its largest-first ranking.
+int mie_wave_for(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+                 int64_t n_fixed, const int *perm, int64_t terms, int64_t nmax, double *d_values,
+                 double *d_jacobian, bool *taken) {
+    *taken = use_wave(ctx, terms, nmax);
+    if (!*taken) return SCATRS_OK;
+    int rc;
+    const int64_t waves = std::min<int64_t>(len, (int64_t)ctx->cus * 8);
+    if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch,
+                                    (size_t)(mie_wave_bytes(nmax) * waves)),
+                        "scratch")))
+        return rc;
+    MieWork w;
+    w.status = ctx->buf[scatrs_ctx::kCounters];
+    w.perm = const_cast<int *>(perm);
+    w.slab = ctx->buf[scatrs_ctx::kScratch];
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(
+        static_cast<char *>(w.status) + sizeof(Status));
+    fill_u64<<<1, 1, 0, ctx->stream>>>(counter, 0ull, 1);
+    return mie_solve_wave(ctx, w, d_x, d_m, len, n_fixed, nmax, waves, d_values, d_jacobian);
 }
 
 // ---- the no-allocation scratch ----
@@ -543,14 +582,14 @@ int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_
     if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, &terms)))
         return rc;
     if (total_terms) *total_terms = terms;
-    if (use_wave(ctx, d_jacobian != nullptr, terms, nmax)) {
+    if (use_wave(ctx, terms, nmax)) {
         const int64_t waves = std::min<int64_t>(len, (int64_t)ctx->cus * 8);
         if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch,
                                         (size_t)(mie_wave_bytes(nmax) * waves)),
                             "scratch")))
             return rc;
         w.slab = ctx->buf[scatrs_ctx::kScratch];
-        return mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values);
+        return mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jacobian);
     }
     // Persistent wavefronts: the tuned number per CU (default 8: two per SIMD, which the
     // kernel's registers allow; 4-wave blocks share the CU's LDS tiles of kSphereBlock orders),
@@ -658,14 +697,14 @@ int scatrs_mie_efficiencies_host(scatrs_ctx *ctx, const double *x, const double 
         if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, &terms)))
             return rc;
     }
-    if (use_wave(ctx, derivs, terms, nmax)) {
+    if (use_wave(ctx, terms, nmax)) {
         const int64_t waves = std::min<int64_t>(len, (int64_t)ctx->cus * 8);
         if ((rc = check_hip(ctx, ensure(ctx, scatrs_ctx::kScratch,
                                         (size_t)(mie_wave_bytes(nmax) * waves)),
                             "scratch")))
             return rc;
         w.slab = ctx->buf[scatrs_ctx::kScratch];
-        rc = mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values);
+        rc = mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jac);
     } else {
         const size_t per_wave = mie_slab_per_wave(nmax);
         int64_t waves = std::min<int64_t>((int64_t)ctx->cus * ctx->tune[SCATRS_TUNE_SPHERE_WAVES],
@@ -726,7 +765,7 @@ int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const do
     if ((rc = mie_validate(ctx, w, d_x, d_m, len, n_max, allow_unvalidated, &nmax, &terms)))
         return rc;
     if (total_terms) *total_terms = terms;
-    if (use_wave(ctx, d_jacobian != nullptr, terms, nmax)) {
+    if (use_wave(ctx, terms, nmax)) {
         // As many wavefronts as the scratch holds columns for, up to two per SIMD.
         const int64_t wb = mie_wave_bytes(nmax);
         const int64_t waves = std::min<int64_t>({len, (int64_t)ctx->cus * 8,
@@ -738,7 +777,7 @@ int scatrs_mie_efficiencies_noalloc(scatrs_ctx *ctx, const double *d_x, const do
                             " terms need more column memory than the no-allocation scratch "
                             "holds (" + std::to_string(kScratchBytes >> 20) + " MiB)");
         }
-        return mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values);
+        return mie_solve_wave(ctx, w, d_x, d_m, len, n_max, nmax, waves, d_values, d_jacobian);
     }
     // As many persistent wavefronts as the scratch holds slabs for, up to the tuned number per
     // CU, in whole blocks of 4.

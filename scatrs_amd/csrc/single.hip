@@ -19,6 +19,12 @@
 #include "util.h"
 
 namespace scatrs {
+
+// The sphere batch's wavefront-per-sphere solve of a checked, ranked batch (scatrs_hip.hip).
+int mie_wave_for(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
+                 int64_t n_fixed, const int *perm, int64_t terms, int64_t nmax, double *d_values,
+                 double *d_jacobian, bool *taken);
+
 namespace {
 
 constexpr int kModeEfficiencies = 0;
@@ -391,6 +397,15 @@ int single_run(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t co
         return fail(ctx, SCATRS_INVALID_INPUT, "null output pointer");
     }
     sort_by_keys(keys, hist, perm, count, s);
+    if (L == 1 && mode != kModeCoefficients) {
+        // One layer takes the sphere batch's kernel choice, so that it gives the sphere bit for
+        // bit: a small batch one sphere per wavefront.
+        bool taken = false;
+        if ((rc = mie_wave_for(ctx, d_x, d_m, count, n_max, perm, (int64_t)hst->total_terms, nmax,
+                               out.values, derivs ? out.jac : nullptr, &taken)))
+            return rc;
+        if (taken) return SCATRS_OK;
+    }
     // Launches over the ranked particles: each takes as many groups of 64 as slabs sized by its
     // first particle's N fit in the budget (a quarter of the free memory, at most 8 GiB).
     size_t free_b = 0, total_b = 0;

@@ -178,6 +178,32 @@ def test_wavefront_per_sphere_path():
     assert np.allclose(one[[0, 1, 4]], r1[[0, 1, 4]], rtol=1e-11, atol=0)
 
 
+def test_wavefront_per_sphere_jacobian():
+    """The Jacobian one sphere per wavefront (its pair terms across the lanes' boundaries
+    included) against the one-per-lane kernel's and the oracle's, in the reference's measure."""
+    x, m = _seeded(4000, seed=19)
+    x = np.concatenate([x, [0.05, 0.7, 1.5, 2500.0]])
+    m = np.concatenate([m, [1.33 + 0j, 2.0 + 1e-3j, 1.33 + 1e-4j, 1.5 + 0.01j]])
+    w = _with_tuning("sphere_wave", 1, lambda: _gpu(x, m, derivatives=True))
+    lane = _with_tuning("sphere_wave", 0, lambda: _gpu(x, m, derivatives=True))
+    ref, jref = oracle.mie_efficiencies_batch(x, m, derivatives=True)
+    gw, gl = w._jacobian, lane._jacobian
+    worst_ref = worst_lane = 0.0
+    for i in range(x.size):
+        sizes = efficiency_sizes(ref[:, i], x[i])
+        sizes[2] = ref[0, i]
+        for p in range(3):
+            pmag = x[i] if p == 0 else abs(m[i])
+            for k in range(5):
+                floor = max(abs(jref[p, k, i]), sizes[k] * (1 + abs(m[i]) * x[i]) / pmag)
+                worst_ref = max(worst_ref, abs(gw[p, k, i] - jref[p, k, i]) / floor)
+                worst_lane = max(worst_lane, abs(gw[p, k, i] - gl[p, k, i]) / floor)
+    print("worst Jacobian error vs oracle:", worst_ref, "vs the lane kernel:", worst_lane)
+    assert worst_ref < 1e-7
+    assert worst_lane < 1e-9
+    assert np.allclose(_stack(w), _stack(lane), rtol=1e-10, atol=0)
+
+
 @pytest.mark.parametrize("n,derivatives", [(3000, False), (3000, True), (300_000, False)])
 def test_host_array_entry_equals_the_device_entry(n, derivatives):
     """scatrs_mie_efficiencies_host (checks, N and ranking on the host, one copy each way)
