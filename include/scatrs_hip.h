@@ -80,7 +80,8 @@ int scatrs_set_angular_kernel(scatrs_ctx *ctx, int kind);
  *     branch-free plain path and redoes it on the checked path (the reference's branch per
  *     complex inverse) only where an argument left the plain range; 0 takes the checked path for
  *     every sphere (same results to rounding; for tests of that path);
- *   SCATRS_TUNE_SPHERE_WAVE: when a values-only sphere batch is solved one sphere per wavefront
+ *   SCATRS_TUNE_SPHERE_WAVE: when a sphere batch (values and Jacobian; the layered entry points'
+ *     one-layer case alike) is solved one sphere per wavefront
  *     (each sphere's recurrences split over the 64 lanes and joined by scans) instead of one per
  *     lane: 0 never, 1 always, f >= 2 (default 8) when its total terms times f are fewer than
  *     the lanes of the GPU times its largest N, i.e. when one sphere per lane would leave most
@@ -116,9 +117,9 @@ int64_t scatrs_default_n_max(double x);
  *   d_x[len], d_m[2*len] (re, im interleaved: complex128), n_max <= 0 for the default
  *   truncation; d_values (5, len) row-major; d_jacobian (3, 5, len) or NULL.
  *   total_terms (host, may be NULL): sum over spheres of N, the (particle, order) terms.
- * A values-only batch too small to keep the GPU's lanes busy (SCATRS_TUNE_SPHERE_WAVE) is solved
- * one sphere per wavefront, its recurrences split over the lanes; larger batches and the
- * Jacobian one sphere per lane. Both give the reference's results to rounding.
+ * A batch too small to keep the GPU's lanes busy (SCATRS_TUNE_SPHERE_WAVE) is solved one sphere
+ * per wavefront, its recurrences split over the lanes; larger batches one sphere per lane. Both
+ * give the reference's results to rounding.
  * Synchronizes once (validation and sizing); the solve itself is queued on the stream. */
 int scatrs_mie_efficiencies(scatrs_ctx *ctx, const double *d_x, const double *d_m, int64_t len,
                             int64_t n_max, int allow_unvalidated, double *d_values,

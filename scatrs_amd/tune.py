@@ -10,7 +10,7 @@ The parameters (keys of ``KEYS``):
   * ``split_blocks``: blocks per CU a small batch's angular products aim for;
   * ``sphere_plain`` (not tuned: a switch for tests): 0 runs every sphere of the sphere batch on
     the checked path instead of trying the branch-free plain path first;
-  * ``sphere_wave`` (not tuned: set by a rule of the batch's size): when a values-only sphere
+  * ``sphere_wave`` (not tuned: set by a rule of the batch's size): when a sphere
     batch takes one wavefront per sphere (0 never, 1 always, f >= 2 when its terms times f are
     fewer than the GPU's lanes times its largest N).
 
