@@ -572,8 +572,9 @@ static __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 // The node solves of a small launch (spheres), one node per wavefront (blocks of one
 // wavefront) from the shared counter over the ranked nodes: what forward_node writes (staging,
 // values, table factor, coefficient exponent, backscatter sum), its orders split over the
-// lanes (sphere_efficiencies_wave). staging == nullptr: nothing staged (a call without angular
-// products).
+// lanes (sphere_efficiencies_wave); in tangent mode (a.tan_dir >= 0) what tangent_node writes
+// (sphere_tangent_wave, the weight times 1 / (2 eps)). staging == nullptr: nothing staged (a
+// call without angular products).
 static __global__ __launch_bounds__(64) void ens_nodes_wave(
     EnsArgs a, int64_t e0, int64_t chunk_nodes, const int *order, const int *node_n,
     const int64_t *node_off, const int64_t *ens_base, unsigned long long *counter, char *cols,
@@ -594,8 +595,20 @@ static __global__ __launch_bounds__(64) void ens_nodes_wave(
         const int n = node_n[e * a.n_radius + i];
         const cx m = mk(a.m[2 * e], a.m[2 * e + 1]);
         double q[5], amax = 0.0;
-        cx back;
-        if (!staging) {
+        cx back = {0.0, 0.0};
+        if (a.tan_dir >= 0) {
+            const double eps = 1.0 / (1.0 + geo.x);
+            const double step = a.tan_sign * eps;
+            const int64_t off = ens_base[e] + node_off[e * a.n_radius + i];
+            if (!staging)
+                sphere_tangent_wave(geo.x, m, n, uz, ux, lane, a.tan_dir, step, NoStage{}, q, &amax);
+            else if (stage32)
+                sphere_tangent_wave(geo.x, m, n, uz, ux, lane, a.tan_dir, step,
+                                    StageF32{reinterpret_cast<float4 *>(staging) + off}, q, &amax);
+            else
+                sphere_tangent_wave(geo.x, m, n, uz, ux, lane, a.tan_dir, step,
+                                    StageF64{staging + 2 * off}, q, &amax);
+        } else if (!staging) {
             sphere_efficiencies_wave(geo.x, m, n, uz, ux, lane, q, NoStage{}, &amax, &back);
         } else {
             const int64_t off = ens_base[e] + node_off[e * a.n_radius + i];
@@ -609,7 +622,7 @@ static __global__ __launch_bounds__(64) void ens_nodes_wave(
         }
         if (lane == 0) {
             const double area = kPi * geo.r * geo.r;
-            const double W = geo.W;
+            const double W = a.tan_dir >= 0 ? geo.W / (2.0 * (1.0 / (1.0 + geo.x))) : geo.W;
             double *v = vals + g * kNodeVals;
             v[0] = W;
             v[1] = W * (area * q[0]);
@@ -1948,7 +1961,8 @@ template <int L>
 static void launch_nodes_ranked(hipStream_t s, const EnsArgs &a, int64_t e0, const EnsSetup &S,
                                 cx *scratch, const ChunkBufs &B, cx *back_out) {
     for (const NodeLaunch &q : B.plan) {
-        if (a.tan_dir >= 0) {
+        // (A one-node-per-wavefront plan, q.waves < 0, serves tangent mode too: ens_nodes_wave.)
+        if (a.tan_dir >= 0 && q.waves >= 0) {
             if constexpr (L == 1) {
                 fill_u64<<<1, 1, 0, s>>>(S.counter, 0ull, 1);
                 ens_nodes_tangent_persist<<<(unsigned)(q.waves / 4), 256, 0, s>>>(
@@ -2049,7 +2063,7 @@ static int plan_nodes(scatrs_ctx *ctx, const EnsSetup &S, int64_t e0, int64_t no
         // first nonempty bucket's N is the chunk's largest, the call's for a whole call).
         const int64_t n = (sphere_whole || b0 == 0) ? S.n_max : std::max(kBuckets - 2 - b0, 0);
         const int64_t rows = n + 2;
-        if (S.forward_only && S.a.tan_dir < 0 &&
+        if (S.forward_only &&
             wave_rule(ctx->tune[SCATRS_TUNE_SPHERE_WAVE], ctx->cus, chunk_terms, n)) {
             // One node per wavefront (ens_nodes_wave), two wavefronts per SIMD at most: the
             // plan's negative count marks it.

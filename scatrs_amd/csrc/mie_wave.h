@@ -217,19 +217,11 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// One sphere's efficiencies on the whole wavefront (every lane calls it with the same sphere;
-// lane 0's q holds the result), with DERIVS also their Jacobian jac15 (sphere_efficiencies'
-// layout). uz (n + 2 complex) and ux (n + 2 doubles) are the wavefront's columns, the u_k(m x)
-// and u_k(x) chains indexed by k. Each order's coefficients also go to `stage` (values only);
-// *amax receives the largest |Re|, |Im| of them and *back the backscatter sum
-// sum (2n+1) (-1)^n (a_n - b_n) (every lane).
-template <bool DERIVS = false, class Stage = NoStage>
-__device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, cx *uz,
-                                                         double *ux, int lane, double q[5],
-                                                         Stage stage = Stage{},
-                                                         double *amax = nullptr,
-                                                         cx *back = nullptr,
-                                                         double *jac15 = nullptr) {
+// The chains of one sphere on the whole wavefront (every lane calls it with the same sphere):
+// u_k(m x) and u_k(x) into the wavefront's columns uz (n + 2 complex) and ux (n + 2 doubles),
+// indexed by k, and this lane's range of orders with the recurrences' state on entry to it.
+__device__ __forceinline__ OrderStart wave_prepare(double x, cx m, int n, cx *uz, double *ux,
+                                                   int lane) {
     const int n_s = upward_orders(x, n);
     // u_k(x) above the upward orders (x_chain's range), kept in ux.
     if (n_s < n) {
@@ -325,6 +317,24 @@ __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, 
     st.t = t0 * et;
     st.inv_xi = ixi0 * ex;
     st.r = kr0 == k0 ? r_in : r0;
+    return st;
+}
+
+// One sphere's efficiencies on the whole wavefront (every lane calls it with the same sphere;
+// lane 0's q holds the result), with DERIVS also their Jacobian jac15 (sphere_efficiencies'
+// layout). uz (n + 2 complex) and ux (n + 2 doubles) are the wavefront's columns, the u_k(m x)
+// and u_k(x) chains indexed by k. Each order's coefficients also go to `stage` (values only);
+// *amax receives the largest |Re|, |Im| of them and *back the backscatter sum
+// sum (2n+1) (-1)^n (a_n - b_n) (every lane).
+template <bool DERIVS = false, class Stage = NoStage>
+__device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, cx *uz,
+                                                         double *ux, int lane, double q[5],
+                                                         Stage stage = Stage{},
+                                                         double *amax = nullptr,
+                                                         cx *back = nullptr,
+                                                         double *jac15 = nullptr) {
+    const OrderStart st = wave_prepare(x, m, n, uz, ux, lane);
+    const int k0 = st.k0, k1 = st.k1;
 
     WaveEfficiencySink<DERIVS, Stage> s;
     s.stage = stage;
@@ -366,6 +376,72 @@ __device__ __forceinline__ void sphere_efficiencies_wave(double x, cx m, int n, 
     }
     if (amax) *amax = wave_max(s.amax);
     if (back) *back = tot.back;
+}
+
+// The sink of one lane's orders in tangent mode (the ensembles' node solves along a direction
+// of the sphere, dir 0: Re m, 1: Im m): each order's coefficients moved to c + step dc/d(dir)
+// (d/d Im m = i d/dm) are staged and summed, and the absorption sum moves by step times its
+// derivative along the direction, as the one-node-per-lane solve's TangentStagingSink forms
+// them; the first order's moved coefficients are kept for the boundary's pair term.
+template <class Stage>
+struct WaveTangentSink : EfficiencySink<false> {
+    using Base = EfficiencySink<false>;
+    Stage stage;
+    int dir = 0;
+    double step = 0.0;
+    int k_first = 0;
+    cx a_first = {0.0, 0.0}, b_first = {0.0, 0.0};
+    double amax = 0.0;
+    __device__ void order_d(int k, cx a, cx b, cx, cx dam, cx, cx dbm) {
+        const cx ta = dir == 0 ? dam : times_i(dam);
+        const cx tb = dir == 0 ? dbm : times_i(dbm);
+        const cx pa = a + ta * step, pb = b + tb * step;
+        if (k == k_first) {
+            a_first = pa;
+            b_first = pb;
+        }
+        stage(k, pa, pb);
+        amax = fmax(amax, fmax(fmax(fabs(pa.re), fabs(pa.im)), fmax(fabs(pb.re), fabs(pb.im))));
+        Base::order(k, pa, pb);
+    }
+    template <class O>
+    __device__ void stored(int, cx, cx, cx, cx, cx, cx, const O &) {}
+    __device__ void done(double s, double j0, double j1, double j2) {
+        Base::done(s + step * (dir == 0 ? j1 : j2), j0, j1, j2);
+    }
+};
+
+// One sphere in tangent mode on the whole wavefront (WaveTangentSink): q the efficiencies of the
+// moved coefficients (lane 0), *amax their largest |Re|, |Im| (every lane).
+template <class Stage>
+__device__ __forceinline__ void sphere_tangent_wave(double x, cx m, int n, cx *uz, double *ux,
+                                                    int lane, int dir, double step, Stage stage,
+                                                    double q[5], double *amax) {
+    const OrderStart st = wave_prepare(x, m, n, uz, ux, lane);
+    const int k0 = st.k0, k1 = st.k1;
+    WaveTangentSink<Stage> s;
+    s.stage = stage;
+    s.dir = dir;
+    s.step = step;
+    s.k_first = k0;
+    s.rk = k0 > 1 ? recip((double)k0) : 1.0;
+    NoOuter none;
+    if (k0 <= k1) mie_orders<true>(x, m, n, ZColumn{uz, 1}, ux, 1, none, s, &st);
+    const int next = (lane + 1) & 63;
+    const cx an = shfl_v(s.a_first, next), bn = shfl_v(s.b_first, next);
+    double g = s.g;
+    if (k0 <= k1 && k1 < n) {
+        const double kf = (double)k1;
+        g += (kf * (kf + 2.0)) * recip(kf + 1.0) *
+             (re_mul_conj(s.a_prev, an) + re_mul_conj(s.b_prev, bn));
+    }
+    EfficiencySink<false> tot;
+    tot.sca = wave_sum(s.sca);
+    tot.g = wave_sum(g);
+    tot.abs_sum = wave_sum(s.abs_sum);
+    tot.back = cx{wave_sum(s.back.re), wave_sum(s.back.im)};
+    tot.values(x, q);
+    *amax = wave_max(s.amax);
 }
 
 // Bytes of one wavefront's columns (u_k(m x) complex, then u_k(x)) for spheres of up to nmax

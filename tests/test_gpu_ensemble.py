@@ -140,6 +140,50 @@ def test_node_solves_one_per_wavefront_stage_the_coefficients(kernel):
         _check(sw[e], gw[e], pw[e], ref, 1e-9 if kernel == 0 else 2e-6)
 
 
+@pytest.mark.parametrize("kernel", [0, 2])
+def test_tangent_node_solves_one_per_wavefront(kernel):
+    """scatrs_ensemble_nodes_tangent (the coefficients moved by sign eps dc along Re m or Im m)
+    with its node solves one node per wavefront (SCATRS_TUNE_SPHERE_WAVE = 1) against one per
+    lane (0), with and without angular products: equal to rounding (to the split products'
+    2^-22 with the fp16-split kernel)."""
+    from scatrs_amd import _native, tune
+    from scatrs_amd.ensemble import ensemble_nodes_tangent_device
+
+    rng = np.random.default_rng(31)
+    count, k = 6, 40
+    r = np.sort(np.exp(rng.uniform(np.log(0.05), np.log(8.0), (count, k))), 1)
+    w = rng.uniform(0.1, 1.0, (count, k))
+    nodes = _dev(np.stack([r, w], -1))
+    wl = _dev(np.exp(rng.uniform(np.log(0.3), np.log(2.0), count)))
+    m = _dev(rng.uniform(1.3, 1.7, count) + 1j * np.exp(rng.uniform(np.log(1e-6), np.log(0.1),
+                                                                      count)))
+    ctx = _native.context(0)
+    tol = 1e-11 if kernel == 0 else 1e-6
+    try:
+        ctx.set_angular_kernel(kernel)
+        for direction in (0, 1):
+            for order, phase in ((None, False), (16, True)):
+                out = {}
+                for mode in (1, 0):
+                    ctx.set_tuning(tune.KEYS["sphere_wave"], mode)
+                    sc, gr, ph, _ = ensemble_nodes_tangent_device(
+                        nodes, wl, m, direction=direction, sign=1, n_angles=48, order=order,
+                        want_phase=phase)
+                    out[mode] = (sc.cpu().numpy(), None if gr is None else gr.cpu().numpy(),
+                                 None if ph is None else ph.cpu().numpy())
+                (sw, gw, pw), (sl, gl, pl) = out[1], out[0]
+                rel = np.abs(sw - sl) / np.maximum(np.abs(sl), 1e-300)
+                rel[:, 7] = np.abs(sw[:, 7] - sl[:, 7])
+                assert rel.max() < 1e-11, (direction, order, rel.max())
+                if gr is not None:
+                    for e in range(count):
+                        assert np.max(np.abs(gw[e] - gl[e])) / np.max(np.abs(gl[e, 0])) < tol
+                        assert np.max(np.abs(pw[e] - pl[e])) / np.max(np.abs(pl[e, 0])) < tol
+    finally:
+        ctx.set_tuning(tune.KEYS["sphere_wave"], tune.DEFAULTS["sphere_wave"])
+        ctx.set_angular_kernel(0)
+
+
 def test_gamma_and_unnormalized_match_oracle():
     p0 = np.array([0.2, 0.5, 1.1])
     p1 = np.array([0.05, 0.2, 0.38])
