@@ -690,24 +690,45 @@ def _rule_ladder(n):
     return _LADDER[i]
 
 
-def _row_rules(tasks, rows, order, n_angles, single: bool):
-    """The Greek projection's rule of each pass-2 row: the caller's n_angles, or the smallest
-    exact for the row's nodes, N_max + (order + 1)/2 cosines (on a ladder, so that rows share
-    calls; one rule for all rows when ``single``, for the phase at the rule's cosines)."""
+def _rules_for(x_max, order, n_angles, single: bool):
+    """The Greek projection's rule of pass-2 rows whose largest size parameter is x_max (an
+    array): the caller's n_angles, or the smallest exact for the row's nodes, N_max + (order +
+    1)/2 cosines, N_max = ceil(x + 6 x^(1/3) + 3) as the library's scatrs_default_n_max (on a
+    ladder, so that rows share calls; one rule for all rows when ``single``, for the phase at the
+    rule's cosines)."""
+    x_max = np.asarray(x_max, np.float64)
     if order is None:
-        return np.full(len(rows), 2 if n_angles is None else int(n_angles))
+        return np.full(x_max.size, 2 if n_angles is None else int(n_angles))
     if n_angles is not None:
-        return np.full(len(rows), int(n_angles))
-    lib = _native.load()
-    need = np.empty(len(rows), np.int64)
-    for k, (ti, lo, hi) in enumerate(rows):
-        q = tasks[ti]
-        r = q.p[0] if lo < 0 else np.exp(q.b[hi - 1])
-        need[k] = int(lib.scatrs_default_n_max(2.0 * np.pi * r / q.wl)) + (order + 2) // 2
-    need = _rule_ladder(need).astype(np.int64)
-    if single:
+        return np.full(x_max.size, int(n_angles))
+    n = np.ceil(x_max + 6.0 * np.cbrt(x_max) + 3.0).astype(np.int64)
+    need = _rule_ladder(n + (order + 2) // 2).astype(np.int64)
+    if single and need.size:
         need[:] = need.max()
     return need
+
+
+def _row_rules(tasks, rows, order, n_angles, single: bool):
+    """_rules_for the rows (task, first panel, end panel; first < 0: a monodisperse task's one
+    node): each row's largest radius, its last panel's end (or the monodisperse radius)."""
+    x = np.array([2.0 * np.pi * (tasks[ti].p[0] if lo < 0 else np.exp(tasks[ti].b[hi - 1])) /
+                  tasks[ti].wl for ti, lo, hi in rows], np.float64)
+    return _rules_for(x, order, n_angles, single)
+
+
+def _panel_rows(tasks):
+    """The pass-2 rows of every task with panels (not monodisperse), in task order, _ROW_PANELS
+    panels each (the last ones shorter): each row's task and the ln r of its last panel's end."""
+    sizes = np.array([q.a.size if q.kind != MONODISPERSE else 0 for q in tasks], np.int64)
+    own = np.flatnonzero(sizes > 0)
+    cnt = sizes[own]
+    n_rows = -(-cnt // _ROW_PANELS)
+    ti = np.repeat(own, n_rows)
+    j = np.arange(int(n_rows.sum())) - np.repeat(np.cumsum(n_rows) - n_rows, n_rows)
+    hi = np.minimum((j + 1) * _ROW_PANELS, sizes[ti])
+    b_flat = np.concatenate([tasks[t].b for t in own]) if own.size else np.zeros(0)
+    b_last = b_flat[np.repeat(np.cumsum(cnt) - cnt, n_rows) + hi - 1]
+    return ti, b_last
 
 
 def _averages(sc):
@@ -862,10 +883,12 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     f64 = torch.float64
     Wd = 0 if order is None else order + 1
     single = want_phase and mu is None
-    rows = [(ti, lo, min(lo + _ROW_PANELS, q.a.size)) for ti, q in enumerate(tasks)
-            if q.kind != MONODISPERSE for lo in range(0, q.a.size, _ROW_PANELS)]
+    row_task, row_end = _panel_rows(tasks)
     mono = [ti for ti, q in enumerate(tasks) if q.kind == MONODISPERSE]
-    rules = _row_rules(tasks, rows + [(t, -1, -1) for t in mono], order, n_angles, single)
+    wl_t = np.array([q.wl for q in tasks], np.float64)
+    x_rows = np.concatenate([2.0 * np.pi * np.exp(row_end) / wl_t[row_task],
+                             [2.0 * np.pi * tasks[t].p[0] / tasks[t].wl for t in mono]])
+    rules = _rules_for(x_rows, order, n_angles, single)
     n_phase = (int(rules[0]) if mu is None else int(np.size(mu))) if want_phase else 0
     avg = torch.zeros((T * R, 7), dtype=f64, device=dev)
     zt = torch.zeros(T * R, dtype=f64, device=dev)
@@ -885,7 +908,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
         return pr, mr, mi, fr
 
     # Each row's sums (detached), at its place among all rows x replicas, for the error bounds.
-    n_rows = len(rows) + len(mono)
+    n_rows = len(row_task) + len(mono)
     row_lin = torch.zeros((n_rows * R, 7), dtype=f64, device=dev)
     row_gr = torch.zeros((n_rows * R, 6, Wd), dtype=f64, device=dev) if gr is not None else None
     row_ph = torch.zeros((n_rows * R, 6, n_phase), dtype=f64, device=dev) if ph is not None \
@@ -912,7 +935,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
 
     wl_all = np.array([q.wl for q in tasks])
     layers = np.array([q.layers for q in tasks])
-    nr = len(rows)
+    nr = len(row_task)
     if nr:
         ti_rows, A, Bd = _row_panels(tasks)
         grad = torch.is_grad_enabled() if derivatives is None else bool(derivatives)
@@ -989,7 +1012,7 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
     ph = None if ph is None else ph / zm[:, None, None]
     shape = lambda a: None if a is None else a.reshape((T, R) + a.shape[1:])  # noqa: E731
     rows_out = dict(key=row_key, lin=row_lin, gr=row_gr, ph=row_ph, z=zm.detach(),
-                    n_rows=len(rows), n_mono=len(mono))
+                    n_rows=len(row_task), n_mono=len(mono))
     return (shape(avg), shape(gr), shape(ph), terms, (int(rules[0]) if len(rules) else 2),
             rows_out)
 

@@ -566,3 +566,33 @@ def test_shared_panels_extend_a_member_with_a_steep_integrand(monkeypatch):
     sca = t.K[:, 1].sum() / t.moments[0]
     exact = r_g ** 6 * np.exp(18 * s * s)
     assert abs(sca - exact) <= 5 * rtol * exact
+
+
+def test_pass2_row_rules_vectorized_equal_the_per_row_rules():
+    """Pass 2's rows and their Greek rules from arrays (_panel_rows, _rules_for) equal the rules
+    of the rows listed one by one (_row_rules), monodisperse tasks included."""
+    from scatrs_amd import adaptive
+
+    rng = np.random.default_rng(5)
+    tasks = []
+    for k in range(40):
+        kind = adaptive.MONODISPERSE if k % 7 == 3 else 0
+        q = adaptive.Task(kind, np.array([rng.uniform(0.05, 3.0), 1.5, 0.0]),
+                          float(rng.uniform(0.3, 2.0)), np.array([1.5 + 0j]), np.zeros(0))
+        n = int(rng.integers(1, 70))
+        edges = np.sort(rng.uniform(-6.0, 3.0, n + 1))
+        q.a, q.b = edges[:-1], edges[1:]
+        tasks.append(q)
+    rows = [(ti, lo, min(lo + adaptive._ROW_PANELS, q.a.size)) for ti, q in enumerate(tasks)
+            if q.kind != adaptive.MONODISPERSE for lo in range(0, q.a.size, adaptive._ROW_PANELS)]
+    mono = [ti for ti, q in enumerate(tasks) if q.kind == adaptive.MONODISPERSE]
+    ti, end = adaptive._panel_rows(tasks)
+    assert np.array_equal(ti, [r[0] for r in rows])
+    wl = np.array([q.wl for q in tasks])
+    x = np.concatenate([2.0 * np.pi * np.exp(end) / wl[ti],
+                        [2.0 * np.pi * tasks[t].p[0] / tasks[t].wl for t in mono]])
+    for order, n_angles, single in ((32, None, False), (63, None, True), (None, None, False),
+                                    (16, 40, False)):
+        old = adaptive._row_rules(tasks, rows + [(t, -1, -1) for t in mono], order, n_angles,
+                                  single)
+        assert np.array_equal(adaptive._rules_for(x, order, n_angles, single), old)
