@@ -243,6 +243,7 @@ class _TaskTable:
         self.wl = np.array([q.wl for q in tasks], np.float64)
         self.layers = np.array([q.m.size for q in tasks], np.int64)
         self.n_dirs = np.array([len(q.dirs) for q in tasks], np.int64)
+        self.m0 = np.array([q.m[0] for q in tasks], np.complex128)
 
 
 def _nodes(tasks, ti, a, b, idx, derivatives: bool = False, table: _TaskTable | None = None):
@@ -474,6 +475,50 @@ def _node_values(tasks, ti, r, allow_unvalidated: bool, table: _TaskTable | None
     return Q, dQ, terms
 
 
+# Pass-1 rounds of at least this many sphere nodes, without refined derivatives, form their
+# panels' sums on the device (_panel_sums_device).
+_DEVICE_SUMS_MIN = 65536
+
+
+def _panel_sums_device(ti, r, wk, fold, allow_unvalidated: bool, table: _TaskTable):
+    """A pass-1 round's Kronrod sums, Gauss sums and Kronrod sums of |integrand| (n, 5) of the
+    refined quantities of n panels of spheres (tasks ti, nodes r (n, 15), Kronrod weights wk
+    (n, 15), the Gauss rule's fold), and the terms: the sphere batch solves the nodes on the
+    device (scatrs_mie_efficiencies) and the cross sections and weighted sums are formed there,
+    so only the sums come back. The same quantities as _node_values and the host sums, summed
+    in the device's order. An invalid node raises with the wavelength-and-radius context."""
+    import torch
+
+    from . import mie
+    from .errors import OutsideValidatedDomainError
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    n = ti.size
+    tn = np.repeat(ti, 15)
+    rr = np.ascontiguousarray(r, np.float64).reshape(-1)
+    wl = table.wl[tn]
+    x = 2.0 * np.pi * rr / wl
+    try:
+        v, _, nt = mie.efficiencies_device(torch.from_numpy(x).to(dev),
+                                           torch.from_numpy(table.m0[tn]).to(dev),
+                                           allow_unvalidated=allow_unvalidated)
+    except (InvalidInputError, OutsideValidatedDomainError) as e:
+        hit = _ELEMENT.fullmatch(str(e))
+        if hit is None:
+            raise
+        i = int(hit[1])
+        raise type(e)(f"at wavelength {wl[i]:e}, radius {rr[i]:e}: {hit[2]}") from None
+    rt = torch.from_numpy(rr).to(dev)
+    area = np.pi * rt * rt
+    q = torch.stack([area * v[0], area * v[1], area * (v[1] * v[4]), area * v[2],
+                     area * v[3]], 1).reshape(n, 15, _Q)
+    w = torch.from_numpy(np.ascontiguousarray(wk)).to(dev)
+    wg = torch.from_numpy(np.ascontiguousarray(wk * fold)).to(dev)
+    sums = torch.stack([(w[..., None] * q).sum(1), (wg[..., None] * q).sum(1),
+                        (w[..., None] * q.abs()).sum(1)]).cpu().numpy()
+    return sums[0], sums[1], sums[2], nt
+
+
 def _targets(tot, asum, rtol: float, D: int, H: int, n_dir, n_theta, refine_backscatter: bool):
     """The targets (T, nq) and the end targets (T, nq) of every refined quantity from the
     tasks' Kronrod totals ``tot`` and their sums of |integrand| ``asum`` (engine.rs:620-716):
@@ -556,24 +601,31 @@ def pass1(tasks, rtol: float, max_panels: int, refine_backscatter: bool,
             out = _nodes(tasks, ti, a[pending], b[pending], np.arange(15), derivatives=H > 0,
                          table=table)
             r, wk = out[0], out[1]
-            q, dq, nt = _node_values(tasks, np.repeat(ti, 15), r.reshape(-1),
-                                     allow_unvalidated, table=table)
-            terms += nt
-            q = q.reshape(n, 15, _Q)
-            f = [q]
-            if D:  # these panels' tasks may ask for fewer directions than the most asked
-                d = np.zeros((n * 15, D, _Q))
-                if dq is not None:
-                    d[:, :dq.shape[1]] = dq
-                f.append(d.reshape(n, 15, D * _Q))
-            for h in range(H):  # d u / d theta = u d ln u / d theta
-                k = theta[ti, h]
-                dl = out[2][np.arange(n)[:, None], np.arange(15)[None, :], np.maximum(k, 0)[:, None]]
-                f.append(q * np.where(k >= 0, 1.0, 0.0)[:, None, None] * dl[..., None])
-            f = np.concatenate(f, -1)  # (n, 15, nq): the integrands per unit weight
-            K[pending] = np.einsum("nk,nkq->nq", wk, f)
-            G[pending] = np.einsum("nk,nkq->nq", wk * fold, f)
-            A[pending] = np.einsum("nk,nkq->nq", wk, np.abs(f))
+            if (D == 0 and H == 0 and n * 15 >= _DEVICE_SUMS_MIN
+                    and np.all(table.layers[ti] == 1)):
+                K[pending], G[pending], A[pending], nt = _panel_sums_device(
+                    ti, r, wk, fold, allow_unvalidated, table)
+                terms += nt
+            else:
+                q, dq, nt = _node_values(tasks, np.repeat(ti, 15), r.reshape(-1),
+                                         allow_unvalidated, table=table)
+                terms += nt
+                q = q.reshape(n, 15, _Q)
+                f = [q]
+                if D:  # these panels' tasks may ask for fewer directions than the most asked
+                    d = np.zeros((n * 15, D, _Q))
+                    if dq is not None:
+                        d[:, :dq.shape[1]] = dq
+                    f.append(d.reshape(n, 15, D * _Q))
+                for h in range(H):  # d u / d theta = u d ln u / d theta
+                    k = theta[ti, h]
+                    dl = out[2][np.arange(n)[:, None], np.arange(15)[None, :],
+                                np.maximum(k, 0)[:, None]]
+                    f.append(q * np.where(k >= 0, 1.0, 0.0)[:, None, None] * dl[..., None])
+                f = np.concatenate(f, -1)  # (n, 15, nq): the integrands per unit weight
+                K[pending] = np.einsum("nk,nkq->nq", wk, f)
+                G[pending] = np.einsum("nk,nkq->nq", wk * fold, f)
+                A[pending] = np.einsum("nk,nkq->nq", wk, np.abs(f))
         act = ~done[task]
         if not act.any():
             break

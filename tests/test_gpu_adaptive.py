@@ -525,3 +525,27 @@ def test_batch_members_share_panels_and_match_single_runs():
         assert dg <= 5e-8 * np.abs(one["greek"]).max(), (spec[0], dg)
         dp = np.abs(b["phase"][:, rows] - one["phase"]).max()
         assert dp <= 5e-8 * np.abs(one["phase"]).max(), (spec[0], dp)
+
+
+def test_pass1_sums_on_the_device_equal_the_host_sums(monkeypatch):
+    """Large pass-1 rounds of spheres without refined derivatives form their panels' sums on the
+    device (_panel_sums_device): the same panels and, to rounding, the same values as the
+    host's sums (every round forced to the device, then none)."""
+    from scatrs_amd import adaptive
+    from scatrs_amd.ensemble import Ensemble
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import LogNormal
+
+    e = Ensemble(Sphere(1.43 + 1e-8j), LogNormal(np.array([0.1, 0.4, 1.0]),
+                                                  np.array([1.4, 1.8, 2.0])))
+    wl = np.array([0.35, 0.7])
+    out = {}
+    for name, limit in (("device", 0), ("host", 1 << 62)):
+        monkeypatch.setattr(adaptive, "_DEVICE_SUMS_MIN", limit)
+        out[name] = e.compute(wl, rtol=3e-4, greek=16, phase=False, n_angles=None)
+    d, h = out["device"], out["host"]
+    assert np.array_equal(np.asarray(d.panels), np.asarray(h.panels))
+    for k in ("c_ext", "c_sca", "ssa", "g", "c_back"):
+        assert np.allclose(getattr(d, k), getattr(h, k), rtol=1e-12, atol=0), k
+    scale = np.abs(h.greek.alpha1).max()
+    assert np.allclose(d.greek.alpha1, h.greek.alpha1, rtol=0, atol=1e-12 * scale)
