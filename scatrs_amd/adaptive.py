@@ -494,26 +494,27 @@ def _panel_sums_device(ti, r, wk, fold, allow_unvalidated: bool, table: _TaskTab
 
     dev = torch.device("cuda", torch.cuda.current_device())
     n = ti.size
-    tn = np.repeat(ti, 15)
     rr = np.ascontiguousarray(r, np.float64).reshape(-1)
-    wl = table.wl[tn]
-    x = 2.0 * np.pi * rr / wl
+    # Only the radii, the weights and the panels' tasks go to the device; each node's x and m
+    # come from the tasks' wavelengths and indices there (the host's operations, bit for bit).
+    tn = torch.from_numpy(np.ascontiguousarray(ti, np.int64)).to(dev).repeat_interleave(15)
+    rt = torch.from_numpy(rr).to(dev)
+    x = 2.0 * np.pi * rt / torch.from_numpy(table.wl).to(dev)[tn]
+    m = torch.from_numpy(table.m0).to(dev)[tn]
     try:
-        v, _, nt = mie.efficiencies_device(torch.from_numpy(x).to(dev),
-                                           torch.from_numpy(table.m0[tn]).to(dev),
-                                           allow_unvalidated=allow_unvalidated)
+        v, _, nt = mie.efficiencies_device(x, m, allow_unvalidated=allow_unvalidated)
     except (InvalidInputError, OutsideValidatedDomainError) as e:
         hit = _ELEMENT.fullmatch(str(e))
         if hit is None:
             raise
         i = int(hit[1])
-        raise type(e)(f"at wavelength {wl[i]:e}, radius {rr[i]:e}: {hit[2]}") from None
-    rt = torch.from_numpy(rr).to(dev)
+        raise type(e)(f"at wavelength {table.wl[ti[i // 15]]:e}, radius {rr[i]:e}: "
+                      f"{hit[2]}") from None
     area = np.pi * rt * rt
     q = torch.stack([area * v[0], area * v[1], area * (v[1] * v[4]), area * v[2],
                      area * v[3]], 1).reshape(n, 15, _Q)
     w = torch.from_numpy(np.ascontiguousarray(wk)).to(dev)
-    wg = torch.from_numpy(np.ascontiguousarray(wk * fold)).to(dev)
+    wg = w * torch.from_numpy(np.ascontiguousarray(fold)).to(dev)
     sums = torch.stack([(w[..., None] * q).sum(1), (wg[..., None] * q).sum(1),
                         (w[..., None] * q.abs()).sum(1)]).cpu().numpy()
     return sums[0], sums[1], sums[2], nt
