@@ -611,6 +611,10 @@ def _run_adaptive(units, wl, *, order, mu, normalized, keys, rtol, max_panels,
                           for idx in (np.arange(15), adaptive._GAUSS)]
             elif shared and not keys:  # every shared node solved once; both rules from it
                 passes = sh.pass2_groups(tasks, **kw)
+            elif (not keys and order is None and mu is None
+                  and all(q.kind != MONODISPERSE and q.a.size for q in tasks)):
+                # no Greek, phase or derivatives: pass 1's panel sums are pass 2's
+                passes = adaptive.pass2_from_panels(tasks)
             else:  # both rules in one call per angle rule: the tasks twice (the same objects:
                 # pass 2 only reads them), Gauss weights second
                 passes = _split_rules(adaptive.pass2_torch(

@@ -1070,6 +1070,37 @@ def pass2_torch(tasks, params, *, R: int, idx, order, n_angles, mu, want_phase,
             rows_out)
 
 
+def pass2_from_panels(tasks):
+    """Pass 2 of a call that asks for neither Greek coefficients nor the phase matrix nor
+    derivatives, from pass 1's own sums: pass 1 has already summed every final panel's refined
+    quantities [C_ext, C_sca, C_sca g, C_abs, C_back] over the same nodes with the same
+    weights, by the Kronrod (K) and the Gauss rule (G), which is all that pass 2 would compute
+    again. Returns [(avg (T, 1, 7), None, None, 0, 2, rows)] for the Kronrod and the Gauss rule
+    as pass2_torch returns one rule (R = 1): the linear averages over the moment integration's
+    Z (r^2 and r^3 from it), and the rows' sums (rows of _ROW_PANELS panels, in task order) for
+    the error bounds. Every task has panels (no monodisperse part)."""
+    import torch
+
+    dev = torch.device("cuda", torch.cuda.current_device())
+    T = len(tasks)
+    mom = np.array([q.moments for q in tasks], np.float64)  # (T, 3): Z, M2, M3
+    row_task, _ = _panel_rows(tasks)
+    out = []
+    for sums in ("K", "G"):
+        # [C_ext, C_sca, C_abs, C_back, C_sca g] from pass 1's [C_ext, C_sca, C_sca g, C_abs, C_back]
+        lin = [getattr(q, sums)[:, [0, 1, 3, 4, 2]] for q in tasks]
+        tot = np.array([x.sum(0) for x in lin])  # (T, 5)
+        avg = np.concatenate([tot, mom[:, 1:]], 1) / mom[:, :1]
+        rows = np.concatenate([np.add.reduceat(x, np.arange(0, x.shape[0], _ROW_PANELS), 0)
+                               for x in lin], 0)
+        rows = np.concatenate([rows, np.zeros((rows.shape[0], 2))], 1)
+        to = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        d = {"key": to(row_task.astype(np.int64)), "lin": to(rows), "gr": None, "ph": None,
+             "z": to(mom[:, 0])}
+        out.append((to(avg).reshape(T, 1, 7), None, None, 0, 2, d))
+    return out
+
+
 def parts_adaptive(parts, wl: np.ndarray, *, order, n_angles, rtol: float, max_panels: int,
                    allow_unvalidated: bool, want_phase: bool, mu=None,
                    refine_backscatter: bool = False):

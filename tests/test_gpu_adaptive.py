@@ -549,3 +549,22 @@ def test_pass1_sums_on_the_device_equal_the_host_sums(monkeypatch):
         assert np.allclose(getattr(d, k), getattr(h, k), rtol=1e-12, atol=0), k
     scale = np.abs(h.greek.alpha1).max()
     assert np.allclose(d.greek.alpha1, h.greek.alpha1, rtol=0, atol=1e-12 * scale)
+
+
+def test_pass2_from_pass1_sums_equals_pass2():
+    """A call without Greek coefficients, phase or derivatives takes pass 2's sums from pass 1's
+    panels (adaptive.pass2_from_panels); the same call with Greek to order 0 runs pass 2 on the
+    same panels: the scalars and their error bounds agree to rounding."""
+    from scatrs_amd.ensemble import Ensemble
+    from scatrs_amd.particles import Sphere
+    from scatrs_amd.size import LogNormal
+
+    e = Ensemble(Sphere(1.43 + 1e-3j), LogNormal(np.array([0.2, 0.8]), np.array([1.5, 1.9])))
+    wl = np.array([0.4, 1.0])
+    a = e.compute(wl, rtol=1e-5, phase=False, n_angles=None)
+    b = e.compute(wl, rtol=1e-5, greek=0, phase=False, n_angles=None)
+    assert np.array_equal(np.asarray(a.panels), np.asarray(b.panels))
+    for k in ("c_ext", "c_sca", "c_abs", "c_back", "ssa", "g", "lidar_ratio", "effective_radius"):
+        assert np.allclose(getattr(a, k), getattr(b, k), rtol=1e-12, atol=0), k
+        assert np.allclose(getattr(a.error, k), getattr(b.error, k), rtol=1e-6,
+                           atol=1e-15 * np.abs(getattr(b, k)).max()), k
