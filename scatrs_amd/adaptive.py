@@ -420,6 +420,17 @@ def _node_values(tasks, ti, r, allow_unvalidated: bool, table: _TaskTable | None
     x = 2.0 * np.pi * r / wl
     area = np.pi * r * r
     layers = tab.layers[ti]
+    if not D and np.all(layers == 1):
+        # Spheres without refined derivatives: the general loop below in one step.
+        try:
+            v, _, nt = mie.efficiencies_host(x, tab.m0[ti], allow_unvalidated=allow_unvalidated)
+        except (InvalidInputError, OutsideValidatedDomainError) as e:
+            hit = _ELEMENT.fullmatch(str(e))
+            if hit is None:
+                raise
+            i = int(hit[1])
+            raise type(e)(f"at wavelength {wl[i]:e}, radius {r[i]:e}: {hit[2]}") from None
+        return (area * np.stack([v[0], v[1], v[1] * v[4], v[2], v[3]])).T, None, nt
     terms = 0
     for L in np.unique(layers):
         sel = np.flatnonzero(layers == L)
